@@ -1,0 +1,158 @@
+/*
+ * spoc_abi.h -- C-ABI of the MI355X (gfx950) SPOC right-hand side.
+ *
+ * The reference's plugin/operator API for this path is the link-time
+ * Fortran subroutine
+ *
+ *   subroutine cal_var(nf,ne,u0,v0,h0,u1,v1,h1,tseg,pseg,ssit,itm,its,norm,
+ *                      nbdl,pbdl,nint,pint,bdl,cip,vect,normt,
+ *                      mu0,mb0,fc,beta,w0,z0,g0,hb,time)
+ *
+ * (src/calvar_lin.f:7-10, identical in src/calvarnl_fs.f:9-12,
+ * src/calvarnl_ns.f:9-12, src/calvarnl_inv.f:6-9), selected by which object
+ * the makefile links (src/mlin:12, src/mfs:11, src/mns:11, src/minv:11) and
+ * reading the operator tables from COMMON /const/ /gauss/ /gaussf/ /gauss2/
+ * /gauss3/ /matrices/ (src/shwwnl.f:61-71).  The drop-in Fortran cal_var
+ * (spoc_amd/fortran/spoc_calvar.f90) forwards to the functions below.
+ *
+ * Conventions
+ *  - Plain pointers and sizes, no library types.  Host pointers unless a
+ *    name says "_dev".
+ *  - Modal arrays are column-major (ld, ne): mode m of element e at
+ *    a[m + ld*e], rows 0..nm-1 used (the reference uses ld = nnmod = 45).
+ *  - Integer connectivity is int32 and 1-based exactly as the reference
+ *    stores it (tseg second entry 0 on a wall face).  LOGICAL bdl is int32,
+ *    nonzero = true.
+ *  - Every function returns 0 on success and a negative SPOC_E* code on
+ *    failure; spoc_last_error() gives the message.  The reference has no
+ *    status returns (it prints and continues or STOPs); the Fortran shim
+ *    turns a nonzero status into STOP.
+ *  - One HIP stream per context.  A context is not thread-safe.
+ */
+#ifndef SPOC_ABI_H
+#define SPOC_ABI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* cal_var variants: the four link-time implementations. */
+enum {
+  SPOC_LIN = 0, /* src/calvar_lin.f   */
+  SPOC_FS = 1,  /* src/calvarnl_fs.f  */
+  SPOC_NS = 2,  /* src/calvarnl_ns.f  */
+  SPOC_INV = 3  /* src/calvarnl_inv.f */
+};
+
+/* Arithmetic of the right-hand side. */
+enum {
+  SPOC_FP64 = 0,     /* plain FP64: reproduces the reference's operation order */
+  SPOC_FP32_MFMA = 1 /* volume/edge quadrature in FP32 on MFMA, FP64 state,
+                        mass solve and integrals */
+};
+
+enum {
+  SPOC_OK = 0,
+  SPOC_EINVAL = -1,   /* bad argument / unsupported (p, ng, ngf) */
+  SPOC_EHIP = -2,     /* HIP runtime error */
+  SPOC_ESTATE = -3,   /* call out of order (e.g. no mesh yet) */
+  SPOC_ESINGULAR = -4 /* mass matrix not positive definite (reference only
+                         prints "pb avec matrice", src/shwwnl.f:153-156) */
+};
+
+typedef struct spoc_ctx spoc_ctx;
+
+const char *spoc_last_error(void);
+int spoc_version(void);
+
+/*
+ * Operator tables, built from the quadrature rule exactly as the reference
+ * driver does (src/shwwnl.f:127-156: cal_p2, cal_pico, cal_mat, cal_matl,
+ * DPOTRF).  For callers that do not hold the Fortran COMMONs.  Output
+ * layouts are the COMMON layouts with caller-chosen leading dimensions:
+ * pi2(ld_g, nm), pico(ld_f, ld_m, 3), matrices (ld_m, ld_m).  amm/amml are
+ * returned factored (upper Cholesky factor in the upper triangle, original
+ * values below), ammo unfactored.  Returns SPOC_ESINGULAR if DPOTRF fails.
+ */
+int spoc_tables_build(int nc, int ng, int ngf, const double *wi, const double *xi,
+                      const double *yi, const double *wif, const double *tif, int ld_g,
+                      int ld_f, int ld_m, double *pi2, double *pico, double *amm, double *amml,
+                      double *agx, double *agy, double *ammo);
+
+/*
+ * Context: quadrature data and the operator tables in their COMMON layout
+ * (replaces /const/ /gauss/ /gaussf/ /gauss2/ /gauss3/ /matrices/,
+ * src/shwwnl.f:61-71).  amm and amml must be the factored matrices the
+ * driver's DPOTRF left (src/shwwnl.f:153-156).  device = HIP device ordinal.
+ */
+int spoc_ctx_create(spoc_ctx **ctx, int device, int nc, int ng, int ngf, const double *wi,
+                    const double *wif, const double *pi2, int ld_g, const double *pico,
+                    int ld_f, int ld_m, const double *amm, const double *amml,
+                    const double *agx, const double *agy, const double *ammo);
+int spoc_ctx_destroy(spoc_ctx *ctx);
+
+/*
+ * Mesh operands of cal_var (computed by the driver: pointer_mesh, cal_its,
+ * cal_rotnf, calpre; src/shwwnl.f:162-168): itm/its (3,ne), tseg/ssit (2,nf),
+ * norm (2,nf), vect (5,ne), normt (2,ne), bdl (ne).  in (3,ne), xgr, ygr (nn)
+ * are used only by spoc_energy (src/energy.f:37-44 recomputes det from the
+ * nodes); pass NULL to skip.  Replaces any previous mesh (AMR remeshing).
+ */
+int spoc_mesh_set(spoc_ctx *ctx, int ne, int nf, const int *itm, const int *its,
+                  const int *tseg, const int *ssit, const double *norm, const double *vect,
+                  const double *normt, const int *bdl, int nn, const int *in,
+                  const double *xgr, const double *ygr);
+
+/*
+ * Per-run physics operands of cal_var: fc (Coriolis, modal), w0/z0 (wind
+ * stress), hb (resting depth), all (ld, ne); mu0, mb0, g0 as in oc.inp.
+ * fc/w0/z0 may be NULL for SPOC_LIN (unused by src/calvar_lin.f).
+ */
+int spoc_fields_set(spoc_ctx *ctx, int ld, const double *fc, const double *w0, const double *z0,
+                    const double *hb, double mu0, double mb0, double g0);
+
+/*
+ * One cal_var evaluation on host arrays: (u0,v0,h0) -> tendencies (u1,v1,h1),
+ * rows 0..nm-1 of each column written, as the reference does.  This is the
+ * call the Fortran drop-in makes; it copies state over PCIe each call.
+ */
+int spoc_rhs(spoc_ctx *ctx, int variant, int precision, int ld, const double *u0,
+             const double *v0, const double *h0, double *u1, double *v1, double *h1);
+
+/* Device-resident state for the fused RK4 fast path. */
+int spoc_state_upload(spoc_ctx *ctx, int ld, const double *u0, const double *v0,
+                      const double *h0);
+int spoc_state_download(spoc_ctx *ctx, int ld, double *u0, double *v0, double *h0);
+
+/*
+ * nsteps classic RK4 steps of the driver loop (src/shwwnl.f:233-284, same in
+ * src/shwwnl_adap.f:302-353) on the device-resident state.  SPOC_FP64
+ * reproduces the reference's arithmetic order, including
+ * u0 + (dt/6)*(((k1 + 2k2) + 2k3) + k4).
+ */
+int spoc_rk4_steps(spoc_ctx *ctx, int variant, int precision, double dt, int nsteps);
+
+/* Wait for all work queued on the context's stream. */
+int spoc_sync(spoc_ctx *ctx);
+
+/*
+ * Global integrals of the device-resident state (src/energy.f:3-96), FP64:
+ * kinetic 1/2 int (h+H)|u|^2, potential 1/2 g int h^2, mass int h.  Needs the
+ * node arrays passed to spoc_mesh_set.
+ */
+int spoc_energy(spoc_ctx *ctx, double *ke, double *pe, double *mass);
+
+/*
+ * Duration (milliseconds) of the last spoc_rk4_steps call measured with HIP
+ * events on the context stream, and the per-kernel totals of that call:
+ * names[i] (NUL-separated, caller buffer of len bytes), ms[i], launches[i]
+ * for i < *nkernels (in/out: capacity in, count out).
+ */
+int spoc_last_timing(spoc_ctx *ctx, double *total_ms, int *nkernels, char *names, int len,
+                     double *ms, int *launches);
+int spoc_set_profiling(spoc_ctx *ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

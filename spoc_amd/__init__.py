@@ -1,0 +1,194 @@
+"""MI355X (gfx950) right-hand side of SPOC's discontinuous-Galerkin shallow-water
+model, behind a C-ABI (include/spoc_abi.h).
+
+This module is the host-side mirror of the reference's operator interface:
+``Context.cal_var`` takes the same operands as the reference subroutine
+``cal_var`` (src/calvar_lin.f:7-10 and the three nonlinear variants) and
+returns the tendencies ``(u1, v1, h1)``; ``Context.rk4`` runs the driver's RK4
+loop (src/shwwnl.f:233-284) on device-resident state.  All arithmetic runs
+in ``libspoc_amd.so``; if the library is missing this module raises instead
+of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libspoc_amd.so")
+
+LIN, FS, NS, INV = 0, 1, 2, 3
+VARIANTS = {"lin": LIN, "fs": FS, "ns": NS, "inv": INV}
+FP64, FP32_MFMA = 0, 1
+PRECISIONS = {"fp64": FP64, "fp32": FP32_MFMA}
+LDM = 45  # nnmod, the reference's leading dimension of modal arrays
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_lib = None
+
+
+class SpocError(RuntimeError):
+    pass
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load libspoc_amd.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise SpocError(f"{path} not found: build it with `python -m spoc_amd.build`")
+    lib = C.CDLL(path)
+    lib.spoc_last_error.restype = C.c_char_p
+    for name in ("spoc_ctx_create", "spoc_ctx_destroy", "spoc_mesh_set", "spoc_fields_set",
+                 "spoc_rhs", "spoc_state_upload", "spoc_state_download", "spoc_rk4_steps",
+                 "spoc_energy", "spoc_sync", "spoc_tables_build", "spoc_last_timing",
+                 "spoc_set_profiling", "spoc_version"):
+        getattr(lib, name).restype = C.c_int
+    lib.spoc_ctx_destroy.argtypes = [C.c_void_p]
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise SpocError(f"spoc error {rc}: {load_library().spoc_last_error().decode()}")
+
+
+def _d(a: np.ndarray):
+    assert a.dtype == np.float64
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a: np.ndarray):
+    assert a.dtype == np.int32
+    return a.ctypes.data_as(_ip)
+
+
+def nm_of(nc: int) -> int:
+    return (nc + 1) * (nc + 2) // 2
+
+
+def build_tables(nc: int, wi, xi, yi, wif, tif) -> dict:
+    """Operator tables as the reference driver builds them (src/shwwnl.f:127-156),
+    returned in the COMMON layouts (pi2(33|ng,45), pico(9,45,3), (45,45))."""
+    lib = load_library()
+    ng, ngf = len(wi), len(wif)
+    ldg, ldf, ldm = max(33, ng), 9, LDM
+    arr = lambda *s: np.zeros(s, order="F")
+    t = dict(pi2=arr(ldg, ldm), pico=arr(ldf, ldm, 3), amm=arr(ldm, ldm), amml=arr(ldm, ldm),
+             agx=arr(ldm, ldm), agy=arr(ldm, ldm), ammo=arr(ldm, ldm))
+    q = [np.ascontiguousarray(a, np.float64) for a in (wi, xi, yi, wif, tif)]
+    _check(lib.spoc_tables_build(nc, ng, ngf, *(_d(a) for a in q), ldg, ldf, ldm,
+                                 _d(t["pi2"]), _d(t["pico"]), _d(t["amm"]), _d(t["amml"]),
+                                 _d(t["agx"]), _d(t["agy"]), _d(t["ammo"])))
+    t.update(nc=nc, ng=ng, ngf=ngf, nm=nm_of(nc), wi=q[0], xi=q[1], yi=q[2], wif=q[3], tif=q[4],
+             ldg=ldg)
+    return t
+
+
+class Context:
+    """One device context: tables, a mesh, physics fields and RK4 state."""
+
+    def __init__(self, tables: dict, device: int = 0):
+        self.lib = load_library()
+        self.t = tables
+        self.nc, self.ng, self.ngf, self.nm = tables["nc"], tables["ng"], tables["ngf"], tables["nm"]
+        self._h = C.c_void_p()
+        t = tables
+        _check(self.lib.spoc_ctx_create(C.byref(self._h), device, self.nc, self.ng, self.ngf,
+                                        _d(t["wi"]), _d(t["wif"]), _d(t["pi2"]), t["ldg"],
+                                        _d(t["pico"]), 9, LDM, _d(t["amm"]), _d(t["amml"]),
+                                        _d(t["agx"]), _d(t["agy"]), _d(t["ammo"])))
+        self.ne = self.nf = 0
+
+    def close(self):
+        if self._h:
+            self.lib.spoc_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- operands -----------------------------------------------------------
+    def set_mesh(self, m: dict) -> None:
+        """m: the arrays of spoc_amd.mesh.build (or the reference's own)."""
+        self._mesh = m  # keep the host arrays alive while in use
+        ne, nf = int(m["itm"].shape[1]), int(m["nf"])
+        have_nodes = "in" in m and "x" in m
+        tri = np.asfortranarray(m["in"], np.int32) if have_nodes else None
+        x = np.ascontiguousarray(m["x"], np.float64) if have_nodes else None
+        y = np.ascontiguousarray(m["y"], np.float64) if have_nodes else None
+        self._nodes = (tri, x, y)
+        _check(self.lib.spoc_mesh_set(
+            self._h, ne, nf, _i(np.asfortranarray(m["itm"], np.int32)),
+            _i(np.asfortranarray(m["its"], np.int32)), _i(np.asfortranarray(m["tseg"], np.int32)),
+            _i(np.asfortranarray(m["ssit"], np.int32)), _d(np.asfortranarray(m["norm"])),
+            _d(np.asfortranarray(m["vect"])), _d(np.asfortranarray(m["normt"])),
+            _i(np.ascontiguousarray(m["bdl"], np.int32)),
+            len(x) if have_nodes else 0, _i(tri) if have_nodes else None,
+            _d(x) if have_nodes else None, _d(y) if have_nodes else None))
+        self.ne, self.nf = ne, nf
+
+    def set_fields(self, hb, g0, fc=None, w0=None, z0=None, mu0=0.0, mb0=0.0) -> None:
+        ld = hb.shape[0]
+        f = [np.asfortranarray(a, np.float64) if a is not None else None for a in (fc, w0, z0, hb)]
+        _check(self.lib.spoc_fields_set(self._h, ld, *(None if a is None else _d(a) for a in f),
+                                        C.c_double(mu0), C.c_double(mb0), C.c_double(g0)))
+
+    # -- the reference call -------------------------------------------------
+    def cal_var(self, variant, u0, v0, h0, precision="fp64"):
+        """Tendencies (u1, v1, h1) of the given state, shaped like u0."""
+        var = VARIANTS[variant] if isinstance(variant, str) else variant
+        prec = PRECISIONS[precision] if isinstance(precision, str) else precision
+        u0, v0, h0 = (np.asfortranarray(a, np.float64) for a in (u0, v0, h0))
+        ld = u0.shape[0]
+        out = [np.zeros_like(u0, order="F") for _ in range(3)]
+        _check(self.lib.spoc_rhs(self._h, var, prec, ld, _d(u0), _d(v0), _d(h0),
+                                 *(_d(a) for a in out)))
+        return tuple(out)
+
+    # -- device-resident RK4 ------------------------------------------------
+    def upload(self, u0, v0, h0) -> None:
+        u0, v0, h0 = (np.asfortranarray(a, np.float64) for a in (u0, v0, h0))
+        _check(self.lib.spoc_state_upload(self._h, u0.shape[0], _d(u0), _d(v0), _d(h0)))
+
+    def download(self, ld: int = LDM):
+        out = [np.zeros((ld, self.ne), order="F") for _ in range(3)]
+        _check(self.lib.spoc_state_download(self._h, ld, *(_d(a) for a in out)))
+        return tuple(out)
+
+    def rk4(self, variant, dt: float, nsteps: int, precision="fp64") -> float:
+        """Advance nsteps; returns the device time of the call in ms."""
+        var = VARIANTS[variant] if isinstance(variant, str) else variant
+        prec = PRECISIONS[precision] if isinstance(precision, str) else precision
+        _check(self.lib.spoc_rk4_steps(self._h, var, prec, C.c_double(dt), nsteps))
+        return self.timing()[0]
+
+    def sync(self) -> None:
+        _check(self.lib.spoc_sync(self._h))
+
+    def energy(self):
+        ke, pe, mass = C.c_double(), C.c_double(), C.c_double()
+        _check(self.lib.spoc_energy(self._h, C.byref(ke), C.byref(pe), C.byref(mass)))
+        return ke.value, pe.value, mass.value
+
+    def set_profiling(self, on: bool) -> None:
+        _check(self.lib.spoc_set_profiling(self._h, int(on)))
+
+    def timing(self):
+        tot = C.c_double()
+        n = C.c_int(16)
+        names = C.create_string_buffer(1024)
+        ms = (C.c_double * 16)()
+        cnt = (C.c_int * 16)()
+        _check(self.lib.spoc_last_timing(self._h, C.byref(tot), C.byref(n), names, 1024, ms, cnt))
+        parts = names.raw.split(b"\0")
+        kern = {parts[i].decode(): (ms[i], cnt[i]) for i in range(n.value)}
+        return tot.value, kern

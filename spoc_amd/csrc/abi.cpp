@@ -1,0 +1,709 @@
+// Host side of the C-ABI (include/spoc_abi.h): contexts, table and mesh
+// ingest, device-resident RK4 driver, timing.  Compiled without FMA
+// contraction so the host-side table builder reproduces the reference
+// driver's set-up arithmetic exactly.
+#include "../../include/spoc_abi.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "launch.h"
+
+using namespace spoc;
+
+namespace spoc {
+int energy_launch(hipStream_t st, int ne, int nm, int ng, const double *u, const double *v,
+                  const double *h, const double *hb, const double *pi2, const double *wi,
+                  const double *ammo1, const double *deta, double g0, double *scratch,
+                  double *out);
+int rhs_f32(int nc, int ng, int ngf, const RhsArgs &a, void *plan);
+}  // namespace spoc
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(x)                                                                     \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess)                                                             \
+      return fail(SPOC_EHIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                                \
+  } while (0)
+
+// ------------------------------------------------------------------------
+// Operator tables from the quadrature rule, in the order of the driver's
+// set-up (src/shwwnl.f:127-156).
+// ------------------------------------------------------------------------
+double legendre(int n, double x) {  // src/legendre2.f:5-30
+  double p0 = 1.0, p1 = x, p = p0;
+  if (n == 0) return p0;
+  if (n == 1) return p1;
+  for (int i = 2; i <= n; i++) {
+    double a0 = (double)i, a2 = (double)(2 * i - 1), a3 = (double)(i - 1);
+    p = (a2 * x * p1 - a3 * p0) / a0;
+    p0 = p1;
+    p1 = p;
+  }
+  return p;
+}
+
+double dlegendre(int n, double x) {  // src/legendre2.f:32-62
+  double p0 = 1.0, p1 = x, d0 = 0.0, d1 = 1.0, d = d0;
+  if (n == 0) return d0;
+  if (n == 1) return d1;
+  for (int i = 2; i <= n; i++) {
+    double a0 = (double)i, a2 = (double)(2 * i - 1), a3 = (double)(i - 1);
+    d = (a2 * x * d1 + a2 * p1 - a3 * d0) / a0;
+    double p2 = (a2 * x * p1 - a3 * p0) / a0;
+    p0 = p1;
+    p1 = p2;
+    d0 = d1;
+    d1 = d;
+  }
+  return d;
+}
+
+// Upper Cholesky in the column order of the unblocked LAPACK routine the
+// reference's DPOTRF takes for these sizes (src/solve.f:108-113, 383-403).
+int cholesky_upper(int n, double *a, int ld) {
+  for (int j = 0; j < n; j++) {
+    double dot = 0.0;
+    for (int i = 0; i < j; i++) dot = dot + a[i + ld * j] * a[i + ld * j];
+    double ajj = a[j + ld * j] - dot;
+    if (ajj <= 0.0) {
+      a[j + ld * j] = ajj;
+      return j + 1;
+    }
+    ajj = std::sqrt(ajj);
+    a[j + ld * j] = ajj;
+    if (j + 1 < n) {
+      if (j > 0)
+        for (int c = j + 1; c < n; c++) {
+          double t = 0.0;
+          for (int i = 0; i < j; i++) t = t + a[i + ld * c] * a[i + ld * j];
+          a[j + ld * c] = a[j + ld * c] + (-1.0) * t;
+        }
+      double r = 1.0 / ajj;
+      for (int c = j + 1; c < n; c++) a[j + ld * c] = r * a[j + ld * c];
+    }
+  }
+  return 0;
+}
+
+struct Prof {
+  bool on = false;
+  std::vector<hipEvent_t> ev;   // pool
+  size_t used = 0;
+  std::vector<std::pair<int, size_t>> marks;  // (slot, event index)
+};
+
+}  // namespace
+
+struct spoc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nc = 0, ng = 0, ngf = 0, nm = 0;
+  std::vector<double> h_wi, h_ammo1;
+  // device tables
+  double *dtab = nullptr;
+  DevTables tab{};
+  // mesh
+  int ne = 0, nf = 0;
+  void *dmesh = nullptr;
+  DevMesh mesh{};
+  bool have_nodes = false;
+  // fields
+  double *dfields = nullptr;  // fc, w0, z0, hb
+  Physics phys{};
+  bool have_fields = false;
+  // state and work
+  double *dstate = nullptr;  // u0 x3, x x3, acc x3, k x3, work x8
+  double *dface = nullptr;
+  double **dFptr = nullptr;
+  double *den = nullptr;     // energy scratch
+  bool have_state = false;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  double last_ms = 0.0;
+  Prof prof;
+  std::vector<std::string> knames;
+  std::vector<double> kms;
+  std::vector<int> kcount;
+  void *f32plan = nullptr;
+};
+
+namespace {
+
+void free_mesh(spoc_ctx *c) {
+  if (c->dmesh) (void)hipFree(c->dmesh);
+  if (c->dstate) (void)hipFree(c->dstate);
+  if (c->dface) (void)hipFree(c->dface);
+  if (c->den) (void)hipFree(c->den);
+  if (c->dfields) (void)hipFree(c->dfields);
+  if (c->dFptr) (void)hipFree(c->dFptr);
+  c->dFptr = nullptr;
+  c->dmesh = nullptr;
+  c->dstate = nullptr;
+  c->dface = nullptr;
+  c->den = nullptr;
+  c->dfields = nullptr;
+  c->have_state = c->have_fields = false;
+}
+
+double *state_ptr(spoc_ctx *c, int slot) { return c->dstate + (size_t)slot * c->nm * c->ne; }
+
+void prof_hook(void *user, int slot, hipStream_t st) {
+  spoc_ctx *c = (spoc_ctx *)user;
+  if (!c->prof.on) return;
+  if (c->prof.used >= c->prof.ev.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    c->prof.ev.push_back(e);
+  }
+  size_t i = c->prof.used++;
+  (void)hipEventRecord(c->prof.ev[i], st);
+  c->prof.marks.push_back({slot, i});
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *spoc_last_error(void) { return g_err.c_str(); }
+int spoc_version(void) { return 1; }
+
+int spoc_tables_build(int nc, int ng, int ngf, const double *wi, const double *xi,
+                      const double *yi, const double *wif, const double *tif, int ld_g,
+                      int ld_f, int ld_m, double *pi2, double *pico, double *amm, double *amml,
+                      double *agx, double *agy, double *ammo) {
+  (void)wif;
+  int nm = nm_of(nc);
+  if (nc < 1 || nc > 8 || ng < 1 || ngf < 1 || ld_g < ng || ld_f < ngf || ld_m < nm)
+    return fail(SPOC_EINVAL, "spoc_tables_build: bad sizes nc=%d ng=%d ngf=%d", nc, ng, ngf);
+  // basis phi_m = P_a(xi) P_b(eta), m = ip(nc, a) + b (src/nonlin.f:19-28)
+  int m = 0;
+  for (int a = 0; a <= nc; a++)
+    for (int b = 0; b <= nc - a; b++, m++) {
+      for (int q = 0; q < ng; q++) pi2[q + (size_t)ld_g * m] = legendre(a, xi[q]) * legendre(b, yi[q]);
+      for (int j = 0; j < ngf; j++) {
+        double t = tif[j];
+        pico[j + (size_t)ld_f * (m + (size_t)ld_m * 0)] = legendre(a, t) * legendre(b, -1.0);
+        pico[j + (size_t)ld_f * (m + (size_t)ld_m * 1)] = legendre(a, -t) * legendre(b, t);
+        pico[j + (size_t)ld_f * (m + (size_t)ld_m * 2)] = legendre(a, -1.0) * legendre(b, -t);
+      }
+    }
+  // mass and derivative matrices by quadrature (src/mat1.f:90-142)
+  std::vector<double> M0((size_t)ld_m * ld_m, 0.0);
+  for (double *mat : {M0.data(), agx, agy})
+    for (int i = 0; i < ld_m * ld_m; i++) mat[i] = 0.0;
+  int i1 = 0;
+  for (int k1 = 0; k1 <= nc; k1++)
+    for (int k2 = 0; k2 <= nc - k1; k2++, i1++) {
+      int i2 = 0;
+      for (int k3 = 0; k3 <= nc; k3++)
+        for (int k4 = 0; k4 <= nc - k3; k4++, i2++) {
+          double s0 = 0.0, sx = 0.0, sy = 0.0;
+          for (int q = 0; q < ng; q++) {
+            double x = xi[q], y = yi[q];
+            s0 = s0 + wi[q] * (legendre(k1, x) * legendre(k2, y) * legendre(k3, x) * legendre(k4, y));
+            sx = sx + wi[q] * (legendre(k1, x) * legendre(k2, y) * dlegendre(k3, x) * legendre(k4, y));
+            sy = sy + wi[q] * (legendre(k1, x) * legendre(k2, y) * legendre(k3, x) * dlegendre(k4, y));
+          }
+          M0[i1 + (size_t)ld_m * i2] = s0;
+          agx[i1 + (size_t)ld_m * i2] = sx;
+          agy[i1 + (size_t)ld_m * i2] = sy;
+        }
+    }
+  // reduced-basis mass matrix (src/mat1.f:285-368)
+  for (int i = 0; i < ld_m * ld_m; i++) amml[i] = 0.0;
+  const int js[4] = {0, 1, 0, 1}, ls[4] = {0, 0, 1, 1};
+  for (int pass = 0; pass < 4; pass++)
+    for (int i = 0; i <= nc - 1; i++)
+      for (int j = js[pass]; j <= nc - 1 - i; j += 2)
+        for (int k = 0; k <= nc - 1; k++)
+          for (int l = ls[pass]; l <= nc - 1 - k; l += 2) {
+            int r = ip_of(nc - 1, i) + j, c = ip_of(nc - 1, k) + l;
+            int ri = ip_of(nc, i) + j + 1, ci = ip_of(nc, k) + l + 1;
+            int r0 = ip_of(nc, i), c0 = ip_of(nc, k);
+            double a22 = M0[ri + (size_t)ld_m * ci], a02 = M0[r0 + (size_t)ld_m * ci];
+            double a20 = M0[ri + (size_t)ld_m * c0], a00 = M0[r0 + (size_t)ld_m * c0];
+            double v = pass == 0   ? a22 + a02 + a20 + a00
+                       : pass == 1 ? a22 - a02 + a20 - a00
+                       : pass == 2 ? a22 + a02 - a20 - a00
+                                   : a22 - a02 - a20 + a00;
+            amml[r + (size_t)ld_m * c] = v;
+          }
+  for (int i = 0; i < ld_m * ld_m; i++) {
+    ammo[i] = M0[i];
+    amm[i] = M0[i];
+  }
+  int info1 = cholesky_upper(nm, amm, ld_m);
+  int info2 = cholesky_upper(nm - nc - 1, amml, ld_m);
+  if (info1 || info2)
+    return fail(SPOC_ESINGULAR,
+                "mass matrix not positive definite (p=%d with a %d-point rule: need >= %d points)",
+                nc, ng, nm);
+  return SPOC_OK;
+}
+
+int spoc_ctx_create(spoc_ctx **out, int device, int nc, int ng, int ngf, const double *wi,
+                    const double *wif, const double *pi2, int ld_g, const double *pico,
+                    int ld_f, int ld_m, const double *amm, const double *amml,
+                    const double *agx, const double *agy, const double *ammo) {
+  if (!out) return fail(SPOC_EINVAL, "spoc_ctx_create: null ctx pointer");
+  *out = nullptr;
+  int nm = nm_of(nc);
+  bool ok = false;
+#define SPOC_OKSHAPE(NC_, NG_, NGF_) ok = ok || (nc == NC_ && ng == NG_ && ngf == NGF_);
+  SPOC_FOR_EACH_SHAPE(SPOC_OKSHAPE)
+#undef SPOC_OKSHAPE
+  if (!ok)
+    return fail(SPOC_EINVAL,
+                "unsupported (p, triangle points, edge points) = (%d, %d, %d); supported: "
+                "(2,6,3) (3,12,4) (4,25,5) (5,25,6) (6,33,7) (7,42,8)",
+                nc, ng, ngf);
+  if (ld_g < ng || ld_f < ngf || ld_m < nm) return fail(SPOC_EINVAL, "leading dimension too small");
+  for (int i = 0; i < nm; i++)
+    if (!(amm[i + (size_t)ld_m * i] > 0.0))
+      return fail(SPOC_ESINGULAR, "amm factor has a non-positive diagonal (DPOTRF failed?)");
+  HIPCHK(hipSetDevice(device));
+  spoc_ctx *c = new spoc_ctx();
+  c->device = device;
+  c->nc = nc;
+  c->ng = ng;
+  c->ngf = ngf;
+  c->nm = nm;
+  int nl = nm - nc - 1;
+  // host-side layout conversion
+  std::vector<double> agxT(nm * nm), agyT(nm * nm), p2(ng * nm), p2T(nm * ng),
+      pc(ngf * nm * 3), pcT(nm * ngf * 3), U(nm * nm, 0.0), UL(nm * nm, 0.0), d1(nm), d2(nm, 0.0),
+      w(ng), wf(ngf), a1(nm);
+  for (int j = 0; j < nm; j++)
+    for (int i = 0; i < nm; i++) {
+      agxT[i + nm * j] = agx[j + (size_t)ld_m * i];
+      agyT[i + nm * j] = agy[j + (size_t)ld_m * i];
+    }
+  for (int m = 0; m < nm; m++)
+    for (int q = 0; q < ng; q++) {
+      p2[q + ng * m] = pi2[q + (size_t)ld_g * m];
+      p2T[m + nm * q] = pi2[q + (size_t)ld_g * m];
+    }
+  for (int k = 0; k < 3; k++)
+    for (int m = 0; m < nm; m++)
+      for (int j = 0; j < ngf; j++) {
+        double v = pico[j + (size_t)ld_f * (m + (size_t)ld_m * k)];
+        pc[j + ngf * (m + nm * k)] = v;
+        pcT[m + nm * (j + ngf * k)] = v;
+      }
+  for (int i = 0; i < nm; i++)
+    for (int k = 0; k <= i; k++) U[k + nm * i] = amm[k + (size_t)ld_m * i];
+  for (int i = 0; i < nl; i++)
+    for (int k = 0; k <= i; k++) UL[k + nm * i] = amml[k + (size_t)ld_m * i];
+  for (int i = 0; i < nm; i++) d1[i] = 1.0 / amm[i + (size_t)ld_m * i];
+  for (int i = 0; i < nl; i++) d2[i] = 1.0 / amml[i + (size_t)ld_m * i];
+  for (int q = 0; q < ng; q++) w[q] = wi[q];
+  for (int j = 0; j < ngf; j++) wf[j] = wif[j];
+  for (int i = 0; i < nm; i++) a1[i] = ammo[i];
+  c->h_wi = w;
+  c->h_ammo1 = a1;
+  std::vector<std::vector<double> *> parts = {&agxT, &agyT, &p2, &p2T, &pc, &pcT, &U,
+                                              &UL,   &d1,   &d2, &w,   &wf,  &a1};
+  size_t tot = 0;
+  for (auto *p : parts) tot += (p->size() + 31) / 32 * 32;
+  hipError_t he = hipStreamCreate(&c->stream);
+  if (he == hipSuccess) he = hipMalloc(&c->dtab, tot * sizeof(double));
+  if (he != hipSuccess) {
+    delete c;
+    return fail(SPOC_EHIP, "device allocation failed: %s", hipGetErrorString(he));
+  }
+  std::vector<double> host(tot, 0.0);
+  std::vector<const double *> dptr;
+  size_t off = 0;
+  for (auto *p : parts) {
+    std::memcpy(host.data() + off, p->data(), p->size() * sizeof(double));
+    dptr.push_back(c->dtab + off);
+    off += (p->size() + 31) / 32 * 32;
+  }
+  he = hipMemcpy(c->dtab, host.data(), tot * sizeof(double), hipMemcpyHostToDevice);
+  if (he != hipSuccess) {
+    (void)hipFree(c->dtab);
+    delete c;
+    return fail(SPOC_EHIP, "table upload failed: %s", hipGetErrorString(he));
+  }
+  c->tab = DevTables{dptr[0], dptr[1], dptr[2], dptr[3], dptr[4],  dptr[5], dptr[6],
+                     dptr[7], dptr[8], dptr[9], dptr[10], dptr[11], dptr[12]};
+  (void)hipEventCreate(&c->t0);
+  (void)hipEventCreate(&c->t1);
+  *out = c;
+  return SPOC_OK;
+}
+
+int spoc_ctx_destroy(spoc_ctx *c) {
+  if (!c) return SPOC_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  free_mesh(c);
+  if (c->dtab) (void)hipFree(c->dtab);
+  for (auto e : c->prof.ev) (void)hipEventDestroy(e);
+  if (c->t0) (void)hipEventDestroy(c->t0);
+  if (c->t1) (void)hipEventDestroy(c->t1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return SPOC_OK;
+}
+
+int spoc_mesh_set(spoc_ctx *c, int ne, int nf, const int *itm, const int *its, const int *tseg,
+                  const int *ssit, const double *norm, const double *vect, const double *normt,
+                  const int *bdl, int nn, const int *in, const double *xgr, const double *ygr) {
+  if (!c) return fail(SPOC_EINVAL, "null context");
+  if (ne <= 0 || nf <= 0) return fail(SPOC_EINVAL, "spoc_mesh_set: ne=%d nf=%d", ne, nf);
+  // validate connectivity on the host before any kernel indexes with it
+  for (int e = 0; e < ne; e++)
+    for (int k = 0; k < 3; k++) {
+      int f = itm[3 * e + k];
+      if (f < 1 || f > nf) return fail(SPOC_EINVAL, "itm(%d,%d)=%d out of range", k + 1, e + 1, f);
+      if (its[3 * e + k] != 1 && its[3 * e + k] != -1)
+        return fail(SPOC_EINVAL, "its(%d,%d)=%d is not +-1", k + 1, e + 1, its[3 * e + k]);
+    }
+  for (int f = 0; f < nf; f++) {
+    int a = tseg[2 * f], b = tseg[2 * f + 1];
+    if (a < 1 || a > ne || b < 0 || b > ne)
+      return fail(SPOC_EINVAL, "tseg(:,%d)=(%d,%d) out of range", f + 1, a, b);
+    int s1 = ssit[2 * f], s2 = ssit[2 * f + 1];
+    if (s1 < 1 || s1 > 3 || (b > 0 && (s2 < 1 || s2 > 3)))
+      return fail(SPOC_EINVAL, "ssit(:,%d)=(%d,%d) out of range", f + 1, s1, s2);
+  }
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  free_mesh(c);
+  int nm = c->nm, ngf = c->ngf;
+  std::vector<double> deta(ne, 0.0);
+  c->have_nodes = (in && xgr && ygr && nn > 0);
+  if (c->have_nodes) {
+    for (int e = 0; e < ne; e++) {  // src/energy.f:37-42
+      int i1 = in[3 * e] - 1, i2 = in[3 * e + 1] - 1, i3 = in[3 * e + 2] - 1;
+      if (i1 < 0 || i2 < 0 || i3 < 0 || i1 >= nn || i2 >= nn || i3 >= nn)
+        return fail(SPOC_EINVAL, "in(:,%d) out of range", e + 1);
+      double ax = (xgr[i2] - xgr[i1]) * 0.5, bx = (xgr[i3] - xgr[i1]) * 0.5;
+      double ay = (ygr[i2] - ygr[i1]) * 0.5, by = (ygr[i3] - ygr[i1]) * 0.5;
+      deta[e] = ax * by - ay * bx;
+    }
+  }
+  size_t ibytes = sizeof(int) * ((size_t)3 * ne * 2 + (size_t)2 * nf * 2 + ne);
+  size_t dbytes = sizeof(double) * ((size_t)2 * nf + (size_t)5 * ne + (size_t)2 * ne + ne);
+  char *blob = nullptr;
+  HIPCHK(hipMalloc((void **)&blob, dbytes + ibytes + 256));
+  c->dmesh = blob;
+  double *d = (double *)blob;
+  double *dnorm = d, *dvect = dnorm + 2 * (size_t)nf, *dnormt = dvect + 5 * (size_t)ne,
+         *ddeta = dnormt + 2 * (size_t)ne;
+  int *ip = (int *)(ddeta + ne);
+  int *ditm = ip, *dits = ditm + 3 * (size_t)ne, *dtseg = dits + 3 * (size_t)ne,
+      *dssit = dtseg + 2 * (size_t)nf, *dbdl = dssit + 2 * (size_t)nf;
+  std::vector<int> b(ne), ss(2 * (size_t)nf);
+  for (int e = 0; e < ne; e++) b[e] = bdl[e] != 0;
+  for (size_t i = 0; i < 2 * (size_t)nf; i++) ss[i] = ssit[i];
+  for (int f = 0; f < nf; f++)
+    if (tseg[2 * f + 1] == 0) ss[2 * f + 1] = 1;  // unused on walls; keep in range
+  HIPCHK(hipMemcpy(dnorm, norm, sizeof(double) * 2 * nf, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dvect, vect, sizeof(double) * 5 * ne, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dnormt, normt, sizeof(double) * 2 * ne, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ddeta, deta.data(), sizeof(double) * ne, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ditm, itm, sizeof(int) * 3 * ne, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dits, its, sizeof(int) * 3 * ne, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dtseg, tseg, sizeof(int) * 2 * nf, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dssit, ss.data(), sizeof(int) * 2 * nf, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dbdl, b.data(), sizeof(int) * ne, hipMemcpyHostToDevice));
+  c->ne = ne;
+  c->nf = nf;
+  c->mesh = DevMesh{ne, nf, ditm, dits, dtseg, dssit, dnorm, dvect, dnormt, dbdl, ddeta};
+  // state: u0(3) x(3) acc(3) k(3) work(8)
+  HIPCHK(hipMalloc((void **)&c->dstate, sizeof(double) * (size_t)20 * nm * ne));
+  HIPCHK(hipMemset(c->dstate, 0, sizeof(double) * (size_t)20 * nm * ne));
+  HIPCHK(hipMalloc((void **)&c->dface, sizeof(double) * (size_t)nf * 6 * ngf));
+  HIPCHK(hipMalloc((void **)&c->den, sizeof(double) * ((size_t)ne * (2 + nm) + 8)));
+  HIPCHK(hipMalloc((void **)&c->dFptr, sizeof(double *) * 4));
+  double *fp[4];
+  for (int i = 0; i < 4; i++) fp[i] = state_ptr(c, 12 + i);
+  HIPCHK(hipMemcpy(c->dFptr, fp, sizeof fp, hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc((void **)&c->dfields, sizeof(double) * (size_t)4 * nm * ne));
+  HIPCHK(hipMemset(c->dfields, 0, sizeof(double) * (size_t)4 * nm * ne));
+  return SPOC_OK;
+}
+
+static int upload_fields(spoc_ctx *c, int ld, const double *const *src, double *const *dst,
+                         int n) {
+  int nm = c->nm, ne = c->ne;
+  std::vector<double> pack((size_t)nm * ne);
+  for (int f = 0; f < n; f++) {
+    if (!src[f]) continue;
+    for (int e = 0; e < ne; e++)
+      std::memcpy(pack.data() + (size_t)e * nm, src[f] + (size_t)ld * e, sizeof(double) * nm);
+    HIPCHK(hipMemcpyAsync(dst[f], pack.data(), sizeof(double) * nm * ne, hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return SPOC_OK;
+}
+
+static int download_fields(spoc_ctx *c, int ld, double *const *dst, double *const *src, int n) {
+  int nm = c->nm, ne = c->ne;
+  std::vector<double> pack((size_t)nm * ne);
+  for (int f = 0; f < n; f++) {
+    HIPCHK(hipMemcpyAsync(pack.data(), src[f], sizeof(double) * nm * ne, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int e = 0; e < ne; e++)
+      std::memcpy(dst[f] + (size_t)ld * e, pack.data() + (size_t)e * nm, sizeof(double) * nm);
+  }
+  return SPOC_OK;
+}
+
+int spoc_fields_set(spoc_ctx *c, int ld, const double *fc, const double *w0, const double *z0,
+                    const double *hb, double mu0, double mb0, double g0) {
+  if (!c || !c->dmesh) return fail(SPOC_ESTATE, "spoc_fields_set: set the mesh first");
+  if (ld < c->nm || !hb) return fail(SPOC_EINVAL, "spoc_fields_set: ld=%d, hb=%p", ld, hb);
+  (void)hipSetDevice(c->device);
+  size_t n = (size_t)c->nm * c->ne;
+  double *dst[4] = {c->dfields, c->dfields + n, c->dfields + 2 * n, c->dfields + 3 * n};
+  const double *src[4] = {fc, w0, z0, hb};
+  HIPCHK(hipMemset(c->dfields, 0, sizeof(double) * 3 * n));
+  int rc = upload_fields(c, ld, src, dst, 4);
+  if (rc) return rc;
+  c->phys = Physics{mu0, mb0, g0, hb[0]};  // hh0 = hb(1,1) (src/calvar_lin.f:65)
+  c->have_fields = true;
+  return SPOC_OK;
+}
+
+int spoc_state_upload(spoc_ctx *c, int ld, const double *u0, const double *v0,
+                      const double *h0) {
+  if (!c || !c->dmesh) return fail(SPOC_ESTATE, "spoc_state_upload: set the mesh first");
+  if (ld < c->nm || !u0 || !v0 || !h0) return fail(SPOC_EINVAL, "spoc_state_upload: bad args");
+  (void)hipSetDevice(c->device);
+  const double *src[3] = {u0, v0, h0};
+  double *dst[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
+  int rc = upload_fields(c, ld, src, dst, 3);
+  if (!rc) c->have_state = true;
+  return rc;
+}
+
+int spoc_state_download(spoc_ctx *c, int ld, double *u0, double *v0, double *h0) {
+  if (!c || !c->have_state) return fail(SPOC_ESTATE, "spoc_state_download: no state");
+  if (ld < c->nm || !u0 || !v0 || !h0) return fail(SPOC_EINVAL, "spoc_state_download: bad args");
+  (void)hipSetDevice(c->device);
+  double *dst[3] = {u0, v0, h0};
+  double *src[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
+  return download_fields(c, ld, dst, src, 3);
+}
+
+static RhsArgs make_args(spoc_ctx *c, int variant, const double *const *x, int stage,
+                         double dt) {
+  RhsArgs a{};
+  a.variant = variant;
+  a.stream = c->stream;
+  a.tab = c->tab;
+  a.mesh = c->mesh;
+  a.phys = c->phys;
+  size_t n = (size_t)c->nm * c->ne;
+  for (int i = 0; i < 3; i++) {
+    a.x[i] = x[i];
+    a.u0[i] = state_ptr(c, i);
+    a.xout[i] = state_ptr(c, 3 + i);
+    a.acc[i] = state_ptr(c, 6 + i);
+    a.k[i] = state_ptr(c, 9 + i);
+  }
+  for (int i = 0; i < 8; i++) a.work[i] = state_ptr(c, 12 + i);
+  a.fc = c->dfields;
+  a.w0 = c->dfields + n;
+  a.z0 = c->dfields + 2 * n;
+  a.hb = c->dfields + 3 * n;
+  a.stage = stage;
+  a.dt = dt;
+  a.Fptr = c->dFptr;
+  a.face = c->dface;
+  a.hook = prof_hook;
+  a.hook_user = c;
+  return a;
+}
+
+static int run_rhs(spoc_ctx *c, int precision, const RhsArgs &a) {
+  int rc = -1;
+  if (precision == SPOC_FP64) {
+#define SPOC_CALL(NC_, NG_, NGF_) \
+  if (c->nc == NC_ && c->ng == NG_ && c->ngf == NGF_) rc = rhs_f64_##NC_##_##NG_##_##NGF_(a);
+    SPOC_FOR_EACH_SHAPE(SPOC_CALL)
+#undef SPOC_CALL
+  } else if (precision == SPOC_FP32_MFMA) {
+    rc = rhs_f32(c->nc, c->ng, c->ngf, a, c->f32plan);
+  } else {
+    return fail(SPOC_EINVAL, "unknown precision %d", precision);
+  }
+  if (rc == -1) return fail(SPOC_EINVAL, "no kernel for this shape/precision");
+  if (rc) {
+    hipError_t e = hipGetLastError();
+    return fail(SPOC_EHIP, "kernel launch failed: %s", hipGetErrorString(e));
+  }
+  return SPOC_OK;
+}
+
+static int check_ready(spoc_ctx *c, int variant) {
+  if (!c) return fail(SPOC_EINVAL, "null context");
+  if (!c->dmesh) return fail(SPOC_ESTATE, "no mesh: call spoc_mesh_set first");
+  if (!c->have_fields) return fail(SPOC_ESTATE, "no fields: call spoc_fields_set first");
+  if (variant < SPOC_LIN || variant > SPOC_INV) return fail(SPOC_EINVAL, "bad variant %d", variant);
+  return SPOC_OK;
+}
+
+int spoc_rhs(spoc_ctx *c, int variant, int precision, int ld, const double *u0,
+             const double *v0, const double *h0, double *u1, double *v1, double *h1) {
+  int rc = check_ready(c, variant);
+  if (rc) return rc;
+  if (ld < c->nm) return fail(SPOC_EINVAL, "ld=%d < nm=%d", ld, c->nm);
+  (void)hipSetDevice(c->device);
+  // stage input goes to the x slots (3..5); tendencies come back from k (9..11)
+  const double *src[3] = {u0, v0, h0};
+  double *xs[3] = {state_ptr(c, 3), state_ptr(c, 4), state_ptr(c, 5)};
+  rc = upload_fields(c, ld, src, xs, 3);
+  if (rc) return rc;
+  const double *x[3] = {xs[0], xs[1], xs[2]};
+  RhsArgs a = make_args(c, variant, x, -1, 0.0);
+  rc = run_rhs(c, precision, a);
+  if (rc) return rc;
+  double *dst[3] = {u1, v1, h1};
+  double *ks[3] = {state_ptr(c, 9), state_ptr(c, 10), state_ptr(c, 11)};
+  return download_fields(c, ld, dst, ks, 3);
+}
+
+int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nsteps) {
+  int rc = check_ready(c, variant);
+  if (rc) return rc;
+  if (!c->have_state) return fail(SPOC_ESTATE, "no state: call spoc_state_upload first");
+  if (nsteps < 0) return fail(SPOC_EINVAL, "nsteps=%d", nsteps);
+  (void)hipSetDevice(c->device);
+  c->prof.used = 0;
+  c->prof.marks.clear();
+  HIPCHK(hipEventRecord(c->t0, c->stream));
+  const double *u0[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
+  const double *xs[3] = {state_ptr(c, 3), state_ptr(c, 4), state_ptr(c, 5)};
+  for (int it = 0; it < nsteps; it++)
+    for (int s = 0; s < 4; s++) {
+      RhsArgs a = make_args(c, variant, s == 0 ? u0 : xs, s, dt);
+      rc = run_rhs(c, precision, a);
+      if (rc) return rc;
+    }
+  HIPCHK(hipEventRecord(c->t1, c->stream));
+  HIPCHK(hipEventSynchronize(c->t1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, c->t0, c->t1));
+  c->last_ms = ms;
+  // per-kernel totals from the hook events: slot k -> k+1 brackets kernel k
+  c->knames.clear();
+  c->kms.clear();
+  c->kcount.clear();
+  if (c->prof.on) {
+    std::vector<double> tot(8, 0.0);
+    std::vector<int> cnt(8, 0);
+    for (size_t i = 0; i + 1 < c->prof.marks.size(); i++) {
+      int s0 = c->prof.marks[i].first, s1 = c->prof.marks[i + 1].first;
+      if (s1 != s0 + 1) continue;
+      float m = 0.f;
+      if (hipEventElapsedTime(&m, c->prof.ev[c->prof.marks[i].second],
+                              c->prof.ev[c->prof.marks[i + 1].second]) == hipSuccess) {
+        tot[s0] += m;
+        cnt[s0]++;
+      }
+    }
+    const char *lin[] = {"face_lin", "elem_lin"};
+    const char *nl[] = {"face_uv", "elem_bc", "face_nl2", "elem_eg"};
+    int nk = variant == SPOC_LIN ? 2 : 4;
+    if (precision == SPOC_FP32_MFMA) {
+      const char *l32[] = {"f32_lin", ""};
+      const char *n32[] = {"f32_bc", "f32_eg", "", ""};
+      nk = variant == SPOC_LIN ? 1 : 2;
+      for (int k = 0; k < nk; k++) {
+        c->knames.push_back(variant == SPOC_LIN ? l32[k] : n32[k]);
+        c->kms.push_back(tot[k]);
+        c->kcount.push_back(cnt[k]);
+      }
+    } else {
+      for (int k = 0; k < nk; k++) {
+        c->knames.push_back(variant == SPOC_LIN ? lin[k] : nl[k]);
+        c->kms.push_back(tot[k]);
+        c->kcount.push_back(cnt[k]);
+      }
+    }
+  }
+  return SPOC_OK;
+}
+
+int spoc_sync(spoc_ctx *c) {
+  if (!c) return fail(SPOC_EINVAL, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SPOC_OK;
+}
+
+int spoc_energy(spoc_ctx *c, double *ke, double *pe, double *mass) {
+  if (!c || !c->have_state || !c->have_fields)
+    return fail(SPOC_ESTATE, "spoc_energy: need mesh, fields and state");
+  if (!c->have_nodes) return fail(SPOC_ESTATE, "spoc_energy: mesh was set without node arrays");
+  (void)hipSetDevice(c->device);
+  size_t n = (size_t)c->nm * c->ne;
+  double *out = c->den + (size_t)c->ne * (2 + c->nm);
+  int rc = energy_launch(c->stream, c->ne, c->nm, c->ng, state_ptr(c, 0), state_ptr(c, 1),
+                         state_ptr(c, 2), c->dfields + 3 * n, c->tab.pi2, c->tab.wi,
+                         c->tab.ammo1, c->mesh.deta, c->phys.g0, c->den, out);
+  if (rc) return fail(SPOC_EHIP, "energy kernel launch failed");
+  double r[3];
+  HIPCHK(hipMemcpyAsync(r, out, sizeof r, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *ke = r[0];
+  *pe = r[1];
+  *mass = r[2];
+  return SPOC_OK;
+}
+
+int spoc_set_profiling(spoc_ctx *c, int on) {
+  if (!c) return fail(SPOC_EINVAL, "null context");
+  c->prof.on = on != 0;
+  return SPOC_OK;
+}
+
+int spoc_last_timing(spoc_ctx *c, double *total_ms, int *nkernels, char *names, int len,
+                     double *ms, int *launches) {
+  if (!c) return fail(SPOC_EINVAL, "null context");
+  if (total_ms) *total_ms = c->last_ms;
+  if (!nkernels) return SPOC_OK;
+  int cap = *nkernels, n = (int)c->knames.size();
+  *nkernels = n;
+  int pos = 0;
+  for (int i = 0; i < n && i < cap; i++) {
+    if (ms) ms[i] = c->kms[i];
+    if (launches) launches[i] = c->kcount[i];
+    if (names) {
+      int l = (int)c->knames[i].size();
+      if (pos + l + 1 > len) break;
+      std::memcpy(names + pos, c->knames[i].c_str(), l + 1);
+      pos += l + 1;
+    }
+  }
+  return SPOC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,69 @@
+// Global integrals of the state (src/energy.f:3-96), FP64.
+//
+// Per-element terms are computed in parallel with the reference's
+// expression order; the three global sums then run in the reference's
+// element (and mode) order on one thread each, so that the result equals
+// the reference's sequential accumulation bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+
+namespace spoc {
+
+__global__ void energy_terms(int ne, int nm, int ng, const double *__restrict__ u,
+                             const double *__restrict__ v, const double *__restrict__ h,
+                             const double *__restrict__ hb, const double *__restrict__ pi2,
+                             const double *__restrict__ wi, const double *__restrict__ ammo1,
+                             const double *__restrict__ deta, double g0, double *__restrict__ ke,
+                             double *__restrict__ pe, double *__restrict__ mass) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const double *ue = u + (size_t)e * nm, *ve = v + (size_t)e * nm, *he = h + (size_t)e * nm;
+  const double *hbe = hb + (size_t)e * nm;
+  double d = deta[e];
+  for (int j = 0; j < nm; j++) mass[(size_t)e * nm + j] = ammo1[j] * d * he[j];
+  double ak = 0.0, ap = 0.0;
+  for (int q = 0; q < ng; q++) {
+    double mu = 0.0, mv = 0.0, mh = 0.0, mhb = 0.0;
+    for (int m = 0; m < nm; m++) {
+      double p = pi2[q + ng * m];
+      mu = mu + ue[m] * p;
+      mv = mv + ve[m] * p;
+      mh = mh + he[m] * p;
+      mhb = mhb + (he[m] + hbe[m]) * p;
+    }
+    double mk = mhb * (mu * mu + mv * mv);
+    ak = ak + wi[q] * mk * pi2[q];
+    double mp = mh * mh;
+    ap = ap + wi[q] * mp * pi2[q];
+  }
+  ke[e] = ak * d * 0.5;
+  pe[e] = ap * d * 0.5 * g0;
+}
+
+__global__ void energy_sum(int ne, int nm, const double *__restrict__ ke,
+                           const double *__restrict__ pe, const double *__restrict__ mass,
+                           double *__restrict__ out) {
+  int t = threadIdx.x;
+  double s = 0.0;
+  if (t == 0)
+    for (int e = 0; e < ne; e++) s = s + ke[e];
+  else if (t == 1)
+    for (int e = 0; e < ne; e++) s = s + pe[e];
+  else if (t == 2)
+    for (size_t i = 0; i < (size_t)ne * nm; i++) s = s + mass[i];
+  if (t < 3) out[t] = s;
+}
+
+int energy_launch(hipStream_t st, int ne, int nm, int ng, const double *u, const double *v,
+                  const double *h, const double *hb, const double *pi2, const double *wi,
+                  const double *ammo1, const double *deta, double g0, double *scratch,
+                  double *out) {
+  double *ke = scratch, *pe = scratch + ne, *mass = scratch + 2 * (size_t)ne;
+  energy_terms<<<(ne + 127) / 128, 128, 0, st>>>(ne, nm, ng, u, v, h, hb, pi2, wi, ammo1, deta,
+                                                 g0, ke, pe, mass);
+  energy_sum<<<1, 64, 0, st>>>(ne, nm, ke, pe, mass, out);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+}  // namespace spoc

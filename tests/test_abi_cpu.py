@@ -1,0 +1,52 @@
+"""C-ABI library checks that need no GPU: exports, host table builder."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import spoc_amd
+from conftest import ROOT, gold, oracle_for, quad_rule
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "spoc_abi.h")).read()
+    names = sorted(set(re.findall(r"^\w[\w\s\*]*?\b(spoc_\w+)\(", hdr, re.M)))
+    assert len(names) >= 15
+    lib = spoc_amd.load_library()
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+@pytest.mark.parametrize("nc,ng,ngf", [(3, 12, 4), (4, 25, 5), (5, 25, 6), (6, 33, 7)])
+def test_host_tables_match_reference_bit_for_bit(nc, ng, ngf):
+    g = gold(f"tables_p{nc}_{ng}.npz")
+    t = spoc_amd.build_tables(nc, *quad_rule(ng, ngf))
+    nm, nl = t["nm"], t["nm"] - nc - 1
+    for k in ("amm", "agx", "agy", "ammo"):
+        assert np.array_equal(t[k][:nm, :nm], g[k][:nm, :nm]), k
+    assert np.array_equal(t["amml"][:nl, :nl], g["amml"][:nl, :nl])
+    assert np.array_equal(t["pi2"][:ng, :nm], g["pi2"][:ng, :nm])
+    assert np.array_equal(t["pico"][:ngf, :nm], g["pico"][:ngf, :nm])
+
+
+def test_host_tables_42_point_rule_match_oracle():
+    t = spoc_amd.build_tables(7, *quad_rule(42, 8))
+    o = oracle_for(7, 42, 8).tables()
+    nm = 36
+    for k in ("amm", "agx", "agy", "ammo"):
+        assert np.array_equal(t[k][:nm, :nm], o[k][:nm, :nm]), k
+
+
+def test_singular_rule_is_an_error_not_nan():
+    # p=4 has 15 modes; the 12-point rule cannot make the mass matrix SPD
+    # (the reference prints "pb avec matrice" and runs on to NaN).
+    with pytest.raises(spoc_amd.SpocError, match="not positive definite"):
+        spoc_amd.build_tables(4, *quad_rule(12, 5))
+
+
+def test_unsupported_shape_rejected_before_device_use():
+    t = spoc_amd.build_tables(3, *quad_rule(25, 4))
+    with pytest.raises(spoc_amd.SpocError, match="unsupported"):
+        spoc_amd.Context(t)

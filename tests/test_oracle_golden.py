@@ -1,0 +1,101 @@
+"""The oracle (C restatement) against golden vectors written by the
+reference Fortran itself (tests/golden/make_goldens.py): bit-exact."""
+import numpy as np
+import pytest
+
+from conftest import gold, oracle_for, quad_rule
+from spoc_amd import configs, mesh
+
+SHAPES = [(3, 12, 4), (4, 25, 5), (5, 25, 6), (6, 33, 7)]
+
+
+@pytest.mark.parametrize("nc,ng,ngf", SHAPES)
+def test_tables_bit_exact(nc, ng, ngf):
+    g = gold(f"tables_p{nc}_{ng}.npz")
+    O = oracle_for(nc, ng, ngf)
+    t = O.tables()
+    nm = O.nm
+    assert list(g["info"]) == [t["info1"], t["info2"]] == [0, 0]
+    for k in ("amm", "agx", "agy", "ammo"):
+        assert np.array_equal(t[k][:nm, :nm], g[k][:nm, :nm]), k
+    nl = nm - nc - 1  # cal_matl writes only the reduced block (src/mat1.f:285-368)
+    assert np.array_equal(t["amml"][:nl, :nl], g["amml"][:nl, :nl])
+    assert np.array_equal(t["pi2"][:ng, :nm], g["pi2"][:ng, :nm])
+    assert np.array_equal(t["pico"][:ngf, :nm], g["pico"][:ngf, :nm])
+    # calip writes cip(1, 0:nc) and cip(2, 0:nc-1) (src/mat1.f:383-410)
+    assert np.array_equal(t["cip"][0, :nc + 1], g["cip"][0, :nc + 1])
+    assert np.array_equal(t["cip"][1, :nc], g["cip"][1, :nc])
+
+
+def _case(nc, ng, ngf):
+    g = gold(f"calvar_p{nc}.npz")
+    O = oracle_for(nc, ng, ngf)
+    tabs = O.tables()
+    tabs.update(nc=nc, ng=ng, ngf=ngf, nm=O.nm)
+    w, x, y, wf, tf = quad_rule(ng, ngf)
+    tabs.update(wi=w, xi=x, yi=y, wif=wf, tif=tf)
+    c = configs.random_case(tabs, int(g["nx"]), int(g["ny"]), seed=int(g["seed"]))
+    nm = O.nm
+    for k, a in zip(("u0", "v0", "h0"), c["state"]):
+        assert np.array_equal(a[:nm], g[k]), k  # the generator reproduces the inputs
+    for k in ("hb", "w0", "z0", "fc"):
+        src = c["phys"][k] if k != "fc" else c["mesh"]["fc"]
+        assert np.array_equal(src[:nm], g[k]), k
+    return O, c, g
+
+
+@pytest.mark.parametrize("nc,ng,ngf", [(3, 12, 4), (6, 33, 7)])
+@pytest.mark.parametrize("variant", ["lin", "fs", "ns", "inv"])
+def test_cal_var_bit_exact(nc, ng, ngf, variant):
+    O, c, g = _case(nc, ng, ngf)
+    p = dict(c["phys"])
+    p["fc"] = c["mesh"]["fc"]
+    out = O.cal_var(variant, c["mesh"], p, *c["state"])
+    for k, a in zip("uvh", out):
+        assert np.array_equal(a[:O.nm], g[f"{variant}_{k}1"]), k
+
+
+@pytest.mark.parametrize("variant", ["lin", "fs", "ns", "inv"])
+def test_rk4_and_energy_bit_exact(variant):
+    g = gold("rk4_p3.npz")
+    O = oracle_for(3, 12, 4)
+    tabs = O.tables()
+    w, x, y, wf, tf = quad_rule(12, 4)
+    tabs.update(nc=3, ng=12, ngf=4, nm=10, wi=w, xi=x, yi=y, wif=wf, tif=tf)
+    c = configs.config1(tabs)
+    assert c["dt"] == float(g["dt"]) == 20.0
+    for k, a in zip(("u0", "v0", "h0"), c["state"]):
+        assert np.array_equal(a[:10], g[k])
+    p = dict(c["phys"])
+    p["mu0"] = float(g["mu0"])
+    e0 = O.energy(c["mesh"], p, *c["state"])
+    out = O.rk4(variant, c["mesh"], p, *c["state"], c["dt"], 10)
+    e1 = O.energy(c["mesh"], p, *out)
+    for k, a in zip("uvh", out):
+        assert np.array_equal(a[:10], g[f"{variant}_{k}"]), k
+    assert np.array_equal(np.array([e0, e1]), g[f"{variant}_energy"])
+
+
+def test_config1_mesh_matches_reference_cm_and_pointers():
+    g = gold("mesh_c1.npz")
+    x, y, tri = mesh.rmal(17, 17, -500e3, 500e3, -500e3, 500e3)
+    assert np.array_equal(tri, g["tri_in"])
+    tri2 = mesh.changemaille(len(x), tri)
+    assert np.array_equal(tri2, g["tri"])
+    m = mesh.build(x, y, tri2, 3, 1e-4, 2e-11)
+    assert m["nf"] == int(g["nf"]) == 800 and m["ne"] == 512
+    for k in ("itm", "its", "tseg", "pseg", "ssit", "norm", "vect", "normt", "bdl", "fc"):
+        assert np.array_equal(m[k], g[k]), k
+    assert int(m["bdl"].sum()) == 64
+
+
+def test_jittered_mesh_matches_reference_cm_and_pointers():
+    g = gold("mesh_jit.npz")
+    x, y = g["x"], g["y"]
+    tri2 = mesh.changemaille(len(x), g["tri_in"])
+    assert np.array_equal(tri2, g["tri"])
+    xs, ys, trs = configs.jittered_mesh(11, 9, 1000e3, 800e3, 0.3, 5)
+    assert np.array_equal(xs, x) and np.array_equal(ys, y) and np.array_equal(trs, g["tri"])
+    m = mesh.build(x, y, tri2, 3, 1e-4, 2e-11)
+    for k in ("itm", "its", "tseg", "pseg", "ssit", "norm", "vect", "normt", "bdl", "fc"):
+        assert np.array_equal(m[k], g[k]), k
