@@ -23,7 +23,22 @@ int energy_launch(hipStream_t st, int ne, int nm, int ng, const double *u, const
                   const double *h, const double *hb, const double *pi2, const double *wi,
                   const double *ammo1, const double *deta, double g0, double *scratch,
                   double *out);
-int rhs_f32(int nc, int ng, int ngf, const RhsArgs &a, void *plan);
+struct F32Plan;
+int f32_plan_create(F32Plan **out, int nc, int ng, int ngf, const double *amm, const double *amml,
+                    const double *agx, const double *agy, const double *pi2, const double *pico,
+                    const double *wi, const double *wif, int ld_g, int ld_f, int ld_m,
+                    const double *dU);
+void f32_plan_destroy(F32Plan *p);
+int f32_mesh_set(F32Plan *p, int ne, int nf, const int *itm, const int *its, const int *tseg,
+                 const int *ssit, const double *norm, const double *vect, const double *normt,
+                 const int *bdl);
+int f32_fields_set(F32Plan *p, int ld, const double *fc, const double *w0, const double *z0,
+                   const double *hb, const double *pi2, int ld_g, double mu0, double mb0,
+                   double g0);
+int f32_rhs(F32Plan *p, hipStream_t s, int variant, const double *const *x, double *const *k,
+            TimerHook hook, void *user);
+int f32_rk4(F32Plan *p, hipStream_t s, int variant, double *const *u0, double dt, int nsteps,
+            TimerHook hook, void *user);
 }  // namespace spoc
 
 namespace {
@@ -146,7 +161,8 @@ struct spoc_ctx {
   std::vector<std::string> knames;
   std::vector<double> kms;
   std::vector<int> kcount;
-  void *f32plan = nullptr;
+  F32Plan *f32 = nullptr;
+  std::vector<double> h_pi2;  // pi2(ng, nm) for projecting fields onto quadrature points
 };
 
 namespace {
@@ -180,6 +196,16 @@ void prof_hook(void *user, int slot, hipStream_t st) {
   size_t i = c->prof.used++;
   (void)hipEventRecord(c->prof.ev[i], st);
   c->prof.marks.push_back({slot, i});
+}
+
+int f32_fail(int rc) {
+  if (rc == -1)
+    return fail(SPOC_EINVAL,
+                "FP32 path: a wall face is not local edge 1 of its element (prepare the mesh "
+                "with the CM corner fix, src/chmal.f) or the shape is unsupported");
+  if (rc == -3) return fail(SPOC_ESTATE, "FP32 path: mesh and fields must be set first");
+  hipError_t e = hipGetLastError();
+  return fail(SPOC_EHIP, "FP32 kernel launch failed: %s", hipGetErrorString(e));
 }
 
 }  // namespace
@@ -351,6 +377,14 @@ int spoc_ctx_create(spoc_ctx **out, int device, int nc, int ng, int ngf, const d
                      dptr[7], dptr[8], dptr[9], dptr[10], dptr[11], dptr[12]};
   (void)hipEventCreate(&c->t0);
   (void)hipEventCreate(&c->t1);
+  c->h_pi2.assign((size_t)ng * nm, 0.0);
+  for (int m = 0; m < nm; m++)
+    for (int q = 0; q < ng; q++) c->h_pi2[q + (size_t)ng * m] = pi2[q + (size_t)ld_g * m];
+  if (f32_plan_create(&c->f32, nc, ng, ngf, amm, amml, agx, agy, pi2, pico, wi, wif, ld_g, ld_f,
+                      ld_m, c->tab.U) != 0) {
+    spoc_ctx_destroy(c);
+    return fail(SPOC_EHIP, "FP32 operator upload failed");
+  }
   *out = c;
   return SPOC_OK;
 }
@@ -360,6 +394,7 @@ int spoc_ctx_destroy(spoc_ctx *c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   free_mesh(c);
+  f32_plan_destroy(c->f32);
   if (c->dtab) (void)hipFree(c->dtab);
   for (auto e : c->prof.ev) (void)hipEventDestroy(e);
   if (c->t0) (void)hipEventDestroy(c->t0);
@@ -445,6 +480,8 @@ int spoc_mesh_set(spoc_ctx *c, int ne, int nf, const int *itm, const int *its, c
   HIPCHK(hipMemcpy(c->dFptr, fp, sizeof fp, hipMemcpyHostToDevice));
   HIPCHK(hipMalloc((void **)&c->dfields, sizeof(double) * (size_t)4 * nm * ne));
   HIPCHK(hipMemset(c->dfields, 0, sizeof(double) * (size_t)4 * nm * ne));
+  if (f32_mesh_set(c->f32, ne, nf, itm, its, tseg, ssit, norm, vect, normt, bdl))
+    return fail(SPOC_EHIP, "FP32 mesh upload failed");
   return SPOC_OK;
 }
 
@@ -488,6 +525,8 @@ int spoc_fields_set(spoc_ctx *c, int ld, const double *fc, const double *w0, con
   int rc = upload_fields(c, ld, src, dst, 4);
   if (rc) return rc;
   c->phys = Physics{mu0, mb0, g0, hb[0]};  // hh0 = hb(1,1) (src/calvar_lin.f:65)
+  if (f32_fields_set(c->f32, ld, fc, w0, z0, hb, c->h_pi2.data(), c->ng, mu0, mb0, g0))
+    return fail(SPOC_EHIP, "FP32 field upload failed");
   c->have_fields = true;
   return SPOC_OK;
 }
@@ -550,8 +589,6 @@ static int run_rhs(spoc_ctx *c, int precision, const RhsArgs &a) {
   if (c->nc == NC_ && c->ng == NG_ && c->ngf == NGF_) rc = rhs_f64_##NC_##_##NG_##_##NGF_(a);
     SPOC_FOR_EACH_SHAPE(SPOC_CALL)
 #undef SPOC_CALL
-  } else if (precision == SPOC_FP32_MFMA) {
-    rc = rhs_f32(c->nc, c->ng, c->ngf, a, c->f32plan);
   } else {
     return fail(SPOC_EINVAL, "unknown precision %d", precision);
   }
@@ -583,11 +620,16 @@ int spoc_rhs(spoc_ctx *c, int variant, int precision, int ld, const double *u0,
   rc = upload_fields(c, ld, src, xs, 3);
   if (rc) return rc;
   const double *x[3] = {xs[0], xs[1], xs[2]};
-  RhsArgs a = make_args(c, variant, x, -1, 0.0);
-  rc = run_rhs(c, precision, a);
-  if (rc) return rc;
-  double *dst[3] = {u1, v1, h1};
   double *ks[3] = {state_ptr(c, 9), state_ptr(c, 10), state_ptr(c, 11)};
+  if (precision == SPOC_FP32_MFMA) {
+    rc = f32_rhs(c->f32, c->stream, variant, x, ks, nullptr, nullptr);
+    if (rc) return f32_fail(rc);
+  } else {
+    RhsArgs a = make_args(c, variant, x, -1, 0.0);
+    rc = run_rhs(c, precision, a);
+    if (rc) return rc;
+  }
+  double *dst[3] = {u1, v1, h1};
   return download_fields(c, ld, dst, ks, 3);
 }
 
@@ -602,12 +644,18 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
   HIPCHK(hipEventRecord(c->t0, c->stream));
   const double *u0[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
   const double *xs[3] = {state_ptr(c, 3), state_ptr(c, 4), state_ptr(c, 5)};
-  for (int it = 0; it < nsteps; it++)
-    for (int s = 0; s < 4; s++) {
-      RhsArgs a = make_args(c, variant, s == 0 ? u0 : xs, s, dt);
-      rc = run_rhs(c, precision, a);
-      if (rc) return rc;
-    }
+  if (precision == SPOC_FP32_MFMA) {
+    double *uw[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
+    rc = f32_rk4(c->f32, c->stream, variant, uw, dt, nsteps, prof_hook, c);
+    if (rc) return f32_fail(rc);
+  } else {
+    for (int it = 0; it < nsteps; it++)
+      for (int s = 0; s < 4; s++) {
+        RhsArgs a = make_args(c, variant, s == 0 ? u0 : xs, s, dt);
+        rc = run_rhs(c, precision, a);
+        if (rc) return rc;
+      }
+  }
   HIPCHK(hipEventRecord(c->t1, c->stream));
   HIPCHK(hipEventSynchronize(c->t1));
   float ms = 0.f;

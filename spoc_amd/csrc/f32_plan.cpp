@@ -1,0 +1,397 @@
+// Host side of the FP32-MFMA path: operators of the mass-orthonormal basis
+// (formed in FP64 from the reference tables, rounded once to FP32),
+// per-element mesh data, device buffers and the launch sequence.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+#include "f32_types.h"
+#include "launch.h"
+
+namespace spoc {
+
+#define SPOC_DECL_F32(NC_, NG_, NGF_) \
+  int rhs_f32_##NC_##_##NG_##_##NGF_(const F32Args &a, int phase, const double *in, double *out);
+SPOC_FOR_EACH_SHAPE(SPOC_DECL_F32)
+#undef SPOC_DECL_F32
+
+namespace {
+int up(int x, int m) { return (x + m - 1) / m * m; }
+
+// Inverse of an upper-triangular matrix (column-major, ld), FP64.
+std::vector<double> upper_inverse(int n, const double *U, int ld) {
+  std::vector<double> X((size_t)n * n, 0.0);
+  for (int c = 0; c < n; c++) {
+    // solve U x = e_c by back substitution
+    for (int i = n - 1; i >= 0; i--) {
+      double s = (i == c) ? 1.0 : 0.0;
+      for (int k = i + 1; k < n; k++) s -= U[i + (size_t)ld * k] * X[k + (size_t)n * c];
+      X[i + (size_t)n * c] = s / U[i + (size_t)ld * i];
+    }
+  }
+  return X;
+}
+}  // namespace
+
+struct F32Plan {
+  int nc, ng, ngf, nm, nl;
+  int NMr, NMk, NGr, NEr, NEk;
+  float *dops = nullptr;
+  F32Ops ops{};
+  // mesh
+  int ne = 0, nepad = 0;
+  bool walls_on_edge1 = true;
+  void *dmesh = nullptr;
+  F32Mesh mesh{};
+  // fields
+  float *dfld = nullptr;
+  F32Fields fld{};
+  bool have_fields = false;
+  // state/work
+  void *dstate = nullptr;
+  F32State st{};
+  // host copy of the basis change for testing
+  std::vector<double> Uinv;
+};
+
+static int call(const F32Plan *p, const F32Args &a, int phase, const double *in, double *out) {
+#define SPOC_CALL(NC_, NG_, NGF_) \
+  if (p->nc == NC_ && p->ng == NG_ && p->ngf == NGF_) return rhs_f32_##NC_##_##NG_##_##NGF_(a, phase, in, out);
+  SPOC_FOR_EACH_SHAPE(SPOC_CALL)
+#undef SPOC_CALL
+  return -1;
+}
+
+int f32_plan_create(F32Plan **out, int nc, int ng, int ngf, const double *amm, const double *amml,
+                    const double *agx, const double *agy, const double *pi2, const double *pico,
+                    const double *wi, const double *wif, int ld_g, int ld_f, int ld_m,
+                    const double *dU) {
+  F32Plan *p = new F32Plan();
+  p->nc = nc;
+  p->ng = ng;
+  p->ngf = ngf;
+  int nm = nm_of(nc), nl = nm - nc - 1, ne3 = 3 * ngf;
+  p->nm = nm;
+  p->nl = nl;
+  p->NMr = up(nm, 16);
+  p->NMk = up(nm, 4);
+  p->NGr = up(ng, 16);
+  p->NEr = up(ne3, 16);
+  p->NEk = up(ne3, 4);
+  // FP64 building blocks
+  std::vector<double> Ui = upper_inverse(nm, amm, ld_m);  // U^{-1}
+  p->Uinv = Ui;
+  auto UI = [&](int a, int b) { return Ui[a + (size_t)nm * b]; };
+  std::vector<double> ULi = upper_inverse(nl, amml, ld_m);
+  auto AGX = [&](int a, int b) { return agx[a + (size_t)ld_m * b]; };
+  auto AGY = [&](int a, int b) { return agy[a + (size_t)ld_m * b]; };
+  auto PI2 = [&](int q, int m) { return pi2[q + (size_t)ld_g * m]; };
+  auto PICO = [&](int j, int m, int k) { return pico[j + (size_t)ld_f * (m + (size_t)ld_m * k)]; };
+  // Dx = U^-T agx^T U^-1: Dx(i,j) = sum_{a,b} UI(a,i) agx(b,a) UI(b,j)
+  std::vector<double> T1((size_t)nm * nm), DX((size_t)nm * nm), DY((size_t)nm * nm);
+  for (int pass = 0; pass < 2; pass++) {
+    // T1(a, j) = sum_b agx(b, a) UI(b, j)
+    for (int a = 0; a < nm; a++)
+      for (int j = 0; j < nm; j++) {
+        double s = 0.0;
+        for (int b = 0; b < nm; b++) s += (pass ? AGY(b, a) : AGX(b, a)) * UI(b, j);
+        T1[a + (size_t)nm * j] = s;
+      }
+    std::vector<double> &D = pass ? DY : DX;
+    for (int i = 0; i < nm; i++)
+      for (int j = 0; j < nm; j++) {
+        double s = 0.0;
+        for (int a = 0; a < nm; a++) s += UI(a, i) * T1[a + (size_t)nm * j];
+        D[i + (size_t)nm * j] = s;
+      }
+  }
+  // PSI(q, m) = sum_a pi2(q, a) UI(a, m)
+  std::vector<double> PSI((size_t)ng * nm);
+  for (int q = 0; q < ng; q++)
+    for (int m = 0; m < nm; m++) {
+      double s = 0.0;
+      for (int a = 0; a < nm; a++) s += PI2(q, a) * UI(a, m);
+      PSI[q + (size_t)ng * m] = s;
+    }
+  // TR((k,j), m) = sum_a pico(j, a, k) UI(a, m)
+  // LF(i, (k,j)) = sum_a UI(a, i) wif(j) pico(j, a, k)
+  std::vector<double> TR((size_t)ne3 * nm), LF((size_t)nm * ne3);
+  for (int k = 0; k < 3; k++)
+    for (int j = 0; j < ngf; j++)
+      for (int m = 0; m < nm; m++) {
+        double s = 0.0, t = 0.0;
+        for (int a = 0; a < nm; a++) {
+          s += PICO(j, a, k) * UI(a, m);
+          t += UI(a, m) * wif[j] * PICO(j, a, k);
+        }
+        TR[(k * ngf + j) + (size_t)ne3 * m] = s;
+        LF[m + (size_t)nm * (k * ngf + j)] = t;
+      }
+  // wall projector: G = U R^T Ml^{-1} R U^T with R the reduction of
+  // src/mat1.f:418-439 and Ml = R amm R^T (the reference's amml)
+  std::vector<double> R((size_t)nl * nm, 0.0);
+  for (int j = 0; j <= nc - 1; j++) {
+    int c1 = ip_of(nc, j), c2 = ip_of(nc - 1, j);
+    for (int l = 0; l <= nc - 1 - j; l++) {
+      R[(c2 + l) + (size_t)nl * (c1 + l + 1)] = 1.0;
+      R[(c2 + l) + (size_t)nl * c1] = (l % 2 == 0) ? 1.0 : -1.0;
+    }
+  }
+  // Z = U R^T (nm x nl); Ml^{-1} = ULi ULi^T
+  std::vector<double> Z((size_t)nm * nl, 0.0), MLi((size_t)nl * nl, 0.0), GL((size_t)nm * nm);
+  for (int i = 0; i < nm; i++)
+    for (int c = 0; c < nl; c++) {
+      double s = 0.0;
+      for (int k = i; k < nm; k++) s += amm[i + (size_t)ld_m * k] * R[c + (size_t)nl * k];
+      Z[i + (size_t)nm * c] = s;
+    }
+  for (int a = 0; a < nl; a++)
+    for (int b = 0; b < nl; b++) {
+      double s = 0.0;
+      for (int k = 0; k < nl; k++) s += ULi[a + (size_t)nl * k] * ULi[b + (size_t)nl * k];
+      MLi[a + (size_t)nl * b] = s;
+    }
+  for (int i = 0; i < nm; i++)
+    for (int j = 0; j < nm; j++) {
+      double s = 0.0;
+      for (int a = 0; a < nl; a++) {
+        double t = 0.0;
+        for (int b = 0; b < nl; b++) t += MLi[a + (size_t)nl * b] * Z[j + (size_t)nm * b];
+        s += Z[i + (size_t)nm * a] * t;
+      }
+      GL[i * nm + j] = s - (i == j ? 1.0 : 0.0);  // row-major
+    }
+  // FP32 padded device images
+  int NMr = p->NMr, NMk = p->NMk, NGr = p->NGr, NEr = p->NEr, NEk = p->NEk;
+  size_t oDX = 0, oDY = oDX + (size_t)NMr * NMk, oLF = oDY + (size_t)NMr * NMk,
+         oPSI = oLF + (size_t)NMr * NEk, oPSIW = oPSI + (size_t)NGr * NMk,
+         oTR = oPSIW + (size_t)NMr * NGr, oGL = oTR + (size_t)NEr * NMk,
+         oXQ = oGL + (size_t)nm * nm, oYQ = oXQ + up(ng, 4), tot = oYQ + up(ng, 4);
+  std::vector<float> h(tot, 0.0f);
+  for (int i = 0; i < nm; i++)
+    for (int j = 0; j < nm; j++) {
+      h[oDX + (size_t)i * NMk + j] = (float)DX[i + (size_t)nm * j];
+      h[oDY + (size_t)i * NMk + j] = (float)DY[i + (size_t)nm * j];
+    }
+  for (int i = 0; i < nm; i++)
+    for (int c = 0; c < ne3; c++) h[oLF + (size_t)i * NEk + c] = (float)LF[i + (size_t)nm * c];
+  for (int q = 0; q < ng; q++)
+    for (int m = 0; m < nm; m++) {
+      h[oPSI + (size_t)q * NMk + m] = (float)PSI[q + (size_t)ng * m];
+      h[oPSIW + (size_t)m * NGr + q] = (float)(wi[q] * PSI[q + (size_t)ng * m]);
+    }
+  for (int r = 0; r < ne3; r++)
+    for (int m = 0; m < nm; m++) h[oTR + (size_t)r * NMk + m] = (float)TR[r + (size_t)ne3 * m];
+  for (int i = 0; i < nm * nm; i++) h[oGL + i] = (float)GL[i];
+  for (int q = 0; q < ng; q++) {
+    h[oXQ + q] = (float)PI2(q, nc + 1);  // P_1(xi) = xi
+    h[oYQ + q] = (float)PI2(q, 1);       // P_1(eta) = eta
+  }
+  if (hipMalloc(&p->dops, tot * sizeof(float)) != hipSuccess ||
+      hipMemcpy(p->dops, h.data(), tot * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+    delete p;
+    return -2;
+  }
+  float *d = p->dops;
+  p->ops = F32Ops{d + oDX, d + oDY, d + oLF, d + oPSI, d + oPSIW, d + oTR, d + oGL,
+                  d + oXQ, d + oYQ, dU};
+  *out = p;
+  return 0;
+}
+
+static void free_mesh(F32Plan *p) {
+  if (p->dmesh) (void)hipFree(p->dmesh);
+  if (p->dfld) (void)hipFree(p->dfld);
+  if (p->dstate) (void)hipFree(p->dstate);
+  p->dmesh = p->dfld = nullptr;
+  p->dstate = nullptr;
+  p->have_fields = false;
+}
+
+void f32_plan_destroy(F32Plan *p) {
+  if (!p) return;
+  free_mesh(p);
+  if (p->dops) (void)hipFree(p->dops);
+  delete p;
+}
+
+int f32_mesh_set(F32Plan *p, int ne, int nf, const int *itm, const int *its, const int *tseg,
+                 const int *ssit, const double *norm, const double *vect, const double *normt,
+                 const int *bdl) {
+  (void)nf;
+  free_mesh(p);
+  int nm = p->nm, ng = p->ng, ngf = p->ngf;
+  int nepad = up(ne, 16);
+  p->ne = ne;
+  p->nepad = nepad;
+  std::vector<float> geo((size_t)ne * GEO, 0.0f);
+  std::vector<int> nbr((size_t)ne * 3), nbe((size_t)ne * 3), b(ne);
+  p->walls_on_edge1 = true;
+  for (int e = 0; e < ne; e++) {
+    float *g = geo.data() + (size_t)e * GEO;
+    for (int i = 0; i < 5; i++) g[i] = (float)vect[5 * e + i];
+    for (int k = 0; k < 3; k++) {
+      int f = itm[3 * e + k] - 1;
+      g[5 + k] = (float)norm[2 * f];
+      g[8 + k] = (float)norm[2 * f + 1];
+      g[11 + k] = (float)its[3 * e + k];
+      int a = tseg[2 * f] - 1, c = tseg[2 * f + 1] - 1;
+      if (c < 0) {
+        nbr[3 * e + k] = -1;
+        nbe[3 * e + k] = 0;
+        if (k != 0) p->walls_on_edge1 = false;
+      } else if (a == e) {
+        nbr[3 * e + k] = c;
+        nbe[3 * e + k] = ssit[2 * f + 1] - 1;
+      } else {
+        nbr[3 * e + k] = a;
+        nbe[3 * e + k] = ssit[2 * f] - 1;
+      }
+    }
+    g[14] = (float)normt[2 * e];
+    g[15] = (float)normt[2 * e + 1];
+    b[e] = bdl[e] != 0;
+  }
+  size_t bytes = geo.size() * 4 + (size_t)ne * 3 * 4 * 2 + (size_t)ne * 4 + 1024;
+  if (hipMalloc(&p->dmesh, bytes) != hipSuccess) return -2;
+  char *m = (char *)p->dmesh;
+  float *dgeo = (float *)m;
+  int *dnbr = (int *)(dgeo + geo.size()), *dnbe = dnbr + (size_t)ne * 3, *dbdl = dnbe + (size_t)ne * 3;
+  if (hipMemcpy(dgeo, geo.data(), geo.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dnbr, nbr.data(), nbr.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dnbe, nbe.data(), nbe.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dbdl, b.data(), b.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+    return -2;
+  p->mesh = F32Mesh{ne, dgeo, dnbr, dnbe, dbdl};
+  // state / work
+  size_t n64 = (size_t)nepad * nm;  // per field
+  size_t b64 = 9 * n64 * 8;         // c0, acc, k
+  size_t b32 = (3 + 8) * n64 * 4 + (size_t)ne * 3 * 2 * ngf * 4 + (size_t)ne * 3 * 3 * ngf * 4;
+  if (hipMalloc(&p->dstate, b64 + b32 + 1024) != hipSuccess) return -2;
+  if (hipMemset(p->dstate, 0, b64 + b32 + 1024) != hipSuccess) return -2;
+  double *s64 = (double *)p->dstate;
+  float *s32 = (float *)(s64 + 9 * n64);
+  F32State &st = p->st;
+  for (int f = 0; f < 3; f++) {
+    st.c0[f] = s64 + f * n64;
+    st.acc[f] = s64 + (3 + f) * n64;
+    st.k[f] = s64 + (6 + f) * n64;
+    st.xs[f] = s32 + f * n64;
+  }
+  for (int f = 0; f < 4; f++) st.F[f] = s32 + (3 + f) * n64;
+  for (int f = 0; f < 2; f++) {
+    st.src[f] = s32 + (7 + f) * n64;
+    st.uh[f] = s32 + (9 + f) * n64;
+  }
+  st.trA = s32 + 11 * n64;
+  st.trS = st.trA + (size_t)ne * 3 * 2 * ngf;
+  // fields buffer: hbq, w0q, z0q (blocked, nepad*ng each), fc3 (ne*3)
+  size_t nq = (size_t)nepad * ng;
+  if (hipMalloc(&p->dfld, (3 * nq + (size_t)ne * 3) * 4 + 256) != hipSuccess) return -2;
+  if (hipMemset(p->dfld, 0, (3 * nq + (size_t)ne * 3) * 4 + 256) != hipSuccess) return -2;
+  return 0;
+}
+
+int f32_fields_set(F32Plan *p, int ld, const double *fc, const double *w0, const double *z0,
+                   const double *hb, const double *pi2, int ld_g, double mu0, double mb0,
+                   double g0) {
+  int ne = p->ne, nepad = p->nepad, nm = p->nm, ng = p->ng, nc = p->nc;
+  size_t nq = (size_t)nepad * ng;
+  std::vector<float> h(3 * nq + (size_t)ne * 3, 0.0f);
+  const double *src[3] = {hb, w0, z0};
+  for (int f = 0; f < 3; f++) {
+    if (!src[f]) continue;
+    for (int e = 0; e < ne; e++)
+      for (int q = 0; q < ng; q++) {
+        double s = 0.0;
+        for (int m = 0; m < nm; m++) s += pi2[q + (size_t)ld_g * m] * src[f][m + (size_t)ld * e];
+        h[f * nq + (size_t)(e & ~15) * ng + (size_t)q * 16 + (e & 15)] = (float)s;
+      }
+  }
+  if (fc)
+    for (int e = 0; e < ne; e++) {
+      h[3 * nq + 3 * (size_t)e] = (float)fc[0 + (size_t)ld * e];
+      h[3 * nq + 3 * (size_t)e + 1] = (float)fc[1 + (size_t)ld * e];
+      h[3 * nq + 3 * (size_t)e + 2] = (float)fc[nc + 1 + (size_t)ld * e];
+    }
+  if (hipMemcpy(p->dfld, h.data(), h.size() * 4, hipMemcpyHostToDevice) != hipSuccess) return -2;
+  p->fld = F32Fields{p->dfld, p->dfld + nq, p->dfld + 2 * nq, p->dfld + 3 * nq, (float)mu0,
+                     (float)mb0, (float)g0, (float)hb[0]};
+  p->have_fields = true;
+  return 0;
+}
+
+static F32Args args(F32Plan *p, hipStream_t s, int variant, int stage, double dt) {
+  F32Args a{};
+  a.variant = variant;
+  a.stage = stage;
+  a.dt = dt;
+  a.stream = s;
+  a.ops = p->ops;
+  a.mesh = p->mesh;
+  a.fld = p->fld;
+  a.st = p->st;
+  return a;
+}
+
+static int rhs_kernels(F32Plan *p, const F32Args &a, TimerHook hook, void *user) {
+  int rc;
+  if (hook) hook(user, 0, a.stream);
+  if (a.variant == 0) {
+    rc = call(p, a, 3, nullptr, nullptr);
+    if (hook) hook(user, 1, a.stream);
+    return rc;
+  }
+  rc = call(p, a, 4, nullptr, nullptr);
+  if (rc) return rc;
+  if (hook) hook(user, 1, a.stream);
+  rc = call(p, a, 5, nullptr, nullptr);
+  if (hook) hook(user, 2, a.stream);
+  return rc;
+}
+
+int f32_ready(const F32Plan *p) {
+  if (!p || !p->dmesh) return -3;
+  if (!p->have_fields) return -3;
+  if (!p->walls_on_edge1) return -1;
+  return 0;
+}
+
+// Tendencies of the reference-layout FP64 state x[3] (device, [e][nm]) into
+// k[3] (device, same layout).
+int f32_rhs(F32Plan *p, hipStream_t s, int variant, const double *const *x, double *const *k,
+            TimerHook hook, void *user) {
+  int rc = f32_ready(p);
+  if (rc) return rc;
+  F32Args a = args(p, s, variant, -1, 0.0);
+  for (int f = 0; f < 3; f++)
+    if ((rc = call(p, a, 0, x[f], p->st.c0[f]))) return rc;
+  if ((rc = call(p, a, 2, nullptr, nullptr))) return rc;
+  if ((rc = rhs_kernels(p, a, hook, user))) return rc;
+  for (int f = 0; f < 3; f++)
+    if ((rc = call(p, a, 1, p->st.k[f], k[f]))) return rc;
+  return 0;
+}
+
+int f32_rk4(F32Plan *p, hipStream_t s, int variant, double *const *u0, double dt, int nsteps,
+            TimerHook hook, void *user) {
+  int rc = f32_ready(p);
+  if (rc) return rc;
+  F32Args a = args(p, s, variant, -1, dt);
+  for (int f = 0; f < 3; f++)
+    if ((rc = call(p, a, 0, u0[f], p->st.c0[f]))) return rc;
+  if ((rc = call(p, a, 2, nullptr, nullptr))) return rc;
+  for (int it = 0; it < nsteps; it++)
+    for (int st = 0; st < 4; st++) {
+      a.stage = st;
+      if ((rc = rhs_kernels(p, a, hook, user))) return rc;
+    }
+  for (int f = 0; f < 3; f++)
+    if ((rc = call(p, a, 1, p->st.c0[f], u0[f]))) return rc;
+  return 0;
+}
+
+}  // namespace spoc
