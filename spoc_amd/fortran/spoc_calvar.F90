@@ -1,0 +1,172 @@
+! Drop-in replacement for the reference's link-time subroutine cal_var
+! (src/calvar_lin.f:7-10, src/calvarnl_fs.f:9-12, src/calvarnl_ns.f:9-12,
+! src/calvarnl_inv.f:6-9): same name, same argument list, same COMMON
+! blocks read (src/shwwnl.f:61-71).  It forwards to the C-ABI of
+! include/spoc_abi.h through ISO_C_BINDING.  Compiled once per variant with
+! -DSPOC_VARIANT=0 (lin), 1 (fs), 2 (ns), 3 (inv); the makefile picks the
+! object exactly as the reference's makefiles pick calvar_*.o.
+!
+! Environment: SPOC_PRECISION=fp32 selects the FP32-MFMA path (default: the
+! plain FP64 path, which reproduces the reference bit for bit);
+! SPOC_DEVICE=n selects the HIP device.
+!
+! The mesh and the per-run fields are uploaded on the first call and again
+! whenever ne, nf or a checksum of the connectivity changes (the adaptive
+! driver remeshes between calls, src/shwwnl_adap.f:420-432).
+
+#ifndef SPOC_VARIANT
+#define SPOC_VARIANT 1
+#endif
+
+module spoc_abi
+  use iso_c_binding
+  implicit none
+  interface
+    integer(c_int) function spoc_ctx_create(ctx, device, nc, ng, ngf, wi, wif, pi2, ld_g, pico, &
+         ld_f, ld_m, amm, amml, agx, agy, ammo) bind(C, name="spoc_ctx_create")
+      import
+      type(c_ptr) :: ctx
+      integer(c_int), value :: device, nc, ng, ngf, ld_g, ld_f, ld_m
+      real(c_double) :: wi(*), wif(*), pi2(*), pico(*), amm(*), amml(*), agx(*), agy(*), ammo(*)
+    end function
+    integer(c_int) function spoc_mesh_set(ctx, ne, nf, itm, its, tseg, ssit, norm, vect, normt, &
+         bdl, nn, in, xgr, ygr) bind(C, name="spoc_mesh_set")
+      import
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ne, nf, nn
+      integer(c_int) :: itm(*), its(*), tseg(*), ssit(*), bdl(*)
+      real(c_double) :: norm(*), vect(*), normt(*)
+      type(c_ptr), value :: in, xgr, ygr
+    end function
+    integer(c_int) function spoc_fields_set(ctx, ld, fc, w0, z0, hb, mu0, mb0, g0) &
+         bind(C, name="spoc_fields_set")
+      import
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ld
+      real(c_double) :: fc(*), w0(*), z0(*), hb(*)
+      real(c_double), value :: mu0, mb0, g0
+    end function
+    integer(c_int) function spoc_rhs(ctx, variant, precision, ld, u0, v0, h0, u1, v1, h1) &
+         bind(C, name="spoc_rhs")
+      import
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: variant, precision, ld
+      real(c_double) :: u0(*), v0(*), h0(*), u1(*), v1(*), h1(*)
+    end function
+    integer(c_int) function spoc_state_upload(ctx, ld, u0, v0, h0) bind(C, name="spoc_state_upload")
+      import
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ld
+      real(c_double) :: u0(*), v0(*), h0(*)
+    end function
+    integer(c_int) function spoc_state_download(ctx, ld, u0, v0, h0) &
+         bind(C, name="spoc_state_download")
+      import
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ld
+      real(c_double) :: u0(*), v0(*), h0(*)
+    end function
+    integer(c_int) function spoc_rk4_steps(ctx, variant, precision, dt, nsteps) &
+         bind(C, name="spoc_rk4_steps")
+      import
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: variant, precision, nsteps
+      real(c_double), value :: dt
+    end function
+    integer(c_int) function spoc_energy(ctx, ke, pe, mass) bind(C, name="spoc_energy")
+      import
+      type(c_ptr), value :: ctx
+      real(c_double) :: ke, pe, mass
+    end function
+    type(c_ptr) function spoc_last_error() bind(C, name="spoc_last_error")
+      import
+    end function
+  end interface
+contains
+  subroutine spoc_check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: where
+    character(kind=c_char), pointer :: msg(:)
+    integer :: n
+    if (rc == 0) return
+    call c_f_pointer(spoc_last_error(), msg, [512])
+    n = 0
+    do while (n < 512)
+      if (msg(n + 1) == c_null_char) exit
+      n = n + 1
+    end do
+    write(*, *) 'spoc: ', where, ' failed (', rc, '): ', msg(1:n)
+    stop 1
+  end subroutine
+end module
+
+subroutine cal_var(nf, ne, u0, v0, h0, u1, v1, h1, tseg, pseg, ssit, itm, its, norm, nbdl, pbdl, &
+     nint, pint, bdl, cip, vect, normt, mu0, mb0, fc, beta, w0, z0, g0, hb, time)
+  use iso_c_binding
+  use spoc_abi
+  implicit none
+  integer, parameter :: nnmod = 45, ngf_max = 9, ng_max = 33
+  integer nc, nm, ng, ngf
+  common /const/ nc, nm, ng, ngf
+  double precision wi(ng_max), xi(ng_max), yi(ng_max), wif(ngf_max), tif(ngf_max)
+  common /gauss/ wi, xi, yi
+  common /gaussf/ wif, tif
+  double precision pi2(ng_max, nnmod), pico(ngf_max, nnmod, 3)
+  common /gauss2/ pi2
+  common /gauss3/ pico
+  double precision amm(nnmod, nnmod), amml(nnmod, nnmod), agx(nnmod, nnmod), agy(nnmod, nnmod), &
+       ammx(nnmod, nnmod), ammy(nnmod, nnmod), ammo(nnmod, nnmod)
+  common /matrices/ amm, amml, agx, agy, ammx, ammy, ammo
+  integer nf, ne, nbdl, nint
+  integer tseg(2, *), pseg(2, *), ssit(2, *), itm(3, *), its(3, *), pbdl(*), pint(*), cip(*)
+  logical bdl(*)
+  double precision u0(nnmod, *), v0(nnmod, *), h0(nnmod, *), u1(nnmod, *), v1(nnmod, *), &
+       h1(nnmod, *), norm(2, *), vect(5, *), normt(2, *), fc(nnmod, *), w0(nnmod, *), &
+       z0(nnmod, *), hb(nnmod, *), mu0, mb0, beta, g0, time
+  type(c_ptr), save :: ctx = c_null_ptr
+  integer, save :: last_ne = -1, last_nf = -1, prec = 0
+  integer(c_int64_t), save :: last_sum = -1
+  double precision, save :: last_par(4) = -1.d300
+  integer(c_int), allocatable, save :: ibdl(:)
+  integer(c_int64_t) :: s
+  integer :: i, dev, stat, l
+  character(len=16) :: env
+  if (.not. c_associated(ctx)) then
+    dev = 0
+    call get_environment_variable('SPOC_DEVICE', env, l, stat)
+    if (stat == 0 .and. l > 0) read(env, *) dev
+    call get_environment_variable('SPOC_PRECISION', env, l, stat)
+    if (stat == 0 .and. l > 0 .and. env(1:4) == 'fp32') prec = 1
+    call spoc_check(spoc_ctx_create(ctx, dev, nc, ng, ngf, wi, wif, pi2, ng_max, pico, ngf_max, &
+         nnmod, amm, amml, agx, agy, ammo), 'spoc_ctx_create')
+  end if
+  ! connectivity checksum: detects remeshing between calls
+  s = 0
+  do i = 1, ne
+    s = s + int(itm(1, i), c_int64_t) * 3 + int(itm(2, i), c_int64_t) * 5 + &
+         int(itm(3, i), c_int64_t) * 7 + int(i, c_int64_t) * int(its(1, i) + 2, c_int64_t)
+  end do
+  do i = 1, nf
+    s = s + int(tseg(1, i), c_int64_t) * 11 + int(tseg(2, i), c_int64_t) * 13
+  end do
+  if (ne /= last_ne .or. nf /= last_nf .or. s /= last_sum) then
+    if (allocated(ibdl)) deallocate(ibdl)
+    allocate(ibdl(ne))
+    do i = 1, ne
+      ibdl(i) = 0
+      if (bdl(i)) ibdl(i) = 1
+    end do
+    call spoc_check(spoc_mesh_set(ctx, ne, nf, itm, its, tseg, ssit, norm, vect, normt, ibdl, &
+         0, c_null_ptr, c_null_ptr, c_null_ptr), 'spoc_mesh_set')
+    last_ne = ne
+    last_nf = nf
+    last_sum = s
+    last_par = -1.d300
+  end if
+  if (last_par(1) /= mu0 .or. last_par(2) /= mb0 .or. last_par(3) /= g0 .or. &
+       last_par(4) /= hb(1, 1)) then
+    call spoc_check(spoc_fields_set(ctx, nnmod, fc, w0, z0, hb, mu0, mb0, g0), 'spoc_fields_set')
+    last_par = [mu0, mb0, g0, hb(1, 1)]
+  end if
+  call spoc_check(spoc_rhs(ctx, SPOC_VARIANT, prec, nnmod, u0, v0, h0, u1, v1, h1), 'spoc_rhs')
+end subroutine
