@@ -80,6 +80,14 @@ __device__ __forceinline__ floatx4 chain(const float (&a)[KS], const float *B, f
   return acc;
 }
 
+// acc += A * B with the B fragment already in registers (b[s] = B[4s + l/16][l%16]).
+template <int KS>
+__device__ __forceinline__ floatx4 chain_r(const float (&a)[KS], const float (&b)[KS], floatx4 acc) {
+#pragma unroll
+  for (int s = 0; s < KS; s++) acc = mfma(a[s], b[s], acc);
+  return acc;
+}
+
 __device__ __forceinline__ floatx4 zero4() { return floatx4{0.f, 0.f, 0.f, 0.f}; }
 
 // (G_l - I) applied to a vector held in LDS (wall elements; lanes < NM).
@@ -275,10 +283,10 @@ __global__ __launch_bounds__(NT) void lin_kernel(F32Args A) {
   using Sh = S<NC, NG, NGF>;
   constexpr int NM = Sh::NM, NMk = Sh::NMk, NEk = Sh::NEk, KM = Sh::KM, KE = Sh::KE,
                 NE3 = Sh::NE3, RBM = Sh::RBM;
-  constexpr int WSZ = 16 * (3 * NMk + 3 * NEk) + 128;
+  constexpr int WSZ = 16 * 3 * NMk + 128;
   __shared__ float lds[NW * WSZ];
   const int w = threadIdx.x >> 6, l = lane(), g = l >> 4, col = l & 15;
-  float *X = lds + w * WSZ, *FB = X + 3 * NMk * 16, *V = FB + 3 * NEk * 16;
+  float *X = lds + w * WSZ, *V = X + 3 * NMk * 16;
   const int ne = A.mesh.ne, e0 = (blockIdx.x * NW + w) * 16;
   if (e0 >= ne) return;  // whole wave idle: no block-level barriers follow
   const int e = e0 + col;
@@ -300,27 +308,26 @@ __global__ __launch_bounds__(NT) void lin_kernel(F32Args A) {
     X[NMk * 16 + i] = xu;
     X[2 * NMk * 16 + i] = xv;
   }
-  // edge operands (the factor 1/2 is the edge Jacobian of tri_f)
-  for (int i = l; i < NEk * 16; i += 64) {
-    int kj = i >> 4, c = i & 15, ee = e0 + c;
-    float bx = 0.f, by = 0.f, bh = 0.f;
-    if (kj < NE3 && ee < ne) {
-      int k = kj / NGF, j = kj % NGF;
-      const float *gg = A.mesh.geo + (size_t)ee * GEO;
-      float s = gg[11 + k];
-      float fh = -g0 * face_avg<NGF, 2>(A.st.trA, A.mesh, ee, k, j, 0);
-      float fs = face_avg<NGF, 2>(A.st.trA, A.mesh, ee, k, j, 1);
-      bx = 0.5f * fh * s * gg[5 + k];
-      by = 0.5f * fh * s * gg[8 + k];
-      bh = -0.5f * H * s * fs;
-    }
-    FB[i] = bx;
-    FB[NEk * 16 + i] = by;
-    FB[2 * NEk * 16 + i] = bh;
-  }
-  wave_sync();
   const float *gg = A.mesh.geo + (size_t)(valid ? e : 0) * GEO;
   float ax = gg[0], bx = gg[1], ay = gg[2], by = gg[3], di = valid ? gg[4] : 0.f;
+  // edge operands as B fragments in registers: row kj = 4s + l/16 of this
+  // lane's element (the factor 1/2 is the edge Jacobian of tri_f)
+  float fbx[KE], fby[KE], fbh[KE];
+#pragma unroll
+  for (int s = 0; s < KE; s++) {
+    int kj = 4 * s + g;
+    fbx[s] = fby[s] = fbh[s] = 0.f;
+    if (kj < NE3 && valid) {
+      int k = kj / NGF, j = kj % NGF;
+      float sg = gg[11 + k];
+      float fh = -g0 * face_avg<NGF, 2>(A.st.trA, A.mesh, e, k, j, 0);
+      float fs = face_avg<NGF, 2>(A.st.trA, A.mesh, e, k, j, 1);
+      fbx[s] = 0.5f * fh * sg * gg[5 + k];
+      fby[s] = 0.5f * fh * sg * gg[8 + k];
+      fbh[s] = -0.5f * H * sg * fs;
+    }
+  }
+  wave_sync();
   floatx4 G[3][RBM];
 #pragma unroll
   for (int rb = 0; rb < RBM; rb++) {
@@ -329,10 +336,10 @@ __global__ __launch_bounds__(NT) void lin_kernel(F32Args A) {
     afrag<KM>(A.ops.DY, NMk, rb, 0, ady);
     afrag<KE>(A.ops.LF, NEk, rb, 0, al);
     floatx4 Ax = chain<KM>(adx, X, zero4()), Ay = chain<KM>(ady, X, zero4());
-    floatx4 Lx = chain<KE>(al, FB, zero4()), Ly = chain<KE>(al, FB + NEk * 16, zero4());
+    floatx4 Lx = chain_r<KE>(al, fbx, zero4()), Ly = chain_r<KE>(al, fby, zero4());
     floatx4 Gh = chain<KM>(adx, X + NMk * 16, zero4());
     Gh = chain<KM>(ady, X + 2 * NMk * 16, Gh);
-    Gh = chain<KE>(al, FB + 2 * NEk * 16, Gh);
+    Gh = chain_r<KE>(al, fbh, Gh);
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       G[0][rb][r] = di * (-by * Ax[r] + ay * Ay[r] + Lx[r]);
@@ -352,13 +359,12 @@ __global__ __launch_bounds__(NT) void bc_kernel(F32Args A) {
   using Sh = S<NC, NG, NGF>;
   constexpr int NM = Sh::NM, NMk = Sh::NMk, NEk = Sh::NEk, KM = Sh::KM, KE = Sh::KE,
                 NE3 = Sh::NE3, RBM = Sh::RBM, RBG = Sh::RBG, RBE = Sh::RBE, NGr = Sh::NGr;
-  constexpr int WSZ = 16 * (7 * NMk + 4 * NEk + 64) + 128;
+  constexpr int WSZ = 16 * (7 * NMk + 64) + 128;
   __shared__ float lds[NW * WSZ];
   const int w = threadIdx.x >> 6, l = lane(), g = l >> 4, col = l & 15;
   float *X = lds + w * WSZ;        // u, v, h          [3][NMk][16]
   float *Q = X + 3 * NMk * 16;     // qxx qxy qyx qyy  [4][NMk][16]
-  float *FB = Q + 4 * NMk * 16;    // edge operands    [4][NEk][16]
-  float *P = FB + 4 * NEk * 16;    // products         [4][16][16]
+  float *P = Q + 4 * NMk * 16;     // products         [4][16][16]
   float *V = P + 4 * 256;          // wall scratch     [128]
   const int ne = A.mesh.ne, e0 = (blockIdx.x * NW + w) * 16;
   if (e0 >= ne) return;
@@ -366,29 +372,27 @@ __global__ __launch_bounds__(NT) void bc_kernel(F32Args A) {
   const bool valid = e < ne;
   const float mu0 = A.fld.mu0, g0 = A.fld.g0, mb0 = A.fld.mb0;
   for (int f = 0; f < 3; f++) load_block<NM, NMk>(X + f * NMk * 16, A.st.xs[f], e0);
-  // velocity traces averaged over each face, times the outward normal / 2
-  for (int i = l; i < NEk * 16; i += 64) {
-    int kj = i >> 4, c = i & 15, ee = e0 + c;
-    float b0 = 0.f, b1 = 0.f, b2 = 0.f, b3 = 0.f;
-    if (kj < NE3 && ee < ne) {
-      int k = kj / NGF, j = kj % NGF;
-      const float *gg = A.mesh.geo + (size_t)ee * GEO;
-      float s = 0.5f * gg[11 + k], nx = s * gg[5 + k], ny = s * gg[8 + k];
-      float fu = face_avg<NGF, 2>(A.st.trA, A.mesh, ee, k, j, 0);
-      float fv = face_avg<NGF, 2>(A.st.trA, A.mesh, ee, k, j, 1);
-      b0 = fu * nx;
-      b1 = fu * ny;
-      b2 = fv * nx;
-      b3 = fv * ny;
-    }
-    FB[i] = b0;
-    FB[NEk * 16 + i] = b1;
-    FB[2 * NEk * 16 + i] = b2;
-    FB[3 * NEk * 16 + i] = b3;
-  }
-  wave_sync();
   const float *gg = A.mesh.geo + (size_t)(valid ? e : 0) * GEO;
   const float ax = gg[0], bx = gg[1], ay = gg[2], by = gg[3], di = valid ? gg[4] : 0.f;
+  // velocity traces averaged over each face, times the outward normal / 2,
+  // as B fragments in registers (row kj = 4s + l/16 of this lane's element)
+  float fb[4][KE];
+#pragma unroll
+  for (int s = 0; s < KE; s++) {
+    int kj = 4 * s + g;
+    fb[0][s] = fb[1][s] = fb[2][s] = fb[3][s] = 0.f;
+    if (kj < NE3 && valid) {
+      int k = kj / NGF, j = kj % NGF;
+      float sg = 0.5f * gg[11 + k], nx = sg * gg[5 + k], ny = sg * gg[8 + k];
+      float fu = face_avg<NGF, 2>(A.st.trA, A.mesh, e, k, j, 0);
+      float fv = face_avg<NGF, 2>(A.st.trA, A.mesh, e, k, j, 1);
+      fb[0][s] = fu * nx;
+      fb[1][s] = fu * ny;
+      fb[2][s] = fv * nx;
+      fb[3][s] = fv * ny;
+    }
+  }
+  wave_sync();
   // B: q = weak gradient (src/calvarnl_fs.f:129-158); mass matrix = identity
 #pragma unroll
   for (int rb = 0; rb < RBM; rb++) {
@@ -401,8 +405,8 @@ __global__ __launch_bounds__(NT) void bc_kernel(F32Args A) {
       // gx1 = -agx^T u, gy1 = -agy^T u
       floatx4 Ax = chain<KM>(adx, X + f * NMk * 16, zero4());
       floatx4 Ay = chain<KM>(ady, X + f * NMk * 16, zero4());
-      floatx4 La = chain<KE>(al, FB + (2 * f) * NEk * 16, zero4());
-      floatx4 Lb = chain<KE>(al, FB + (2 * f + 1) * NEk * 16, zero4());
+      floatx4 La = chain_r<KE>(al, fb[2 * f], zero4());
+      floatx4 Lb = chain_r<KE>(al, fb[2 * f + 1], zero4());
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         int row = rb * 16 + 4 * g + r;
@@ -600,10 +604,10 @@ __global__ __launch_bounds__(NT) void eg_kernel(F32Args A) {
   using Sh = S<NC, NG, NGF>;
   constexpr int NM = Sh::NM, NMk = Sh::NMk, NEk = Sh::NEk, KM = Sh::KM, KE = Sh::KE,
                 NE3 = Sh::NE3, RBM = Sh::RBM;
-  constexpr int WSZ = 16 * (6 * NMk + 3 * NEk) + 128;
+  constexpr int WSZ = 16 * 6 * NMk + 128;
   __shared__ float lds[NW * WSZ];
   const int w = threadIdx.x >> 6, l = lane(), g = l >> 4, col = l & 15;
-  float *X = lds + w * WSZ, *FB = X + 6 * NMk * 16, *V = FB + 3 * NEk * 16;
+  float *X = lds + w * WSZ, *V = X + 6 * NMk * 16;
   const int ne = A.mesh.ne, e0 = (blockIdx.x * NW + w) * 16;
   if (e0 >= ne) return;
   const int e = e0 + col;
@@ -634,22 +638,24 @@ __global__ __launch_bounds__(NT) void eg_kernel(F32Args A) {
     X[4 * NMk * 16 + i] = o4;
     X[5 * NMk * 16 + i] = o5;
   }
-  for (int i = l; i < NEk * 16; i += 64) {
-    int kj = i >> 4, c = i & 15, ee = e0 + c;
-    float b0 = 0.f, b1 = 0.f, b2 = 0.f;
-    if (kj < NE3 && ee < ne) {
+  // combined flux traces averaged over each face, times its / 2, as B
+  // fragments in registers; the eta equation's lift carries -1/det
+  float fb[3][KE];
+  const float *gg = A.mesh.geo + (size_t)(valid ? e : 0) * GEO;
+#pragma unroll
+  for (int s = 0; s < KE; s++) {
+    int kj = 4 * s + g;
+    fb[0][s] = fb[1][s] = fb[2][s] = 0.f;
+    if (kj < NE3 && valid) {
       int k = kj / NGF, j = kj % NGF;
-      float s = 0.5f * A.mesh.geo[(size_t)ee * GEO + 11 + k];
-      b0 = s * face_avg<NGF, 3>(A.st.trS, A.mesh, ee, k, j, 0);
-      b1 = s * face_avg<NGF, 3>(A.st.trS, A.mesh, ee, k, j, 1);
-      b2 = -s * face_avg<NGF, 3>(A.st.trS, A.mesh, ee, k, j, 2);
+      float sg = 0.5f * gg[11 + k];
+      fb[0][s] = sg * face_avg<NGF, 3>(A.st.trS, A.mesh, e, k, j, 0);
+      fb[1][s] = sg * face_avg<NGF, 3>(A.st.trS, A.mesh, e, k, j, 1);
+      fb[2][s] = -sg * face_avg<NGF, 3>(A.st.trS, A.mesh, e, k, j, 2);
     }
-    FB[i] = b0;
-    FB[NEk * 16 + i] = b1;
-    FB[2 * NEk * 16 + i] = b2;
   }
   wave_sync();
-  const float di = valid ? A.mesh.geo[(size_t)e * GEO + 4] : 0.f;
+  const float di = valid ? gg[4] : 0.f;
   floatx4 G[3][RBM];
 #pragma unroll
   for (int rb = 0; rb < RBM; rb++) {
@@ -661,7 +667,7 @@ __global__ __launch_bounds__(NT) void eg_kernel(F32Args A) {
     for (int f = 0; f < 3; f++) {
       floatx4 acc = chain<KM>(adx, X + (2 * f) * NMk * 16, zero4());
       acc = chain<KM>(ady, X + (2 * f + 1) * NMk * 16, acc);
-      acc = chain<KE>(al, FB + f * NEk * 16, acc);
+      acc = chain_r<KE>(al, fb[f], acc);
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         int row = rb * 16 + 4 * g + r;
