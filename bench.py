@@ -59,7 +59,11 @@ def algorithmic_counts(nm, ng, ngf, variant):
     eg_fl = 2 * (3 * 2 * nm * nm + 3 * nm * e3 + 2 * e3 * nm)
     eg_by = (8 * nm * f4 + 2 * 3 * e3 * f4 + 16 * f4 + 6 * f4
              + 3 * nm * (f8 + 2 * f8) + 3 * nm * f4 + 2 * e3 * f4)
-    return {"f32_bc": (bc_fl, bc_by), "f32_eg": (eg_fl, eg_by)}
+    # FP64 linear gravity-wave core: g grad eta and div(hh0 u) by DX/DY and
+    # the edge lift; reads x64 and the FP64 traces of both sides, writes kL
+    co_fl = 2 * (4 * nm * nm + 3 * nm * e3)
+    co_by = 3 * nm * f8 + 2 * 2 * e3 * f8 + 16 * f8 + 6 * f4 + 3 * nm * f8
+    return {"f32_bc": (bc_fl, bc_by), "f32_eg": (eg_fl, eg_by), "f64_core": (co_fl, co_by)}
 
 
 def cpu_baseline(variant, nc, ng, ngf, budget_s=15.0):
@@ -125,6 +129,9 @@ def main():
                          "(default: same as --steps)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32all"],
+                    help="fp32: linear gravity-wave core in FP64 (default); fp32all: every "
+                         "term in FP32")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported",
@@ -148,12 +155,12 @@ def main():
     # FP32-MFMA path: warm up, then time K steps from the same initial state
     ctx.upload(*c["state"])
     if W:
-        ctx.rk4(variant, dt, W, precision="fp32")
+        ctx.rk4(variant, dt, W, precision=args.precision)
     ctx.upload(*c["state"])
     ctx.sync()
     ctx.set_profiling(True)
     t0 = time.perf_counter()
-    ctx.rk4(variant, dt, K, precision="fp32")
+    ctx.rk4(variant, dt, K, precision=args.precision)
     ctx.sync()
     wall = time.perf_counter() - t0
     dev_ms, kern = ctx.timing()
@@ -169,7 +176,10 @@ def main():
         "config": {"workload": f"calvarnl_fs p={nc} ng={ng} ngf={ngf} on {ne} triangles "
                                f"(jittered {args.nx}x{args.ny} nodes, 4000x2000 km, beta plane, "
                                f"double-gyre wind, mu0=100), dt={dt} s",
-                   "ne": ne, "nm": nm, "precision": "fp32-mfma quadrature, fp64 state/RK4"},
+                   "ne": ne, "nm": nm,
+                   "precision": ("fp32-mfma quadrature, fp64-mfma linear gravity-wave core, "
+                                 "fp64 state/RK4" if args.precision == "fp32" else
+                                 "fp32-mfma for every term, fp64 state/RK4")},
     }
     # roofline of the dominant kernel (HIP events on the context stream)
     counts = algorithmic_counts(nm, ng, ngf, variant)
@@ -180,13 +190,14 @@ def main():
         avg = ms_tot / max(nlaunch, 1) / 1e3
         flops_s, bytes_s = fl * ne / avg, by * ne / avg
         ai = fl / by
-        hbm_bound = ai < FP32_PEAK / HBM_PEAK
+        fpeak = FP64_PEAK if name.startswith("f64") else FP32_PEAK
+        hbm_bound = ai < fpeak / HBM_PEAK
         if hbm_bound:
             rf = {"bound": "hbm", "achieved": bytes_s / 1e9, "peak": HBM_PEAK / 1e9,
                   "unit": "GB/s", "frac": bytes_s / HBM_PEAK}
         else:
-            rf = {"bound": "mfma", "achieved": flops_s / 1e12, "peak": FP32_PEAK / 1e12,
-                  "unit": "TFLOP/s", "frac": flops_s / FP32_PEAK}
+            rf = {"bound": "mfma", "achieved": flops_s / 1e12, "peak": fpeak / 1e12,
+                  "unit": "TFLOP/s", "frac": flops_s / fpeak}
         rf.update(kernel=name, avg_launch_us=avg * 1e6, traffic=None,
                   algorithmic_flops_per_elem=fl, algorithmic_bytes_per_elem=by,
                   other={k: {"ms": v[0], "launches": v[1]} for k, v in kern.items()})

@@ -46,9 +46,14 @@ enum {
 
 /* Arithmetic of the right-hand side. */
 enum {
-  SPOC_FP64 = 0,     /* plain FP64: reproduces the reference's operation order */
-  SPOC_FP32_MFMA = 1 /* volume/edge quadrature in FP32 on MFMA, FP64 state,
-                        mass solve and integrals */
+  SPOC_FP64 = 0,         /* plain FP64: reproduces the reference's operation order */
+  SPOC_FP32_MFMA = 1,    /* quadrature (all nonlinear, viscous, Coriolis and
+                            forcing terms) in FP32 on MFMA; the linear
+                            gravity-wave core, the RK4 state, the mass-matrix
+                            factor and the integrals in FP64 */
+  SPOC_FP32_MFMA_ALL = 2 /* every right-hand-side term in FP32 on MFMA (for
+                            error studies: its eta error grows ~1e-6 per
+                            1000 steps at p = 6) */
 };
 
 enum {

@@ -40,6 +40,7 @@ struct F32Plan {
   int nc, ng, ngf, nm, nl;
   int NMr, NMk, NGr, NEr, NEk;
   float *dops = nullptr;
+  double *dops64 = nullptr;
   F32Ops ops{};
   // mesh
   int ne = 0, nepad = 0;
@@ -169,35 +170,53 @@ int f32_plan_create(F32Plan **out, int nc, int ng, int ngf, const double *amm, c
   size_t oDX = 0, oDY = oDX + (size_t)NMr * NMk, oLF = oDY + (size_t)NMr * NMk,
          oPSI = oLF + (size_t)NMr * NEk, oPSIW = oPSI + (size_t)NGr * NMk,
          oTR = oPSIW + (size_t)NMr * NGr, oGL = oTR + (size_t)NEr * NMk,
-         oXQ = oGL + (size_t)nm * nm, oYQ = oXQ + up(ng, 4), tot = oYQ + up(ng, 4);
+         oXQ = oGL + (size_t)nm * nm, oYQ = oXQ + up(ng, 4), oLO = oYQ + up(ng, 4),
+         tot = oLO + (oGL - oDX);  // low parts of DX..TR, same layout
   std::vector<float> h(tot, 0.0f);
+  std::vector<double> h64(oGL - oDX, 0.0);  // exact values of DX..TR
   for (int i = 0; i < nm; i++)
     for (int j = 0; j < nm; j++) {
-      h[oDX + (size_t)i * NMk + j] = (float)DX[i + (size_t)nm * j];
-      h[oDY + (size_t)i * NMk + j] = (float)DY[i + (size_t)nm * j];
+      h64[oDX + (size_t)i * NMk + j] = DX[i + (size_t)nm * j];
+      h64[oDY + (size_t)i * NMk + j] = DY[i + (size_t)nm * j];
     }
   for (int i = 0; i < nm; i++)
-    for (int c = 0; c < ne3; c++) h[oLF + (size_t)i * NEk + c] = (float)LF[i + (size_t)nm * c];
+    for (int c = 0; c < ne3; c++) h64[oLF + (size_t)i * NEk + c] = LF[i + (size_t)nm * c];
   for (int q = 0; q < ng; q++)
     for (int m = 0; m < nm; m++) {
-      h[oPSI + (size_t)q * NMk + m] = (float)PSI[q + (size_t)ng * m];
-      h[oPSIW + (size_t)m * NGr + q] = (float)(wi[q] * PSI[q + (size_t)ng * m]);
+      h64[oPSI + (size_t)q * NMk + m] = PSI[q + (size_t)ng * m];
+      h64[oPSIW + (size_t)m * NGr + q] = wi[q] * PSI[q + (size_t)ng * m];
     }
   for (int r = 0; r < ne3; r++)
-    for (int m = 0; m < nm; m++) h[oTR + (size_t)r * NMk + m] = (float)TR[r + (size_t)ne3 * m];
+    for (int m = 0; m < nm; m++) h64[oTR + (size_t)r * NMk + m] = TR[r + (size_t)ne3 * m];
+  for (size_t i = 0; i < h64.size(); i++) {
+    float hi = (float)h64[i];
+    h[oDX + i] = hi;
+    h[oLO + i] = (float)(h64[i] - (double)hi);
+  }
   for (int i = 0; i < nm * nm; i++) h[oGL + i] = (float)GL[i];
   for (int q = 0; q < ng; q++) {
     h[oXQ + q] = (float)PI2(q, nc + 1);  // P_1(xi) = xi
     h[oYQ + q] = (float)PI2(q, 1);       // P_1(eta) = eta
   }
+  // FP64 images for the linear core: DX..TR (same layout) then G - I
+  std::vector<double> h64b(h64.size() + (size_t)nm * nm);
+  std::copy(h64.begin(), h64.end(), h64b.begin());
+  for (int i = 0; i < nm * nm; i++) h64b[h64.size() + i] = GL[i];
   if (hipMalloc(&p->dops, tot * sizeof(float)) != hipSuccess ||
-      hipMemcpy(p->dops, h.data(), tot * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+      hipMemcpy(p->dops, h.data(), tot * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMalloc(&p->dops64, h64b.size() * sizeof(double)) != hipSuccess ||
+      hipMemcpy(p->dops64, h64b.data(), h64b.size() * sizeof(double), hipMemcpyHostToDevice) !=
+          hipSuccess) {
     delete p;
     return -2;
   }
   float *d = p->dops;
-  p->ops = F32Ops{d + oDX, d + oDY, d + oLF, d + oPSI, d + oPSIW, d + oTR, d + oGL,
-                  d + oXQ, d + oYQ, dU};
+  double *d64 = p->dops64;
+  float *lo = d + oLO - oDX;
+  p->ops = F32Ops{d + oDX,  d + oDY,   d + oLF,   d + oPSI,  d + oPSIW, d + oTR,
+                  d + oGL,  d + oXQ,   d + oYQ,   dU,        lo + oDX,  lo + oDY,
+                  lo + oLF, lo + oPSI, lo + oPSIW, lo + oTR,
+                  d64 + oDX, d64 + oDY, d64 + oLF, d64 + oTR, d64 + (oGL - oDX)};
   *out = p;
   return 0;
 }
@@ -215,6 +234,7 @@ void f32_plan_destroy(F32Plan *p) {
   if (!p) return;
   free_mesh(p);
   if (p->dops) (void)hipFree(p->dops);
+  if (p->dops64) (void)hipFree(p->dops64);
   delete p;
 }
 
@@ -228,6 +248,7 @@ int f32_mesh_set(F32Plan *p, int ne, int nf, const int *itm, const int *its, con
   p->ne = ne;
   p->nepad = nepad;
   std::vector<float> geo((size_t)ne * GEO, 0.0f);
+  std::vector<double> geo64((size_t)ne * GEO, 0.0);
   std::vector<int> nbr((size_t)ne * 3), nbe((size_t)ne * 3), b(ne);
   p->walls_on_edge1 = true;
   for (int e = 0; e < ne; e++) {
@@ -253,34 +274,49 @@ int f32_mesh_set(F32Plan *p, int ne, int nf, const int *itm, const int *its, con
     }
     g[14] = (float)normt[2 * e];
     g[15] = (float)normt[2 * e + 1];
+    double *g64 = geo64.data() + (size_t)e * GEO;
+    for (int i = 0; i < 5; i++) g64[i] = vect[5 * e + i];
+    for (int k = 0; k < 3; k++) {
+      int f = itm[3 * e + k] - 1;
+      g64[5 + k] = norm[2 * f];
+      g64[8 + k] = norm[2 * f + 1];
+      g64[11 + k] = (double)its[3 * e + k];
+    }
+    g64[14] = normt[2 * e];
+    g64[15] = normt[2 * e + 1];
     b[e] = bdl[e] != 0;
   }
-  size_t bytes = geo.size() * 4 + (size_t)ne * 3 * 4 * 2 + (size_t)ne * 4 + 1024;
+  size_t bytes = geo64.size() * 8 + geo.size() * 4 + (size_t)ne * 3 * 4 * 2 + (size_t)ne * 4 + 1024;
   if (hipMalloc(&p->dmesh, bytes) != hipSuccess) return -2;
   char *m = (char *)p->dmesh;
-  float *dgeo = (float *)m;
+  double *dgeo64 = (double *)m;
+  float *dgeo = (float *)(dgeo64 + geo64.size());
   int *dnbr = (int *)(dgeo + geo.size()), *dnbe = dnbr + (size_t)ne * 3, *dbdl = dnbe + (size_t)ne * 3;
-  if (hipMemcpy(dgeo, geo.data(), geo.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+  if (hipMemcpy(dgeo64, geo64.data(), geo64.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dgeo, geo.data(), geo.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(dnbr, nbr.data(), nbr.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(dnbe, nbe.data(), nbe.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(dbdl, b.data(), b.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
     return -2;
-  p->mesh = F32Mesh{ne, dgeo, dnbr, dnbe, dbdl};
+  p->mesh = F32Mesh{ne, dgeo, dgeo64, dnbr, dnbe, dbdl};
   // state / work
   size_t n64 = (size_t)nepad * nm;  // per field
-  size_t b64 = 9 * n64 * 8;         // c0, acc, k
+  size_t b64 = 15 * n64 * 8 + (size_t)ne * 3 * 2 * ngf * 8;  // c0, acc, k, x64, kL, trL
   size_t b32 = (3 + 8) * n64 * 4 + (size_t)ne * 3 * 2 * ngf * 4 + (size_t)ne * 3 * 3 * ngf * 4;
   if (hipMalloc(&p->dstate, b64 + b32 + 1024) != hipSuccess) return -2;
   if (hipMemset(p->dstate, 0, b64 + b32 + 1024) != hipSuccess) return -2;
   double *s64 = (double *)p->dstate;
-  float *s32 = (float *)(s64 + 9 * n64);
+  float *s32 = (float *)(s64 + 15 * n64 + (size_t)ne * 3 * 2 * ngf);
   F32State &st = p->st;
   for (int f = 0; f < 3; f++) {
     st.c0[f] = s64 + f * n64;
     st.acc[f] = s64 + (3 + f) * n64;
     st.k[f] = s64 + (6 + f) * n64;
+    st.x64[f] = s64 + (9 + f) * n64;
+    st.kL[f] = s64 + (12 + f) * n64;
     st.xs[f] = s32 + f * n64;
   }
+  st.trL = s64 + 15 * n64;
   for (int f = 0; f < 4; f++) st.F[f] = s32 + (3 + f) * n64;
   for (int f = 0; f < 2; f++) {
     st.src[f] = s32 + (7 + f) * n64;
@@ -290,8 +326,8 @@ int f32_mesh_set(F32Plan *p, int ne, int nf, const int *itm, const int *its, con
   st.trS = st.trA + (size_t)ne * 3 * 2 * ngf;
   // fields buffer: hbq, w0q, z0q (blocked, nepad*ng each), fc3 (ne*3)
   size_t nq = (size_t)nepad * ng;
-  if (hipMalloc(&p->dfld, (3 * nq + (size_t)ne * 3) * 4 + 256) != hipSuccess) return -2;
-  if (hipMemset(p->dfld, 0, (3 * nq + (size_t)ne * 3) * 4 + 256) != hipSuccess) return -2;
+  if (hipMalloc(&p->dfld, (4 * nq + (size_t)ne * 3) * 4 + 256) != hipSuccess) return -2;
+  if (hipMemset(p->dfld, 0, (4 * nq + (size_t)ne * 3) * 4 + 256) != hipSuccess) return -2;
   return 0;
 }
 
@@ -300,7 +336,15 @@ int f32_fields_set(F32Plan *p, int ld, const double *fc, const double *w0, const
                    double g0) {
   int ne = p->ne, nepad = p->nepad, nm = p->nm, ng = p->ng, nc = p->nc;
   size_t nq = (size_t)nepad * ng;
-  std::vector<float> h(3 * nq + (size_t)ne * 3, 0.0f);
+  std::vector<float> h(4 * nq + (size_t)ne * 3, 0.0f);
+  const double hh0 = hb[0];
+  for (int e = 0; e < ne; e++)  // hb - hh0 at the quadrature points (hybrid mass flux)
+    for (int q = 0; q < ng; q++) {
+      double s = 0.0;
+      for (int m = 0; m < nm; m++)
+        s += pi2[q + (size_t)ld_g * m] * (hb[m + (size_t)ld * e] - (m == 0 ? hh0 : 0.0));
+      h[3 * nq + (size_t)ne * 3 + (size_t)(e & ~15) * ng + (size_t)q * 16 + (e & 15)] = (float)s;
+    }
   const double *src[3] = {hb, w0, z0};
   for (int f = 0; f < 3; f++) {
     if (!src[f]) continue;
@@ -318,15 +362,17 @@ int f32_fields_set(F32Plan *p, int ld, const double *fc, const double *w0, const
       h[3 * nq + 3 * (size_t)e + 2] = (float)fc[nc + 1 + (size_t)ld * e];
     }
   if (hipMemcpy(p->dfld, h.data(), h.size() * 4, hipMemcpyHostToDevice) != hipSuccess) return -2;
-  p->fld = F32Fields{p->dfld, p->dfld + nq, p->dfld + 2 * nq, p->dfld + 3 * nq, (float)mu0,
-                     (float)mb0, (float)g0, (float)hb[0]};
+  p->fld = F32Fields{p->dfld, p->dfld + nq, p->dfld + 2 * nq, p->dfld + 3 * nq,
+                     p->dfld + 3 * nq + (size_t)ne * 3, (float)mu0, (float)mb0, (float)g0,
+                     (float)hb[0], g0, hb[0]};
   p->have_fields = true;
   return 0;
 }
 
-static F32Args args(F32Plan *p, hipStream_t s, int variant, int stage, double dt) {
+static F32Args args(F32Plan *p, hipStream_t s, int variant, int stage, double dt, int hybrid) {
   F32Args a{};
   a.variant = variant;
+  a.hybrid = hybrid && variant != 0;
   a.stage = stage;
   a.dt = dt;
   a.stream = s;
@@ -345,11 +391,16 @@ static int rhs_kernels(F32Plan *p, const F32Args &a, TimerHook hook, void *user)
     if (hook) hook(user, 1, a.stream);
     return rc;
   }
+  if (a.hybrid) {
+    rc = call(p, a, 6, nullptr, nullptr);
+    if (rc) return rc;
+  }
+  if (hook) hook(user, 1, a.stream);
   rc = call(p, a, 4, nullptr, nullptr);
   if (rc) return rc;
-  if (hook) hook(user, 1, a.stream);
-  rc = call(p, a, 5, nullptr, nullptr);
   if (hook) hook(user, 2, a.stream);
+  rc = call(p, a, 5, nullptr, nullptr);
+  if (hook) hook(user, 3, a.stream);
   return rc;
 }
 
@@ -362,11 +413,11 @@ int f32_ready(const F32Plan *p) {
 
 // Tendencies of the reference-layout FP64 state x[3] (device, [e][nm]) into
 // k[3] (device, same layout).
-int f32_rhs(F32Plan *p, hipStream_t s, int variant, const double *const *x, double *const *k,
-            TimerHook hook, void *user) {
+int f32_rhs(F32Plan *p, hipStream_t s, int variant, int hybrid, const double *const *x,
+            double *const *k, TimerHook hook, void *user) {
   int rc = f32_ready(p);
   if (rc) return rc;
-  F32Args a = args(p, s, variant, -1, 0.0);
+  F32Args a = args(p, s, variant, -1, 0.0, hybrid);
   for (int f = 0; f < 3; f++)
     if ((rc = call(p, a, 0, x[f], p->st.c0[f]))) return rc;
   if ((rc = call(p, a, 2, nullptr, nullptr))) return rc;
@@ -376,11 +427,11 @@ int f32_rhs(F32Plan *p, hipStream_t s, int variant, const double *const *x, doub
   return 0;
 }
 
-int f32_rk4(F32Plan *p, hipStream_t s, int variant, double *const *u0, double dt, int nsteps,
-            TimerHook hook, void *user) {
+int f32_rk4(F32Plan *p, hipStream_t s, int variant, int hybrid, double *const *u0, double dt,
+            int nsteps, TimerHook hook, void *user) {
   int rc = f32_ready(p);
   if (rc) return rc;
-  F32Args a = args(p, s, variant, -1, dt);
+  F32Args a = args(p, s, variant, -1, dt, hybrid);
   for (int f = 0; f < 3; f++)
     if ((rc = call(p, a, 0, u0[f], p->st.c0[f]))) return rc;
   if ((rc = call(p, a, 2, nullptr, nullptr))) return rc;

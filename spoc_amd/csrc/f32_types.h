@@ -25,6 +25,12 @@ struct F32Ops {
   const float *GL;       // [NM][NM]    U Pl U^T - I (wall projector minus identity)
   const float *xq, *yq;  // quadrature points on the reference triangle
   const double *U;       // [NM*NM] upper factor, column-major (basis change)
+  // low parts: A = hi + lo with hi = fl32(A), lo = fl32(A - hi), so that a
+  // product costs two MFMAs and carries ~46 significant bits of the operator
+  const float *DXl, *DYl, *LFl, *PSIl, *PSIWl, *TRl;
+  // FP64 images of DX, DY, LF, TR (same padded layouts) and G - I, for the
+  // linear gravity-wave core that runs on FP64 MFMA
+  const double *DX64, *DY64, *LF64, *TR64, *GL64;
 };
 
 // Per-element mesh data of the FP32 path.
@@ -36,6 +42,7 @@ struct F32Ops {
 struct F32Mesh {
   int ne;
   const float *geo;
+  const double *geo64;  // same 16 words in FP64
   const int *nbr;  // [ne*3] neighbour element across local edge k, -1 on a wall
   const int *nbe;  // [ne*3] that neighbour's local edge
   const int *bdl;  // [ne]
@@ -44,7 +51,9 @@ struct F32Mesh {
 struct F32Fields {
   const float *hbq, *w0q, *z0q;  // [ne][NG] values at quadrature points
   const float *fc3;              // [ne][3] Coriolis f = fc0 + fc1*eta + fc2*xi
+  const float *hbdq;             // [ne][NG] hb - hh0 at quadrature points (blocked)
   float mu0, mb0, g0, hh0;
+  double g0d, hh0d;
 };
 
 struct F32State {
@@ -54,11 +63,15 @@ struct F32State {
   float *trS;              // [ne][3][3][NGF] combined flux traces (BC -> EG)
   float *F[4], *src[2], *uh[2];  // [ne][NM] BC -> EG
   double *k[3];            // [ne][NM] psi tendencies (stage < 0)
+  double *x64[3];          // [ne][NM] FP64 stage input (linear core)
+  double *trL;             // [ne][3][2][NGF] FP64 traces: T_h, hh0 (T_u, T_v).n
+  double *kL[3];           // [ne][NM] linear-core tendencies
 };
 
 struct F32Args {
   int variant;
   int stage;  // -1: tendencies only, 0..3: RK4 stage
+  int hybrid; // nonlinear variants: linear gravity-wave core in FP64 (kL)
   double dt;
   hipStream_t stream;
   F32Ops ops;

@@ -2,7 +2,9 @@
 the mass-matrix norm, sqrt(sum_e det_e d^T M d / sum_e det_e x^T M x):
 FP32 quadrature (unit roundoff 6e-8) over ~100 terms per value gives a few
 1e-7; the tests allow 2e-5 for one right-hand side and 1e-6 for eta after
-RK4 steps of the linear config."""
+RK4 steps of the linear config.  "fp32" is the production mode (linear
+gravity-wave core on FP64 MFMA, the rest in FP32); "fp32all" puts every
+term in FP32 and is held to the same bounds over these short runs."""
 import sys
 
 import numpy as np
@@ -45,16 +47,19 @@ def say(*a):
     sys.stdout.flush()
 
 
+CASES = [("lin", "fp32")] + [(v, p) for v in ("fs", "ns", "inv") for p in ("fp32", "fp32all")]
+
+
 @pytest.mark.parametrize("nc,ng,ngf", SHAPES)
-@pytest.mark.parametrize("variant", ["lin", "fs", "ns", "inv"])
-def test_cal_var_fp32_close_to_oracle(nc, ng, ngf, variant):
+@pytest.mark.parametrize("variant,prec", CASES)
+def test_cal_var_fp32_close_to_oracle(nc, ng, ngf, variant, prec):
     t, ctx = ctx_for(nc, ng, ngf)
     O = oracle_for(nc, ng, ngf)
     nx = 17 if nc <= 3 else 12
     c = configs.random_case(t, nx, nx - 2, seed=200 + nc, scale=0.05)
-    say("f32 case", variant, nc, ng, ngf, "ne", c["mesh"]["ne"])
+    say("f32 case", variant, prec, nc, ng, ngf, "ne", c["mesh"]["ne"])
     setup_case(ctx, c)
-    got = ctx.cal_var(variant, *c["state"], precision="fp32")
+    got = ctx.cal_var(variant, *c["state"], precision=prec)
     ref = O.cal_var(variant, c["mesh"], c["phys"], *c["state"])
     errs = [mrel(t, c["mesh"], a, b) for a, b in zip(got, ref)]
     say("  M-norm rel err u v h:", errs)
@@ -75,16 +80,17 @@ def test_rk4_fp32_linear_config1_eta_error():
     assert e[2] < 1e-6, e
 
 
-@pytest.mark.parametrize("variant", ["fs", "ns", "inv"])
-def test_rk4_fp32_nonlinear_p6(variant):
+@pytest.mark.parametrize("variant,prec", CASES[1:])
+def test_rk4_fp32_nonlinear_p6(variant, prec):
     t, ctx = ctx_for(6, 33, 7)
     O = oracle_for(6, 33, 7)
     c = configs.random_case(t, 13, 9, seed=31, scale=0.01)
     setup_case(ctx, c)
     ctx.upload(*c["state"])
-    ctx.rk4(variant, 0.5, 5, precision="fp32")
+    say("f32 rk4 p6", variant, prec)
+    ctx.rk4(variant, 0.5, 5, precision=prec)
     got = ctx.download()
     ref = O.rk4(variant, c["mesh"], c["phys"], *c["state"], 0.5, 5)
     e = [mrel(t, c["mesh"], a, b) for a, b in zip(got, ref)]
-    say(variant, "p6 5 steps fp32 rel err u v h:", e)
+    say(variant, prec, "p6 5 steps rel err u v h:", e)
     assert max(e) < 1e-5, e
