@@ -43,27 +43,23 @@ def quad_rule(ng, ngf):
 
 def algorithmic_counts(nm, ng, ngf, variant):
     """Useful (unpadded) flops and compulsory HBM bytes per element per
-    right-hand side of each FP32 kernel, from its GEMM shapes and the arrays
-    it must read and write once."""
+    right-hand side of each kernel, from its product shapes and the arrays it
+    must read and write once (neighbour traces counted once, RK4 traffic
+    averaged over the four stages)."""
     e3 = 3 * ngf
     f4, f8 = 4, 8
+    mesh = 16 * f8 + 6 * f4 + f4                       # geo64, nbr/nbe, bdl
+    rk = 3 * nm * f8 * (1 + 0.75) + 3 * nm * f8 * 2     # c0, acc reads; acc|c0, x64 writes
+    core_fl = 2 * (4 * nm * nm + 6 * nm * e3)            # -g grad eta, div(hh0 u), traces
+    core_by = 3 * nm * f8 + 2 * e3 * f8 + rk + 2 * e3 * f8   # x64, trL in, RK, trL out
     if variant == "lin":
-        flops = 2 * (3 * nm * nm * 2 + 3 * nm * e3) + 2 * (2 * e3 * nm)
-        byt = (3 * nm * f4 + 2 * 2 * e3 * f4 + 16 * f4 + 6 * f4   # xs, own+nbr traces, geo, nbr
-               + 3 * nm * (2 * f8 + f8)                            # c0, acc r/w (avg), xs out
-               + 2 * e3 * f4)
-        return {"f32_lin": (flops, byt)}
-    bc_fl = 2 * (2 * 2 * nm * nm + 4 * nm * e3 + 5 * e3 * nm + 7 * ng * nm + 4 * nm * ng + 2 * e3 * nm)
-    bc_by = (3 * nm * f4 + 3 * ng * f4 + 16 * f4 + 3 * f4 + 2 * 2 * e3 * f4 + 6 * f4
-             + 8 * nm * f4 + 3 * e3 * f4)
-    eg_fl = 2 * (3 * 2 * nm * nm + 3 * nm * e3 + 2 * e3 * nm)
-    eg_by = (8 * nm * f4 + 2 * 3 * e3 * f4 + 16 * f4 + 6 * f4
-             + 3 * nm * (f8 + 2 * f8) + 3 * nm * f4 + 2 * e3 * f4)
-    # FP64 linear gravity-wave core: g grad eta and div(hh0 u) by DX/DY and
-    # the edge lift; reads x64 and the FP64 traces of both sides, writes kL
-    co_fl = 2 * (4 * nm * nm + 3 * nm * e3)
-    co_by = 3 * nm * f8 + 2 * 2 * e3 * f8 + 16 * f8 + 6 * f4 + 3 * nm * f8
-    return {"f32_bc": (bc_fl, bc_by), "f32_eg": (eg_fl, eg_by), "f64_core": (co_fl, co_by)}
+        return {"lin_f64": (core_fl, core_by + mesh)}
+    bc_fl = 2 * (4 * nm * nm + 10 * nm * e3 + 11 * nm * ng)
+    bc_by = (3 * nm * f8 + 2 * e3 * f4 + 16 * f4 + 6 * f4 + f4 + 4 * ng * f4 + 3 * f4
+             + (4 + 2 + 2) * nm * f4 + 3 * e3 * f4)
+    eg_fl = 2 * (6 * nm * nm + 3 * nm * e3) + core_fl
+    eg_by = (8 * nm * f4 + 3 * e3 * f4 + 16 * f4 + mesh + core_by + 2 * e3 * f4)
+    return {"bc_f32": (bc_fl, bc_by), "eg_f32f64": (eg_fl, eg_by)}
 
 
 def cpu_baseline(variant, nc, ng, ngf, budget_s=15.0):
@@ -129,9 +125,6 @@ def main():
                          "(default: same as --steps)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32all"],
-                    help="fp32: linear gravity-wave core in FP64 (default); fp32all: every "
-                         "term in FP32")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported",
@@ -155,12 +148,12 @@ def main():
     # FP32-MFMA path: warm up, then time K steps from the same initial state
     ctx.upload(*c["state"])
     if W:
-        ctx.rk4(variant, dt, W, precision=args.precision)
+        ctx.rk4(variant, dt, W, precision="fp32")
     ctx.upload(*c["state"])
     ctx.sync()
     ctx.set_profiling(True)
     t0 = time.perf_counter()
-    ctx.rk4(variant, dt, K, precision=args.precision)
+    ctx.rk4(variant, dt, K, precision="fp32")
     ctx.sync()
     wall = time.perf_counter() - t0
     dev_ms, kern = ctx.timing()
@@ -177,9 +170,8 @@ def main():
                                f"(jittered {args.nx}x{args.ny} nodes, 4000x2000 km, beta plane, "
                                f"double-gyre wind, mu0=100), dt={dt} s",
                    "ne": ne, "nm": nm,
-                   "precision": ("fp32-mfma quadrature, fp64-mfma linear gravity-wave core, "
-                                 "fp64 state/RK4" if args.precision == "fp32" else
-                                 "fp32-mfma for every term, fp64 state/RK4")},
+                   "precision": "fp32-mfma quadrature, fp64-mfma linear gravity-wave core, "
+                                "fp64 state/RK4"},
     }
     # roofline of the dominant kernel (HIP events on the context stream)
     counts = algorithmic_counts(nm, ng, ngf, variant)
@@ -190,7 +182,7 @@ def main():
         avg = ms_tot / max(nlaunch, 1) / 1e3
         flops_s, bytes_s = fl * ne / avg, by * ne / avg
         ai = fl / by
-        fpeak = FP64_PEAK if name.startswith("f64") else FP32_PEAK
+        fpeak = FP32_PEAK if name == "bc_f32" else FP64_PEAK
         hbm_bound = ai < fpeak / HBM_PEAK
         if hbm_bound:
             rf = {"bound": "hbm", "achieved": bytes_s / 1e9, "peak": HBM_PEAK / 1e9,

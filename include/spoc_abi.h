@@ -46,14 +46,12 @@ enum {
 
 /* Arithmetic of the right-hand side. */
 enum {
-  SPOC_FP64 = 0,         /* plain FP64: reproduces the reference's operation order */
-  SPOC_FP32_MFMA = 1,    /* quadrature (all nonlinear, viscous, Coriolis and
-                            forcing terms) in FP32 on MFMA; the linear
-                            gravity-wave core, the RK4 state, the mass-matrix
-                            factor and the integrals in FP64 */
-  SPOC_FP32_MFMA_ALL = 2 /* every right-hand-side term in FP32 on MFMA (for
-                            error studies: its eta error grows ~1e-6 per
-                            1000 steps at p = 6) */
+  SPOC_FP64 = 0,     /* plain FP64: reproduces the reference's operation order */
+  SPOC_FP32_MFMA = 1 /* mixed precision on the matrix cores: the quadrature of
+                        the nonlinear, viscous, Coriolis and forcing terms in
+                        FP32 MFMA; the linear gravity-wave core (all of
+                        SPOC_LIN), the wall projections, the RK4 state, the
+                        mass-matrix factor and the integrals in FP64 */
 };
 
 enum {

@@ -21,8 +21,8 @@ LIB_PATH = os.path.join(HERE, "libspoc_amd.so")
 
 LIN, FS, NS, INV = 0, 1, 2, 3
 VARIANTS = {"lin": LIN, "fs": FS, "ns": NS, "inv": INV}
-FP64, FP32_MFMA, FP32_MFMA_ALL = 0, 1, 2
-PRECISIONS = {"fp64": FP64, "fp32": FP32_MFMA, "fp32all": FP32_MFMA_ALL}
+FP64, FP32_MFMA = 0, 1
+PRECISIONS = {"fp64": FP64, "fp32": FP32_MFMA}
 LDM = 45  # nnmod, the reference's leading dimension of modal arrays
 
 _dp = C.POINTER(C.c_double)

@@ -35,10 +35,10 @@ int f32_mesh_set(F32Plan *p, int ne, int nf, const int *itm, const int *its, con
 int f32_fields_set(F32Plan *p, int ld, const double *fc, const double *w0, const double *z0,
                    const double *hb, const double *pi2, int ld_g, double mu0, double mb0,
                    double g0);
-int f32_rhs(F32Plan *p, hipStream_t s, int variant, int hybrid, const double *const *x,
-            double *const *k, TimerHook hook, void *user);
-int f32_rk4(F32Plan *p, hipStream_t s, int variant, int hybrid, double *const *u0, double dt,
-            int nsteps, TimerHook hook, void *user);
+int f32_rhs(F32Plan *p, hipStream_t s, int variant, const double *const *x, double *const *k,
+            TimerHook hook, void *user);
+int f32_rk4(F32Plan *p, hipStream_t s, int variant, double *const *u0, double dt, int nsteps,
+            TimerHook hook, void *user);
 }  // namespace spoc
 
 namespace {
@@ -621,8 +621,8 @@ int spoc_rhs(spoc_ctx *c, int variant, int precision, int ld, const double *u0,
   if (rc) return rc;
   const double *x[3] = {xs[0], xs[1], xs[2]};
   double *ks[3] = {state_ptr(c, 9), state_ptr(c, 10), state_ptr(c, 11)};
-  if (precision == SPOC_FP32_MFMA || precision == SPOC_FP32_MFMA_ALL) {
-    rc = f32_rhs(c->f32, c->stream, variant, precision == SPOC_FP32_MFMA, x, ks, nullptr, nullptr);
+  if (precision == SPOC_FP32_MFMA) {
+    rc = f32_rhs(c->f32, c->stream, variant, x, ks, nullptr, nullptr);
     if (rc) return f32_fail(rc);
   } else {
     RhsArgs a = make_args(c, variant, x, -1, 0.0);
@@ -644,10 +644,9 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
   HIPCHK(hipEventRecord(c->t0, c->stream));
   const double *u0[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
   const double *xs[3] = {state_ptr(c, 3), state_ptr(c, 4), state_ptr(c, 5)};
-  if (precision == SPOC_FP32_MFMA || precision == SPOC_FP32_MFMA_ALL) {
+  if (precision == SPOC_FP32_MFMA) {
     double *uw[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
-    rc = f32_rk4(c->f32, c->stream, variant, precision == SPOC_FP32_MFMA, uw, dt, nsteps,
-                 prof_hook, c);
+    rc = f32_rk4(c->f32, c->stream, variant, uw, dt, nsteps, prof_hook, c);
     if (rc) return f32_fail(rc);
   } else {
     for (int it = 0; it < nsteps; it++)
@@ -682,12 +681,11 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
     const char *lin[] = {"face_lin", "elem_lin"};
     const char *nl[] = {"face_uv", "elem_bc", "face_nl2", "elem_eg"};
     int nk = variant == SPOC_LIN ? 2 : 4;
-    if (precision == SPOC_FP32_MFMA || precision == SPOC_FP32_MFMA_ALL) {
-      const char *l32[] = {"f32_lin"};
-      const char *n32[] = {"f64_core", "f32_bc", "f32_eg"};
-      nk = variant == SPOC_LIN ? 1 : 3;
+    if (precision == SPOC_FP32_MFMA) {
+      const char *l32[] = {"lin_f64"};
+      const char *n32[] = {"bc_f32", "eg_f32f64"};
+      nk = variant == SPOC_LIN ? 1 : 2;
       for (int k = 0; k < nk; k++) {
-        if (variant != SPOC_LIN && k == 0 && precision == SPOC_FP32_MFMA_ALL) continue;
         c->knames.push_back(variant == SPOC_LIN ? l32[k] : n32[k]);
         c->kms.push_back(tot[k]);
         c->kcount.push_back(cnt[k]);

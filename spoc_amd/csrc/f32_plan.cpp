@@ -1,6 +1,7 @@
-// Host side of the FP32-MFMA path: operators of the mass-orthonormal basis
-// (formed in FP64 from the reference tables, rounded once to FP32),
-// per-element mesh data, device buffers and the launch sequence.
+// Host side of the mixed-precision MFMA path: operators of the
+// mass-orthonormal basis (formed in FP64 from the reference tables, stored
+// as MFMA operand images in FP32 and FP64), per-element mesh data, device
+// buffers and the launch sequence.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -34,13 +35,38 @@ std::vector<double> upper_inverse(int n, const double *U, int ld) {
   }
   return X;
 }
+
+// FP32 image of the R x K operator op(i, k), rows permuted by sigma_row.
+template <class F>
+void img32(float *dst, int R, int K, F op) {
+  const int RB = cdiv(R, 16), KQ = cdiv(K, 16);
+  for (int rb = 0; rb < RB; rb++)
+    for (int q = 0; q < KQ; q++)
+      for (int l = 0; l < 64; l++)
+        for (int t = 0; t < 4; t++) {
+          int i = sigma_row(16 * rb + (l & 15)), k = 4 * (4 * q + t) + (l >> 4);
+          dst[((size_t)(rb * KQ + q) * 64 + l) * 4 + t] = (i < R && k < K) ? (float)op(i, k) : 0.0f;
+        }
+}
+
+// FP64 image of op(i, k), rows in order.
+template <class F>
+void img64(double *dst, int R, int K, F op) {
+  const int RB = cdiv(R, 16), KH = cdiv(K, 8);
+  for (int rb = 0; rb < RB; rb++)
+    for (int h = 0; h < KH; h++)
+      for (int l = 0; l < 64; l++)
+        for (int t = 0; t < 2; t++) {
+          int i = 16 * rb + (l & 15), k = 4 * (2 * h + t) + (l >> 4);
+          dst[((size_t)(rb * KH + h) * 64 + l) * 2 + t] = (i < R && k < K) ? op(i, k) : 0.0;
+        }
+}
 }  // namespace
 
 struct F32Plan {
   int nc, ng, ngf, nm, nl;
-  int NMr, NMk, NGr, NEr, NEk;
-  float *dops = nullptr;
-  double *dops64 = nullptr;
+  float *dpack32 = nullptr, *dgl32 = nullptr;
+  double *dpack64 = nullptr, *dgl64 = nullptr;
   F32Ops ops{};
   // mesh
   int ne = 0, nepad = 0;
@@ -54,9 +80,9 @@ struct F32Plan {
   // state/work
   void *dstate = nullptr;
   F32State st{};
-  // host copy of the basis change for testing
-  std::vector<double> Uinv;
 };
+
+void f32_plan_destroy(F32Plan *p);
 
 static int call(const F32Plan *p, const F32Args &a, int phase, const double *in, double *out) {
 #define SPOC_CALL(NC_, NG_, NGF_) \
@@ -77,14 +103,8 @@ int f32_plan_create(F32Plan **out, int nc, int ng, int ngf, const double *amm, c
   int nm = nm_of(nc), nl = nm - nc - 1, ne3 = 3 * ngf;
   p->nm = nm;
   p->nl = nl;
-  p->NMr = up(nm, 16);
-  p->NMk = up(nm, 4);
-  p->NGr = up(ng, 16);
-  p->NEr = up(ne3, 16);
-  p->NEk = up(ne3, 4);
   // FP64 building blocks
   std::vector<double> Ui = upper_inverse(nm, amm, ld_m);  // U^{-1}
-  p->Uinv = Ui;
   auto UI = [&](int a, int b) { return Ui[a + (size_t)nm * b]; };
   std::vector<double> ULi = upper_inverse(nl, amml, ld_m);
   auto AGX = [&](int a, int b) { return agx[a + (size_t)ld_m * b]; };
@@ -165,58 +185,45 @@ int f32_plan_create(F32Plan **out, int nc, int ng, int ngf, const double *amm, c
       }
       GL[i * nm + j] = s - (i == j ? 1.0 : 0.0);  // row-major
     }
-  // FP32 padded device images
-  int NMr = p->NMr, NMk = p->NMk, NGr = p->NGr, NEr = p->NEr, NEk = p->NEk;
-  size_t oDX = 0, oDY = oDX + (size_t)NMr * NMk, oLF = oDY + (size_t)NMr * NMk,
-         oPSI = oLF + (size_t)NMr * NEk, oPSIW = oPSI + (size_t)NGr * NMk,
-         oTR = oPSIW + (size_t)NMr * NGr, oGL = oTR + (size_t)NEr * NMk,
-         oXQ = oGL + (size_t)nm * nm, oYQ = oXQ + up(ng, 4), oLO = oYQ + up(ng, 4),
-         tot = oLO + (oGL - oDX);  // low parts of DX..TR, same layout
-  std::vector<float> h(tot, 0.0f);
-  std::vector<double> h64(oGL - oDX, 0.0);  // exact values of DX..TR
-  for (int i = 0; i < nm; i++)
-    for (int j = 0; j < nm; j++) {
-      h64[oDX + (size_t)i * NMk + j] = DX[i + (size_t)nm * j];
-      h64[oDY + (size_t)i * NMk + j] = DY[i + (size_t)nm * j];
-    }
-  for (int i = 0; i < nm; i++)
-    for (int c = 0; c < ne3; c++) h64[oLF + (size_t)i * NEk + c] = LF[i + (size_t)nm * c];
-  for (int q = 0; q < ng; q++)
-    for (int m = 0; m < nm; m++) {
-      h64[oPSI + (size_t)q * NMk + m] = PSI[q + (size_t)ng * m];
-      h64[oPSIW + (size_t)m * NGr + q] = wi[q] * PSI[q + (size_t)ng * m];
-    }
-  for (int r = 0; r < ne3; r++)
-    for (int m = 0; m < nm; m++) h64[oTR + (size_t)r * NMk + m] = TR[r + (size_t)ne3 * m];
-  for (size_t i = 0; i < h64.size(); i++) {
-    float hi = (float)h64[i];
-    h[oDX + i] = hi;
-    h[oLO + i] = (float)(h64[i] - (double)hi);
-  }
-  for (int i = 0; i < nm * nm; i++) h[oGL + i] = (float)GL[i];
+  // device images (layouts in f32_types.h)
+  const Pack32 P = pack32(nm, ng, ne3);
+  const Pack64 Q = pack64(nm, ne3);
+  std::vector<float> h32(P.total, 0.0f), g32(img32_size(nm, nm), 0.0f);
+  std::vector<double> h64(Q.total, 0.0), g64(img64_size(nm, nm), 0.0);
+  auto dx = [&](int i, int k) { return DX[i + (size_t)nm * k]; };
+  auto dy = [&](int i, int k) { return DY[i + (size_t)nm * k]; };
+  auto lf = [&](int i, int k) { return LF[i + (size_t)nm * k]; };
+  auto tr = [&](int i, int k) { return TR[i + (size_t)ne3 * k]; };
+  auto psi = [&](int i, int k) { return PSI[i + (size_t)ng * k]; };
+  auto psiw = [&](int i, int k) { return wi[k] * PSI[k + (size_t)ng * i]; };
+  auto gl = [&](int i, int k) { return GL[(size_t)i * nm + k]; };
+  img32(h32.data() + P.dx, nm, nm, dx);
+  img32(h32.data() + P.dy, nm, nm, dy);
+  img32(h32.data() + P.lf, nm, ne3, lf);
+  img32(h32.data() + P.tr, ne3, nm, tr);
+  img32(h32.data() + P.psi, ng, nm, psi);
+  img32(h32.data() + P.psiw, nm, ng, psiw);
   for (int q = 0; q < ng; q++) {
-    h[oXQ + q] = (float)PI2(q, nc + 1);  // P_1(xi) = xi
-    h[oYQ + q] = (float)PI2(q, 1);       // P_1(eta) = eta
+    h32[P.xq + q] = (float)PI2(q, nc + 1);  // P_1(xi) = xi
+    h32[P.yq + q] = (float)PI2(q, 1);       // P_1(eta) = eta
   }
-  // FP64 images for the linear core: DX..TR (same layout) then G - I
-  std::vector<double> h64b(h64.size() + (size_t)nm * nm);
-  std::copy(h64.begin(), h64.end(), h64b.begin());
-  for (int i = 0; i < nm * nm; i++) h64b[h64.size() + i] = GL[i];
-  if (hipMalloc(&p->dops, tot * sizeof(float)) != hipSuccess ||
-      hipMemcpy(p->dops, h.data(), tot * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMalloc(&p->dops64, h64b.size() * sizeof(double)) != hipSuccess ||
-      hipMemcpy(p->dops64, h64b.data(), h64b.size() * sizeof(double), hipMemcpyHostToDevice) !=
-          hipSuccess) {
-    delete p;
+  img32(g32.data(), nm, nm, gl);
+  img64(h64.data() + Q.dx, nm, nm, dx);
+  img64(h64.data() + Q.dy, nm, nm, dy);
+  img64(h64.data() + Q.lf, nm, ne3, lf);
+  img64(h64.data() + Q.tr, ne3, nm, tr);
+  img64(g64.data(), nm, nm, gl);
+  auto put = [](auto **d, const auto &h) {
+    size_t b = h.size() * sizeof(h[0]);
+    return hipMalloc(d, b) == hipSuccess &&
+           hipMemcpy(*d, h.data(), b, hipMemcpyHostToDevice) == hipSuccess;
+  };
+  if (!put(&p->dpack32, h32) || !put(&p->dgl32, g32) || !put(&p->dpack64, h64) ||
+      !put(&p->dgl64, g64)) {
+    f32_plan_destroy(p);
     return -2;
   }
-  float *d = p->dops;
-  double *d64 = p->dops64;
-  float *lo = d + oLO - oDX;
-  p->ops = F32Ops{d + oDX,  d + oDY,   d + oLF,   d + oPSI,  d + oPSIW, d + oTR,
-                  d + oGL,  d + oXQ,   d + oYQ,   dU,        lo + oDX,  lo + oDY,
-                  lo + oLF, lo + oPSI, lo + oPSIW, lo + oTR,
-                  d64 + oDX, d64 + oDY, d64 + oLF, d64 + oTR, d64 + (oGL - oDX)};
+  p->ops = F32Ops{p->dpack32, p->dpack64, p->dgl32, p->dgl64, dU};
   *out = p;
   return 0;
 }
@@ -233,8 +240,8 @@ static void free_mesh(F32Plan *p) {
 void f32_plan_destroy(F32Plan *p) {
   if (!p) return;
   free_mesh(p);
-  if (p->dops) (void)hipFree(p->dops);
-  if (p->dops64) (void)hipFree(p->dops64);
+  for (void *d : {(void *)p->dpack32, (void *)p->dgl32, (void *)p->dpack64, (void *)p->dgl64})
+    if (d) (void)hipFree(d);
   delete p;
 }
 
@@ -299,32 +306,31 @@ int f32_mesh_set(F32Plan *p, int ne, int nf, const int *itm, const int *its, con
       hipMemcpy(dbdl, b.data(), b.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
     return -2;
   p->mesh = F32Mesh{ne, dgeo, dgeo64, dnbr, dnbe, dbdl};
-  // state / work
-  size_t n64 = (size_t)nepad * nm;  // per field
-  size_t b64 = 15 * n64 * 8 + (size_t)ne * 3 * 2 * ngf * 8;  // c0, acc, k, x64, kL, trL
-  size_t b32 = (3 + 8) * n64 * 4 + (size_t)ne * 3 * 2 * ngf * 4 + (size_t)ne * 3 * 3 * ngf * 4;
-  if (hipMalloc(&p->dstate, b64 + b32 + 1024) != hipSuccess) return -2;
-  if (hipMemset(p->dstate, 0, b64 + b32 + 1024) != hipSuccess) return -2;
+  // state / work (element-blocked, see f32_types.h)
+  const int ne3 = 3 * ngf;
+  const size_t n64 = (size_t)nepad * nm, nt = (size_t)nepad * ne3;
+  const size_t b64 = (12 * n64 + 4 * nt) * 8, b32 = (8 * n64 + 5 * nt) * 4;
+  if (hipMalloc(&p->dstate, b64 + b32 + 256) != hipSuccess) return -2;
+  if (hipMemset(p->dstate, 0, b64 + b32 + 256) != hipSuccess) return -2;
   double *s64 = (double *)p->dstate;
-  float *s32 = (float *)(s64 + 15 * n64 + (size_t)ne * 3 * 2 * ngf);
+  float *s32 = (float *)(s64 + 12 * n64 + 4 * nt);
   F32State &st = p->st;
   for (int f = 0; f < 3; f++) {
     st.c0[f] = s64 + f * n64;
     st.acc[f] = s64 + (3 + f) * n64;
-    st.k[f] = s64 + (6 + f) * n64;
-    st.x64[f] = s64 + (9 + f) * n64;
-    st.kL[f] = s64 + (12 + f) * n64;
-    st.xs[f] = s32 + f * n64;
+    st.x64[f] = s64 + (6 + f) * n64;
+    st.k[f] = s64 + (9 + f) * n64;
   }
-  st.trL = s64 + 15 * n64;
-  for (int f = 0; f < 4; f++) st.F[f] = s32 + (3 + f) * n64;
+  st.trL[0] = s64 + 12 * n64;
+  st.trL[1] = st.trL[0] + 2 * nt;
+  for (int f = 0; f < 4; f++) st.F[f] = s32 + f * n64;
   for (int f = 0; f < 2; f++) {
-    st.src[f] = s32 + (7 + f) * n64;
-    st.uh[f] = s32 + (9 + f) * n64;
+    st.src[f] = s32 + (4 + f) * n64;
+    st.uh[f] = s32 + (6 + f) * n64;
   }
-  st.trA = s32 + 11 * n64;
-  st.trS = st.trA + (size_t)ne * 3 * 2 * ngf;
-  // fields buffer: hbq, w0q, z0q (blocked, nepad*ng each), fc3 (ne*3)
+  st.trA = s32 + 8 * n64;
+  st.trS = st.trA + 2 * nt;
+  // fields buffer: hbq, w0q, z0q, hbdq (blocked, nepad*ng each), fc3 (ne*3)
   size_t nq = (size_t)nepad * ng;
   if (hipMalloc(&p->dfld, (4 * nq + (size_t)ne * 3) * 4 + 256) != hipSuccess) return -2;
   if (hipMemset(p->dfld, 0, (4 * nq + (size_t)ne * 3) * 4 + 256) != hipSuccess) return -2;
@@ -338,7 +344,7 @@ int f32_fields_set(F32Plan *p, int ld, const double *fc, const double *w0, const
   size_t nq = (size_t)nepad * ng;
   std::vector<float> h(4 * nq + (size_t)ne * 3, 0.0f);
   const double hh0 = hb[0];
-  for (int e = 0; e < ne; e++)  // hb - hh0 at the quadrature points (hybrid mass flux)
+  for (int e = 0; e < ne; e++)  // hb - hh0 at the quadrature points (FP32 mass flux)
     for (int q = 0; q < ng; q++) {
       double s = 0.0;
       for (int m = 0; m < nm; m++)
@@ -362,17 +368,15 @@ int f32_fields_set(F32Plan *p, int ld, const double *fc, const double *w0, const
       h[3 * nq + 3 * (size_t)e + 2] = (float)fc[nc + 1 + (size_t)ld * e];
     }
   if (hipMemcpy(p->dfld, h.data(), h.size() * 4, hipMemcpyHostToDevice) != hipSuccess) return -2;
-  p->fld = F32Fields{p->dfld, p->dfld + nq, p->dfld + 2 * nq, p->dfld + 3 * nq,
-                     p->dfld + 3 * nq + (size_t)ne * 3, (float)mu0, (float)mb0, (float)g0,
-                     (float)hb[0], g0, hb[0]};
+  p->fld = F32Fields{p->dfld, p->dfld + nq, p->dfld + 2 * nq, p->dfld + 3 * nq + (size_t)ne * 3,
+                     p->dfld + 3 * nq, (float)mu0, (float)mb0, g0, hh0};
   p->have_fields = true;
   return 0;
 }
 
-static F32Args args(F32Plan *p, hipStream_t s, int variant, int stage, double dt, int hybrid) {
+static F32Args args(F32Plan *p, hipStream_t s, int variant, int stage, double dt) {
   F32Args a{};
   a.variant = variant;
-  a.hybrid = hybrid && variant != 0;
   a.stage = stage;
   a.dt = dt;
   a.stream = s;
@@ -391,16 +395,11 @@ static int rhs_kernels(F32Plan *p, const F32Args &a, TimerHook hook, void *user)
     if (hook) hook(user, 1, a.stream);
     return rc;
   }
-  if (a.hybrid) {
-    rc = call(p, a, 6, nullptr, nullptr);
-    if (rc) return rc;
-  }
-  if (hook) hook(user, 1, a.stream);
   rc = call(p, a, 4, nullptr, nullptr);
   if (rc) return rc;
-  if (hook) hook(user, 2, a.stream);
+  if (hook) hook(user, 1, a.stream);
   rc = call(p, a, 5, nullptr, nullptr);
-  if (hook) hook(user, 3, a.stream);
+  if (hook) hook(user, 2, a.stream);
   return rc;
 }
 
@@ -413,11 +412,11 @@ int f32_ready(const F32Plan *p) {
 
 // Tendencies of the reference-layout FP64 state x[3] (device, [e][nm]) into
 // k[3] (device, same layout).
-int f32_rhs(F32Plan *p, hipStream_t s, int variant, int hybrid, const double *const *x,
-            double *const *k, TimerHook hook, void *user) {
+int f32_rhs(F32Plan *p, hipStream_t s, int variant, const double *const *x, double *const *k,
+            TimerHook hook, void *user) {
   int rc = f32_ready(p);
   if (rc) return rc;
-  F32Args a = args(p, s, variant, -1, 0.0, hybrid);
+  F32Args a = args(p, s, variant, -1, 0.0);
   for (int f = 0; f < 3; f++)
     if ((rc = call(p, a, 0, x[f], p->st.c0[f]))) return rc;
   if ((rc = call(p, a, 2, nullptr, nullptr))) return rc;
@@ -427,11 +426,11 @@ int f32_rhs(F32Plan *p, hipStream_t s, int variant, int hybrid, const double *co
   return 0;
 }
 
-int f32_rk4(F32Plan *p, hipStream_t s, int variant, int hybrid, double *const *u0, double dt,
-            int nsteps, TimerHook hook, void *user) {
+int f32_rk4(F32Plan *p, hipStream_t s, int variant, double *const *u0, double dt, int nsteps,
+            TimerHook hook, void *user) {
   int rc = f32_ready(p);
   if (rc) return rc;
-  F32Args a = args(p, s, variant, -1, dt, hybrid);
+  F32Args a = args(p, s, variant, -1, dt);
   for (int f = 0; f < 3; f++)
     if ((rc = call(p, a, 0, u0[f], p->st.c0[f]))) return rc;
   if ((rc = call(p, a, 2, nullptr, nullptr))) return rc;

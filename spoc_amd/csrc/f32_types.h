@@ -1,12 +1,21 @@
-// Data of the FP32-MFMA right-hand side.
+// Data of the mixed-precision (FP32 + FP64 MFMA) right-hand side.
 //
 // This path works in the coordinates of the mass-orthonormal basis
 // psi = U^{-T} phi (U the upper Cholesky factor of the reference mass matrix,
 // src/shwwnl.f:153): the state is c = U a, the mass matrix is the identity,
-// and every operator is a well-conditioned dense matrix formed once in FP64
-// and rounded to FP32.  (In the reference's own modal basis the mass matrix
-// has condition number ~2e8 at p = 6, which turns FP32 rounding of the
-// test-function integrals into ~1e-4 relative errors after the solve.)
+// and every operator is a dense matrix formed once in FP64.  (In the
+// reference's own modal basis the mass matrix has condition number ~2e8 at
+// p = 6, which turns FP32 rounding of the test-function integrals into ~1e-4
+// relative errors after the solve.)
+//
+// Register layout shared by every kernel: for a block of 16 elements, lane
+// l = 16 g + c (c = element column) holds mode m = 4 s + g of element c at
+// k-step s.  That is the B-operand layout of v_mfma_*_16x16x4, and it is
+// also the D (result) layout of the FP64 instruction, so an FP64 product can
+// feed the next product from registers.  The FP32 instruction's D layout
+// holds rows 4 g + r of each 16-row block instead; the FP32 operator images
+// store their rows permuted by (16 b + 4 i + j) <-> (16 b + 4 j + i), which
+// makes their results come out in the shared layout too.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -14,64 +23,105 @@ namespace spoc {
 
 constexpr int GEO = 16;  // per-element geometry words (see F32Mesh)
 
-// Operators (device, FP32, row-major with rows padded to 16 and the
-// contraction length padded to 4 or 16, zeros in the padding).
+__host__ __device__ constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// Operator images.  An R x K operator is the A operand of v_mfma_*_16x16x4:
+// row block rb (16 rows) at k-step s (4 columns); lane l = i + 16 g holds
+// A[16 rb + i][4 s + g].  FP32 images keep 4 k-steps per lane together (one
+// 16-byte load): img[((rb * KQ + q) * 64 + l) * 4 + t] with s = 4 q + t,
+// KQ = ceil(K / 16).  FP64 images keep 2: img[((rb * KH + h) * 64 + l) * 2 + t]
+// with s = 2 h + t, KH = ceil(K / 8).  Padding is zero.
+__host__ __device__ constexpr int img32_size(int R, int K) { return cdiv(R, 16) * cdiv(K, 16) * 256; }
+__host__ __device__ constexpr int img64_size(int R, int K) { return cdiv(R, 16) * cdiv(K, 8) * 128; }
+// Physical row of an FP32 image that holds logical row i (an involution).
+__host__ __device__ constexpr int sigma_row(int i) {
+  return (i & ~15) | ((i & 3) << 2) | ((i >> 2) & 3);
+}
+
+// FP32 operators, one buffer, offsets in floats.  The element kernels stage
+// [0, eg_end) (E+G) or [0, total) (B+C) in LDS.
+struct Pack32 {
+  int dx, dy, lf, eg_end, tr, psi, psiw, xq, yq, total;
+};
+__host__ __device__ constexpr Pack32 pack32(int nm, int ng, int ne3) {
+  Pack32 p{};
+  int o = 0;
+  p.dx = o;   o += img32_size(nm, nm);   // U^-T agx^T U^-1
+  p.dy = o;   o += img32_size(nm, nm);   // U^-T agy^T U^-1
+  p.lf = o;   o += img32_size(nm, ne3);  // U^-T (wif pico)^T: edge lift
+  p.eg_end = o;
+  p.tr = o;   o += img32_size(ne3, nm);  // pico U^-1: traces at the edge points
+  p.psi = o;  o += img32_size(ng, nm);   // pi2 U^-1: values at the quadrature points
+  p.psiw = o; o += img32_size(nm, ng);   // (W pi2 U^-1)^T: projection
+  p.xq = o;   o += cdiv(ng, 16) * 16;    // quadrature points
+  p.yq = o;   o += cdiv(ng, 16) * 16;
+  p.total = o;
+  return p;
+}
+
+// FP64 operators (same meaning), offsets in doubles.
+struct Pack64 {
+  int dx, dy, lf, tr, total;
+};
+__host__ __device__ constexpr Pack64 pack64(int nm, int ne3) {
+  Pack64 p{};
+  int o = 0;
+  p.dx = o; o += img64_size(nm, nm);
+  p.dy = o; o += img64_size(nm, nm);
+  p.lf = o; o += img64_size(nm, ne3);
+  p.tr = o; o += img64_size(ne3, nm);
+  p.total = o;
+  return p;
+}
+
 struct F32Ops {
-  const float *DX, *DY;  // [NMr][NMk]  U^-T agx^T U^-1, U^-T agy^T U^-1
-  const float *LF;       // [NMr][NEk]  U^-T (wif * pico)^T : edge lift
-  const float *PSI;      // [NGr][NMk]  pi2 U^-1 : values at quadrature points
-  const float *PSIW;     // [NMr][NGr]  (W pi2 U^-1)^T : projection
-  const float *TR;       // [NEr][NMk]  pico U^-1 : traces at edge points
-  const float *GL;       // [NM][NM]    U Pl U^T - I (wall projector minus identity)
-  const float *xq, *yq;  // quadrature points on the reference triangle
+  const float *pack32;   // Pack32 layout
+  const double *pack64;  // Pack64 layout
+  const float *gl32;     // img32 of G - I, G = U R^T (R amm R^T)^-1 R U^T (wall projector)
+  const double *gl64;    // img64 of G - I
   const double *U;       // [NM*NM] upper factor, column-major (basis change)
-  // low parts: A = hi + lo with hi = fl32(A), lo = fl32(A - hi), so that a
-  // product costs two MFMAs and carries ~46 significant bits of the operator
-  const float *DXl, *DYl, *LFl, *PSIl, *PSIWl, *TRl;
-  // FP64 images of DX, DY, LF, TR (same padded layouts) and G - I, for the
-  // linear gravity-wave core that runs on FP64 MFMA
-  const double *DX64, *DY64, *LF64, *TR64, *GL64;
 };
 
-// Per-element mesh data of the FP32 path.
-//   geo[e*GEO + 0..4]  ax bx ay by 1/det
-//   geo[e*GEO + 5..7]  x-component of the face normal of local edge k
-//   geo[e*GEO + 8..10] y-component of the face normal of local edge k
+// Per-element mesh data.
+//   geo[e*GEO + 0..4]   ax bx ay by 1/det
+//   geo[e*GEO + 5..7]   x-component of the face normal of local edge k
+//   geo[e*GEO + 8..10]  y-component of the face normal of local edge k
 //   geo[e*GEO + 11..13] its(k, e) as +-1
 //   geo[e*GEO + 14..15] unit wall normal normt (wall elements)
 struct F32Mesh {
   int ne;
   const float *geo;
   const double *geo64;  // same 16 words in FP64
-  const int *nbr;  // [ne*3] neighbour element across local edge k, -1 on a wall
-  const int *nbe;  // [ne*3] that neighbour's local edge
-  const int *bdl;  // [ne]
+  const int *nbr;       // [ne*3] neighbour element across local edge k, -1 on a wall
+  const int *nbe;       // [ne*3] that neighbour's local edge
+  const int *bdl;       // [ne]
 };
 
+// Element-blocked arrays: element e = 16 b + c, row r of an array with R
+// rows per element at [(b * R + r) * 16 + c].
 struct F32Fields {
-  const float *hbq, *w0q, *z0q;  // [ne][NG] values at quadrature points
+  const float *hbq, *w0q, *z0q;  // [NG rows] values at quadrature points
+  const float *hbdq;             // [NG rows] hb - hh0 at quadrature points
   const float *fc3;              // [ne][3] Coriolis f = fc0 + fc1*eta + fc2*xi
-  const float *hbdq;             // [ne][NG] hb - hh0 at quadrature points (blocked)
-  float mu0, mb0, g0, hh0;
-  double g0d, hh0d;
+  float mu0, mb0;
+  double g0, hh0;
 };
 
 struct F32State {
-  double *c0[3], *acc[3];  // [ne][NM] psi coordinates, FP64
-  float *xs[3];            // [ne][NM] FP32 shadow of the stage input
-  float *trA;              // [ne][3][2][NGF] traces of the stage input
-  float *trS;              // [ne][3][3][NGF] combined flux traces (BC -> EG)
-  float *F[4], *src[2], *uh[2];  // [ne][NM] BC -> EG
-  double *k[3];            // [ne][NM] psi tendencies (stage < 0)
-  double *x64[3];          // [ne][NM] FP64 stage input (linear core)
-  double *trL;             // [ne][3][2][NGF] FP64 traces: T_h, hh0 (T_u, T_v).n
-  double *kL[3];           // [ne][NM] linear-core tendencies
+  double *c0[3], *acc[3];  // [NM rows] psi coordinates: step start, RK4 sum
+  double *x64[3];          // [NM rows] stage input
+  double *k[3];            // [NM rows] psi tendencies (stage < 0)
+  float *F[4], *src[2], *uh[2];  // [NM rows] B+C -> E+G
+  float *trA;              // [2*NE3 rows] T_u, T_v of the stage input
+  double *trL[2];          // [2*NE3 rows] T_eta, hh0 (T_u, T_v).n_face (wall-projected);
+                           // RK4 stage s reads trL[s & 1] and writes trL[(s + 1) & 1],
+                           // because one launch reads neighbours' traces and writes its own
+  float *trS;              // [3*NE3 rows] flux traces (B+C -> E+G)
 };
 
 struct F32Args {
   int variant;
   int stage;  // -1: tendencies only, 0..3: RK4 stage
-  int hybrid; // nonlinear variants: linear gravity-wave core in FP64 (kL)
   double dt;
   hipStream_t stream;
   F32Ops ops;

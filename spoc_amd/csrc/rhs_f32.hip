@@ -1,28 +1,30 @@
-// FP32-MFMA right-hand side for gfx950 (v_mfma_f32_16x16x4_f32).
+// Mixed-precision right-hand side for gfx950: FP32 MFMA
+// (v_mfma_f32_16x16x4_f32) for the quadrature of the nonlinear, viscous,
+// Coriolis and forcing terms; FP64 MFMA (v_mfma_f64_16x16x4_f64) for the
+// linear gravity-wave core, the wall projections of the tendencies and the
+// traces of the stage input; FP64 RK4 state.
 //
-// Coordinates: the mass-orthonormal basis psi = U^{-T} phi (see f32_types.h),
-// so every step of the reference's cal_var is a dense small GEMM
+// Coordinates: the mass-orthonormal basis psi = U^{-T} phi, so each step of
+// the reference's cal_var is a small dense product
 //   out[row, element] = Op[row, k] * X[k, element]
-// with the operator rows on the MFMA M side and 16 elements on the N side.
+// with the operator on the MFMA A side and 16 elements on the N side.  One
+// wavefront owns a block of 16 elements.  Element operands stay in
+// registers from load to store: a product's result is already the B operand
+// of the next product (layout in f32_types.h), and the per-element scalars
+// (Jacobian terms, normals, quadrature-point products) are lane-local
+// because an MFMA column is one element.  The operator images of a kernel
+// are staged in LDS once per workgroup.
 //
-// Work split: one wavefront owns a block of 16 elements for the whole
-// kernel; the element operands live in that wave's slice of LDS in
-// [k][16] layout (the MFMA B fragment), operator fragments are loaded from
-// global memory (L1/L2-resident, a few KB) into registers once per row block
-// and reused across fields.  Per-element scalars (Jacobian terms, normals)
-// multiply the D fragments lane-locally because an MFMA output column is one
-// element.
-//
-// Edge coupling: every element stores the FP32 traces of its fields at the
+// Edge coupling: every element stores the traces of its fields at the
 // Gauss-Legendre points of its three edges; the consumer averages its own
 // trace with the neighbour's (reversed along the shared edge), as the
 // reference's face average does (src/calflu.f:41-62).  Both sides of a face
 // read the same two stored numbers, so the face flux is single-valued.
 //
-// Kernels per right-hand side:  linear: lin_kernel;  nonlinear: bc_kernel
-// (gradient q, quadrature-point products, mass fluxes) then eg_kernel
-// (u, v, eta tendencies).  Both end in the FP64 RK4 stage update and the
-// traces of the next stage input.
+// Kernels per right-hand side:  linear: lin_kernel (all FP64);  nonlinear:
+// bc_kernel (FP32: weak gradient q, quadrature-point products, mass flux)
+// then eg_kernel (FP32 divergence of the B+C results plus the FP64 linear
+// core, wall projection, RK4 stage, traces of the next stage input).
 #include <hip/hip_runtime.h>
 
 #include "common.h"
@@ -32,758 +34,785 @@ namespace spoc {
 namespace f32 {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
-constexpr int NT = 256, NW = 4;
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+typedef double doublex2 __attribute__((ext_vector_type(2)));
+constexpr int NW = 4, NT = 64 * NW;  // waves (element blocks) per workgroup
 
-constexpr int up(int x, int m) { return (x + m - 1) / m * m; }
+// Register budgets as waves per SIMD (512 registers per lane shared by the
+// resident waves): the element kernels are memory-latency bound, so the
+// compiler is held to at least two resident waves.
+#ifndef BC_WAVES
+#define BC_WAVES 3
+#endif
+#ifndef EG_WAVES
+#define EG_WAVES 2
+#endif
 
 template <int NC, int NG, int NGF>
 struct S {
   static constexpr int NM = nm_of(NC), NE3 = 3 * NGF;
-  static constexpr int NMr = up(NM, 16), NMk = up(NM, 4);
-  static constexpr int NGr = up(NG, 16);
-  static constexpr int NEr = up(NE3, 16), NEk = up(NE3, 4);
-  static constexpr int KM = NMk / 4, KE = NEk / 4;
-  static constexpr int RBM = NMr / 16, RBG = NGr / 16, RBE = NEr / 16;
+  static constexpr int KM = cdiv(NM, 4), KE = cdiv(NE3, 4), KG = cdiv(NG, 4);     // k-steps
+  static constexpr int RBM = cdiv(NM, 16), RBE = cdiv(NE3, 16), RBG = cdiv(NG, 16);  // row blocks
+  static constexpr Pack32 P32 = pack32(NM, NG, NE3);
+  static constexpr Pack64 P64 = pack64(NM, NE3);
 };
-
-__device__ __forceinline__ floatx4 mfma(float a, float b, floatx4 c) {
-  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-}
 
 __device__ __forceinline__ int lane() { return threadIdx.x & 63; }
 
-// Order this wave's LDS writes before its later LDS reads by other lanes.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+__device__ __forceinline__ floatx4 mfma32(float a, float b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
-
-// Operators are applied as hi + lo parts (two MFMAs per k-step) unless the
-// build sets SPOC_F32_SPLIT=0: a single FP32 rounding of the operators
-// shifts the discrete wave frequencies by ~6e-8 relative, which grows into
-// ~1e-6 relative error after 1000 steps at p = 6.
-#ifndef SPOC_F32_SPLIT
-#define SPOC_F32_SPLIT 1
-#endif
-
-template <int KS>
-struct Frag {
-  float h[KS];
-#if SPOC_F32_SPLIT
-  float l[KS];
-#endif
-};
-
-// A fragments of row block rb, k-steps ks0..ks0+KS-1 of a row-major operator:
-// lane l holds A[16*rb + l%16][4*ks + l/16].
-template <int KS>
-__device__ __forceinline__ void afrag(const float *__restrict__ A, const float *__restrict__ Alo,
-                                      int lda, int rb, int ks0, Frag<KS> &a) {
-  int l = lane();
-  size_t o = (size_t)(rb * 16 + (l & 15)) * lda + 4 * ks0 + (l >> 4);
-#pragma unroll
-  for (int s = 0; s < KS; s++) a.h[s] = A[o + 4 * s];
-#if SPOC_F32_SPLIT
-#pragma unroll
-  for (int s = 0; s < KS; s++) a.l[s] = Alo[o + 4 * s];
-#else
-  (void)Alo;
-#endif
-}
-
-// acc += A * B, B in LDS as [4*KS][16]: lane l reads B[4s + l/16][l%16].
-template <int KS>
-__device__ __forceinline__ floatx4 chain(const Frag<KS> &a, const float *B, floatx4 acc) {
-  int l = lane();
-  const float *p = B + (l >> 4) * 16 + (l & 15);
-#pragma unroll
-  for (int s = 0; s < KS; s++) {
-    float b = p[64 * s];
-#if SPOC_F32_SPLIT
-    acc = mfma(a.l[s], b, acc);
-#endif
-    acc = mfma(a.h[s], b, acc);
-  }
-  return acc;
-}
-
-// acc += A * B with the B fragment already in registers (b[s] = B[4s + l/16][l%16]).
-template <int KS>
-__device__ __forceinline__ floatx4 chain_r(const Frag<KS> &a, const float (&b)[KS], floatx4 acc) {
-#pragma unroll
-  for (int s = 0; s < KS; s++) {
-#if SPOC_F32_SPLIT
-    acc = mfma(a.l[s], b[s], acc);
-#endif
-    acc = mfma(a.h[s], b[s], acc);
-  }
-  return acc;
-}
-
-__device__ __forceinline__ floatx4 zero4() { return floatx4{0.f, 0.f, 0.f, 0.f}; }
-
-// FP64 MFMA (v_mfma_f64_16x16x4_f64): the A and B lane layouts are those of
-// the FP32 instruction, but the four D values of lane l are rows
-// l/16, 4 + l/16, 8 + l/16, 12 + l/16 of column l%16 (the FP32 instruction
-// holds rows 4(l/16) .. 4(l/16)+3).
-typedef double doublex4 __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ int drow64(int rb, int g, int r) { return rb * 16 + 4 * r + g; }
 __device__ __forceinline__ doublex4 mfma64(double a, double b, doublex4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
-__device__ __forceinline__ doublex4 zero4d() { return doublex4{0.0, 0.0, 0.0, 0.0}; }
+__device__ __forceinline__ floatx4 zero32() { return floatx4{0.f, 0.f, 0.f, 0.f}; }
+__device__ __forceinline__ doublex4 zero64() { return doublex4{0.0, 0.0, 0.0, 0.0}; }
 
-template <int KS>
-__device__ __forceinline__ void afrag64(const double *__restrict__ A, int lda, int rb,
-                                        double (&a)[KS]) {
-  int l = lane();
-  const double *p = A + (size_t)(rb * 16 + (l & 15)) * lda + (l >> 4);
+// Keep the scheduler from interleaving the phases of a kernel: hoisting the
+// loads of a later phase above an earlier one only raises register
+// pressure (and with it lowers occupancy).
+__device__ __forceinline__ void phase_fence() { __builtin_amdgcn_sched_barrier(0); }
+
+// Mode (or row) held by register r of row block rb in lane group g.
+__device__ __forceinline__ int drow(int rb, int r, int g) { return 16 * rb + 4 * r + g; }
+
+// A fragments, k-steps 0 .. ceil(K/4)-1 of row block rb.
+template <int K>
+__device__ __forceinline__ void afrag32(const float *img, int rb, float (&a)[cdiv(K, 4)]) {
+  constexpr int KS = cdiv(K, 4), KQ = cdiv(K, 16);
+  const floatx4 *p = reinterpret_cast<const floatx4 *>(img) + (size_t)rb * KQ * 64 + lane();
 #pragma unroll
-  for (int s = 0; s < KS; s++) a[s] = p[4 * s];
+  for (int q = 0; q < KQ; q++) {
+    floatx4 v = p[q * 64];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+      if (4 * q + t < KS) a[4 * q + t] = v[t];
+  }
+}
+template <int K>
+__device__ __forceinline__ void afrag64(const double *img, int rb, double (&a)[cdiv(K, 4)]) {
+  constexpr int KS = cdiv(K, 4), KH = cdiv(K, 8);
+  const doublex2 *p = reinterpret_cast<const doublex2 *>(img) + (size_t)rb * KH * 64 + lane();
+#pragma unroll
+  for (int h = 0; h < KH; h++) {
+    doublex2 v = p[h * 64];
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+      if (2 * h + t < KS) a[2 * h + t] = v[t];
+  }
 }
 
 template <int KS>
-__device__ __forceinline__ doublex4 chain64(const double (&a)[KS], const double *B, doublex4 acc) {
-  int l = lane();
-  const double *p = B + (l >> 4) * 16 + (l & 15);
+__device__ __forceinline__ floatx4 chain32(const float (&a)[KS], const float (&b)[KS], floatx4 acc) {
 #pragma unroll
-  for (int s = 0; s < KS; s++) acc = mfma64(a[s], p[64 * s], acc);
+  for (int s = 0; s < KS; s++) acc = mfma32(a[s], b[s], acc);
   return acc;
 }
-
 template <int KS>
-__device__ __forceinline__ doublex4 chain64_r(const double (&a)[KS], const double (&b)[KS],
-                                              doublex4 acc) {
+__device__ __forceinline__ doublex4 chain64(const double (&a)[KS], const double (&b)[KS],
+                                            doublex4 acc) {
 #pragma unroll
   for (int s = 0; s < KS; s++) acc = mfma64(a[s], b[s], acc);
   return acc;
 }
 
-template <int NM>
-__device__ __forceinline__ double gl_apply64(const double *__restrict__ GL, const double *v, int i) {
-  double d = 0.0;
-  for (int j = 0; j < NM; j++) d += GL[i * NM + j] * v[j];
-  return d;
+// Result fragments (row blocks) -> B fragments (k-steps).
+template <int KS, int RB>
+__device__ __forceinline__ void d2b(const floatx4 (&D)[RB], float (&b)[KS]) {
+  static_assert(KS <= 4 * RB, "k-steps exceed the rows");
+#pragma unroll
+  for (int s = 0; s < KS; s++) b[s] = D[s >> 2][s & 3];
 }
 
-// Wall projection of a vector pair held in LDS at stride 16 (column c),
-// FP64: free slip (wallmode 1) puts the normal component on the reduced
-// basis, no slip (2) both components.  Lanes < NM take part; V holds 128
-// doubles of scratch.
-template <int NM>
-__device__ void wall_project64(const double *__restrict__ GL, double *gx, double *gy, double nx,
-                               double ny, int wallmode, double *V) {
-  int l = lane();
-  if (wallmode == 2) {
-    if (l < NM) {
-      V[l] = gx[16 * l];
-      V[64 + l] = gy[16 * l];
+// B fragments of an element-blocked [rows][16] array (rows >= R read as 0).
+template <int R, int KS>
+__device__ __forceinline__ void bload(const double *base, int g, int col, bool valid,
+                                      double (&b)[KS]) {
+#pragma unroll
+  for (int s = 0; s < KS; s++) {
+    int m = 4 * s + g;
+    b[s] = (valid && m < R) ? base[m * 16 + col] : 0.0;
+  }
+}
+template <int R, int KS>
+__device__ __forceinline__ void bload(const double *base, int g, int col, bool valid,
+                                      float (&b)[KS]) {
+#pragma unroll
+  for (int s = 0; s < KS; s++) {
+    int m = 4 * s + g;
+    b[s] = (valid && m < R) ? (float)base[m * 16 + col] : 0.f;
+  }
+}
+template <int R, int KS>
+__device__ __forceinline__ void bload(const float *base, int g, int col, bool valid,
+                                      float (&b)[KS]) {
+#pragma unroll
+  for (int s = 0; s < KS; s++) {
+    int m = 4 * s + g;
+    b[s] = (valid && m < R) ? base[m * 16 + col] : 0.f;
+  }
+}
+
+// Row kj of field f of element e in a blocked trace array with NF fields.
+template <int NE3, int NF, typename T>
+__device__ __forceinline__ T tr_at(const T *tr, int e, int f, int kj) {
+  return tr[((size_t)(e >> 4) * (NF * NE3) + f * NE3 + kj) * 16 + (e & 15)];
+}
+template <int NE3, int NF, typename T>
+__device__ __forceinline__ void tr_put(T *tr, int eb, int col, int f, int kj, T v) {
+  tr[((size_t)eb * (NF * NE3) + f * NE3 + kj) * 16 + col] = v;
+}
+
+// Average of own and neighbour traces of element e, local edge k, point j
+// (local orientation), field f.  Walls use the element's own trace on its
+// local edge 1, like the reference's wall faces (src/calflu.f:31-39).
+template <int NGF, int NF, typename T>
+__device__ __forceinline__ T face_avg(const T *__restrict__ tr, const F32Mesh &M, int e, int k,
+                                      int j, int f) {
+  constexpr int NE3 = 3 * NGF;
+  int nb = M.nbr[e * 3 + k];
+  if (nb < 0) return tr_at<NE3, NF>(tr, e, f, j);
+  int kn = M.nbe[e * 3 + k];
+  T a = tr_at<NE3, NF>(tr, e, f, k * NGF + j);
+  T b = tr_at<NE3, NF>(tr, nb, f, kn * NGF + (NGF - 1 - j));
+  return T(0.5) * (a + b);
+}
+
+// Element block of this wave.  The workgroups on XCD x (blockIdx % 8, the
+// hardware's round-robin placement) take one contiguous eighth of the
+// blocks, so the neighbour traces they read were written through the same L2.
+__device__ __forceinline__ int first_block() {
+  int g8 = gridDim.x >> 3;
+  return ((blockIdx.x & 7) * g8 + (blockIdx.x >> 3)) * NW;
+}
+
+// Copy n 16-byte words of operator images to LDS (whole workgroup).
+__device__ __forceinline__ void stage_lds(floatx4 *dst, const floatx4 *__restrict__ src, int n) {
+  for (int i = threadIdx.x; i < n; i += NT) dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
+// FP64 pieces shared by the linear and the nonlinear paths.
+// ---------------------------------------------------------------------------
+
+// Wall projection of a vector pair (a, b) held as B fragments of this wave's
+// 16 elements, in place: free slip (NS == false) puts the normal component
+// on the reduced basis (src/calvarnl_fs.f:298-315), no slip (NS) both
+// components (src/calvarnl_ns.f:247-265).  G - I is applied by MFMA to the
+// columns of wall elements only (the others are masked to zero).
+template <int NM, bool NS>
+__device__ __forceinline__ void wall_pair64(const double *__restrict__ gl, bool wall, double nx,
+                                            double ny, double (&a)[cdiv(NM, 4)],
+                                            double (&b)[cdiv(NM, 4)]) {
+  constexpr int KM = cdiv(NM, 4), RBM = cdiv(NM, 16);
+  if (NS) {
+    double ma[KM], mb[KM];
+#pragma unroll
+    for (int s = 0; s < KM; s++) {
+      ma[s] = wall ? a[s] : 0.0;
+      mb[s] = wall ? b[s] : 0.0;
     }
-    wave_sync();
-    if (l < NM) {
-      double d1 = gl_apply64<NM>(GL, V, l), d2 = gl_apply64<NM>(GL, V + 64, l);
-      gx[16 * l] += d1;
-      gy[16 * l] += d2;
+#pragma unroll
+    for (int rb = 0; rb < RBM; rb++) {
+      double ag[KM];
+      afrag64<NM>(gl, rb, ag);
+      doublex4 d1 = chain64(ag, ma, zero64()), d2 = chain64(ag, mb, zero64());
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        if (4 * rb + r < KM) {
+          a[4 * rb + r] += d1[r];
+          b[4 * rb + r] += d2[r];
+        }
     }
   } else {
-    if (l < NM) V[l] = nx * gx[16 * l] + ny * gy[16 * l];
-    wave_sync();
-    if (l < NM) {
-      double d = gl_apply64<NM>(GL, V, l);
-      gx[16 * l] += nx * d;
-      gy[16 * l] += ny * d;
-    }
-  }
-  wave_sync();
-}
-
-// (G_l - I) applied to a vector held in LDS (wall elements; lanes < NM).
-template <int NM>
-__device__ __forceinline__ float gl_apply(const float *__restrict__ GL, const float *v, int i) {
-  float d = 0.f;
-  for (int j = 0; j < NM; j++) d += GL[i * NM + j] * v[j];
-  return d;
-}
-
-// ---------------------------------------------------------------------------
-// Traces of the stage input at the 3*NGF edge points, written per element as
-// trA[e][k][2][NGF].  Nonlinear: (T_u, T_v).  Linear: (T_h, S_uv) with
-// S_uv = T_u * nfx + T_v * nfy on the face normal of each edge.
-// X holds u, v, h of the wave's 16 elements in [NMk][16] layout.
-// ---------------------------------------------------------------------------
-template <int NC, int NG, int NGF>
-__device__ void emit_traces(const F32Args &A, const float *X, int e0, bool lin) {
-  using Sh = S<NC, NG, NGF>;
-  constexpr int NMk = Sh::NMk, KM = Sh::KM, NE3 = Sh::NE3, RBE = Sh::RBE;
-  const int l = lane(), g = l >> 4, col = l & 15, e = e0 + col;
-  const bool valid = e < A.mesh.ne;
-  const float *geo = A.mesh.geo + (size_t)(valid ? e : 0) * GEO;
+    double v[KM];
 #pragma unroll
-  for (int rb = 0; rb < RBE; rb++) {
-    Frag<KM> at;
-    afrag<KM>(A.ops.TR, A.ops.TRl, NMk, rb, 0, at);
-    floatx4 T0, T1;
-    if (lin) {
-      T0 = chain<KM>(at, X + 2 * NMk * 16, zero4());  // h
-      floatx4 Tu = chain<KM>(at, X, zero4()), Tv = chain<KM>(at, X + NMk * 16, zero4());
+    for (int s = 0; s < KM; s++) v[s] = wall ? nx * a[s] + ny * b[s] : 0.0;
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        int row = rb * 16 + 4 * g + r, k = row / NGF;
-        float nx = k < 3 ? geo[5 + k] : 0.f, ny = k < 3 ? geo[8 + k] : 0.f;
-        T1[r] = Tu[r] * nx + Tv[r] * ny;
-      }
-    } else {
-      T0 = chain<KM>(at, X, zero4());
-      T1 = chain<KM>(at, X + NMk * 16, zero4());
-    }
-    if (valid) {
+    for (int rb = 0; rb < RBM; rb++) {
+      double ag[KM];
+      afrag64<NM>(gl, rb, ag);
+      doublex4 d = chain64(ag, v, zero64());
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        int row = rb * 16 + 4 * g + r;
-        if (row < NE3) {
-          int k = row / NGF, j = row % NGF;
-          float *t = A.st.trA + ((size_t)e * 3 + k) * 2 * NGF;
-          t[j] = T0[r];
-          t[NGF + j] = T1[r];
+      for (int r = 0; r < 4; r++)
+        if (4 * rb + r < KM) {
+          a[4 * rb + r] += nx * d[r];
+          b[4 * rb + r] += ny * d[r];
         }
-      }
     }
   }
 }
 
-// FP64 traces of the stage input for the hybrid path.  X holds u, v, h in
-// FP64, [3][NMk][16].  Writes trA (FP32 T_u, T_v, for the FP32 gradient)
-// and trL (FP64 T_h, and S = hh0 (T_u nfx + T_v nfy) of the wall-projected
-// mass flux, src/calvarnl_fs.f:298-321) for the linear core.  Destroys X's
-// u, v columns of wall elements.
-template <int NC, int NG, int NGF>
-__device__ void emit_traces64(const F32Args &A, double *X, double *V, int e0, int wallmode) {
+// Linear gravity-wave core (VAR 0: the whole linear right-hand side,
+// src/calvar_lin.f; nonlinear variants: the -g grad(eta) terms and the
+// -div(hh0 u) term, src/calvarnl_fs.f:298-321, 436-471 and the ns/inv
+// equivalents) from the FP64 stage input x64 and its traces trL.
+// G[f] receives the three tendencies (u, v, eta), result layout.
+template <int NC, int NG, int NGF, int VAR>
+__device__ __forceinline__ void core64(const F32Args &A, const double *O, int eb,
+                                       doublex4 (&G)[3][S<NC, NG, NGF>::RBM]) {
   using Sh = S<NC, NG, NGF>;
-  constexpr int NM = Sh::NM, NMk = Sh::NMk, KM = Sh::KM, NE3 = Sh::NE3, RBE = Sh::RBE;
-  const int l = lane(), g = l >> 4, col = l & 15, e = e0 + col;
+  constexpr int NM = Sh::NM, NE3 = Sh::NE3, KM = Sh::KM, KE = Sh::KE, RBM = Sh::RBM;
+  constexpr Pack64 P = Sh::P64;
+  const int l = lane(), g = l >> 4, col = l & 15, e0 = eb * 16, e = e0 + col;
   const bool valid = e < A.mesh.ne;
-  const double *geo = A.mesh.geo64 + (size_t)(valid ? e : 0) * GEO;
-  const double hh0 = A.fld.hh0d;
+  const double *gg = A.mesh.geo64 + (size_t)(valid ? e : 0) * GEO;
+  const double ax = gg[0], bx = gg[1], ay = gg[2], by = gg[3], di = valid ? gg[4] : 0.0;
+  const double g0 = A.fld.g0, hh0 = A.fld.hh0;
+  const size_t mb = (size_t)e0 * NM;
+  // (a select, not an index: a runtime index into the kernel argument would
+  // copy the whole argument block to scratch)
+  const double *trL = (A.stage > 0 && (A.stage & 1)) ? A.st.trL[1] : A.st.trL[0];
+  {  // u, v: -g grad(eta), volume and face terms
+    double bh[KM], fx[KE], fy[KE];
+    bload<NM>(A.st.x64[2] + mb, g, col, valid, bh);
+#pragma unroll
+    for (int s = 0; s < KM; s++) bh[s] *= -g0;
+#pragma unroll
+    for (int s = 0; s < KE; s++) {
+      int kj = 4 * s + g;
+      fx[s] = fy[s] = 0.0;
+      if (kj < NE3 && valid) {
+        int k = kj / NGF, j = kj % NGF;
+        double fh = -g0 * face_avg<NGF, 2>(trL, A.mesh, e, k, j, 0);
+        fx[s] = 0.5 * fh * gg[11 + k] * gg[5 + k];
+        fy[s] = 0.5 * fh * gg[11 + k] * gg[8 + k];
+      }
+    }
+#pragma unroll
+    for (int rb = 0; rb < RBM; rb++) {
+      double adx[KM], ady[KM], al[KE];
+      afrag64<NM>(O + P.dx, rb, adx);
+      afrag64<NM>(O + P.dy, rb, ady);
+      afrag64<NE3>(O + P.lf, rb, al);
+      doublex4 Ax = chain64(adx, bh, zero64()), Ay = chain64(ady, bh, zero64());
+      doublex4 Lx = chain64(al, fx, zero64()), Ly = chain64(al, fy, zero64());
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        G[0][rb][r] = di * (-by * Ax[r] + ay * Ay[r] + Lx[r]);
+        G[1][rb][r] = di * (bx * Ax[r] - ax * Ay[r] + Ly[r]);
+      }
+    }
+  }
+  phase_fence();
+  {  // eta: -div(hh0 u); the nonlinear variants project the flux on walls
+    double bu[KM], bv[KM], fh[KE];
+    bload<NM>(A.st.x64[0] + mb, g, col, valid, bu);
+    bload<NM>(A.st.x64[1] + mb, g, col, valid, bv);
+#pragma unroll
+    for (int s = 0; s < KM; s++) {
+      bu[s] *= hh0;
+      bv[s] *= hh0;
+    }
+    if (VAR != 0) {
+      const bool wall = valid && A.mesh.bdl[e] != 0;
+      if (__ballot(wall)) wall_pair64<NM, VAR == 2>(A.ops.gl64, wall, gg[14], gg[15], bu, bv);
+    }
+#pragma unroll
+    for (int s = 0; s < KM; s++) {
+      double u = bu[s], v = bv[s];
+      bu[s] = by * u - bx * v;
+      bv[s] = -ay * u + ax * v;
+    }
+#pragma unroll
+    for (int s = 0; s < KE; s++) {
+      int kj = 4 * s + g;
+      fh[s] = 0.0;
+      if (kj < NE3 && valid) {
+        int k = kj / NGF, j = kj % NGF;
+        fh[s] = -0.5 * gg[11 + k] * face_avg<NGF, 2>(trL, A.mesh, e, k, j, 1);
+      }
+    }
+#pragma unroll
+    for (int rb = 0; rb < RBM; rb++) {
+      double adx[KM], ady[KM], al[KE];
+      afrag64<NM>(O + P.dx, rb, adx);
+      afrag64<NM>(O + P.dy, rb, ady);
+      afrag64<NE3>(O + P.lf, rb, al);
+      doublex4 Gh = chain64(adx, bu, zero64());
+      Gh = chain64(ady, bv, Gh);
+      Gh = chain64(al, fh, Gh);
+#pragma unroll
+      for (int r = 0; r < 4; r++) G[2][rb][r] = di * Gh[r];
+    }
+  }
+}
+
+// Traces of a new stage input (B fragments, FP64), written to trL[w]:
+// T_eta (field 0) ...
+template <int NC, int NG, int NGF>
+__device__ __forceinline__ void traces_h(const F32Args &A, const double *O, int eb, int w,
+                                         const double (&xh)[S<NC, NG, NGF>::KM]) {
+  using Sh = S<NC, NG, NGF>;
+  constexpr int NM = Sh::NM, NE3 = Sh::NE3, KM = Sh::KM, RBE = Sh::RBE;
+  const int l = lane(), g = l >> 4, col = l & 15;
+  const bool valid = eb * 16 + col < A.mesh.ne;
 #pragma unroll
   for (int rb = 0; rb < RBE; rb++) {
     double at[KM];
-    afrag64<KM>(A.ops.TR64, NMk, rb, at);
-    doublex4 Tu = chain64<KM>(at, X, zero4d()), Tv = chain64<KM>(at, X + NMk * 16, zero4d());
-    doublex4 Th = chain64<KM>(at, X + 2 * NMk * 16, zero4d());
+    afrag64<NM>(O + Sh::P64.tr, rb, at);
+    doublex4 Th = chain64(at, xh, zero64());
     if (valid) {
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        int row = drow64(rb, g, r);
-        if (row < NE3) {
-          int k = row / NGF, j = row % NGF;
-          float *ta = A.st.trA + ((size_t)e * 3 + k) * 2 * NGF;
-          ta[j] = (float)Tu[r];
-          ta[NGF + j] = (float)Tv[r];
-          double *tl = A.st.trL + ((size_t)e * 3 + k) * 2 * NGF;
-          tl[j] = Th[r];
-          tl[NGF + j] = hh0 * (Tu[r] * geo[5 + k] + Tv[r] * geo[8 + k]);
+        int kj = drow(rb, r, g);
+        if (kj < NE3) tr_put<NE3, 2>(w ? A.st.trL[1] : A.st.trL[0], eb, col, 0, kj, Th[r]);
+      }
+    }
+  }
+}
+
+// ... and S = hh0 (T_u, T_v).n_face (field 1), with the mass flux of wall
+// elements projected first as the core projects it; trA <- (T_u, T_v) in
+// FP32 for the nonlinear gradient.
+template <int NC, int NG, int NGF, int VAR>
+__device__ __forceinline__ void traces_uv(const F32Args &A, const double *O, int eb, int w,
+                                          const double (&xu)[S<NC, NG, NGF>::KM],
+                                          const double (&xv)[S<NC, NG, NGF>::KM]) {
+  using Sh = S<NC, NG, NGF>;
+  constexpr int NM = Sh::NM, NE3 = Sh::NE3, KM = Sh::KM, RBE = Sh::RBE;
+  const int l = lane(), g = l >> 4, col = l & 15, e = eb * 16 + col;
+  const bool valid = e < A.mesh.ne;
+  const double *gg = A.mesh.geo64 + (size_t)(valid ? e : 0) * GEO;
+  const double hh0 = A.fld.hh0;
+  double *trL = w ? A.st.trL[1] : A.st.trL[0];
+#pragma unroll
+  for (int rb = 0; rb < RBE; rb++) {
+    double at[KM];
+    afrag64<NM>(O + Sh::P64.tr, rb, at);
+    doublex4 Tu = chain64(at, xu, zero64()), Tv = chain64(at, xv, zero64());
+    if (valid) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        int kj = drow(rb, r, g);
+        if (kj < NE3) {
+          int k = kj / NGF;
+          if (VAR != 0) {
+            tr_put<NE3, 2>(A.st.trA, eb, col, 0, kj, (float)Tu[r]);
+            tr_put<NE3, 2>(A.st.trA, eb, col, 1, kj, (float)Tv[r]);
+          }
+          tr_put<NE3, 2>(trL, eb, col, 1, kj, hh0 * (Tu[r] * gg[5 + k] + Tv[r] * gg[8 + k]));
         }
       }
     }
   }
-  // wall elements: the mass flux is projected before its trace is taken
-  unsigned long long wm = __ballot(valid && g == 0 && A.mesh.bdl[e] != 0);
-  if (!wm) return;
-  wave_sync();
-  while (wm) {
-    int c = __ffsll((long long)wm) - 1;
-    wm &= wm - 1;
-    const double *gw = A.mesh.geo64 + (size_t)(e0 + c) * GEO;
-    wall_project64<NM>(A.ops.GL64, X + c, X + NMk * 16 + c, gw[14], gw[15], wallmode, V);
-    if (l < NE3) {
-      int k = l / NGF, j = l % NGF;
-      double tu = 0.0, tv = 0.0;
-      for (int m = 0; m < NM; m++) {
-        double t = A.ops.TR64[(size_t)l * NMk + m];
-        tu += t * X[16 * m + c];
-        tv += t * X[NMk * 16 + 16 * m + c];
+  if (VAR == 0) return;
+  const bool wall = valid && A.mesh.bdl[e] != 0;
+  if (!__ballot(wall)) return;
+  // wall elements: add the traces of (projected flux - flux) / hh0 to the
+  // values this lane stored above
+  double du[KM], dv[KM];
+#pragma unroll
+  for (int s = 0; s < KM; s++) {
+    du[s] = xu[s];
+    dv[s] = xv[s];
+  }
+  wall_pair64<NM, VAR == 2>(A.ops.gl64, wall, gg[14], gg[15], du, dv);
+#pragma unroll
+  for (int s = 0; s < KM; s++) {
+    du[s] -= xu[s];
+    dv[s] -= xv[s];
+  }
+#pragma unroll
+  for (int rb = 0; rb < RBE; rb++) {
+    double at[KM];
+    afrag64<NM>(O + Sh::P64.tr, rb, at);
+    doublex4 Cu = chain64(at, du, zero64()), Cv = chain64(at, dv, zero64());
+    if (wall) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        int kj = drow(rb, r, g);
+        if (kj < NE3) {
+          int k = kj / NGF;
+          double *t = trL + ((size_t)eb * (2 * NE3) + NE3 + kj) * 16 + col;
+          *t += hh0 * (Cu[r] * gg[5 + k] + Cv[r] * gg[8 + k]);
+        }
       }
-      A.st.trL[((size_t)(e0 + c) * 3 + k) * 2 * NGF + NGF + j] = hh0 * (tu * gw[5 + k] + tv * gw[8 + k]);
     }
-    wave_sync();
   }
 }
 
-template <int NGF>
-__device__ __forceinline__ double face_avg64(const double *__restrict__ tr, const F32Mesh &M, int e,
-                                             int k, int j, int f) {
-  int nb = M.nbr[e * 3 + k];
-  if (nb < 0) return tr[((size_t)e * 3 + 0) * 2 * NGF + f * NGF + j];
-  int kn = M.nbe[e * 3 + k];
-  double a = tr[((size_t)e * 3 + k) * 2 * NGF + f * NGF + j];
-  double b = tr[((size_t)nb * 3 + kn) * 2 * NGF + f * NGF + (NGF - 1 - j)];
-  return 0.5 * (a + b);
-}
-
-// Copy NF blocked FP32 fields [eb][NM][16] of block e0 into LDS [NMk][16].
-template <int NM, int NMk>
-__device__ __forceinline__ void load_block(float *dst, const float *__restrict__ src, int e0) {
-  int l = lane();
-  const float *s = src + (size_t)e0 * NM;
-  for (int i = l; i < NMk * 16; i += 64) dst[i] = i < NM * 16 ? s[i] : 0.f;
-}
-
-// Average of own and neighbour edge traces for element e, local edge k,
-// point j (local orientation), field f of NF stored per edge.  Walls use
-// the element's own trace on its edge 1, like the reference's wall faces
-// (src/calflu.f:31-39).
-template <int NGF, int NF>
-__device__ __forceinline__ float face_avg(const float *__restrict__ tr, const F32Mesh &M, int e,
-                                          int k, int j, int f) {
-  int nb = M.nbr[e * 3 + k];
-  if (nb < 0) return tr[((size_t)e * 3 + 0) * NF * NGF + f * NGF + j];
-  int kn = M.nbe[e * 3 + k];
-  float a = tr[((size_t)e * 3 + k) * NF * NGF + f * NGF + j];
-  float b = tr[((size_t)nb * 3 + kn) * NF * NGF + f * NGF + (NGF - 1 - j)];
-  return 0.5f * (a + b);
-}
-
-// RK4 stage update of one value in FP64 (src/shwwnl.f:242-284) and the FP32
-// shadow of the next stage input.  Returns that shadow value.
-__device__ __forceinline__ float rk_stage(const F32Args &A, int f, size_t idx, float kf,
-                                          double *x64 = nullptr) {
-  double k = (double)kf;
-  if (A.hybrid) k += A.st.kL[f][idx];
-  int s = A.stage;
-  if (s < 0) {
-    A.st.k[f][idx] = k;
-    return 0.f;
-  }
+// RK4 stage update of one value (src/shwwnl.f:242-284) in FP64; stores and
+// returns the next stage input.
+__device__ __forceinline__ double rk_value(const F32Args &A, int f, size_t idx, double k) {
   double *c0 = A.st.c0[f] + idx, *acc = A.st.acc[f] + idx;
   double x;
-  if (s == 0) {
-    *acc = k;
-    x = *c0 + A.dt * k * 0.5;
-  } else if (s == 1) {
-    *acc = *acc + 2.0 * k;
-    x = *c0 + A.dt * k * 0.5;
-  } else if (s == 2) {
-    *acc = *acc + 2.0 * k;
-    x = *c0 + A.dt * k;
-  } else {
-    x = *c0 + (A.dt / 6.0) * (*acc + k);
-    *c0 = x;
+  switch (A.stage) {
+    case 0:
+      *acc = k;
+      x = *c0 + A.dt * k * 0.5;
+      break;
+    case 1:
+      *acc = *acc + 2.0 * k;
+      x = *c0 + A.dt * k * 0.5;
+      break;
+    case 2:
+      *acc = *acc + 2.0 * k;
+      x = *c0 + A.dt * k;
+      break;
+    default:
+      x = *c0 + (A.dt / 6.0) * (*acc + k);
+      *c0 = x;
   }
-  float xf = (float)x;
-  A.st.xs[f][idx] = xf;
-  if (A.hybrid) A.st.x64[f][idx] = x;
-  if (x64) *x64 = x;
-  return xf;
+  A.st.x64[f][idx] = x;
+  return x;
 }
 
-// Epilogue shared by the element kernels: wall projection of (gx, gy), RK
-// update, traces of the next stage input.  G[f][rb] hold the 3 tendencies in
-// D layout; W is 3*NMk*16 floats of this wave's LDS, V 128 floats.
-template <int NC, int NG, int NGF>
-__device__ void finish(const F32Args &A, floatx4 (&G)[3][S<NC, NG, NGF>::RBM], float *W,
-                       float *V, int e0, int wallmode) {
+// Tendency of field f (B fragments) -> k (stage < 0) or RK4 stage in place.
+template <int NM, int KM>
+__device__ __forceinline__ void update(const F32Args &A, int f, int eb, double (&x)[KM]) {
+  const int g = lane() >> 4, col = lane() & 15;
+  const bool valid = eb * 16 + col < A.mesh.ne;
+  const size_t mb = (size_t)eb * 16 * NM;
+#pragma unroll
+  for (int s = 0; s < KM; s++) {
+    const int m = 4 * s + g;
+    double v = 0.0;
+    if (valid && m < NM) {
+      size_t idx = mb + m * 16 + col;
+      if (A.stage < 0) A.st.k[f][idx] = x[s];
+      else v = rk_value(A, f, idx, x[s]);
+    }
+    x[s] = v;
+  }
+}
+
+// Epilogue of the element kernels: wall projection of the (u, v) tendency
+// (free slip for VAR 0, 1, 3 and no slip for VAR 2, src/calvar_lin.f and
+// src/calvarnl_*.f at the end of cal_var), then either the tendencies
+// (stage < 0) or the RK4 stage and the traces of the next stage input.
+template <int NC, int NG, int NGF, int VAR>
+__device__ __forceinline__ void finish64(const F32Args &A, const double *O, int eb,
+                                         doublex4 (&G)[3][S<NC, NG, NGF>::RBM]) {
   using Sh = S<NC, NG, NGF>;
-  constexpr int NM = Sh::NM, NMk = Sh::NMk, RBM = Sh::RBM;
-  const int l = lane(), g = l >> 4, col = l & 15, e = e0 + col;
+  constexpr int NM = Sh::NM, KM = Sh::KM;
+  const int col = lane() & 15, e = eb * 16 + col;
   const bool valid = e < A.mesh.ne;
-  // wall elements of this block (one ballot bit per column)
-  unsigned long long wm = __ballot(valid && g == 0 && A.mesh.bdl[e] != 0);
-  if (wm) {
+  const int w = (A.stage + 1) & 1;
+  {
+    double xh[KM];
 #pragma unroll
-    for (int f = 0; f < 2; f++)
+    for (int s = 0; s < KM; s++) xh[s] = G[2][s >> 2][s & 3];
+    update<NM>(A, 2, eb, xh);
+    if (A.stage >= 0) traces_h<NC, NG, NGF>(A, O, eb, w, xh);
+  }
+  phase_fence();
+  double xu[KM], xv[KM];
 #pragma unroll
-      for (int rb = 0; rb < RBM; rb++)
+  for (int s = 0; s < KM; s++) {
+    xu[s] = G[0][s >> 2][s & 3];
+    xv[s] = G[1][s >> 2][s & 3];
+  }
+  const bool wall = valid && A.mesh.bdl[e] != 0;
+  if (__ballot(wall)) {
+    const double *gw = A.mesh.geo64 + (size_t)(valid ? e : 0) * GEO;
+    wall_pair64<NM, VAR == 2>(A.ops.gl64, wall, gw[14], gw[15], xu, xv);
+  }
+  update<NM>(A, 0, eb, xu);
+  update<NM>(A, 1, eb, xv);
+  if (A.stage >= 0) traces_uv<NC, NG, NGF, VAR>(A, O, eb, w, xu, xv);
+}
+
+// ---------------------------------------------------------------------------
+// Stage input -> FP64 copy and traces (before the first stage).
+// ---------------------------------------------------------------------------
+template <int NC, int NG, int NGF, int VAR>
+__global__ __launch_bounds__(NT) void prep_kernel(F32Args A, int nblk) {
+  using Sh = S<NC, NG, NGF>;
+  constexpr int NM = Sh::NM, KM = Sh::KM;
+  constexpr Pack64 P = Sh::P64;
+  __shared__ floatx4 lds[P.total / 2];
+  const int b0 = first_block();
+  if (b0 >= nblk) return;
+  stage_lds(lds, reinterpret_cast<const floatx4 *>(A.ops.pack64), P.total / 2);
+  __syncthreads();
+  const int eb = b0 + (threadIdx.x >> 6);
+  if (eb >= nblk) return;
+  const double *O = reinterpret_cast<const double *>(lds);
+  const int l = lane(), g = l >> 4, col = l & 15, e = eb * 16 + col;
+  const bool valid = e < A.mesh.ne;
+  const size_t mb = (size_t)eb * 16 * NM;
+  double x[3][KM];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          int row = rb * 16 + 4 * g + r;
-          if (row < NMk) W[(f * NMk + row) * 16 + col] = G[f][rb][r];
-        }
-    wave_sync();
-    while (wm) {
-      int c = __ffsll((long long)wm) - 1;
-      wm &= wm - 1;
-      const float *gg = A.mesh.geo + (size_t)(e0 + c) * GEO;
-      float nx = gg[14], ny = gg[15];
-      float *gx = W + c, *gy = W + NMk * 16 + c;  // stride 16
-      if (wallmode == 2) {  // no-slip: both components on the reduced basis
-        if (l < NM) {
-          V[l] = gx[l * 16];
-          V[64 + l] = gy[l * 16];
-        }
-        wave_sync();
-        if (l < NM) {
-          float d1 = gl_apply<NM>(A.ops.GL, V, l), d2 = gl_apply<NM>(A.ops.GL, V + 64, l);
-          gx[l * 16] += d1;
-          gy[l * 16] += d2;
-        }
-      } else {  // free slip: normal component on the reduced basis
-        if (l < NM) V[l] = nx * gx[l * 16] + ny * gy[l * 16];
-        wave_sync();
-        if (l < NM) {
-          float d = gl_apply<NM>(A.ops.GL, V, l);
-          gx[l * 16] += nx * d;
-          gy[l * 16] += ny * d;
-        }
+  for (int f = 0; f < 3; f++) {
+    bload<NM>(A.st.c0[f] + mb, g, col, valid, x[f]);
+#pragma unroll
+    for (int s = 0; s < KM; s++) {
+      int m = 4 * s + g;
+      if (valid && m < NM) A.st.x64[f][mb + m * 16 + col] = x[f][s];
+    }
+  }
+  traces_h<NC, NG, NGF>(A, O, eb, 0, x[2]);
+  traces_uv<NC, NG, NGF, VAR>(A, O, eb, 0, x[0], x[1]);
+}
+
+// ---------------------------------------------------------------------------
+// Linear right-hand side (src/calvar_lin.f:7-287) + RK4 stage, all FP64.
+// ---------------------------------------------------------------------------
+template <int NC, int NG, int NGF>
+__global__ __launch_bounds__(NT) void lin_kernel(F32Args A, int nblk) {
+  using Sh = S<NC, NG, NGF>;
+  constexpr Pack64 P = Sh::P64;
+  __shared__ floatx4 lds[P.total / 2];
+  const int b0 = first_block();
+  if (b0 >= nblk) return;
+  stage_lds(lds, reinterpret_cast<const floatx4 *>(A.ops.pack64), P.total / 2);
+  __syncthreads();
+  const int eb = b0 + (threadIdx.x >> 6);
+  if (eb >= nblk) return;
+  const double *O = reinterpret_cast<const double *>(lds);
+  doublex4 G[3][Sh::RBM];
+  core64<NC, NG, NGF, 0>(A, O, eb, G);
+  phase_fence();
+  finish64<NC, NG, NGF, 0>(A, O, eb, G);
+}
+
+// ---------------------------------------------------------------------------
+// Nonlinear stages B + C (src/calvarnl_fs.f:115-332 and the ns/inv
+// variants), FP32, without the linear core's terms: the viscous flux
+// F = mu0 q, the quadrature-point sources (advection, Coriolis, wind stress,
+// bottom friction) and the mass flux u (h + hb - hh0).
+// ---------------------------------------------------------------------------
+template <int NC, int NG, int NGF, int VAR>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(BC_WAVES))) void bc_kernel(F32Args A, int nblk) {
+  using Sh = S<NC, NG, NGF>;
+  constexpr int NM = Sh::NM, NE3 = Sh::NE3, KM = Sh::KM, KE = Sh::KE, KG = Sh::KG,
+                RBM = Sh::RBM, RBE = Sh::RBE, RBG = Sh::RBG;
+  constexpr Pack32 P = Sh::P32;
+  __shared__ floatx4 lds[P.total / 4];
+  const int b0 = first_block();
+  if (b0 >= nblk) return;
+  stage_lds(lds, reinterpret_cast<const floatx4 *>(A.ops.pack32), P.total / 4);
+  __syncthreads();
+  const int eb = b0 + (threadIdx.x >> 6);
+  if (eb >= nblk) return;
+  const float *O = reinterpret_cast<const float *>(lds);
+  const int l = lane(), g = l >> 4, col = l & 15, e0 = eb * 16, e = e0 + col;
+  const bool valid = e < A.mesh.ne;
+  const size_t mb = (size_t)e0 * NM;
+  const float mu0 = A.fld.mu0, mb0 = A.fld.mb0;
+  const float *gg = A.mesh.geo + (size_t)(valid ? e : 0) * GEO;
+  const float ax = gg[0], bx = gg[1], ay = gg[2], by = gg[3], di = valid ? gg[4] : 0.f;
+  const bool wall = valid && A.mesh.bdl[e] != 0;
+  const bool anyw = __ballot(wall) != 0;
+  const float nwx = wall ? gg[14] : 0.f, nwy = wall ? gg[15] : 0.f;
+
+  float bu[KM], bv[KM], bh[KM];
+  bload<NM>(A.st.x64[0] + mb, g, col, valid, bu);
+  bload<NM>(A.st.x64[1] + mb, g, col, valid, bv);
+  bload<NM>(A.st.x64[2] + mb, g, col, valid, bh);
+
+  // B: q = weak gradient of (u, v) (src/calvarnl_fs.f:129-158); the face
+  // terms use the averaged velocity traces times n/2
+  floatx4 Q[4][RBM];  // qxx, qxy, qyx, qyy
+  {
+    float fb[4][KE];
+#pragma unroll
+    for (int s = 0; s < KE; s++) {
+      int kj = 4 * s + g;
+      fb[0][s] = fb[1][s] = fb[2][s] = fb[3][s] = 0.f;
+      if (kj < NE3 && valid) {
+        int k = kj / NGF, j = kj % NGF;
+        float sg = 0.5f * gg[11 + k], nx = sg * gg[5 + k], ny = sg * gg[8 + k];
+        float fu = face_avg<NGF, 2>(A.st.trA, A.mesh, e, k, j, 0);
+        float fv = face_avg<NGF, 2>(A.st.trA, A.mesh, e, k, j, 1);
+        fb[0][s] = fu * nx;
+        fb[1][s] = fu * ny;
+        fb[2][s] = fv * nx;
+        fb[3][s] = fv * ny;
       }
-      wave_sync();
     }
 #pragma unroll
-    for (int f = 0; f < 2; f++)
+    for (int rb = 0; rb < RBM; rb++) {
+      float adx[KM], ady[KM], al[KE];
+      afrag32<NM>(O + P.dx, rb, adx);
+      afrag32<NM>(O + P.dy, rb, ady);
+      afrag32<NE3>(O + P.lf, rb, al);
+      floatx4 Au = chain32(adx, bu, zero32()), Bu = chain32(ady, bu, zero32());
+      floatx4 Av = chain32(adx, bv, zero32()), Bv = chain32(ady, bv, zero32());
+      floatx4 L0 = chain32(al, fb[0], zero32()), L1 = chain32(al, fb[1], zero32());
+      floatx4 L2 = chain32(al, fb[2], zero32()), L3 = chain32(al, fb[3], zero32());
 #pragma unroll
-      for (int rb = 0; rb < RBM; rb++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          int row = rb * 16 + 4 * g + r;
-          if (row < NMk) G[f][rb][r] = W[(f * NMk + row) * 16 + col];
-        }
-    wave_sync();
+      for (int r = 0; r < 4; r++) {
+        Q[0][rb][r] = di * (-by * Au[r] + ay * Bu[r] + L0[r]);
+        Q[1][rb][r] = di * (bx * Au[r] - ax * Bu[r] + L1[r]);
+        Q[2][rb][r] = di * (-by * Av[r] + ay * Bv[r] + L2[r]);
+        Q[3][rb][r] = di * (bx * Av[r] - ax * Bv[r] + L3[r]);
+      }
+    }
   }
-  // RK update; next stage input into W as [3][NMk][16] (FP64 in the hybrid
-  // path, where W must hold 6*NMk*16 floats)
-  double *W64 = reinterpret_cast<double *>(W);
+  if (VAR == 1 && anyw) {
+    // free-slip wall projection of the tensor (src/calvarnl_fs.f:165-204):
+    // only the t-n component goes through the reduced basis
+    float vb[KM];
 #pragma unroll
-  for (int f = 0; f < 3; f++)
+    for (int s = 0; s < KM; s++) {
+      const int rb = s >> 2, r = s & 3;
+      vb[s] = (Q[0][rb][r] * nwy - Q[2][rb][r] * nwx) * nwx +
+              (Q[1][rb][r] * nwy - Q[3][rb][r] * nwx) * nwy;
+    }
+#pragma unroll
+    for (int rb = 0; rb < RBM; rb++) {
+      float ag[KM];
+      afrag32<NM>(A.ops.gl32, rb, ag);
+      floatx4 d = chain32(ag, vb, zero32());
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        Q[0][rb][r] += nwx * nwy * d[r];
+        Q[2][rb][r] -= nwx * nwx * d[r];
+        Q[1][rb][r] += nwy * nwy * d[r];
+        Q[3][rb][r] -= nwx * nwy * d[r];
+      }
+    }
+  }
+  // D: viscous flux F = mu0 q (the -g eta part belongs to the FP64 core)
+#pragma unroll
+  for (int f = 0; f < 4; f++)
 #pragma unroll
     for (int rb = 0; rb < RBM; rb++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        int row = rb * 16 + 4 * g + r;
-        float xf = 0.f;
-        double x = 0.0;
-        if (row < NM && valid) xf = rk_stage(A, f, (size_t)e0 * NM + row * 16 + col, G[f][rb][r], &x);
-        if (row < NMk) {
-          if (A.hybrid) W64[(f * NMk + row) * 16 + col] = x;
-          else W[(f * NMk + row) * 16 + col] = xf;
-        }
+        int m = drow(rb, r, g);
+        if (valid && m < NM) A.st.F[f][mb + m * 16 + col] = mu0 * Q[f][rb][r];
       }
-  if (A.stage < 0) return;
-  wave_sync();
-  if (A.hybrid) emit_traces64<NC, NG, NGF>(A, W64, reinterpret_cast<double *>(V), e0, wallmode);
-  else emit_traces<NC, NG, NGF>(A, W, e0, A.variant == 0);
-}
-
-// ---------------------------------------------------------------------------
-// Linear RHS (src/calvar_lin.f:7-287) + RK4 stage.
-// ---------------------------------------------------------------------------
-template <int NC, int NG, int NGF>
-__global__ __launch_bounds__(NT) void lin_kernel(F32Args A) {
-  using Sh = S<NC, NG, NGF>;
-  constexpr int NM = Sh::NM, NMk = Sh::NMk, NEk = Sh::NEk, KM = Sh::KM, KE = Sh::KE,
-                NE3 = Sh::NE3, RBM = Sh::RBM;
-  constexpr int WSZ = 16 * 3 * NMk + 128;
-  __shared__ float lds[NW * WSZ];
-  const int w = threadIdx.x >> 6, l = lane(), g = l >> 4, col = l & 15;
-  float *X = lds + w * WSZ, *V = X + 3 * NMk * 16;
-  const int ne = A.mesh.ne, e0 = (blockIdx.x * NW + w) * 16;
-  if (e0 >= ne) return;  // whole wave idle: no block-level barriers follow
-  const int e = e0 + col;
-  const bool valid = e < ne;
-  const float g0 = A.fld.g0, H = A.fld.hh0;
-  // volume operands: -g h, H (by u - bx v), H (-ay u + ax v)
-  for (int i = l; i < NMk * 16; i += 64) {
-    int m = i >> 4, c = i & 15, ee = e0 + c;
-    float xf = 0.f, xu = 0.f, xv = 0.f;
-    if (m < NM && ee < ne) {
-      const float *gg = A.mesh.geo + (size_t)ee * GEO;
-      size_t o = (size_t)e0 * NM + i;
-      float u = A.st.xs[0][o], v = A.st.xs[1][o], h = A.st.xs[2][o];
-      xf = -g0 * h;
-      xu = H * (gg[3] * u - gg[1] * v);
-      xv = H * (-gg[2] * u + gg[0] * v);
-    }
-    X[i] = xf;
-    X[NMk * 16 + i] = xu;
-    X[2 * NMk * 16 + i] = xv;
-  }
-  const float *gg = A.mesh.geo + (size_t)(valid ? e : 0) * GEO;
-  float ax = gg[0], bx = gg[1], ay = gg[2], by = gg[3], di = valid ? gg[4] : 0.f;
-  // edge operands as B fragments in registers: row kj = 4s + l/16 of this
-  // lane's element (the factor 1/2 is the edge Jacobian of tri_f)
-  float fbx[KE], fby[KE], fbh[KE];
-#pragma unroll
-  for (int s = 0; s < KE; s++) {
-    int kj = 4 * s + g;
-    fbx[s] = fby[s] = fbh[s] = 0.f;
-    if (kj < NE3 && valid) {
-      int k = kj / NGF, j = kj % NGF;
-      float sg = gg[11 + k];
-      float fh = -g0 * face_avg<NGF, 2>(A.st.trA, A.mesh, e, k, j, 0);
-      float fs = face_avg<NGF, 2>(A.st.trA, A.mesh, e, k, j, 1);
-      fbx[s] = 0.5f * fh * sg * gg[5 + k];
-      fby[s] = 0.5f * fh * sg * gg[8 + k];
-      fbh[s] = -0.5f * H * sg * fs;
-    }
-  }
-  wave_sync();
-  floatx4 G[3][RBM];
-#pragma unroll
-  for (int rb = 0; rb < RBM; rb++) {
-    Frag<KM> adx, ady;
-    Frag<KE> al;
-    afrag<KM>(A.ops.DX, A.ops.DXl, NMk, rb, 0, adx);
-    afrag<KM>(A.ops.DY, A.ops.DYl, NMk, rb, 0, ady);
-    afrag<KE>(A.ops.LF, A.ops.LFl, NEk, rb, 0, al);
-    floatx4 Ax = chain<KM>(adx, X, zero4()), Ay = chain<KM>(ady, X, zero4());
-    floatx4 Lx = chain_r<KE>(al, fbx, zero4()), Ly = chain_r<KE>(al, fby, zero4());
-    floatx4 Gh = chain<KM>(adx, X + NMk * 16, zero4());
-    Gh = chain<KM>(ady, X + 2 * NMk * 16, Gh);
-    Gh = chain_r<KE>(al, fbh, Gh);
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      G[0][rb][r] = di * (-by * Ax[r] + ay * Ay[r] + Lx[r]);
-      G[1][rb][r] = di * (bx * Ax[r] - ax * Ay[r] + Ly[r]);
-      G[2][rb][r] = di * Gh[r];
-    }
-  }
-  wave_sync();
-  finish<NC, NG, NGF>(A, G, X, V, e0, 1);
-}
-
-// ---------------------------------------------------------------------------
-// Nonlinear stages B + C (src/calvarnl_fs.f:115-332 and the ns/inv variants).
-// ---------------------------------------------------------------------------
-template <int NC, int NG, int NGF, int VAR>
-__global__ __launch_bounds__(NT) void bc_kernel(F32Args A) {
-  using Sh = S<NC, NG, NGF>;
-  constexpr int NM = Sh::NM, NMk = Sh::NMk, NEk = Sh::NEk, KM = Sh::KM, KE = Sh::KE,
-                NE3 = Sh::NE3, RBM = Sh::RBM, RBG = Sh::RBG, RBE = Sh::RBE, NGr = Sh::NGr;
-  constexpr int WSZ = 16 * (7 * NMk + 64) + 128;
-  __shared__ float lds[NW * WSZ];
-  const int w = threadIdx.x >> 6, l = lane(), g = l >> 4, col = l & 15;
-  float *X = lds + w * WSZ;        // u, v, h          [3][NMk][16]
-  float *Q = X + 3 * NMk * 16;     // qxx qxy qyx qyy  [4][NMk][16]
-  float *P = Q + 4 * NMk * 16;     // products         [4][16][16]
-  float *V = P + 4 * 256;          // wall scratch     [128]
-  const int ne = A.mesh.ne, e0 = (blockIdx.x * NW + w) * 16;
-  if (e0 >= ne) return;
-  const int e = e0 + col;
-  const bool valid = e < ne;
-  const float mu0 = A.fld.mu0, g0 = A.fld.g0, mb0 = A.fld.mb0;
-  for (int f = 0; f < 3; f++) load_block<NM, NMk>(X + f * NMk * 16, A.st.xs[f], e0);
-  const float *gg = A.mesh.geo + (size_t)(valid ? e : 0) * GEO;
-  const float ax = gg[0], bx = gg[1], ay = gg[2], by = gg[3], di = valid ? gg[4] : 0.f;
-  // velocity traces averaged over each face, times the outward normal / 2,
-  // as B fragments in registers (row kj = 4s + l/16 of this lane's element)
-  float fb[4][KE];
-#pragma unroll
-  for (int s = 0; s < KE; s++) {
-    int kj = 4 * s + g;
-    fb[0][s] = fb[1][s] = fb[2][s] = fb[3][s] = 0.f;
-    if (kj < NE3 && valid) {
-      int k = kj / NGF, j = kj % NGF;
-      float sg = 0.5f * gg[11 + k], nx = sg * gg[5 + k], ny = sg * gg[8 + k];
-      float fu = face_avg<NGF, 2>(A.st.trA, A.mesh, e, k, j, 0);
-      float fv = face_avg<NGF, 2>(A.st.trA, A.mesh, e, k, j, 1);
-      fb[0][s] = fu * nx;
-      fb[1][s] = fu * ny;
-      fb[2][s] = fv * nx;
-      fb[3][s] = fv * ny;
-    }
-  }
-  wave_sync();
-  // B: q = weak gradient (src/calvarnl_fs.f:129-158); mass matrix = identity
-#pragma unroll
-  for (int rb = 0; rb < RBM; rb++) {
-    Frag<KM> adx, ady;
-    Frag<KE> al;
-    afrag<KM>(A.ops.DX, A.ops.DXl, NMk, rb, 0, adx);
-    afrag<KM>(A.ops.DY, A.ops.DYl, NMk, rb, 0, ady);
-    afrag<KE>(A.ops.LF, A.ops.LFl, NEk, rb, 0, al);
-#pragma unroll
-    for (int f = 0; f < 2; f++) {
-      // gx1 = -agx^T u, gy1 = -agy^T u
-      floatx4 Ax = chain<KM>(adx, X + f * NMk * 16, zero4());
-      floatx4 Ay = chain<KM>(ady, X + f * NMk * 16, zero4());
-      floatx4 La = chain_r<KE>(al, fb[2 * f], zero4());
-      floatx4 Lb = chain_r<KE>(al, fb[2 * f + 1], zero4());
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        int row = rb * 16 + 4 * g + r;
-        if (row < NMk) {
-          Q[((2 * f) * NMk + row) * 16 + col] = di * (-by * Ax[r] + ay * Ay[r] + La[r]);
-          Q[((2 * f + 1) * NMk + row) * 16 + col] = di * (bx * Ax[r] - ax * Ay[r] + Lb[r]);
-        }
-      }
-    }
-  }
-  wave_sync();
-  if (VAR == 1) {
-    // free-slip wall projection of the tensor (src/calvarnl_fs.f:165-204):
-    // only the t-n component goes through the reduced basis
-    unsigned long long wm = __ballot(valid && g == 0 && A.mesh.bdl[e] != 0);
-    while (wm) {
-      int c = __ffsll((long long)wm) - 1;
-      wm &= wm - 1;
-      const float *gw = A.mesh.geo + (size_t)(e0 + c) * GEO;
-      float nx = gw[14], ny = gw[15];
-      float *qxx = Q + c, *qxy = Q + NMk * 16 + c, *qyx = Q + 2 * NMk * 16 + c,
-            *qyy = Q + 3 * NMk * 16 + c;
-      if (l < NM)
-        V[l] = (qxx[16 * l] * ny - qyx[16 * l] * nx) * nx + (qxy[16 * l] * ny - qyy[16 * l] * nx) * ny;
-      wave_sync();
-      if (l < NM) {
-        float d = gl_apply<NM>(A.ops.GL, V, l);
-        qxx[16 * l] += nx * ny * d;
-        qyx[16 * l] -= nx * nx * d;
-        qxy[16 * l] += ny * ny * d;
-        qyy[16 * l] -= nx * ny * d;
-      }
-      wave_sync();
-    }
-  }
-  // D: F = mu0 q - g0 h, and its edge traces combined with the face normal
-  {
-    size_t o = (size_t)e0 * NM;
-    for (int i = l; i < NM * 16; i += 64) {
-      // the -g0 h part belongs to the FP64 linear core in the hybrid path
-      float gh = A.hybrid ? 0.f : g0 * X[2 * NMk * 16 + i];
-      A.st.F[0][o + i] = mu0 * Q[i] - gh;
-      A.st.F[1][o + i] = mu0 * Q[NMk * 16 + i];
-      A.st.F[2][o + i] = mu0 * Q[2 * NMk * 16 + i];
-      A.st.F[3][o + i] = mu0 * Q[3 * NMk * 16 + i] - gh;
-    }
-  }
+  float q0[KM], q1[KM], q2[KM], q3[KM];
+  d2b(Q[0], q0);
+  d2b(Q[1], q1);
+  d2b(Q[2], q2);
+  d2b(Q[3], q3);
 #pragma unroll
   for (int rb = 0; rb < RBE; rb++) {
-    Frag<KM> at;
-    afrag<KM>(A.ops.TR, A.ops.TRl, NMk, rb, 0, at);
-    floatx4 T0 = chain<KM>(at, Q, zero4()), T1 = chain<KM>(at, Q + NMk * 16, zero4());
-    floatx4 T2 = chain<KM>(at, Q + 2 * NMk * 16, zero4());
-    floatx4 T3 = chain<KM>(at, Q + 3 * NMk * 16, zero4());
-    floatx4 Th = chain<KM>(at, X + 2 * NMk * 16, zero4());
+    float at[KM];
+    afrag32<NM>(O + P.tr, rb, at);
+    floatx4 T0 = chain32(at, q0, zero32()), T1 = chain32(at, q1, zero32());
+    floatx4 T2 = chain32(at, q2, zero32()), T3 = chain32(at, q3, zero32());
     if (valid) {
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        int row = rb * 16 + 4 * g + r;
-        if (row < NE3) {
-          int k = row / NGF, j = row % NGF;
+        int kj = drow(rb, r, g);
+        if (kj < NE3) {
+          int k = kj / NGF;
           float nx = gg[5 + k], ny = gg[8 + k];
-          float *t = A.st.trS + ((size_t)e * 3 + k) * 3 * NGF;
-          float gth = A.hybrid ? 0.f : g0 * Th[r];
-          t[j] = (mu0 * T0[r] - gth) * nx + mu0 * T1[r] * ny;
-          t[NGF + j] = mu0 * T2[r] * nx + (mu0 * T3[r] - gth) * ny;
+          tr_put<NE3, 3>(A.st.trS, eb, col, 0, kj, mu0 * (T0[r] * nx + T1[r] * ny));
+          tr_put<NE3, 3>(A.st.trS, eb, col, 1, kj, mu0 * (T2[r] * nx + T3[r] * ny));
         }
       }
     }
   }
   // C: values at the quadrature points, products, projection
-  // (src/calvarnl_fs.f:245-296, src/nonlin.f:35-133)
+  // (src/calvarnl_fs.f:245-296, src/nonlin.f:35-133).  A product at point
+  // q = 16 qb + 4 r + g is already the B operand of k-step 4 qb + r of the
+  // projection.
   floatx4 Pr[4][RBM];
 #pragma unroll
   for (int f = 0; f < 4; f++)
 #pragma unroll
-    for (int rb = 0; rb < RBM; rb++) Pr[f][rb] = zero4();
+    for (int rb = 0; rb < RBM; rb++) Pr[f][rb] = zero32();
   const float *fc3 = A.fld.fc3 + (size_t)(valid ? e : 0) * 3;
   const float fc0 = fc3[0], fc1 = fc3[1], fc2 = fc3[2];
+#pragma unroll
   for (int qb = 0; qb < RBG; qb++) {
-    Frag<KM> ap;
-    afrag<KM>(A.ops.PSI, A.ops.PSIl, NMk, qb, 0, ap);
-    floatx4 Vu = chain<KM>(ap, X, zero4()), Vv = chain<KM>(ap, X + NMk * 16, zero4());
-    floatx4 Vh = chain<KM>(ap, X + 2 * NMk * 16, zero4());
-    floatx4 Vxx = chain<KM>(ap, Q, zero4()), Vxy = chain<KM>(ap, Q + NMk * 16, zero4());
-    floatx4 Vyx = chain<KM>(ap, Q + 2 * NMk * 16, zero4());
-    floatx4 Vyy = chain<KM>(ap, Q + 3 * NMk * 16, zero4());
+    float ap[KM];
+    afrag32<NM>(O + P.psi, qb, ap);
+    floatx4 Vu = chain32(ap, bu, zero32()), Vv = chain32(ap, bv, zero32());
+    floatx4 Vh = chain32(ap, bh, zero32());
+    floatx4 Vxx = chain32(ap, q0, zero32()), Vxy = chain32(ap, q1, zero32());
+    floatx4 Vyx = chain32(ap, q2, zero32()), Vyy = chain32(ap, q3, zero32());
+    float pg[4][4];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      int q = qb * 16 + 4 * g + r;
-      float gx = 0.f, gy = 0.f, uh = 0.f, vh = 0.f;
+      int q = drow(qb, r, g);
+      pg[0][r] = pg[1][r] = pg[2][r] = pg[3][r] = 0.f;
       if (q < NG && valid) {
-        size_t o = ((size_t)e0 * NG) + (size_t)q * 16 + col;
-        float f = fc0 + fc1 * A.ops.yq[q] + fc2 * A.ops.xq[q];
+        size_t o = (size_t)e0 * NG + (size_t)q * 16 + col;
+        float f = fc0 + fc1 * O[P.yq + q] + fc2 * O[P.xq + q];
         float hk = Vh[r] + A.fld.hbq[o];
         float u = Vu[r], v = Vv[r];
-        gx = -(u * Vxx[r] + v * (Vxy[r] - f)) + (A.fld.w0q[o] - mb0 * u) / hk;
-        gy = -(u * (Vyx[r] + f) + v * Vyy[r]) + (A.fld.z0q[o] - mb0 * v) / hk;
-        // hybrid: the flux hh0 u is the FP64 core's; only u (h + hb - hh0) here
-        float hf = A.hybrid ? Vh[r] + A.fld.hbdq[o] : hk;
-        uh = u * hf;
-        vh = v * hf;
+        pg[0][r] = -(u * Vxx[r] + v * (Vxy[r] - f)) + (A.fld.w0q[o] - mb0 * u) / hk;
+        pg[1][r] = -(u * (Vyx[r] + f) + v * Vyy[r]) + (A.fld.z0q[o] - mb0 * v) / hk;
+        // the flux hh0 u is the FP64 core's; only u (h + hb - hh0) here
+        float hf = Vh[r] + A.fld.hbdq[o];
+        pg[2][r] = u * hf;
+        pg[3][r] = v * hf;
       }
-      int pr = (4 * g + r) * 16 + col;
-      P[pr] = gx;
-      P[256 + pr] = gy;
-      P[512 + pr] = uh;
-      P[768 + pr] = vh;
     }
-    wave_sync();
 #pragma unroll
     for (int rb = 0; rb < RBM; rb++) {
-      Frag<4> aw;
-      afrag<4>(A.ops.PSIW, A.ops.PSIWl, NGr, rb, 4 * qb, aw);
+      floatx4 aw = reinterpret_cast<const floatx4 *>(O + P.psiw)[(rb * RBG + qb) * 64 + l];
 #pragma unroll
-      for (int f = 0; f < 4; f++) Pr[f][rb] = chain<4>(aw, P + 256 * f, Pr[f][rb]);
+      for (int r = 0; r < 4; r++)
+        if (4 * qb + r < KG) {
+#pragma unroll
+          for (int f = 0; f < 4; f++) Pr[f][rb] = mfma32(aw[r], pg[f][r], Pr[f][rb]);
+        }
     }
-    wave_sync();
   }
-  // outputs: sources, mass fluxes (wall-projected), combined flux traces
-  float *UH = X, *VH = X + NMk * 16;
+  // outputs: sources, mass flux (wall-projected), its normal traces
 #pragma unroll
   for (int rb = 0; rb < RBM; rb++)
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      int row = rb * 16 + 4 * g + r;
-      if (row < NMk) {
-        UH[row * 16 + col] = Pr[2][rb][r];
-        VH[row * 16 + col] = Pr[3][rb][r];
-      }
-      if (row < NM && valid) {
-        size_t o = (size_t)e0 * NM + row * 16 + col;
-        A.st.src[0][o] = Pr[0][rb][r];
-        A.st.src[1][o] = Pr[1][rb][r];
+      int m = drow(rb, r, g);
+      if (valid && m < NM) {
+        A.st.src[0][mb + m * 16 + col] = Pr[0][rb][r];
+        A.st.src[1][mb + m * 16 + col] = Pr[1][rb][r];
       }
     }
-  wave_sync();
-  {
-    unsigned long long wm = __ballot(valid && g == 0 && A.mesh.bdl[e] != 0);
-    while (wm) {
-      int c = __ffsll((long long)wm) - 1;
-      wm &= wm - 1;
-      const float *gw = A.mesh.geo + (size_t)(e0 + c) * GEO;
-      float nx = gw[14], ny = gw[15];
-      float *u = UH + c, *v = VH + c;
+  float uh[KM], vh[KM];
+  d2b(Pr[2], uh);
+  d2b(Pr[3], vh);
+  if (anyw) {
+    float a1[KM], a2[KM];
+#pragma unroll
+    for (int s = 0; s < KM; s++) {
       if (VAR == 2) {  // src/calvarnl_ns.f:247-265
-        if (l < NM) {
-          V[l] = u[16 * l];
-          V[64 + l] = v[16 * l];
-        }
-        wave_sync();
-        if (l < NM) {
-          float d1 = gl_apply<NM>(A.ops.GL, V, l), d2 = gl_apply<NM>(A.ops.GL, V + 64, l);
-          u[16 * l] += d1;
-          v[16 * l] += d2;
-        }
+        a1[s] = wall ? uh[s] : 0.f;
+        a2[s] = wall ? vh[s] : 0.f;
       } else {  // src/calvarnl_fs.f:298-315
-        if (l < NM) V[l] = nx * u[16 * l] + ny * v[16 * l];
-        wave_sync();
-        if (l < NM) {
-          float d = gl_apply<NM>(A.ops.GL, V, l);
-          u[16 * l] += nx * d;
-          v[16 * l] += ny * d;
-        }
+        a1[s] = nwx * uh[s] + nwy * vh[s];
+        a2[s] = 0.f;
       }
-      wave_sync();
+    }
+#pragma unroll
+    for (int rb = 0; rb < RBM; rb++) {
+      float ag[KM];
+      afrag32<NM>(A.ops.gl32, rb, ag);
+      floatx4 d1 = chain32(ag, a1, zero32());
+      floatx4 d2 = zero32();
+      if (VAR == 2) d2 = chain32(ag, a2, zero32());
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        if (4 * rb + r < KM) {
+          if (VAR == 2) {
+            uh[4 * rb + r] += d1[r];
+            vh[4 * rb + r] += d2[r];
+          } else {
+            uh[4 * rb + r] += nwx * d1[r];
+            vh[4 * rb + r] += nwy * d1[r];
+          }
+        }
     }
   }
-  {
-    size_t o = (size_t)e0 * NM;
-    for (int i = l; i < NM * 16; i += 64) {
-      A.st.uh[0][o + i] = UH[i];
-      A.st.uh[1][o + i] = VH[i];
+#pragma unroll
+  for (int s = 0; s < KM; s++) {
+    int m = 4 * s + g;
+    if (valid && m < NM) {
+      A.st.uh[0][mb + m * 16 + col] = uh[s];
+      A.st.uh[1][mb + m * 16 + col] = vh[s];
     }
   }
 #pragma unroll
   for (int rb = 0; rb < RBE; rb++) {
-    Frag<KM> at;
-    afrag<KM>(A.ops.TR, A.ops.TRl, NMk, rb, 0, at);
-    floatx4 Tu = chain<KM>(at, UH, zero4()), Tv = chain<KM>(at, VH, zero4());
+    float at[KM];
+    afrag32<NM>(O + P.tr, rb, at);
+    floatx4 Tu = chain32(at, uh, zero32()), Tv = chain32(at, vh, zero32());
     if (valid) {
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        int row = rb * 16 + 4 * g + r;
-        if (row < NE3) {
-          int k = row / NGF, j = row % NGF;
-          A.st.trS[((size_t)e * 3 + k) * 3 * NGF + 2 * NGF + j] = Tu[r] * gg[5 + k] + Tv[r] * gg[8 + k];
+        int kj = drow(rb, r, g);
+        if (kj < NE3) {
+          int k = kj / NGF;
+          tr_put<NE3, 3>(A.st.trS, eb, col, 2, kj, Tu[r] * gg[5 + k] + Tv[r] * gg[8 + k]);
         }
       }
     }
@@ -791,255 +820,100 @@ __global__ __launch_bounds__(NT) void bc_kernel(F32Args A) {
 }
 
 // ---------------------------------------------------------------------------
-// Nonlinear stages E + G (src/calvarnl_fs.f:382-544) + RK4 stage.
+// Nonlinear stages E + G (src/calvarnl_fs.f:382-544): FP32 divergence of the
+// B+C results, plus the FP64 linear core, wall projection, RK4 stage.
 // ---------------------------------------------------------------------------
 template <int NC, int NG, int NGF, int VAR>
-__global__ __launch_bounds__(NT) void eg_kernel(F32Args A) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EG_WAVES))) void eg_kernel(F32Args A, int nblk) {
   using Sh = S<NC, NG, NGF>;
-  constexpr int NM = Sh::NM, NMk = Sh::NMk, NEk = Sh::NEk, KM = Sh::KM, KE = Sh::KE,
-                NE3 = Sh::NE3, RBM = Sh::RBM;
-  constexpr int WSZ = 16 * 6 * NMk + 256;
-  __shared__ __attribute__((aligned(16))) float lds[NW * WSZ];
-  const int w = threadIdx.x >> 6, l = lane(), g = l >> 4, col = l & 15;
-  float *X = lds + w * WSZ, *V = X + 6 * NMk * 16;
-  const int ne = A.mesh.ne, e0 = (blockIdx.x * NW + w) * 16;
-  if (e0 >= ne) return;
-  const int e = e0 + col;
-  const bool valid = e < ne;
-  // volume operands with the Jacobian terms folded in:
-  //   u: -by Fxx + bx Fxy (with agx), ay Fxx - ax Fxy (with agy); v likewise;
-  //   eta: by uh - bx vh (agx), -ay uh + ax vh (agy)
-  for (int i = l; i < NMk * 16; i += 64) {
-    int m = i >> 4, c = i & 15, ee = e0 + c;
-    float o0 = 0.f, o1 = 0.f, o2 = 0.f, o3 = 0.f, o4 = 0.f, o5 = 0.f;
-    if (m < NM && ee < ne) {
-      const float *gg = A.mesh.geo + (size_t)ee * GEO;
-      float ax = gg[0], bx = gg[1], ay = gg[2], by = gg[3];
-      size_t o = (size_t)e0 * NM + i;
-      float fxx = A.st.F[0][o], fxy = A.st.F[1][o], fyx = A.st.F[2][o], fyy = A.st.F[3][o];
-      float uh = A.st.uh[0][o], vh = A.st.uh[1][o];
-      o0 = -by * fxx + bx * fxy;
-      o1 = ay * fxx - ax * fxy;
-      o2 = -by * fyx + bx * fyy;
-      o3 = ay * fyx - ax * fyy;
-      o4 = by * uh - bx * vh;
-      o5 = -ay * uh + ax * vh;
-    }
-    X[i] = o0;
-    X[NMk * 16 + i] = o1;
-    X[2 * NMk * 16 + i] = o2;
-    X[3 * NMk * 16 + i] = o3;
-    X[4 * NMk * 16 + i] = o4;
-    X[5 * NMk * 16 + i] = o5;
-  }
-  // combined flux traces averaged over each face, times its / 2, as B
-  // fragments in registers; the eta equation's lift carries -1/det
-  float fb[3][KE];
-  const float *gg = A.mesh.geo + (size_t)(valid ? e : 0) * GEO;
-#pragma unroll
-  for (int s = 0; s < KE; s++) {
-    int kj = 4 * s + g;
-    fb[0][s] = fb[1][s] = fb[2][s] = 0.f;
-    if (kj < NE3 && valid) {
-      int k = kj / NGF, j = kj % NGF;
-      float sg = 0.5f * gg[11 + k];
-      fb[0][s] = sg * face_avg<NGF, 3>(A.st.trS, A.mesh, e, k, j, 0);
-      fb[1][s] = sg * face_avg<NGF, 3>(A.st.trS, A.mesh, e, k, j, 1);
-      fb[2][s] = -sg * face_avg<NGF, 3>(A.st.trS, A.mesh, e, k, j, 2);
-    }
-  }
-  wave_sync();
-  const float di = valid ? gg[4] : 0.f;
-  floatx4 G[3][RBM];
-#pragma unroll
-  for (int rb = 0; rb < RBM; rb++) {
-    Frag<KM> adx, ady;
-    Frag<KE> al;
-    afrag<KM>(A.ops.DX, A.ops.DXl, NMk, rb, 0, adx);
-    afrag<KM>(A.ops.DY, A.ops.DYl, NMk, rb, 0, ady);
-    afrag<KE>(A.ops.LF, A.ops.LFl, NEk, rb, 0, al);
-#pragma unroll
-    for (int f = 0; f < 3; f++) {
-      floatx4 acc = chain<KM>(adx, X + (2 * f) * NMk * 16, zero4());
-      acc = chain<KM>(ady, X + (2 * f + 1) * NMk * 16, acc);
-      acc = chain_r<KE>(al, fb[f], acc);
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        int row = rb * 16 + 4 * g + r;
-        float src = 0.f;
-        if (f < 2 && row < NM && valid) src = A.st.src[f][(size_t)e0 * NM + row * 16 + col];
-        G[f][rb][r] = src + di * acc[r];
-      }
-    }
-  }
-  wave_sync();
-  finish<NC, NG, NGF>(A, G, X, V, e0, VAR == 2 ? 2 : 1);
-}
+  constexpr int NM = Sh::NM, NE3 = Sh::NE3, KM = Sh::KM, KE = Sh::KE, RBM = Sh::RBM;
+  constexpr Pack32 P = Sh::P32;
+  constexpr Pack64 P64 = Sh::P64;
+  constexpr int N32 = P.eg_end / 4, N64 = P64.total / 2;  // 16-byte words
+  __shared__ floatx4 lds[N32 + N64];
+  const int b0 = first_block();
+  if (b0 >= nblk) return;
+  stage_lds(lds, reinterpret_cast<const floatx4 *>(A.ops.pack32), N32);
+  stage_lds(lds + N32, reinterpret_cast<const floatx4 *>(A.ops.pack64), N64);
+  __syncthreads();
+  const int eb = b0 + (threadIdx.x >> 6);
+  if (eb >= nblk) return;
+  const float *O = reinterpret_cast<const float *>(lds);
+  const double *O64 = reinterpret_cast<const double *>(lds + N32);
+  const int l = lane(), g = l >> 4, col = l & 15, e0 = eb * 16, e = e0 + col;
+  const bool valid = e < A.mesh.ne;
+  const size_t mb = (size_t)e0 * NM;
 
-// Stage input -> FP32 shadow and its traces (before the first stage).
-template <int NC, int NG, int NGF>
-__global__ __launch_bounds__(NT) void prep_kernel(F32Args A) {
-  using Sh = S<NC, NG, NGF>;
-  constexpr int NM = Sh::NM, NMk = Sh::NMk;
-  constexpr int WSZ = 6 * NMk * 16 + 256;
-  __shared__ __attribute__((aligned(16))) float lds[NW * WSZ];
-  const int w = threadIdx.x >> 6, l = lane();
-  float *X = lds + w * WSZ;
-  double *X64 = reinterpret_cast<double *>(X);
-  const int ne = A.mesh.ne, e0 = (blockIdx.x * NW + w) * 16;
-  if (e0 >= ne) return;
-  for (int f = 0; f < 3; f++)
-    for (int i = l; i < NMk * 16; i += 64) {
-      int m = i >> 4, c = i & 15;
-      double v = 0.0;
-      if (m < NM && e0 + c < ne) {
-        size_t o = (size_t)e0 * NM + i;
-        v = A.st.c0[f][o];
-        A.st.xs[f][o] = (float)v;
-        if (A.hybrid) A.st.x64[f][o] = v;
-      }
-      if (A.hybrid) X64[f * NMk * 16 + i] = v;
-      else X[f * NMk * 16 + i] = (float)v;
-    }
-  wave_sync();
-  if (A.hybrid)
-    emit_traces64<NC, NG, NGF>(A, X64, reinterpret_cast<double *>(X + 6 * NMk * 16), e0,
-                               A.variant == 2 ? 2 : 1);
-  else
-    emit_traces<NC, NG, NGF>(A, X, e0, A.variant == 0);
-}
-
-// ---------------------------------------------------------------------------
-// Linear gravity-wave core of the nonlinear variants in FP64 on MFMA
-// (hybrid path): the -g grad(eta) terms of the u, v equations and the
-// -div(hh0 u) term of the eta equation, with the reference's wall
-// projections (src/calvarnl_fs.f:298-321, 436-471; src/calvarnl_ns.f:247-265,
-// 403-421).  In psi coordinates these are exact operators, no quadrature.
-// Writes kL, which the FP32 element kernel adds before the RK update.
-// ---------------------------------------------------------------------------
-template <int NC, int NG, int NGF, int VAR>
-__global__ __launch_bounds__(NT) void core64_kernel(F32Args A) {
-  using Sh = S<NC, NG, NGF>;
-  constexpr int NM = Sh::NM, NMk = Sh::NMk, NEk = Sh::NEk, KM = Sh::KM, KE = Sh::KE,
-                NE3 = Sh::NE3, RBM = Sh::RBM;
-  constexpr int WSZ = 3 * NMk * 16 + 128;
-  __shared__ double lds[NW * WSZ];
-  const int w = threadIdx.x >> 6, l = lane(), g = l >> 4, col = l & 15;
-  double *X = lds + w * WSZ, *V = X + 3 * NMk * 16;
-  const int ne = A.mesh.ne, e0 = (blockIdx.x * NW + w) * 16;
-  if (e0 >= ne) return;
-  const int e = e0 + col;
-  const bool valid = e < ne;
-  const double g0 = A.fld.g0d, hh0 = A.fld.hh0d;
-  const int wallmode = VAR == 2 ? 2 : 1;
-  for (int f = 0; f < 3; f++)
-    for (int i = l; i < NMk * 16; i += 64) {
-      int m = i >> 4, c = i & 15;
-      double v = 0.0;
-      if (m < NM && e0 + c < ne) v = A.st.x64[f][(size_t)e0 * NM + i];
-      X[f * NMk * 16 + i] = f < 2 ? hh0 * v : v;
-    }
-  wave_sync();
-  // mass flux hh0 (u, v), wall-projected
-  unsigned long long wm = __ballot(valid && g == 0 && A.mesh.bdl[e] != 0);
-  {
-    unsigned long long m2 = wm;
-    while (m2) {
-      int c = __ffsll((long long)m2) - 1;
-      m2 &= m2 - 1;
-      const double *gw = A.mesh.geo64 + (size_t)(e0 + c) * GEO;
-      wall_project64<NM>(A.ops.GL64, X + c, X + NMk * 16 + c, gw[14], gw[15], wallmode, V);
-    }
-  }
-  // volume operands: by uL - bx vL (agx), -ay uL + ax vL (agy), -g0 h
-  for (int i = l; i < NMk * 16; i += 64) {
-    int c = i & 15;
-    const double *gc = A.mesh.geo64 + (size_t)(e0 + c < ne ? e0 + c : e0) * GEO;
-    double uL = X[i], vL = X[NMk * 16 + i];
-    X[i] = gc[3] * uL - gc[1] * vL;
-    X[NMk * 16 + i] = -gc[2] * uL + gc[0] * vL;
-    X[2 * NMk * 16 + i] = -g0 * X[2 * NMk * 16 + i];
-  }
-  wave_sync();
-  const double *gg = A.mesh.geo64 + (size_t)(valid ? e : 0) * GEO;
-  const double ax = gg[0], bx = gg[1], ay = gg[2], by = gg[3], di = valid ? gg[4] : 0.0;
-  double fbx[KE], fby[KE], fbh[KE];
-#pragma unroll
-  for (int s = 0; s < KE; s++) {
-    int kj = 4 * s + g;
-    fbx[s] = fby[s] = fbh[s] = 0.0;
-    if (kj < NE3 && valid) {
-      int k = kj / NGF, j = kj % NGF;
-      double sg = gg[11 + k];
-      double fh = -g0 * face_avg64<NGF>(A.st.trL, A.mesh, e, k, j, 0);
-      double fs = face_avg64<NGF>(A.st.trL, A.mesh, e, k, j, 1);
-      fbx[s] = 0.5 * fh * sg * gg[5 + k];
-      fby[s] = 0.5 * fh * sg * gg[8 + k];
-      fbh[s] = -0.5 * sg * fs;
-    }
-  }
   doublex4 G[3][RBM];
+  core64<NC, NG, NGF, VAR>(A, O64, eb, G);
+  phase_fence();
+  {
+    const float *gg = A.mesh.geo + (size_t)(valid ? e : 0) * GEO;
+    const float ax = gg[0], bx = gg[1], ay = gg[2], by = gg[3], di = valid ? gg[4] : 0.f;
+    // volume operands with the Jacobian terms folded in:
+    //   u: -by Fxx + bx Fxy (with Dx), ay Fxx - ax Fxy (with Dy); v likewise;
+    //   eta: by uh - bx vh (Dx), -ay uh + ax vh (Dy)
+    float o[6][KM];
+    {
+      float fxx[KM], fxy[KM], fyx[KM], fyy[KM], uh[KM], vh[KM];
+      bload<NM>(A.st.F[0] + mb, g, col, valid, fxx);
+      bload<NM>(A.st.F[1] + mb, g, col, valid, fxy);
+      bload<NM>(A.st.F[2] + mb, g, col, valid, fyx);
+      bload<NM>(A.st.F[3] + mb, g, col, valid, fyy);
+      bload<NM>(A.st.uh[0] + mb, g, col, valid, uh);
+      bload<NM>(A.st.uh[1] + mb, g, col, valid, vh);
 #pragma unroll
-  for (int rb = 0; rb < RBM; rb++) {
-    double adx[KM], ady[KM], al[KE];
-    afrag64<KM>(A.ops.DX64, NMk, rb, adx);
-    afrag64<KM>(A.ops.DY64, NMk, rb, ady);
-    afrag64<KE>(A.ops.LF64, NEk, rb, al);
-    doublex4 Ax = chain64<KM>(adx, X + 2 * NMk * 16, zero4d());
-    doublex4 Ay = chain64<KM>(ady, X + 2 * NMk * 16, zero4d());
-    doublex4 Lx = chain64_r<KE>(al, fbx, zero4d()), Ly = chain64_r<KE>(al, fby, zero4d());
-    doublex4 Gh = chain64<KM>(adx, X, zero4d());
-    Gh = chain64<KM>(ady, X + NMk * 16, Gh);
-    Gh = chain64_r<KE>(al, fbh, Gh);
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      G[0][rb][r] = di * (-by * Ax[r] + ay * Ay[r] + Lx[r]);
-      G[1][rb][r] = di * (bx * Ax[r] - ax * Ay[r] + Ly[r]);
-      G[2][rb][r] = di * Gh[r];
-    }
-  }
-  if (wm) {
-    wave_sync();
-#pragma unroll
-    for (int f = 0; f < 2; f++)
-#pragma unroll
-      for (int rb = 0; rb < RBM; rb++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          int row = drow64(rb, g, r);
-          if (row < NMk) X[(f * NMk + row) * 16 + col] = G[f][rb][r];
-        }
-    wave_sync();
-    while (wm) {
-      int c = __ffsll((long long)wm) - 1;
-      wm &= wm - 1;
-      const double *gw = A.mesh.geo64 + (size_t)(e0 + c) * GEO;
-      wall_project64<NM>(A.ops.GL64, X + c, X + NMk * 16 + c, gw[14], gw[15], wallmode, V);
-    }
-#pragma unroll
-    for (int f = 0; f < 2; f++)
-#pragma unroll
-      for (int rb = 0; rb < RBM; rb++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          int row = drow64(rb, g, r);
-          if (row < NMk) G[f][rb][r] = X[(f * NMk + row) * 16 + col];
-        }
-  }
-  if (!valid) return;
-#pragma unroll
-  for (int f = 0; f < 3; f++)
-#pragma unroll
-    for (int rb = 0; rb < RBM; rb++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        int row = drow64(rb, g, r);
-        if (row < NM) A.st.kL[f][(size_t)e0 * NM + row * 16 + col] = G[f][rb][r];
+      for (int s = 0; s < KM; s++) {
+        o[0][s] = -by * fxx[s] + bx * fxy[s];
+        o[1][s] = ay * fxx[s] - ax * fxy[s];
+        o[2][s] = -by * fyx[s] + bx * fyy[s];
+        o[3][s] = ay * fyx[s] - ax * fyy[s];
+        o[4][s] = by * uh[s] - bx * vh[s];
+        o[5][s] = -ay * uh[s] + ax * vh[s];
       }
+    }
+    // combined flux traces averaged over each face, times its/2; the eta
+    // equation's lift carries the minus sign
+    float fb[3][KE];
+#pragma unroll
+    for (int s = 0; s < KE; s++) {
+      int kj = 4 * s + g;
+      fb[0][s] = fb[1][s] = fb[2][s] = 0.f;
+      if (kj < NE3 && valid) {
+        int k = kj / NGF, j = kj % NGF;
+        float sg = 0.5f * gg[11 + k];
+        fb[0][s] = sg * face_avg<NGF, 3>(A.st.trS, A.mesh, e, k, j, 0);
+        fb[1][s] = sg * face_avg<NGF, 3>(A.st.trS, A.mesh, e, k, j, 1);
+        fb[2][s] = -sg * face_avg<NGF, 3>(A.st.trS, A.mesh, e, k, j, 2);
+      }
+    }
+#pragma unroll
+    for (int rb = 0; rb < RBM; rb++) {
+      float adx[KM], ady[KM], al[KE];
+      afrag32<NM>(O + P.dx, rb, adx);
+      afrag32<NM>(O + P.dy, rb, ady);
+      afrag32<NE3>(O + P.lf, rb, al);
+#pragma unroll
+      for (int f = 0; f < 3; f++) {
+        floatx4 acc = chain32(adx, o[2 * f], zero32());
+        acc = chain32(ady, o[2 * f + 1], acc);
+        acc = chain32(al, fb[f], acc);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          int m = drow(rb, r, g);
+          float src = 0.f;
+          if (f < 2 && valid && m < NM) src = A.st.src[f][mb + m * 16 + col];
+          G[f][rb][r] += (double)(src + di * acc[r]);
+        }
+      }
+    }
+  }
+  phase_fence();
+  finish64<NC, NG, NGF, VAR>(A, O64, eb, G);
 }
 
 // Basis change, FP64.  to_psi: c = U a from the reference layout a[e*NM + m]
-// into the blocked layout; from_psi: a = U^{-1} c by back substitution.
+// into the blocked layout; from_psi: a = U^-1 c back.
 template <int NM>
 __global__ void to_psi_kernel(int ne, const double *__restrict__ U, const double *__restrict__ a,
                               double *__restrict__ c) {
@@ -1073,11 +947,11 @@ __global__ void from_psi_kernel(int ne, const double *__restrict__ U, const doub
 
 template <int NC, int NG, int NGF>
 int launch(const F32Args &A, int phase, const double *in, double *out) {
-  using Sh = S<NC, NG, NGF>;
-  constexpr int NM = Sh::NM;
-  const int ne = A.mesh.ne;
-  const int nb = (ne + 16 * NW - 1) / (16 * NW);
+  constexpr int NM = nm_of(NC);
+  const int ne = A.mesh.ne, nblk = cdiv(ne, 16);
+  const int grid = 8 * cdiv(cdiv(nblk, NW), 8);  // a multiple of the 8 XCDs
   hipStream_t st = A.stream;
+  const int v = A.variant;
   switch (phase) {
     case 0:  // to psi
       to_psi_kernel<NM><<<(ne * NM + 255) / 256, 256, 0, st>>>(ne, A.ops.U, in, out);
@@ -1085,26 +959,24 @@ int launch(const F32Args &A, int phase, const double *in, double *out) {
     case 1:  // from psi
       from_psi_kernel<NM><<<(ne + 127) / 128, 128, 0, st>>>(ne, A.ops.U, in, out);
       break;
-    case 2:
-      prep_kernel<NC, NG, NGF><<<nb, NT, 0, st>>>(A);
+    case 2:  // stage input copy and traces
+      if (v == 0) prep_kernel<NC, NG, NGF, 0><<<grid, NT, 0, st>>>(A, nblk);
+      else if (v == 1) prep_kernel<NC, NG, NGF, 1><<<grid, NT, 0, st>>>(A, nblk);
+      else if (v == 2) prep_kernel<NC, NG, NGF, 2><<<grid, NT, 0, st>>>(A, nblk);
+      else prep_kernel<NC, NG, NGF, 3><<<grid, NT, 0, st>>>(A, nblk);
       break;
     case 3:  // linear right-hand side
-      lin_kernel<NC, NG, NGF><<<nb, NT, 0, st>>>(A);
+      lin_kernel<NC, NG, NGF><<<grid, NT, 0, st>>>(A, nblk);
       break;
     case 4:  // nonlinear B+C
-      if (A.variant == 1) bc_kernel<NC, NG, NGF, 1><<<nb, NT, 0, st>>>(A);
-      else if (A.variant == 2) bc_kernel<NC, NG, NGF, 2><<<nb, NT, 0, st>>>(A);
-      else bc_kernel<NC, NG, NGF, 3><<<nb, NT, 0, st>>>(A);
-      break;
-    case 6:  // FP64 linear core (hybrid nonlinear path)
-      if (A.variant == 1) core64_kernel<NC, NG, NGF, 1><<<nb, NT, 0, st>>>(A);
-      else if (A.variant == 2) core64_kernel<NC, NG, NGF, 2><<<nb, NT, 0, st>>>(A);
-      else core64_kernel<NC, NG, NGF, 3><<<nb, NT, 0, st>>>(A);
+      if (v == 1) bc_kernel<NC, NG, NGF, 1><<<grid, NT, 0, st>>>(A, nblk);
+      else if (v == 2) bc_kernel<NC, NG, NGF, 2><<<grid, NT, 0, st>>>(A, nblk);
+      else bc_kernel<NC, NG, NGF, 3><<<grid, NT, 0, st>>>(A, nblk);
       break;
     case 5:  // nonlinear E+G
-      if (A.variant == 1) eg_kernel<NC, NG, NGF, 1><<<nb, NT, 0, st>>>(A);
-      else if (A.variant == 2) eg_kernel<NC, NG, NGF, 2><<<nb, NT, 0, st>>>(A);
-      else eg_kernel<NC, NG, NGF, 3><<<nb, NT, 0, st>>>(A);
+      if (v == 1) eg_kernel<NC, NG, NGF, 1><<<grid, NT, 0, st>>>(A, nblk);
+      else if (v == 2) eg_kernel<NC, NG, NGF, 2><<<grid, NT, 0, st>>>(A, nblk);
+      else eg_kernel<NC, NG, NGF, 3><<<grid, NT, 0, st>>>(A, nblk);
       break;
     default:
       return -1;
