@@ -107,6 +107,28 @@ int spoc_mesh_set(spoc_ctx *ctx, int ne, int nf, const int *itm, const int *its,
                   const double *xgr, const double *ygr);
 
 /*
+ * Ensemble: nmem independent meshes run together as one union mesh, so that
+ * every kernel launch covers all members.  Arrays are the members' own
+ * arrays (as for spoc_mesh_set, 1-based within each member) concatenated
+ * member after member; ne_m, nf_m, nn_m give each member's sizes (nn_m,
+ * in, xgr, ygr may be NULL).  The library shifts the numbering.  Fields and
+ * state are then passed for the union: (ld, sum ne_m) arrays, members
+ * concatenated; the linear variant's depth hb(1,1) is taken per member.
+ * A member's results equal those of running it alone (the kernels treat
+ * each element column independently).  The members share g0, mu0, mb0 and
+ * dt.  No reference counterpart (the reference runs members serially).
+ */
+int spoc_ensemble_mesh_set(spoc_ctx *ctx, int nmem, const int *ne_m, const int *nf_m,
+                           const int *nn_m, const int *itm, const int *its, const int *tseg,
+                           const int *ssit, const double *norm, const double *vect,
+                           const double *normt, const int *bdl, const int *in,
+                           const double *xgr, const double *ygr);
+
+/* Per-member integrals of the device-resident state (src/energy.f:3-96):
+ * arrays of nmem values, each summed in the member's own element order. */
+int spoc_ensemble_energy(spoc_ctx *ctx, int nmem, double *ke, double *pe, double *mass);
+
+/*
  * Per-run physics operands of cal_var: fc (Coriolis, modal), w0/z0 (wind
  * stress), hb (resting depth), all (ld, ne); mu0, mb0, g0 as in oc.inp.
  * fc/w0/z0 may be NULL for SPOC_LIN (unused by src/calvar_lin.f).
@@ -134,6 +156,15 @@ int spoc_state_download(spoc_ctx *ctx, int ld, double *u0, double *v0, double *h
  * u0 + (dt/6)*(((k1 + 2k2) + 2k3) + k4).
  */
 int spoc_rk4_steps(spoc_ctx *ctx, int variant, int precision, double dt, int nsteps);
+
+/*
+ * Replay the RK4 step of spoc_rk4_steps from a HIP graph (on = 1, the
+ * default): the step's launches are captured once per (variant, precision,
+ * dt, mesh, fields) and replayed nsteps times, which removes the host launch
+ * cost that dominates small meshes.  Used when nsteps >= 2 and profiling is
+ * off; results are identical either way.  No reference counterpart.
+ */
+int spoc_set_graph(spoc_ctx *ctx, int on);
 
 /* Wait for all work queued on the context's stream. */
 int spoc_sync(spoc_ctx *ctx);

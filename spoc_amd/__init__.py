@@ -46,7 +46,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     for name in ("spoc_ctx_create", "spoc_ctx_destroy", "spoc_mesh_set", "spoc_fields_set",
                  "spoc_rhs", "spoc_state_upload", "spoc_state_download", "spoc_rk4_steps",
                  "spoc_energy", "spoc_sync", "spoc_tables_build", "spoc_last_timing",
-                 "spoc_set_profiling", "spoc_version"):
+                 "spoc_set_profiling", "spoc_set_graph", "spoc_version",
+                 "spoc_ensemble_mesh_set", "spoc_ensemble_energy"):
         getattr(lib, name).restype = C.c_int
     lib.spoc_ctx_destroy.argtypes = [C.c_void_p]
     _lib = lib
@@ -136,6 +137,44 @@ class Context:
             _d(x) if have_nodes else None, _d(y) if have_nodes else None))
         self.ne, self.nf = ne, nf
 
+    def set_ensemble_mesh(self, meshes) -> None:
+        """Several member meshes (dicts as for set_mesh) run as one union mesh;
+        fields and state are then the members' arrays concatenated along
+        elements (see concat)."""
+        self._mesh = meshes
+        ne = np.array([int(m["itm"].shape[1]) for m in meshes], np.int32)
+        nf = np.array([int(m["nf"]) for m in meshes], np.int32)
+        have_nodes = all("in" in m and "x" in m for m in meshes)
+        nn = np.array([len(m["x"]) for m in meshes], np.int32) if have_nodes else None
+        cat = lambda k, dt: np.ascontiguousarray(
+            np.concatenate([np.asfortranarray(m[k], dt).ravel(order="F") for m in meshes]))
+        arrs = dict(itm=cat("itm", np.int32), its=cat("its", np.int32), tseg=cat("tseg", np.int32),
+                    ssit=cat("ssit", np.int32), norm=cat("norm", np.float64),
+                    vect=cat("vect", np.float64), normt=cat("normt", np.float64),
+                    bdl=cat("bdl", np.int32))
+        if have_nodes:
+            arrs.update(tri=cat("in", np.int32), x=cat("x", np.float64), y=cat("y", np.float64))
+        self._ens = (ne, nf, nn, arrs)
+        _check(self.lib.spoc_ensemble_mesh_set(
+            self._h, len(meshes), _i(ne), _i(nf), _i(nn) if have_nodes else None,
+            _i(arrs["itm"]), _i(arrs["its"]), _i(arrs["tseg"]), _i(arrs["ssit"]),
+            _d(arrs["norm"]), _d(arrs["vect"]), _d(arrs["normt"]), _i(arrs["bdl"]),
+            _i(arrs["tri"]) if have_nodes else None, _d(arrs["x"]) if have_nodes else None,
+            _d(arrs["y"]) if have_nodes else None))
+        self.ne, self.nf = int(ne.sum()), int(nf.sum())
+        self.nmem = len(meshes)
+
+    @staticmethod
+    def concat(arrays):
+        """Members' (ld, ne_m) arrays -> one (ld, sum ne_m) array."""
+        return np.asfortranarray(np.concatenate([np.asarray(a) for a in arrays], axis=1))
+
+    def ensemble_energy(self):
+        n = self.nmem
+        ke, pe, mass = (np.zeros(n) for _ in range(3))
+        _check(self.lib.spoc_ensemble_energy(self._h, n, _d(ke), _d(pe), _d(mass)))
+        return ke, pe, mass
+
     def set_fields(self, hb, g0, fc=None, w0=None, z0=None, mu0=0.0, mb0=0.0) -> None:
         ld = hb.shape[0]
         f = [np.asfortranarray(a, np.float64) if a is not None else None for a in (fc, w0, z0, hb)]
@@ -181,6 +220,10 @@ class Context:
 
     def set_profiling(self, on: bool) -> None:
         _check(self.lib.spoc_set_profiling(self._h, int(on)))
+
+    def set_graph(self, on: bool) -> None:
+        """Replay the RK4 step from a HIP graph (default on)."""
+        _check(self.lib.spoc_set_graph(self._h, int(on)))
 
     def timing(self):
         tot = C.c_double()

@@ -21,8 +21,8 @@ using namespace spoc;
 namespace spoc {
 int energy_launch(hipStream_t st, int ne, int nm, int ng, const double *u, const double *v,
                   const double *h, const double *hb, const double *pi2, const double *wi,
-                  const double *ammo1, const double *deta, double g0, double *scratch,
-                  double *out);
+                  const double *ammo1, const double *deta, double g0, int nr, const int *off,
+                  double *scratch, double *out);
 struct F32Plan;
 int f32_plan_create(F32Plan **out, int nc, int ng, int ngf, const double *amm, const double *amml,
                     const double *agx, const double *agy, const double *pi2, const double *pico,
@@ -33,12 +33,13 @@ int f32_mesh_set(F32Plan *p, int ne, int nf, const int *itm, const int *its, con
                  const int *ssit, const double *norm, const double *vect, const double *normt,
                  const int *bdl);
 int f32_fields_set(F32Plan *p, int ld, const double *fc, const double *w0, const double *z0,
-                   const double *hb, const double *pi2, int ld_g, double mu0, double mb0,
-                   double g0);
+                   const double *hb, const double *hh0e, const double *pi2, int ld_g, double mu0,
+                   double mb0, double g0);
 int f32_rhs(F32Plan *p, hipStream_t s, int variant, const double *const *x, double *const *k,
             TimerHook hook, void *user);
-int f32_rk4(F32Plan *p, hipStream_t s, int variant, double *const *u0, double dt, int nsteps,
-            TimerHook hook, void *user);
+int f32_rk4_begin(F32Plan *p, hipStream_t s, int variant, double *const *u0);
+int f32_rk4_step(F32Plan *p, hipStream_t s, int variant, double dt, TimerHook hook, void *user);
+int f32_rk4_end(F32Plan *p, hipStream_t s, int variant, double *const *u0);
 }  // namespace spoc
 
 namespace {
@@ -154,6 +155,11 @@ struct spoc_ctx {
   double *dface = nullptr;
   double **dFptr = nullptr;
   double *den = nullptr;     // energy scratch
+  // element offsets of the meshes run together: {0, ne} for one mesh, or the
+  // members of an ensemble (spoc_ensemble_mesh_set); device copy holds
+  // {0, ne, mem_e0...}
+  std::vector<int> mem_e0;
+  int *dmem = nullptr;
   bool have_state = false;
   hipEvent_t t0 = nullptr, t1 = nullptr;
   double last_ms = 0.0;
@@ -163,12 +169,34 @@ struct spoc_ctx {
   std::vector<int> kcount;
   F32Plan *f32 = nullptr;
   std::vector<double> h_pi2;  // pi2(ng, nm) for projecting fields onto quadrature points
+  // one RK4 step captured as a graph, valid while its key matches
+  hipGraphExec_t graph = nullptr;
+  struct GraphKey {
+    int variant, precision;
+    double dt;
+    long version;
+    bool operator==(const GraphKey &o) const {
+      return variant == o.variant && precision == o.precision && dt == o.dt &&
+             version == o.version;
+    }
+  } graph_key{-1, -1, 0.0, -1};
+  long version = 0;     // bumped by every mesh/field change
+  int use_graph = 1;    // spoc_set_graph
 };
 
 namespace {
 
+void drop_graph(spoc_ctx *c) {
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
+  c->graph = nullptr;
+  c->version++;
+}
+
 void free_mesh(spoc_ctx *c) {
+  drop_graph(c);
   if (c->dmesh) (void)hipFree(c->dmesh);
+  if (c->dmem) (void)hipFree(c->dmem);
+  c->dmem = nullptr;
   if (c->dstate) (void)hipFree(c->dstate);
   if (c->dface) (void)hipFree(c->dface);
   if (c->den) (void)hipFree(c->den);
@@ -404,9 +432,10 @@ int spoc_ctx_destroy(spoc_ctx *c) {
   return SPOC_OK;
 }
 
-int spoc_mesh_set(spoc_ctx *c, int ne, int nf, const int *itm, const int *its, const int *tseg,
-                  const int *ssit, const double *norm, const double *vect, const double *normt,
-                  const int *bdl, int nn, const int *in, const double *xgr, const double *ygr) {
+static int mesh_set(spoc_ctx *c, int ne, int nf, const int *itm, const int *its,
+                    const int *tseg, const int *ssit, const double *norm, const double *vect,
+                    const double *normt, const int *bdl, int nn, const int *in,
+                    const double *xgr, const double *ygr, const std::vector<int> &mem_e0) {
   if (!c) return fail(SPOC_EINVAL, "null context");
   if (ne <= 0 || nf <= 0) return fail(SPOC_EINVAL, "spoc_mesh_set: ne=%d nf=%d", ne, nf);
   // validate connectivity on the host before any kernel indexes with it
@@ -473,16 +502,68 @@ int spoc_mesh_set(spoc_ctx *c, int ne, int nf, const int *itm, const int *its, c
   HIPCHK(hipMalloc((void **)&c->dstate, sizeof(double) * (size_t)20 * nm * ne));
   HIPCHK(hipMemset(c->dstate, 0, sizeof(double) * (size_t)20 * nm * ne));
   HIPCHK(hipMalloc((void **)&c->dface, sizeof(double) * (size_t)nf * 6 * ngf));
-  HIPCHK(hipMalloc((void **)&c->den, sizeof(double) * ((size_t)ne * (2 + nm) + 8)));
+  const int nmem = (int)mem_e0.size() - 1;
+  HIPCHK(hipMalloc((void **)&c->den, sizeof(double) * ((size_t)ne * (2 + nm) + 3 * (nmem + 1) + 8)));
+  std::vector<int> off = {0, ne};
+  off.insert(off.end(), mem_e0.begin(), mem_e0.end());
+  HIPCHK(hipMalloc((void **)&c->dmem, sizeof(int) * off.size()));
+  HIPCHK(hipMemcpy(c->dmem, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
+  c->mem_e0 = mem_e0;
   HIPCHK(hipMalloc((void **)&c->dFptr, sizeof(double *) * 4));
   double *fp[4];
   for (int i = 0; i < 4; i++) fp[i] = state_ptr(c, 12 + i);
   HIPCHK(hipMemcpy(c->dFptr, fp, sizeof fp, hipMemcpyHostToDevice));
-  HIPCHK(hipMalloc((void **)&c->dfields, sizeof(double) * (size_t)4 * nm * ne));
-  HIPCHK(hipMemset(c->dfields, 0, sizeof(double) * (size_t)4 * nm * ne));
+  HIPCHK(hipMalloc((void **)&c->dfields, sizeof(double) * ((size_t)4 * nm * ne + ne)));
+  HIPCHK(hipMemset(c->dfields, 0, sizeof(double) * ((size_t)4 * nm * ne + ne)));
   if (f32_mesh_set(c->f32, ne, nf, itm, its, tseg, ssit, norm, vect, normt, bdl))
     return fail(SPOC_EHIP, "FP32 mesh upload failed");
   return SPOC_OK;
+}
+
+int spoc_mesh_set(spoc_ctx *c, int ne, int nf, const int *itm, const int *its, const int *tseg,
+                  const int *ssit, const double *norm, const double *vect, const double *normt,
+                  const int *bdl, int nn, const int *in, const double *xgr, const double *ygr) {
+  return mesh_set(c, ne, nf, itm, its, tseg, ssit, norm, vect, normt, bdl, nn, in, xgr, ygr,
+                  {0, ne});
+}
+
+int spoc_ensemble_mesh_set(spoc_ctx *c, int nmem, const int *ne_m, const int *nf_m,
+                           const int *nn_m, const int *itm, const int *its, const int *tseg,
+                           const int *ssit, const double *norm, const double *vect,
+                           const double *normt, const int *bdl, const int *in,
+                           const double *xgr, const double *ygr) {
+  if (!c) return fail(SPOC_EINVAL, "null context");
+  if (nmem < 1 || !ne_m || !nf_m) return fail(SPOC_EINVAL, "spoc_ensemble_mesh_set: nmem=%d", nmem);
+  const bool nodes = nn_m && in && xgr && ygr;
+  long ne = 0, nf = 0, nn = 0;
+  std::vector<int> e0(nmem + 1, 0);
+  for (int m = 0; m < nmem; m++) {
+    if (ne_m[m] <= 0 || nf_m[m] <= 0 || (nodes && nn_m[m] <= 0))
+      return fail(SPOC_EINVAL, "member %d: ne=%d nf=%d", m, ne_m[m], nf_m[m]);
+    ne += ne_m[m];
+    nf += nf_m[m];
+    if (nodes) nn += nn_m[m];
+    e0[m + 1] = (int)ne;
+    if (ne > (1L << 30) || nf > (1L << 30)) return fail(SPOC_EINVAL, "ensemble too large");
+  }
+  // union mesh: member m's 1-based element, face and node numbers shifted by
+  // the totals of the members before it (0, a wall, stays 0)
+  std::vector<int> uitm(3 * ne), utseg(2 * nf), uin(nodes ? 3 * ne : 0);
+  long eo = 0, fo = 0, no = 0;
+  for (int m = 0; m < nmem; m++) {
+    for (long i = 3 * eo; i < 3 * (eo + ne_m[m]); i++) {
+      uitm[i] = itm[i] + (int)fo;
+      if (nodes) uin[i] = in[i] + (int)no;
+    }
+    for (long i = 2 * fo; i < 2 * (fo + nf_m[m]); i++)
+      utseg[i] = tseg[i] == 0 ? 0 : tseg[i] + (int)eo;
+    eo += ne_m[m];
+    fo += nf_m[m];
+    if (nodes) no += nn_m[m];
+  }
+  return mesh_set(c, (int)ne, (int)nf, uitm.data(), its, utseg.data(), ssit, norm, vect, normt,
+                  bdl, nodes ? (int)nn : 0, nodes ? uin.data() : nullptr, nodes ? xgr : nullptr,
+                  nodes ? ygr : nullptr, e0);
 }
 
 static int upload_fields(spoc_ctx *c, int ld, const double *const *src, double *const *dst,
@@ -518,14 +599,22 @@ int spoc_fields_set(spoc_ctx *c, int ld, const double *fc, const double *w0, con
   if (!c || !c->dmesh) return fail(SPOC_ESTATE, "spoc_fields_set: set the mesh first");
   if (ld < c->nm || !hb) return fail(SPOC_EINVAL, "spoc_fields_set: ld=%d, hb=%p", ld, hb);
   (void)hipSetDevice(c->device);
+  drop_graph(c);
   size_t n = (size_t)c->nm * c->ne;
   double *dst[4] = {c->dfields, c->dfields + n, c->dfields + 2 * n, c->dfields + 3 * n};
   const double *src[4] = {fc, w0, z0, hb};
   HIPCHK(hipMemset(c->dfields, 0, sizeof(double) * 3 * n));
   int rc = upload_fields(c, ld, src, dst, 4);
   if (rc) return rc;
-  c->phys = Physics{mu0, mb0, g0, hb[0]};  // hh0 = hb(1,1) (src/calvar_lin.f:65)
-  if (f32_fields_set(c->f32, ld, fc, w0, z0, hb, c->h_pi2.data(), c->ng, mu0, mb0, g0))
+  // hh0 = hb(1,1) of each element's own mesh (src/calvar_lin.f:65)
+  std::vector<double> hh0e(c->ne);
+  for (size_t m = 0; m + 1 < c->mem_e0.size(); m++)
+    for (int e = c->mem_e0[m]; e < c->mem_e0[m + 1]; e++)
+      hh0e[e] = hb[(size_t)ld * c->mem_e0[m]];
+  HIPCHK(hipMemcpy(c->dfields + 4 * n, hh0e.data(), sizeof(double) * c->ne, hipMemcpyHostToDevice));
+  c->phys = Physics{mu0, mb0, g0, hb[0], c->dfields + 4 * n};
+  if (f32_fields_set(c->f32, ld, fc, w0, z0, hb, hh0e.data(), c->h_pi2.data(), c->ng, mu0, mb0,
+                     g0))
     return fail(SPOC_EHIP, "FP32 field upload failed");
   c->have_fields = true;
   return SPOC_OK;
@@ -638,24 +727,62 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
   if (rc) return rc;
   if (!c->have_state) return fail(SPOC_ESTATE, "no state: call spoc_state_upload first");
   if (nsteps < 0) return fail(SPOC_EINVAL, "nsteps=%d", nsteps);
+  if (precision != SPOC_FP64 && precision != SPOC_FP32_MFMA)
+    return fail(SPOC_EINVAL, "unknown precision %d", precision);
   (void)hipSetDevice(c->device);
   c->prof.used = 0;
   c->prof.marks.clear();
-  HIPCHK(hipEventRecord(c->t0, c->stream));
   const double *u0[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
   const double *xs[3] = {state_ptr(c, 3), state_ptr(c, 4), state_ptr(c, 5)};
-  if (precision == SPOC_FP32_MFMA) {
-    double *uw[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
-    rc = f32_rk4(c->f32, c->stream, variant, uw, dt, nsteps, prof_hook, c);
-    if (rc) return f32_fail(rc);
+  double *uw[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
+  // one RK4 step (4 right-hand sides with their fused stage updates)
+  auto step = [&](TimerHook hook) -> int {
+    if (precision == SPOC_FP32_MFMA) {
+      int r = f32_rk4_step(c->f32, c->stream, variant, dt, hook, c);
+      return r ? f32_fail(r) : SPOC_OK;
+    }
+    for (int s = 0; s < 4; s++) {
+      RhsArgs a = make_args(c, variant, s == 0 ? u0 : xs, s, dt);
+      if (!hook) a.hook = nullptr;
+      int r = run_rhs(c, precision, a);
+      if (r) return r;
+    }
+    return SPOC_OK;
+  };
+  HIPCHK(hipEventRecord(c->t0, c->stream));
+  if (precision == SPOC_FP32_MFMA && (rc = f32_rk4_begin(c->f32, c->stream, variant, uw)))
+    return f32_fail(rc);
+  const bool graph = c->use_graph && !c->prof.on && nsteps >= 2;
+  if (graph) {
+    // the step's launches are identical every step: replay them as a graph
+    spoc_ctx::GraphKey key{variant, precision, dt, c->version};
+    if (!c->graph || !(c->graph_key == key)) {
+      if (c->graph) (void)hipGraphExecDestroy(c->graph);
+      c->graph = nullptr;
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int r = step(nullptr);
+      hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (r) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+      }
+      if (e != hipSuccess) return fail(SPOC_EHIP, "graph capture: %s", hipGetErrorString(e));
+      e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        c->graph = nullptr;
+        return fail(SPOC_EHIP, "graph instantiate: %s", hipGetErrorString(e));
+      }
+      c->graph_key = key;
+    }
+    for (int it = 0; it < nsteps; it++) HIPCHK(hipGraphLaunch(c->graph, c->stream));
   } else {
     for (int it = 0; it < nsteps; it++)
-      for (int s = 0; s < 4; s++) {
-        RhsArgs a = make_args(c, variant, s == 0 ? u0 : xs, s, dt);
-        rc = run_rhs(c, precision, a);
-        if (rc) return rc;
-      }
+      if ((rc = step(prof_hook))) return rc;
   }
+  if (precision == SPOC_FP32_MFMA && (rc = f32_rk4_end(c->f32, c->stream, variant, uw)))
+    return f32_fail(rc);
   HIPCHK(hipEventRecord(c->t1, c->stream));
   HIPCHK(hipEventSynchronize(c->t1));
   float ms = 0.f;
@@ -701,8 +828,29 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
   return SPOC_OK;
 }
 
+int spoc_set_graph(spoc_ctx *c, int on) {
+  if (!c) return fail(SPOC_EINVAL, "null context");
+  c->use_graph = on != 0;
+  if (!on) drop_graph(c);
+  return SPOC_OK;
+}
+
 int spoc_sync(spoc_ctx *c) {
   if (!c) return fail(SPOC_EINVAL, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SPOC_OK;
+}
+
+// (ke, pe, mass) of nr element ranges given by the device offsets off.
+static int energies(spoc_ctx *c, int nr, const int *off, double *r) {
+  (void)hipSetDevice(c->device);
+  size_t n = (size_t)c->nm * c->ne;
+  double *out = c->den + (size_t)c->ne * (2 + c->nm);
+  int rc = energy_launch(c->stream, c->ne, c->nm, c->ng, state_ptr(c, 0), state_ptr(c, 1),
+                         state_ptr(c, 2), c->dfields + 3 * n, c->tab.pi2, c->tab.wi,
+                         c->tab.ammo1, c->mesh.deta, c->phys.g0, nr, off, c->den, out);
+  if (rc) return fail(SPOC_EHIP, "energy kernel launch failed");
+  HIPCHK(hipMemcpyAsync(r, out, sizeof(double) * 3 * nr, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return SPOC_OK;
 }
@@ -712,18 +860,30 @@ int spoc_energy(spoc_ctx *c, double *ke, double *pe, double *mass) {
     return fail(SPOC_ESTATE, "spoc_energy: need mesh, fields and state");
   if (!c->have_nodes) return fail(SPOC_ESTATE, "spoc_energy: mesh was set without node arrays");
   (void)hipSetDevice(c->device);
-  size_t n = (size_t)c->nm * c->ne;
-  double *out = c->den + (size_t)c->ne * (2 + c->nm);
-  int rc = energy_launch(c->stream, c->ne, c->nm, c->ng, state_ptr(c, 0), state_ptr(c, 1),
-                         state_ptr(c, 2), c->dfields + 3 * n, c->tab.pi2, c->tab.wi,
-                         c->tab.ammo1, c->mesh.deta, c->phys.g0, c->den, out);
-  if (rc) return fail(SPOC_EHIP, "energy kernel launch failed");
   double r[3];
-  HIPCHK(hipMemcpyAsync(r, out, sizeof r, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  int rc = energies(c, 1, c->dmem, r);
+  if (rc) return rc;
   *ke = r[0];
   *pe = r[1];
   *mass = r[2];
+  return SPOC_OK;
+}
+
+int spoc_ensemble_energy(spoc_ctx *c, int nmem, double *ke, double *pe, double *mass) {
+  if (!c || !c->have_state || !c->have_fields)
+    return fail(SPOC_ESTATE, "spoc_ensemble_energy: need mesh, fields and state");
+  if (!c->have_nodes) return fail(SPOC_ESTATE, "spoc_ensemble_energy: mesh set without nodes");
+  if (nmem != (int)c->mem_e0.size() - 1)
+    return fail(SPOC_EINVAL, "spoc_ensemble_energy: nmem=%d, the mesh has %d members", nmem,
+                (int)c->mem_e0.size() - 1);
+  std::vector<double> r(3 * (size_t)nmem);
+  int rc = energies(c, nmem, c->dmem + 2, r.data());
+  if (rc) return rc;
+  for (int m = 0; m < nmem; m++) {
+    ke[m] = r[3 * m];
+    pe[m] = r[3 * m + 1];
+    mass[m] = r[3 * m + 2];
+  }
   return SPOC_OK;
 }
 

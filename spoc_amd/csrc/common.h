@@ -55,6 +55,9 @@ struct DevMesh {
 
 struct Physics {
   double mu0, mb0, g0, hh0;
+  // hb(1,1) of each element's own mesh (an ensemble member's first element):
+  // the constant depth of the linear variant (src/calvar_lin.f:65)
+  const double *hh0e;
 };
 
 }  // namespace spoc

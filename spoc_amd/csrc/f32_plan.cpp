@@ -75,6 +75,7 @@ struct F32Plan {
   F32Mesh mesh{};
   // fields
   float *dfld = nullptr;
+  double *dhh0e = nullptr;
   F32Fields fld{};
   bool have_fields = false;
   // state/work
@@ -231,6 +232,8 @@ int f32_plan_create(F32Plan **out, int nc, int ng, int ngf, const double *amm, c
 static void free_mesh(F32Plan *p) {
   if (p->dmesh) (void)hipFree(p->dmesh);
   if (p->dfld) (void)hipFree(p->dfld);
+  if (p->dhh0e) (void)hipFree(p->dhh0e);
+  p->dhh0e = nullptr;
   if (p->dstate) (void)hipFree(p->dstate);
   p->dmesh = p->dfld = nullptr;
   p->dstate = nullptr;
@@ -333,22 +336,22 @@ int f32_mesh_set(F32Plan *p, int ne, int nf, const int *itm, const int *its, con
   // fields buffer: hbq, w0q, z0q, hbdq (blocked, nepad*ng each), fc3 (ne*3)
   size_t nq = (size_t)nepad * ng;
   if (hipMalloc(&p->dfld, (4 * nq + (size_t)ne * 3) * 4 + 256) != hipSuccess) return -2;
+  if (hipMalloc(&p->dhh0e, (size_t)ne * sizeof(double)) != hipSuccess) return -2;
   if (hipMemset(p->dfld, 0, (4 * nq + (size_t)ne * 3) * 4 + 256) != hipSuccess) return -2;
   return 0;
 }
 
 int f32_fields_set(F32Plan *p, int ld, const double *fc, const double *w0, const double *z0,
-                   const double *hb, const double *pi2, int ld_g, double mu0, double mb0,
-                   double g0) {
+                   const double *hb, const double *hh0e, const double *pi2, int ld_g, double mu0,
+                   double mb0, double g0) {
   int ne = p->ne, nepad = p->nepad, nm = p->nm, ng = p->ng, nc = p->nc;
   size_t nq = (size_t)nepad * ng;
   std::vector<float> h(4 * nq + (size_t)ne * 3, 0.0f);
-  const double hh0 = hb[0];
   for (int e = 0; e < ne; e++)  // hb - hh0 at the quadrature points (FP32 mass flux)
     for (int q = 0; q < ng; q++) {
       double s = 0.0;
       for (int m = 0; m < nm; m++)
-        s += pi2[q + (size_t)ld_g * m] * (hb[m + (size_t)ld * e] - (m == 0 ? hh0 : 0.0));
+        s += pi2[q + (size_t)ld_g * m] * (hb[m + (size_t)ld * e] - (m == 0 ? hh0e[e] : 0.0));
       h[3 * nq + (size_t)ne * 3 + (size_t)(e & ~15) * ng + (size_t)q * 16 + (e & 15)] = (float)s;
     }
   const double *src[3] = {hb, w0, z0};
@@ -367,9 +370,12 @@ int f32_fields_set(F32Plan *p, int ld, const double *fc, const double *w0, const
       h[3 * nq + 3 * (size_t)e + 1] = (float)fc[1 + (size_t)ld * e];
       h[3 * nq + 3 * (size_t)e + 2] = (float)fc[nc + 1 + (size_t)ld * e];
     }
-  if (hipMemcpy(p->dfld, h.data(), h.size() * 4, hipMemcpyHostToDevice) != hipSuccess) return -2;
-  p->fld = F32Fields{p->dfld, p->dfld + nq, p->dfld + 2 * nq, p->dfld + 3 * nq + (size_t)ne * 3,
-                     p->dfld + 3 * nq, (float)mu0, (float)mb0, g0, hh0};
+  if (hipMemcpy(p->dfld, h.data(), h.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(p->dhh0e, hh0e, (size_t)ne * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    return -2;
+  p->fld = F32Fields{p->dfld,        p->dfld + nq, p->dfld + 2 * nq, p->dfld + 3 * nq + (size_t)ne * 3,
+                     p->dfld + 3 * nq, p->dhh0e,   (float)mu0,       (float)mb0,
+                     g0};
   p->have_fields = true;
   return 0;
 }
@@ -426,21 +432,34 @@ int f32_rhs(F32Plan *p, hipStream_t s, int variant, const double *const *x, doub
   return 0;
 }
 
-int f32_rk4(F32Plan *p, hipStream_t s, int variant, double *const *u0, double dt, int nsteps,
-            TimerHook hook, void *user) {
+// The RK4 loop in three parts so that the caller can capture the step in a
+// graph: begin (reference layout -> psi, stage-0 traces), one step (4
+// stages), end (psi -> reference layout).
+int f32_rk4_begin(F32Plan *p, hipStream_t s, int variant, double *const *u0) {
   int rc = f32_ready(p);
   if (rc) return rc;
-  F32Args a = args(p, s, variant, -1, dt);
+  F32Args a = args(p, s, variant, -1, 0.0);
   for (int f = 0; f < 3; f++)
     if ((rc = call(p, a, 0, u0[f], p->st.c0[f]))) return rc;
-  if ((rc = call(p, a, 2, nullptr, nullptr))) return rc;
-  for (int it = 0; it < nsteps; it++)
-    for (int st = 0; st < 4; st++) {
-      a.stage = st;
-      if ((rc = rhs_kernels(p, a, hook, user))) return rc;
-    }
-  for (int f = 0; f < 3; f++)
-    if ((rc = call(p, a, 1, p->st.c0[f], u0[f]))) return rc;
+  return call(p, a, 2, nullptr, nullptr);
+}
+
+int f32_rk4_step(F32Plan *p, hipStream_t s, int variant, double dt, TimerHook hook, void *user) {
+  F32Args a = args(p, s, variant, -1, dt);
+  for (int st = 0; st < 4; st++) {
+    a.stage = st;
+    int rc = rhs_kernels(p, a, hook, user);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int f32_rk4_end(F32Plan *p, hipStream_t s, int variant, double *const *u0) {
+  F32Args a = args(p, s, variant, -1, 0.0);
+  for (int f = 0; f < 3; f++) {
+    int rc = call(p, a, 1, p->st.c0[f], u0[f]);
+    if (rc) return rc;
+  }
   return 0;
 }
 

@@ -103,8 +103,10 @@ struct F32Fields {
   const float *hbq, *w0q, *z0q;  // [NG rows] values at quadrature points
   const float *hbdq;             // [NG rows] hb - hh0 at quadrature points
   const float *fc3;              // [ne][3] Coriolis f = fc0 + fc1*eta + fc2*xi
+  const double *hh0e;            // [ne] depth of the linear core: hb(1,1) of the
+                                 // element's own mesh (ensemble member)
   float mu0, mb0;
-  double g0, hh0;
+  double g0;
 };
 
 struct F32State {

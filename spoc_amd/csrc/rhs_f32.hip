@@ -258,7 +258,7 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *O, int eb
   const bool valid = e < A.mesh.ne;
   const double *gg = A.mesh.geo64 + (size_t)(valid ? e : 0) * GEO;
   const double ax = gg[0], bx = gg[1], ay = gg[2], by = gg[3], di = valid ? gg[4] : 0.0;
-  const double g0 = A.fld.g0, hh0 = A.fld.hh0;
+  const double g0 = A.fld.g0, hh0 = valid ? A.fld.hh0e[e] : 0.0;
   const size_t mb = (size_t)e0 * NM;
   // (a select, not an index: a runtime index into the kernel argument would
   // copy the whole argument block to scratch)
@@ -374,7 +374,7 @@ __device__ __forceinline__ void traces_uv(const F32Args &A, const double *O, int
   const int l = lane(), g = l >> 4, col = l & 15, e = eb * 16 + col;
   const bool valid = e < A.mesh.ne;
   const double *gg = A.mesh.geo64 + (size_t)(valid ? e : 0) * GEO;
-  const double hh0 = A.fld.hh0;
+  const double hh0 = valid ? A.fld.hh0e[e] : 0.0;
   double *trL = w ? A.st.trL[1] : A.st.trL[0];
 #pragma unroll
   for (int rb = 0; rb < RBE; rb++) {

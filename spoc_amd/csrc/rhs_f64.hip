@@ -207,6 +207,7 @@ struct FaceSpec {
   int comp[6];          // normal component of each output (0: n_x, 1: n_y)
   double post[6];       // extra factor after the normal when haspost
   int haspost[6];
+  const double *hpost;  // if set: per-element factor (of the face's first element) in place of post
 };
 
 template <int NC, int NGF, int NIN>
@@ -256,7 +257,7 @@ __global__ __launch_bounds__(NT) void face_kernel(int nf, const int *__restrict_
   for (int o = 0; o < spec.nout; o++) {
     double *dst = out + ((size_t)f * spec.nout + o) * NGF;
     int c = spec.src[o];
-    double nn = n[spec.comp[o]], post = spec.post[o];
+    double nn = n[spec.comp[o]], post = spec.hpost ? spec.hpost[k1] : spec.post[o];
     int haspost = spec.haspost[o];
 #pragma unroll
     for (int cc = 0; cc < NIN; cc++)
@@ -404,14 +405,15 @@ __global__ __launch_bounds__(NT) void elem_lin(DevMesh M, DevTables T, Physics P
     gy = gy + sy * (-(ax * detai));
     gy = tri_f<NM, NGF>(gy, T.picoT, T.wif, i, Fy, sits + 3 * e, detai);
     // h (src/calvar_lin.f:244-274)
+    const double H = P.hh0e[e0 + e < ne ? e0 + e : 0];
     double ak[NM];
 #pragma unroll
-    for (int j = 0; j < NM; j++) ak[j] = sf[(E + e) * NMP + j] * P.hh0;
+    for (int j = 0; j < NM; j++) ak[j] = sf[(E + e) * NMP + j] * H;
     double b = 0.0;
     b = b + dot_col<NM>(T.agxT, i, ak) * (by * detai);
     b = b + dot_col<NM>(T.agyT, i, ak) * (-ay * detai);
 #pragma unroll
-    for (int j = 0; j < NM; j++) ak[j] = sf[(2 * E + e) * NMP + j] * P.hh0;
+    for (int j = 0; j < NM; j++) ak[j] = sf[(2 * E + e) * NMP + j] * H;
     b = b + dot_col<NM>(T.agxT, i, ak) * (-bx * detai);
     b = b + dot_col<NM>(T.agyT, i, ak) * (ax * detai);
     b = tri_f<NM, NGF>(b, T.picoT, T.wif, i, Fu, sits + 3 * e, -detai);
@@ -783,6 +785,7 @@ int launch_rhs(const RhsArgs &a) {
       fs.post[o] = P.hh0;
       fs.haspost[o] = o >= 2;
     }
+    fs.hpost = P.hh0e;
     a.timer(0, st);
     face_kernel<NC, NGF, 3><<<nfb, NT, 0, st>>>(M.nf, M.tseg, M.ssit, M.norm, a.tab.pico, fs,
                                                   a.face);

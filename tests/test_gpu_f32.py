@@ -1,9 +1,15 @@
-"""FP32-MFMA path against the FP64 oracle.  Tolerances are relative L2 in
-the mass-matrix norm, sqrt(sum_e det_e d^T M d / sum_e det_e x^T M x):
-FP32 quadrature (unit roundoff 6e-8) over ~100 terms per value gives a few
-1e-7; the tests allow 2e-5 for one right-hand side and 1e-6 for eta after
-RK4 steps of the linear config.  The linear variant and the linear
-gravity-wave core of the others run in FP64 on this path."""
+"""Mixed-precision MFMA path against the FP64 oracle.  Tolerances are
+relative L2 in the mass-matrix norm, sqrt(sum_e det_e d^T M d / sum_e det_e
+x^T M x).
+
+- Nonlinear variants: FP32 quadrature (unit roundoff 6e-8) of the
+  nonlinear, viscous, Coriolis and forcing terms on top of an FP64 linear
+  core; one right-hand side measures ~1.4e-8, the bound is 1e-6.  Five RK4
+  steps at p=6 measure ~7e-10, the bound is 1e-8.
+- Linear variant: all FP64 (MFMA), in the mass-orthonormal basis, so the
+  error is FP64 rounding amplified by the basis change (cond(U) grows with
+  p): 9e-16 at p=2 up to 4e-9 at p=7 for one right-hand side, bound 1e-7;
+  100 RK4 steps of config 1 (p=3) measure 1.5e-14, bound 1e-12."""
 import sys
 
 import numpy as np
@@ -62,7 +68,7 @@ def test_cal_var_fp32_close_to_oracle(nc, ng, ngf, variant, prec):
     ref = O.cal_var(variant, c["mesh"], c["phys"], *c["state"])
     errs = [mrel(t, c["mesh"], a, b) for a, b in zip(got, ref)]
     say("  M-norm rel err u v h:", errs)
-    assert max(errs) < 2e-5, errs
+    assert max(errs) < (1e-7 if variant == "lin" else 1e-6), errs
 
 
 def test_rk4_fp32_linear_config1_eta_error():
@@ -76,7 +82,7 @@ def test_rk4_fp32_linear_config1_eta_error():
     ref = O.rk4("lin", c["mesh"], c["phys"], *c["state"], c["dt"], 100)
     e = [mrel(t, c["mesh"], a, b) for a, b in zip(got, ref)]
     say("config1 lin 100 steps fp32 rel err u v h:", e)
-    assert e[2] < 1e-6, e
+    assert max(e) < 1e-12, e
 
 
 @pytest.mark.parametrize("variant,prec", CASES[1:])
@@ -92,4 +98,4 @@ def test_rk4_fp32_nonlinear_p6(variant, prec):
     ref = O.rk4(variant, c["mesh"], c["phys"], *c["state"], 0.5, 5)
     e = [mrel(t, c["mesh"], a, b) for a, b in zip(got, ref)]
     say(variant, prec, "p6 5 steps rel err u v h:", e)
-    assert max(e) < 1e-5, e
+    assert max(e) < 1e-8, e
