@@ -39,10 +39,12 @@ def _i(a):
 
 
 def build(ref: bool = False) -> None:
-    """Compile liboracle.so (and the reference libraries when ``ref``)."""
+    """Compile liboracle.so (and, when ``ref``, the reference libraries and
+    the reference driver built with its own cal_var and with the drop-in)."""
     subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
     if ref:
         subprocess.run(["make", "-s", "-j8", "-C", HERE, "ref"], check=True)
+        subprocess.run(["make", "-s", "-j8", "-C", HERE, "drivers"], check=True)
 
 
 def ref_available() -> bool:
