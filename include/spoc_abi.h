@@ -166,6 +166,26 @@ int spoc_rk4_steps(spoc_ctx *ctx, int variant, int precision, double dt, int nst
  */
 int spoc_set_graph(spoc_ctx *ctx, int on);
 
+/*
+ * Discrete adjoint (no reference counterpart).  With R the right-hand side
+ * in the reference's modal coordinates (what spoc_rhs computes), the
+ * adjoint of one evaluation is (ru,rv,rh) = J_R(u0,v0,h0)^T (au,av,ah)
+ * under the plain dot product of the (nm, ne) coefficient arrays.
+ * spoc_rk4_adjoint_steps runs nsteps steps of the transposed RK4 recursion
+ * backwards on the device-resident adjoint state (spoc_adjoint_upload sets
+ * the adjoint of the final state, spoc_adjoint_download returns that of the
+ * initial state).  The right-hand side is evaluated in FP64 on MFMA.
+ * Implemented for SPOC_LIN (whose Jacobian does not depend on the state;
+ * u0, v0, h0 may be NULL); the nonlinear variants return SPOC_EINVAL.
+ */
+int spoc_rhs_adjoint(spoc_ctx *ctx, int variant, int ld, const double *u0, const double *v0,
+                     const double *h0, const double *au, const double *av, const double *ah,
+                     double *ru, double *rv, double *rh);
+int spoc_adjoint_upload(spoc_ctx *ctx, int ld, const double *au, const double *av,
+                        const double *ah);
+int spoc_adjoint_download(spoc_ctx *ctx, int ld, double *au, double *av, double *ah);
+int spoc_rk4_adjoint_steps(spoc_ctx *ctx, int variant, double dt, int nsteps);
+
 /* Wait for all work queued on the context's stream. */
 int spoc_sync(spoc_ctx *ctx);
 

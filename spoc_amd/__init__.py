@@ -47,7 +47,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                  "spoc_rhs", "spoc_state_upload", "spoc_state_download", "spoc_rk4_steps",
                  "spoc_energy", "spoc_sync", "spoc_tables_build", "spoc_last_timing",
                  "spoc_set_profiling", "spoc_set_graph", "spoc_version",
-                 "spoc_ensemble_mesh_set", "spoc_ensemble_energy"):
+                 "spoc_ensemble_mesh_set", "spoc_ensemble_energy", "spoc_rhs_adjoint",
+                 "spoc_adjoint_upload", "spoc_adjoint_download", "spoc_rk4_adjoint_steps"):
         getattr(lib, name).restype = C.c_int
     lib.spoc_ctx_destroy.argtypes = [C.c_void_p]
     _lib = lib
@@ -192,6 +193,33 @@ class Context:
         _check(self.lib.spoc_rhs(self._h, var, prec, ld, _d(u0), _d(v0), _d(h0),
                                  *(_d(a) for a in out)))
         return tuple(out)
+
+    def cal_var_adjoint(self, variant, au, av, ah, state=None):
+        """Transpose of the right-hand side's Jacobian at `state` applied to
+        (au, av, ah); returns (ru, rv, rh) shaped like au."""
+        var = VARIANTS[variant] if isinstance(variant, str) else variant
+        a = [np.asfortranarray(x, np.float64) for x in (au, av, ah)]
+        ld = a[0].shape[0]
+        st = [None] * 3 if state is None else [_d(np.asfortranarray(x, np.float64)) for x in state]
+        out = [np.zeros_like(a[0], order="F") for _ in range(3)]
+        _check(self.lib.spoc_rhs_adjoint(self._h, var, ld, *st, *(_d(x) for x in a),
+                                         *(_d(x) for x in out)))
+        return tuple(out)
+
+    def adjoint_upload(self, au, av, ah) -> None:
+        a = [np.asfortranarray(x, np.float64) for x in (au, av, ah)]
+        _check(self.lib.spoc_adjoint_upload(self._h, a[0].shape[0], *(_d(x) for x in a)))
+
+    def adjoint_download(self, ld: int = LDM):
+        out = [np.zeros((ld, self.ne), order="F") for _ in range(3)]
+        _check(self.lib.spoc_adjoint_download(self._h, ld, *(_d(x) for x in out)))
+        return tuple(out)
+
+    def rk4_adjoint(self, variant, dt: float, nsteps: int) -> float:
+        """nsteps transposed RK4 steps, backwards; returns device ms."""
+        var = VARIANTS[variant] if isinstance(variant, str) else variant
+        _check(self.lib.spoc_rk4_adjoint_steps(self._h, var, C.c_double(dt), nsteps))
+        return self.timing()[0]
 
     # -- device-resident RK4 ------------------------------------------------
     def upload(self, u0, v0, h0) -> None:

@@ -35,6 +35,7 @@ def _units():
         d = [f"-DSPOC_NC={nc}", f"-DSPOC_NG={ng}", f"-DSPOC_NGF={ngf}"]
         units.append(("rhs_f64.hip", f"rhs_f64_{tag}.o", EXACT + d))
         units.append(("rhs_f32.hip", f"rhs_f32_{tag}.o", d))
+        units.append(("adjoint.hip", f"adj_f64_{tag}.o", d))
     units.append(("f32_plan.cpp", "f32_plan.o", ["-x", "hip"]))
     units.append(("energy.hip", "energy.o", EXACT))
     units.append(("abi.cpp", "abi.o", EXACT + ["-x", "hip"]))

@@ -40,6 +40,11 @@ int f32_rhs(F32Plan *p, hipStream_t s, int variant, const double *const *x, doub
 int f32_rk4_begin(F32Plan *p, hipStream_t s, int variant, double *const *u0);
 int f32_rk4_step(F32Plan *p, hipStream_t s, int variant, double dt, TimerHook hook, void *user);
 int f32_rk4_end(F32Plan *p, hipStream_t s, int variant, double *const *u0);
+int f32_adj_rhs(F32Plan *p, hipStream_t s, int variant, const double *const *lam,
+                double *const *y);
+int f32_adj_begin(F32Plan *p, hipStream_t s, int variant, double dt, const double *const *lam);
+int f32_adj_step(F32Plan *p, hipStream_t s, int variant, double dt);
+int f32_adj_end(F32Plan *p, hipStream_t s, int variant, double *const *lam);
 }  // namespace spoc
 
 namespace {
@@ -160,6 +165,8 @@ struct spoc_ctx {
   // {0, ne, mem_e0...}
   std::vector<int> mem_e0;
   int *dmem = nullptr;
+  double *dadj = nullptr;    // adjoint state, reference layout (nm, ne) x 3
+  bool have_adj = false;
   bool have_state = false;
   hipEvent_t t0 = nullptr, t1 = nullptr;
   double last_ms = 0.0;
@@ -197,6 +204,9 @@ void free_mesh(spoc_ctx *c) {
   if (c->dmesh) (void)hipFree(c->dmesh);
   if (c->dmem) (void)hipFree(c->dmem);
   c->dmem = nullptr;
+  if (c->dadj) (void)hipFree(c->dadj);
+  c->dadj = nullptr;
+  c->have_adj = false;
   if (c->dstate) (void)hipFree(c->dstate);
   if (c->dface) (void)hipFree(c->dface);
   if (c->den) (void)hipFree(c->den);
@@ -513,6 +523,7 @@ static int mesh_set(spoc_ctx *c, int ne, int nf, const int *itm, const int *its,
   double *fp[4];
   for (int i = 0; i < 4; i++) fp[i] = state_ptr(c, 12 + i);
   HIPCHK(hipMemcpy(c->dFptr, fp, sizeof fp, hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc((void **)&c->dadj, sizeof(double) * (size_t)3 * nm * ne));
   HIPCHK(hipMalloc((void **)&c->dfields, sizeof(double) * ((size_t)4 * nm * ne + ne)));
   HIPCHK(hipMemset(c->dfields, 0, sizeof(double) * ((size_t)4 * nm * ne + ne)));
   if (f32_mesh_set(c->f32, ne, nf, itm, its, tseg, ssit, norm, vect, normt, bdl))
@@ -825,6 +836,77 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
       }
     }
   }
+  return SPOC_OK;
+}
+
+static int adj_fail(int rc, int variant) {
+  if (rc == -1 && variant != SPOC_LIN)
+    return fail(SPOC_EINVAL, "adjoint: implemented for the linear variant (SPOC_LIN) only");
+  return f32_fail(rc);
+}
+
+int spoc_rhs_adjoint(spoc_ctx *c, int variant, int ld, const double *u0, const double *v0,
+                     const double *h0, const double *au, const double *av, const double *ah,
+                     double *ru, double *rv, double *rh) {
+  int rc = check_ready(c, variant);
+  if (rc) return rc;
+  (void)u0, (void)v0, (void)h0;  // the linear right-hand side does not depend on the state
+  if (ld < c->nm || !au || !av || !ah || !ru || !rv || !rh)
+    return fail(SPOC_EINVAL, "spoc_rhs_adjoint: bad arguments");
+  (void)hipSetDevice(c->device);
+  size_t n = (size_t)c->nm * c->ne;
+  const double *src[3] = {au, av, ah};
+  double *lam[3] = {c->dadj, c->dadj + n, c->dadj + 2 * n};
+  double *ys[3] = {state_ptr(c, 9), state_ptr(c, 10), state_ptr(c, 11)};
+  if ((rc = upload_fields(c, ld, src, lam, 3))) return rc;
+  rc = f32_adj_rhs(c->f32, c->stream, variant, lam, ys);
+  if (rc) return adj_fail(rc, variant);
+  c->have_adj = false;  // the adjoint state slot now holds this call's input
+  double *dst[3] = {ru, rv, rh};
+  return download_fields(c, ld, dst, ys, 3);
+}
+
+int spoc_adjoint_upload(spoc_ctx *c, int ld, const double *au, const double *av,
+                        const double *ah) {
+  if (!c || !c->dmesh) return fail(SPOC_ESTATE, "spoc_adjoint_upload: set the mesh first");
+  if (ld < c->nm || !au || !av || !ah) return fail(SPOC_EINVAL, "spoc_adjoint_upload: bad args");
+  (void)hipSetDevice(c->device);
+  size_t n = (size_t)c->nm * c->ne;
+  const double *src[3] = {au, av, ah};
+  double *dst[3] = {c->dadj, c->dadj + n, c->dadj + 2 * n};
+  int rc = upload_fields(c, ld, src, dst, 3);
+  if (!rc) c->have_adj = true;
+  return rc;
+}
+
+int spoc_adjoint_download(spoc_ctx *c, int ld, double *au, double *av, double *ah) {
+  if (!c || !c->have_adj) return fail(SPOC_ESTATE, "spoc_adjoint_download: no adjoint state");
+  if (ld < c->nm || !au || !av || !ah) return fail(SPOC_EINVAL, "spoc_adjoint_download: bad args");
+  (void)hipSetDevice(c->device);
+  size_t n = (size_t)c->nm * c->ne;
+  double *dst[3] = {au, av, ah};
+  double *src[3] = {c->dadj, c->dadj + n, c->dadj + 2 * n};
+  return download_fields(c, ld, dst, src, 3);
+}
+
+int spoc_rk4_adjoint_steps(spoc_ctx *c, int variant, double dt, int nsteps) {
+  int rc = check_ready(c, variant);
+  if (rc) return rc;
+  if (!c->have_adj) return fail(SPOC_ESTATE, "no adjoint state: call spoc_adjoint_upload first");
+  if (nsteps < 0) return fail(SPOC_EINVAL, "nsteps=%d", nsteps);
+  (void)hipSetDevice(c->device);
+  size_t n = (size_t)c->nm * c->ne;
+  double *lam[3] = {c->dadj, c->dadj + n, c->dadj + 2 * n};
+  HIPCHK(hipEventRecord(c->t0, c->stream));
+  if ((rc = f32_adj_begin(c->f32, c->stream, variant, dt, lam))) return adj_fail(rc, variant);
+  for (int it = 0; it < nsteps; it++)
+    if ((rc = f32_adj_step(c->f32, c->stream, variant, dt))) return adj_fail(rc, variant);
+  if ((rc = f32_adj_end(c->f32, c->stream, variant, lam))) return adj_fail(rc, variant);
+  HIPCHK(hipEventRecord(c->t1, c->stream));
+  HIPCHK(hipEventSynchronize(c->t1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, c->t0, c->t1));
+  c->last_ms = ms;
   return SPOC_OK;
 }
 

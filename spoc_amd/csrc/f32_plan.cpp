@@ -14,8 +14,9 @@
 
 namespace spoc {
 
-#define SPOC_DECL_F32(NC_, NG_, NGF_) \
-  int rhs_f32_##NC_##_##NG_##_##NGF_(const F32Args &a, int phase, const double *in, double *out);
+#define SPOC_DECL_F32(NC_, NG_, NGF_)                                                        \
+  int rhs_f32_##NC_##_##NG_##_##NGF_(const F32Args &a, int phase, const double *in, double *out); \
+  int adj_f64_##NC_##_##NG_##_##NGF_(const F32Args &a, int phase, const double *in, double *out);
 SPOC_FOR_EACH_SHAPE(SPOC_DECL_F32)
 #undef SPOC_DECL_F32
 
@@ -66,7 +67,7 @@ void img64(double *dst, int R, int K, F op) {
 struct F32Plan {
   int nc, ng, ngf, nm, nl;
   float *dpack32 = nullptr, *dgl32 = nullptr;
-  double *dpack64 = nullptr, *dgl64 = nullptr;
+  double *dpack64 = nullptr, *dgl64 = nullptr, *dpack64t = nullptr;
   F32Ops ops{};
   // mesh
   int ne = 0, nepad = 0;
@@ -88,6 +89,14 @@ void f32_plan_destroy(F32Plan *p);
 static int call(const F32Plan *p, const F32Args &a, int phase, const double *in, double *out) {
 #define SPOC_CALL(NC_, NG_, NGF_) \
   if (p->nc == NC_ && p->ng == NG_ && p->ngf == NGF_) return rhs_f32_##NC_##_##NG_##_##NGF_(a, phase, in, out);
+  SPOC_FOR_EACH_SHAPE(SPOC_CALL)
+#undef SPOC_CALL
+  return -1;
+}
+
+static int call_adj(const F32Plan *p, const F32Args &a, int phase, const double *in, double *out) {
+#define SPOC_CALL(NC_, NG_, NGF_) \
+  if (p->nc == NC_ && p->ng == NG_ && p->ngf == NGF_) return adj_f64_##NC_##_##NG_##_##NGF_(a, phase, in, out);
   SPOC_FOR_EACH_SHAPE(SPOC_CALL)
 #undef SPOC_CALL
   return -1;
@@ -214,17 +223,23 @@ int f32_plan_create(F32Plan **out, int nc, int ng, int ngf, const double *amm, c
   img64(h64.data() + Q.lf, nm, ne3, lf);
   img64(h64.data() + Q.tr, ne3, nm, tr);
   img64(g64.data(), nm, nm, gl);
+  const Pack64T QT = pack64t(nm, ne3);
+  std::vector<double> h64t(QT.total, 0.0);
+  img64(h64t.data() + QT.dxt, nm, nm, [&](int i, int k) { return dx(k, i); });
+  img64(h64t.data() + QT.dyt, nm, nm, [&](int i, int k) { return dy(k, i); });
+  img64(h64t.data() + QT.lft, ne3, nm, [&](int i, int k) { return lf(k, i); });
+  img64(h64t.data() + QT.trt, nm, ne3, [&](int i, int k) { return tr(k, i); });
   auto put = [](auto **d, const auto &h) {
     size_t b = h.size() * sizeof(h[0]);
     return hipMalloc(d, b) == hipSuccess &&
            hipMemcpy(*d, h.data(), b, hipMemcpyHostToDevice) == hipSuccess;
   };
   if (!put(&p->dpack32, h32) || !put(&p->dgl32, g32) || !put(&p->dpack64, h64) ||
-      !put(&p->dgl64, g64)) {
+      !put(&p->dgl64, g64) || !put(&p->dpack64t, h64t)) {
     f32_plan_destroy(p);
     return -2;
   }
-  p->ops = F32Ops{p->dpack32, p->dpack64, p->dgl32, p->dgl64, dU};
+  p->ops = F32Ops{p->dpack32, p->dpack64, p->dpack64t, p->dgl32, p->dgl64, dU};
   *out = p;
   return 0;
 }
@@ -243,7 +258,8 @@ static void free_mesh(F32Plan *p) {
 void f32_plan_destroy(F32Plan *p) {
   if (!p) return;
   free_mesh(p);
-  for (void *d : {(void *)p->dpack32, (void *)p->dgl32, (void *)p->dpack64, (void *)p->dgl64})
+  for (void *d : {(void *)p->dpack32, (void *)p->dgl32, (void *)p->dpack64, (void *)p->dgl64,
+                  (void *)p->dpack64t})
     if (d) (void)hipFree(d);
   delete p;
 }
@@ -312,11 +328,11 @@ int f32_mesh_set(F32Plan *p, int ne, int nf, const int *itm, const int *its, con
   // state / work (element-blocked, see f32_types.h)
   const int ne3 = 3 * ngf;
   const size_t n64 = (size_t)nepad * nm, nt = (size_t)nepad * ne3;
-  const size_t b64 = (12 * n64 + 4 * nt) * 8, b32 = (8 * n64 + 5 * nt) * 4;
+  const size_t b64 = (24 * n64 + 6 * nt) * 8, b32 = (8 * n64 + 5 * nt) * 4;
   if (hipMalloc(&p->dstate, b64 + b32 + 256) != hipSuccess) return -2;
   if (hipMemset(p->dstate, 0, b64 + b32 + 256) != hipSuccess) return -2;
   double *s64 = (double *)p->dstate;
-  float *s32 = (float *)(s64 + 12 * n64 + 4 * nt);
+  float *s32 = (float *)(s64 + 24 * n64 + 6 * nt);
   F32State &st = p->st;
   for (int f = 0; f < 3; f++) {
     st.c0[f] = s64 + f * n64;
@@ -326,6 +342,14 @@ int f32_mesh_set(F32Plan *p, int ne, int nf, const int *itm, const int *its, con
   }
   st.trL[0] = s64 + 12 * n64;
   st.trL[1] = st.trL[0] + 2 * nt;
+  for (int f = 0; f < 3; f++) {
+    double *a = s64 + 12 * n64 + 4 * nt;
+    st.lam[f] = a + f * n64;
+    st.xbar[f] = a + (3 + f) * n64;
+    st.aw[f] = a + (6 + f) * n64;
+    st.avol[f] = a + (9 + f) * n64;
+  }
+  st.abar = s64 + 24 * n64 + 4 * nt;
   for (int f = 0; f < 4; f++) st.F[f] = s32 + f * n64;
   for (int f = 0; f < 2; f++) {
     st.src[f] = s32 + (4 + f) * n64;
@@ -458,6 +482,52 @@ int f32_rk4_end(F32Plan *p, hipStream_t s, int variant, double *const *u0) {
   F32Args a = args(p, s, variant, -1, 0.0);
   for (int f = 0; f < 3; f++) {
     int rc = call(p, a, 1, p->st.c0[f], u0[f]);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+// Adjoint (transpose of the linearised right-hand side and of the RK4
+// step), FP64.  Only the linear variant is implemented; the others return
+// -1 from the launcher.
+int f32_adj_rhs(F32Plan *p, hipStream_t s, int variant, const double *const *lam,
+                double *const *y) {
+  int rc = f32_ready(p);
+  if (rc) return rc;
+  F32Args a = args(p, s, variant, -1, 0.0);
+  for (int f = 0; f < 3; f++)
+    if ((rc = call_adj(p, a, 0, lam[f], p->st.aw[f]))) return rc;
+  if ((rc = call_adj(p, a, 3, nullptr, nullptr)) || (rc = call_adj(p, a, 4, nullptr, nullptr)))
+    return rc;
+  for (int f = 0; f < 3; f++)
+    if ((rc = call_adj(p, a, 1, p->st.k[f], y[f]))) return rc;
+  return 0;
+}
+
+int f32_adj_begin(F32Plan *p, hipStream_t s, int variant, double dt, const double *const *lam) {
+  int rc = f32_ready(p);
+  if (rc) return rc;
+  F32Args a = args(p, s, variant, -1, dt);
+  for (int f = 0; f < 3; f++)
+    if ((rc = call_adj(p, a, 0, lam[f], p->st.lam[f]))) return rc;
+  return call_adj(p, a, 2, nullptr, nullptr);
+}
+
+int f32_adj_step(F32Plan *p, hipStream_t s, int variant, double dt) {
+  F32Args a = args(p, s, variant, -1, dt);
+  for (int st = 0; st < 4; st++) {
+    a.stage = st;
+    int rc = call_adj(p, a, 3, nullptr, nullptr);
+    if (!rc) rc = call_adj(p, a, 4, nullptr, nullptr);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int f32_adj_end(F32Plan *p, hipStream_t s, int variant, double *const *lam) {
+  F32Args a = args(p, s, variant, -1, 0.0);
+  for (int f = 0; f < 3; f++) {
+    int rc = call_adj(p, a, 1, p->st.lam[f], lam[f]);
     if (rc) return rc;
   }
   return 0;

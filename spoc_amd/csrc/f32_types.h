@@ -74,9 +74,25 @@ __host__ __device__ constexpr Pack64 pack64(int nm, int ne3) {
   return p;
 }
 
+// Transposed FP64 operators for the adjoint, offsets in doubles.
+struct Pack64T {
+  int dxt, dyt, lft, trt, total;
+};
+__host__ __device__ constexpr Pack64T pack64t(int nm, int ne3) {
+  Pack64T p{};
+  int o = 0;
+  p.dxt = o; o += img64_size(nm, nm);   // Dx^T
+  p.dyt = o; o += img64_size(nm, nm);   // Dy^T
+  p.lft = o; o += img64_size(ne3, nm);  // LF^T: modes -> edge points
+  p.trt = o; o += img64_size(nm, ne3);  // TR^T: edge points -> modes
+  p.total = o;
+  return p;
+}
+
 struct F32Ops {
   const float *pack32;   // Pack32 layout
   const double *pack64;  // Pack64 layout
+  const double *pack64t; // Pack64T layout
   const float *gl32;     // img32 of G - I, G = U R^T (R amm R^T)^-1 R U^T (wall projector)
   const double *gl64;    // img64 of G - I
   const double *U;       // [NM*NM] upper factor, column-major (basis change)
@@ -119,6 +135,12 @@ struct F32State {
                            // RK4 stage s reads trL[s & 1] and writes trL[(s + 1) & 1],
                            // because one launch reads neighbours' traces and writes its own
   float *trS;              // [3*NE3 rows] flux traces (B+C -> E+G)
+  // adjoint (psi coordinates, FP64): lam = adjoint of the step's end state,
+  // xbar = its running sum over the stages, aw = the stage input (adjoint of
+  // the stage tendency), avol = volume part of the stage result, abar = the
+  // lifted adjoints at the element's own edge points [2*NE3 rows]
+  double *lam[3], *xbar[3], *aw[3], *avol[3];
+  double *abar;
 };
 
 struct F32Args {
