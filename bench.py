@@ -10,8 +10,14 @@ double-gyre wind stress 0.1 N/m^2, mu0=100 m^2/s, eta white noise 1 cm.
 Synthetic data, seeded.
 
 Timed region: K RK4 steps of the device-resident state (inputs already in
-HBM), bracketed by device synchronisation.  A step is 4 right-hand sides;
-element-DOF stage updates per step = 4 * ne * nm * 3.
+HBM; the step replayed from a HIP graph), bracketed by device
+synchronisation.  A step is 4 right-hand sides; element-DOF stage updates
+per step = 4 * ne * nm * 3.  Per-kernel times for the roofline come from a
+separate run of min(K, 20) steps with HIP events around each launch.
+
+--workload selects one of the other configurations (c1 linear basin, c3
+no-slip eddies at p=7, c4 inviscid at p=5, c5 ensemble of 64 members at
+p=4); the default, c2, is the headline.
 
 Prints ONE JSON line.  Multi-GPU runs are not part of this build.
 """
@@ -39,6 +45,27 @@ def quad_rule(ng, ngf):
     q = np.load(os.path.join(ROOT, "tests", "golden", "quadrature.npz"), allow_pickle=False)
     t, g = q[f"tri{ng}"], q[f"legen{ngf}"]
     return t[0], t[1], t[2], g[0], g[1]
+
+
+METRIC = "element-DOF RK4 stage updates/s, FP32-MFMA vs FP64 GPU at \u03b7 rel-L2 \u22641e-6"
+
+WORKLOADS = {
+    "c2": dict(shape=(6, 33, 7), steps=100, name=lambda nc, ng, ngf, ne, dt, a: (
+        f"calvarnl_fs p={nc} ng={ng} ngf={ngf} on {ne} triangles (jittered {a.nx}x{a.ny} nodes, "
+        f"4000x2000 km, beta plane, double-gyre wind, mu0=100), dt={dt} s")),
+    "c1": dict(shape=(3, 12, 4), steps=1000, name=lambda nc, ng, ngf, ne, dt, a: (
+        f"calvar_lin p={nc} ng={ng} ngf={ngf} on {ne} triangles (1000 km square, 16x16 "
+        f"squares), H=4000, 1 m Gaussian eta, dt={dt} s")),
+    "c3": dict(shape=(7, 42, 8), steps=50, name=lambda nc, ng, ngf, ne, dt, a: (
+        f"calvarnl_ns p={nc} ng={ng} ngf={ngf} on {ne} triangles (4000x2000 km), 20 seamounts, "
+        f"k^-3 eddies RMS 0.3 m/s, nu=100, dt={dt} s")),
+    "c4": dict(shape=(5, 25, 6), steps=100, name=lambda nc, ng, ngf, ne, dt, a: (
+        f"calvarnl_inv p={nc} ng={ng} ngf={ngf} on {ne} triangles (4000x2000 km), mu0=0, "
+        f"1 m Gaussian eta, dt={dt} s")),
+    "c5": dict(shape=(4, 25, 5), steps=50, name=lambda nc, ng, ngf, ne, dt, a: (
+        f"ensemble of {a.members} calvarnl_fs members, p={nc} ng={ng} ngf={ngf}, 5k-60k "
+        f"triangles each, {ne} in total, one union mesh per launch, dt={dt} s")),
+}
 
 
 def algorithmic_counts(nm, ng, ngf, variant):
@@ -92,10 +119,12 @@ def calib_copy(nbytes):
         raise RuntimeError("calibration copy failed")
 
 
-def cpu_baseline(variant, nc, ng, ngf, budget_s=15.0):
+def cpu_baseline(variant, nc, ng, ngf, budget_s=15.0, wl="c2"):
     """The reference Fortran (oracle/_ref) on a bounded sample of the same
-    workload: same p, rule and physics on a 25 x 25 node jittered mesh (1152
-    triangles; the reference's static arrays hold at most 1600), one core."""
+    workload: same p, rule and physics on a reduced mesh (at most 1600
+    triangles, what the reference's static arrays hold), one core.  Rules of
+    more than 33 points exceed the reference's arrays; then the C
+    restatement runs instead (kind "port")."""
     import oracle
     from spoc_amd import configs
     t = {"nc": nc, "ng": ng, "ngf": ngf}
@@ -105,11 +134,17 @@ def cpu_baseline(variant, nc, ng, ngf, budget_s=15.0):
     w, x, y, wf, tf = quad_rule(ng, ngf)
     tabs.update(nc=nc, ng=ng, ngf=ngf, nm=(nc + 1) * (nc + 2) // 2, wi=w, xi=x, yi=y, wif=wf,
                 tif=tf)
-    c = configs.config2(tabs, nx=25, ny=25)
+    small = {"c1": lambda: configs.config1(tabs),
+             "c2": lambda: configs.config2(tabs, nx=25, ny=25),
+             "c3": lambda: configs.config3(tabs, nx=25, ny=13),
+             "c4": lambda: configs.config4(tabs, nx=25, ny=13),
+             "c5": lambda: configs.config5(tabs, nmem=1, ntri=(1100, 1100))[0]}
+    c = small[wl]()
     m, p = c["mesh"], c["phys"]
     ne, nm = m["ne"], tabs["nm"]
     kind, runner = "port", None
-    if oracle.ref_available():
+    # the reference's static arrays hold at most 33 quadrature points
+    if oracle.ref_available() and ng <= 33:
         try:
             R = oracle.RefLib(variant)
             R.setup(nc, *quad_rule(ng, ngf))
@@ -138,15 +173,19 @@ def cpu_baseline(variant, nc, ng, ngf, budget_s=15.0):
     threading.stack_size(old)
     val = 4 * ne * nm * 3 * res["n"] / res["s"]
     return {"value": val, "unit": "element-DOF stage updates/s", "cores": 1, "kind": kind,
-            "sample": f"{variant} p={nc} ng={ng} on {ne} triangles (25x25 jittered nodes), "
-                      f"{res['n']} RK4 steps in {res['s']:.1f} s, 1 thread, "
-                      f"{platform.processor() or platform.machine()}"}
+            "sample": f"{variant} p={nc} ng={ng} on {ne} triangles (the workload's generator "
+                      f"at a reduced node count), {res['n']} RK4 steps in {res['s']:.1f} s, "
+                      f"1 thread, {platform.processor() or platform.machine()}"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS),
+                    help="c2 (default): the headline configuration; c1, c3, c4, c5: the other "
+                         "configurations (c5 is the ensemble)")
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--members", type=int, default=64, help="c5 ensemble size")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--nx", type=int, default=448)
     ap.add_argument("--ny", type=int, default=224)
@@ -167,47 +206,75 @@ def main():
     import spoc_amd
     from spoc_amd import configs
 
-    nc, ng, ngf, variant = 6, 33, 7, "fs"
+    wl = args.workload
+    nc, ng, ngf = WORKLOADS[wl]["shape"]
     t = spoc_amd.build_tables(nc, *quad_rule(ng, ngf))
     tb = time.time()
-    c = configs.config2(t, nx=args.nx, ny=args.ny)
-    setup_s = time.time() - tb
-    m, p = c["mesh"], c["phys"]
-    ne, nm = m["ne"], t["nm"]
     ctx = spoc_amd.Context(t)
-    ctx.set_mesh(m)
-    ctx.set_fields(p["hb"], p["g0"], p["fc"], p["w0"], p["z0"], p["mu0"], p["mb0"])
-    dt = c["dt"]
-    K, W = args.steps, args.warmup
+    if wl == "c5":
+        members = configs.config5(t, nmem=args.members)
+        c = members[0]
+        ctx.set_ensemble_mesh([mm["mesh"] for mm in members])
+        P = [mm["phys"] for mm in members]
+        cat = spoc_amd.Context.concat
+        ctx.set_fields(cat([q["hb"] for q in P]), P[0]["g0"], cat([q["fc"] for q in P]),
+                       cat([q["w0"] for q in P]), cat([q["z0"] for q in P]), P[0]["mu0"],
+                       P[0]["mb0"])
+        state0 = tuple(cat([mm["state"][f] for mm in members]) for f in range(3))
+        vect = np.concatenate([mm["mesh"]["vect"] for mm in members], axis=1)
+        ne = int(sum(mm["mesh"]["ne"] for mm in members))
+    else:
+        if wl == "c2":
+            c = configs.config2(t, nx=args.nx, ny=args.ny)
+        elif wl == "c1":
+            c = configs.config1(t)
+            c["variant"] = "lin"
+        elif wl == "c3":
+            c = configs.config3(t)
+        else:
+            c = configs.config4(t)
+        m, p = c["mesh"], c["phys"]
+        ctx.set_mesh(m)
+        ctx.set_fields(p["hb"], p["g0"], p["fc"], p["w0"], p["z0"], p["mu0"], p["mb0"])
+        state0 = c["state"]
+        vect = m["vect"]
+        ne = m["ne"]
+    setup_s = time.time() - tb
+    variant, dt, nm = c["variant"], c["dt"], t["nm"]
+    K = args.steps if args.steps is not None else WORKLOADS[wl]["steps"]
+    W = args.warmup
     if args.calib_copy:
         calib_copy(args.calib_copy << 20)
-    # FP32-MFMA path: warm up, then time K steps from the same initial state
-    ctx.upload(*c["state"])
+    # mixed-precision path: warm up, then time K steps from the same initial state
+    ctx.upload(*state0)
     if W:
         ctx.rk4(variant, dt, W, precision="fp32")
-    ctx.upload(*c["state"])
+    ctx.upload(*state0)
     ctx.sync()
-    ctx.set_profiling(True)
     t0 = time.perf_counter()
-    ctx.rk4(variant, dt, K, precision="fp32")
+    ctx.rk4(variant, dt, K, precision="fp32")   # the RK4 step replayed from a HIP graph
     ctx.sync()
     wall = time.perf_counter() - t0
+    s32 = ctx.download()
+    # per-kernel times (HIP events around each launch; eager launches)
+    ctx.upload(*state0)
+    ctx.set_profiling(True)
+    ctx.rk4(variant, dt, min(K, 20), precision="fp32")
     dev_ms, kern = ctx.timing()
     ctx.set_profiling(False)
-    s32 = ctx.download()
     dof = 4 * ne * nm * 3
     value = dof * K / wall
     out = {
-        "metric": "element-DOF RK4 stage updates/s, FP32-MFMA vs FP64 GPU at eta rel-L2 <=1e-6",
+        "metric": METRIC,
         "value": value, "unit": "element-DOF stage updates/s", "n_gpus": 1, "steps": K,
         "warmup": W, "ms_per_step": 1e3 * wall / K, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": f"calvarnl_fs p={nc} ng={ng} ngf={ngf} on {ne} triangles "
-                               f"(jittered {args.nx}x{args.ny} nodes, 4000x2000 km, beta plane, "
-                               f"double-gyre wind, mu0=100), dt={dt} s",
+        "vs_baseline": None, "dtype": "fp32" if variant != "lin" else "f64",
+        "data": "synthetic",
+        "config": {"workload": WORKLOADS[wl]["name"](nc, ng, ngf, ne, dt, args),
                    "ne": ne, "nm": nm,
-                   "precision": "fp32-mfma quadrature, fp64-mfma linear gravity-wave core, "
-                                "fp64 state/RK4"},
+                   "precision": ("fp32-mfma quadrature, fp64-mfma linear gravity-wave core, "
+                                 "fp64 state/RK4") if variant != "lin" else
+                                "fp64-mfma (the linear variant is all linear core)"},
     }
     # roofline of the dominant kernel (HIP events on the context stream)
     counts = algorithmic_counts(nm, ng, ngf, variant)
@@ -233,7 +300,7 @@ def main():
     # plain FP64 path: same initial state, same steps; eta error of FP32
     if not args.no_fp64:
         K64 = args.fp64_steps or K
-        ctx.upload(*c["state"])
+        ctx.upload(*state0)
         ctx.sync()
         t0 = time.perf_counter()
         ctx.rk4(variant, dt, K64, precision="fp64")
@@ -244,7 +311,7 @@ def main():
         out["fp32_over_fp64"] = value / out["fp64_value"]
         if K64 == K:
             ammo = t["ammo"][:nm, :nm]
-            det = 1.0 / m["vect"][4]
+            det = 1.0 / vect[4]
 
             def mrel(a, b):
                 d = a[:nm] - b[:nm]
@@ -256,7 +323,7 @@ def main():
                                       "eta": mrel(s32[2], s64[2]), "steps": K}
     if not args.no_cpu:
         try:
-            out["cpu_baseline"] = cpu_baseline(variant, nc, ng, ngf)
+            out["cpu_baseline"] = cpu_baseline(variant, nc, ng, ngf, wl=wl)
         except Exception as ex:  # reported, not fatal
             out["cpu_baseline"] = {"error": repr(ex)}
     out["setup_s"] = setup_s

@@ -197,6 +197,123 @@ def config2(tables: dict, nx: int = 448, ny: int = 224, seed: int = 2001) -> dic
     return c
 
 
+def config3(tables: dict, nx: int = 317, ny: int = 159, seed: int = 3001) -> dict:
+    """No-slip eddies over seamounts (p=7, 42-point rule, 8 edge points; the
+    survey's p=8 case has a singular mass matrix with its 42-point rule):
+    the config-2 basin at half its triangle count (99,856 triangles),
+    H = 4000 - sum of 20 Gaussian seamounts (amplitude U(0, 2000) m, radius
+    U(50, 200) km, centres uniform; seed 3001), beta plane f0=1e-4,
+    beta=2e-11, nu=100 m^2/s, mb0=0.  Initial state: a random-phase
+    streamfunction psi with one wave per shell |k| = m (2 pi / 1000 km), m = 1..8,
+    amplitudes ~ m^-2.5 (kinetic energy spectrum ~ k^-3; seed 3002), times
+    sin^2 windows of the basin so that the flow vanishes at the walls,
+    u = -psi_y, v = psi_x, eta = f psi / g, scaled so that the element-mean
+    speed has an RMS of 0.3 m/s."""
+    lx, ly = 4000e3, 2000e3
+    x, y, tri = jittered_mesh(nx, ny, lx, ly, 0.3, seed)
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(0.0, 2000.0, 20)
+    r = rng.uniform(50e3, 200e3, 20)
+    cx, cy = rng.uniform(-lx / 2, lx / 2, 20), rng.uniform(-ly / 2, ly / 2, 20)
+
+    def H(X, Y):
+        h = np.full_like(X, 4000.0)
+        for i in range(20):
+            h -= A[i] * np.exp(-((X - cx[i]) ** 2 + (Y - cy[i]) ** 2) / r[i] ** 2)
+        return h
+
+    rng2 = np.random.default_rng(seed + 1)
+    k0 = 2 * math.pi / 1000e3
+    ms = np.arange(1, 9)
+    th = rng2.uniform(0, 2 * math.pi, len(ms))
+    ph = rng2.uniform(0, 2 * math.pi, len(ms))
+    amp = ms ** -2.5
+    kx, ky = k0 * ms * np.cos(th), k0 * ms * np.sin(th)
+
+    def psi_d(X, Y, which):
+        # psi = W * sum_i a_i cos(k_i.x + phi_i), W = sin^2 sin^2 of the basin,
+        # so that psi and its gradient (the velocity) vanish on the walls
+        p = np.zeros_like(X)
+        px = np.zeros_like(X)
+        py = np.zeros_like(X)
+        for i in range(len(ms)):
+            arg = kx[i] * X + ky[i] * Y + ph[i]
+            p += amp[i] * np.cos(arg)
+            px -= amp[i] * kx[i] * np.sin(arg)
+            py -= amp[i] * ky[i] * np.sin(arg)
+        sx, sy = np.sin(math.pi * (X / lx + 0.5)), np.sin(math.pi * (Y / ly + 0.5))
+        cx_, cy_ = np.cos(math.pi * (X / lx + 0.5)), np.cos(math.pi * (Y / ly + 0.5))
+        W = sx ** 2 * sy ** 2
+        Wx = 2 * sx * cx_ * (math.pi / lx) * sy ** 2
+        Wy = 2 * sy * cy_ * (math.pi / ly) * sx ** 2
+        if which == "psi":
+            return W * p
+        if which == "u":  # -psi_y
+            return -(Wy * p + W * py)
+        return Wx * p + W * px  # psi_x
+
+    f0, beta, g0 = 1e-4, 2e-11, 9.81
+    c = make_case(tables, x, y, tri, f0=f0, beta=beta, g0=g0, H=H, mu0=100.0, mb0=0.0,
+                  u=lambda X, Y: psi_d(X, Y, "u"), v=lambda X, Y: psi_d(X, Y, "v"),
+                  eta=lambda X, Y: (f0 + beta * Y) * psi_d(X, Y, "psi") / g0)
+    u0, v0, h0 = c["state"]
+    scale = 0.3 / math.sqrt(np.mean(u0[0] ** 2 + v0[0] ** 2))
+    c["state"] = tuple(np.asfortranarray(a * scale) for a in (u0, v0, h0))
+    c["dt"] = cfl_dt_altitude(x, y, tri, tables["nc"], 4000.0, g0)
+    c["variant"], c["nsteps"] = "ns", 2000
+    return c
+
+
+def config4(tables: dict, nx: int = 224, ny: int = 113, seed: int = 4001) -> dict:
+    """Inviscid case of the adjoint study (p=5, 25-point rule, 6 edge points):
+    the config-2 basin at 49,952 triangles, beta plane, H=1000 m, the
+    config-2 double-gyre wind, mu0=0, mb0=0, a 1 m Gaussian eta of radius
+    100 km at the centre.  The objective is J = integral of eta^2 over the
+    disc |x| < 200 km after 500 steps."""
+    lx, ly = 4000e3, 2000e3
+    x, y, tri = jittered_mesh(nx, ny, lx, ly, 0.3, seed)
+    tau = 0.1 / 1000.0
+    c = make_case(tables, x, y, tri, f0=1e-4, beta=2e-11, g0=9.81, H=1000.0, mu0=0.0, mb0=0.0,
+                  wind=lambda X, Y: tau * np.sin(math.pi * Y / (ly / 2)),
+                  eta=lambda X, Y: np.exp(-(X ** 2 + Y ** 2) / 100e3 ** 2))
+    c["dt"] = cfl_dt_altitude(x, y, tri, tables["nc"], 1000.0, 9.81)
+    c["variant"], c["nsteps"] = "inv", 500
+    return c
+
+
+def config5(tables: dict, nmem: int = 64, seed: int = 5001, ntri=(5e3, 60e3)) -> list:
+    """Ensemble of nmem free-slip members (p=4, 25-point rule, 5 edge points;
+    the 12-point rule is singular at p=4).  Member m (seed 5001 + m): a
+    jittered mesh of the config-2 basin whose triangle count is log-uniform
+    in 5k..60k (standing in for the adaptive driver's refined meshes), a
+    Gaussian eta bump of amplitude U(0.5, 2) m and radius 100 km centred at
+    U(-200, 200) km in x and y, the config-2 wind scaled by U(0.8, 1.2),
+    beta plane, H=1000 m, mu0=100, mb0=0.  All members share one dt (the
+    smallest member's CFL step) and run 500 steps."""
+    lx, ly = 4000e3, 2000e3
+    tau = 0.1 / 1000.0
+    out = []
+    for m in range(nmem):
+        rng = np.random.default_rng(seed + m)
+        nt = math.exp(rng.uniform(math.log(ntri[0]), math.log(ntri[1])))
+        ny = max(4, int(round(math.sqrt(nt / 4.0))) + 1)
+        nx = 2 * ny - 1
+        x, y, tri = jittered_mesh(nx, ny, lx, ly, 0.3, seed + m)
+        a, x0, y0 = rng.uniform(0.5, 2.0), rng.uniform(-200e3, 200e3), rng.uniform(-200e3, 200e3)
+        ws = rng.uniform(0.8, 1.2)
+        c = make_case(tables, x, y, tri, f0=1e-4, beta=2e-11, g0=9.81, H=1000.0, mu0=100.0,
+                      mb0=0.0, wind=lambda X, Y, ws=ws: ws * tau * np.sin(math.pi * Y / (ly / 2)),
+                      eta=lambda X, Y, a=a, x0=x0, y0=y0:
+                      a * np.exp(-((X - x0) ** 2 + (Y - y0) ** 2) / 100e3 ** 2))
+        c["dt"] = cfl_dt_altitude(x, y, tri, tables["nc"], 1000.0, 9.81)
+        c["variant"], c["nsteps"] = "fs", 500
+        out.append(c)
+    dt = min(c["dt"] for c in out)
+    for c in out:
+        c["dt"] = dt
+    return out
+
+
 def random_case(tables: dict, nx=9, ny=9, seed=0, jitter=0.25, scale=0.1) -> dict:
     """Small jittered mesh with random modal states and forcing, for parity
     tests of single right-hand-side evaluations."""
