@@ -170,21 +170,44 @@ int spoc_set_graph(spoc_ctx *ctx, int on);
  * Discrete adjoint (no reference counterpart).  With R the right-hand side
  * in the reference's modal coordinates (what spoc_rhs computes), the
  * adjoint of one evaluation is (ru,rv,rh) = J_R(u0,v0,h0)^T (au,av,ah)
- * under the plain dot product of the (nm, ne) coefficient arrays.
+ * under the plain dot product of the (nm, ne) coefficient arrays, and, for
+ * the nonlinear variants, rhb = (dR/dhb)^T (au,av,ah) (the bathymetry hb
+ * enters through the total depth eta + hb; NULL to skip; zero for SPOC_LIN,
+ * whose depth is the constant hb(1,1)).  The state (u0,v0,h0) is the
+ * linearisation point; SPOC_LIN ignores it (may be NULL).
+ *
  * spoc_rk4_adjoint_steps runs nsteps steps of the transposed RK4 recursion
- * backwards on the device-resident adjoint state (spoc_adjoint_upload sets
- * the adjoint of the final state, spoc_adjoint_download returns that of the
- * initial state).  The right-hand side is evaluated in FP64 on MFMA.
- * Implemented for SPOC_LIN (whose Jacobian does not depend on the state;
- * u0, v0, h0 may be NULL); the nonlinear variants return SPOC_EINVAL.
+ * backwards on the device-resident adjoint state: spoc_adjoint_upload sets
+ * the adjoint of the state after nsteps steps, spoc_adjoint_download
+ * returns that of the initial state, spoc_adjoint_hb_download the gradient
+ * with respect to hb.  For the nonlinear variants the trajectory starts at
+ * the device-resident state (spoc_state_upload; left unchanged) and is
+ * recomputed with the mixed-precision forward path from checkpoints taken
+ * every spoc_set_checkpoint steps (default 50).  Everything runs in FP64 on
+ * MFMA.
  */
 int spoc_rhs_adjoint(spoc_ctx *ctx, int variant, int ld, const double *u0, const double *v0,
                      const double *h0, const double *au, const double *av, const double *ah,
-                     double *ru, double *rv, double *rh);
+                     double *ru, double *rv, double *rh, double *rhb);
 int spoc_adjoint_upload(spoc_ctx *ctx, int ld, const double *au, const double *av,
                         const double *ah);
 int spoc_adjoint_download(spoc_ctx *ctx, int ld, double *au, double *av, double *ah);
+int spoc_adjoint_hb_download(spoc_ctx *ctx, int ld, double *ahb);
+int spoc_set_checkpoint(spoc_ctx *ctx, int every);
 int spoc_rk4_adjoint_steps(spoc_ctx *ctx, int variant, double dt, int nsteps);
+
+/*
+ * Autotuner of the mixed-precision element kernels (no reference
+ * counterpart).  Each element kernel (lin_kernel, or bc_kernel and
+ * eg_kernel) can run with 2, 4 or 8 waves of 16 elements per workgroup;
+ * spoc_autotune times every choice on the current mesh and keeps the
+ * fastest for `variant` (tiles[0] = lin/bc, tiles[1] = eg, 0 for lin).  It
+ * also runs by itself on the first mixed-precision call after a mesh
+ * change.  spoc_set_tiles forces a choice (0 = automatic) and turns the
+ * automatic tuning on or off.  Results do not depend on the tiles.
+ */
+int spoc_autotune(spoc_ctx *ctx, int variant, int *tiles);
+int spoc_set_tiles(spoc_ctx *ctx, int variant, int tile0, int tile1, int autotune);
 
 /* Wait for all work queued on the context's stream. */
 int spoc_sync(spoc_ctx *ctx);

@@ -48,7 +48,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                  "spoc_energy", "spoc_sync", "spoc_tables_build", "spoc_last_timing",
                  "spoc_set_profiling", "spoc_set_graph", "spoc_version",
                  "spoc_ensemble_mesh_set", "spoc_ensemble_energy", "spoc_rhs_adjoint",
-                 "spoc_adjoint_upload", "spoc_adjoint_download", "spoc_rk4_adjoint_steps"):
+                 "spoc_adjoint_upload", "spoc_adjoint_download", "spoc_rk4_adjoint_steps",
+                 "spoc_autotune", "spoc_set_tiles", "spoc_adjoint_hb_download",
+                 "spoc_set_checkpoint"):
         getattr(lib, name).restype = C.c_int
     lib.spoc_ctx_destroy.argtypes = [C.c_void_p]
     _lib = lib
@@ -194,17 +196,28 @@ class Context:
                                  *(_d(a) for a in out)))
         return tuple(out)
 
-    def cal_var_adjoint(self, variant, au, av, ah, state=None):
+    def cal_var_adjoint(self, variant, au, av, ah, state=None, with_hb=False):
         """Transpose of the right-hand side's Jacobian at `state` applied to
-        (au, av, ah); returns (ru, rv, rh) shaped like au."""
+        (au, av, ah); returns (ru, rv, rh) shaped like au, and the
+        bathymetry adjoint as a 4th entry when with_hb."""
         var = VARIANTS[variant] if isinstance(variant, str) else variant
         a = [np.asfortranarray(x, np.float64) for x in (au, av, ah)]
         ld = a[0].shape[0]
-        st = [None] * 3 if state is None else [_d(np.asfortranarray(x, np.float64)) for x in state]
-        out = [np.zeros_like(a[0], order="F") for _ in range(3)]
+        st_arrays = None if state is None else [np.asfortranarray(x, np.float64) for x in state]
+        st = [None] * 3 if st_arrays is None else [_d(x) for x in st_arrays]
+        out = [np.zeros_like(a[0], order="F") for _ in range(4)]
         _check(self.lib.spoc_rhs_adjoint(self._h, var, ld, *st, *(_d(x) for x in a),
-                                         *(_d(x) for x in out)))
-        return tuple(out)
+                                         *(_d(x) for x in out[:3]),
+                                         _d(out[3]) if with_hb else None))
+        return tuple(out) if with_hb else tuple(out[:3])
+
+    def adjoint_hb_download(self, ld: int = LDM):
+        out = np.zeros((ld, self.ne), order="F")
+        _check(self.lib.spoc_adjoint_hb_download(self._h, ld, _d(out)))
+        return out
+
+    def set_checkpoint(self, every: int) -> None:
+        _check(self.lib.spoc_set_checkpoint(self._h, every))
 
     def adjoint_upload(self, au, av, ah) -> None:
         a = [np.asfortranarray(x, np.float64) for x in (au, av, ah)]
@@ -248,6 +261,18 @@ class Context:
 
     def set_profiling(self, on: bool) -> None:
         _check(self.lib.spoc_set_profiling(self._h, int(on)))
+
+    def autotune(self, variant):
+        """Time the element kernels at 2, 4 and 8 waves per workgroup on this
+        mesh; returns the (lin|bc, eg) tiles kept."""
+        var = VARIANTS[variant] if isinstance(variant, str) else variant
+        out = (C.c_int * 2)()
+        _check(self.lib.spoc_autotune(self._h, var, out))
+        return out[0], out[1]
+
+    def set_tiles(self, variant, tile0=0, tile1=0, autotune=True) -> None:
+        var = VARIANTS[variant] if isinstance(variant, str) else variant
+        _check(self.lib.spoc_set_tiles(self._h, var, tile0, tile1, int(autotune)))
 
     def set_graph(self, on: bool) -> None:
         """Replay the RK4 step from a HIP graph (default on)."""

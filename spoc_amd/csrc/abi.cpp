@@ -40,11 +40,12 @@ int f32_rhs(F32Plan *p, hipStream_t s, int variant, const double *const *x, doub
 int f32_rk4_begin(F32Plan *p, hipStream_t s, int variant, double *const *u0);
 int f32_rk4_step(F32Plan *p, hipStream_t s, int variant, double dt, TimerHook hook, void *user);
 int f32_rk4_end(F32Plan *p, hipStream_t s, int variant, double *const *u0);
-int f32_adj_rhs(F32Plan *p, hipStream_t s, int variant, const double *const *lam,
-                double *const *y);
-int f32_adj_begin(F32Plan *p, hipStream_t s, int variant, double dt, const double *const *lam);
-int f32_adj_step(F32Plan *p, hipStream_t s, int variant, double dt);
-int f32_adj_end(F32Plan *p, hipStream_t s, int variant, double *const *lam);
+int f32_autotune(F32Plan *p, hipStream_t s, int variant, int *out);
+void f32_set_tiles(F32Plan *p, int variant, int t0, int t1, int autotune);
+int f32_adj_rhs(F32Plan *p, hipStream_t s, int variant, const double *const *x,
+                const double *const *lam, double *const *y, double *hb);
+int f32_adj_run(F32Plan *p, hipStream_t s, int variant, double dt, int nsteps, int ncheck,
+                const double *const *x0, double *const *lam, double *hb);
 }  // namespace spoc
 
 namespace {
@@ -165,8 +166,9 @@ struct spoc_ctx {
   // {0, ne, mem_e0...}
   std::vector<int> mem_e0;
   int *dmem = nullptr;
-  double *dadj = nullptr;    // adjoint state, reference layout (nm, ne) x 3
-  bool have_adj = false;
+  double *dadj = nullptr;    // adjoint state, reference layout (nm, ne) x 3, then hb's adjoint
+  bool have_adj = false, have_adj_hb = false;
+  int ncheck = 50;           // checkpoint interval of the nonlinear adjoint
   bool have_state = false;
   hipEvent_t t0 = nullptr, t1 = nullptr;
   double last_ms = 0.0;
@@ -206,7 +208,7 @@ void free_mesh(spoc_ctx *c) {
   c->dmem = nullptr;
   if (c->dadj) (void)hipFree(c->dadj);
   c->dadj = nullptr;
-  c->have_adj = false;
+  c->have_adj = c->have_adj_hb = false;
   if (c->dstate) (void)hipFree(c->dstate);
   if (c->dface) (void)hipFree(c->dface);
   if (c->den) (void)hipFree(c->den);
@@ -523,7 +525,7 @@ static int mesh_set(spoc_ctx *c, int ne, int nf, const int *itm, const int *its,
   double *fp[4];
   for (int i = 0; i < 4; i++) fp[i] = state_ptr(c, 12 + i);
   HIPCHK(hipMemcpy(c->dFptr, fp, sizeof fp, hipMemcpyHostToDevice));
-  HIPCHK(hipMalloc((void **)&c->dadj, sizeof(double) * (size_t)3 * nm * ne));
+  HIPCHK(hipMalloc((void **)&c->dadj, sizeof(double) * (size_t)4 * nm * ne));
   HIPCHK(hipMalloc((void **)&c->dfields, sizeof(double) * ((size_t)4 * nm * ne + ne)));
   HIPCHK(hipMemset(c->dfields, 0, sizeof(double) * ((size_t)4 * nm * ne + ne)));
   if (f32_mesh_set(c->f32, ne, nf, itm, its, tseg, ssit, norm, vect, normt, bdl))
@@ -839,31 +841,48 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
   return SPOC_OK;
 }
 
-static int adj_fail(int rc, int variant) {
-  if (rc == -1 && variant != SPOC_LIN)
-    return fail(SPOC_EINVAL, "adjoint: implemented for the linear variant (SPOC_LIN) only");
+static int adj_fail(int rc) {
+  if (rc == -1) return fail(SPOC_EINVAL, "adjoint: the nonlinear variants need the base state");
   return f32_fail(rc);
 }
 
 int spoc_rhs_adjoint(spoc_ctx *c, int variant, int ld, const double *u0, const double *v0,
                      const double *h0, const double *au, const double *av, const double *ah,
-                     double *ru, double *rv, double *rh) {
+                     double *ru, double *rv, double *rh, double *rhb) {
   int rc = check_ready(c, variant);
   if (rc) return rc;
-  (void)u0, (void)v0, (void)h0;  // the linear right-hand side does not depend on the state
   if (ld < c->nm || !au || !av || !ah || !ru || !rv || !rh)
     return fail(SPOC_EINVAL, "spoc_rhs_adjoint: bad arguments");
+  if (variant != SPOC_LIN && (!u0 || !v0 || !h0))
+    return fail(SPOC_EINVAL, "spoc_rhs_adjoint: the nonlinear variants need the state (u0,v0,h0)");
   (void)hipSetDevice(c->device);
   size_t n = (size_t)c->nm * c->ne;
   const double *src[3] = {au, av, ah};
   double *lam[3] = {c->dadj, c->dadj + n, c->dadj + 2 * n};
   double *ys[3] = {state_ptr(c, 9), state_ptr(c, 10), state_ptr(c, 11)};
+  double *xs[3] = {state_ptr(c, 12), state_ptr(c, 13), state_ptr(c, 14)};
   if ((rc = upload_fields(c, ld, src, lam, 3))) return rc;
-  rc = f32_adj_rhs(c->f32, c->stream, variant, lam, ys);
-  if (rc) return adj_fail(rc, variant);
-  c->have_adj = false;  // the adjoint state slot now holds this call's input
+  if (variant != SPOC_LIN) {
+    const double *xsrc[3] = {u0, v0, h0};
+    if ((rc = upload_fields(c, ld, xsrc, xs, 3))) return rc;
+  }
+  double *hb = c->dadj + 3 * n;
+  rc = f32_adj_rhs(c->f32, c->stream, variant, variant != SPOC_LIN ? xs : nullptr, lam, ys,
+                   variant != SPOC_LIN ? hb : nullptr);
+  if (rc) return adj_fail(rc);
+  c->have_adj = c->have_adj_hb = false;  // the adjoint slots now hold this call's data
   double *dst[3] = {ru, rv, rh};
-  return download_fields(c, ld, dst, ys, 3);
+  if ((rc = download_fields(c, ld, dst, ys, 3))) return rc;
+  if (rhb) {
+    if (variant == SPOC_LIN) {
+      for (int e = 0; e < c->ne; e++)
+        for (int m = 0; m < c->nm; m++) rhb[m + (size_t)ld * e] = 0.0;
+    } else {
+      double *d1[1] = {rhb}, *s1[1] = {hb};
+      if ((rc = download_fields(c, ld, d1, s1, 1))) return rc;
+    }
+  }
+  return SPOC_OK;
 }
 
 int spoc_adjoint_upload(spoc_ctx *c, int ld, const double *au, const double *av,
@@ -876,6 +895,7 @@ int spoc_adjoint_upload(spoc_ctx *c, int ld, const double *au, const double *av,
   double *dst[3] = {c->dadj, c->dadj + n, c->dadj + 2 * n};
   int rc = upload_fields(c, ld, src, dst, 3);
   if (!rc) c->have_adj = true;
+  c->have_adj_hb = false;
   return rc;
 }
 
@@ -889,24 +909,66 @@ int spoc_adjoint_download(spoc_ctx *c, int ld, double *au, double *av, double *a
   return download_fields(c, ld, dst, src, 3);
 }
 
+int spoc_adjoint_hb_download(spoc_ctx *c, int ld, double *ahb) {
+  if (!c || !c->have_adj_hb)
+    return fail(SPOC_ESTATE, "spoc_adjoint_hb_download: no bathymetry adjoint (run "
+                             "spoc_rk4_adjoint_steps with a nonlinear variant first)");
+  if (ld < c->nm || !ahb) return fail(SPOC_EINVAL, "spoc_adjoint_hb_download: bad args");
+  (void)hipSetDevice(c->device);
+  size_t n = (size_t)c->nm * c->ne;
+  double *dst[1] = {ahb}, *src[1] = {c->dadj + 3 * n};
+  return download_fields(c, ld, dst, src, 1);
+}
+
+int spoc_set_checkpoint(spoc_ctx *c, int every) {
+  if (!c) return fail(SPOC_EINVAL, "null context");
+  if (every < 1) return fail(SPOC_EINVAL, "checkpoint interval %d", every);
+  c->ncheck = every;
+  return SPOC_OK;
+}
+
 int spoc_rk4_adjoint_steps(spoc_ctx *c, int variant, double dt, int nsteps) {
   int rc = check_ready(c, variant);
   if (rc) return rc;
   if (!c->have_adj) return fail(SPOC_ESTATE, "no adjoint state: call spoc_adjoint_upload first");
+  if (variant != SPOC_LIN && !c->have_state)
+    return fail(SPOC_ESTATE, "the nonlinear adjoint needs the initial state: spoc_state_upload");
   if (nsteps < 0) return fail(SPOC_EINVAL, "nsteps=%d", nsteps);
   (void)hipSetDevice(c->device);
+  drop_graph(c);
   size_t n = (size_t)c->nm * c->ne;
   double *lam[3] = {c->dadj, c->dadj + n, c->dadj + 2 * n};
+  const double *x0[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
   HIPCHK(hipEventRecord(c->t0, c->stream));
-  if ((rc = f32_adj_begin(c->f32, c->stream, variant, dt, lam))) return adj_fail(rc, variant);
-  for (int it = 0; it < nsteps; it++)
-    if ((rc = f32_adj_step(c->f32, c->stream, variant, dt))) return adj_fail(rc, variant);
-  if ((rc = f32_adj_end(c->f32, c->stream, variant, lam))) return adj_fail(rc, variant);
+  rc = f32_adj_run(c->f32, c->stream, variant, dt, nsteps, c->ncheck, x0, lam,
+                   variant != SPOC_LIN ? c->dadj + 3 * n : nullptr);
+  if (rc) return adj_fail(rc);
   HIPCHK(hipEventRecord(c->t1, c->stream));
   HIPCHK(hipEventSynchronize(c->t1));
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, c->t0, c->t1));
   c->last_ms = ms;
+  c->have_adj_hb = variant != SPOC_LIN;
+  return SPOC_OK;
+}
+
+int spoc_autotune(spoc_ctx *c, int variant, int *tiles) {
+  int rc = check_ready(c, variant);
+  if (rc) return rc;
+  (void)hipSetDevice(c->device);
+  drop_graph(c);
+  rc = f32_autotune(c->f32, c->stream, variant, tiles);
+  return rc ? f32_fail(rc) : SPOC_OK;
+}
+
+int spoc_set_tiles(spoc_ctx *c, int variant, int tile0, int tile1, int autotune) {
+  if (!c) return fail(SPOC_EINVAL, "null context");
+  if (variant < SPOC_LIN || variant > SPOC_INV) return fail(SPOC_EINVAL, "variant=%d", variant);
+  for (int t : {tile0, tile1})
+    if (t != 0 && t != 2 && t != 4 && t != 8)
+      return fail(SPOC_EINVAL, "tile %d: choose 2, 4 or 8 waves per workgroup (0: automatic)", t);
+  drop_graph(c);
+  f32_set_tiles(c->f32, variant, tile0, tile1, autotune);
   return SPOC_OK;
 }
 

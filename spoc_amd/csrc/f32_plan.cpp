@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -82,6 +83,12 @@ struct F32Plan {
   // state/work
   void *dstate = nullptr;
   F32State st{};
+  // workgroup tiles per variant: [0] lin or bc kernel, [1] eg kernel;
+  // tuned = picked by timing on this mesh (0: not yet), forced = set by the
+  // caller (0: automatic)
+  int tuned[4][2] = {};
+  int forced[4][2] = {};
+  bool autotune = true;
 };
 
 void f32_plan_destroy(F32Plan *p);
@@ -223,12 +230,19 @@ int f32_plan_create(F32Plan **out, int nc, int ng, int ngf, const double *amm, c
   img64(h64.data() + Q.lf, nm, ne3, lf);
   img64(h64.data() + Q.tr, ne3, nm, tr);
   img64(g64.data(), nm, nm, gl);
-  const Pack64T QT = pack64t(nm, ne3);
+  const Pack64T QT = pack64t(nm, ng, ne3);
   std::vector<double> h64t(QT.total, 0.0);
   img64(h64t.data() + QT.dxt, nm, nm, [&](int i, int k) { return dx(k, i); });
   img64(h64t.data() + QT.dyt, nm, nm, [&](int i, int k) { return dy(k, i); });
   img64(h64t.data() + QT.lft, ne3, nm, [&](int i, int k) { return lf(k, i); });
   img64(h64t.data() + QT.trt, nm, ne3, [&](int i, int k) { return tr(k, i); });
+  img64(h64t.data() + QT.psi, ng, nm, psi);
+  img64(h64t.data() + QT.psiwt, ng, nm, [&](int i, int k) { return wi[i] * PSI[i + (size_t)ng * k]; });
+  img64(h64t.data() + QT.psit, nm, ng, [&](int i, int k) { return psi(k, i); });
+  for (int q = 0; q < ng; q++) {
+    h64t[QT.xq + q] = PI2(q, nc + 1);
+    h64t[QT.yq + q] = PI2(q, 1);
+  }
   auto put = [](auto **d, const auto &h) {
     size_t b = h.size() * sizeof(h[0]);
     return hipMalloc(d, b) == hipSuccess &&
@@ -269,6 +283,7 @@ int f32_mesh_set(F32Plan *p, int ne, int nf, const int *itm, const int *its, con
                  const int *bdl) {
   (void)nf;
   free_mesh(p);
+  std::memset(p->tuned, 0, sizeof p->tuned);
   int nm = p->nm, ng = p->ng, ngf = p->ngf;
   int nepad = up(ne, 16);
   p->ne = ne;
@@ -328,11 +343,13 @@ int f32_mesh_set(F32Plan *p, int ne, int nf, const int *itm, const int *its, con
   // state / work (element-blocked, see f32_types.h)
   const int ne3 = 3 * ngf;
   const size_t n64 = (size_t)nepad * nm, nt = (size_t)nepad * ne3;
-  const size_t b64 = (24 * n64 + 6 * nt) * 8, b32 = (8 * n64 + 5 * nt) * 4;
+  // FP64: c0 acc x64 k (12), lam xbar aw avol (12), aF aUH aL (8), xb (3), hbbar (1);
+  // trL (4 nt), abar (4 nt), abar2 (2 nt), btr (2 nt)
+  const size_t b64 = (36 * n64 + 12 * nt) * 8, b32 = (8 * n64 + 5 * nt) * 4;
   if (hipMalloc(&p->dstate, b64 + b32 + 256) != hipSuccess) return -2;
   if (hipMemset(p->dstate, 0, b64 + b32 + 256) != hipSuccess) return -2;
   double *s64 = (double *)p->dstate;
-  float *s32 = (float *)(s64 + 24 * n64 + 6 * nt);
+  float *s32 = (float *)(s64 + 36 * n64 + 12 * nt);
   F32State &st = p->st;
   for (int f = 0; f < 3; f++) {
     st.c0[f] = s64 + f * n64;
@@ -349,7 +366,19 @@ int f32_mesh_set(F32Plan *p, int ne, int nf, const int *itm, const int *its, con
     st.aw[f] = a + (6 + f) * n64;
     st.avol[f] = a + (9 + f) * n64;
   }
-  st.abar = s64 + 24 * n64 + 4 * nt;
+  {
+    double *a = s64 + 24 * n64 + 4 * nt;
+    for (int f = 0; f < 4; f++) st.aF[f] = a + f * n64;
+    for (int f = 0; f < 2; f++) {
+      st.aUH[f] = a + (4 + f) * n64;
+      st.aL[f] = a + (6 + f) * n64;
+    }
+    for (int f = 0; f < 3; f++) st.xb[f] = a + (8 + f) * n64;
+    st.hbbar = a + 11 * n64;
+    st.abar = a + 12 * n64;
+    st.abar2 = st.abar + 4 * nt;
+    st.btr = st.abar2 + 2 * nt;
+  }
   for (int f = 0; f < 4; f++) st.F[f] = s32 + f * n64;
   for (int f = 0; f < 2; f++) {
     st.src[f] = s32 + (4 + f) * n64;
@@ -407,6 +436,9 @@ int f32_fields_set(F32Plan *p, int ld, const double *fc, const double *w0, const
 static F32Args args(F32Plan *p, hipStream_t s, int variant, int stage, double dt) {
   F32Args a{};
   a.variant = variant;
+  for (int k = 0; k < 2; k++)
+    a.tile[k] = p->forced[variant & 3][k] ? p->forced[variant & 3][k]
+              : p->tuned[variant & 3][k] ? p->tuned[variant & 3][k] : 4;
   a.stage = stage;
   a.dt = dt;
   a.stream = s;
@@ -433,6 +465,67 @@ static int rhs_kernels(F32Plan *p, const F32Args &a, TimerHook hook, void *user)
   return rc;
 }
 
+// Time each element kernel of `variant` at every workgroup size in
+// TILE_CHOICES on the current mesh (tendency evaluations, which leave the
+// RK4 state untouched) and keep the fastest.  Results do not depend on the
+// choice: it only changes how element blocks map to workgroups.
+int f32_ready(const F32Plan *p);
+int f32_autotune(F32Plan *p, hipStream_t s, int variant, int *out) {
+  int rc = f32_ready(p);
+  if (rc) return rc;
+  F32Args a = args(p, s, variant, -1, 0.0);
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return -2;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return -2;
+  }
+  const int nk = variant == 0 ? 1 : 2;
+  for (int k = 0; k < nk && !rc; k++) {
+    const int phase = variant == 0 ? 3 : 4 + k;
+    float best = 1e30f;
+    int pick = 4;
+    for (int wg : TILE_CHOICES) {
+      a.tile[k] = wg;
+      if ((rc = call(p, a, phase, nullptr, nullptr))) break;  // warm-up
+      (void)hipEventRecord(e0, s);
+      for (int r = 0; r < 3 && !rc; r++) rc = call(p, a, phase, nullptr, nullptr);
+      (void)hipEventRecord(e1, s);
+      float ms = 0.f;
+      if (rc || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = rc ? rc : -2;
+        break;
+      }
+      if (ms < best) {
+        best = ms;
+        pick = wg;
+      }
+    }
+    a.tile[k] = pick;
+    p->tuned[variant & 3][k] = pick;
+  }
+  if (nk == 1) p->tuned[variant & 3][1] = 4;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (out) {
+    out[0] = a.tile[0];
+    out[1] = nk == 2 ? a.tile[1] : 0;
+  }
+  return rc;
+}
+
+void f32_set_tiles(F32Plan *p, int variant, int t0, int t1, int autotune) {
+  p->forced[variant & 3][0] = t0;
+  p->forced[variant & 3][1] = t1;
+  p->autotune = autotune != 0;
+}
+
+static int maybe_tune(F32Plan *p, hipStream_t s, int variant) {
+  if (!p->autotune || p->forced[variant & 3][0] || p->tuned[variant & 3][0]) return 0;
+  return f32_autotune(p, s, variant, nullptr);
+}
+
 int f32_ready(const F32Plan *p) {
   if (!p || !p->dmesh) return -3;
   if (!p->have_fields) return -3;
@@ -445,7 +538,7 @@ int f32_ready(const F32Plan *p) {
 int f32_rhs(F32Plan *p, hipStream_t s, int variant, const double *const *x, double *const *k,
             TimerHook hook, void *user) {
   int rc = f32_ready(p);
-  if (rc) return rc;
+  if (rc || (rc = maybe_tune(p, s, variant))) return rc;
   F32Args a = args(p, s, variant, -1, 0.0);
   for (int f = 0; f < 3; f++)
     if ((rc = call(p, a, 0, x[f], p->st.c0[f]))) return rc;
@@ -461,7 +554,7 @@ int f32_rhs(F32Plan *p, hipStream_t s, int variant, const double *const *x, doub
 // stages), end (psi -> reference layout).
 int f32_rk4_begin(F32Plan *p, hipStream_t s, int variant, double *const *u0) {
   int rc = f32_ready(p);
-  if (rc) return rc;
+  if (rc || (rc = maybe_tune(p, s, variant))) return rc;
   F32Args a = args(p, s, variant, -1, 0.0);
   for (int f = 0; f < 3; f++)
     if ((rc = call(p, a, 0, u0[f], p->st.c0[f]))) return rc;
@@ -488,49 +581,137 @@ int f32_rk4_end(F32Plan *p, hipStream_t s, int variant, double *const *u0) {
 }
 
 // Adjoint (transpose of the linearised right-hand side and of the RK4
-// step), FP64.  Only the linear variant is implemented; the others return
-// -1 from the launcher.
-int f32_adj_rhs(F32Plan *p, hipStream_t s, int variant, const double *const *lam,
-                double *const *y) {
-  int rc = f32_ready(p);
-  if (rc) return rc;
-  F32Args a = args(p, s, variant, -1, 0.0);
-  for (int f = 0; f < 3; f++)
-    if ((rc = call_adj(p, a, 0, lam[f], p->st.aw[f]))) return rc;
+// step), FP64.  Phases of the adjoint launcher: 0/1 basis change of adjoint
+// variables in/out, 2 w = dt/6 lam, 3/4/5 the passes of R^T (the linear
+// variant needs 3 and 4), 6 base-state traces.
+static int adj_rhs_passes(F32Plan *p, F32Args &a) {
+  int rc;
+  if (a.variant != 0 && (rc = call_adj(p, a, 6, nullptr, nullptr))) return rc;
   if ((rc = call_adj(p, a, 3, nullptr, nullptr)) || (rc = call_adj(p, a, 4, nullptr, nullptr)))
     return rc;
+  if (a.variant != 0) rc = call_adj(p, a, 5, nullptr, nullptr);
+  return rc;
+}
+
+// One adjoint right-hand side: y = J_R(x)^T lam (reference coordinates,
+// device arrays [e][nm]); x may be null for the linear variant; hb, if not
+// null, receives J_{R,hb}(x)^T lam.
+int f32_adj_rhs(F32Plan *p, hipStream_t s, int variant, const double *const *x,
+                const double *const *lam, double *const *y, double *hb) {
+  int rc = f32_ready(p);
+  if (rc) return rc;
+  F32Args a = args(p, s, variant, -1, 0.0);
+  if (variant != 0) {
+    if (!x) return -1;
+    for (int f = 0; f < 3; f++)
+      if ((rc = call(p, a, 0, x[f], p->st.xb[f]))) return rc;
+    if (hipMemsetAsync(p->st.hbbar, 0, (size_t)p->nepad * p->nm * sizeof(double), s) !=
+        hipSuccess)
+      return -2;
+  }
+  for (int f = 0; f < 3; f++)
+    if ((rc = call_adj(p, a, 0, lam[f], p->st.aw[f]))) return rc;
+  if ((rc = adj_rhs_passes(p, a))) return rc;
   for (int f = 0; f < 3; f++)
     if ((rc = call_adj(p, a, 1, p->st.k[f], y[f]))) return rc;
+  if (hb && variant != 0 && (rc = call_adj(p, a, 1, p->st.hbbar, hb))) return rc;
   return 0;
 }
 
-int f32_adj_begin(F32Plan *p, hipStream_t s, int variant, double dt, const double *const *lam) {
+// nsteps adjoint RK4 steps backwards from lam (reference coordinates, in
+// place), linearised along the trajectory that starts at x0 (reference
+// coordinates; unused by the linear variant) and is recomputed by the
+// mixed-precision forward path: checkpoints every `ncheck` steps, then per
+// segment the step states, then per step its four stage inputs.  hb (may be
+// null) receives the accumulated adjoint of the bathymetry.
+int f32_adj_run(F32Plan *p, hipStream_t s, int variant, double dt, int nsteps, int ncheck,
+                const double *const *x0, double *const *lam, double *hb) {
   int rc = f32_ready(p);
   if (rc) return rc;
   F32Args a = args(p, s, variant, -1, dt);
-  for (int f = 0; f < 3; f++)
-    if ((rc = call_adj(p, a, 0, lam[f], p->st.lam[f]))) return rc;
-  return call_adj(p, a, 2, nullptr, nullptr);
-}
-
-int f32_adj_step(F32Plan *p, hipStream_t s, int variant, double dt) {
-  F32Args a = args(p, s, variant, -1, dt);
-  for (int st = 0; st < 4; st++) {
-    a.stage = st;
-    int rc = call_adj(p, a, 3, nullptr, nullptr);
-    if (!rc) rc = call_adj(p, a, 4, nullptr, nullptr);
-    if (rc) return rc;
+  const size_t n64 = (size_t)p->nepad * p->nm, sbytes = 3 * n64 * sizeof(double);
+  double *mem = nullptr;
+  int nseg = 0;
+  if (variant != 0) {
+    if (!x0) return -1;
+    if (ncheck < 1) ncheck = 1;
+    nseg = (nsteps + ncheck - 1) / ncheck;
+    // checkpoints (nseg), one segment's step states (ncheck), the 4 stage inputs
+    if (hipMalloc(&mem, sbytes * (nseg + ncheck + 4)) != hipSuccess) return -2;
   }
-  return 0;
-}
-
-int f32_adj_end(F32Plan *p, hipStream_t s, int variant, double *const *lam) {
-  F32Args a = args(p, s, variant, -1, 0.0);
-  for (int f = 0; f < 3; f++) {
-    int rc = call_adj(p, a, 1, p->st.lam[f], lam[f]);
-    if (rc) return rc;
+  auto state = [&](int i) { return mem + (size_t)i * 3 * n64; };
+  auto copy = [&](double *dst, const double *src) {
+    return hipMemcpyAsync(dst, src, sbytes, hipMemcpyDeviceToDevice, s) == hipSuccess ? 0 : -2;
+  };
+  double *ckpt0 = state(0), *seg0 = state(nseg), *base0 = state(nseg + ncheck);
+  do {
+    if (variant != 0) {
+      // forward sweep: checkpoints at the start of every segment
+      for (int f = 0; f < 3 && !rc; f++) rc = call(p, a, 0, x0[f], p->st.c0[f]);
+      if (!rc) rc = maybe_tune(p, s, variant);
+      if (rc) break;
+      F32Args fa = args(p, s, variant, -1, dt);
+      if ((rc = call(p, fa, 2, nullptr, nullptr))) break;
+      for (int n = 0; n < nsteps && !rc; n++) {
+        if (n % ncheck == 0) rc = copy(ckpt0 + (size_t)(n / ncheck) * 3 * n64, p->st.c0[0]);
+        for (int st = 0; st < 4 && !rc; st++) {
+          fa.stage = st;
+          rc = rhs_kernels(p, fa, nullptr, nullptr);
+        }
+      }
+      if (rc) break;
+      if (hipMemsetAsync(p->st.hbbar, 0, n64 * sizeof(double), s) != hipSuccess) {
+        rc = -2;
+        break;
+      }
+    }
+    for (int f = 0; f < 3 && !rc; f++) rc = call_adj(p, a, 0, lam[f], p->st.lam[f]);
+    if (rc || (rc = call_adj(p, a, 2, nullptr, nullptr))) break;
+    for (int sg = (variant != 0 ? nseg - 1 : 0); sg >= 0 && !rc; sg--) {
+      const int n0 = variant != 0 ? sg * ncheck : 0;
+      const int n1 = variant != 0 ? std::min(nsteps, n0 + ncheck) : nsteps;
+      F32Args fa = args(p, s, variant, -1, dt);
+      if (variant != 0) {
+        // the segment's step states
+        rc = copy(p->st.c0[0], ckpt0 + (size_t)sg * 3 * n64);
+        if (!rc) rc = call(p, fa, 2, nullptr, nullptr);
+        for (int n = n0; n < n1 && !rc; n++) {
+          rc = copy(seg0 + (size_t)(n - n0) * 3 * n64, p->st.c0[0]);
+          for (int st = 0; st < 4 && !rc && n + 1 < n1; st++) {
+            fa.stage = st;
+            rc = rhs_kernels(p, fa, nullptr, nullptr);
+          }
+        }
+      }
+      for (int n = n1 - 1; n >= n0 && !rc; n--) {
+        if (variant != 0) {
+          // the four stage inputs of step n
+          rc = copy(p->st.c0[0], seg0 + (size_t)(n - n0) * 3 * n64);
+          if (!rc) rc = call(p, fa, 2, nullptr, nullptr);
+          for (int st = 0; st < 4 && !rc; st++) {
+            rc = copy(base0 + (size_t)st * 3 * n64, p->st.x64[0]);
+            if (st < 3 && !rc) {
+              fa.stage = st;
+              rc = rhs_kernels(p, fa, nullptr, nullptr);
+            }
+          }
+        }
+        for (int as = 0; as < 4 && !rc; as++) {
+          a.stage = as;
+          if (variant != 0)
+            for (int f = 0; f < 3; f++) a.st.xb[f] = base0 + (size_t)(3 - as) * 3 * n64 + f * n64;
+          rc = adj_rhs_passes(p, a);
+        }
+      }
+    }
+    for (int f = 0; f < 3 && !rc; f++) rc = call_adj(p, a, 1, p->st.lam[f], lam[f]);
+    if (!rc && hb && variant != 0) rc = call_adj(p, a, 1, p->st.hbbar, hb);
+  } while (false);
+  if (mem) {
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(mem);
   }
-  return 0;
+  return rc;
 }
 
 }  // namespace spoc

@@ -76,15 +76,21 @@ __host__ __device__ constexpr Pack64 pack64(int nm, int ne3) {
 
 // Transposed FP64 operators for the adjoint, offsets in doubles.
 struct Pack64T {
-  int dxt, dyt, lft, trt, total;
+  int dxt, dyt, lft, trt, total_lin, psi, psiwt, psit, xq, yq, total;
 };
-__host__ __device__ constexpr Pack64T pack64t(int nm, int ne3) {
+__host__ __device__ constexpr Pack64T pack64t(int nm, int ng, int ne3) {
   Pack64T p{};
   int o = 0;
-  p.dxt = o; o += img64_size(nm, nm);   // Dx^T
-  p.dyt = o; o += img64_size(nm, nm);   // Dy^T
-  p.lft = o; o += img64_size(ne3, nm);  // LF^T: modes -> edge points
-  p.trt = o; o += img64_size(nm, ne3);  // TR^T: edge points -> modes
+  p.dxt = o; o += img64_size(nm, nm);    // Dx^T
+  p.dyt = o; o += img64_size(nm, nm);    // Dy^T
+  p.lft = o; o += img64_size(ne3, nm);   // LF^T: modes -> edge points
+  p.trt = o; o += img64_size(nm, ne3);   // TR^T: edge points -> modes
+  p.total_lin = o;                       // (what the linear adjoint uses)
+  p.psi = o;  o += img64_size(ng, nm);   // PSI: modes -> quadrature points
+  p.psiwt = o; o += img64_size(ng, nm);  // PSIW^T = W PSI
+  p.psit = o; o += img64_size(nm, ng);   // PSI^T: quadrature points -> modes
+  p.xq = o;  o += cdiv(ng, 8) * 8;       // quadrature points
+  p.yq = o;  o += cdiv(ng, 8) * 8;
   p.total = o;
   return p;
 }
@@ -137,15 +143,30 @@ struct F32State {
   float *trS;              // [3*NE3 rows] flux traces (B+C -> E+G)
   // adjoint (psi coordinates, FP64): lam = adjoint of the step's end state,
   // xbar = its running sum over the stages, aw = the stage input (adjoint of
-  // the stage tendency), avol = volume part of the stage result, abar = the
-  // lifted adjoints at the element's own edge points [2*NE3 rows]
+  // the stage tendency), avol = partial stage result, abar = the lifted
+  // adjoints at the element's own edge points [4*NE3 rows: eta trace,
+  // mass-flux trace, the two viscous-flux traces], abar2 = those of the
+  // velocity gradient [2*NE3 rows]
   double *lam[3], *xbar[3], *aw[3], *avol[3];
-  double *abar;
+  double *abar, *abar2;
+  // nonlinear adjoint: aF (4) = adjoint of F = mu q, aUH (2) = of the raw
+  // mass flux, aL (2) = wall-projected adjoint of the (u, v) tendency,
+  // xb = the stage's base state (the linearisation point) and btr its
+  // velocity traces [2*NE3 rows], hbbar = accumulated adjoint of hb
+  double *aF[4], *aUH[2], *aL[2];
+  double *xb[3];
+  double *btr;
+  double *hbbar;
 };
+
+// Workgroup sizes (waves of 16 elements) the element kernels can run at;
+// the autotuner picks one per kernel.
+constexpr int TILE_CHOICES[3] = {2, 4, 8};
 
 struct F32Args {
   int variant;
   int stage;  // -1: tendencies only, 0..3: RK4 stage
+  int tile[2];  // waves per workgroup: [0] lin_kernel or bc_kernel, [1] eg_kernel
   double dt;
   hipStream_t stream;
   F32Ops ops;

@@ -149,14 +149,16 @@ __device__ __forceinline__ T face_avg(const T *__restrict__ tr, const F32Mesh &M
 // Element block of this wave.  The workgroups on XCD x (blockIdx % 8, the
 // hardware's round-robin placement) take one contiguous eighth of the
 // blocks, so the neighbour traces they read were written through the same L2.
+template <int W = NW>
 __device__ __forceinline__ int first_block() {
   int g8 = gridDim.x >> 3;
-  return ((blockIdx.x & 7) * g8 + (blockIdx.x >> 3)) * NW;
+  return ((blockIdx.x & 7) * g8 + (blockIdx.x >> 3)) * W;
 }
 
 // Copy n 16-byte words of operator images to LDS (whole workgroup).
+template <int W = NW>
 __device__ __forceinline__ void stage_lds(floatx4 *dst, const floatx4 *__restrict__ src, int n) {
-  for (int i = threadIdx.x; i < n; i += NT) dst[i] = src[i];
+  for (int i = threadIdx.x; i < n; i += 64 * W) dst[i] = src[i];
 }
 
 // ---------------------------------------------------------------------------

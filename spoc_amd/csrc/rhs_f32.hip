@@ -348,14 +348,14 @@ __global__ __launch_bounds__(NT) void prep_kernel(F32Args A, int nblk) {
 // ---------------------------------------------------------------------------
 // Linear right-hand side (src/calvar_lin.f:7-287) + RK4 stage, all FP64.
 // ---------------------------------------------------------------------------
-template <int NC, int NG, int NGF>
-__global__ __launch_bounds__(NT) void lin_kernel(F32Args A, int nblk) {
+template <int NC, int NG, int NGF, int WG>
+__global__ __launch_bounds__(64 * WG) void lin_kernel(F32Args A, int nblk) {
   using Sh = S<NC, NG, NGF>;
   constexpr Pack64 P = Sh::P64;
   __shared__ floatx4 lds[P.total / 2];
-  const int b0 = first_block();
+  const int b0 = first_block<WG>();
   if (b0 >= nblk) return;
-  stage_lds(lds, reinterpret_cast<const floatx4 *>(A.ops.pack64), P.total / 2);
+  stage_lds<WG>(lds, reinterpret_cast<const floatx4 *>(A.ops.pack64), P.total / 2);
   __syncthreads();
   const int eb = b0 + (threadIdx.x >> 6);
   if (eb >= nblk) return;
@@ -372,16 +372,16 @@ __global__ __launch_bounds__(NT) void lin_kernel(F32Args A, int nblk) {
 // F = mu0 q, the quadrature-point sources (advection, Coriolis, wind stress,
 // bottom friction) and the mass flux u (h + hb - hh0).
 // ---------------------------------------------------------------------------
-template <int NC, int NG, int NGF, int VAR>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(BC_WAVES))) void bc_kernel(F32Args A, int nblk) {
+template <int NC, int NG, int NGF, int VAR, int WG>
+__global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(BC_WAVES))) void bc_kernel(F32Args A, int nblk) {
   using Sh = S<NC, NG, NGF>;
   constexpr int NM = Sh::NM, NE3 = Sh::NE3, KM = Sh::KM, KE = Sh::KE, KG = Sh::KG,
                 RBM = Sh::RBM, RBE = Sh::RBE, RBG = Sh::RBG;
   constexpr Pack32 P = Sh::P32;
   __shared__ floatx4 lds[P.total / 4];
-  const int b0 = first_block();
+  const int b0 = first_block<WG>();
   if (b0 >= nblk) return;
-  stage_lds(lds, reinterpret_cast<const floatx4 *>(A.ops.pack32), P.total / 4);
+  stage_lds<WG>(lds, reinterpret_cast<const floatx4 *>(A.ops.pack32), P.total / 4);
   __syncthreads();
   const int eb = b0 + (threadIdx.x >> 6);
   if (eb >= nblk) return;
@@ -622,18 +622,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(BC_WAVES))) 
 // Nonlinear stages E + G (src/calvarnl_fs.f:382-544): FP32 divergence of the
 // B+C results, plus the FP64 linear core, wall projection, RK4 stage.
 // ---------------------------------------------------------------------------
-template <int NC, int NG, int NGF, int VAR>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EG_WAVES))) void eg_kernel(F32Args A, int nblk) {
+template <int NC, int NG, int NGF, int VAR, int WG>
+__global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(EG_WAVES))) void eg_kernel(F32Args A, int nblk) {
   using Sh = S<NC, NG, NGF>;
   constexpr int NM = Sh::NM, NE3 = Sh::NE3, KM = Sh::KM, KE = Sh::KE, RBM = Sh::RBM;
   constexpr Pack32 P = Sh::P32;
   constexpr Pack64 P64 = Sh::P64;
   constexpr int N32 = P.eg_end / 4, N64 = P64.total / 2;  // 16-byte words
   __shared__ floatx4 lds[N32 + N64];
-  const int b0 = first_block();
+  const int b0 = first_block<WG>();
   if (b0 >= nblk) return;
-  stage_lds(lds, reinterpret_cast<const floatx4 *>(A.ops.pack32), N32);
-  stage_lds(lds + N32, reinterpret_cast<const floatx4 *>(A.ops.pack64), N64);
+  stage_lds<WG>(lds, reinterpret_cast<const floatx4 *>(A.ops.pack32), N32);
+  stage_lds<WG>(lds + N32, reinterpret_cast<const floatx4 *>(A.ops.pack64), N64);
   __syncthreads();
   const int eb = b0 + (threadIdx.x >> 6);
   if (eb >= nblk) return;
@@ -744,6 +744,27 @@ __global__ void from_psi_kernel(int ne, const double *__restrict__ U, const doub
   for (int i = 0; i < NM; i++) a[(size_t)e * NM + i] = x[i];
 }
 
+// Element kernels of one right-hand side at a chosen workgroup size (waves
+// of 16 elements per workgroup: the tile the autotuner picks).
+template <int NC, int NG, int NGF, int WG>
+int launch_elem(const F32Args &A, int phase, int nblk) {
+  const int grid = 8 * cdiv(cdiv(nblk, WG), 8);  // a multiple of the 8 XCDs
+  hipStream_t st = A.stream;
+  const int v = A.variant;
+  if (phase == 3) {
+    lin_kernel<NC, NG, NGF, WG><<<grid, 64 * WG, 0, st>>>(A, nblk);
+  } else if (phase == 4) {
+    if (v == 1) bc_kernel<NC, NG, NGF, 1, WG><<<grid, 64 * WG, 0, st>>>(A, nblk);
+    else if (v == 2) bc_kernel<NC, NG, NGF, 2, WG><<<grid, 64 * WG, 0, st>>>(A, nblk);
+    else bc_kernel<NC, NG, NGF, 3, WG><<<grid, 64 * WG, 0, st>>>(A, nblk);
+  } else {
+    if (v == 1) eg_kernel<NC, NG, NGF, 1, WG><<<grid, 64 * WG, 0, st>>>(A, nblk);
+    else if (v == 2) eg_kernel<NC, NG, NGF, 2, WG><<<grid, 64 * WG, 0, st>>>(A, nblk);
+    else eg_kernel<NC, NG, NGF, 3, WG><<<grid, 64 * WG, 0, st>>>(A, nblk);
+  }
+  return 0;
+}
+
 template <int NC, int NG, int NGF>
 int launch(const F32Args &A, int phase, const double *in, double *out) {
   constexpr int NM = nm_of(NC);
@@ -765,18 +786,15 @@ int launch(const F32Args &A, int phase, const double *in, double *out) {
       else prep_kernel<NC, NG, NGF, 3><<<grid, NT, 0, st>>>(A, nblk);
       break;
     case 3:  // linear right-hand side
-      lin_kernel<NC, NG, NGF><<<grid, NT, 0, st>>>(A, nblk);
-      break;
     case 4:  // nonlinear B+C
-      if (v == 1) bc_kernel<NC, NG, NGF, 1><<<grid, NT, 0, st>>>(A, nblk);
-      else if (v == 2) bc_kernel<NC, NG, NGF, 2><<<grid, NT, 0, st>>>(A, nblk);
-      else bc_kernel<NC, NG, NGF, 3><<<grid, NT, 0, st>>>(A, nblk);
+    case 5: {  // nonlinear E+G
+      if (phase > 3 && v == 0) return -1;
+      const int wg = A.tile[phase == 5 ? 1 : 0];
+      if (wg == 2) launch_elem<NC, NG, NGF, 2>(A, phase, nblk);
+      else if (wg == 8) launch_elem<NC, NG, NGF, 8>(A, phase, nblk);
+      else launch_elem<NC, NG, NGF, 4>(A, phase, nblk);
       break;
-    case 5:  // nonlinear E+G
-      if (v == 1) eg_kernel<NC, NG, NGF, 1><<<grid, NT, 0, st>>>(A, nblk);
-      else if (v == 2) eg_kernel<NC, NG, NGF, 2><<<grid, NT, 0, st>>>(A, nblk);
-      else eg_kernel<NC, NG, NGF, 3><<<grid, NT, 0, st>>>(A, nblk);
-      break;
+    }
     default:
       return -1;
   }

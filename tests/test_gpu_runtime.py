@@ -122,3 +122,26 @@ def test_graph_replay_equals_eager(prec, variant):
     ctx.set_graph(True)
     for f in range(3):
         assert np.array_equal(eager[f], graph[f]), (f, np.abs(eager[f] - graph[f]).max())
+
+
+@pytest.mark.parametrize("variant", ["lin", "fs", "ns"])
+def test_tiles_do_not_change_results(variant):
+    """The autotuner's choices (2, 4 or 8 waves of 16 elements per
+    workgroup) only change how element blocks map to workgroups."""
+    t, ctx = ctx_for(6, 33, 7)
+    c = configs.random_case(t, 14, 12, seed=77, scale=0.01)
+    ctx.set_mesh(c["mesh"])
+    set_fields(ctx, c["phys"])
+    tiles = ctx.autotune(variant)
+    say("autotuned tiles", variant, tiles)
+    assert tiles[0] in (2, 4, 8) and tiles[1] in ((0,) if variant == "lin" else (2, 4, 8))
+    res = []
+    for t0, t1 in [(2, 2), (4, 4), (8, 8), (2, 8), (8, 4)]:
+        ctx.set_tiles(variant, t0, t1)
+        ctx.upload(*c["state"])
+        ctx.rk4(variant, 2.0, 3, precision="fp32")
+        res.append(ctx.download())
+    ctx.set_tiles(variant, 0, 0)
+    for r in res[1:]:
+        for f in range(3):
+            assert np.array_equal(r[f], res[0][f])
