@@ -104,6 +104,68 @@ def measured_traffic(kernel, ne):
     return d["bytes_per_launch"].get(kernel)
 
 
+def adjoint_check(ctx, t, c, nsteps, ndir, seed=4001):
+    """Gradient of J = integral of eta^2 over the elements whose centroid is
+    within 200 km of the centre, after nsteps, with respect to the initial
+    eta and to hb: adjoint time (checkpoints every 50 steps) and relative
+    agreement with central differences of the FP64 path (step 1e-6 relative
+    to the direction's scale) along ndir seeded random directions."""
+    m, p = c["mesh"], c["phys"]
+    nm, ne, variant, dt = t["nm"], m["ne"], c["variant"], c["dt"]
+    M = t["ammo"][:nm, :nm]
+    det = 1.0 / m["vect"][4]
+    tri = np.asarray(m["in"]) - 1
+    cx, cy = m["x"][tri].mean(axis=0), m["y"][tri].mean(axis=0)
+    inside = (cx ** 2 + cy ** 2) < 200e3 ** 2
+    x0 = [np.asfortranarray(a) for a in c["state"]]
+
+    def fwd(x, hb):
+        ctx.set_fields(hb, p["g0"], p["fc"], p["w0"], p["z0"], p["mu0"], p["mb0"])
+        ctx.upload(*x)
+        ctx.rk4(variant, dt, nsteps, precision="fp64")
+        h = ctx.download()[2][:nm]
+        return h, float(np.sum(det[inside] * np.einsum("ie,ij,je->e", h[:, inside], M,
+                                                        h[:, inside])))
+
+    h, J0 = fwd(x0, p["hb"])
+    lam = np.zeros((x0[0].shape[0], ne), order="F")
+    lam[:nm, inside] = 2.0 * det[inside] * (M @ h[:, inside])
+    zero = np.zeros_like(lam)
+    ctx.upload(*x0)
+    ctx.adjoint_upload(zero, zero, lam)
+    ctx.set_checkpoint(50)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.rk4_adjoint(variant, dt, nsteps)
+    ctx.sync()
+    adj_s = time.perf_counter() - t0
+    g_eta = ctx.adjoint_download()[2]
+    g_hb = ctx.adjoint_hb_download()
+    rng = np.random.default_rng(seed)
+    errs = {"eta0": [], "hb": []}
+    for _ in range(ndir):
+        for key, scale in (("eta0", 0.01), ("hb", 1.0)):
+            v = np.zeros_like(lam)
+            v[:nm] = rng.standard_normal((nm, ne)) * scale
+            eps = 1e-6 / scale * max(1.0, float(np.abs(x0[2][:nm]).max()) if key == "eta0"
+                                     else float(np.abs(p["hb"][:nm]).max()))
+            if key == "eta0":
+                jp = fwd([x0[0], x0[1], x0[2] + eps * v], p["hb"])[1]
+                jm = fwd([x0[0], x0[1], x0[2] - eps * v], p["hb"])[1]
+                ad = float(np.sum(g_eta[:nm] * v[:nm]))
+            else:
+                jp = fwd(x0, p["hb"] + eps * v)[1]
+                jm = fwd(x0, p["hb"] - eps * v)[1]
+                ad = float(np.sum(g_hb[:nm] * v[:nm]))
+            fd = (jp - jm) / (2 * eps)
+            errs[key].append(abs(fd - ad) / max(abs(fd), 1e-300))
+    ctx.set_fields(p["hb"], p["g0"], p["fc"], p["w0"], p["z0"], p["mu0"], p["mb0"])
+    return {"J": J0, "steps": nsteps, "adjoint_s": adj_s,
+            "adjoint_ms_per_step": 1e3 * adj_s / max(nsteps, 1),
+            "rel_err_vs_central_differences": {k: max(v) if v else None for k, v in errs.items()},
+            "directions": ndir, "checkpoint_every": 50}
+
+
 def calib_copy(nbytes):
     """One device-to-device copy of nbytes (reads and writes nbytes)."""
     import ctypes as C
@@ -186,6 +248,11 @@ def main():
                          "configurations (c5 is the ensemble)")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--members", type=int, default=64, help="c5 ensemble size")
+    ap.add_argument("--adjoint", type=int, default=0, metavar="NDIR",
+                    help="c4: also time the checkpointed adjoint of the run (gradient of the "
+                         "eta^2 integral over the central 200 km disc after --steps steps, "
+                         "with respect to the initial eta and to hb) and check it against "
+                         "central differences of the FP64 path along NDIR seeded directions")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--nx", type=int, default=448)
     ap.add_argument("--ny", type=int, default=224)
@@ -321,6 +388,8 @@ def main():
 
             out["rel_l2_vs_fp64"] = {"u": mrel(s32[0], s64[0]), "v": mrel(s32[1], s64[1]),
                                       "eta": mrel(s32[2], s64[2]), "steps": K}
+    if args.adjoint and wl == "c4":
+        out["adjoint"] = adjoint_check(ctx, t, c, K, args.adjoint)
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(variant, nc, ng, ngf, wl=wl)
