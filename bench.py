@@ -77,15 +77,20 @@ def algorithmic_counts(nm, ng, ngf, variant):
     f4, f8 = 4, 8
     mesh = 16 * f8 + 6 * f4 + f4                       # geo64, nbr/nbe, bdl
     rk = 3 * nm * f8 * (1 + 0.75) + 3 * nm * f8 * 2     # c0, acc reads; acc|c0, x64 writes
-    core_fl = 2 * (4 * nm * nm + 6 * nm * e3)            # -g grad eta, div(hh0 u), traces
-    core_by = 3 * nm * f8 + 2 * e3 * f8 + rk + 2 * e3 * f8   # x64, trL in, RK, trL out
+    vol_fl = 2 * 4 * nm * nm                              # core: -g grad eta, div(hh0 u)
+    face_fl = 2 * 6 * nm * e3                             # core: lifts, traces
+    trl_by = 2 * e3 * f8 + rk + 2 * e3 * f8               # trL in, RK, trL out
     if variant == "lin":
-        return {"lin_f64": (core_fl, core_by + mesh)}
-    bc_fl = 2 * (4 * nm * nm + 10 * nm * e3 + 11 * nm * ng)
-    bc_by = (3 * nm * f8 + 2 * e3 * f4 + 16 * f4 + 6 * f4 + f4 + 4 * ng * f4 + 3 * f4
-             + (4 + 2 + 2) * nm * f4 + 3 * e3 * f4)
-    eg_fl = 2 * (6 * nm * nm + 3 * nm * e3) + core_fl
-    eg_by = (8 * nm * f4 + 3 * e3 * f4 + 16 * f4 + mesh + core_by + 2 * e3 * f4)
+        return {"lin_f64": (vol_fl + face_fl, 3 * nm * f8 + trl_by + mesh)}
+    # B+C: q (Dx, Dy, LF), traces of F and of the mass flux (TR), values at
+    # and projection from the quadrature points, the FP32 volume divergence
+    # (6 products) and the core's FP64 volume terms
+    bc_fl = 2 * (4 * nm * nm + 10 * nm * e3 + 11 * nm * ng + 6 * nm * nm) + vol_fl
+    bc_by = (3 * nm * f8 + 2 * e3 * f4 + mesh + 16 * f4 + f8 + 4 * ng * f4 + 3 * f4
+             + 3 * nm * f8 + 3 * e3 * f4)                 # x64, trA, mesh, fields | gvol, trS
+    # E+G: the FP32 lift of the flux traces and the core's face terms
+    eg_fl = 2 * 3 * nm * e3 + face_fl
+    eg_by = (3 * nm * f8 + 3 * e3 * f4 + 16 * f4 + mesh + f8 + trl_by + 2 * e3 * f4)
     return {"bc_f32": (bc_fl, bc_by), "eg_f32f64": (eg_fl, eg_by)}
 
 

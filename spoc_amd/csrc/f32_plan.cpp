@@ -343,13 +343,13 @@ int f32_mesh_set(F32Plan *p, int ne, int nf, const int *itm, const int *its, con
   // state / work (element-blocked, see f32_types.h)
   const int ne3 = 3 * ngf;
   const size_t n64 = (size_t)nepad * nm, nt = (size_t)nepad * ne3;
-  // FP64: c0 acc x64 k (12), lam xbar aw avol (12), aF aUH aL (8), xb (3), hbbar (1);
-  // trL (4 nt), abar (4 nt), abar2 (2 nt), btr (2 nt)
-  const size_t b64 = (36 * n64 + 12 * nt) * 8, b32 = (8 * n64 + 5 * nt) * 4;
+  // FP64: c0 acc x64 k (12), lam xbar aw avol (12), aF aUH aL (8), xb (3), hbbar (1),
+  // gvol (3); trL (4 nt), abar (4 nt), abar2 (2 nt), btr (2 nt).  FP32: trA, trS (5 nt)
+  const size_t b64 = (39 * n64 + 12 * nt) * 8, b32 = (5 * nt) * 4;
   if (hipMalloc(&p->dstate, b64 + b32 + 256) != hipSuccess) return -2;
   if (hipMemset(p->dstate, 0, b64 + b32 + 256) != hipSuccess) return -2;
   double *s64 = (double *)p->dstate;
-  float *s32 = (float *)(s64 + 36 * n64 + 12 * nt);
+  float *s32 = (float *)(s64 + 39 * n64 + 12 * nt);
   F32State &st = p->st;
   for (int f = 0; f < 3; f++) {
     st.c0[f] = s64 + f * n64;
@@ -379,12 +379,8 @@ int f32_mesh_set(F32Plan *p, int ne, int nf, const int *itm, const int *its, con
     st.abar2 = st.abar + 4 * nt;
     st.btr = st.abar2 + 2 * nt;
   }
-  for (int f = 0; f < 4; f++) st.F[f] = s32 + f * n64;
-  for (int f = 0; f < 2; f++) {
-    st.src[f] = s32 + (4 + f) * n64;
-    st.uh[f] = s32 + (6 + f) * n64;
-  }
-  st.trA = s32 + 8 * n64;
+  for (int f = 0; f < 3; f++) st.gvol[f] = s64 + (36 + f) * n64 + 12 * nt;
+  st.trA = s32;
   st.trS = st.trA + 2 * nt;
   // fields buffer: hbq, w0q, z0q, hbdq (blocked, nepad*ng each), fc3 (ne*3)
   size_t nq = (size_t)nepad * ng;

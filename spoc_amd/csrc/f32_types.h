@@ -38,10 +38,10 @@ __host__ __device__ constexpr int sigma_row(int i) {
   return (i & ~15) | ((i & 3) << 2) | ((i >> 2) & 3);
 }
 
-// FP32 operators, one buffer, offsets in floats.  The element kernels stage
-// [0, eg_end) (E+G) or [0, total) (B+C) in LDS.
+// FP32 operators, one buffer, offsets in floats.  B+C stages all of it in
+// LDS, E+G only [lf, tr).
 struct Pack32 {
-  int dx, dy, lf, eg_end, tr, psi, psiw, xq, yq, total;
+  int dx, dy, lf, tr, psi, psiw, xq, yq, total;
 };
 __host__ __device__ constexpr Pack32 pack32(int nm, int ng, int ne3) {
   Pack32 p{};
@@ -49,7 +49,6 @@ __host__ __device__ constexpr Pack32 pack32(int nm, int ng, int ne3) {
   p.dx = o;   o += img32_size(nm, nm);   // U^-T agx^T U^-1
   p.dy = o;   o += img32_size(nm, nm);   // U^-T agy^T U^-1
   p.lf = o;   o += img32_size(nm, ne3);  // U^-T (wif pico)^T: edge lift
-  p.eg_end = o;
   p.tr = o;   o += img32_size(ne3, nm);  // pico U^-1: traces at the edge points
   p.psi = o;  o += img32_size(ng, nm);   // pi2 U^-1: values at the quadrature points
   p.psiw = o; o += img32_size(nm, ng);   // (W pi2 U^-1)^T: projection
@@ -59,17 +58,18 @@ __host__ __device__ constexpr Pack32 pack32(int nm, int ng, int ne3) {
   return p;
 }
 
-// FP64 operators (same meaning), offsets in doubles.
+// FP64 operators (same meaning), offsets in doubles.  The face operators come
+// first: E+G and the prep kernel stage [0, dx), B+C stages [dx, total).
 struct Pack64 {
-  int dx, dy, lf, tr, total;
+  int lf, tr, dx, dy, total;
 };
 __host__ __device__ constexpr Pack64 pack64(int nm, int ne3) {
   Pack64 p{};
   int o = 0;
-  p.dx = o; o += img64_size(nm, nm);
-  p.dy = o; o += img64_size(nm, nm);
   p.lf = o; o += img64_size(nm, ne3);
   p.tr = o; o += img64_size(ne3, nm);
+  p.dx = o; o += img64_size(nm, nm);
+  p.dy = o; o += img64_size(nm, nm);
   p.total = o;
   return p;
 }
@@ -135,7 +135,7 @@ struct F32State {
   double *c0[3], *acc[3];  // [NM rows] psi coordinates: step start, RK4 sum
   double *x64[3];          // [NM rows] stage input
   double *k[3];            // [NM rows] psi tendencies (stage < 0)
-  float *F[4], *src[2], *uh[2];  // [NM rows] B+C -> E+G
+  double *gvol[3];         // [NM rows] volume and source part of the tendency (B+C -> E+G)
   float *trA;              // [2*NE3 rows] T_u, T_v of the stage input
   double *trL[2];          // [2*NE3 rows] T_eta, hh0 (T_u, T_v).n_face (wall-projected);
                            // RK4 stage s reads trL[s & 1] and writes trL[(s + 1) & 1],

@@ -22,9 +22,10 @@
 // read the same two stored numbers, so the face flux is single-valued.
 //
 // Kernels per right-hand side:  linear: lin_kernel (all FP64);  nonlinear:
-// bc_kernel (FP32: weak gradient q, quadrature-point products, mass flux)
-// then eg_kernel (FP32 divergence of the B+C results plus the FP64 linear
-// core, wall projection, RK4 stage, traces of the next stage input).
+// bc_kernel (every volume term: FP32 weak gradient q, quadrature-point
+// products, mass flux and their divergence, plus the FP64 linear core's
+// volume terms) then eg_kernel (every face term: FP32 and FP64 lifts, wall
+// projection, RK4 stage, traces of the next stage input).
 #include <hip/hip_runtime.h>
 
 #include "mfma.h"
@@ -34,25 +35,32 @@ namespace f32 {
 
 // Register budgets as waves per SIMD (512 registers per lane shared by the
 // resident waves): the element kernels are memory-latency bound, so the
-// compiler is held to at least two resident waves.
+// compiler is held to at least three resident waves.
 #ifndef BC_WAVES
 #define BC_WAVES 3
 #endif
 #ifndef EG_WAVES
-#define EG_WAVES 2
+#define EG_WAVES 3
 #endif
+// (at p >= 7, 36 modes, both kernels get two waves, or they spill)
+constexpr int bc_waves(int nc) { return nc >= 7 ? 2 : BC_WAVES; }
+constexpr int eg_waves(int nc) { return nc >= 7 ? 2 : EG_WAVES; }
 
 // Linear gravity-wave core (VAR 0: the whole linear right-hand side,
 // src/calvar_lin.f; nonlinear variants: the -g grad(eta) terms and the
 // -div(hh0 u) term, src/calvarnl_fs.f:298-321, 436-471 and the ns/inv
-// equivalents) from the FP64 stage input x64 and its traces trL.
-// G[f] receives the three tendencies (u, v, eta), result layout.
-template <int NC, int NG, int NGF, int VAR>
-__device__ __forceinline__ void core64(const F32Args &A, const double *O, int eb,
+// equivalents).  MODE 0: volume and face terms, G = ...; MODE 1: volume
+// terms only (from the FP64 stage input x64), G += ...; MODE 2: face terms
+// only (from the FP64 traces trL), G += ....  dx, dy, lf: FP64 images of
+// Dx, Dy (MODE 0, 1) and LF (MODE 0, 2).  G[f] holds the three tendencies
+// (u, v, eta) in the result layout.
+template <int NC, int NG, int NGF, int VAR, int MODE>
+__device__ __forceinline__ void core64(const F32Args &A, const double *dx, const double *dy,
+                                       const double *lf, int eb,
                                        doublex4 (&G)[3][S<NC, NG, NGF>::RBM]) {
   using Sh = S<NC, NG, NGF>;
   constexpr int NM = Sh::NM, NE3 = Sh::NE3, KM = Sh::KM, KE = Sh::KE, RBM = Sh::RBM;
-  constexpr Pack64 P = Sh::P64;
+  constexpr bool VOL = MODE != 2, LIFT = MODE != 1;
   const int l = lane(), g = l >> 4, col = l & 15, e0 = eb * 16, e = e0 + col;
   const bool valid = e < A.mesh.ne;
   const double *gg = A.mesh.geo64 + (size_t)(valid ? e : 0) * GEO;
@@ -64,75 +72,101 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *O, int eb
   const double *trL = (A.stage > 0 && (A.stage & 1)) ? A.st.trL[1] : A.st.trL[0];
   {  // u, v: -g grad(eta), volume and face terms
     double bh[KM], fx[KE], fy[KE];
-    bload<NM>(A.st.x64[2] + mb, g, col, valid, bh);
+    if (VOL) {
+      bload<NM>(A.st.x64[2] + mb, g, col, valid, bh);
 #pragma unroll
-    for (int s = 0; s < KM; s++) bh[s] *= -g0;
+      for (int s = 0; s < KM; s++) bh[s] *= -g0;
+    }
+    if (LIFT) {
 #pragma unroll
-    for (int s = 0; s < KE; s++) {
-      int kj = 4 * s + g;
-      fx[s] = fy[s] = 0.0;
-      if (kj < NE3 && valid) {
-        int k = kj / NGF, j = kj % NGF;
-        double fh = -g0 * face_avg<NGF, 2>(trL, A.mesh, e, k, j, 0);
-        fx[s] = 0.5 * fh * gg[11 + k] * gg[5 + k];
-        fy[s] = 0.5 * fh * gg[11 + k] * gg[8 + k];
+      for (int s = 0; s < KE; s++) {
+        int kj = 4 * s + g;
+        fx[s] = fy[s] = 0.0;
+        if (kj < NE3 && valid) {
+          int k = kj / NGF, j = kj % NGF;
+          double fh = -g0 * face_avg<NGF, 2>(trL, A.mesh, e, k, j, 0);
+          fx[s] = 0.5 * fh * gg[11 + k] * gg[5 + k];
+          fy[s] = 0.5 * fh * gg[11 + k] * gg[8 + k];
+        }
       }
     }
 #pragma unroll
     for (int rb = 0; rb < RBM; rb++) {
-      double adx[KM], ady[KM], al[KE];
-      afrag64<NM>(O + P.dx, rb, adx);
-      afrag64<NM>(O + P.dy, rb, ady);
-      afrag64<NE3>(O + P.lf, rb, al);
-      doublex4 Ax = chain64(adx, bh, zero64()), Ay = chain64(ady, bh, zero64());
-      doublex4 Lx = chain64(al, fx, zero64()), Ly = chain64(al, fy, zero64());
+      doublex4 Gx = zero64(), Gy = zero64();
+      if (VOL) {
+        double adx[KM], ady[KM];
+        afrag64<NM>(dx, rb, adx);
+        afrag64<NM>(dy, rb, ady);
+        doublex4 Ax = chain64(adx, bh, zero64()), Ay = chain64(ady, bh, zero64());
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          Gx[r] = -by * Ax[r] + ay * Ay[r];
+          Gy[r] = bx * Ax[r] - ax * Ay[r];
+        }
+      }
+      if (LIFT) {
+        double al[KE];
+        afrag64<NE3>(lf, rb, al);
+        Gx = chain64(al, fx, Gx);
+        Gy = chain64(al, fy, Gy);
+      }
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        G[0][rb][r] = di * (-by * Ax[r] + ay * Ay[r] + Lx[r]);
-        G[1][rb][r] = di * (bx * Ax[r] - ax * Ay[r] + Ly[r]);
+        G[0][rb][r] = (MODE != 0 ? G[0][rb][r] : 0.0) + di * Gx[r];
+        G[1][rb][r] = (MODE != 0 ? G[1][rb][r] : 0.0) + di * Gy[r];
       }
     }
   }
   phase_fence();
   {  // eta: -div(hh0 u); the nonlinear variants project the flux on walls
     double bu[KM], bv[KM], fh[KE];
-    bload<NM>(A.st.x64[0] + mb, g, col, valid, bu);
-    bload<NM>(A.st.x64[1] + mb, g, col, valid, bv);
+    if (VOL) {
+      bload<NM>(A.st.x64[0] + mb, g, col, valid, bu);
+      bload<NM>(A.st.x64[1] + mb, g, col, valid, bv);
 #pragma unroll
-    for (int s = 0; s < KM; s++) {
-      bu[s] *= hh0;
-      bv[s] *= hh0;
-    }
-    if (VAR != 0) {
-      const bool wall = valid && A.mesh.bdl[e] != 0;
-      if (__ballot(wall)) wall_pair64<NM, VAR == 2>(A.ops.gl64, wall, gg[14], gg[15], bu, bv);
-    }
+      for (int s = 0; s < KM; s++) {
+        bu[s] *= hh0;
+        bv[s] *= hh0;
+      }
+      if (VAR != 0) {
+        const bool wall = valid && A.mesh.bdl[e] != 0;
+        if (__ballot(wall)) wall_pair64<NM, VAR == 2>(A.ops.gl64, wall, gg[14], gg[15], bu, bv);
+      }
 #pragma unroll
-    for (int s = 0; s < KM; s++) {
-      double u = bu[s], v = bv[s];
-      bu[s] = by * u - bx * v;
-      bv[s] = -ay * u + ax * v;
+      for (int s = 0; s < KM; s++) {
+        double u = bu[s], v = bv[s];
+        bu[s] = by * u - bx * v;
+        bv[s] = -ay * u + ax * v;
+      }
     }
+    if (LIFT) {
 #pragma unroll
-    for (int s = 0; s < KE; s++) {
-      int kj = 4 * s + g;
-      fh[s] = 0.0;
-      if (kj < NE3 && valid) {
-        int k = kj / NGF, j = kj % NGF;
-        fh[s] = -0.5 * gg[11 + k] * face_avg<NGF, 2>(trL, A.mesh, e, k, j, 1);
+      for (int s = 0; s < KE; s++) {
+        int kj = 4 * s + g;
+        fh[s] = 0.0;
+        if (kj < NE3 && valid) {
+          int k = kj / NGF, j = kj % NGF;
+          fh[s] = -0.5 * gg[11 + k] * face_avg<NGF, 2>(trL, A.mesh, e, k, j, 1);
+        }
       }
     }
 #pragma unroll
     for (int rb = 0; rb < RBM; rb++) {
-      double adx[KM], ady[KM], al[KE];
-      afrag64<NM>(O + P.dx, rb, adx);
-      afrag64<NM>(O + P.dy, rb, ady);
-      afrag64<NE3>(O + P.lf, rb, al);
-      doublex4 Gh = chain64(adx, bu, zero64());
-      Gh = chain64(ady, bv, Gh);
-      Gh = chain64(al, fh, Gh);
+      doublex4 Gh = zero64();
+      if (VOL) {
+        double adx[KM], ady[KM];
+        afrag64<NM>(dx, rb, adx);
+        afrag64<NM>(dy, rb, ady);
+        Gh = chain64(adx, bu, Gh);
+        Gh = chain64(ady, bv, Gh);
+      }
+      if (LIFT) {
+        double al[KE];
+        afrag64<NE3>(lf, rb, al);
+        Gh = chain64(al, fh, Gh);
+      }
 #pragma unroll
-      for (int r = 0; r < 4; r++) G[2][rb][r] = di * Gh[r];
+      for (int r = 0; r < 4; r++) G[2][rb][r] = (MODE != 0 ? G[2][rb][r] : 0.0) + di * Gh[r];
     }
   }
 }
@@ -320,10 +354,10 @@ __global__ __launch_bounds__(NT) void prep_kernel(F32Args A, int nblk) {
   using Sh = S<NC, NG, NGF>;
   constexpr int NM = Sh::NM, KM = Sh::KM;
   constexpr Pack64 P = Sh::P64;
-  __shared__ floatx4 lds[P.total / 2];
+  __shared__ floatx4 lds[P.dx / 2];  // LF, TR
   const int b0 = first_block();
   if (b0 >= nblk) return;
-  stage_lds(lds, reinterpret_cast<const floatx4 *>(A.ops.pack64), P.total / 2);
+  stage_lds(lds, reinterpret_cast<const floatx4 *>(A.ops.pack64), P.dx / 2);
   __syncthreads();
   const int eb = b0 + (threadIdx.x >> 6);
   if (eb >= nblk) return;
@@ -361,31 +395,43 @@ __global__ __launch_bounds__(64 * WG) void lin_kernel(F32Args A, int nblk) {
   if (eb >= nblk) return;
   const double *O = reinterpret_cast<const double *>(lds);
   doublex4 G[3][Sh::RBM];
-  core64<NC, NG, NGF, 0>(A, O, eb, G);
+  core64<NC, NG, NGF, 0, 0>(A, O + P.dx, O + P.dy, O + P.lf, eb, G);
   phase_fence();
   finish64<NC, NG, NGF, 0>(A, O, eb, G);
 }
 
 // ---------------------------------------------------------------------------
 // Nonlinear stages B + C (src/calvarnl_fs.f:115-332 and the ns/inv
-// variants), FP32, without the linear core's terms: the viscous flux
-// F = mu0 q, the quadrature-point sources (advection, Coriolis, wind stress,
-// bottom friction) and the mass flux u (h + hb - hh0).
+// variants) and the volume half of E (src/calvarnl_fs.f:382-420): the
+// velocity gradient q, the viscous flux F = mu0 q, the quadrature-point
+// sources (advection, Coriolis, wind stress, bottom friction) and the mass
+// flux u (h + hb - hh0) in FP32, their volume divergence in FP32, plus the
+// linear core's volume terms in FP64.  Stores the sum (gvol, FP64) and the
+// flux traces (trS, FP32); the face terms are E+G's.
 // ---------------------------------------------------------------------------
 template <int NC, int NG, int NGF, int VAR, int WG>
-__global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(BC_WAVES))) void bc_kernel(F32Args A, int nblk) {
+__global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_waves(NC)))) void bc_kernel(F32Args A, int nblk) {
   using Sh = S<NC, NG, NGF>;
   constexpr int NM = Sh::NM, NE3 = Sh::NE3, KM = Sh::KM, KE = Sh::KE, KG = Sh::KG,
                 RBM = Sh::RBM, RBE = Sh::RBE, RBG = Sh::RBG;
   constexpr Pack32 P = Sh::P32;
-  __shared__ floatx4 lds[P.total / 4];
+  constexpr Pack64 P64 = Sh::P64;
+  // 16-byte words; the FP64 Dx, Dy join the FP32 operators in LDS up to
+  // 64 KiB per workgroup, else they are read through the caches
+  constexpr int N32 = P.total / 4, N64 = (P64.total - P64.dx) / 2;
+  constexpr bool D64_LDS = (N32 + N64) * 16 <= 65536;
+  __shared__ floatx4 lds[N32 + (D64_LDS ? N64 : 0)];
   const int b0 = first_block<WG>();
   if (b0 >= nblk) return;
-  stage_lds<WG>(lds, reinterpret_cast<const floatx4 *>(A.ops.pack32), P.total / 4);
+  stage_lds<WG>(lds, reinterpret_cast<const floatx4 *>(A.ops.pack32), N32);
+  if (D64_LDS)
+    stage_lds<WG>(lds + N32, reinterpret_cast<const floatx4 *>(A.ops.pack64 + P64.dx), N64);
   __syncthreads();
   const int eb = b0 + (threadIdx.x >> 6);
   if (eb >= nblk) return;
   const float *O = reinterpret_cast<const float *>(lds);
+  const double *D64 = D64_LDS ? reinterpret_cast<const double *>(lds + N32)
+                              : A.ops.pack64 + P64.dx;  // FP64 Dx, Dy
   const int l = lane(), g = l >> 4, col = l & 15, e0 = eb * 16, e = e0 + col;
   const bool valid = e < A.mesh.ne;
   const size_t mb = (size_t)e0 * NM;
@@ -464,21 +510,13 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(BC_WAVE
       }
     }
   }
-  // D: viscous flux F = mu0 q (the -g eta part belongs to the FP64 core)
-#pragma unroll
-  for (int f = 0; f < 4; f++)
-#pragma unroll
-    for (int rb = 0; rb < RBM; rb++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        int m = drow(rb, r, g);
-        if (valid && m < NM) A.st.F[f][mb + m * 16 + col] = mu0 * Q[f][rb][r];
-      }
   float q0[KM], q1[KM], q2[KM], q3[KM];
   d2b(Q[0], q0);
   d2b(Q[1], q1);
   d2b(Q[2], q2);
   d2b(Q[3], q3);
+  // D: traces of the viscous flux F = mu0 q (the -g eta part belongs to the
+  // FP64 core), normal components
 #pragma unroll
   for (int rb = 0; rb < RBE; rb++) {
     float at[KM];
@@ -498,15 +536,40 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(BC_WAVE
       }
     }
   }
+  // E, volume part for (u, v): div F with the Jacobian folded into the
+  // operands (-by Fxx + bx Fxy with Dx, ay Fxx - ax Fxy with Dy; v likewise);
+  // it starts the accumulators the projection below adds the sources to
+  floatx4 Pr[4][RBM];
+  {
+    float o0[KM], o1[KM], o2[KM], o3[KM];
+#pragma unroll
+    for (int s = 0; s < KM; s++) {
+      o0[s] = mu0 * (-by * q0[s] + bx * q1[s]);
+      o1[s] = mu0 * (ay * q0[s] - ax * q1[s]);
+      o2[s] = mu0 * (-by * q2[s] + bx * q3[s]);
+      o3[s] = mu0 * (ay * q2[s] - ax * q3[s]);
+    }
+#pragma unroll
+    for (int rb = 0; rb < RBM; rb++) {
+      float adx[KM], ady[KM];
+      afrag32<NM>(O + P.dx, rb, adx);
+      afrag32<NM>(O + P.dy, rb, ady);
+      floatx4 Vu = chain32(ady, o1, chain32(adx, o0, zero32()));
+      floatx4 Vv = chain32(ady, o3, chain32(adx, o2, zero32()));
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        Pr[0][rb][r] = di * Vu[r];
+        Pr[1][rb][r] = di * Vv[r];
+      }
+      Pr[2][rb] = zero32();
+      Pr[3][rb] = zero32();
+    }
+  }
+  phase_fence();
   // C: values at the quadrature points, products, projection
   // (src/calvarnl_fs.f:245-296, src/nonlin.f:35-133).  A product at point
   // q = 16 qb + 4 r + g is already the B operand of k-step 4 qb + r of the
   // projection.
-  floatx4 Pr[4][RBM];
-#pragma unroll
-  for (int f = 0; f < 4; f++)
-#pragma unroll
-    for (int rb = 0; rb < RBM; rb++) Pr[f][rb] = zero32();
   const float *fc3 = A.fld.fc3 + (size_t)(valid ? e : 0) * 3;
   const float fc0 = fc3[0], fc1 = fc3[1], fc2 = fc3[2];
 #pragma unroll
@@ -546,17 +609,8 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(BC_WAVE
         }
     }
   }
-  // outputs: sources, mass flux (wall-projected), its normal traces
-#pragma unroll
-  for (int rb = 0; rb < RBM; rb++)
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      int m = drow(rb, r, g);
-      if (valid && m < NM) {
-        A.st.src[0][mb + m * 16 + col] = Pr[0][rb][r];
-        A.st.src[1][mb + m * 16 + col] = Pr[1][rb][r];
-      }
-    }
+  phase_fence();
+  // mass flux (wall-projected), its normal traces and its volume divergence
   float uh[KM], vh[KM];
   d2b(Pr[2], uh);
   d2b(Pr[3], vh);
@@ -593,14 +647,6 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(BC_WAVE
     }
   }
 #pragma unroll
-  for (int s = 0; s < KM; s++) {
-    int m = 4 * s + g;
-    if (valid && m < NM) {
-      A.st.uh[0][mb + m * 16 + col] = uh[s];
-      A.st.uh[1][mb + m * 16 + col] = vh[s];
-    }
-  }
-#pragma unroll
   for (int rb = 0; rb < RBE; rb++) {
     float at[KM];
     afrag32<NM>(O + P.tr, rb, at);
@@ -616,63 +662,85 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(BC_WAVE
       }
     }
   }
+  doublex4 G[3][RBM];
+  {
+#pragma unroll
+    for (int s = 0; s < KM; s++) {
+      float u = uh[s], v = vh[s];
+      uh[s] = by * u - bx * v;
+      vh[s] = -ay * u + ax * v;
+    }
+#pragma unroll
+    for (int rb = 0; rb < RBM; rb++) {
+      float adx[KM], ady[KM];
+      afrag32<NM>(O + P.dx, rb, adx);
+      afrag32<NM>(O + P.dy, rb, ady);
+      floatx4 Vh = chain32(ady, vh, chain32(adx, uh, zero32()));
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        G[0][rb][r] = Pr[0][rb][r];
+        G[1][rb][r] = Pr[1][rb][r];
+        G[2][rb][r] = di * Vh[r];
+      }
+    }
+  }
+  phase_fence();
+  core64<NC, NG, NGF, VAR, 1>(A, D64, D64 + (P64.dy - P64.dx), nullptr, eb, G);
+#pragma unroll
+  for (int f = 0; f < 3; f++)
+#pragma unroll
+    for (int rb = 0; rb < RBM; rb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        int m = drow(rb, r, g);
+        if (valid && m < NM) A.st.gvol[f][mb + m * 16 + col] = G[f][rb][r];
+      }
 }
 
 // ---------------------------------------------------------------------------
-// Nonlinear stages E + G (src/calvarnl_fs.f:382-544): FP32 divergence of the
-// B+C results, plus the FP64 linear core, wall projection, RK4 stage.
+// Nonlinear stages E + G (src/calvarnl_fs.f:382-544): the face terms (FP32
+// lift of the flux traces, FP64 lift of the linear core's traces) added to
+// the B+C volume part, wall projection, RK4 stage, traces of the next stage
+// input.
 // ---------------------------------------------------------------------------
 template <int NC, int NG, int NGF, int VAR, int WG>
-__global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(EG_WAVES))) void eg_kernel(F32Args A, int nblk) {
+__global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(eg_waves(NC)))) void eg_kernel(F32Args A, int nblk) {
   using Sh = S<NC, NG, NGF>;
-  constexpr int NM = Sh::NM, NE3 = Sh::NE3, KM = Sh::KM, KE = Sh::KE, RBM = Sh::RBM;
+  constexpr int NM = Sh::NM, NE3 = Sh::NE3, KE = Sh::KE, RBM = Sh::RBM;
   constexpr Pack32 P = Sh::P32;
   constexpr Pack64 P64 = Sh::P64;
-  constexpr int N32 = P.eg_end / 4, N64 = P64.total / 2;  // 16-byte words
+  constexpr int N32 = (P.tr - P.lf) / 4, N64 = P64.dx / 2;  // 16-byte words
   __shared__ floatx4 lds[N32 + N64];
   const int b0 = first_block<WG>();
   if (b0 >= nblk) return;
-  stage_lds<WG>(lds, reinterpret_cast<const floatx4 *>(A.ops.pack32), N32);
+  stage_lds<WG>(lds, reinterpret_cast<const floatx4 *>(A.ops.pack32 + P.lf), N32);
   stage_lds<WG>(lds + N32, reinterpret_cast<const floatx4 *>(A.ops.pack64), N64);
   __syncthreads();
   const int eb = b0 + (threadIdx.x >> 6);
   if (eb >= nblk) return;
-  const float *O = reinterpret_cast<const float *>(lds);
-  const double *O64 = reinterpret_cast<const double *>(lds + N32);
+  const float *LF = reinterpret_cast<const float *>(lds);           // FP32 LF
+  const double *O64 = reinterpret_cast<const double *>(lds + N32);  // FP64 LF, TR
   const int l = lane(), g = l >> 4, col = l & 15, e0 = eb * 16, e = e0 + col;
   const bool valid = e < A.mesh.ne;
   const size_t mb = (size_t)e0 * NM;
 
   doublex4 G[3][RBM];
-  core64<NC, NG, NGF, VAR>(A, O64, eb, G);
+#pragma unroll
+  for (int f = 0; f < 3; f++)
+#pragma unroll
+    for (int rb = 0; rb < RBM; rb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        int m = drow(rb, r, g);
+        G[f][rb][r] = (valid && m < NM) ? A.st.gvol[f][mb + m * 16 + col] : 0.0;
+      }
+  core64<NC, NG, NGF, VAR, 2>(A, nullptr, nullptr, O64 + P64.lf, eb, G);
   phase_fence();
   {
+    // flux traces averaged over each face, times its/2; the eta equation's
+    // lift carries the minus sign
     const float *gg = A.mesh.geo + (size_t)(valid ? e : 0) * GEO;
-    const float ax = gg[0], bx = gg[1], ay = gg[2], by = gg[3], di = valid ? gg[4] : 0.f;
-    // volume operands with the Jacobian terms folded in:
-    //   u: -by Fxx + bx Fxy (with Dx), ay Fxx - ax Fxy (with Dy); v likewise;
-    //   eta: by uh - bx vh (Dx), -ay uh + ax vh (Dy)
-    float o[6][KM];
-    {
-      float fxx[KM], fxy[KM], fyx[KM], fyy[KM], uh[KM], vh[KM];
-      bload<NM>(A.st.F[0] + mb, g, col, valid, fxx);
-      bload<NM>(A.st.F[1] + mb, g, col, valid, fxy);
-      bload<NM>(A.st.F[2] + mb, g, col, valid, fyx);
-      bload<NM>(A.st.F[3] + mb, g, col, valid, fyy);
-      bload<NM>(A.st.uh[0] + mb, g, col, valid, uh);
-      bload<NM>(A.st.uh[1] + mb, g, col, valid, vh);
-#pragma unroll
-      for (int s = 0; s < KM; s++) {
-        o[0][s] = -by * fxx[s] + bx * fxy[s];
-        o[1][s] = ay * fxx[s] - ax * fxy[s];
-        o[2][s] = -by * fyx[s] + bx * fyy[s];
-        o[3][s] = ay * fyx[s] - ax * fyy[s];
-        o[4][s] = by * uh[s] - bx * vh[s];
-        o[5][s] = -ay * uh[s] + ax * vh[s];
-      }
-    }
-    // combined flux traces averaged over each face, times its/2; the eta
-    // equation's lift carries the minus sign
+    const float di = valid ? gg[4] : 0.f;
     float fb[3][KE];
 #pragma unroll
     for (int s = 0; s < KE; s++) {
@@ -688,22 +756,13 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(EG_WAVE
     }
 #pragma unroll
     for (int rb = 0; rb < RBM; rb++) {
-      float adx[KM], ady[KM], al[KE];
-      afrag32<NM>(O + P.dx, rb, adx);
-      afrag32<NM>(O + P.dy, rb, ady);
-      afrag32<NE3>(O + P.lf, rb, al);
+      float al[KE];
+      afrag32<NE3>(LF, rb, al);
 #pragma unroll
       for (int f = 0; f < 3; f++) {
-        floatx4 acc = chain32(adx, o[2 * f], zero32());
-        acc = chain32(ady, o[2 * f + 1], acc);
-        acc = chain32(al, fb[f], acc);
+        floatx4 acc = chain32(al, fb[f], zero32());
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          int m = drow(rb, r, g);
-          float src = 0.f;
-          if (f < 2 && valid && m < NM) src = A.st.src[f][mb + m * 16 + col];
-          G[f][rb][r] += (double)(src + di * acc[r]);
-        }
+        for (int r = 0; r < 4; r++) G[f][rb][r] += (double)(di * acc[r]);
       }
     }
   }
