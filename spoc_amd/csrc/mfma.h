@@ -92,39 +92,29 @@ __device__ __forceinline__ void d2b(const floatx4 (&D)[RB], float (&b)[KS]) {
   for (int s = 0; s < KS; s++) b[s] = D[s >> 2][s & 3];
 }
 
-// B fragments of an element-blocked [rows][16] array (rows >= R read as 0).
-template <int R, int KS>
-__device__ __forceinline__ void bload(const double *base, int g, int col, bool valid,
-                                      double (&b)[KS]) {
+// B fragments of an element-blocked [rows][16] array (rows >= R and
+// elements past the mesh read as 0).  The loads themselves are
+// unconditional (row clamped to R - 1; the blocked arrays are padded to
+// whole blocks of 16 elements), so they issue back to back with no branch.
+template <int R, int KS, typename T, typename V>
+__device__ __forceinline__ void bload(const T *base, int g, int col, bool valid, V (&b)[KS]) {
 #pragma unroll
   for (int s = 0; s < KS; s++) {
-    int m = 4 * s + g;
-    b[s] = (valid && m < R) ? base[m * 16 + col] : 0.0;
-  }
-}
-template <int R, int KS>
-__device__ __forceinline__ void bload(const double *base, int g, int col, bool valid,
-                                      float (&b)[KS]) {
-#pragma unroll
-  for (int s = 0; s < KS; s++) {
-    int m = 4 * s + g;
-    b[s] = (valid && m < R) ? (float)base[m * 16 + col] : 0.f;
-  }
-}
-template <int R, int KS>
-__device__ __forceinline__ void bload(const float *base, int g, int col, bool valid,
-                                      float (&b)[KS]) {
-#pragma unroll
-  for (int s = 0; s < KS; s++) {
-    int m = 4 * s + g;
-    b[s] = (valid && m < R) ? base[m * 16 + col] : 0.f;
+    const int m = 4 * s + g, mc = m < R ? m : R - 1;
+    const T v = base[mc * 16 + col];
+    b[s] = (valid && m < R) ? (V)v : V(0);
   }
 }
 
-// Row kj of field f of element e in a blocked trace array with NF fields.
+// Offset of row kj of field 0 of element e in a blocked trace array with NF
+// fields ([block][NF][NE3][16]); field f adds f * NE3 * 16.
+template <int NE3, int NF>
+__device__ __forceinline__ size_t tr_off(int e, int kj) {
+  return ((size_t)(e >> 4) * (NF * NE3) + kj) * 16 + (e & 15);
+}
 template <int NE3, int NF, typename T>
 __device__ __forceinline__ T tr_at(const T *tr, int e, int f, int kj) {
-  return tr[((size_t)(e >> 4) * (NF * NE3) + f * NE3 + kj) * 16 + (e & 15)];
+  return tr[tr_off<NE3, NF>(e, kj) + (size_t)f * NE3 * 16];
 }
 template <int NE3, int NF, typename T>
 __device__ __forceinline__ void tr_put(T *tr, int eb, int col, int f, int kj, T v) {
@@ -146,6 +136,40 @@ __device__ __forceinline__ T face_avg(const T *__restrict__ tr, const F32Mesh &M
   return T(0.5) * (a + b);
 }
 
+// The same average without branches, for the hot kernels: the neighbours of
+// the lane's element are loaded once (nbr3), each edge point resolves to the
+// two (element, row) pairs it averages (face_pt), and face_mean reads them
+// from a trace array.  A wall point averages the element's own trace on its
+// local edge 1 with itself: 0.5 (t + t) == t exactly.
+struct Nbr3 {
+  int nb[3], kn[3];  // facing element (negative: wall) and its local edge
+};
+__device__ __forceinline__ Nbr3 nbr3(const F32Mesh &M, int e) {
+  Nbr3 n;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    n.nb[k] = M.nbr[e * 3 + k];
+    n.kn[k] = M.nbe[e * 3 + k];
+  }
+  return n;
+}
+struct FacePt {
+  int ea, ka, eb, kb;
+};
+template <int NGF>
+__device__ __forceinline__ FacePt face_pt(const Nbr3 &N, int e, int kj) {
+  const int k = kj / NGF, j = kj - k * NGF;
+  const int nb = k == 0 ? N.nb[0] : (k == 1 ? N.nb[1] : N.nb[2]);
+  const int kn = k == 0 ? N.kn[0] : (k == 1 ? N.kn[1] : N.kn[2]);
+  const bool wall = nb < 0;
+  return {e, wall ? j : kj, wall ? e : nb, wall ? j : kn * NGF + (NGF - 1 - j)};
+}
+template <int NGF, int NF, typename T>
+__device__ __forceinline__ T face_mean(const T *__restrict__ tr, const FacePt &p, int f) {
+  constexpr int NE3 = 3 * NGF;
+  return T(0.5) * (tr_at<NE3, NF>(tr, p.ea, f, p.ka) + tr_at<NE3, NF>(tr, p.eb, f, p.kb));
+}
+
 // Element block of this wave.  The workgroups on XCD x (blockIdx % 8, the
 // hardware's round-robin placement) take one contiguous eighth of the
 // blocks, so the neighbour traces they read were written through the same L2.
@@ -153,6 +177,14 @@ template <int W = NW>
 __device__ __forceinline__ int first_block() {
   int g8 = gridDim.x >> 3;
   return ((blockIdx.x & 7) * g8 + (blockIdx.x >> 3)) * W;
+}
+
+// Element block of wave threadIdx.x / 64 of a workgroup whose first block is
+// b0, as a scalar (the compiler cannot see that it is wave-uniform): the
+// addresses of the element's rows then split into a scalar base and a
+// 32-bit lane offset.
+__device__ __forceinline__ int wave_block(int b0) {
+  return __builtin_amdgcn_readfirstlane(b0 + (int)(threadIdx.x >> 6));
 }
 
 // Copy n 16-byte words of operator images to LDS (whole workgroup).

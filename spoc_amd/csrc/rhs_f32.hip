@@ -56,7 +56,7 @@ constexpr int eg_waves(int nc) { return nc >= 7 ? 2 : EG_WAVES; }
 // (u, v, eta) in the result layout.
 template <int NC, int NG, int NGF, int VAR, int MODE>
 __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const double *dy,
-                                       const double *lf, int eb,
+                                       const double *lf, const Nbr3 &N, int eb,
                                        doublex4 (&G)[3][S<NC, NG, NGF>::RBM]) {
   using Sh = S<NC, NG, NGF>;
   constexpr int NM = Sh::NM, NE3 = Sh::NE3, KM = Sh::KM, KE = Sh::KE, RBM = Sh::RBM;
@@ -65,7 +65,7 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
   const bool valid = e < A.mesh.ne;
   const double *gg = A.mesh.geo64 + (size_t)(valid ? e : 0) * GEO;
   const double ax = gg[0], bx = gg[1], ay = gg[2], by = gg[3], di = valid ? gg[4] : 0.0;
-  const double g0 = A.fld.g0, hh0 = valid ? A.fld.hh0e[e] : 0.0;
+  const double g0 = A.fld.g0, hh0 = (valid ? 1.0 : 0.0) * A.fld.hh0e[valid ? e : 0];
   const size_t mb = (size_t)e0 * NM;
   // (a select, not an index: a runtime index into the kernel argument would
   // copy the whole argument block to scratch)
@@ -80,14 +80,11 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
     if (LIFT) {
 #pragma unroll
       for (int s = 0; s < KE; s++) {
-        int kj = 4 * s + g;
-        fx[s] = fy[s] = 0.0;
-        if (kj < NE3 && valid) {
-          int k = kj / NGF, j = kj % NGF;
-          double fh = -g0 * face_avg<NGF, 2>(trL, A.mesh, e, k, j, 0);
-          fx[s] = 0.5 * fh * gg[11 + k] * gg[5 + k];
-          fy[s] = 0.5 * fh * gg[11 + k] * gg[8 + k];
-        }
+        const int kj = 4 * s + g, kc = kj < NE3 ? kj : NE3 - 1, k = kc / NGF;
+        const bool ok = valid && kj < NE3;
+        const double fh = -g0 * face_mean<NGF, 2>(trL, face_pt<NGF>(N, e, kc), 0);
+        fx[s] = ok ? 0.5 * fh * gg[11 + k] * gg[5 + k] : 0.0;
+        fy[s] = ok ? 0.5 * fh * gg[11 + k] * gg[8 + k] : 0.0;
       }
     }
 #pragma unroll
@@ -129,7 +126,7 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
         bv[s] *= hh0;
       }
       if (VAR != 0) {
-        const bool wall = valid && A.mesh.bdl[e] != 0;
+        const bool wall = A.mesh.bdl[valid ? e : 0] != 0 && valid;
         if (__ballot(wall)) wall_pair64<NM, VAR == 2>(A.ops.gl64, wall, gg[14], gg[15], bu, bv);
       }
 #pragma unroll
@@ -142,12 +139,10 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
     if (LIFT) {
 #pragma unroll
       for (int s = 0; s < KE; s++) {
-        int kj = 4 * s + g;
-        fh[s] = 0.0;
-        if (kj < NE3 && valid) {
-          int k = kj / NGF, j = kj % NGF;
-          fh[s] = -0.5 * gg[11 + k] * face_avg<NGF, 2>(trL, A.mesh, e, k, j, 1);
-        }
+        const int kj = 4 * s + g, kc = kj < NE3 ? kj : NE3 - 1, k = kc / NGF;
+        const bool ok = valid && kj < NE3;
+        const double v = -0.5 * gg[11 + k] * face_mean<NGF, 2>(trL, face_pt<NGF>(N, e, kc), 1);
+        fh[s] = ok ? v : 0.0;
       }
     }
 #pragma unroll
@@ -179,18 +174,15 @@ __device__ __forceinline__ void traces_h(const F32Args &A, const double *O, int 
   using Sh = S<NC, NG, NGF>;
   constexpr int NM = Sh::NM, NE3 = Sh::NE3, KM = Sh::KM, RBE = Sh::RBE;
   const int l = lane(), g = l >> 4, col = l & 15;
-  const bool valid = eb * 16 + col < A.mesh.ne;
 #pragma unroll
   for (int rb = 0; rb < RBE; rb++) {
     double at[KM];
     afrag64<NM>(O + Sh::P64.tr, rb, at);
     doublex4 Th = chain64(at, xh, zero64());
-    if (valid) {
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        int kj = drow(rb, r, g);
-        if (kj < NE3) tr_put<NE3, 2>(w ? A.st.trL[1] : A.st.trL[0], eb, col, 0, kj, Th[r]);
-      }
+    for (int r = 0; r < 4; r++) {
+      int kj = drow(rb, r, g);
+      if (kj < NE3) tr_put<NE3, 2>(w ? A.st.trL[1] : A.st.trL[0], eb, col, 0, kj, Th[r]);
     }
   }
 }
@@ -207,30 +199,28 @@ __device__ __forceinline__ void traces_uv(const F32Args &A, const double *O, int
   const int l = lane(), g = l >> 4, col = l & 15, e = eb * 16 + col;
   const bool valid = e < A.mesh.ne;
   const double *gg = A.mesh.geo64 + (size_t)(valid ? e : 0) * GEO;
-  const double hh0 = valid ? A.fld.hh0e[e] : 0.0;
+  const double hh0 = (valid ? 1.0 : 0.0) * A.fld.hh0e[valid ? e : 0];
   double *trL = w ? A.st.trL[1] : A.st.trL[0];
 #pragma unroll
   for (int rb = 0; rb < RBE; rb++) {
     double at[KM];
     afrag64<NM>(O + Sh::P64.tr, rb, at);
     doublex4 Tu = chain64(at, xu, zero64()), Tv = chain64(at, xv, zero64());
-    if (valid) {
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        int kj = drow(rb, r, g);
-        if (kj < NE3) {
-          int k = kj / NGF;
-          if (VAR != 0) {
-            tr_put<NE3, 2>(A.st.trA, eb, col, 0, kj, (float)Tu[r]);
-            tr_put<NE3, 2>(A.st.trA, eb, col, 1, kj, (float)Tv[r]);
-          }
-          tr_put<NE3, 2>(trL, eb, col, 1, kj, hh0 * (Tu[r] * gg[5 + k] + Tv[r] * gg[8 + k]));
+    for (int r = 0; r < 4; r++) {
+      int kj = drow(rb, r, g);
+      if (kj < NE3) {
+        int k = kj / NGF;
+        if (VAR != 0) {
+          tr_put<NE3, 2>(A.st.trA, eb, col, 0, kj, (float)Tu[r]);
+          tr_put<NE3, 2>(A.st.trA, eb, col, 1, kj, (float)Tv[r]);
         }
+        tr_put<NE3, 2>(trL, eb, col, 1, kj, hh0 * (Tu[r] * gg[5 + k] + Tv[r] * gg[8 + k]));
       }
     }
   }
   if (VAR == 0) return;
-  const bool wall = valid && A.mesh.bdl[e] != 0;
+  const bool wall = A.mesh.bdl[valid ? e : 0] != 0 && valid;
   if (!__ballot(wall)) return;
   // wall elements: add the traces of (projected flux - flux) / hh0 to the
   // values this lane stored above
@@ -265,49 +255,67 @@ __device__ __forceinline__ void traces_uv(const F32Args &A, const double *O, int
   }
 }
 
-// RK4 stage update of one value (src/shwwnl.f:242-284) in FP64; stores and
-// returns the next stage input.
-__device__ __forceinline__ double rk_value(const F32Args &A, int f, size_t idx, double k) {
-  double *c0 = A.st.c0[f] + idx, *acc = A.st.acc[f] + idx;
-  double x;
-  switch (A.stage) {
-    case 0:
-      *acc = k;
-      x = *c0 + A.dt * k * 0.5;
-      break;
-    case 1:
-      *acc = *acc + 2.0 * k;
-      x = *c0 + A.dt * k * 0.5;
-      break;
-    case 2:
-      *acc = *acc + 2.0 * k;
-      x = *c0 + A.dt * k;
-      break;
-    default:
-      x = *c0 + (A.dt / 6.0) * (*acc + k);
-      *c0 = x;
-  }
-  A.st.x64[f][idx] = x;
-  return x;
-}
-
-// Tendency of field f (B fragments) -> k (stage < 0) or RK4 stage in place.
+// RK4 stage update (src/shwwnl.f:242-284) of field f, FP64: x holds the
+// stage's tendency (B fragments) and leaves as the next stage input, which
+// is stored to x64; acc gathers the weighted tendencies and c0 takes the
+// step's result at the last stage.  stage < 0: the tendency goes to k and x
+// leaves as 0.  Elements past the mesh (the padding of the last block) take
+// a zero tendency, so their stored values stay 0; rows past NM belong to the
+// next block and are not stored.  The stage is uniform over the launch, so
+// its branches are scalar and every load of a field issues at once.
 template <int NM, int KM>
 __device__ __forceinline__ void update(const F32Args &A, int f, int eb, double (&x)[KM]) {
   const int g = lane() >> 4, col = lane() & 15;
   const bool valid = eb * 16 + col < A.mesh.ne;
   const size_t mb = (size_t)eb * 16 * NM;
+  const int stage = A.stage;
+  const double dt = A.dt;
+  int ix[KM];
 #pragma unroll
   for (int s = 0; s < KM; s++) {
     const int m = 4 * s + g;
-    double v = 0.0;
-    if (valid && m < NM) {
-      size_t idx = mb + m * 16 + col;
-      if (A.stage < 0) A.st.k[f][idx] = x[s];
-      else v = rk_value(A, f, idx, x[s]);
-    }
-    x[s] = v;
+    ix[s] = (m < NM ? m : NM - 1) * 16 + col;
+    x[s] = valid ? x[s] : 0.0;
   }
+  auto store = [&](double *p, const double(&v)[KM]) {
+#pragma unroll
+    for (int s = 0; s < KM; s++)
+      if (4 * s + g < NM) p[mb + ix[s]] = v[s];
+  };
+  if (stage < 0) {
+    store(A.st.k[f], x);
+#pragma unroll
+    for (int s = 0; s < KM; s++) x[s] = 0.0;
+    return;
+  }
+  double c[KM], a[KM], y[KM];
+#pragma unroll
+  for (int s = 0; s < KM; s++) c[s] = A.st.c0[f][mb + ix[s]];
+  if (stage == 0) {
+#pragma unroll
+    for (int s = 0; s < KM; s++) {
+      a[s] = x[s];
+      y[s] = c[s] + dt * x[s] * 0.5;
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < KM; s++) a[s] = A.st.acc[f][mb + ix[s]];
+    if (stage == 3) {
+#pragma unroll
+      for (int s = 0; s < KM; s++) y[s] = c[s] + (dt / 6.0) * (a[s] + x[s]);
+    } else {
+      const double h = stage == 1 ? 0.5 : 1.0;  // (dt x) 1.0 == dt x exactly
+#pragma unroll
+      for (int s = 0; s < KM; s++) {
+        a[s] = a[s] + 2.0 * x[s];
+        y[s] = c[s] + dt * x[s] * h;
+      }
+    }
+  }
+  store(A.st.x64[f], y);
+  store(stage == 3 ? A.st.c0[f] : A.st.acc[f], stage == 3 ? y : a);
+#pragma unroll
+  for (int s = 0; s < KM; s++) x[s] = y[s];
 }
 
 // Epilogue of the element kernels: wall projection of the (u, v) tendency
@@ -336,7 +344,7 @@ __device__ __forceinline__ void finish64(const F32Args &A, const double *O, int 
     xu[s] = G[0][s >> 2][s & 3];
     xv[s] = G[1][s >> 2][s & 3];
   }
-  const bool wall = valid && A.mesh.bdl[e] != 0;
+  const bool wall = A.mesh.bdl[valid ? e : 0] != 0 && valid;
   if (__ballot(wall)) {
     const double *gw = A.mesh.geo64 + (size_t)(valid ? e : 0) * GEO;
     wall_pair64<NM, VAR == 2>(A.ops.gl64, wall, gw[14], gw[15], xu, xv);
@@ -359,7 +367,7 @@ __global__ __launch_bounds__(NT) void prep_kernel(F32Args A, int nblk) {
   if (b0 >= nblk) return;
   stage_lds(lds, reinterpret_cast<const floatx4 *>(A.ops.pack64), P.dx / 2);
   __syncthreads();
-  const int eb = b0 + (threadIdx.x >> 6);
+  const int eb = wave_block(b0);
   if (eb >= nblk) return;
   const double *O = reinterpret_cast<const double *>(lds);
   const int l = lane(), g = l >> 4, col = l & 15, e = eb * 16 + col;
@@ -372,7 +380,7 @@ __global__ __launch_bounds__(NT) void prep_kernel(F32Args A, int nblk) {
 #pragma unroll
     for (int s = 0; s < KM; s++) {
       int m = 4 * s + g;
-      if (valid && m < NM) A.st.x64[f][mb + m * 16 + col] = x[f][s];
+      if (m < NM) A.st.x64[f][mb + m * 16 + col] = x[f][s];
     }
   }
   traces_h<NC, NG, NGF>(A, O, eb, 0, x[2]);
@@ -383,7 +391,7 @@ __global__ __launch_bounds__(NT) void prep_kernel(F32Args A, int nblk) {
 // Linear right-hand side (src/calvar_lin.f:7-287) + RK4 stage, all FP64.
 // ---------------------------------------------------------------------------
 template <int NC, int NG, int NGF, int WG>
-__global__ __launch_bounds__(64 * WG) void lin_kernel(F32Args A, int nblk) {
+__global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(3))) void lin_kernel(F32Args A, int nblk) {
   using Sh = S<NC, NG, NGF>;
   constexpr Pack64 P = Sh::P64;
   __shared__ floatx4 lds[P.total / 2];
@@ -391,11 +399,13 @@ __global__ __launch_bounds__(64 * WG) void lin_kernel(F32Args A, int nblk) {
   if (b0 >= nblk) return;
   stage_lds<WG>(lds, reinterpret_cast<const floatx4 *>(A.ops.pack64), P.total / 2);
   __syncthreads();
-  const int eb = b0 + (threadIdx.x >> 6);
+  const int eb = wave_block(b0);
   if (eb >= nblk) return;
   const double *O = reinterpret_cast<const double *>(lds);
+  const int e = eb * 16 + (lane() & 15);
+  const Nbr3 N = nbr3(A.mesh, e < A.mesh.ne ? e : 0);
   doublex4 G[3][Sh::RBM];
-  core64<NC, NG, NGF, 0, 0>(A, O + P.dx, O + P.dy, O + P.lf, eb, G);
+  core64<NC, NG, NGF, 0, 0>(A, O + P.dx, O + P.dy, O + P.lf, N, eb, G);
   phase_fence();
   finish64<NC, NG, NGF, 0>(A, O, eb, G);
 }
@@ -427,7 +437,7 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
   if (D64_LDS)
     stage_lds<WG>(lds + N32, reinterpret_cast<const floatx4 *>(A.ops.pack64 + P64.dx), N64);
   __syncthreads();
-  const int eb = b0 + (threadIdx.x >> 6);
+  const int eb = wave_block(b0);
   if (eb >= nblk) return;
   const float *O = reinterpret_cast<const float *>(lds);
   const double *D64 = D64_LDS ? reinterpret_cast<const double *>(lds + N32)
@@ -438,7 +448,7 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
   const float mu0 = A.fld.mu0, mb0 = A.fld.mb0;
   const float *gg = A.mesh.geo + (size_t)(valid ? e : 0) * GEO;
   const float ax = gg[0], bx = gg[1], ay = gg[2], by = gg[3], di = valid ? gg[4] : 0.f;
-  const bool wall = valid && A.mesh.bdl[e] != 0;
+  const bool wall = A.mesh.bdl[valid ? e : 0] != 0 && valid;
   const bool anyw = __ballot(wall) != 0;
   const float nwx = wall ? gg[14] : 0.f, nwy = wall ? gg[15] : 0.f;
 
@@ -452,20 +462,19 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
   floatx4 Q[4][RBM];  // qxx, qxy, qyx, qyy
   {
     float fb[4][KE];
+    const Nbr3 N = nbr3(A.mesh, valid ? e : 0);
 #pragma unroll
     for (int s = 0; s < KE; s++) {
-      int kj = 4 * s + g;
-      fb[0][s] = fb[1][s] = fb[2][s] = fb[3][s] = 0.f;
-      if (kj < NE3 && valid) {
-        int k = kj / NGF, j = kj % NGF;
-        float sg = 0.5f * gg[11 + k], nx = sg * gg[5 + k], ny = sg * gg[8 + k];
-        float fu = face_avg<NGF, 2>(A.st.trA, A.mesh, e, k, j, 0);
-        float fv = face_avg<NGF, 2>(A.st.trA, A.mesh, e, k, j, 1);
-        fb[0][s] = fu * nx;
-        fb[1][s] = fu * ny;
-        fb[2][s] = fv * nx;
-        fb[3][s] = fv * ny;
-      }
+      const int kj = 4 * s + g, kc = kj < NE3 ? kj : NE3 - 1, k = kc / NGF;
+      const bool ok = valid && kj < NE3;
+      const float sg = 0.5f * gg[11 + k], nx = sg * gg[5 + k], ny = sg * gg[8 + k];
+      const FacePt fp = face_pt<NGF>(N, e, kc);
+      const float fu = face_mean<NGF, 2>(A.st.trA, fp, 0);
+      const float fv = face_mean<NGF, 2>(A.st.trA, fp, 1);
+      fb[0][s] = ok ? fu * nx : 0.f;
+      fb[1][s] = ok ? fu * ny : 0.f;
+      fb[2][s] = ok ? fv * nx : 0.f;
+      fb[3][s] = ok ? fv * ny : 0.f;
     }
 #pragma unroll
     for (int rb = 0; rb < RBM; rb++) {
@@ -523,16 +532,14 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
     afrag32<NM>(O + P.tr, rb, at);
     floatx4 T0 = chain32(at, q0, zero32()), T1 = chain32(at, q1, zero32());
     floatx4 T2 = chain32(at, q2, zero32()), T3 = chain32(at, q3, zero32());
-    if (valid) {
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        int kj = drow(rb, r, g);
-        if (kj < NE3) {
-          int k = kj / NGF;
-          float nx = gg[5 + k], ny = gg[8 + k];
-          tr_put<NE3, 3>(A.st.trS, eb, col, 0, kj, mu0 * (T0[r] * nx + T1[r] * ny));
-          tr_put<NE3, 3>(A.st.trS, eb, col, 1, kj, mu0 * (T2[r] * nx + T3[r] * ny));
-        }
+    for (int r = 0; r < 4; r++) {
+      int kj = drow(rb, r, g);
+      if (kj < NE3) {
+        int k = kj / NGF;
+        float nx = gg[5 + k], ny = gg[8 + k];
+        tr_put<NE3, 3>(A.st.trS, eb, col, 0, kj, mu0 * (T0[r] * nx + T1[r] * ny));
+        tr_put<NE3, 3>(A.st.trS, eb, col, 1, kj, mu0 * (T2[r] * nx + T3[r] * ny));
       }
     }
   }
@@ -583,20 +590,22 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
     float pg[4][4];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      int q = drow(qb, r, g);
-      pg[0][r] = pg[1][r] = pg[2][r] = pg[3][r] = 0.f;
-      if (q < NG && valid) {
-        size_t o = (size_t)e0 * NG + (size_t)q * 16 + col;
-        float f = fc0 + fc1 * O[P.yq + q] + fc2 * O[P.xq + q];
-        float hk = Vh[r] + A.fld.hbq[o];
-        float u = Vu[r], v = Vv[r];
-        pg[0][r] = -(u * Vxx[r] + v * (Vxy[r] - f)) + (A.fld.w0q[o] - mb0 * u) / hk;
-        pg[1][r] = -(u * (Vyx[r] + f) + v * Vyy[r]) + (A.fld.z0q[o] - mb0 * v) / hk;
-        // the flux hh0 u is the FP64 core's; only u (h + hb - hh0) here
-        float hf = Vh[r] + A.fld.hbdq[o];
-        pg[2][r] = u * hf;
-        pg[3][r] = v * hf;
-      }
+      // (rows past NG and elements past the mesh give 0; their loads are
+      // clamped into the padded arrays so that none of them branches)
+      const int q = drow(qb, r, g), qc = q < NG ? q : NG - 1;
+      const bool ok = valid && q < NG;
+      const size_t o = (size_t)e0 * NG + (size_t)qc * 16 + col;
+      const float f = fc0 + fc1 * O[P.yq + qc] + fc2 * O[P.xq + qc];
+      const float hk = Vh[r] + A.fld.hbq[o];
+      const float u = Vu[r], v = Vv[r];
+      const float p0 = -(u * Vxx[r] + v * (Vxy[r] - f)) + (A.fld.w0q[o] - mb0 * u) / hk;
+      const float p1 = -(u * (Vyx[r] + f) + v * Vyy[r]) + (A.fld.z0q[o] - mb0 * v) / hk;
+      // the flux hh0 u is the FP64 core's; only u (h + hb - hh0) here
+      const float hf = Vh[r] + A.fld.hbdq[o];
+      pg[0][r] = ok ? p0 : 0.f;
+      pg[1][r] = ok ? p1 : 0.f;
+      pg[2][r] = ok ? u * hf : 0.f;
+      pg[3][r] = ok ? v * hf : 0.f;
     }
 #pragma unroll
     for (int rb = 0; rb < RBM; rb++) {
@@ -651,14 +660,12 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
     float at[KM];
     afrag32<NM>(O + P.tr, rb, at);
     floatx4 Tu = chain32(at, uh, zero32()), Tv = chain32(at, vh, zero32());
-    if (valid) {
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        int kj = drow(rb, r, g);
-        if (kj < NE3) {
-          int k = kj / NGF;
-          tr_put<NE3, 3>(A.st.trS, eb, col, 2, kj, Tu[r] * gg[5 + k] + Tv[r] * gg[8 + k]);
-        }
+    for (int r = 0; r < 4; r++) {
+      int kj = drow(rb, r, g);
+      if (kj < NE3) {
+        int k = kj / NGF;
+        tr_put<NE3, 3>(A.st.trS, eb, col, 2, kj, Tu[r] * gg[5 + k] + Tv[r] * gg[8 + k]);
       }
     }
   }
@@ -685,7 +692,7 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
     }
   }
   phase_fence();
-  core64<NC, NG, NGF, VAR, 1>(A, D64, D64 + (P64.dy - P64.dx), nullptr, eb, G);
+  core64<NC, NG, NGF, VAR, 1>(A, D64, D64 + (P64.dy - P64.dx), nullptr, Nbr3{}, eb, G);
 #pragma unroll
   for (int f = 0; f < 3; f++)
 #pragma unroll
@@ -693,7 +700,7 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         int m = drow(rb, r, g);
-        if (valid && m < NM) A.st.gvol[f][mb + m * 16 + col] = G[f][rb][r];
+        if (m < NM) A.st.gvol[f][mb + m * 16 + col] = G[f][rb][r];
       }
 }
 
@@ -716,13 +723,14 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(eg_wave
   stage_lds<WG>(lds, reinterpret_cast<const floatx4 *>(A.ops.pack32 + P.lf), N32);
   stage_lds<WG>(lds + N32, reinterpret_cast<const floatx4 *>(A.ops.pack64), N64);
   __syncthreads();
-  const int eb = b0 + (threadIdx.x >> 6);
+  const int eb = wave_block(b0);
   if (eb >= nblk) return;
   const float *LF = reinterpret_cast<const float *>(lds);           // FP32 LF
   const double *O64 = reinterpret_cast<const double *>(lds + N32);  // FP64 LF, TR
   const int l = lane(), g = l >> 4, col = l & 15, e0 = eb * 16, e = e0 + col;
   const bool valid = e < A.mesh.ne;
   const size_t mb = (size_t)e0 * NM;
+  const Nbr3 N = nbr3(A.mesh, valid ? e : 0);
 
   doublex4 G[3][RBM];
 #pragma unroll
@@ -731,10 +739,11 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(eg_wave
     for (int rb = 0; rb < RBM; rb++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        int m = drow(rb, r, g);
-        G[f][rb][r] = (valid && m < NM) ? A.st.gvol[f][mb + m * 16 + col] : 0.0;
+        const int m = drow(rb, r, g), mc = m < NM ? m : NM - 1;
+        const double v = A.st.gvol[f][mb + mc * 16 + col];
+        G[f][rb][r] = (valid && m < NM) ? v : 0.0;
       }
-  core64<NC, NG, NGF, VAR, 2>(A, nullptr, nullptr, O64 + P64.lf, eb, G);
+  core64<NC, NG, NGF, VAR, 2>(A, nullptr, nullptr, O64 + P64.lf, N, eb, G);
   phase_fence();
   {
     // flux traces averaged over each face, times its/2; the eta equation's
@@ -744,15 +753,16 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(eg_wave
     float fb[3][KE];
 #pragma unroll
     for (int s = 0; s < KE; s++) {
-      int kj = 4 * s + g;
-      fb[0][s] = fb[1][s] = fb[2][s] = 0.f;
-      if (kj < NE3 && valid) {
-        int k = kj / NGF, j = kj % NGF;
-        float sg = 0.5f * gg[11 + k];
-        fb[0][s] = sg * face_avg<NGF, 3>(A.st.trS, A.mesh, e, k, j, 0);
-        fb[1][s] = sg * face_avg<NGF, 3>(A.st.trS, A.mesh, e, k, j, 1);
-        fb[2][s] = -sg * face_avg<NGF, 3>(A.st.trS, A.mesh, e, k, j, 2);
-      }
+      const int kj = 4 * s + g, kc = kj < NE3 ? kj : NE3 - 1;
+      const bool ok = valid && kj < NE3;
+      const float sg = 0.5f * gg[11 + kc / NGF];
+      const FacePt fp = face_pt<NGF>(N, e, kc);
+      const float f0 = sg * face_mean<NGF, 3>(A.st.trS, fp, 0);
+      const float f1 = sg * face_mean<NGF, 3>(A.st.trS, fp, 1);
+      const float f2 = -sg * face_mean<NGF, 3>(A.st.trS, fp, 2);
+      fb[0][s] = ok ? f0 : 0.f;
+      fb[1][s] = ok ? f1 : 0.f;
+      fb[2][s] = ok ? f2 : 0.f;
     }
 #pragma unroll
     for (int rb = 0; rb < RBM; rb++) {
