@@ -53,11 +53,13 @@ constexpr int eg_waves(int nc) { return nc >= 7 ? 2 : EG_WAVES; }
 // terms only (from the FP64 stage input x64), G += ...; MODE 2: face terms
 // only (from the FP64 traces trL), G += ....  dx, dy, lf: FP64 images of
 // Dx, Dy (MODE 0, 1) and LF (MODE 0, 2).  G[f] holds the three tendencies
-// (u, v, eta) in the result layout.
+// (u, v, eta) in the result layout.  X, when given, holds the stage input
+// (u, v, eta B fragments) already loaded by the caller.
 template <int NC, int NG, int NGF, int VAR, int MODE>
 __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const double *dy,
                                        const double *lf, const Nbr3 &N, int eb,
-                                       doublex4 (&G)[3][S<NC, NG, NGF>::RBM]) {
+                                       doublex4 (&G)[3][S<NC, NG, NGF>::RBM],
+                                       const double (*X)[S<NC, NG, NGF>::KM] = nullptr) {
   using Sh = S<NC, NG, NGF>;
   constexpr int NM = Sh::NM, NE3 = Sh::NE3, KM = Sh::KM, KE = Sh::KE, RBM = Sh::RBM;
   constexpr bool VOL = MODE != 2, LIFT = MODE != 1;
@@ -73,7 +75,12 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
   {  // u, v: -g grad(eta), volume and face terms
     double bh[KM], fx[KE], fy[KE];
     if (VOL) {
-      bload<NM>(A.st.x64[2] + mb, g, col, valid, bh);
+      if (X) {
+#pragma unroll
+        for (int s = 0; s < KM; s++) bh[s] = X[2][s];
+      } else {
+        bload<NM>(A.st.x64[2] + mb, g, col, valid, bh);
+      }
 #pragma unroll
       for (int s = 0; s < KM; s++) bh[s] *= -g0;
     }
@@ -118,8 +125,16 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
   {  // eta: -div(hh0 u); the nonlinear variants project the flux on walls
     double bu[KM], bv[KM], fh[KE];
     if (VOL) {
-      bload<NM>(A.st.x64[0] + mb, g, col, valid, bu);
-      bload<NM>(A.st.x64[1] + mb, g, col, valid, bv);
+      if (X) {
+#pragma unroll
+        for (int s = 0; s < KM; s++) {
+          bu[s] = X[0][s];
+          bv[s] = X[1][s];
+        }
+      } else {
+        bload<NM>(A.st.x64[0] + mb, g, col, valid, bu);
+        bload<NM>(A.st.x64[1] + mb, g, col, valid, bv);
+      }
 #pragma unroll
       for (int s = 0; s < KM; s++) {
         bu[s] *= hh0;
@@ -619,6 +634,11 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
     }
   }
   phase_fence();
+  // the FP64 stage input for the core's volume terms, loaded here so that
+  // its latency hides behind the mass-flux work below
+  double X[3][KM];
+#pragma unroll
+  for (int f = 0; f < 3; f++) bload<NM>(A.st.x64[f] + mb, g, col, valid, X[f]);
   // mass flux (wall-projected), its normal traces and its volume divergence
   float uh[KM], vh[KM];
   d2b(Pr[2], uh);
@@ -692,7 +712,7 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
     }
   }
   phase_fence();
-  core64<NC, NG, NGF, VAR, 1>(A, D64, D64 + (P64.dy - P64.dx), nullptr, Nbr3{}, eb, G);
+  core64<NC, NG, NGF, VAR, 1>(A, D64, D64 + (P64.dy - P64.dx), nullptr, Nbr3{}, eb, G, X);
 #pragma unroll
   for (int f = 0; f < 3; f++)
 #pragma unroll
