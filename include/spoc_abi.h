@@ -213,6 +213,30 @@ int spoc_set_tiles(spoc_ctx *ctx, int variant, int tile0, int tile1, int autotun
 int spoc_sync(spoc_ctx *ctx);
 
 /*
+ * Refinement indicator of the device-resident state (checkconv,
+ * src/adapt.f:1260-1332), FP64, bit-identical to the reference: for field
+ * 0 (u), 1 (v) or 2 (h), amax[e] (ne values) = the largest |difference| of
+ * the two traces at the ngf points of e's faces that have a second
+ * element, divided by max |a(1,.)| of the field over e's mesh (ensemble:
+ * over e's member), or 0 when that maximum is <= 1e-6.
+ */
+int spoc_checkconv(spoc_ctx *ctx, int field, double *amax);
+
+/*
+ * Refinement decisions (cal_conv, src/adapt.f:1122-1192) on the
+ * device-resident state, from the three indicators of spoc_checkconv:
+ * fconv[e] = 1 if some indicator of e exceeds fac1, fderaff[e] = 1 if all
+ * three are below fac2 (ne values each).  Per mesh (ensemble: per member,
+ * nmem values each): deraff = some fderaff, supmax = the largest indicator,
+ * timedir = (supmax <= fac3) (0 asks the driver to roll back), conv =
+ * (supmax > fac1 or deraff).  LOGICALs as 0/1.  supmax and amax3 (3 x ne,
+ * the indicators of u, v, h) may be NULL.
+ */
+int spoc_cal_conv(spoc_ctx *ctx, double fac1, double fac2, double fac3, int *fconv,
+                  int *fderaff, int *conv, int *deraff, int *timedir, double *supmax,
+                  double *amax3);
+
+/*
  * Global integrals of the device-resident state (src/energy.f:3-96), FP64:
  * kinetic 1/2 int (h+H)|u|^2, potential 1/2 g int h^2, mass int h.  Needs the
  * node arrays passed to spoc_mesh_set.

@@ -128,6 +128,32 @@ class Oracle:
                          *self._args(m, p), C.c_double(dt), nsteps)
         return u, v, h
 
+    def checkconv(self, m, a0):
+        """Edge-jump indicator of one field (src/adapt.f:1260-1332)."""
+        ld, ne = a0.shape
+        amax = np.zeros(ne)
+        a0 = np.asfortranarray(a0)
+        self.lib.orc_checkconv(self._tab, m["nf"], ne, ld, _d(a0), _i(m["tseg"]), _i(m["ssit"]),
+                               _d(amax))
+        return amax
+
+    def cal_conv(self, m, u0, v0, h0, fac1, fac2, fac3):
+        """Refinement decisions (src/adapt.f:1122-1192): dict of fconv,
+        fderaff (0/1 per element), conv, deraff, timedir (0/1), supmax and
+        the three indicators amax (3, ne)."""
+        ld, ne = u0.shape
+        u0, v0, h0 = (np.asfortranarray(a) for a in (u0, v0, h0))
+        fconv, fderaff = np.zeros(ne, np.int32), np.zeros(ne, np.int32)
+        conv, deraff, timedir = C.c_int(), C.c_int(), C.c_int()
+        sup = C.c_double()
+        amax = np.zeros((3, ne))
+        self.lib.orc_cal_conv(self._tab, m["nf"], ne, ld, _d(u0), _d(v0), _d(h0), _i(m["tseg"]),
+                              _i(m["ssit"]), C.c_double(fac1), C.c_double(fac2),
+                              C.c_double(fac3), _i(fconv), _i(fderaff), C.byref(conv),
+                              C.byref(deraff), C.byref(timedir), C.byref(sup), _d(amax))
+        return {"fconv": fconv, "fderaff": fderaff, "conv": conv.value, "deraff": deraff.value,
+                "timedir": timedir.value, "supmax": sup.value, "amax": amax}
+
     def energy(self, m, p, u0, v0, h0):
         ld, ne = u0.shape
         ke, pe, mass = C.c_double(), C.c_double(), C.c_double()
@@ -249,3 +275,45 @@ class RefCM:
         tri = np.array(tri, dtype=np.int32, order="F", copy=True)
         self.lib.ref_changemaille(nn, tri.shape[1], _i(tri))
         return tri
+
+
+class RefAdapt:
+    """The reference's AMR decision routines checkconv and cal_conv
+    (src/adapt.f), compiled from its sources.  Modal arrays are (45, ne)
+    Fortran-ordered; pico is the (9, 45, 3) table."""
+
+    def __init__(self):
+        self.lib = C.CDLL(os.path.join(REF_DIR, "libspoc_ref_adapt.so"), mode=os.RTLD_LOCAL)
+
+    @staticmethod
+    def available() -> bool:
+        return os.path.exists(os.path.join(REF_DIR, "libspoc_ref_adapt.so"))
+
+    @staticmethod
+    def _pad(a):
+        out = np.zeros((LDM, a.shape[1]), order="F")
+        out[:a.shape[0]] = a
+        return out
+
+    def checkconv(self, nc, ng, ngf, m, pico, a0):
+        ne = a0.shape[1]
+        a0 = self._pad(a0)
+        pico = np.asfortranarray(pico)
+        amax = np.zeros(ne)
+        self.lib.ref_checkconv(nc, ng, ngf, ne, m["nf"], _d(a0), _d(pico), _i(m["tseg"]),
+                               _i(m["ssit"]), _d(amax))
+        return amax
+
+    def cal_conv(self, nc, ng, ngf, m, pico, u0, v0, h0, fac1, fac2, fac3):
+        ne = u0.shape[1]
+        u0, v0, h0 = (self._pad(a) for a in (u0, v0, h0))
+        pico = np.asfortranarray(pico)
+        fconv, fderaff = np.zeros(ne, np.int32), np.zeros(ne, np.int32)
+        conv, deraff, timedir = C.c_int(), C.c_int(), C.c_int()
+        self.lib.ref_cal_conv(nc, ng, ngf, ne, m["nf"], _d(pico), _i(m["tseg"]), _i(m["ssit"]),
+                              _d(u0), _d(v0), _d(h0), C.c_double(fac1), C.c_double(fac2),
+                              C.c_double(fac3), _i(fconv), _i(fderaff), C.byref(conv),
+                              C.byref(deraff), C.byref(timedir))
+        return {"fconv": fconv, "fderaff": fderaff, "conv": conv.value, "deraff": deraff.value,
+                "timedir": timedir.value}
+

@@ -829,3 +829,81 @@ void orc_energy(const orc_tables *t, int ne, int ld, const double *u0, const dou
 }
 
 size_t orc_tables_size(void) { return sizeof(orc_tables); }
+
+/* Edge-jump indicator of one modal field: src/adapt.f:1260-1332
+ * (checkconv).  For every face with two elements, the field's traces at
+ * the ngf points from both sides (the second reversed), |difference|, and
+ * amax(e) = the largest over e's faces and points; then amax /= max_e
+ * |a0(1,e)| when that exceeds 1e-6, else amax = 0. */
+void orc_checkconv(const orc_tables *t, int nf, int ne, int ld, const double *a0,
+                   const int *tseg, const int *ssit, double *amax) {
+  int nm = t->nm, ngf = t->ngf;
+  double fbdy1[ORC_LDF], fbdy2[ORC_LDF];
+  for (int i = 0; i < ne; i++) amax[i] = 0.0;
+  for (int i = 0; i < nf; i++) {
+    for (int j = 0; j < ngf; j++) fbdy1[j] = fbdy2[j] = 0.0;
+    int k1 = tseg[2 * i] - 1, k2 = tseg[2 * i + 1] - 1;
+    if (tseg[2 * i + 1] > 0) {
+      int c = ssit[2 * i] - 1;
+      for (int m = 0; m < nm; m++)
+        for (int j = 0; j < ngf; j++) fbdy1[j] = fbdy1[j] + a0[m + ld * k1] * PICO(t, j, m, c);
+      c = ssit[2 * i + 1] - 1;
+      for (int m = 0; m < nm; m++)
+        for (int j = 0; j < ngf; j++)
+          fbdy2[j] = fbdy2[j] + a0[m + ld * k2] * PICO(t, ngf - 1 - j, m, c);
+      for (int j = 0; j < ngf; j++) {
+        double dif = fabs(fbdy2[j] - fbdy1[j]);
+        if (amax[k1] < dif) amax[k1] = dif;
+        if (amax[k2] < dif) amax[k2] = dif;
+      }
+    }
+  }
+  double vmax = fabs(a0[0]);
+  for (int i = 1; i < ne; i++) {
+    double v = fabs(a0[ld * i]);
+    vmax = v > vmax ? v : vmax;
+  }
+  if (vmax > 1.e-6) {
+    for (int i = 0; i < ne; i++) amax[i] = amax[i] / vmax;
+  } else {
+    for (int i = 0; i < ne; i++) amax[i] = 0.0;
+  }
+}
+
+/* Refinement decisions: src/adapt.f:1122-1192 (cal_conv).  fconv(e): some
+ * field's jump exceeds fac1; fderaff(e): every field's jump is below fac2;
+ * deraff: some fderaff; supmax: the largest jump; timedir = supmax <= fac3
+ * (false asks the driver to roll back); conv = supmax > fac1 or deraff.
+ * amax3 (3*ne, may be NULL) receives the three indicators (u, v, h). */
+void orc_cal_conv(const orc_tables *t, int nf, int ne, int ld, const double *u0,
+                  const double *v0, const double *h0, const int *tseg, const int *ssit,
+                  double fac1, double fac2, double fac3, int *fconv, int *fderaff, int *conv,
+                  int *deraff, int *timedir, double *supmax, double *amax3) {
+  double *um = malloc(sizeof(double) * 3 * (size_t)(ne > 0 ? ne : 1));
+  double *vm = um + ne, *hm = vm + ne;
+  for (int i = 0; i < ne; i++) fconv[i] = fderaff[i] = 0;
+  *conv = 0;
+  *deraff = 0;
+  orc_checkconv(t, nf, ne, ld, u0, tseg, ssit, um);
+  orc_checkconv(t, nf, ne, ld, v0, tseg, ssit, vm);
+  orc_checkconv(t, nf, ne, ld, h0, tseg, ssit, hm);
+  double sup = um[0];
+  for (int i = 0; i < ne; i++) {
+    if (um[i] > fac1 || vm[i] > fac1 || hm[i] > fac1) fconv[i] = 1;
+    if (um[i] < fac2 && vm[i] < fac2 && hm[i] < fac2) {
+      fderaff[i] = 1;
+      *deraff = 1;
+    }
+    sup = um[i] > sup ? um[i] : sup;
+    sup = vm[i] > sup ? vm[i] : sup;
+    sup = hm[i] > sup ? hm[i] : sup;
+  }
+  *timedir = sup > fac3 ? 0 : 1;
+  if (sup > fac1) *conv = 1;
+  if (*deraff) *conv = 1;
+  *supmax = sup;
+  if (amax3)
+    for (int i = 0; i < 3 * ne; i++) amax3[i] = um[i];
+  free(um);
+}
+

@@ -73,6 +73,14 @@ void orc_energy(const orc_tables *t, int ne, int ld, const double *u0, const dou
                 const double *h0, double g0, const double *hb, const int *in, const double *x,
                 const double *y, double *ke, double *pe, double *mass);
 
+/* checkconv and cal_conv (src/adapt.f:1260-1332, 1122-1192); flags as 0/1 ints. */
+void orc_checkconv(const orc_tables *t, int nf, int ne, int ld, const double *a0,
+                   const int *tseg, const int *ssit, double *amax);
+void orc_cal_conv(const orc_tables *t, int nf, int ne, int ld, const double *u0,
+                  const double *v0, const double *h0, const int *tseg, const int *ssit,
+                  double fac1, double fac2, double fac3, int *fconv, int *fderaff, int *conv,
+                  int *deraff, int *timedir, double *supmax, double *amax3);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,7 +50,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                  "spoc_ensemble_mesh_set", "spoc_ensemble_energy", "spoc_rhs_adjoint",
                  "spoc_adjoint_upload", "spoc_adjoint_download", "spoc_rk4_adjoint_steps",
                  "spoc_autotune", "spoc_set_tiles", "spoc_adjoint_hb_download",
-                 "spoc_set_checkpoint"):
+                 "spoc_set_checkpoint", "spoc_checkconv", "spoc_cal_conv"):
         getattr(lib, name).restype = C.c_int
     lib.spoc_ctx_destroy.argtypes = [C.c_void_p]
     _lib = lib
@@ -108,6 +108,7 @@ class Context:
                                         _d(t["pico"]), 9, LDM, _d(t["amm"]), _d(t["amml"]),
                                         _d(t["agx"]), _d(t["agy"]), _d(t["ammo"])))
         self.ne = self.nf = 0
+        self.nmem = 1
 
     def close(self):
         if self._h:
@@ -139,6 +140,7 @@ class Context:
             len(x) if have_nodes else 0, _i(tri) if have_nodes else None,
             _d(x) if have_nodes else None, _d(y) if have_nodes else None))
         self.ne, self.nf = ne, nf
+        self.nmem = 1
 
     def set_ensemble_mesh(self, meshes) -> None:
         """Several member meshes (dicts as for set_mesh) run as one union mesh;
@@ -261,6 +263,31 @@ class Context:
 
     def set_profiling(self, on: bool) -> None:
         _check(self.lib.spoc_set_profiling(self._h, int(on)))
+
+    # -- the adaptive driver's refinement indicator ------------------------
+    def checkconv(self, field):
+        """Jump indicator of field 0 (u), 1 (v) or 2 (h) of the resident
+        state, per element (checkconv, src/adapt.f:1260-1332)."""
+        amax = np.zeros(self.ne)
+        _check(self.lib.spoc_checkconv(self._h, int(field), _d(amax)))
+        return amax
+
+    def cal_conv(self, fac1, fac2, fac3):
+        """Refinement decisions of the resident state (cal_conv,
+        src/adapt.f:1122-1192): dict with per-element fconv, fderaff (0/1)
+        and amax (3, ne), and conv, deraff, timedir (0/1) and supmax, one
+        per ensemble member (scalars for a single mesh)."""
+        ne, n = self.ne, self.nmem
+        fconv, fderaff = np.zeros(ne, np.int32), np.zeros(ne, np.int32)
+        conv, deraff, timedir = (np.zeros(n, np.int32) for _ in range(3))
+        sup, amax = np.zeros(n), np.zeros((3, ne))
+        _check(self.lib.spoc_cal_conv(self._h, C.c_double(fac1), C.c_double(fac2),
+                                      C.c_double(fac3), _i(fconv), _i(fderaff), _i(conv),
+                                      _i(deraff), _i(timedir), _d(sup), _d(amax)))
+        r = {"fconv": fconv, "fderaff": fderaff, "amax": amax}
+        for k, v in (("conv", conv), ("deraff", deraff), ("timedir", timedir), ("supmax", sup)):
+            r[k] = v if n > 1 else v[0].item()
+        return r
 
     def autotune(self, variant):
         """Time the element kernels at 2, 4 and 8 waves per workgroup on this

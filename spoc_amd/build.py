@@ -38,6 +38,7 @@ def _units():
         units.append(("adjoint.hip", f"adj_f64_{tag}.o", d))
     units.append(("f32_plan.cpp", "f32_plan.o", ["-x", "hip"]))
     units.append(("energy.hip", "energy.o", EXACT))
+    units.append(("adapt.hip", "adapt.o", EXACT))
     units.append(("abi.cpp", "abi.o", EXACT + ["-x", "hip"]))
     return units
 

@@ -19,6 +19,10 @@
 using namespace spoc;
 
 namespace spoc {
+int adapt_launch(hipStream_t st, int ne, int nf, int nm, int ngf, int nmem, const double *const *a,
+                 const int *tseg, const int *ssit, const int *itm, const double *pico,
+                 const int *off, double *work, double *amax, bool decide, double fac1,
+                 double fac2, int *flags, double *out);
 int energy_launch(hipStream_t st, int ne, int nm, int ng, const double *u, const double *v,
                   const double *h, const double *hb, const double *pi2, const double *wi,
                   const double *ammo1, const double *deta, double g0, int nr, const int *off,
@@ -161,6 +165,7 @@ struct spoc_ctx {
   double *dface = nullptr;
   double **dFptr = nullptr;
   double *den = nullptr;     // energy scratch
+  double *dadapt = nullptr;  // refinement indicator scratch (adapt_launch)
   // element offsets of the meshes run together: {0, ne} for one mesh, or the
   // members of an ensemble (spoc_ensemble_mesh_set); device copy holds
   // {0, ne, mem_e0...}
@@ -212,6 +217,8 @@ void free_mesh(spoc_ctx *c) {
   if (c->dstate) (void)hipFree(c->dstate);
   if (c->dface) (void)hipFree(c->dface);
   if (c->den) (void)hipFree(c->den);
+  if (c->dadapt) (void)hipFree(c->dadapt);
+  c->dadapt = nullptr;
   if (c->dfields) (void)hipFree(c->dfields);
   if (c->dFptr) (void)hipFree(c->dFptr);
   c->dFptr = nullptr;
@@ -516,6 +523,10 @@ static int mesh_set(spoc_ctx *c, int ne, int nf, const int *itm, const int *its,
   HIPCHK(hipMalloc((void **)&c->dface, sizeof(double) * (size_t)nf * 6 * ngf));
   const int nmem = (int)mem_e0.size() - 1;
   HIPCHK(hipMalloc((void **)&c->den, sizeof(double) * ((size_t)ne * (2 + nm) + 3 * (nmem + 1) + 8)));
+  // fmax (3 nf), vmax (3 nmem), member results (2 nmem), indicators (3 ne),
+  // flags (2 ne ints)
+  HIPCHK(hipMalloc((void **)&c->dadapt,
+                   sizeof(double) * (3 * (size_t)nf + 5 * (size_t)nmem + 4 * (size_t)ne + 8)));
   std::vector<int> off = {0, ne};
   off.insert(off.end(), mem_e0.begin(), mem_e0.end());
   HIPCHK(hipMalloc((void **)&c->dmem, sizeof(int) * off.size()));
@@ -1010,6 +1021,68 @@ int spoc_energy(spoc_ctx *c, double *ke, double *pe, double *mass) {
   *ke = r[0];
   *pe = r[1];
   *mass = r[2];
+  return SPOC_OK;
+}
+
+// The refinement indicator of the resident state (adapt_launch); on return
+// the normalised indicators are at c->dadapt + 3 nf + 5 nmem (3 ne), the
+// flags after them, the member results at c->dadapt + 3 nf + 3 nmem.
+static int adapt_run(spoc_ctx *c, bool decide, double fac1, double fac2) {
+  const int nmem = (int)c->mem_e0.size() - 1;
+  double *work = c->dadapt, *out = work + 3 * (size_t)c->nf + 3 * (size_t)nmem;
+  double *amax = out + 2 * (size_t)nmem;
+  int *flags = (int *)(amax + 3 * (size_t)c->ne);
+  const double *a[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
+  if (adapt_launch(c->stream, c->ne, c->nf, c->nm, c->ngf, nmem, a, c->mesh.tseg, c->mesh.ssit,
+                   c->mesh.itm, c->tab.pico, c->dmem + 2, work, amax, decide, fac1, fac2, flags,
+                   out))
+    return fail(SPOC_EHIP, "refinement indicator kernel launch failed");
+  return SPOC_OK;
+}
+
+int spoc_checkconv(spoc_ctx *c, int field, double *amax) {
+  if (!c || !c->have_state) return fail(SPOC_ESTATE, "spoc_checkconv: no state");
+  if (field < 0 || field > 2 || !amax)
+    return fail(SPOC_EINVAL, "spoc_checkconv: field=%d (0 u, 1 v, 2 h)", field);
+  (void)hipSetDevice(c->device);
+  int rc = adapt_run(c, false, 0.0, 0.0);
+  if (rc) return rc;
+  const size_t nmem = c->mem_e0.size() - 1;
+  const double *d = c->dadapt + 3 * (size_t)c->nf + 5 * nmem + (size_t)field * c->ne;
+  HIPCHK(hipMemcpyAsync(amax, d, sizeof(double) * c->ne, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SPOC_OK;
+}
+
+int spoc_cal_conv(spoc_ctx *c, double fac1, double fac2, double fac3, int *fconv, int *fderaff,
+                  int *conv, int *deraff, int *timedir, double *supmax, double *amax3) {
+  if (!c || !c->have_state) return fail(SPOC_ESTATE, "spoc_cal_conv: no state");
+  if (!fconv || !fderaff || !conv || !deraff || !timedir)
+    return fail(SPOC_EINVAL, "spoc_cal_conv: null output");
+  (void)hipSetDevice(c->device);
+  int rc = adapt_run(c, true, fac1, fac2);
+  if (rc) return rc;
+  const int nmem = (int)c->mem_e0.size() - 1;
+  const size_t ne = c->ne;
+  const double *out = c->dadapt + 3 * (size_t)c->nf + 3 * (size_t)nmem;
+  const double *amax = out + 2 * (size_t)nmem;
+  const int *flags = (const int *)(amax + 3 * ne);
+  std::vector<double> r(2 * (size_t)nmem);
+  HIPCHK(hipMemcpyAsync(r.data(), out, sizeof(double) * r.size(), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipMemcpyAsync(fconv, flags, sizeof(int) * ne, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(fderaff, flags + ne, sizeof(int) * ne, hipMemcpyDeviceToHost, c->stream));
+  if (amax3)
+    HIPCHK(hipMemcpyAsync(amax3, amax, sizeof(double) * 3 * ne, hipMemcpyDeviceToHost,
+                          c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int m = 0; m < nmem; m++) {  // src/adapt.f:1180-1189
+    const double sup = r[2 * m];
+    deraff[m] = r[2 * m + 1] > 0.0;
+    timedir[m] = sup > fac3 ? 0 : 1;
+    conv[m] = (sup > fac1 || deraff[m]) ? 1 : 0;
+    if (supmax) supmax[m] = sup;
+  }
   return SPOC_OK;
 }
 

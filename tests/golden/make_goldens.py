@@ -19,6 +19,11 @@ Fixtures:
   calvar_p<p>.npz      one cal_var per variant on seeded random states
   rk4_p3.npz           10 RK4 steps per variant from the config-1 initial state,
                        with energy/mass before and after
+  calconv_p<p>.npz     checkconv indicators and cal_conv decisions
+                       (src/adapt.f:1122-1332) on seeded random states, for
+                       threshold sets that hit every branch
+
+    python tests/golden/make_goldens.py [/root/reference] [--only-calconv]
 """
 from __future__ import annotations
 
@@ -79,9 +84,47 @@ def rule(q, nc, ng, ngf):
     return t[0], t[1], t[2], g[0], g[1]
 
 
-def main(refroot="/root/reference"):
+def calconv_goldens(q):
+    """cal_conv on random states of random_case meshes.  Threshold sets:
+    'mid' puts fac1 and fac2 exactly on indicator values (the tests are
+    strict) and fac3 above the maximum; 'rollback' puts fac3 below it with
+    no refinement or coarsening; 'flat' zeroes h, whose indicator then
+    vanishes (max |h(1,e)| <= 1e-6)."""
+    A = oracle.RefAdapt()
+    for nc, ng, ngf, nx, ny in ((3, 12, 4, 17, 17), (6, 33, 7, 9, 9)):
+        tb = _tables_for(q, nc, ng, ngf)
+        nm = tb["nm"]
+        c = configs.random_case(tb, nx, ny, seed=31 + nc, scale=0.01)
+        m = c["mesh"]
+        u0, v0, h0 = (a[:nm] for a in c["state"])
+        pico = tb["pico"]
+        amax = np.array([A.checkconv(nc, ng, ngf, m, pico, a) for a in (u0, v0, h0)])
+        top = np.sort(amax.max(axis=0))
+        sup = float(amax.max())
+        sets = {"mid": (float(top[int(0.7 * len(top))]), float(top[int(0.3 * len(top))]),
+                        2.0 * sup, (u0, v0, h0)),
+                "rollback": (10.0 * sup, 0.0, 0.5 * sup, (u0, v0, h0)),
+                "flat": (float(top[len(top) // 2]), float(np.sort(amax[:2].min(axis=0))[len(top) // 2]),
+                         2.0 * sup, (u0, v0, np.zeros_like(h0)))}
+        out = dict(nx=nx, ny=ny, seed=31 + nc, u0=u0, v0=v0, h0=h0, amax=amax)
+        for name, (f1, f2, f3, st) in sets.items():
+            r = A.cal_conv(nc, ng, ngf, m, pico, *st, f1, f2, f3)
+            out[f"{name}_fac"] = np.array([f1, f2, f3])
+            for k in ("fconv", "fderaff"):
+                out[f"{name}_{k}"] = r[k]
+            out[f"{name}_flags"] = np.array([r["conv"], r["deraff"], r["timedir"]], np.int32)
+            if name == "flat":
+                out["flat_amax_h"] = A.checkconv(nc, ng, ngf, m, pico, st[2])
+        np.savez_compressed(os.path.join(OUT, f"calconv_p{nc}.npz"), **out)
+
+
+def main(refroot="/root/reference", *flags):
     oracle.build(ref=True)
     q = quad_tables(refroot)
+    if "--only-calconv" in flags:
+        calconv_goldens(q)
+        print("cal_conv goldens written to", OUT)
+        return
     for nc, ng, ngf in ((3, 12, 4), (6, 33, 7), (4, 25, 5), (5, 25, 6)):
         R = oracle.RefLib("fs")
         i1, i2 = R.setup(nc, *rule(q, nc, ng, ngf))
@@ -144,6 +187,7 @@ def main(refroot="/root/reference"):
         out[f"{var}_energy"] = np.array([e0, e1])
     np.savez_compressed(os.path.join(OUT, "rk4_p3.npz"), dt=c["dt"], mu0=100.0,
                         u0=c["state"][0][:10], v0=c["state"][1][:10], h0=c["state"][2][:10], **out)
+    calconv_goldens(q)
     print("goldens written to", OUT)
 
 
