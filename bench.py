@@ -186,6 +186,50 @@ def calib_copy(nbytes):
         raise RuntimeError("calibration copy failed")
 
 
+def adapt_timing(ctx, state, ne, nc, ng, ngf, cpu, reps=20, fac=(0.1, 0.01, 1.0)):
+    """The adaptive driver's refinement indicator and decisions (cal_conv,
+    src/adapt.f:1122-1332) on the given state: wall time per call of
+    spoc_cal_conv, flags copied to the host; with cpu, the reference
+    Fortran's indicator (oracle/_ref) timed on the c2 generator's
+    1152-triangle mesh, one core, scaled per element."""
+    ctx.upload(*state)
+    ctx.cal_conv(*fac)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        r = ctx.cal_conv(*fac, with_amax=False)
+    dev = (time.perf_counter() - t0) / reps
+    out = {"us_per_call": dev * 1e6, "ne": ne, "calls": reps,
+           "decisions": {k: int(r[k]) for k in ("conv", "deraff", "timedir")} if ctx.nmem == 1
+           else None, "fac": list(fac)}
+    if not cpu:
+        return out
+    import oracle
+    from spoc_amd import configs
+    if not oracle.RefAdapt.available() or ng > 33:
+        return out
+    O = oracle.Oracle()
+    O.setup(nc, *quad_rule(ng, ngf))
+    tabs = O.tables()
+    w, x, y, wf, tf = quad_rule(ng, ngf)
+    tabs.update(nc=nc, ng=ng, ngf=ngf, nm=O.nm, wi=w, xi=x, yi=y, wif=wf, tif=tf)
+    c = configs.config2(tabs, nx=25, ny=25)
+    A = oracle.RefAdapt()
+    m, st = c["mesh"], [a[:O.nm] for a in c["state"]]
+    # (the three checkconv calls are cal_conv's work; cal_conv itself also
+    # prints its decisions to stdout, which would mix with this JSON line)
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < 2.0:
+        for a in st:
+            A.checkconv(nc, ng, ngf, m, tabs["pico"], a)
+        n += 1
+    per_elem = (time.perf_counter() - t0) / n / m["ne"]
+    out["cpu_reference"] = {"us_per_call_at_this_ne": per_elem * ne * 1e6, "cores": 1,
+                            "sample": f"reference checkconv of u, v, h, p={nc} on {m['ne']} "
+                                      f"triangles, {n} calls, scaled per element"}
+    return out
+
+
 def cpu_baseline(variant, nc, ng, ngf, budget_s=15.0, wl="c2"):
     """The reference Fortran (oracle/_ref) on a bounded sample of the same
     workload: same p, rule and physics on a reduced mesh (at most 1600
@@ -399,6 +443,7 @@ def main():
                                       "eta": mrel(s32[2], s64[2]), "steps": K}
     if args.adjoint and wl == "c4":
         out["adjoint"] = adjoint_check(ctx, t, c, K, args.adjoint)
+    out["cal_conv"] = adapt_timing(ctx, s32, ne, nc, ng, ngf, cpu=not args.no_cpu)
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(variant, nc, ng, ngf, wl=wl)

@@ -272,18 +272,21 @@ class Context:
         _check(self.lib.spoc_checkconv(self._h, int(field), _d(amax)))
         return amax
 
-    def cal_conv(self, fac1, fac2, fac3):
+    def cal_conv(self, fac1, fac2, fac3, with_amax=True):
         """Refinement decisions of the resident state (cal_conv,
         src/adapt.f:1122-1192): dict with per-element fconv, fderaff (0/1)
-        and amax (3, ne), and conv, deraff, timedir (0/1) and supmax, one
-        per ensemble member (scalars for a single mesh)."""
+        and amax (3, ne; None without with_amax), and conv, deraff, timedir
+        (0/1) and supmax, one per ensemble member (scalars for a single
+        mesh)."""
         ne, n = self.ne, self.nmem
         fconv, fderaff = np.zeros(ne, np.int32), np.zeros(ne, np.int32)
         conv, deraff, timedir = (np.zeros(n, np.int32) for _ in range(3))
-        sup, amax = np.zeros(n), np.zeros((3, ne))
+        sup = np.zeros(n)
+        amax = np.zeros((3, ne)) if with_amax else None
         _check(self.lib.spoc_cal_conv(self._h, C.c_double(fac1), C.c_double(fac2),
                                       C.c_double(fac3), _i(fconv), _i(fderaff), _i(conv),
-                                      _i(deraff), _i(timedir), _d(sup), _d(amax)))
+                                      _i(deraff), _i(timedir), _d(sup),
+                                      _d(amax) if with_amax else None))
         r = {"fconv": fconv, "fderaff": fderaff, "amax": amax}
         for k, v in (("conv", conv), ("deraff", deraff), ("timedir", timedir), ("supmax", sup)):
             r[k] = v if n > 1 else v[0].item()

@@ -19,10 +19,12 @@
 using namespace spoc;
 
 namespace spoc {
-int adapt_launch(hipStream_t st, int ne, int nf, int nm, int ngf, int nmem, const double *const *a,
-                 const int *tseg, const int *ssit, const int *itm, const double *pico,
-                 const int *off, double *work, double *amax, bool decide, double fac1,
-                 double fac2, int *flags, double *out);
+int adapt_launch(hipStream_t st, int nc, int ngf, int ne, int nmem, int nch,
+                 const double *const *a, const int4 *side, const double *pico,
+                 const int *off, const int *cst, const int *cof,
+                 double *work, double *T, double *amax, bool decide, double fac1, double fac2,
+                 int *flags);
+int adapt_chunk();
 int energy_launch(hipStream_t st, int ne, int nm, int ng, const double *u, const double *v,
                   const double *h, const double *hb, const double *pi2, const double *wi,
                   const double *ammo1, const double *deta, double g0, int nr, const int *off,
@@ -166,6 +168,9 @@ struct spoc_ctx {
   double **dFptr = nullptr;
   double *den = nullptr;     // energy scratch
   double *dadapt = nullptr;  // refinement indicator scratch (adapt_launch)
+  int *dchunk = nullptr;     // its reduction chunks: starts (nch + 1), members' first (nmem + 1)
+  int4 *dside = nullptr;     // its face per element side: (k1, c1, k2, c2), 0-based
+  int nch = 0;
   // element offsets of the meshes run together: {0, ne} for one mesh, or the
   // members of an ensemble (spoc_ensemble_mesh_set); device copy holds
   // {0, ne, mem_e0...}
@@ -219,6 +224,10 @@ void free_mesh(spoc_ctx *c) {
   if (c->den) (void)hipFree(c->den);
   if (c->dadapt) (void)hipFree(c->dadapt);
   c->dadapt = nullptr;
+  if (c->dchunk) (void)hipFree(c->dchunk);
+  c->dchunk = nullptr;
+  if (c->dside) (void)hipFree(c->dside);
+  c->dside = nullptr;
   if (c->dfields) (void)hipFree(c->dfields);
   if (c->dFptr) (void)hipFree(c->dFptr);
   c->dFptr = nullptr;
@@ -523,10 +532,32 @@ static int mesh_set(spoc_ctx *c, int ne, int nf, const int *itm, const int *its,
   HIPCHK(hipMalloc((void **)&c->dface, sizeof(double) * (size_t)nf * 6 * ngf));
   const int nmem = (int)mem_e0.size() - 1;
   HIPCHK(hipMalloc((void **)&c->den, sizeof(double) * ((size_t)ne * (2 + nm) + 3 * (nmem + 1) + 8)));
-  // fmax (3 nf), vmax (3 nmem), member results (2 nmem), indicators (3 ne),
-  // flags (2 ne ints)
-  HIPCHK(hipMalloc((void **)&c->dadapt,
-                   sizeof(double) * (3 * (size_t)nf + 5 * (size_t)nmem + 4 * (size_t)ne + 8)));
+  // refinement indicator: reduction chunks of consecutive elements inside
+  // one member; scratch = partial and member maxima (5 nch + 5 nmem),
+  // indicators (3 ne), flags (2 ne ints)
+  {
+    std::vector<int> cst, cof;
+    for (int r = 0; r < nmem; r++) {
+      cof.push_back((int)cst.size());
+      for (int e = mem_e0[r]; e < mem_e0[r + 1]; e += adapt_chunk()) cst.push_back(e);
+    }
+    c->nch = (int)cst.size();
+    cof.push_back(c->nch);
+    cst.push_back(ne);
+    cst.insert(cst.end(), cof.begin(), cof.end());
+    HIPCHK(hipMalloc((void **)&c->dchunk, sizeof(int) * cst.size()));
+    HIPCHK(hipMemcpy(c->dchunk, cst.data(), sizeof(int) * cst.size(), hipMemcpyHostToDevice));
+    std::vector<int4> side(3 * (size_t)ne);
+    for (size_t t = 0; t < side.size(); t++) {
+      const int i = itm[t] - 1, k2 = tseg[2 * i + 1] - 1;
+      side[t] = make_int4(tseg[2 * i] - 1, ssit[2 * i] - 1, k2, k2 >= 0 ? ssit[2 * i + 1] - 1 : 0);
+    }
+    HIPCHK(hipMalloc((void **)&c->dside, sizeof(int4) * side.size()));
+    HIPCHK(hipMemcpy(c->dside, side.data(), sizeof(int4) * side.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc((void **)&c->dadapt,
+                     sizeof(double) * (5 * (size_t)c->nch + 5 * (size_t)nmem + 4 * (size_t)ne +
+                                       9 * (size_t)ngf * ne + 8)));
+  }
   std::vector<int> off = {0, ne};
   off.insert(off.end(), mem_e0.begin(), mem_e0.end());
   HIPCHK(hipMalloc((void **)&c->dmem, sizeof(int) * off.size()));
@@ -1024,19 +1055,35 @@ int spoc_energy(spoc_ctx *c, double *ke, double *pe, double *mass) {
   return SPOC_OK;
 }
 
-// The refinement indicator of the resident state (adapt_launch); on return
-// the normalised indicators are at c->dadapt + 3 nf + 5 nmem (3 ne), the
-// flags after them, the member results at c->dadapt + 3 nf + 3 nmem.
+// The refinement indicator of the resident state (adapt_launch).  Layout of
+// c->dadapt: partial and member maxima (5 nch + 5 nmem doubles; the
+// per-member supmax and deraff at 5 nch + 3 nmem), normalised indicators
+// (3 ne), flags (2 ne ints), traces (9 ngf ne).
+struct AdaptBufs {
+  double *work, *res, *amax, *T;
+  int *flags;
+};
+static AdaptBufs adapt_bufs(spoc_ctx *c) {
+  const size_t nmem = c->mem_e0.size() - 1, ne = c->ne, nch = c->nch;
+  AdaptBufs b;
+  b.work = c->dadapt;
+  b.res = b.work + 5 * nch + 3 * nmem;
+  b.amax = b.work + 5 * nch + 5 * nmem;
+  b.flags = (int *)(b.amax + 3 * ne);
+  b.T = b.amax + 4 * ne;
+  return b;
+}
+
 static int adapt_run(spoc_ctx *c, bool decide, double fac1, double fac2) {
   const int nmem = (int)c->mem_e0.size() - 1;
-  double *work = c->dadapt, *out = work + 3 * (size_t)c->nf + 3 * (size_t)nmem;
-  double *amax = out + 2 * (size_t)nmem;
-  int *flags = (int *)(amax + 3 * (size_t)c->ne);
+  AdaptBufs b = adapt_bufs(c);
   const double *a[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
-  if (adapt_launch(c->stream, c->ne, c->nf, c->nm, c->ngf, nmem, a, c->mesh.tseg, c->mesh.ssit,
-                   c->mesh.itm, c->tab.pico, c->dmem + 2, work, amax, decide, fac1, fac2, flags,
-                   out))
-    return fail(SPOC_EHIP, "refinement indicator kernel launch failed");
+  const int rc = adapt_launch(c->stream, c->nc, c->ngf, c->ne, nmem, c->nch, a, c->dside,
+                              c->tab.pico, c->dmem + 2, c->dchunk,
+                              c->dchunk + c->nch + 1, b.work, b.T, b.amax, decide, fac1, fac2,
+                              b.flags);
+  if (rc == -1) return fail(SPOC_EINVAL, "refinement indicator: no kernel for this shape");
+  if (rc) return fail(SPOC_EHIP, "refinement indicator kernel launch failed");
   return SPOC_OK;
 }
 
@@ -1047,8 +1094,7 @@ int spoc_checkconv(spoc_ctx *c, int field, double *amax) {
   (void)hipSetDevice(c->device);
   int rc = adapt_run(c, false, 0.0, 0.0);
   if (rc) return rc;
-  const size_t nmem = c->mem_e0.size() - 1;
-  const double *d = c->dadapt + 3 * (size_t)c->nf + 5 * nmem + (size_t)field * c->ne;
+  const double *d = adapt_bufs(c).amax + (size_t)field * c->ne;
   HIPCHK(hipMemcpyAsync(amax, d, sizeof(double) * c->ne, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return SPOC_OK;
@@ -1064,21 +1110,20 @@ int spoc_cal_conv(spoc_ctx *c, double fac1, double fac2, double fac3, int *fconv
   if (rc) return rc;
   const int nmem = (int)c->mem_e0.size() - 1;
   const size_t ne = c->ne;
-  const double *out = c->dadapt + 3 * (size_t)c->nf + 3 * (size_t)nmem;
-  const double *amax = out + 2 * (size_t)nmem;
-  const int *flags = (const int *)(amax + 3 * ne);
+  AdaptBufs b = adapt_bufs(c);
   std::vector<double> r(2 * (size_t)nmem);
-  HIPCHK(hipMemcpyAsync(r.data(), out, sizeof(double) * r.size(), hipMemcpyDeviceToHost,
+  HIPCHK(hipMemcpyAsync(r.data(), b.res, sizeof(double) * r.size(), hipMemcpyDeviceToHost,
                         c->stream));
-  HIPCHK(hipMemcpyAsync(fconv, flags, sizeof(int) * ne, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipMemcpyAsync(fderaff, flags + ne, sizeof(int) * ne, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(fconv, b.flags, sizeof(int) * ne, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(fderaff, b.flags + ne, sizeof(int) * ne, hipMemcpyDeviceToHost,
+                        c->stream));
   if (amax3)
-    HIPCHK(hipMemcpyAsync(amax3, amax, sizeof(double) * 3 * ne, hipMemcpyDeviceToHost,
+    HIPCHK(hipMemcpyAsync(amax3, b.amax, sizeof(double) * 3 * ne, hipMemcpyDeviceToHost,
                           c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   for (int m = 0; m < nmem; m++) {  // src/adapt.f:1180-1189
-    const double sup = r[2 * m];
-    deraff[m] = r[2 * m + 1] > 0.0;
+    const double sup = r[m];
+    deraff[m] = r[nmem + m] > 0.0;
     timedir[m] = sup > fac3 ? 0 : 1;
     conv[m] = (sup > fac1 || deraff[m]) ? 1 : 0;
     if (supmax) supmax[m] = sup;
