@@ -45,6 +45,7 @@ def build(ref: bool = False) -> None:
     if ref:
         subprocess.run(["make", "-s", "-j8", "-C", HERE, "ref"], check=True)
         subprocess.run(["make", "-s", "-j8", "-C", HERE, "drivers"], check=True)
+        subprocess.run(["make", "-s", "-j8", "-C", HERE, "adaptive"], check=True)
 
 
 def ref_available() -> bool:

@@ -70,7 +70,8 @@ def build(jobs: int | None = None, verbose: bool = True) -> str:
 
 
 def build_fortran(verbose: bool = True) -> None:
-    """Compile the ISO_C_BINDING drop-in cal_var objects (one per variant)."""
+    """Compile the ISO_C_BINDING drop-in objects: cal_var (one per variant)
+    and the adaptive driver's cal_conv."""
     fc = shutil.which("flang") or shutil.which("amdflang")
     if not fc:
         if verbose:
@@ -85,6 +86,9 @@ def build_fortran(verbose: bool = True) -> None:
         out = os.path.join(fdir, f"spoc_calvar_{v}.o")
         subprocess.run([fc, "-O2", "-fPIC", f"-DSPOC_VARIANT={code}", "-J", OBJ, "-c", src,
                         "-o", out], check=True)
+    # the adaptive driver's cal_conv (uses the spoc_abi module written above)
+    subprocess.run([fc, "-O2", "-fPIC", "-J", OBJ, "-c", os.path.join(fdir, "spoc_calconv.F90"),
+                    "-o", os.path.join(fdir, "spoc_calconv.o")], check=True)
     if verbose:
         print("built Fortran drop-in objects in", fdir)
 

@@ -21,6 +21,10 @@
 module spoc_abi
   use iso_c_binding
   implicit none
+  ! the device context, shared by the cal_var and cal_conv shims, and the
+  ! element count of the mesh it holds
+  type(c_ptr), save :: spoc_ctx_h = c_null_ptr
+  integer, save :: spoc_ctx_ne = -1
   interface
     integer(c_int) function spoc_ctx_create(ctx, device, nc, ng, ngf, wi, wif, pi2, ld_g, pico, &
          ld_f, ld_m, amm, amml, agx, agy, ammo) bind(C, name="spoc_ctx_create")
@@ -78,6 +82,14 @@ module spoc_abi
       type(c_ptr), value :: ctx
       real(c_double) :: ke, pe, mass
     end function
+    integer(c_int) function spoc_cal_conv(ctx, fac1, fac2, fac3, fconv, fderaff, conv, deraff, &
+         timedir, supmax, amax3) bind(C, name="spoc_cal_conv")
+      import
+      type(c_ptr), value :: ctx
+      real(c_double), value :: fac1, fac2, fac3
+      integer(c_int) :: fconv(*), fderaff(*), conv(*), deraff(*), timedir(*)
+      type(c_ptr), value :: supmax, amax3
+    end function
     type(c_ptr) function spoc_last_error() bind(C, name="spoc_last_error")
       import
     end function
@@ -123,7 +135,6 @@ subroutine cal_var(nf, ne, u0, v0, h0, u1, v1, h1, tseg, pseg, ssit, itm, its, n
   double precision u0(nnmod, *), v0(nnmod, *), h0(nnmod, *), u1(nnmod, *), v1(nnmod, *), &
        h1(nnmod, *), norm(2, *), vect(5, *), normt(2, *), fc(nnmod, *), w0(nnmod, *), &
        z0(nnmod, *), hb(nnmod, *), mu0, mb0, beta, g0, time
-  type(c_ptr), save :: ctx = c_null_ptr
   integer, save :: last_ne = -1, last_nf = -1, prec = 0
   integer(c_int64_t), save :: last_sum = -1
   double precision, save :: last_par(4) = -1.d300
@@ -131,13 +142,13 @@ subroutine cal_var(nf, ne, u0, v0, h0, u1, v1, h1, tseg, pseg, ssit, itm, its, n
   integer(c_int64_t) :: s
   integer :: i, dev, stat, l
   character(len=16) :: env
-  if (.not. c_associated(ctx)) then
+  if (.not. c_associated(spoc_ctx_h)) then
     dev = 0
     call get_environment_variable('SPOC_DEVICE', env, l, stat)
     if (stat == 0 .and. l > 0) read(env, *) dev
     call get_environment_variable('SPOC_PRECISION', env, l, stat)
     if (stat == 0 .and. l > 0 .and. env(1:4) == 'fp32') prec = 1
-    call spoc_check(spoc_ctx_create(ctx, dev, nc, ng, ngf, wi, wif, pi2, ng_max, pico, ngf_max, &
+    call spoc_check(spoc_ctx_create(spoc_ctx_h, dev, nc, ng, ngf, wi, wif, pi2, ng_max, pico, ngf_max, &
          nnmod, amm, amml, agx, agy, ammo), 'spoc_ctx_create')
   end if
   ! connectivity checksum: detects remeshing between calls
@@ -156,8 +167,9 @@ subroutine cal_var(nf, ne, u0, v0, h0, u1, v1, h1, tseg, pseg, ssit, itm, its, n
       ibdl(i) = 0
       if (bdl(i)) ibdl(i) = 1
     end do
-    call spoc_check(spoc_mesh_set(ctx, ne, nf, itm, its, tseg, ssit, norm, vect, normt, ibdl, &
-         0, c_null_ptr, c_null_ptr, c_null_ptr), 'spoc_mesh_set')
+    call spoc_check(spoc_mesh_set(spoc_ctx_h, ne, nf, itm, its, tseg, ssit, norm, vect, normt, &
+         ibdl, 0, c_null_ptr, c_null_ptr, c_null_ptr), 'spoc_mesh_set')
+    spoc_ctx_ne = ne
     last_ne = ne
     last_nf = nf
     last_sum = s
@@ -165,8 +177,10 @@ subroutine cal_var(nf, ne, u0, v0, h0, u1, v1, h1, tseg, pseg, ssit, itm, its, n
   end if
   if (last_par(1) /= mu0 .or. last_par(2) /= mb0 .or. last_par(3) /= g0 .or. &
        last_par(4) /= hb(1, 1)) then
-    call spoc_check(spoc_fields_set(ctx, nnmod, fc, w0, z0, hb, mu0, mb0, g0), 'spoc_fields_set')
+    call spoc_check(spoc_fields_set(spoc_ctx_h, nnmod, fc, w0, z0, hb, mu0, mb0, g0), &
+         'spoc_fields_set')
     last_par = [mu0, mb0, g0, hb(1, 1)]
   end if
-  call spoc_check(spoc_rhs(ctx, SPOC_VARIANT, prec, nnmod, u0, v0, h0, u1, v1, h1), 'spoc_rhs')
+  call spoc_check(spoc_rhs(spoc_ctx_h, SPOC_VARIANT, prec, nnmod, u0, v0, h0, u1, v1, h1), &
+       'spoc_rhs')
 end subroutine

@@ -50,3 +50,20 @@ def test_unsupported_shape_rejected_before_device_use():
     t = spoc_amd.build_tables(3, *quad_rule(25, 4))
     with pytest.raises(spoc_amd.SpocError, match="unsupported"):
         spoc_amd.Context(t)
+
+
+def test_adaptive_driver_links_the_dropin_cal_conv():
+    """The reference's adaptive driver (src/shwwnl_adap.f) built with the
+    drop-in objects (oracle/Makefile, target adaptive): its cal_conv is the
+    ISO_C_BINDING shim, which calls spoc_cal_conv from libspoc_amd.so, and
+    adapt.f's own cal_conv is present only under its renamed symbol."""
+    import shutil
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", "shwwnl_adap_fs_spoc")
+    if not os.path.exists(exe) or not shutil.which("nm"):
+        pytest.skip("adaptive driver not built (make -C oracle adaptive)")
+    syms = subprocess.run(["nm", exe], capture_output=True, text=True, check=True).stdout
+    defined = {ln.split()[-1] for ln in syms.splitlines() if " T " in ln}
+    undefined = {ln.split()[-1] for ln in syms.splitlines() if ln.strip().startswith("U ")}
+    assert {"cal_conv_", "cal_conv_reference_", "cal_var_", "checkconv_"} <= defined
+    assert "spoc_cal_conv" in undefined and "spoc_rhs" in undefined
