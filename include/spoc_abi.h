@@ -59,8 +59,11 @@ enum {
   SPOC_EINVAL = -1,   /* bad argument / unsupported (p, ng, ngf) */
   SPOC_EHIP = -2,     /* HIP runtime error */
   SPOC_ESTATE = -3,   /* call out of order (e.g. no mesh yet) */
-  SPOC_ESINGULAR = -4 /* mass matrix not positive definite (reference only
-                         prints "pb avec matrice", src/shwwnl.f:153-156) */
+  SPOC_ESINGULAR = -4, /* mass matrix not positive definite (reference only
+                          prints "pb avec matrice", src/shwwnl.f:153-156) */
+  SPOC_EMESH = -5       /* mesh rejected: a triangle with two wall faces or a
+                          wall face that is not its edge 1 (src/cmal_s.f:222-277),
+                          or inconsistent connectivity */
 };
 
 typedef struct spoc_ctx spoc_ctx;
@@ -211,6 +214,31 @@ int spoc_set_tiles(spoc_ctx *ctx, int variant, int tile0, int tile1, int autotun
 
 /* Wait for all work queued on the context's stream. */
 int spoc_sync(spoc_ctx *ctx);
+
+/*
+ * Mesh operands of cal_var from the nodes and triangles, host side, as the
+ * reference driver computes them (src/shwwnl.f:162-174): face numbering
+ * pointer_mesh (src/cmal_s.f:164-208), check_mesh (src/cmal_s.f:222-277,
+ * skipped when check = 0), cal_rotnf (src/mat1.f:7-23), cal_its
+ * (src/mat1.f:28-64), calpre's geometry and wall flags (src/shwwnl.f:350-422)
+ * and coriolis (src/mat1.f:69-83).  in (3, ne) 1-based CCW triangles; x, y
+ * (nn).  Outputs (caller-allocated; nf <= 3 ne): *nf; itm, its (3, ne);
+ * tseg, pseg, ssit (2, nf); norm (2, nf); vect (5, ne); normt (2, ne);
+ * bdl (ne, 0/1); fc (ld, ne) with ld >= nm, or NULL; deta (ne) or NULL.
+ * No device work.  SPOC_EMESH if the mesh is rejected.
+ */
+int spoc_mesh_build(int nn, int ne, const int *in, const double *x, const double *y, int nc,
+                    double f0, double beta, int ld, int check, int *nf, int *itm, int *its,
+                    int *tseg, int *pseg, int *ssit, double *norm, double *vect, double *normt,
+                    int *bdl, double *fc, double *deta);
+
+/*
+ * The CM tool (src/chmal.f:52-142), in place on in (3, ne): flip the
+ * diagonal of triangles with two wall faces, then rotate each wall
+ * triangle's local numbering so that its wall face is its edge 1.  Host
+ * side.
+ */
+int spoc_mesh_changemaille(int nn, int ne, int *in);
 
 /*
  * Refinement indicator of the device-resident state (checkconv,

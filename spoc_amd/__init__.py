@@ -50,7 +50,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                  "spoc_ensemble_mesh_set", "spoc_ensemble_energy", "spoc_rhs_adjoint",
                  "spoc_adjoint_upload", "spoc_adjoint_download", "spoc_rk4_adjoint_steps",
                  "spoc_autotune", "spoc_set_tiles", "spoc_adjoint_hb_download",
-                 "spoc_set_checkpoint", "spoc_checkconv", "spoc_cal_conv"):
+                 "spoc_set_checkpoint", "spoc_checkconv", "spoc_cal_conv", "spoc_mesh_build",
+                 "spoc_mesh_changemaille"):
         getattr(lib, name).restype = C.c_int
     lib.spoc_ctx_destroy.argtypes = [C.c_void_p]
     _lib = lib

@@ -39,6 +39,7 @@ def _units():
     units.append(("f32_plan.cpp", "f32_plan.o", ["-x", "hip"]))
     units.append(("energy.hip", "energy.o", EXACT))
     units.append(("adapt.hip", "adapt.o", EXACT))
+    units.append(("mesh.cpp", "mesh.o", EXACT))
     units.append(("abi.cpp", "abi.o", EXACT + ["-x", "hip"]))
     return units
 
