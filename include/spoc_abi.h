@@ -61,9 +61,10 @@ enum {
   SPOC_ESTATE = -3,   /* call out of order (e.g. no mesh yet) */
   SPOC_ESINGULAR = -4, /* mass matrix not positive definite (reference only
                           prints "pb avec matrice", src/shwwnl.f:153-156) */
-  SPOC_EMESH = -5       /* mesh rejected: a triangle with two wall faces or a
+  SPOC_EMESH = -5,      /* mesh rejected: a triangle with two wall faces or a
                           wall face that is not its edge 1 (src/cmal_s.f:222-277),
                           or inconsistent connectivity */
+  SPOC_EFILE = -6       /* file missing, unreadable or not in the expected format */
 };
 
 typedef struct spoc_ctx spoc_ctx;
@@ -239,6 +240,33 @@ int spoc_mesh_build(int nn, int ne, const int *in, const double *x, const double
  * side.
  */
 int spoc_mesh_changemaille(int nn, int ne, int *in);
+
+/*
+ * The driver's state files init.bin / temp.bin, host side: 7 Fortran
+ * unformatted sequential records (4-byte little-endian byte counts around
+ * each): time, then u, v, h, w, z, hb as (nm, ne) element-major
+ * (src/shwwnl.f:180-187 reads them, src/shwwnl_adap.f:703-724 writes them).
+ * Arrays are (ld, ne) column-major, ld >= nm; on read any of the six may be
+ * NULL to skip it.  Records of 2 GiB or more are not supported (SPOC_EFILE).
+ */
+int spoc_state_file_read(const char *path, int nm, int ne, int ld, double *time, double *u0,
+                         double *v0, double *h0, double *w0, double *z0, double *hb);
+int spoc_state_file_write(const char *path, int nm, int ne, int ld, double time,
+                          const double *u0, const double *v0, const double *h0,
+                          const double *w0, const double *z0, const double *hb);
+
+/*
+ * The mesh file cm.dat, host side (src/cmal_s.f:7-48 reads it, 56-78 writes
+ * it): "nn ne", then nn lines "i x y", then ne lines "e n1 n2 n3",
+ * list-directed.  Read multiplies the coordinates by scx, scy as read_mesh
+ * does; write divides by them and prints 17 significant digits, which read
+ * back to the same doubles.  spoc_mesh_file_size reads the header only.
+ */
+int spoc_mesh_file_size(const char *path, int *nn, int *ne);
+int spoc_mesh_file_read(const char *path, int nn, int ne, double scx, double scy, double *x,
+                        double *y, int *in);
+int spoc_mesh_file_write(const char *path, int nn, int ne, const double *x, const double *y,
+                         const int *in, double scx, double scy);
 
 /*
  * Refinement indicator of the device-resident state (checkconv,

@@ -51,7 +51,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                  "spoc_adjoint_upload", "spoc_adjoint_download", "spoc_rk4_adjoint_steps",
                  "spoc_autotune", "spoc_set_tiles", "spoc_adjoint_hb_download",
                  "spoc_set_checkpoint", "spoc_checkconv", "spoc_cal_conv", "spoc_mesh_build",
-                 "spoc_mesh_changemaille"):
+                 "spoc_mesh_changemaille", "spoc_state_file_read", "spoc_state_file_write",
+                 "spoc_mesh_file_size", "spoc_mesh_file_read", "spoc_mesh_file_write"):
         getattr(lib, name).restype = C.c_int
     lib.spoc_ctx_destroy.argtypes = [C.c_void_p]
     _lib = lib
@@ -75,6 +76,51 @@ def _i(a: np.ndarray):
 
 def nm_of(nc: int) -> int:
     return (nc + 1) * (nc + 2) // 2
+
+
+# -- the driver's files (host side, no device) --------------------------------
+def read_state_file(path: str, nm: int, ne: int, ld: int = LDM):
+    """init.bin / temp.bin (src/shwwnl.f:180-187): (time, u, v, h, w, z, hb)
+    with (ld, ne) Fortran-ordered arrays."""
+    lib = load_library()
+    out = [np.zeros((ld, ne), order="F") for _ in range(6)]
+    t = C.c_double()
+    _check(lib.spoc_state_file_read(os.fsencode(path), nm, ne, ld, C.byref(t),
+                                    *(_d(a) for a in out)))
+    return (t.value, *out)
+
+
+def write_state_file(path: str, nm: int, time: float, u, v, h, w, z, hb) -> None:
+    """Write a state file the reference driver reads as init.bin
+    (save_temp, src/shwwnl_adap.f:703-724)."""
+    lib = load_library()
+    arrs = [np.asfortranarray(a, np.float64) for a in (u, v, h, w, z, hb)]
+    ld, ne = arrs[0].shape
+    _check(lib.spoc_state_file_write(os.fsencode(path), nm, ne, ld, C.c_double(time),
+                                     *(_d(a) for a in arrs)))
+
+
+def read_mesh_file(path: str, scx: float = 1.0, scy: float = 1.0):
+    """cm.dat (src/cmal_s.f:7-48): x, y (nn) scaled by scx, scy and the
+    triangles (3, ne), 1-based int32."""
+    lib = load_library()
+    nn, ne = C.c_int(), C.c_int()
+    _check(lib.spoc_mesh_file_size(os.fsencode(path), C.byref(nn), C.byref(ne)))
+    x, y = np.zeros(nn.value), np.zeros(nn.value)
+    tri = np.zeros((3, ne.value), np.int32, order="F")
+    _check(lib.spoc_mesh_file_read(os.fsencode(path), nn.value, ne.value, C.c_double(scx),
+                                   C.c_double(scy), _d(x), _d(y), _i(tri)))
+    return x, y, tri
+
+
+def write_mesh_file(path: str, x, y, tri, scx: float = 1.0, scy: float = 1.0) -> None:
+    """cm.dat the reference reads (save_mesh's layout, src/cmal_s.f:56-78)."""
+    lib = load_library()
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    tri = np.asfortranarray(tri, np.int32)
+    _check(lib.spoc_mesh_file_write(os.fsencode(path), len(x), tri.shape[1], _d(x), _d(y),
+                                    _i(tri), C.c_double(scx), C.c_double(scy)))
 
 
 def build_tables(nc: int, wi, xi, yi, wif, tif) -> dict:

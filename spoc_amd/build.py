@@ -40,6 +40,7 @@ def _units():
     units.append(("energy.hip", "energy.o", EXACT))
     units.append(("adapt.hip", "adapt.o", EXACT))
     units.append(("mesh.cpp", "mesh.o", EXACT))
+    units.append(("files.cpp", "files.o", []))
     units.append(("abi.cpp", "abi.o", EXACT + ["-x", "hip"]))
     return units
 

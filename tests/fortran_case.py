@@ -1,8 +1,8 @@
 """Run directories for the reference's fixed-mesh driver (program shwater,
-src/shwwnl.f): oc.inp (src/shwwnl.f:84-98), cm.dat (src/cmal_s.f:20-33),
-init.bin (7 unformatted sequential records, src/shwwnl.f:180-187) and the
-quadrature files include/tri2_<ng>.dat, include/legen<ngf>.dat
-(src/readgauss.f:16-41), written from the committed fixtures."""
+src/shwwnl.f): oc.inp (src/shwwnl.f:84-98), cm.dat (src/cmal_s.f:20-33) and
+init.bin (7 unformatted sequential records, src/shwwnl.f:180-187) through
+the library's file writers, and the quadrature files include/tri2_<ng>.dat,
+include/legen<ngf>.dat (src/readgauss.f:16-41) from the committed fixtures."""
 from __future__ import annotations
 
 import os
@@ -46,16 +46,19 @@ def write_case(d: str, *, nc: int, ng: int, ngf: int, x, y, tri, state, hb, w0, 
                      r(mb0), r(f0), r(beta), "1.0 1.0", r(g0), r(hb[0, 0]), "1", str(per2),
                      "0.0"):
             f.write(line + "\n")
-    tri = np.asarray(tri)
-    with open(os.path.join(d, "cm.dat"), "w") as f:
-        f.write(f"{len(x)} {tri.shape[1]}\n")
-        for i, (a, b) in enumerate(zip(x, y)):
-            f.write(f"{i + 1} {float(a)!r} {float(b)!r}\n")
-        for e in range(tri.shape[1]):
-            f.write(f"{e + 1} {tri[0, e]} {tri[1, e]} {tri[2, e]}\n")
-    with open(os.path.join(d, "init.bin"), "wb") as f:
-        _record(f, struct.pack("<d", 0.0))
-        for a in (*state, w0, z0, hb):
+    # cm.dat and init.bin through the library's own writers
+    import spoc_amd
+    spoc_amd.write_mesh_file(os.path.join(d, "cm.dat"), x, y, np.asarray(tri))
+    spoc_amd.write_state_file(os.path.join(d, "init.bin"), nm, 0.0,
+                              *(np.asarray(a) for a in (*state, w0, z0, hb)))
+
+
+def write_init_python(path: str, nm: int, time: float, arrays) -> None:
+    """init.bin written here in Python (record framing by hand), a second
+    writer for the tests of the library's."""
+    with open(path, "wb") as f:
+        _record(f, struct.pack("<d", time))
+        for a in arrays:
             _record(f, np.ascontiguousarray(np.asarray(a)[:nm].T, "<f8").tobytes())
 
 
