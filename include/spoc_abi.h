@@ -269,6 +269,38 @@ int spoc_mesh_file_write(const char *path, int nn, int ne, const double *x, cons
                          const int *in, double scx, double scy);
 
 /*
+ * Fields carried onto a new mesh by the adaptive driver, host side, bit for
+ * bit as the reference computes them.  Tables as spoc_tables_build returns
+ * them: the triangle rule wi, xi, yi (ng), pi2 (ld_g, nm), amm (ld_m, nm)
+ * the mass matrix's upper Cholesky factor.
+ *
+ * spoc_interpol (interpol + interlin, src/adapt.f:1662-1824): nfield modal
+ * fields u[f] (ld, ne) of the old mesh (nodes xgr, ygr; in, itm (3, ne);
+ * tseg, pseg (2, nf)) onto the new mesh (xgrnew, ygrnew; innew (3, nenew)):
+ * element i with tinter[i] = t > 0 copies old element t, the others are the
+ * L2 projection of the old field sampled at their quadrature points, each
+ * located by the reference's walk (from the previous point's element; a
+ * point outside the mesh is moved onto the wall edge it left by).  The walk
+ * does not depend on the field, so one call carries u, v, h together.
+ * SPOC_EMESH if a walk does not end.
+ *
+ * spoc_calh2 (src/adapt.f:303-373): L2 projection onto the mesh of the
+ * function ff(x, y, user) (the driver's wind and bathymetry), or h0's
+ * element tinter[i] where tinter[i] > 0.
+ */
+int spoc_interpol(int nc, int ng, const double *wi, const double *xi, const double *yi,
+                  const double *pi2, int ld_g, const double *amm, int ld_m, int nn, int ne,
+                  const double *xgr, const double *ygr, const int *in, const int *itm,
+                  const int *tseg, const int *pseg, int nenew, const double *xgrnew,
+                  const double *ygrnew, const int *innew, const int *tinter, int nfield, int ld,
+                  const double *const *u, double *const *unew);
+int spoc_calh2(int nc, int ng, const double *wi, const double *xi, const double *yi,
+               const double *pi2, int ld_g, const double *amm, int ld_m,
+               double (*ff)(double x, double y, void *user), void *user, int ne,
+               const double *xgr, const double *ygr, const int *in, const int *tinter, int ld,
+               const double *h0, double *h1);
+
+/*
  * Refinement indicator of the device-resident state (checkconv,
  * src/adapt.f:1260-1332), FP64, bit-identical to the reference: for field
  * 0 (u), 1 (v) or 2 (h), amax[e] (ne values) = the largest |difference| of

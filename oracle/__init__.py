@@ -296,6 +296,37 @@ class RefAdapt:
         out[:a.shape[0]] = a
         return out
 
+    def setup(self, nc, wi, xi, yi, wif, tif):
+        """Install the driver's tables in this library's COMMONs (ref_setup)."""
+        i1, i2 = C.c_int(), C.c_int()
+        wi, xi, yi, wif, tif = (np.ascontiguousarray(a, np.float64) for a in (wi, xi, yi, wif, tif))
+        self.lib.ref_setup(nc, len(wi), len(wif), _d(wi), _d(xi), _d(yi), _d(wif), _d(tif),
+                           C.byref(i1), C.byref(i2))
+        return i1.value, i2.value
+
+    def interpol(self, old, xn, yn, trin, tinter, u):
+        """interpol (src/adapt.f:1662-1739) of one field; setup first."""
+        ne = old["in"].shape[1]
+        u = self._pad(u)
+        xn, yn = np.ascontiguousarray(xn, np.float64), np.ascontiguousarray(yn, np.float64)
+        trin = np.asfortranarray(trin, np.int32)
+        tinter = np.ascontiguousarray(tinter, np.int32)
+        unew = np.zeros((LDM, trin.shape[1]), order="F")
+        x, y = np.ascontiguousarray(old["x"]), np.ascontiguousarray(old["y"])
+        self.lib.ref_interpol(ne, len(x), _d(x), _d(y), _i(np.asfortranarray(old["in"], np.int32)),
+                              _i(old["tseg"]), _i(old["pseg"]), _i(old["itm"]), _i(tinter),
+                              _d(u), _d(unew), trin.shape[1], len(xn), _d(xn), _d(yn), _i(trin))
+        return unew
+
+    def calh2(self, x, y, tri, tinter, h0):
+        """calh2 (src/adapt.f:303-373) of the harness's test function."""
+        tri = np.asfortranarray(tri, np.int32)
+        h0 = self._pad(h0)
+        h1 = np.zeros((LDM, tri.shape[1]), order="F")
+        self.lib.ref_calh2(tri.shape[1], _d(np.ascontiguousarray(x)), _d(np.ascontiguousarray(y)),
+                           _i(tri), _i(np.ascontiguousarray(tinter, np.int32)), _d(h0), _d(h1))
+        return h1
+
     def checkconv(self, nc, ng, ngf, m, pico, a0):
         ne = a0.shape[1]
         a0 = self._pad(a0)

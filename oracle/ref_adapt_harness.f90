@@ -63,4 +63,34 @@ contains
     timedir = merge(1, 0, ltimedir)
     deallocate(lfconv, lfderaff)
   end subroutine
+
+  ! interpol (src/adapt.f:1662-1739) of one field; the COMMON tables must be
+  ! installed first (ref_setup of the spoc_ref_harness module, linked into
+  ! the same library).
+  subroutine ref_interpol(ne, nn, xgr, ygr, in, tseg, pseg, itm, tinter, u, unew, nenew, nnnew, &
+      xgrnew, ygrnew, innew) bind(C, name="ref_interpol")
+    integer(c_int), value :: ne, nn, nenew, nnnew
+    real(c_double) :: xgr(*), ygr(*), u(LDM, *), unew(LDM, *), xgrnew(*), ygrnew(*)
+    integer(c_int) :: in(3, *), tseg(2, *), pseg(2, *), itm(3, *), tinter(*), innew(3, *)
+    integer nen, nnn, nel, nnl
+    nel = ne; nnl = nn; nen = nenew; nnn = nnnew
+    call interpol(nel, nnl, xgr, ygr, in, tseg, pseg, itm, tinter, u, unew, nen, nnn, xgrnew, &
+                  ygrnew, innew)
+  end subroutine
+
+  ! calh2 (src/adapt.f:303-373) of the test function harness_ff below.
+  subroutine ref_calh2(ne, xgr, ygr, in, tinter, h0, h1) bind(C, name="ref_calh2")
+    integer(c_int), value :: ne
+    real(c_double) :: xgr(*), ygr(*), h0(LDM, *), h1(LDM, *)
+    integer(c_int) :: in(3, *), tinter(*)
+    integer nel
+    nel = ne
+    call calh2(harness_ff, nel, h0, h1, in, xgr, ygr, tinter)
+  end subroutine
+
+  ! A smooth test field: tests/test_interp.py evaluates the same expression.
+  double precision function harness_ff(x, y)
+    double precision x, y
+    harness_ff = x * 1.0d-6 * y * 2.0d-6 + 3.0d0 * x * 1.0d-6 - y * y * 1.0d-12 + 7.0d0
+  end function
 end module

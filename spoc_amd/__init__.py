@@ -52,7 +52,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                  "spoc_autotune", "spoc_set_tiles", "spoc_adjoint_hb_download",
                  "spoc_set_checkpoint", "spoc_checkconv", "spoc_cal_conv", "spoc_mesh_build",
                  "spoc_mesh_changemaille", "spoc_state_file_read", "spoc_state_file_write",
-                 "spoc_mesh_file_size", "spoc_mesh_file_read", "spoc_mesh_file_write"):
+                 "spoc_mesh_file_size", "spoc_mesh_file_read", "spoc_mesh_file_write",
+                 "spoc_interpol", "spoc_calh2"):
         getattr(lib, name).restype = C.c_int
     lib.spoc_ctx_destroy.argtypes = [C.c_void_p]
     _lib = lib
@@ -139,6 +140,58 @@ def build_tables(nc: int, wi, xi, yi, wif, tif) -> dict:
     t.update(nc=nc, ng=ng, ngf=ngf, nm=nm_of(nc), wi=q[0], xi=q[1], yi=q[2], wif=q[3], tif=q[4],
              ldg=ldg)
     return t
+
+
+# -- the adaptive driver's re-projection onto a new mesh (host side) ----------
+def interpol(tables: dict, old: dict, new_x, new_y, new_tri, tinter, fields):
+    """Carry modal fields (each (ld, ne_old)) onto a new mesh (interpol +
+    interlin, src/adapt.f:1662-1824): old is a mesh dict of mesh.build
+    (x, y, in, itm, tseg, pseg); tinter[i] > 0 copies old element tinter[i].
+    Returns the new fields, (ld, ne_new) each."""
+    lib = load_library()
+    t = tables
+    fields = [np.asfortranarray(f, np.float64) for f in fields]
+    ld = fields[0].shape[0]
+    new_x = np.ascontiguousarray(new_x, np.float64)
+    new_y = np.ascontiguousarray(new_y, np.float64)
+    new_tri = np.asfortranarray(new_tri, np.int32)
+    tinter = np.ascontiguousarray(tinter, np.int32)
+    nenew = new_tri.shape[1]
+    out = [np.zeros((ld, nenew), order="F") for _ in fields]
+    P = C.POINTER(C.c_double)
+    u = (P * len(fields))(*(_d(f) for f in fields))
+    un = (P * len(out))(*(_d(f) for f in out))
+    tri, itm = np.asfortranarray(old["in"], np.int32), np.asfortranarray(old["itm"], np.int32)
+    tseg, pseg = (np.asfortranarray(old[k], np.int32) for k in ("tseg", "pseg"))
+    x, y = np.ascontiguousarray(old["x"], np.float64), np.ascontiguousarray(old["y"], np.float64)
+    _check(lib.spoc_interpol(t["nc"], t["ng"], _d(t["wi"]), _d(t["xi"]), _d(t["yi"]),
+                             _d(t["pi2"]), t["ldg"], _d(t["amm"]), LDM, len(x), tri.shape[1],
+                             _d(x), _d(y), _i(tri), _i(itm), _i(tseg), _i(pseg), nenew,
+                             _d(new_x), _d(new_y), _i(new_tri), _i(tinter), len(fields), ld, u,
+                             un))
+    return out
+
+
+_FF = C.CFUNCTYPE(C.c_double, C.c_double, C.c_double, C.c_void_p)
+
+
+def calh2(tables: dict, ff, x, y, tri, tinter=None, h0=None, ld: int = LDM):
+    """L2 projection of ff(x, y) onto the mesh (calh2, src/adapt.f:303-373);
+    elements with tinter[i] > 0 copy h0's element tinter[i]."""
+    lib = load_library()
+    t = tables
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    tri = np.asfortranarray(tri, np.int32)
+    ne = tri.shape[1]
+    tinter = np.zeros(ne, np.int32) if tinter is None else np.ascontiguousarray(tinter, np.int32)
+    h1 = np.zeros((ld, ne), order="F")
+    cb = _FF(lambda a, b, _u: float(ff(a, b)))
+    h0p = _d(np.asfortranarray(h0, np.float64)) if h0 is not None else None
+    _check(lib.spoc_calh2(t["nc"], t["ng"], _d(t["wi"]), _d(t["xi"]), _d(t["yi"]), _d(t["pi2"]),
+                          t["ldg"], _d(t["amm"]), LDM, cb, None, ne, _d(x), _d(y), _i(tri),
+                          _i(tinter), ld, h0p, _d(h1)))
+    return h1
 
 
 class Context:

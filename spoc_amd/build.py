@@ -41,6 +41,7 @@ def _units():
     units.append(("adapt.hip", "adapt.o", EXACT))
     units.append(("mesh.cpp", "mesh.o", EXACT))
     units.append(("files.cpp", "files.o", []))
+    units.append(("interp.cpp", "interp.o", EXACT))
     units.append(("abi.cpp", "abi.o", EXACT + ["-x", "hip"]))
     return units
 

@@ -22,8 +22,11 @@ Fixtures:
   calconv_p<p>.npz     checkconv indicators and cal_conv decisions
                        (src/adapt.f:1122-1332) on seeded random states, for
                        threshold sets that hit every branch
+  interp_p<p>.npz      interpol onto a 4-split refinement (some elements kept)
+                       and onto a larger jittered mesh; calh2 of a test function
+                       (src/adapt.f:1662-1824, 303-373)
 
-    python tests/golden/make_goldens.py [/root/reference] [--only-calconv]
+    python tests/golden/make_goldens.py [/root/reference] [--only-calconv|--only-interp]
 """
 from __future__ import annotations
 
@@ -118,12 +121,75 @@ def calconv_goldens(q):
         np.savez_compressed(os.path.join(OUT, f"calconv_p{nc}.npz"), **out)
 
 
+def refine4(x, y, tri, keep):
+    """Split every triangle not in keep into 4 through its edge midpoints
+    (new nodes appended, shared midpoints once); kept triangles stay.
+    Returns x, y, the new triangles and tinter (old index of a kept
+    triangle, else 0)."""
+    x, y = list(x), list(y)
+    mid = {}
+
+    def m(a, b):
+        k = (min(a, b), max(a, b))
+        if k not in mid:
+            x.append(0.5 * (x[a - 1] + x[b - 1]))
+            y.append(0.5 * (y[a - 1] + y[b - 1]))
+            mid[k] = len(x)
+        return mid[k]
+
+    out, tinter = [], []
+    for e in range(tri.shape[1]):
+        a, b, c = (int(v) for v in tri[:, e])
+        if e in keep:
+            out.append((a, b, c))
+            tinter.append(e + 1)
+            continue
+        ab, bc, ca = m(a, b), m(b, c), m(c, a)
+        for t in ((a, ab, ca), (ab, b, bc), (ca, bc, c), (ab, bc, ca)):
+            out.append(t)
+            tinter.append(0)
+    return (np.array(x), np.array(y), np.asfortranarray(np.array(out, np.int32).T),
+            np.array(tinter, np.int32))
+
+
+def interp_goldens(q):
+    """interpol and calh2 (src/adapt.f:1662-1824, 303-373) of random fields of
+    a random_case mesh onto (a) its 4-split refinement with every 5th
+    element kept and (b) a different jittered mesh of a slightly larger
+    square, whose quadrature points outside the old mesh take the wall
+    projection of interlin."""
+    for nc, ng, ngf, nx, ny in ((3, 12, 4, 9, 9), (6, 33, 7, 7, 7)):
+        tb = _tables_for(q, nc, ng, ngf)
+        nm = tb["nm"]
+        c = configs.random_case(tb, nx, ny, seed=51 + nc, scale=0.01)
+        m = c["mesh"]
+        A = oracle.RefAdapt()
+        A.setup(nc, *rule(q, nc, ng, ngf))
+        fields = [c["state"][0][:nm], c["state"][2][:nm], c["phys"]["hb"][:nm]]
+        keep = set(range(0, m["ne"], 5))
+        xa, ya, ta, tia = refine4(m["x"], m["y"], m["in"], keep)
+        xb, yb, tb_ = configs.jittered_mesh(nx + 2, ny + 1, 1.02e6, 1.03e6, 0.25, 90 + nc)
+        xb, yb = xb - 0.01e6, yb - 0.015e6
+        tib = np.zeros(tb_.shape[1], np.int32)
+        out = dict(nx=nx, ny=ny, seed=51 + nc, xa=xa, ya=ya, ta=ta, tia=tia, xb=xb, yb=yb, tb=tb_)
+        for f, u in enumerate(fields):
+            out[f"u{f}"] = u
+            out[f"a{f}"] = A.interpol(m, xa, ya, ta, tia, u)[:nm]
+            out[f"b{f}"] = A.interpol(m, xb, yb, tb_, tib, u)[:nm]
+        out["calh2"] = A.calh2(xa, ya, ta, tia, fields[2])[:nm]
+        np.savez_compressed(os.path.join(OUT, f"interp_p{nc}.npz"), **out)
+
+
 def main(refroot="/root/reference", *flags):
     oracle.build(ref=True)
     q = quad_tables(refroot)
     if "--only-calconv" in flags:
         calconv_goldens(q)
         print("cal_conv goldens written to", OUT)
+        return
+    if "--only-interp" in flags:
+        interp_goldens(q)
+        print("interpolation goldens written to", OUT)
         return
     for nc, ng, ngf in ((3, 12, 4), (6, 33, 7), (4, 25, 5), (5, 25, 6)):
         R = oracle.RefLib("fs")
@@ -188,6 +254,7 @@ def main(refroot="/root/reference", *flags):
     np.savez_compressed(os.path.join(OUT, "rk4_p3.npz"), dt=c["dt"], mu0=100.0,
                         u0=c["state"][0][:10], v0=c["state"][1][:10], h0=c["state"][2][:10], **out)
     calconv_goldens(q)
+    interp_goldens(q)
     print("goldens written to", OUT)
 
 
