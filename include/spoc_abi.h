@@ -213,6 +213,18 @@ int spoc_rk4_adjoint_steps(spoc_ctx *ctx, int variant, double dt, int nsteps);
 int spoc_autotune(spoc_ctx *ctx, int variant, int *tiles);
 int spoc_set_tiles(spoc_ctx *ctx, int variant, int tile0, int tile1, int autotune);
 
+/*
+ * Face flux of the FP32-MFMA path's gravity-wave core.  alpha = 0 (the
+ * default) is the reference's central flux (src/calflu.f:41-62).  alpha > 0
+ * adds a local Lax-Friedrichs (Rusanov) penalty: the normal flux of eta,
+ * u and v gains (c/2)(q_own - q_neighbour) with c = alpha sqrt(g0 hh0), the
+ * linear gravity-wave speed (alpha = 1: Rusanov).  It has no counterpart in
+ * the reference: it dissipates energy at the jumps, conserves mass, and is
+ * zero at walls (walls stay the projection of the reference).  The plain
+ * FP64 path and the adjoint support the central flux only (SPOC_EINVAL).
+ */
+int spoc_set_flux(spoc_ctx *ctx, double alpha);
+
 /* Wait for all work queued on the context's stream. */
 int spoc_sync(spoc_ctx *ctx);
 

@@ -53,7 +53,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                  "spoc_set_checkpoint", "spoc_checkconv", "spoc_cal_conv", "spoc_mesh_build",
                  "spoc_mesh_changemaille", "spoc_state_file_read", "spoc_state_file_write",
                  "spoc_mesh_file_size", "spoc_mesh_file_read", "spoc_mesh_file_write",
-                 "spoc_interpol", "spoc_calh2"):
+                 "spoc_interpol", "spoc_calh2", "spoc_set_flux"):
         getattr(lib, name).restype = C.c_int
     lib.spoc_ctx_destroy.argtypes = [C.c_void_p]
     _lib = lib
@@ -399,6 +399,11 @@ class Context:
         out = (C.c_int * 2)()
         _check(self.lib.spoc_autotune(self._h, var, out))
         return out[0], out[1]
+
+    def set_flux(self, alpha: float = 0.0) -> None:
+        """Face flux of the FP32-MFMA path: 0 central (the reference's),
+        alpha > 0 local Lax-Friedrichs with speed alpha sqrt(g hh0)."""
+        _check(self.lib.spoc_set_flux(self._h, C.c_double(alpha)))
 
     def set_tiles(self, variant, tile0=0, tile1=0, autotune=True) -> None:
         var = VARIANTS[variant] if isinstance(variant, str) else variant

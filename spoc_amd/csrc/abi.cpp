@@ -48,6 +48,8 @@ int f32_rk4_step(F32Plan *p, hipStream_t s, int variant, double dt, TimerHook ho
 int f32_rk4_end(F32Plan *p, hipStream_t s, int variant, double *const *u0);
 int f32_autotune(F32Plan *p, hipStream_t s, int variant, int *out);
 void f32_set_tiles(F32Plan *p, int variant, int t0, int t1, int autotune);
+void f32_set_flux(F32Plan *p, double alpha);
+double f32_flux(const F32Plan *p);
 int f32_adj_rhs(F32Plan *p, hipStream_t s, int variant, const double *const *x,
                 const double *const *lam, double *const *y, double *hb);
 int f32_adj_run(F32Plan *p, hipStream_t s, int variant, double dt, int nsteps, int ncheck,
@@ -765,6 +767,8 @@ int spoc_rhs(spoc_ctx *c, int variant, int precision, int ld, const double *u0,
   if (rc) return rc;
   const double *x[3] = {xs[0], xs[1], xs[2]};
   double *ks[3] = {state_ptr(c, 9), state_ptr(c, 10), state_ptr(c, 11)};
+  if (precision != SPOC_FP32_MFMA && f32_flux(c->f32) != 0.0)
+    return fail(SPOC_EINVAL, "the Lax-Friedrichs flux option runs on the FP32-MFMA path only");
   if (precision == SPOC_FP32_MFMA) {
     rc = f32_rhs(c->f32, c->stream, variant, x, ks, nullptr, nullptr);
     if (rc) return f32_fail(rc);
@@ -784,6 +788,8 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
   if (nsteps < 0) return fail(SPOC_EINVAL, "nsteps=%d", nsteps);
   if (precision != SPOC_FP64 && precision != SPOC_FP32_MFMA)
     return fail(SPOC_EINVAL, "unknown precision %d", precision);
+  if (precision != SPOC_FP32_MFMA && f32_flux(c->f32) != 0.0)
+    return fail(SPOC_EINVAL, "the Lax-Friedrichs flux option runs on the FP32-MFMA path only");
   (void)hipSetDevice(c->device);
   c->prof.used = 0;
   c->prof.marks.clear();
@@ -897,6 +903,8 @@ int spoc_rhs_adjoint(spoc_ctx *c, int variant, int ld, const double *u0, const d
     return fail(SPOC_EINVAL, "spoc_rhs_adjoint: bad arguments");
   if (variant != SPOC_LIN && (!u0 || !v0 || !h0))
     return fail(SPOC_EINVAL, "spoc_rhs_adjoint: the nonlinear variants need the state (u0,v0,h0)");
+  if (f32_flux(c->f32) != 0.0)
+    return fail(SPOC_EINVAL, "the adjoint is of the central flux: spoc_set_flux(ctx, 0) first");
   (void)hipSetDevice(c->device);
   size_t n = (size_t)c->nm * c->ne;
   const double *src[3] = {au, av, ah};
@@ -976,6 +984,8 @@ int spoc_rk4_adjoint_steps(spoc_ctx *c, int variant, double dt, int nsteps) {
   if (variant != SPOC_LIN && !c->have_state)
     return fail(SPOC_ESTATE, "the nonlinear adjoint needs the initial state: spoc_state_upload");
   if (nsteps < 0) return fail(SPOC_EINVAL, "nsteps=%d", nsteps);
+  if (f32_flux(c->f32) != 0.0)
+    return fail(SPOC_EINVAL, "the adjoint is of the central flux: spoc_set_flux(ctx, 0) first");
   (void)hipSetDevice(c->device);
   drop_graph(c);
   size_t n = (size_t)c->nm * c->ne;
@@ -1011,6 +1021,14 @@ int spoc_set_tiles(spoc_ctx *c, int variant, int tile0, int tile1, int autotune)
       return fail(SPOC_EINVAL, "tile %d: choose 2, 4 or 8 waves per workgroup (0: automatic)", t);
   drop_graph(c);
   f32_set_tiles(c->f32, variant, tile0, tile1, autotune);
+  return SPOC_OK;
+}
+
+int spoc_set_flux(spoc_ctx *c, double alpha) {
+  if (!c) return fail(SPOC_EINVAL, "null context");
+  if (!(alpha >= 0.0) || alpha > 100.0) return fail(SPOC_EINVAL, "flux alpha=%g", alpha);
+  drop_graph(c);
+  f32_set_flux(c->f32, alpha);
   return SPOC_OK;
 }
 

@@ -89,6 +89,7 @@ struct F32Plan {
   int tuned[4][2] = {};
   int forced[4][2] = {};
   bool autotune = true;
+  double lf_alpha = 0.0;  // Lax-Friedrichs option (0: central flux)
 };
 
 void f32_plan_destroy(F32Plan *p);
@@ -344,8 +345,8 @@ int f32_mesh_set(F32Plan *p, int ne, int nf, const int *itm, const int *its, con
   const int ne3 = 3 * ngf;
   const size_t n64 = (size_t)nepad * nm, nt = (size_t)nepad * ne3;
   // FP64: c0 acc x64 k (12), lam xbar aw avol (12), aF aUH aL (8), xb (3), hbbar (1),
-  // gvol (3); trL (4 nt), abar (4 nt), abar2 (2 nt), btr (2 nt).  FP32: trA, trS (5 nt)
-  const size_t b64 = (39 * n64 + 12 * nt) * 8, b32 = (5 * nt) * 4;
+  // gvol (3); trL (4 nt), abar (4 nt), abar2 (2 nt), btr (2 nt).  FP32: trA x2, trS (7 nt)
+  const size_t b64 = (39 * n64 + 12 * nt) * 8, b32 = (7 * nt) * 4;
   if (hipMalloc(&p->dstate, b64 + b32 + 256) != hipSuccess) return -2;
   if (hipMemset(p->dstate, 0, b64 + b32 + 256) != hipSuccess) return -2;
   double *s64 = (double *)p->dstate;
@@ -380,8 +381,9 @@ int f32_mesh_set(F32Plan *p, int ne, int nf, const int *itm, const int *its, con
     st.btr = st.abar2 + 2 * nt;
   }
   for (int f = 0; f < 3; f++) st.gvol[f] = s64 + (36 + f) * n64 + 12 * nt;
-  st.trA = s32;
-  st.trS = st.trA + 2 * nt;
+  st.trA[0] = s32;
+  st.trA[1] = s32 + 2 * nt;
+  st.trS = s32 + 4 * nt;
   // fields buffer: hbq, w0q, z0q, hbdq (blocked, nepad*ng each), fc3 (ne*3)
   size_t nq = (size_t)nepad * ng;
   if (hipMalloc(&p->dfld, (4 * nq + (size_t)ne * 3) * 4 + 256) != hipSuccess) return -2;
@@ -437,6 +439,7 @@ static F32Args args(F32Plan *p, hipStream_t s, int variant, int stage, double dt
               : p->tuned[variant & 3][k] ? p->tuned[variant & 3][k] : 4;
   a.stage = stage;
   a.dt = dt;
+  a.lf_alpha = p->lf_alpha;
   a.stream = s;
   a.ops = p->ops;
   a.mesh = p->mesh;
@@ -513,6 +516,9 @@ int f32_autotune(F32Plan *p, hipStream_t s, int variant, int *out) {
   }
   return rc;
 }
+
+void f32_set_flux(F32Plan *p, double alpha) { p->lf_alpha = alpha; }
+double f32_flux(const F32Plan *p) { return p->lf_alpha; }
 
 void f32_set_tiles(F32Plan *p, int variant, int t0, int t1, int autotune) {
   p->forced[variant & 3][0] = t0;

@@ -136,7 +136,8 @@ struct F32State {
   double *x64[3];          // [NM rows] stage input
   double *k[3];            // [NM rows] psi tendencies (stage < 0)
   double *gvol[3];         // [NM rows] volume and source part of the tendency (B+C -> E+G)
-  float *trA;              // [2*NE3 rows] T_u, T_v of the stage input
+  float *trA[2];           // [2*NE3 rows] T_u, T_v of the stage input (by stage parity,
+                           // like trL: the local Lax-Friedrichs option reads them in E+G)
   double *trL[2];          // [2*NE3 rows] T_eta, hh0 (T_u, T_v).n_face (wall-projected);
                            // RK4 stage s reads trL[s & 1] and writes trL[(s + 1) & 1],
                            // because one launch reads neighbours' traces and writes its own
@@ -168,6 +169,10 @@ struct F32Args {
   int stage;  // -1: tendencies only, 0..3: RK4 stage
   int tile[2];  // waves per workgroup: [0] lin_kernel or bc_kernel, [1] eg_kernel
   double dt;
+  // local Lax-Friedrichs (Rusanov) penalty of the gravity-wave core's face
+  // flux: alpha * sqrt(g hh0) times the jumps of eta and (u, v); 0: the
+  // reference's central flux
+  double lf_alpha;
   hipStream_t stream;
   F32Ops ops;
   F32Mesh mesh;

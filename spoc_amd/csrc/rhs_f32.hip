@@ -72,6 +72,12 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
   // (a select, not an index: a runtime index into the kernel argument would
   // copy the whole argument block to scratch)
   const double *trL = (A.stage > 0 && (A.stage & 1)) ? A.st.trL[1] : A.st.trL[0];
+  // local Lax-Friedrichs option: F.n_out gains (c/2) |n| (q_own - q_nbr) for
+  // q = eta, u, v with c = alpha sqrt(g hh0); lifted as (c/4) |n| (q_nbr -
+  // q_own) (a wall point is its own neighbour: no jump)
+  const bool lfo = A.lf_alpha != 0.0;
+  const double lfc = lfo ? A.lf_alpha * sqrt(g0 * hh0) : 0.0;
+  const float *trA = (A.stage > 0 && (A.stage & 1)) ? A.st.trA[1] : A.st.trA[0];
   {  // u, v: -g grad(eta), volume and face terms
     double bh[KM], fx[KE], fy[KE];
     if (VOL) {
@@ -89,9 +95,16 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
       for (int s = 0; s < KE; s++) {
         const int kj = 4 * s + g, kc = kj < NE3 ? kj : NE3 - 1, k = kc / NGF;
         const bool ok = valid && kj < NE3;
-        const double fh = -g0 * face_mean<NGF, 2>(trL, face_pt<NGF>(N, e, kc), 0);
-        fx[s] = ok ? 0.5 * fh * gg[11 + k] * gg[5 + k] : 0.0;
-        fy[s] = ok ? 0.5 * fh * gg[11 + k] * gg[8 + k] : 0.0;
+        const FacePt fp = face_pt<NGF>(N, e, kc);
+        const double fh = -g0 * face_mean<NGF, 2>(trL, fp, 0);
+        double px = 0.0, py = 0.0;
+        if (lfo) {
+          const double nx = gg[5 + k], ny = gg[8 + k], w = 0.25 * lfc * sqrt(nx * nx + ny * ny);
+          px = w * (double)(tr_at<NE3, 2>(trA, fp.eb, 0, fp.kb) - tr_at<NE3, 2>(trA, fp.ea, 0, fp.ka));
+          py = w * (double)(tr_at<NE3, 2>(trA, fp.eb, 1, fp.kb) - tr_at<NE3, 2>(trA, fp.ea, 1, fp.ka));
+        }
+        fx[s] = ok ? 0.5 * fh * gg[11 + k] * gg[5 + k] + px : 0.0;
+        fy[s] = ok ? 0.5 * fh * gg[11 + k] * gg[8 + k] + py : 0.0;
       }
     }
 #pragma unroll
@@ -156,7 +169,13 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
       for (int s = 0; s < KE; s++) {
         const int kj = 4 * s + g, kc = kj < NE3 ? kj : NE3 - 1, k = kc / NGF;
         const bool ok = valid && kj < NE3;
-        const double v = -0.5 * gg[11 + k] * face_mean<NGF, 2>(trL, face_pt<NGF>(N, e, kc), 1);
+        const FacePt fp = face_pt<NGF>(N, e, kc);
+        double v = -0.5 * gg[11 + k] * face_mean<NGF, 2>(trL, fp, 1);
+        if (lfo) {
+          const double nx = gg[5 + k], ny = gg[8 + k];
+          v += 0.25 * lfc * sqrt(nx * nx + ny * ny) *
+               (tr_at<NE3, 2>(trL, fp.eb, 0, fp.kb) - tr_at<NE3, 2>(trL, fp.ea, 0, fp.ka));
+        }
         fh[s] = ok ? v : 0.0;
       }
     }
@@ -203,8 +222,8 @@ __device__ __forceinline__ void traces_h(const F32Args &A, const double *O, int 
 }
 
 // ... and S = hh0 (T_u, T_v).n_face (field 1), with the mass flux of wall
-// elements projected first as the core projects it; trA <- (T_u, T_v) in
-// FP32 for the nonlinear gradient.
+// elements projected first as the core projects it; trA[w] <- (T_u, T_v) in
+// FP32 for the nonlinear gradient and the Lax-Friedrichs option.
 template <int NC, int NG, int NGF, int VAR>
 __device__ __forceinline__ void traces_uv(const F32Args &A, const double *O, int eb, int w,
                                           const double (&xu)[S<NC, NG, NGF>::KM],
@@ -226,9 +245,10 @@ __device__ __forceinline__ void traces_uv(const F32Args &A, const double *O, int
       int kj = drow(rb, r, g);
       if (kj < NE3) {
         int k = kj / NGF;
-        if (VAR != 0) {
-          tr_put<NE3, 2>(A.st.trA, eb, col, 0, kj, (float)Tu[r]);
-          tr_put<NE3, 2>(A.st.trA, eb, col, 1, kj, (float)Tv[r]);
+        if (VAR != 0 || A.lf_alpha != 0.0) {
+          float *trA = w ? A.st.trA[1] : A.st.trA[0];
+          tr_put<NE3, 2>(trA, eb, col, 0, kj, (float)Tu[r]);
+          tr_put<NE3, 2>(trA, eb, col, 1, kj, (float)Tv[r]);
         }
         tr_put<NE3, 2>(trL, eb, col, 1, kj, hh0 * (Tu[r] * gg[5 + k] + Tv[r] * gg[8 + k]));
       }
@@ -478,14 +498,15 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
   {
     float fb[4][KE];
     const Nbr3 N = nbr3(A.mesh, valid ? e : 0);
+    const float *trA = (A.stage > 0 && (A.stage & 1)) ? A.st.trA[1] : A.st.trA[0];
 #pragma unroll
     for (int s = 0; s < KE; s++) {
       const int kj = 4 * s + g, kc = kj < NE3 ? kj : NE3 - 1, k = kc / NGF;
       const bool ok = valid && kj < NE3;
       const float sg = 0.5f * gg[11 + k], nx = sg * gg[5 + k], ny = sg * gg[8 + k];
       const FacePt fp = face_pt<NGF>(N, e, kc);
-      const float fu = face_mean<NGF, 2>(A.st.trA, fp, 0);
-      const float fv = face_mean<NGF, 2>(A.st.trA, fp, 1);
+      const float fu = face_mean<NGF, 2>(trA, fp, 0);
+      const float fv = face_mean<NGF, 2>(trA, fp, 1);
       fb[0][s] = ok ? fu * nx : 0.f;
       fb[1][s] = ok ? fu * ny : 0.f;
       fb[2][s] = ok ? fv * nx : 0.f;
