@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "common.h"
+#include "error.h"
 #include "launch.h"
 
 using namespace spoc;
@@ -69,6 +70,21 @@ int fail(int code, const char *fmt, ...) {
   g_err = buf;
   return code;
 }
+
+}  // namespace
+
+// (the other translation units' way to fail(): error.h)
+int spoc::report(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+namespace {
 
 #define HIPCHK(x)                                                                     \
   do {                                                                                \

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/spoc_abi.h"
+#include "error.h"
 
 namespace {
 
@@ -90,7 +91,7 @@ int spoc_interpol(int nc, int ng, const double *wi, const double *xi, const doub
   if (nc < 0 || nm > 64 || ng <= 0 || ng > 64 || ld_g < ng || ld_m < nm || ld < nm || ne <= 0 ||
       nenew <= 0 || nfield <= 0 || !wi || !xi || !yi || !pi2 || !amm || !xgr || !ygr || !in ||
       !itm || !tseg || !pseg || !xgrnew || !ygrnew || !innew || !tinter || !u || !unew)
-    return SPOC_EINVAL;
+    return spoc::report(SPOC_EINVAL, "spoc_interpol: bad arguments");
   const Rule R{nc, nm, ng, wi, xi, yi, pi2, amm, ld_g, ld_m};
   auto X = [&](int n) { return xgr[n - 1]; };  // 1-based node access
   auto Y = [&](int n) { return ygr[n - 1]; };
@@ -100,7 +101,8 @@ int spoc_interpol(int nc, int ng, const double *wi, const double *xi, const doub
   for (int i = 0; i < nenew; i++) {
     if (tinter[i] != 0) {  // an element of the old mesh, kept: copy it
       const int t = tinter[i];
-      if (t < 1 || t > ne) return SPOC_EINVAL;
+      if (t < 1 || t > ne)
+        return spoc::report(SPOC_EINVAL, "spoc_interpol: tinter(%d)=%d out of range", i + 1, t);
       for (int f = 0; f < nfield; f++)
         for (int m = 0; m < nm; m++)
           unew[f][(size_t)ld * i + m] = u[f][(size_t)ld * (t - 1) + m];
@@ -118,7 +120,9 @@ int spoc_interpol(int nc, int ng, const double *wi, const double *xi, const doub
       double lab[3];
       long steps = 0;
       for (;;) {
-        if (++steps > max_steps) return SPOC_EMESH;
+        if (++steps > max_steps)
+          return spoc::report(SPOC_EMESH, "spoc_interpol: the walk for point %d of new element "
+                              "%d does not end", j + 1, i + 1);
         const int *n = in + 3 * (tk - 1);
         const double x1 = X(n[0]), y1 = Y(n[0]), x2 = X(n[1]), y2 = Y(n[1]);
         const double x3 = X(n[2]), y3 = Y(n[2]);
@@ -166,12 +170,14 @@ int spoc_calh2(int nc, int ng, const double *wi, const double *xi, const double 
   const int nm = (nc + 1) * (nc + 2) / 2;
   if (nc < 0 || nm > 64 || ng <= 0 || ng > 64 || ld_g < ng || ld_m < nm || ld < nm || ne <= 0 ||
       !ff || !wi || !xi || !yi || !pi2 || !amm || !xgr || !ygr || !in || !tinter || !h1)
-    return SPOC_EINVAL;
+    return spoc::report(SPOC_EINVAL, "spoc_calh2: bad arguments");
   const Rule R{nc, nm, ng, wi, xi, yi, pi2, amm, ld_g, ld_m};
   std::vector<double> mff(ng), b(nm);
   for (int i = 0; i < ne; i++) {
     if (tinter[i] != 0) {
-      if (!h0) return SPOC_EINVAL;
+      if (!h0 || tinter[i] < 1)
+        return spoc::report(SPOC_EINVAL, "spoc_calh2: element %d copies from h0, which is missing "
+                            "or the index %d is bad", i + 1, tinter[i]);
       for (int m = 0; m < nm; m++) h1[(size_t)ld * i + m] = h0[(size_t)ld * (tinter[i] - 1) + m];
       continue;
     }

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/spoc_abi.h"
+#include "error.h"
 
 namespace {
 
@@ -86,9 +87,11 @@ int spoc_mesh_build(int nn, int ne, const int *in, const double *x, const double
                     int *bdl, double *fc, double *deta) {
   if (nn <= 0 || ne <= 0 || !in || !x || !y || !nf_out || !itm || !its || !tseg || !pseg ||
       !ssit || !norm || !vect || !normt || !bdl || nc < 0 || (fc && ld < (nc + 1) * (nc + 2) / 2))
-    return SPOC_EINVAL;
+    return spoc::report(SPOC_EINVAL, "spoc_mesh_build: bad arguments");
   for (size_t t = 0; t < 3 * (size_t)ne; t++)
-    if (in[t] < 1 || in[t] > nn) return SPOC_EINVAL;
+    if (in[t] < 1 || in[t] > nn)
+      return spoc::report(SPOC_EINVAL, "spoc_mesh_build: in(%d,%d)=%d not a node", (int)(t % 3) + 1,
+                          (int)(t / 3) + 1, in[t]);
   const int nf = pointer_mesh(ne, in, itm, tseg, pseg);
   *nf_out = nf;
   if (check) {  // src/cmal_s.f:222-277: at most one wall face, and it is edge 1
@@ -96,10 +99,14 @@ int spoc_mesh_build(int nn, int ne, const int *in, const double *x, const double
     for (int f = 1; f <= nf; f++) {
       if (tseg[2 * (f - 1) + 1] != 0) continue;
       const int tk = tseg[2 * (f - 1)];
-      if (++cnt[tk] > 1 || itm[3 * (tk - 1)] != f) return SPOC_EMESH;
+      if (++cnt[tk] > 1)
+        return spoc::report(SPOC_EMESH, "triangle %d has more than one face on the wall", tk);
+      if (itm[3 * (tk - 1)] != f)
+        return spoc::report(SPOC_EMESH, "triangle %d: its wall face is not its edge 1", tk);
     }
   }
-  if (cal_its(ne, nf, itm, tseg, its, ssit)) return SPOC_EMESH;
+  if (const int bad = cal_its(ne, nf, itm, tseg, its, ssit))
+    return spoc::report(SPOC_EMESH, "face %d is not an edge of its elements", bad);
   for (int f = 0; f < nf; f++) {  // src/mat1.f:7-23
     const int p1 = pseg[2 * f] - 1, p2 = pseg[2 * f + 1] - 1;
     norm[2 * f] = y[p2] - y[p1];
@@ -136,9 +143,12 @@ int spoc_mesh_build(int nn, int ne, const int *in, const double *x, const double
 }
 
 int spoc_mesh_changemaille(int nn, int ne, int *in) {
-  if (nn <= 0 || ne <= 0 || !in) return SPOC_EINVAL;
+  if (nn <= 0 || ne <= 0 || !in)
+    return spoc::report(SPOC_EINVAL, "spoc_mesh_changemaille: bad arguments");
   for (size_t t = 0; t < 3 * (size_t)ne; t++)
-    if (in[t] < 1 || in[t] > nn) return SPOC_EINVAL;
+    if (in[t] < 1 || in[t] > nn)
+      return spoc::report(SPOC_EINVAL, "spoc_mesh_changemaille: in(%d,%d)=%d not a node",
+                          (int)(t % 3) + 1, (int)(t / 3) + 1, in[t]);
   std::vector<int> itm(3 * (size_t)ne), tseg(6 * (size_t)ne), pseg(6 * (size_t)ne);
   const int nf = pointer_mesh(ne, in, itm.data(), tseg.data(), pseg.data());
   auto I = [&](int k, int e) -> int & { return in[3 * (e - 1) + k - 1]; };
@@ -156,7 +166,8 @@ int spoc_mesh_changemaille(int nn, int ne, int *in) {
     if (pebd[i] <= 1) continue;
     int j = 1, ns1 = M(j, i);
     while (T(2, ns1) == 0) {
-      if (++j > 3) return SPOC_EMESH;  // every face on the wall
+      if (++j > 3)  // every face on the wall
+        return spoc::report(SPOC_EMESH, "triangle %d has all its faces on the wall", i);
       ns1 = M(j, i);
     }
     const int tk1 = T(1, ns1), tk2 = T(2, ns1);
