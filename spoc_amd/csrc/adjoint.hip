@@ -693,18 +693,28 @@ __global__ __launch_bounds__(NT) void adj_final_kernel(F32Args A, int nblk) {
 
 // Adjoint variables transform with U^-T: lam_c = U^-T lam_a on the way in
 // (forward substitution with the lower factor U^T), lam_a = U^T lam_c out.
+// Both run one thread per element, 64 elements per workgroup, with U and
+// the element-major side staged in LDS as the forward basis change does.
 template <int NM>
-__global__ void adj_to_psi_kernel(int ne, const double *__restrict__ U, const double *__restrict__ a,
-                                  double *__restrict__ c) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(PSI_WG) void adj_to_psi_kernel(int ne, const double *__restrict__ U,
+                                                            const double *__restrict__ a,
+                                                            double *__restrict__ c) {
+  __shared__ double Us[NM * NM];
+  __shared__ double T[PSI_WG * (NM + 1)];
+  const int e0 = blockIdx.x * PSI_WG, n = min(PSI_WG, ne - e0);
+  for (int i = threadIdx.x; i < NM * NM; i += PSI_WG) Us[i] = U[i];
+  const double *src = a + (size_t)e0 * NM;
+  for (int t = threadIdx.x; t < n * NM; t += PSI_WG) T[(t / NM) * (NM + 1) + t % NM] = src[t];
+  __syncthreads();
+  const int e = e0 + threadIdx.x;
   if (e >= ne) return;
   double x[NM];
 #pragma unroll
   for (int i = 0; i < NM; i++) {
-    double s = a[(size_t)e * NM + i];
+    double s = T[threadIdx.x * (NM + 1) + i];
 #pragma unroll
-    for (int k = 0; k < i; k++) s -= U[k + NM * i] * x[k];
-    x[i] = s / U[i + NM * i];
+    for (int k = 0; k < i; k++) s -= Us[k + NM * i] * x[k];
+    x[i] = s / Us[i + NM * i];
   }
   double *cb = c + (size_t)(e & ~15) * NM + (e & 15);
 #pragma unroll
@@ -712,15 +722,31 @@ __global__ void adj_to_psi_kernel(int ne, const double *__restrict__ U, const do
 }
 
 template <int NM>
-__global__ void adj_from_psi_kernel(int ne, const double *__restrict__ U,
-                                    const double *__restrict__ c, double *__restrict__ a) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ne * NM) return;
-  int e = t / NM, i = t % NM;
-  const double *cb = c + (size_t)(e & ~15) * NM + (e & 15);
-  double s = 0.0;
-  for (int k = 0; k <= i; k++) s += U[k + NM * i] * cb[k * 16];
-  a[(size_t)e * NM + i] = s;
+__global__ __launch_bounds__(PSI_WG) void adj_from_psi_kernel(int ne, const double *__restrict__ U,
+                                                              const double *__restrict__ c,
+                                                              double *__restrict__ a) {
+  __shared__ double Us[NM * NM];
+  __shared__ double T[PSI_WG * (NM + 1)];
+  const int e0 = blockIdx.x * PSI_WG, n = min(PSI_WG, ne - e0);
+  for (int i = threadIdx.x; i < NM * NM; i += PSI_WG) Us[i] = U[i];
+  __syncthreads();
+  const int e = e0 + threadIdx.x;
+  if (e < ne) {
+    double x[NM];
+    const double *cb = c + (size_t)(e & ~15) * NM + (e & 15);
+#pragma unroll
+    for (int k = 0; k < NM; k++) x[k] = cb[k * 16];
+#pragma unroll
+    for (int i = 0; i < NM; i++) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k <= i; k++) s += Us[k + NM * i] * x[k];
+      T[threadIdx.x * (NM + 1) + i] = s;
+    }
+  }
+  __syncthreads();
+  double *dst = a + (size_t)e0 * NM;
+  for (int t = threadIdx.x; t < n * NM; t += PSI_WG) dst[t] = T[(t / NM) * (NM + 1) + t % NM];
 }
 
 // w = dt/6 lam: first input of a backward sweep.
@@ -745,10 +771,10 @@ int adj_launch(const F32Args &A, int phase, const double *in, double *out) {
   const int v = A.variant;
   switch (phase) {
     case 0:  // adjoint into psi coordinates
-      adj_to_psi_kernel<NM><<<(ne + 127) / 128, 128, 0, st>>>(ne, A.ops.U, in, out);
+      adj_to_psi_kernel<NM><<<(ne + PSI_WG - 1) / PSI_WG, PSI_WG, 0, st>>>(ne, A.ops.U, in, out);
       break;
     case 1:  // adjoint back to the reference coordinates
-      adj_from_psi_kernel<NM><<<(ne * NM + 255) / 256, 256, 0, st>>>(ne, A.ops.U, in, out);
+      adj_from_psi_kernel<NM><<<(ne + PSI_WG - 1) / PSI_WG, PSI_WG, 0, st>>>(ne, A.ops.U, in, out);
       break;
     case 2: {  // w = dt/6 lam
       const int n = nblk * 16 * NM;

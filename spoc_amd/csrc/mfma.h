@@ -15,6 +15,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 typedef double doublex2 __attribute__((ext_vector_type(2)));
 constexpr int NW = 4, NT = 64 * NW;  // waves (element blocks) per workgroup
+constexpr int PSI_WG = 64;            // elements per workgroup of the basis changes
 
 template <int NC, int NG, int NGF>
 struct S {

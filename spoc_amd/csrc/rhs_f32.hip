@@ -823,35 +823,66 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(eg_wave
 
 // Basis change, FP64.  to_psi: c = U a from the reference layout a[e*NM + m]
 // into the blocked layout; from_psi: a = U^-1 c back.
+// Basis change of a field between the reference's modal basis (a,
+// element-major, NM per element) and the mass-orthonormal one (c, blocked
+// by 16 elements: c[(e & ~15) * NM + i * 16 + (e & 15)]), c = U a with U
+// upper triangular.  One thread per element; U sits in LDS and every lane
+// reads it at the same address.  The element's coefficients are read
+// straight from global memory: an LDS-staged variant (contiguous reads)
+// measured slower, its uniform LDS reads of U raising register use to 244.
 template <int NM>
-__global__ void to_psi_kernel(int ne, const double *__restrict__ U, const double *__restrict__ a,
-                              double *__restrict__ c) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ne * NM) return;
-  int e = t / NM, i = t % NM;
-  double s = 0.0;
-  for (int k = i; k < NM; k++) s += U[i + NM * k] * a[(size_t)e * NM + k];
-  c[(size_t)(e & ~15) * NM + i * 16 + (e & 15)] = s;
-}
-
-template <int NM>
-__global__ void from_psi_kernel(int ne, const double *__restrict__ U, const double *__restrict__ c,
-                                double *__restrict__ a) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void to_psi_kernel(int ne, const double *__restrict__ U,
+                                                     const double *__restrict__ a,
+                                                     double *__restrict__ c) {
+  __shared__ double Us[NM * NM];
+  for (int i = threadIdx.x; i < NM * NM; i += 256) Us[i] = U[i];
+  __syncthreads();
+  const int e = blockIdx.x * 256 + threadIdx.x;
   if (e >= ne) return;
   double x[NM];
-  const double *cb = c + (size_t)(e & ~15) * NM + (e & 15);
 #pragma unroll
-  for (int i = 0; i < NM; i++) x[i] = cb[i * 16];
+  for (int k = 0; k < NM; k++) x[k] = a[(size_t)e * NM + k];
+  double *ce = c + (size_t)(e & ~15) * NM + (e & 15);
 #pragma unroll
-  for (int i = NM - 1; i >= 0; i--) {
-    double s = x[i];
+  for (int i = 0; i < NM; i++) {
+    double s = 0.0;
 #pragma unroll
-    for (int k = i + 1; k < NM; k++) s -= U[i + NM * k] * x[k];
-    x[i] = s / U[i + NM * i];
+    for (int k = i; k < NM; k++) s += Us[i + NM * k] * x[k];
+    ce[i * 16] = s;
   }
+}
+
+// a = U^-1 c by back substitution (the inverse of to_psi_kernel), 64
+// elements per workgroup.  The element-major output goes through an LDS
+// tile (row stride NM + 1) so that the global writes are contiguous.
+template <int NM>
+__global__ __launch_bounds__(PSI_WG) void from_psi_kernel(int ne, const double *__restrict__ U,
+                                                          const double *__restrict__ c,
+                                                          double *__restrict__ a) {
+  __shared__ double Us[NM * NM];
+  __shared__ double T[PSI_WG * (NM + 1)];
+  const int e0 = blockIdx.x * PSI_WG, n = min(PSI_WG, ne - e0);
+  for (int i = threadIdx.x; i < NM * NM; i += PSI_WG) Us[i] = U[i];
+  __syncthreads();
+  const int e = e0 + threadIdx.x;
+  if (e < ne) {
+    double x[NM];
+    const double *cb = c + (size_t)(e & ~15) * NM + (e & 15);
 #pragma unroll
-  for (int i = 0; i < NM; i++) a[(size_t)e * NM + i] = x[i];
+    for (int i = 0; i < NM; i++) x[i] = cb[i * 16];
+#pragma unroll
+    for (int i = NM - 1; i >= 0; i--) {
+      double s = x[i];
+#pragma unroll
+      for (int k = i + 1; k < NM; k++) s -= Us[i + NM * k] * x[k];
+      x[i] = s / Us[i + NM * i];
+    }
+#pragma unroll
+    for (int i = 0; i < NM; i++) T[threadIdx.x * (NM + 1) + i] = x[i];
+  }
+  __syncthreads();
+  double *dst = a + (size_t)e0 * NM;
+  for (int t = threadIdx.x; t < n * NM; t += PSI_WG) dst[t] = T[(t / NM) * (NM + 1) + t % NM];
 }
 
 // Element kernels of one right-hand side at a chosen workgroup size (waves
@@ -884,10 +915,10 @@ int launch(const F32Args &A, int phase, const double *in, double *out) {
   const int v = A.variant;
   switch (phase) {
     case 0:  // to psi
-      to_psi_kernel<NM><<<(ne * NM + 255) / 256, 256, 0, st>>>(ne, A.ops.U, in, out);
+      to_psi_kernel<NM><<<(ne + 255) / 256, 256, 0, st>>>(ne, A.ops.U, in, out);
       break;
     case 1:  // from psi
-      from_psi_kernel<NM><<<(ne + 127) / 128, 128, 0, st>>>(ne, A.ops.U, in, out);
+      from_psi_kernel<NM><<<(ne + PSI_WG - 1) / PSI_WG, PSI_WG, 0, st>>>(ne, A.ops.U, in, out);
       break;
     case 2:  // stage input copy and traces
       if (v == 0) prep_kernel<NC, NG, NGF, 0><<<grid, NT, 0, st>>>(A, nblk);
