@@ -532,14 +532,14 @@ __global__ __launch_bounds__(NT) void adj_mid_kernel(F32Args A, int nblk) {
       const int q = drow(qb, r, g);
       pu[r] = pv[r] = ph[r] = p0[r] = p1[r] = p2[r] = p3[r] = 0.0;
       if (q < NG && valid) {
-        const size_t o = (size_t)e0 * NG + (size_t)q * 16 + col;
+        const float *F = A.fld.qf + (((size_t)eb * KG + 4 * qb + r) * 64 + l) * 4;
         const double f = fc0 + fc1 * OT[T.yq + q] + fc2 * OT[T.xq + q];
-        const double hk = Vh[r] + (double)A.fld.hbq[o], hf = Vh[r] + (double)A.fld.hbdq[o];
+        const double hk = Vh[r] + (double)F[0], hf = Vh[r] + (double)F[3];
         const double ihk = 1.0 / hk;
         const double u = Vu[r], v = Vv[r], gx = Gx[r], gy = Gy[r];
         pu[r] = gx * (-Vq0[r] - mb0 * ihk) - gy * (Vq2[r] + f) + Uh[r] * hf;
         pv[r] = -gx * (Vq1[r] - f) + gy * (-Vq3[r] - mb0 * ihk) + Wh[r] * hf;
-        ph[r] = -(gx * ((double)A.fld.w0q[o] - mb0 * u) + gy * ((double)A.fld.z0q[o] - mb0 * v)) *
+        ph[r] = -(gx * ((double)F[1] - mb0 * u) + gy * ((double)F[2] - mb0 * v)) *
                     ihk * ihk +
                 Uh[r] * u + Wh[r] * v;
         p0[r] = -u * gx;

@@ -23,6 +23,8 @@ SPOC_FOR_EACH_SHAPE(SPOC_DECL_F32)
 
 namespace {
 int up(int x, int m) { return (x + m - 1) / m * m; }
+// floats of the quadrature-point values: blocks x k-steps x 64 lanes x 4
+size_t qf_floats(int nepad, int ng) { return (size_t)(nepad / 16) * ((ng + 3) / 4) * 256; }
 
 // Inverse of an upper-triangular matrix (column-major, ld), FP64.
 std::vector<double> upper_inverse(int n, const double *U, int ld) {
@@ -384,11 +386,12 @@ int f32_mesh_set(F32Plan *p, int ne, int nf, const int *itm, const int *its, con
   st.trA[0] = s32;
   st.trA[1] = s32 + 2 * nt;
   st.trS = s32 + 4 * nt;
-  // fields buffer: hbq, w0q, z0q, hbdq (blocked, nepad*ng each), fc3 (ne*3)
-  size_t nq = (size_t)nepad * ng;
-  if (hipMalloc(&p->dfld, (4 * nq + (size_t)ne * 3) * 4 + 256) != hipSuccess) return -2;
+  // fields buffer: quadrature-point values (nepad/16 blocks x KG k-steps x
+  // 64 lanes x 4 floats), then fc3 (ne*3)
+  const size_t nq = qf_floats(nepad, ng);
+  if (hipMalloc(&p->dfld, (nq + (size_t)ne * 3) * 4 + 256) != hipSuccess) return -2;
   if (hipMalloc(&p->dhh0e, (size_t)ne * sizeof(double)) != hipSuccess) return -2;
-  if (hipMemset(p->dfld, 0, (4 * nq + (size_t)ne * 3) * 4 + 256) != hipSuccess) return -2;
+  if (hipMemset(p->dfld, 0, (nq + (size_t)ne * 3) * 4 + 256) != hipSuccess) return -2;
   return 0;
 }
 
@@ -396,37 +399,43 @@ int f32_fields_set(F32Plan *p, int ld, const double *fc, const double *w0, const
                    const double *hb, const double *hh0e, const double *pi2, int ld_g, double mu0,
                    double mb0, double g0) {
   int ne = p->ne, nepad = p->nepad, nm = p->nm, ng = p->ng, nc = p->nc;
-  size_t nq = (size_t)nepad * ng;
-  std::vector<float> h(4 * nq + (size_t)ne * 3, 0.0f);
-  for (int e = 0; e < ne; e++)  // hb - hh0 at the quadrature points (FP32 mass flux)
-    for (int q = 0; q < ng; q++) {
-      double s = 0.0;
-      for (int m = 0; m < nm; m++)
-        s += pi2[q + (size_t)ld_g * m] * (hb[m + (size_t)ld * e] - (m == 0 ? hh0e[e] : 0.0));
-      h[3 * nq + (size_t)ne * 3 + (size_t)(e & ~15) * ng + (size_t)q * 16 + (e & 15)] = (float)s;
-    }
+  const size_t nq = qf_floats(nepad, ng);
+  const int kg = (ng + 3) / 4;
+  std::vector<float> h(nq + (size_t)ne * 3, 0.0f);
+  // word f of point q of element e (see F32Fields::qf)
+  auto at = [&](int e, int q, int f) -> float & {
+    const size_t ks = q >> 2, l = (size_t)(q & 3) * 16 + (e & 15);
+    return h[(((size_t)(e >> 4) * kg + ks) * 64 + l) * 4 + f];
+  };
+  for (int e = 0; e < nepad; e++)
+    for (int q = 0; q < 4 * kg; q++) at(e, q, 0) = 1.0f;  // finite hb where nothing is read
   const double *src[3] = {hb, w0, z0};
-  for (int f = 0; f < 3; f++) {
-    if (!src[f]) continue;
-    for (int e = 0; e < ne; e++)
-      for (int q = 0; q < ng; q++) {
+  for (int e = 0; e < ne; e++)
+    for (int q = 0; q < ng; q++) {
+      for (int f = 0; f < 3; f++) {
+        if (!src[f]) {
+          at(e, q, f) = 0.0f;
+          continue;
+        }
         double s = 0.0;
         for (int m = 0; m < nm; m++) s += pi2[q + (size_t)ld_g * m] * src[f][m + (size_t)ld * e];
-        h[f * nq + (size_t)(e & ~15) * ng + (size_t)q * 16 + (e & 15)] = (float)s;
+        at(e, q, f) = (float)s;
       }
-  }
+      double s = 0.0;  // hb - hh0 (FP32 mass flux)
+      for (int m = 0; m < nm; m++)
+        s += pi2[q + (size_t)ld_g * m] * (hb[m + (size_t)ld * e] - (m == 0 ? hh0e[e] : 0.0));
+      at(e, q, 3) = (float)s;
+    }
   if (fc)
     for (int e = 0; e < ne; e++) {
-      h[3 * nq + 3 * (size_t)e] = (float)fc[0 + (size_t)ld * e];
-      h[3 * nq + 3 * (size_t)e + 1] = (float)fc[1 + (size_t)ld * e];
-      h[3 * nq + 3 * (size_t)e + 2] = (float)fc[nc + 1 + (size_t)ld * e];
+      h[nq + 3 * (size_t)e] = (float)fc[0 + (size_t)ld * e];
+      h[nq + 3 * (size_t)e + 1] = (float)fc[1 + (size_t)ld * e];
+      h[nq + 3 * (size_t)e + 2] = (float)fc[nc + 1 + (size_t)ld * e];
     }
   if (hipMemcpy(p->dfld, h.data(), h.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(p->dhh0e, hh0e, (size_t)ne * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
     return -2;
-  p->fld = F32Fields{p->dfld,        p->dfld + nq, p->dfld + 2 * nq, p->dfld + 3 * nq + (size_t)ne * 3,
-                     p->dfld + 3 * nq, p->dhh0e,   (float)mu0,       (float)mb0,
-                     g0};
+  p->fld = F32Fields{p->dfld, p->dfld + nq, p->dhh0e, (float)mu0, (float)mb0, g0};
   p->have_fields = true;
   return 0;
 }

@@ -122,8 +122,13 @@ struct F32Mesh {
 // Element-blocked arrays: element e = 16 b + c, row r of an array with R
 // rows per element at [(b * R + r) * 16 + c].
 struct F32Fields {
-  const float *hbq, *w0q, *z0q;  // [NG rows] values at quadrature points
-  const float *hbdq;             // [NG rows] hb - hh0 at quadrature points
+  // Values at the quadrature points, four per point (hb, w0, z0, hb - hh0),
+  // ordered for the MFMA lane layout: quadrature k-step ks (points
+  // 4 ks .. 4 ks + 3) of element block b is 64 float4 at [(b * KG + ks) * 64],
+  // lane l holding point 4 ks + (l >> 4) of element 16 b + (l & 15), so that
+  // one k-step is one contiguous 1 KiB wave load.  Points past NG hold
+  // (1, 0, 0, 0).
+  const float *qf;
   const float *fc3;              // [ne][3] Coriolis f = fc0 + fc1*eta + fc2*xi
   const double *hh0e;            // [ne] depth of the linear core: hb(1,1) of the
                                  // element's own mesh (ensemble member)

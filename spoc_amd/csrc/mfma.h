@@ -188,10 +188,35 @@ __device__ __forceinline__ int wave_block(int b0) {
   return __builtin_amdgcn_readfirstlane(b0 + (int)(threadIdx.x >> 6));
 }
 
-// Copy n 16-byte words of operator images to LDS (whole workgroup).
+// Copy n 16-byte words of operator images to LDS (whole workgroup), eight
+// loads in flight per thread before their stores.
 template <int W = NW>
 __device__ __forceinline__ void stage_lds(floatx4 *dst, const floatx4 *__restrict__ src, int n) {
-  for (int i = threadIdx.x; i < n; i += 64 * W) dst[i] = src[i];
+  constexpr int T = 64 * W, B = 8;
+  for (int i0 = threadIdx.x; i0 < n; i0 += B * T) {
+    floatx4 v[B];
+#pragma unroll
+    for (int j = 0; j < B; j++)
+      if (i0 + j * T < n) v[j] = src[i0 + j * T];
+#pragma unroll
+    for (int j = 0; j < B; j++)
+      if (i0 + j * T < n) dst[i0 + j * T] = v[j];
+  }
+}
+
+// The element's face normals (x and y components for local edges 1..3,
+// geometry words 5..10), read as two 16-byte loads, and the pick of one by
+// local edge with selects.
+struct Nrm3 {
+  float x[3], y[3];
+};
+__device__ __forceinline__ Nrm3 normals(const float *gg) {
+  const floatx4 a = *reinterpret_cast<const floatx4 *>(gg + 4);
+  const floatx4 b = *reinterpret_cast<const floatx4 *>(gg + 8);
+  return Nrm3{{a[1], a[2], a[3]}, {b[0], b[1], b[2]}};
+}
+__device__ __forceinline__ float pick3(const float (&v)[3], int k) {
+  return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]);
 }
 
 // ---------------------------------------------------------------------------

@@ -562,6 +562,8 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
   d2b(Q[3], q3);
   // D: traces of the viscous flux F = mu0 q (the -g eta part belongs to the
   // FP64 core), normal components
+  {
+  const Nrm3 nr = normals(gg);
 #pragma unroll
   for (int rb = 0; rb < RBE; rb++) {
     float at[KM];
@@ -573,11 +575,12 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
       int kj = drow(rb, r, g);
       if (kj < NE3) {
         int k = kj / NGF;
-        float nx = gg[5 + k], ny = gg[8 + k];
+        float nx = pick3(nr.x, k), ny = pick3(nr.y, k);
         tr_put<NE3, 3>(A.st.trS, eb, col, 0, kj, mu0 * (T0[r] * nx + T1[r] * ny));
         tr_put<NE3, 3>(A.st.trS, eb, col, 1, kj, mu0 * (T2[r] * nx + T3[r] * ny));
       }
     }
+  }
   }
   // E, volume part for (u, v): div F with the Jacobian folded into the
   // operands (-by Fxx + bx Fxy with Dx, ay Fxx - ax Fxy with Dy; v likewise);
@@ -615,8 +618,15 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
   // projection.
   const float *fc3 = A.fld.fc3 + (size_t)(valid ? e : 0) * 3;
   const float fc0 = fc3[0], fc1 = fc3[1], fc2 = fc3[2];
+  // the block's quadrature-point values: one 1 KiB wave load per k-step,
+  // each k-block's issued before the products that need them
+  const floatx4 *qf = reinterpret_cast<const floatx4 *>(A.fld.qf) + (size_t)eb * KG * 64 + l;
 #pragma unroll
   for (int qb = 0; qb < RBG; qb++) {
+    floatx4 F[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      if (4 * qb + r < KG) F[r] = qf[(4 * qb + r) * 64];
     float ap[KM];
     afrag32<NM>(O + P.psi, qb, ap);
     floatx4 Vu = chain32(ap, bu, zero32()), Vv = chain32(ap, bv, zero32());
@@ -626,18 +636,20 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
     float pg[4][4];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      // (rows past NG and elements past the mesh give 0; their loads are
-      // clamped into the padded arrays so that none of them branches)
+      if (4 * qb + r >= KG) {
+        pg[0][r] = pg[1][r] = pg[2][r] = pg[3][r] = 0.f;
+        continue;
+      }
+      // (rows past NG and elements past the mesh give 0, by selects)
       const int q = drow(qb, r, g), qc = q < NG ? q : NG - 1;
       const bool ok = valid && q < NG;
-      const size_t o = (size_t)e0 * NG + (size_t)qc * 16 + col;
       const float f = fc0 + fc1 * O[P.yq + qc] + fc2 * O[P.xq + qc];
-      const float hk = Vh[r] + A.fld.hbq[o];
+      const float hk = Vh[r] + F[r][0], rhk = 1.0f / hk;
       const float u = Vu[r], v = Vv[r];
-      const float p0 = -(u * Vxx[r] + v * (Vxy[r] - f)) + (A.fld.w0q[o] - mb0 * u) / hk;
-      const float p1 = -(u * (Vyx[r] + f) + v * Vyy[r]) + (A.fld.z0q[o] - mb0 * v) / hk;
+      const float p0 = -(u * Vxx[r] + v * (Vxy[r] - f)) + (F[r][1] - mb0 * u) * rhk;
+      const float p1 = -(u * (Vyx[r] + f) + v * Vyy[r]) + (F[r][2] - mb0 * v) * rhk;
       // the flux hh0 u is the FP64 core's; only u (h + hb - hh0) here
-      const float hf = Vh[r] + A.fld.hbdq[o];
+      const float hf = Vh[r] + F[r][3];
       pg[0][r] = ok ? p0 : 0.f;
       pg[1][r] = ok ? p1 : 0.f;
       pg[2][r] = ok ? u * hf : 0.f;
@@ -696,6 +708,8 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
         }
     }
   }
+  {
+  const Nrm3 nr = normals(gg);
 #pragma unroll
   for (int rb = 0; rb < RBE; rb++) {
     float at[KM];
@@ -706,9 +720,10 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
       int kj = drow(rb, r, g);
       if (kj < NE3) {
         int k = kj / NGF;
-        tr_put<NE3, 3>(A.st.trS, eb, col, 2, kj, Tu[r] * gg[5 + k] + Tv[r] * gg[8 + k]);
+        tr_put<NE3, 3>(A.st.trS, eb, col, 2, kj, Tu[r] * pick3(nr.x, k) + Tv[r] * pick3(nr.y, k));
       }
     }
+  }
   }
   doublex4 G[3][RBM];
   {
