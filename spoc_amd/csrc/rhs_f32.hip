@@ -51,11 +51,11 @@ constexpr int eg_waves(int nc) { return nc >= 7 ? 2 : EG_WAVES; }
 // -div(hh0 u) term, src/calvarnl_fs.f:298-321, 436-471 and the ns/inv
 // equivalents).  MODE 0: volume and face terms, G = ...; MODE 1: volume
 // terms only (from the FP64 stage input x64), G += ...; MODE 2: face terms
-// only (from the FP64 traces trL), G += ....  dx, dy, lf: FP64 images of
+// only (from the FP64 traces trL), G = ....  dx, dy, lf: FP64 images of
 // Dx, Dy (MODE 0, 1) and LF (MODE 0, 2).  G[f] holds the three tendencies
 // (u, v, eta) in the result layout.  X, when given, holds the stage input
 // (u, v, eta B fragments) already loaded by the caller.
-template <int NC, int NG, int NGF, int VAR, int MODE>
+template <int NC, int NG, int NGF, int VAR, int MODE, bool LF = false>
 __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const double *dy,
                                        const double *lf, const Nbr3 &N, int eb,
                                        doublex4 (&G)[3][S<NC, NG, NGF>::RBM],
@@ -74,8 +74,10 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
   const double *trL = (A.stage > 0 && (A.stage & 1)) ? A.st.trL[1] : A.st.trL[0];
   // local Lax-Friedrichs option: F.n_out gains (c/2) |n| (q_own - q_nbr) for
   // q = eta, u, v with c = alpha sqrt(g hh0); lifted as (c/4) |n| (q_nbr -
-  // q_own) (a wall point is its own neighbour: no jump)
-  const bool lfo = A.lf_alpha != 0.0;
+  // q_own) (a wall point is its own neighbour: no jump).  A compile-time
+  // switch (the launch picks the instance from alpha != 0): a runtime branch
+  // in each face iteration would keep the loads of successive faces apart.
+  constexpr bool lfo = LF;
   const double lfc = lfo ? A.lf_alpha * sqrt(g0 * hh0) : 0.0;
   const float *trA = (A.stage > 0 && (A.stage & 1)) ? A.st.trA[1] : A.st.trA[0];
   {  // u, v: -g grad(eta), volume and face terms
@@ -129,8 +131,8 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
       }
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        G[0][rb][r] = (MODE != 0 ? G[0][rb][r] : 0.0) + di * Gx[r];
-        G[1][rb][r] = (MODE != 0 ? G[1][rb][r] : 0.0) + di * Gy[r];
+        G[0][rb][r] = (MODE == 1 ? G[0][rb][r] : 0.0) + di * Gx[r];
+        G[1][rb][r] = (MODE == 1 ? G[1][rb][r] : 0.0) + di * Gy[r];
       }
     }
   }
@@ -195,7 +197,7 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
         Gh = chain64(al, fh, Gh);
       }
 #pragma unroll
-      for (int r = 0; r < 4; r++) G[2][rb][r] = (MODE != 0 ? G[2][rb][r] : 0.0) + di * Gh[r];
+      for (int r = 0; r < 4; r++) G[2][rb][r] = (MODE == 1 ? G[2][rb][r] : 0.0) + di * Gh[r];
     }
   }
 }
@@ -425,7 +427,7 @@ __global__ __launch_bounds__(NT) void prep_kernel(F32Args A, int nblk) {
 // ---------------------------------------------------------------------------
 // Linear right-hand side (src/calvar_lin.f:7-287) + RK4 stage, all FP64.
 // ---------------------------------------------------------------------------
-template <int NC, int NG, int NGF, int WG>
+template <int NC, int NG, int NGF, int WG, bool LF>
 __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(3))) void lin_kernel(F32Args A, int nblk) {
   using Sh = S<NC, NG, NGF>;
   constexpr Pack64 P = Sh::P64;
@@ -440,7 +442,7 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(3))) vo
   const int e = eb * 16 + (lane() & 15);
   const Nbr3 N = nbr3(A.mesh, e < A.mesh.ne ? e : 0);
   doublex4 G[3][Sh::RBM];
-  core64<NC, NG, NGF, 0, 0>(A, O + P.dx, O + P.dy, O + P.lf, N, eb, G);
+  core64<NC, NG, NGF, 0, 0, LF>(A, O + P.dx, O + P.dy, O + P.lf, N, eb, G);
   phase_fence();
   finish64<NC, NG, NGF, 0>(A, O, eb, G);
 }
@@ -764,9 +766,9 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
 // Nonlinear stages E + G (src/calvarnl_fs.f:382-544): the face terms (FP32
 // lift of the flux traces, FP64 lift of the linear core's traces) added to
 // the B+C volume part, wall projection, RK4 stage, traces of the next stage
-// input.
+// input.  RUS: the local Lax-Friedrichs (Rusanov) option is on.
 // ---------------------------------------------------------------------------
-template <int NC, int NG, int NGF, int VAR, int WG>
+template <int NC, int NG, int NGF, int VAR, int WG, bool RUS>
 __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(eg_waves(NC)))) void eg_kernel(F32Args A, int nblk) {
   using Sh = S<NC, NG, NGF>;
   constexpr int NM = Sh::NM, NE3 = Sh::NE3, KE = Sh::KE, RBM = Sh::RBM;
@@ -788,7 +790,11 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(eg_wave
   const size_t mb = (size_t)e0 * NM;
   const Nbr3 N = nbr3(A.mesh, valid ? e : 0);
 
+  // FP64 face terms, then the volume part B+C stored (gvol) added to them
+  // (read only now, so that its 48 registers are free while the face loop
+  // keeps its loads in flight)
   doublex4 G[3][RBM];
+  core64<NC, NG, NGF, VAR, 2, RUS>(A, nullptr, nullptr, O64 + P64.lf, N, eb, G);
 #pragma unroll
   for (int f = 0; f < 3; f++)
 #pragma unroll
@@ -797,9 +803,8 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(eg_wave
       for (int r = 0; r < 4; r++) {
         const int m = drow(rb, r, g), mc = m < NM ? m : NM - 1;
         const double v = A.st.gvol[f][mb + mc * 16 + col];
-        G[f][rb][r] = (valid && m < NM) ? v : 0.0;
+        G[f][rb][r] += (valid && m < NM) ? v : 0.0;
       }
-  core64<NC, NG, NGF, VAR, 2>(A, nullptr, nullptr, O64 + P64.lf, N, eb, G);
   phase_fence();
   {
     // flux traces averaged over each face, times its/2; the eta equation's
@@ -907,16 +912,22 @@ int launch_elem(const F32Args &A, int phase, int nblk) {
   const int grid = 8 * cdiv(cdiv(nblk, WG), 8);  // a multiple of the 8 XCDs
   hipStream_t st = A.stream;
   const int v = A.variant;
+  const bool lf = A.lf_alpha != 0.0;
   if (phase == 3) {
-    lin_kernel<NC, NG, NGF, WG><<<grid, 64 * WG, 0, st>>>(A, nblk);
+    if (lf) lin_kernel<NC, NG, NGF, WG, true><<<grid, 64 * WG, 0, st>>>(A, nblk);
+    else lin_kernel<NC, NG, NGF, WG, false><<<grid, 64 * WG, 0, st>>>(A, nblk);
   } else if (phase == 4) {
     if (v == 1) bc_kernel<NC, NG, NGF, 1, WG><<<grid, 64 * WG, 0, st>>>(A, nblk);
     else if (v == 2) bc_kernel<NC, NG, NGF, 2, WG><<<grid, 64 * WG, 0, st>>>(A, nblk);
     else bc_kernel<NC, NG, NGF, 3, WG><<<grid, 64 * WG, 0, st>>>(A, nblk);
+  } else if (lf) {
+    if (v == 1) eg_kernel<NC, NG, NGF, 1, WG, true><<<grid, 64 * WG, 0, st>>>(A, nblk);
+    else if (v == 2) eg_kernel<NC, NG, NGF, 2, WG, true><<<grid, 64 * WG, 0, st>>>(A, nblk);
+    else eg_kernel<NC, NG, NGF, 3, WG, true><<<grid, 64 * WG, 0, st>>>(A, nblk);
   } else {
-    if (v == 1) eg_kernel<NC, NG, NGF, 1, WG><<<grid, 64 * WG, 0, st>>>(A, nblk);
-    else if (v == 2) eg_kernel<NC, NG, NGF, 2, WG><<<grid, 64 * WG, 0, st>>>(A, nblk);
-    else eg_kernel<NC, NG, NGF, 3, WG><<<grid, 64 * WG, 0, st>>>(A, nblk);
+    if (v == 1) eg_kernel<NC, NG, NGF, 1, WG, false><<<grid, 64 * WG, 0, st>>>(A, nblk);
+    else if (v == 2) eg_kernel<NC, NG, NGF, 2, WG, false><<<grid, 64 * WG, 0, st>>>(A, nblk);
+    else eg_kernel<NC, NG, NGF, 3, WG, false><<<grid, 64 * WG, 0, st>>>(A, nblk);
   }
   return 0;
 }
