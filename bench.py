@@ -76,7 +76,9 @@ def algorithmic_counts(nm, ng, ngf, variant):
     e3 = 3 * ngf
     f4, f8 = 4, 8
     mesh = 16 * f8 + 6 * f4 + f4                       # geo64, nbr/nbe, bdl
-    rk = 3 * nm * f8 * (1 + 0.75) + 3 * nm * f8 * 2     # c0, acc reads; acc|c0, x64 writes
+    # RK4, averaged over the 4 stages: c0 read at stages 0-2, acc at 1-3;
+    # x64 written at stages 0-2, acc at 0-2, c0 at 3
+    rk = 3 * nm * f8 * (0.75 + 0.75) + 3 * nm * f8 * (0.75 + 1.0)
     vol_fl = 2 * 4 * nm * nm                              # core: -g grad eta, div(hh0 u)
     face_fl = 2 * 6 * nm * e3                             # core: lifts, traces
     trl_by = 2 * e3 * f8 + rk + 2 * e3 * f8               # trL in, RK, trL out
@@ -90,7 +92,7 @@ def algorithmic_counts(nm, ng, ngf, variant):
              + 3 * nm * f8 + 3 * e3 * f4)                 # x64, trA, mesh, fields | gvol, trS
     # E+G: the FP32 lift of the flux traces and the core's face terms
     eg_fl = 2 * 3 * nm * e3
-    eg_by = (3 * nm * f8 + 3 * e3 * f4 + 16 * f4 + mesh + f8 + trl_by + 2 * e3 * f4)
+    eg_by = (3 * nm * f8 + 3 * e3 * f4 + mesh + f8 + trl_by + 2 * e3 * f4)
     return {"bc_f32": (bc_fl, vol_fl, bc_by), "eg_f32f64": (eg_fl, face_fl, eg_by)}
 
 

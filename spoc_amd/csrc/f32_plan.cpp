@@ -703,7 +703,8 @@ int f32_adj_run(F32Plan *p, hipStream_t s, int variant, double dt, int nsteps, i
           rc = copy(p->st.c0[0], seg0 + (size_t)(n - n0) * 3 * n64);
           if (!rc) rc = call(p, fa, 2, nullptr, nullptr);
           for (int st = 0; st < 4 && !rc; st++) {
-            rc = copy(base0 + (size_t)st * 3 * n64, p->st.x64[0]);
+            // (the first stage's input is the step's start state itself)
+            rc = copy(base0 + (size_t)st * 3 * n64, st == 0 ? p->st.c0[0] : p->st.x64[0]);
             if (st < 3 && !rc) {
               fa.stage = st;
               rc = rhs_kernels(p, fa, nullptr, nullptr);

@@ -46,11 +46,18 @@ namespace f32 {
 constexpr int bc_waves(int nc) { return nc >= 7 ? 2 : BC_WAVES; }
 constexpr int eg_waves(int nc) { return nc >= 7 ? 2 : EG_WAVES; }
 
+// Stage input of field f: the step's start state c0 at the first stage (and
+// for a bare right-hand side), else what the previous stage stored in x64.
+// (A select on the launch-uniform stage, not a branch.)
+__device__ __forceinline__ const double *stage_in(const F32Args &A, int f) {
+  return A.stage <= 0 ? A.st.c0[f] : A.st.x64[f];
+}
+
 // Linear gravity-wave core (VAR 0: the whole linear right-hand side,
 // src/calvar_lin.f; nonlinear variants: the -g grad(eta) terms and the
 // -div(hh0 u) term, src/calvarnl_fs.f:298-321, 436-471 and the ns/inv
 // equivalents).  MODE 0: volume and face terms, G = ...; MODE 1: volume
-// terms only (from the FP64 stage input x64), G += ...; MODE 2: face terms
+// terms only (from the FP64 stage input), G += ...; MODE 2: face terms
 // only (from the FP64 traces trL), G = ....  dx, dy, lf: FP64 images of
 // Dx, Dy (MODE 0, 1) and LF (MODE 0, 2).  G[f] holds the three tendencies
 // (u, v, eta) in the result layout.  X, when given, holds the stage input
@@ -87,7 +94,7 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
 #pragma unroll
         for (int s = 0; s < KM; s++) bh[s] = X[2][s];
       } else {
-        bload<NM>(A.st.x64[2] + mb, g, col, valid, bh);
+        bload<NM>(stage_in(A, 2) + mb, g, col, valid, bh);
       }
 #pragma unroll
       for (int s = 0; s < KM; s++) bh[s] *= -g0;
@@ -147,8 +154,8 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
           bv[s] = X[1][s];
         }
       } else {
-        bload<NM>(A.st.x64[0] + mb, g, col, valid, bu);
-        bload<NM>(A.st.x64[1] + mb, g, col, valid, bv);
+        bload<NM>(stage_in(A, 0) + mb, g, col, valid, bu);
+        bload<NM>(stage_in(A, 1) + mb, g, col, valid, bv);
       }
 #pragma unroll
       for (int s = 0; s < KM; s++) {
@@ -293,9 +300,14 @@ __device__ __forceinline__ void traces_uv(const F32Args &A, const double *O, int
 }
 
 // RK4 stage update (src/shwwnl.f:242-284) of field f, FP64: x holds the
-// stage's tendency (B fragments) and leaves as the next stage input, which
-// is stored to x64; acc gathers the weighted tendencies and c0 takes the
-// step's result at the last stage.  stage < 0: the tendency goes to k and x
+// stage's tendency (B fragments) and leaves as the next stage input.  The
+// running sum acc includes the step's start state, acc = c0 + dt/6 k1 +
+// dt/3 k2 + dt/3 k3, so the last stage reads acc alone and writes the step's
+// result to c0, which is also the next step's first stage input (stage_in):
+// per step 6 field reads and 7 field writes instead of 7 and 8 for a sum of
+// bare tendencies with a separate copy of the stage input.  The order of the
+// additions differs from the reference's (c0 + dt/6 (((k1 + 2 k2) + 2 k3) +
+// k4)) by FP64 rounding only.  stage < 0: the tendency goes to k and x
 // leaves as 0.  Elements past the mesh (the padding of the last block) take
 // a zero tendency, so their stored values stay 0; rows past NM belong to the
 // next block and are not stored.  The stage is uniform over the launch, so
@@ -325,32 +337,32 @@ __device__ __forceinline__ void update(const F32Args &A, int f, int eb, double (
     for (int s = 0; s < KM; s++) x[s] = 0.0;
     return;
   }
-  double c[KM], a[KM], y[KM];
+  // weight of this stage's tendency in the step's sum
+  const double w = (stage == 0 || stage == 3) ? dt / 6.0 : dt / 3.0;
+  double y[KM];
+  if (stage == 3) {
 #pragma unroll
-  for (int s = 0; s < KM; s++) c[s] = A.st.c0[f][mb + ix[s]];
-  if (stage == 0) {
+    for (int s = 0; s < KM; s++) y[s] = A.st.acc[f][mb + ix[s]];
+#pragma unroll
+    for (int s = 0; s < KM; s++) y[s] = y[s] + w * x[s];
+    store(A.st.c0[f], y);
+  } else {
+    double c[KM], a[KM];
+#pragma unroll
+    for (int s = 0; s < KM; s++) c[s] = A.st.c0[f][mb + ix[s]];
+    if (stage > 0) {
+#pragma unroll
+      for (int s = 0; s < KM; s++) a[s] = A.st.acc[f][mb + ix[s]];
+    }
+    const double h = stage == 2 ? dt : 0.5 * dt;
 #pragma unroll
     for (int s = 0; s < KM; s++) {
-      a[s] = x[s];
-      y[s] = c[s] + dt * x[s] * 0.5;
+      a[s] = (stage == 0 ? c[s] : a[s]) + w * x[s];
+      y[s] = c[s] + h * x[s];
     }
-  } else {
-#pragma unroll
-    for (int s = 0; s < KM; s++) a[s] = A.st.acc[f][mb + ix[s]];
-    if (stage == 3) {
-#pragma unroll
-      for (int s = 0; s < KM; s++) y[s] = c[s] + (dt / 6.0) * (a[s] + x[s]);
-    } else {
-      const double h = stage == 1 ? 0.5 : 1.0;  // (dt x) 1.0 == dt x exactly
-#pragma unroll
-      for (int s = 0; s < KM; s++) {
-        a[s] = a[s] + 2.0 * x[s];
-        y[s] = c[s] + dt * x[s] * h;
-      }
-    }
+    store(A.st.x64[f], y);
+    store(A.st.acc[f], a);
   }
-  store(A.st.x64[f], y);
-  store(stage == 3 ? A.st.c0[f] : A.st.acc[f], stage == 3 ? y : a);
 #pragma unroll
   for (int s = 0; s < KM; s++) x[s] = y[s];
 }
@@ -392,7 +404,7 @@ __device__ __forceinline__ void finish64(const F32Args &A, const double *O, int 
 }
 
 // ---------------------------------------------------------------------------
-// Stage input -> FP64 copy and traces (before the first stage).
+// Traces of the first stage input c0 (before the first stage).
 // ---------------------------------------------------------------------------
 template <int NC, int NG, int NGF, int VAR>
 __global__ __launch_bounds__(NT) void prep_kernel(F32Args A, int nblk) {
@@ -412,14 +424,7 @@ __global__ __launch_bounds__(NT) void prep_kernel(F32Args A, int nblk) {
   const size_t mb = (size_t)eb * 16 * NM;
   double x[3][KM];
 #pragma unroll
-  for (int f = 0; f < 3; f++) {
-    bload<NM>(A.st.c0[f] + mb, g, col, valid, x[f]);
-#pragma unroll
-    for (int s = 0; s < KM; s++) {
-      int m = 4 * s + g;
-      if (m < NM) A.st.x64[f][mb + m * 16 + col] = x[f][s];
-    }
-  }
+  for (int f = 0; f < 3; f++) bload<NM>(A.st.c0[f] + mb, g, col, valid, x[f]);
   traces_h<NC, NG, NGF>(A, O, eb, 0, x[2]);
   traces_uv<NC, NG, NGF, VAR>(A, O, eb, 0, x[0], x[1]);
 }
@@ -490,9 +495,9 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
   const float nwx = wall ? gg[14] : 0.f, nwy = wall ? gg[15] : 0.f;
 
   float bu[KM], bv[KM], bh[KM];
-  bload<NM>(A.st.x64[0] + mb, g, col, valid, bu);
-  bload<NM>(A.st.x64[1] + mb, g, col, valid, bv);
-  bload<NM>(A.st.x64[2] + mb, g, col, valid, bh);
+  bload<NM>(stage_in(A, 0) + mb, g, col, valid, bu);
+  bload<NM>(stage_in(A, 1) + mb, g, col, valid, bv);
+  bload<NM>(stage_in(A, 2) + mb, g, col, valid, bh);
 
   // B: q = weak gradient of (u, v) (src/calvarnl_fs.f:129-158); the face
   // terms use the averaged velocity traces times n/2
@@ -673,7 +678,7 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
   // its latency hides behind the mass-flux work below
   double X[3][KM];
 #pragma unroll
-  for (int f = 0; f < 3; f++) bload<NM>(A.st.x64[f] + mb, g, col, valid, X[f]);
+  for (int f = 0; f < 3; f++) bload<NM>(stage_in(A, f) + mb, g, col, valid, X[f]);
   // mass flux (wall-projected), its normal traces and its volume divergence
   float uh[KM], vh[KM];
   d2b(Pr[2], uh);
@@ -809,14 +814,16 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(eg_wave
   {
     // flux traces averaged over each face, times its/2; the eta equation's
     // lift carries the minus sign
-    const float *gg = A.mesh.geo + (size_t)(valid ? e : 0) * GEO;
-    const float di = valid ? gg[4] : 0.f;
+    // (the FP64 geometry the core just read, rounded: the FP32 copy holds
+    // the same values, and a second array would be a second HBM stream)
+    const double *gg = A.mesh.geo64 + (size_t)(valid ? e : 0) * GEO;
+    const float di = valid ? (float)gg[4] : 0.f;
     float fb[3][KE];
 #pragma unroll
     for (int s = 0; s < KE; s++) {
       const int kj = 4 * s + g, kc = kj < NE3 ? kj : NE3 - 1;
       const bool ok = valid && kj < NE3;
-      const float sg = 0.5f * gg[11 + kc / NGF];
+      const float sg = 0.5f * (float)gg[11 + kc / NGF];
       const FacePt fp = face_pt<NGF>(N, e, kc);
       const float f0 = sg * face_mean<NGF, 3>(A.st.trS, fp, 0);
       const float f1 = sg * face_mean<NGF, 3>(A.st.trS, fp, 1);
@@ -946,7 +953,7 @@ int launch(const F32Args &A, int phase, const double *in, double *out) {
     case 1:  // from psi
       from_psi_kernel<NM><<<(ne + PSI_WG - 1) / PSI_WG, PSI_WG, 0, st>>>(ne, A.ops.U, in, out);
       break;
-    case 2:  // stage input copy and traces
+    case 2:  // traces of the first stage input
       if (v == 0) prep_kernel<NC, NG, NGF, 0><<<grid, NT, 0, st>>>(A, nblk);
       else if (v == 1) prep_kernel<NC, NG, NGF, 1><<<grid, NT, 0, st>>>(A, nblk);
       else if (v == 2) prep_kernel<NC, NG, NGF, 2><<<grid, NT, 0, st>>>(A, nblk);
