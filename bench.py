@@ -83,17 +83,21 @@ def algorithmic_counts(nm, ng, ngf, variant):
     face_fl = 2 * 6 * nm * e3                             # core: lifts, traces
     trl_by = 2 * e3 * f8 + rk + 2 * e3 * f8               # trL in, RK, trL out
     if variant == "lin":
-        return {"lin_f64": (0, vol_fl + face_fl, 3 * nm * f8 + trl_by + mesh)}
-    # B+C: q (Dx, Dy, LF), traces of F and of the mass flux (TR), values at
-    # and projection from the quadrature points, the FP32 volume divergence
-    # (6 products) and the core's FP64 volume terms
+        # (the stage input is c0 at stage 0, read for the RK4 update anyway)
+        return {"lin_f64": (0, vol_fl + face_fl, 0.75 * 3 * nm * f8 + trl_by + mesh)}
+    # B+C (FP32 only): q (Dx, Dy, LF), traces of F and of the mass flux (TR),
+    # values at and projection from the quadrature points, the volume
+    # divergence (6 products)
     bc_fl = 2 * (4 * nm * nm + 10 * nm * e3 + 11 * nm * ng + 6 * nm * nm)
-    bc_by = (3 * nm * f8 + 2 * e3 * f4 + mesh + 16 * f4 + f8 + 4 * ng * f4 + 3 * f4
-             + 3 * nm * f8 + 3 * e3 * f4)                 # x64, trA, mesh, fields | gvol, trS
-    # E+G: the FP32 lift of the flux traces and the core's face terms
+    bc_by = (3 * nm * f8 + 2 * e3 * f4 + 16 * f4 + 6 * f4 + f4 + 4 * ng * f4 + 3 * f4
+             + 3 * nm * f4 + 3 * e3 * f4)      # x, trA, geo, nbr/nbe, bdl, fields | gvol, trS
+    # E+G: the FP32 lift of the flux traces; the FP64 core (volume and face
+    # terms) from the stage input, which is c0 at stage 0 (read for the RK4
+    # update anyway: 3 of 4 stages count)
     eg_fl = 2 * 3 * nm * e3
-    eg_by = (3 * nm * f8 + 3 * e3 * f4 + mesh + f8 + trl_by + 2 * e3 * f4)
-    return {"bc_f32": (bc_fl, vol_fl, bc_by), "eg_f32f64": (eg_fl, face_fl, eg_by)}
+    eg_by = (0.75 * 3 * nm * f8 + 3 * nm * f4 + 3 * e3 * f4 + mesh + f8 + trl_by
+             + 2 * e3 * f4)
+    return {"bc_f32": (bc_fl, 0, bc_by), "eg_f32f64": (eg_fl, vol_fl + face_fl, eg_by)}
 
 
 def measured_traffic(kernel, ne):

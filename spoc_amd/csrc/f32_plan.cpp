@@ -382,7 +382,7 @@ int f32_mesh_set(F32Plan *p, int ne, int nf, const int *itm, const int *its, con
     st.abar2 = st.abar + 4 * nt;
     st.btr = st.abar2 + 2 * nt;
   }
-  for (int f = 0; f < 3; f++) st.gvol[f] = s64 + (36 + f) * n64 + 12 * nt;
+  for (int f = 0; f < 3; f++) st.gvol[f] = (float *)(s64 + 36 * n64 + 12 * nt) + f * n64;
   st.trA[0] = s32;
   st.trA[1] = s32 + 2 * nt;
   st.trS = s32 + 4 * nt;

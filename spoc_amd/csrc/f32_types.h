@@ -140,7 +140,8 @@ struct F32State {
   double *c0[3], *acc[3];  // [NM rows] psi coordinates: step start, RK4 sum
   double *x64[3];          // [NM rows] stage input
   double *k[3];            // [NM rows] psi tendencies (stage < 0)
-  double *gvol[3];         // [NM rows] volume and source part of the tendency (B+C -> E+G)
+  float *gvol[3];          // [NM rows] FP32 volume and source part of the tendency
+                           // (B+C -> E+G; the FP64 core is E+G's own)
   float *trA[2];           // [2*NE3 rows] T_u, T_v of the stage input (by stage parity,
                            // like trL: the local Lax-Friedrichs option reads them in E+G)
   double *trL[2];          // [2*NE3 rows] T_eta, hh0 (T_u, T_v).n_face (wall-projected);

@@ -22,9 +22,9 @@
 // read the same two stored numbers, so the face flux is single-valued.
 //
 // Kernels per right-hand side:  linear: lin_kernel (all FP64);  nonlinear:
-// bc_kernel (every volume term: FP32 weak gradient q, quadrature-point
-// products, mass flux and their divergence, plus the FP64 linear core's
-// volume terms) then eg_kernel (every face term: FP32 and FP64 lifts, wall
+// bc_kernel (every FP32 volume term: weak gradient q, quadrature-point
+// products, mass flux and their divergence) then eg_kernel (the FP64 linear
+// core, volume and face terms; the FP32 lift of the flux traces; wall
 // projection, RK4 stage, traces of the next stage input).
 #include <hip/hip_runtime.h>
 
@@ -56,20 +56,15 @@ __device__ __forceinline__ const double *stage_in(const F32Args &A, int f) {
 // Linear gravity-wave core (VAR 0: the whole linear right-hand side,
 // src/calvar_lin.f; nonlinear variants: the -g grad(eta) terms and the
 // -div(hh0 u) term, src/calvarnl_fs.f:298-321, 436-471 and the ns/inv
-// equivalents).  MODE 0: volume and face terms, G = ...; MODE 1: volume
-// terms only (from the FP64 stage input), G += ...; MODE 2: face terms
-// only (from the FP64 traces trL), G = ....  dx, dy, lf: FP64 images of
-// Dx, Dy (MODE 0, 1) and LF (MODE 0, 2).  G[f] holds the three tendencies
-// (u, v, eta) in the result layout.  X, when given, holds the stage input
-// (u, v, eta B fragments) already loaded by the caller.
-template <int NC, int NG, int NGF, int VAR, int MODE, bool LF = false>
+// equivalents): volume terms from the FP64 stage input, face terms from
+// the FP64 traces trL.  dx, dy, lf: FP64 images of Dx, Dy and LF.  G[f]
+// receives the three tendencies (u, v, eta) in the result layout.
+template <int NC, int NG, int NGF, int VAR, bool LF = false>
 __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const double *dy,
                                        const double *lf, const Nbr3 &N, int eb,
-                                       doublex4 (&G)[3][S<NC, NG, NGF>::RBM],
-                                       const double (*X)[S<NC, NG, NGF>::KM] = nullptr) {
+                                       doublex4 (&G)[3][S<NC, NG, NGF>::RBM]) {
   using Sh = S<NC, NG, NGF>;
   constexpr int NM = Sh::NM, NE3 = Sh::NE3, KM = Sh::KM, KE = Sh::KE, RBM = Sh::RBM;
-  constexpr bool VOL = MODE != 2, LIFT = MODE != 1;
   const int l = lane(), g = l >> 4, col = l & 15, e0 = eb * 16, e = e0 + col;
   const bool valid = e < A.mesh.ne;
   const double *gg = A.mesh.geo64 + (size_t)(valid ? e : 0) * GEO;
@@ -89,17 +84,12 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
   const float *trA = (A.stage > 0 && (A.stage & 1)) ? A.st.trA[1] : A.st.trA[0];
   {  // u, v: -g grad(eta), volume and face terms
     double bh[KM], fx[KE], fy[KE];
-    if (VOL) {
-      if (X) {
-#pragma unroll
-        for (int s = 0; s < KM; s++) bh[s] = X[2][s];
-      } else {
-        bload<NM>(stage_in(A, 2) + mb, g, col, valid, bh);
-      }
+    {
+      bload<NM>(stage_in(A, 2) + mb, g, col, valid, bh);
 #pragma unroll
       for (int s = 0; s < KM; s++) bh[s] *= -g0;
     }
-    if (LIFT) {
+    {
 #pragma unroll
       for (int s = 0; s < KE; s++) {
         const int kj = 4 * s + g, kc = kj < NE3 ? kj : NE3 - 1, k = kc / NGF;
@@ -118,8 +108,8 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
     }
 #pragma unroll
     for (int rb = 0; rb < RBM; rb++) {
-      doublex4 Gx = zero64(), Gy = zero64();
-      if (VOL) {
+      doublex4 Gx, Gy;
+      {
         double adx[KM], ady[KM];
         afrag64<NM>(dx, rb, adx);
         afrag64<NM>(dy, rb, ady);
@@ -130,7 +120,7 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
           Gy[r] = bx * Ax[r] - ax * Ay[r];
         }
       }
-      if (LIFT) {
+      {
         double al[KE];
         afrag64<NE3>(lf, rb, al);
         Gx = chain64(al, fx, Gx);
@@ -138,25 +128,17 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
       }
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        G[0][rb][r] = (MODE == 1 ? G[0][rb][r] : 0.0) + di * Gx[r];
-        G[1][rb][r] = (MODE == 1 ? G[1][rb][r] : 0.0) + di * Gy[r];
+        G[0][rb][r] = di * Gx[r];
+        G[1][rb][r] = di * Gy[r];
       }
     }
   }
   phase_fence();
   {  // eta: -div(hh0 u); the nonlinear variants project the flux on walls
     double bu[KM], bv[KM], fh[KE];
-    if (VOL) {
-      if (X) {
-#pragma unroll
-        for (int s = 0; s < KM; s++) {
-          bu[s] = X[0][s];
-          bv[s] = X[1][s];
-        }
-      } else {
-        bload<NM>(stage_in(A, 0) + mb, g, col, valid, bu);
-        bload<NM>(stage_in(A, 1) + mb, g, col, valid, bv);
-      }
+    {
+      bload<NM>(stage_in(A, 0) + mb, g, col, valid, bu);
+      bload<NM>(stage_in(A, 1) + mb, g, col, valid, bv);
 #pragma unroll
       for (int s = 0; s < KM; s++) {
         bu[s] *= hh0;
@@ -173,7 +155,7 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
         bv[s] = -ay * u + ax * v;
       }
     }
-    if (LIFT) {
+    {
 #pragma unroll
       for (int s = 0; s < KE; s++) {
         const int kj = 4 * s + g, kc = kj < NE3 ? kj : NE3 - 1, k = kc / NGF;
@@ -191,20 +173,20 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
 #pragma unroll
     for (int rb = 0; rb < RBM; rb++) {
       doublex4 Gh = zero64();
-      if (VOL) {
+      {
         double adx[KM], ady[KM];
         afrag64<NM>(dx, rb, adx);
         afrag64<NM>(dy, rb, ady);
         Gh = chain64(adx, bu, Gh);
         Gh = chain64(ady, bv, Gh);
       }
-      if (LIFT) {
+      {
         double al[KE];
         afrag64<NE3>(lf, rb, al);
         Gh = chain64(al, fh, Gh);
       }
 #pragma unroll
-      for (int r = 0; r < 4; r++) G[2][rb][r] = (MODE == 1 ? G[2][rb][r] : 0.0) + di * Gh[r];
+      for (int r = 0; r < 4; r++) G[2][rb][r] = di * Gh[r];
     }
   }
 }
@@ -447,7 +429,7 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(3))) vo
   const int e = eb * 16 + (lane() & 15);
   const Nbr3 N = nbr3(A.mesh, e < A.mesh.ne ? e : 0);
   doublex4 G[3][Sh::RBM];
-  core64<NC, NG, NGF, 0, 0, LF>(A, O + P.dx, O + P.dy, O + P.lf, N, eb, G);
+  core64<NC, NG, NGF, 0, LF>(A, O + P.dx, O + P.dy, O + P.lf, N, eb, G);
   phase_fence();
   finish64<NC, NG, NGF, 0>(A, O, eb, G);
 }
@@ -457,9 +439,9 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(3))) vo
 // variants) and the volume half of E (src/calvarnl_fs.f:382-420): the
 // velocity gradient q, the viscous flux F = mu0 q, the quadrature-point
 // sources (advection, Coriolis, wind stress, bottom friction) and the mass
-// flux u (h + hb - hh0) in FP32, their volume divergence in FP32, plus the
-// linear core's volume terms in FP64.  Stores the sum (gvol, FP64) and the
-// flux traces (trS, FP32); the face terms are E+G's.
+// flux u (h + hb - hh0) in FP32, and their volume divergence in FP32.
+// Stores that volume part (gvol, FP32, as computed) and the flux traces
+// (trS, FP32); the linear core and every face term are E+G's.
 // ---------------------------------------------------------------------------
 template <int NC, int NG, int NGF, int VAR, int WG>
 __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_waves(NC)))) void bc_kernel(F32Args A, int nblk) {
@@ -467,23 +449,15 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
   constexpr int NM = Sh::NM, NE3 = Sh::NE3, KM = Sh::KM, KE = Sh::KE, KG = Sh::KG,
                 RBM = Sh::RBM, RBE = Sh::RBE, RBG = Sh::RBG;
   constexpr Pack32 P = Sh::P32;
-  constexpr Pack64 P64 = Sh::P64;
-  // 16-byte words; the FP64 Dx, Dy join the FP32 operators in LDS up to
-  // 64 KiB per workgroup, else they are read through the caches
-  constexpr int N32 = P.total / 4, N64 = (P64.total - P64.dx) / 2;
-  constexpr bool D64_LDS = (N32 + N64) * 16 <= 65536;
-  __shared__ floatx4 lds[N32 + (D64_LDS ? N64 : 0)];
+  constexpr int N32 = P.total / 4;  // 16-byte words
+  __shared__ floatx4 lds[N32];
   const int b0 = first_block<WG>();
   if (b0 >= nblk) return;
   stage_lds<WG>(lds, reinterpret_cast<const floatx4 *>(A.ops.pack32), N32);
-  if (D64_LDS)
-    stage_lds<WG>(lds + N32, reinterpret_cast<const floatx4 *>(A.ops.pack64 + P64.dx), N64);
   __syncthreads();
   const int eb = wave_block(b0);
   if (eb >= nblk) return;
   const float *O = reinterpret_cast<const float *>(lds);
-  const double *D64 = D64_LDS ? reinterpret_cast<const double *>(lds + N32)
-                              : A.ops.pack64 + P64.dx;  // FP64 Dx, Dy
   const int l = lane(), g = l >> 4, col = l & 15, e0 = eb * 16, e = e0 + col;
   const bool valid = e < A.mesh.ne;
   const size_t mb = (size_t)e0 * NM;
@@ -674,11 +648,6 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
     }
   }
   phase_fence();
-  // the FP64 stage input for the core's volume terms, loaded here so that
-  // its latency hides behind the mass-flux work below
-  double X[3][KM];
-#pragma unroll
-  for (int f = 0; f < 3; f++) bload<NM>(stage_in(A, f) + mb, g, col, valid, X[f]);
   // mass flux (wall-projected), its normal traces and its volume divergence
   float uh[KM], vh[KM];
   d2b(Pr[2], uh);
@@ -732,46 +701,38 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
     }
   }
   }
-  doublex4 G[3][RBM];
-  {
+  // the FP32 volume part of the three tendencies, stored as computed
 #pragma unroll
-    for (int s = 0; s < KM; s++) {
-      float u = uh[s], v = vh[s];
-      uh[s] = by * u - bx * v;
-      vh[s] = -ay * u + ax * v;
-    }
+  for (int s = 0; s < KM; s++) {
+    float u = uh[s], v = vh[s];
+    uh[s] = by * u - bx * v;
+    vh[s] = -ay * u + ax * v;
+  }
 #pragma unroll
-    for (int rb = 0; rb < RBM; rb++) {
-      float adx[KM], ady[KM];
-      afrag32<NM>(O + P.dx, rb, adx);
-      afrag32<NM>(O + P.dy, rb, ady);
-      floatx4 Vh = chain32(ady, vh, chain32(adx, uh, zero32()));
+  for (int rb = 0; rb < RBM; rb++) {
+    float adx[KM], ady[KM];
+    afrag32<NM>(O + P.dx, rb, adx);
+    afrag32<NM>(O + P.dy, rb, ady);
+    floatx4 Vh = chain32(ady, vh, chain32(adx, uh, zero32()));
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        G[0][rb][r] = Pr[0][rb][r];
-        G[1][rb][r] = Pr[1][rb][r];
-        G[2][rb][r] = di * Vh[r];
+    for (int r = 0; r < 4; r++) {
+      const int m = drow(rb, r, g);
+      if (m < NM) {
+        A.st.gvol[0][mb + m * 16 + col] = Pr[0][rb][r];
+        A.st.gvol[1][mb + m * 16 + col] = Pr[1][rb][r];
+        A.st.gvol[2][mb + m * 16 + col] = di * Vh[r];
       }
     }
   }
-  phase_fence();
-  core64<NC, NG, NGF, VAR, 1>(A, D64, D64 + (P64.dy - P64.dx), nullptr, Nbr3{}, eb, G, X);
-#pragma unroll
-  for (int f = 0; f < 3; f++)
-#pragma unroll
-    for (int rb = 0; rb < RBM; rb++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        int m = drow(rb, r, g);
-        if (m < NM) A.st.gvol[f][mb + m * 16 + col] = G[f][rb][r];
-      }
 }
 
 // ---------------------------------------------------------------------------
-// Nonlinear stages E + G (src/calvarnl_fs.f:382-544): the face terms (FP32
-// lift of the flux traces, FP64 lift of the linear core's traces) added to
-// the B+C volume part, wall projection, RK4 stage, traces of the next stage
-// input.  RUS: the local Lax-Friedrichs (Rusanov) option is on.
+// Nonlinear stages E + G (src/calvarnl_fs.f:382-544): the FP64 linear core
+// (volume terms from the FP64 stage input, face terms from its traces), the
+// FP32 lift of the flux traces and the B+C volume part, wall projection,
+// RK4 stage, traces of the next stage input.  The core runs here rather
+// than in B+C so that the B+C -> E+G hand-off is FP32 and B+C reads the
+// stage input once.  RUS: the local Lax-Friedrichs (Rusanov) option is on.
 // ---------------------------------------------------------------------------
 template <int NC, int NG, int NGF, int VAR, int WG, bool RUS>
 __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(eg_waves(NC)))) void eg_kernel(F32Args A, int nblk) {
@@ -779,7 +740,7 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(eg_wave
   constexpr int NM = Sh::NM, NE3 = Sh::NE3, KE = Sh::KE, RBM = Sh::RBM;
   constexpr Pack32 P = Sh::P32;
   constexpr Pack64 P64 = Sh::P64;
-  constexpr int N32 = (P.tr - P.lf) / 4, N64 = P64.dx / 2;  // 16-byte words
+  constexpr int N32 = (P.tr - P.lf) / 4, N64 = P64.total / 2;  // 16-byte words
   __shared__ floatx4 lds[N32 + N64];
   const int b0 = first_block<WG>();
   if (b0 >= nblk) return;
@@ -789,17 +750,18 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(eg_wave
   const int eb = wave_block(b0);
   if (eb >= nblk) return;
   const float *LF = reinterpret_cast<const float *>(lds);           // FP32 LF
-  const double *O64 = reinterpret_cast<const double *>(lds + N32);  // FP64 LF, TR
+  const double *O64 = reinterpret_cast<const double *>(lds + N32);  // FP64 LF, TR, Dx, Dy
   const int l = lane(), g = l >> 4, col = l & 15, e0 = eb * 16, e = e0 + col;
   const bool valid = e < A.mesh.ne;
   const size_t mb = (size_t)e0 * NM;
   const Nbr3 N = nbr3(A.mesh, valid ? e : 0);
 
-  // FP64 face terms, then the volume part B+C stored (gvol) added to them
-  // (read only now, so that its 48 registers are free while the face loop
-  // keeps its loads in flight)
+  // the FP64 gravity-wave core (volume and face terms, from the FP64 stage
+  // input and traces), then the FP32 volume part B+C stored (gvol) added to
+  // it (read only now, so that its registers are free while the core keeps
+  // its loads in flight)
   doublex4 G[3][RBM];
-  core64<NC, NG, NGF, VAR, 2, RUS>(A, nullptr, nullptr, O64 + P64.lf, N, eb, G);
+  core64<NC, NG, NGF, VAR, RUS>(A, O64 + P64.dx, O64 + P64.dy, O64 + P64.lf, N, eb, G);
 #pragma unroll
   for (int f = 0; f < 3; f++)
 #pragma unroll
@@ -807,8 +769,8 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(eg_wave
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int m = drow(rb, r, g), mc = m < NM ? m : NM - 1;
-        const double v = A.st.gvol[f][mb + mc * 16 + col];
-        G[f][rb][r] += (valid && m < NM) ? v : 0.0;
+        const float v = A.st.gvol[f][mb + mc * 16 + col];
+        G[f][rb][r] += (valid && m < NM) ? (double)v : 0.0;
       }
   phase_fence();
   {
