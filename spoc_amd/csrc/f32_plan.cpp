@@ -227,6 +227,7 @@ int f32_plan_create(F32Plan **out, int nc, int ng, int ngf, const double *amm, c
     h32[P.xq + q] = (float)PI2(q, nc + 1);  // P_1(xi) = xi
     h32[P.yq + q] = (float)PI2(q, 1);       // P_1(eta) = eta
   }
+  for (int m = 0; m < nm; m++) h32[P.tail + m] = (float)psi(ng - 1, m);
   img32(g32.data(), nm, nm, gl);
   img64(h64.data() + Q.dx, nm, nm, dx);
   img64(h64.data() + Q.dy, nm, nm, dy);

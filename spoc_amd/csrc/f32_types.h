@@ -41,7 +41,7 @@ __host__ __device__ constexpr int sigma_row(int i) {
 // FP32 operators, one buffer, offsets in floats.  B+C stages all of it in
 // LDS, E+G only [lf, tr).
 struct Pack32 {
-  int dx, dy, lf, tr, psi, psiw, xq, yq, total;
+  int dx, dy, lf, tr, psi, psiw, xq, yq, tail, total;
 };
 __host__ __device__ constexpr Pack32 pack32(int nm, int ng, int ne3) {
   Pack32 p{};
@@ -54,6 +54,8 @@ __host__ __device__ constexpr Pack32 pack32(int nm, int ng, int ne3) {
   p.psiw = o; o += img32_size(nm, ng);   // (W pi2 U^-1)^T: projection
   p.xq = o;   o += cdiv(ng, 16) * 16;    // quadrature points
   p.yq = o;   o += cdiv(ng, 16) * 16;
+  p.tail = o; o += cdiv(nm, 16) * 16;    // row ng-1 of pi2 U^-1, modes in order (a
+                                         // last row block holding one point)
   p.total = o;
   return p;
 }

@@ -602,18 +602,55 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
   // the block's quadrature-point values: one 1 KiB wave load per k-step,
   // each k-block's issued before the products that need them
   const floatx4 *qf = reinterpret_cast<const floatx4 *>(A.fld.qf) + (size_t)eb * KG * 64 + l;
+  // (the last row block of points holds only point NG - 1, at k-step KG - 1)
+  constexpr bool TAIL1 = NG % 16 == 1;
+  static_assert(!TAIL1 || KG == 4 * (RBG - 1) + 1, "tail point layout");
 #pragma unroll
   for (int qb = 0; qb < RBG; qb++) {
     floatx4 F[4];
 #pragma unroll
     for (int r = 0; r < 4; r++)
       if (4 * qb + r < KG) F[r] = qf[(4 * qb + r) * 64];
-    float ap[KM];
-    afrag32<NM>(O + P.psi, qb, ap);
-    floatx4 Vu = chain32(ap, bu, zero32()), Vv = chain32(ap, bv, zero32());
-    floatx4 Vh = chain32(ap, bh, zero32());
-    floatx4 Vxx = chain32(ap, q0, zero32()), Vxy = chain32(ap, q1, zero32());
-    floatx4 Vyx = chain32(ap, q2, zero32()), Vyy = chain32(ap, q3, zero32());
+    floatx4 Vu, Vv, Vh, Vxx, Vxy, Vyx, Vyy;
+    if (TAIL1 && qb == RBG - 1) {
+      // one point (the last) in this row block: lane-local dot products over
+      // the lane's modes, summed over the four lane groups, in place of
+      // 7 x KM MFMAs with one useful row
+      float t[7] = {};
+#pragma unroll
+      for (int s = 0; s < KM; s++) {
+        const float a = O[P.tail + 4 * s + g];
+        t[0] += a * bu[s];
+        t[1] += a * bv[s];
+        t[2] += a * bh[s];
+        t[3] += a * q0[s];
+        t[4] += a * q1[s];
+        t[5] += a * q2[s];
+        t[6] += a * q3[s];
+      }
+#pragma unroll
+      for (int i = 0; i < 7; i++) {
+        t[i] += __shfl_xor(t[i], 16);
+        t[i] += __shfl_xor(t[i], 32);
+      }
+      Vu = floatx4{t[0], 0.f, 0.f, 0.f};
+      Vv = floatx4{t[1], 0.f, 0.f, 0.f};
+      Vh = floatx4{t[2], 0.f, 0.f, 0.f};
+      Vxx = floatx4{t[3], 0.f, 0.f, 0.f};
+      Vxy = floatx4{t[4], 0.f, 0.f, 0.f};
+      Vyx = floatx4{t[5], 0.f, 0.f, 0.f};
+      Vyy = floatx4{t[6], 0.f, 0.f, 0.f};
+    } else {
+      float ap[KM];
+      afrag32<NM>(O + P.psi, qb, ap);
+      Vu = chain32(ap, bu, zero32());
+      Vv = chain32(ap, bv, zero32());
+      Vh = chain32(ap, bh, zero32());
+      Vxx = chain32(ap, q0, zero32());
+      Vxy = chain32(ap, q1, zero32());
+      Vyx = chain32(ap, q2, zero32());
+      Vyy = chain32(ap, q3, zero32());
+    }
     float pg[4][4];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
