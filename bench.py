@@ -91,10 +91,10 @@ def algorithmic_counts(nm, ng, ngf, variant):
     bc_fl = 2 * (4 * nm * nm + 10 * nm * e3 + 11 * nm * ng + 6 * nm * nm)
     bc_by = (3 * nm * f8 + 2 * e3 * f4 + 16 * f4 + 6 * f4 + f4 + 4 * ng * f4 + 3 * f4
              + 3 * nm * f4 + 3 * e3 * f4)      # x, trA, geo, nbr/nbe, bdl, fields | gvol, trS
-    # E+G: the FP32 lift of the flux traces; the FP64 core (volume and face
-    # terms) from the stage input, which is c0 at stage 0 (read for the RK4
-    # update anyway: 3 of 4 stages count)
-    eg_fl = 2 * 3 * nm * e3
+    # E+G: the FP64 core (volume and face terms; the flux traces' face
+    # values are lifted by the same product) from the stage input, which is
+    # c0 at stage 0 (read for the RK4 update anyway: 3 of 4 stages count)
+    eg_fl = 0
     eg_by = (0.75 * 3 * nm * f8 + 3 * nm * f4 + 3 * e3 * f4 + mesh + f8 + trl_by
              + 2 * e3 * f4)
     return {"bc_f32": (bc_fl, 0, bc_by), "eg_f32f64": (eg_fl, vol_fl + face_fl, eg_by)}

@@ -24,8 +24,8 @@
 // Kernels per right-hand side:  linear: lin_kernel (all FP64);  nonlinear:
 // bc_kernel (every FP32 volume term: weak gradient q, quadrature-point
 // products, mass flux and their divergence) then eg_kernel (the FP64 linear
-// core, volume and face terms; the FP32 lift of the flux traces; wall
-// projection, RK4 stage, traces of the next stage input).
+// core, volume and face terms, with the flux traces' face values lifted
+// along; wall projection, RK4 stage, traces of the next stage input).
 #include <hip/hip_runtime.h>
 
 #include "mfma.h"
@@ -58,8 +58,12 @@ __device__ __forceinline__ const double *stage_in(const F32Args &A, int f) {
 // -div(hh0 u) term, src/calvarnl_fs.f:298-321, 436-471 and the ns/inv
 // equivalents): volume terms from the FP64 stage input, face terms from
 // the FP64 traces trL.  dx, dy, lf: FP64 images of Dx, Dy and LF.  G[f]
-// receives the three tendencies (u, v, eta) in the result layout.
-template <int NC, int NG, int NGF, int VAR, bool LF = false>
+// receives the three tendencies (u, v, eta) in the result layout.  FLUX
+// (nonlinear variants): the face values of the FP32 flux traces trS (the
+// viscous flux for u and v, the remaining mass flux for eta) join the
+// core's own before the lift, which is linear, so one FP64 product lifts
+// both, and their gathers issue with the core's.
+template <int NC, int NG, int NGF, int VAR, bool LF = false, bool FLUX = false>
 __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const double *dy,
                                        const double *lf, const Nbr3 &N, int eb,
                                        doublex4 (&G)[3][S<NC, NG, NGF>::RBM]) {
@@ -102,8 +106,13 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
           px = w * (double)(tr_at<NE3, 2>(trA, fp.eb, 0, fp.kb) - tr_at<NE3, 2>(trA, fp.ea, 0, fp.ka));
           py = w * (double)(tr_at<NE3, 2>(trA, fp.eb, 1, fp.kb) - tr_at<NE3, 2>(trA, fp.ea, 1, fp.ka));
         }
-        fx[s] = ok ? 0.5 * fh * gg[11 + k] * gg[5 + k] + px : 0.0;
-        fy[s] = ok ? 0.5 * fh * gg[11 + k] * gg[8 + k] + py : 0.0;
+        double sx = 0.0, sy = 0.0;
+        if (FLUX) {
+          sx = 0.5 * gg[11 + k] * (double)face_mean<NGF, 3>(A.st.trS, fp, 0);
+          sy = 0.5 * gg[11 + k] * (double)face_mean<NGF, 3>(A.st.trS, fp, 1);
+        }
+        fx[s] = ok ? 0.5 * fh * gg[11 + k] * gg[5 + k] + px + sx : 0.0;
+        fy[s] = ok ? 0.5 * fh * gg[11 + k] * gg[8 + k] + py + sy : 0.0;
       }
     }
 #pragma unroll
@@ -162,6 +171,7 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
         const bool ok = valid && kj < NE3;
         const FacePt fp = face_pt<NGF>(N, e, kc);
         double v = -0.5 * gg[11 + k] * face_mean<NGF, 2>(trL, fp, 1);
+        if (FLUX) v -= 0.5 * gg[11 + k] * (double)face_mean<NGF, 3>(A.st.trS, fp, 2);
         if (lfo) {
           const double nx = gg[5 + k], ny = gg[8 + k];
           v += 0.25 * lfc * sqrt(nx * nx + ny * ny) *
@@ -766,39 +776,37 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
 // ---------------------------------------------------------------------------
 // Nonlinear stages E + G (src/calvarnl_fs.f:382-544): the FP64 linear core
 // (volume terms from the FP64 stage input, face terms from its traces), the
-// FP32 lift of the flux traces and the B+C volume part, wall projection,
-// RK4 stage, traces of the next stage input.  The core runs here rather
-// than in B+C so that the B+C -> E+G hand-off is FP32 and B+C reads the
-// stage input once.  RUS: the local Lax-Friedrichs (Rusanov) option is on.
+// face values of the flux traces (lifted with the core's, in FP64) and the
+// B+C volume part, wall projection, RK4 stage, traces of the next stage
+// input.  The core runs here rather than in B+C so that the B+C -> E+G
+// hand-off is FP32 and B+C reads the stage input once.  RUS: the local
+// Lax-Friedrichs (Rusanov) option is on.
 // ---------------------------------------------------------------------------
 template <int NC, int NG, int NGF, int VAR, int WG, bool RUS>
 __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(eg_waves(NC)))) void eg_kernel(F32Args A, int nblk) {
   using Sh = S<NC, NG, NGF>;
-  constexpr int NM = Sh::NM, NE3 = Sh::NE3, KE = Sh::KE, RBM = Sh::RBM;
-  constexpr Pack32 P = Sh::P32;
+  constexpr int NM = Sh::NM, RBM = Sh::RBM;
   constexpr Pack64 P64 = Sh::P64;
-  constexpr int N32 = (P.tr - P.lf) / 4, N64 = P64.total / 2;  // 16-byte words
-  __shared__ floatx4 lds[N32 + N64];
+  constexpr int N64 = P64.total / 2;  // 16-byte words
+  __shared__ floatx4 lds[N64];
   const int b0 = first_block<WG>();
   if (b0 >= nblk) return;
-  stage_lds<WG>(lds, reinterpret_cast<const floatx4 *>(A.ops.pack32 + P.lf), N32);
-  stage_lds<WG>(lds + N32, reinterpret_cast<const floatx4 *>(A.ops.pack64), N64);
+  stage_lds<WG>(lds, reinterpret_cast<const floatx4 *>(A.ops.pack64), N64);
   __syncthreads();
   const int eb = wave_block(b0);
   if (eb >= nblk) return;
-  const float *LF = reinterpret_cast<const float *>(lds);           // FP32 LF
-  const double *O64 = reinterpret_cast<const double *>(lds + N32);  // FP64 LF, TR, Dx, Dy
+  const double *O64 = reinterpret_cast<const double *>(lds);  // FP64 LF, TR, Dx, Dy
   const int l = lane(), g = l >> 4, col = l & 15, e0 = eb * 16, e = e0 + col;
   const bool valid = e < A.mesh.ne;
   const size_t mb = (size_t)e0 * NM;
   const Nbr3 N = nbr3(A.mesh, valid ? e : 0);
 
   // the FP64 gravity-wave core (volume and face terms, from the FP64 stage
-  // input and traces), then the FP32 volume part B+C stored (gvol) added to
-  // it (read only now, so that its registers are free while the core keeps
-  // its loads in flight)
+  // input and traces) with the face values of the flux traces lifted along,
+  // then the FP32 volume part B+C stored (gvol) added to it (read only now,
+  // so that its registers are free while the core keeps its loads in flight)
   doublex4 G[3][RBM];
-  core64<NC, NG, NGF, VAR, RUS>(A, O64 + P64.dx, O64 + P64.dy, O64 + P64.lf, N, eb, G);
+  core64<NC, NG, NGF, VAR, RUS, true>(A, O64 + P64.dx, O64 + P64.dy, O64 + P64.lf, N, eb, G);
 #pragma unroll
   for (int f = 0; f < 3; f++)
 #pragma unroll
@@ -809,40 +817,6 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(eg_wave
         const float v = A.st.gvol[f][mb + mc * 16 + col];
         G[f][rb][r] += (valid && m < NM) ? (double)v : 0.0;
       }
-  phase_fence();
-  {
-    // flux traces averaged over each face, times its/2; the eta equation's
-    // lift carries the minus sign
-    // (the FP64 geometry the core just read, rounded: the FP32 copy holds
-    // the same values, and a second array would be a second HBM stream)
-    const double *gg = A.mesh.geo64 + (size_t)(valid ? e : 0) * GEO;
-    const float di = valid ? (float)gg[4] : 0.f;
-    float fb[3][KE];
-#pragma unroll
-    for (int s = 0; s < KE; s++) {
-      const int kj = 4 * s + g, kc = kj < NE3 ? kj : NE3 - 1;
-      const bool ok = valid && kj < NE3;
-      const float sg = 0.5f * (float)gg[11 + kc / NGF];
-      const FacePt fp = face_pt<NGF>(N, e, kc);
-      const float f0 = sg * face_mean<NGF, 3>(A.st.trS, fp, 0);
-      const float f1 = sg * face_mean<NGF, 3>(A.st.trS, fp, 1);
-      const float f2 = -sg * face_mean<NGF, 3>(A.st.trS, fp, 2);
-      fb[0][s] = ok ? f0 : 0.f;
-      fb[1][s] = ok ? f1 : 0.f;
-      fb[2][s] = ok ? f2 : 0.f;
-    }
-#pragma unroll
-    for (int rb = 0; rb < RBM; rb++) {
-      float al[KE];
-      afrag32<NE3>(LF, rb, al);
-#pragma unroll
-      for (int f = 0; f < 3; f++) {
-        floatx4 acc = chain32(al, fb[f], zero32());
-#pragma unroll
-        for (int r = 0; r < 4; r++) G[f][rb][r] += (double)(di * acc[r]);
-      }
-    }
-  }
   phase_fence();
   finish64<NC, NG, NGF, VAR>(A, O64, eb, G);
 }
