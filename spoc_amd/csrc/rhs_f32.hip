@@ -33,9 +33,16 @@
 namespace spoc {
 namespace f32 {
 
+// E+G gives each block two waves (one for the u and v equations, one for
+// eta) at p = 5 and 6: either half fits in 128 registers where the whole
+// needs about 165, so four waves share a SIMD instead of three (measured:
+// E+G 240 -> 234 us at p = 6 on 199,362 triangles, 52 -> 46 us at p = 5 on
+// 49,952).  At p = 4 the whole already fits four waves and at p = 7 the
+// halves still need two; there one wave per block measured faster.
+constexpr bool eg_split(int nc) { return nc == 5 || nc == 6; }
 // Register budgets as waves per SIMD (512 registers per lane shared by the
 // resident waves): the element kernels are memory-latency bound, so the
-// compiler is held to at least three resident waves.
+// compiler is held to at least three resident waves (four for a split E+G).
 #ifndef BC_WAVES
 #define BC_WAVES 3
 #endif
@@ -44,7 +51,7 @@ namespace f32 {
 #endif
 // (at p >= 7, 36 modes, both kernels get two waves, or they spill)
 constexpr int bc_waves(int nc) { return nc >= 7 ? 2 : BC_WAVES; }
-constexpr int eg_waves(int nc) { return nc >= 7 ? 2 : EG_WAVES; }
+constexpr int eg_waves(int nc) { return nc >= 7 ? 2 : eg_split(nc) ? 4 : EG_WAVES; }
 
 // Stage input of field f: the step's start state c0 at the first stage (and
 // for a bare right-hand side), else what the previous stage stored in x64.
@@ -62,8 +69,9 @@ __device__ __forceinline__ const double *stage_in(const F32Args &A, int f) {
 // (nonlinear variants): the face values of the FP32 flux traces trS (the
 // viscous flux for u and v, the remaining mass flux for eta) join the
 // core's own before the lift, which is linear, so one FP64 product lifts
-// both, and their gathers issue with the core's.
-template <int NC, int NG, int NGF, int VAR, bool LF = false, bool FLUX = false>
+// both, and their gathers issue with the core's.  ROLE: 0 all three
+// equations, 1 those of u and v only (G[0], G[1]), 2 that of eta only (G[2]).
+template <int NC, int NG, int NGF, int VAR, bool LF = false, bool FLUX = false, int ROLE = 0>
 __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const double *dy,
                                        const double *lf, const Nbr3 &N, int eb,
                                        doublex4 (&G)[3][S<NC, NG, NGF>::RBM]) {
@@ -86,7 +94,7 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
   constexpr bool lfo = LF;
   const double lfc = lfo ? A.lf_alpha * sqrt(g0 * hh0) : 0.0;
   const float *trA = (A.stage > 0 && (A.stage & 1)) ? A.st.trA[1] : A.st.trA[0];
-  {  // u, v: -g grad(eta), volume and face terms
+  if (ROLE != 2) {  // u, v: -g grad(eta), volume and face terms
     double bh[KM], fx[KE], fy[KE];
     {
       bload<NM>(stage_in(A, 2) + mb, g, col, valid, bh);
@@ -143,7 +151,7 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
     }
   }
   phase_fence();
-  {  // eta: -div(hh0 u); the nonlinear variants project the flux on walls
+  if (ROLE != 1) {  // eta: -div(hh0 u); the nonlinear variants project the flux on walls
     double bu[KM], bv[KM], fh[KE];
     {
       bload<NM>(stage_in(A, 0) + mb, g, col, valid, bu);
@@ -363,7 +371,7 @@ __device__ __forceinline__ void update(const F32Args &A, int f, int eb, double (
 // (free slip for VAR 0, 1, 3 and no slip for VAR 2, src/calvar_lin.f and
 // src/calvarnl_*.f at the end of cal_var), then either the tendencies
 // (stage < 0) or the RK4 stage and the traces of the next stage input.
-template <int NC, int NG, int NGF, int VAR>
+template <int NC, int NG, int NGF, int VAR, int ROLE = 0>
 __device__ __forceinline__ void finish64(const F32Args &A, const double *O, int eb,
                                          doublex4 (&G)[3][S<NC, NG, NGF>::RBM]) {
   using Sh = S<NC, NG, NGF>;
@@ -371,13 +379,14 @@ __device__ __forceinline__ void finish64(const F32Args &A, const double *O, int 
   const int col = lane() & 15, e = eb * 16 + col;
   const bool valid = e < A.mesh.ne;
   const int w = (A.stage + 1) & 1;
-  {
+  if (ROLE != 1) {
     double xh[KM];
 #pragma unroll
     for (int s = 0; s < KM; s++) xh[s] = G[2][s >> 2][s & 3];
     update<NM>(A, 2, eb, xh);
     if (A.stage >= 0) traces_h<NC, NG, NGF>(A, O, eb, w, xh);
   }
+  if (ROLE == 2) return;
   phase_fence();
   double xu[KM], xv[KM];
 #pragma unroll
@@ -782,33 +791,24 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
 // hand-off is FP32 and B+C reads the stage input once.  RUS: the local
 // Lax-Friedrichs (Rusanov) option is on.
 // ---------------------------------------------------------------------------
-template <int NC, int NG, int NGF, int VAR, int WG, bool RUS>
-__global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(eg_waves(NC)))) void eg_kernel(F32Args A, int nblk) {
+template <int NC, int NG, int NGF, int VAR, int ROLE, bool RUS>
+__device__ __forceinline__ void eg_part(const F32Args &A, const double *O64, int eb) {
   using Sh = S<NC, NG, NGF>;
   constexpr int NM = Sh::NM, RBM = Sh::RBM;
   constexpr Pack64 P64 = Sh::P64;
-  constexpr int N64 = P64.total / 2;  // 16-byte words
-  __shared__ floatx4 lds[N64];
-  const int b0 = first_block<WG>();
-  if (b0 >= nblk) return;
-  stage_lds<WG>(lds, reinterpret_cast<const floatx4 *>(A.ops.pack64), N64);
-  __syncthreads();
-  const int eb = wave_block(b0);
-  if (eb >= nblk) return;
-  const double *O64 = reinterpret_cast<const double *>(lds);  // FP64 LF, TR, Dx, Dy
   const int l = lane(), g = l >> 4, col = l & 15, e0 = eb * 16, e = e0 + col;
   const bool valid = e < A.mesh.ne;
   const size_t mb = (size_t)e0 * NM;
   const Nbr3 N = nbr3(A.mesh, valid ? e : 0);
-
   // the FP64 gravity-wave core (volume and face terms, from the FP64 stage
   // input and traces) with the face values of the flux traces lifted along,
   // then the FP32 volume part B+C stored (gvol) added to it (read only now,
   // so that its registers are free while the core keeps its loads in flight)
   doublex4 G[3][RBM];
-  core64<NC, NG, NGF, VAR, RUS, true>(A, O64 + P64.dx, O64 + P64.dy, O64 + P64.lf, N, eb, G);
+  core64<NC, NG, NGF, VAR, RUS, true, ROLE>(A, O64 + P64.dx, O64 + P64.dy, O64 + P64.lf, N, eb,
+                                            G);
 #pragma unroll
-  for (int f = 0; f < 3; f++)
+  for (int f = (ROLE == 2 ? 2 : 0); f < (ROLE == 1 ? 2 : 3); f++)
 #pragma unroll
     for (int rb = 0; rb < RBM; rb++)
 #pragma unroll
@@ -818,7 +818,32 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(eg_wave
         G[f][rb][r] += (valid && m < NM) ? (double)v : 0.0;
       }
   phase_fence();
-  finish64<NC, NG, NGF, VAR>(A, O64, eb, G);
+  finish64<NC, NG, NGF, VAR, ROLE>(A, O64, eb, G);
+}
+
+// Nonlinear stages E + G.  Split (eg_split): each block of 16 elements gets
+// two waves, one for the u and v equations (core, lift, wall projection,
+// RK4 stage, velocity traces) and one for the eta equation (core, lift, RK4
+// stage, eta traces); WG waves per workgroup then cover WG / 2 blocks.
+// Otherwise one wave does the whole block.
+template <int NC, int NG, int NGF, int VAR, int WG, bool RUS>
+__global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(eg_waves(NC)))) void eg_kernel(F32Args A, int nblk) {
+  using Sh = S<NC, NG, NGF>;
+  constexpr Pack64 P64 = Sh::P64;
+  constexpr int N64 = P64.total / 2;  // 16-byte words
+  constexpr bool SPLIT = eg_split(NC);
+  __shared__ floatx4 lds[N64];
+  const int b0 = first_block<SPLIT ? WG / 2 : WG>();
+  if (b0 >= nblk) return;
+  stage_lds<WG>(lds, reinterpret_cast<const floatx4 *>(A.ops.pack64), N64);
+  __syncthreads();
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int eb = b0 + (SPLIT ? wv >> 1 : wv);
+  if (eb >= nblk) return;
+  const double *O64 = reinterpret_cast<const double *>(lds);  // FP64 LF, TR, Dx, Dy
+  if (!SPLIT) eg_part<NC, NG, NGF, VAR, 0, RUS>(A, O64, eb);
+  else if (wv & 1) eg_part<NC, NG, NGF, VAR, 2, RUS>(A, O64, eb);
+  else eg_part<NC, NG, NGF, VAR, 1, RUS>(A, O64, eb);
 }
 
 // Basis change, FP64.  to_psi: c = U a from the reference layout a[e*NM + m]
@@ -889,7 +914,8 @@ __global__ __launch_bounds__(PSI_WG) void from_psi_kernel(int ne, const double *
 // of 16 elements per workgroup: the tile the autotuner picks).
 template <int NC, int NG, int NGF, int WG>
 int launch_elem(const F32Args &A, int phase, int nblk) {
-  const int grid = 8 * cdiv(cdiv(nblk, WG), 8);  // a multiple of the 8 XCDs
+  // a multiple of the 8 XCDs; a split E+G gives each block two waves
+  const int grid = 8 * cdiv(cdiv(nblk, phase == 5 && eg_split(NC) ? WG / 2 : WG), 8);
   hipStream_t st = A.stream;
   const int v = A.variant;
   const bool lf = A.lf_alpha != 0.0;
