@@ -39,7 +39,7 @@ __host__ __device__ constexpr int sigma_row(int i) {
 }
 
 // FP32 operators, one buffer, offsets in floats.  B+C stages all of it in
-// LDS, E+G only [lf, tr).
+// LDS.
 struct Pack32 {
   int dx, dy, lf, tr, psi, psiw, xq, yq, tail, total;
 };
@@ -60,8 +60,9 @@ __host__ __device__ constexpr Pack32 pack32(int nm, int ng, int ne3) {
   return p;
 }
 
-// FP64 operators (same meaning), offsets in doubles.  The face operators come
-// first: E+G and the prep kernel stage [0, dx), B+C stages [dx, total).
+// FP64 operators (same meaning), offsets in doubles.  The face operators
+// come first: the prep kernel stages [0, dx); E+G and the linear kernel
+// stage all of it.
 struct Pack64 {
   int lf, tr, dx, dy, total;
 };
