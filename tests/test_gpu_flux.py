@@ -114,6 +114,50 @@ def test_nonlinear_variant_and_graph_replay():
         assert np.isfinite(a).all()
 
 
+def test_split_eg_path_p6():
+    """At p = 6 E+G runs two waves per element block (u, v and eta); the
+    option there, on the nonlinear free-slip variant started from the
+    smooth Gaussian of the linear basin: graph replay equals eager launches,
+    mass is conserved as with the central flux (the penalty moves the same
+    flux across both sides of a face, and none through walls), and the
+    smooth solution changes little."""
+    t = host_tables(6, 33, 7)
+    c = configs.config1(t)
+    ctx = spoc_amd.Context(t)
+    ctx.set_mesh(c["mesh"])
+    p = c["phys"]
+    ctx.set_fields(p["hb"], p["g0"], p["fc"], p["w0"], p["z0"], p["mu0"], p["mb0"])
+    say("flux: fs p=6 (split E+G)")
+    out, mass = [], []
+    for alpha, graph in ((0.0, True), (1.0, False), (1.0, True)):
+        ctx.set_graph(graph)
+        ctx.set_flux(alpha)
+        ctx.upload(*c["state"])
+        m0 = ctx.energy()[2]
+        ctx.rk4("fs", c["dt"], 40, precision="fp32")
+        mass.append((m0, ctx.energy()[2]))
+        out.append(ctx.download())
+    ctx.set_flux(0.0)
+    ctx.set_graph(True)
+    for a, b in zip(out[1], out[2]):
+        assert np.array_equal(a, b)
+        assert np.isfinite(a).all()
+    scale = abs(mass[0][0]) + 1.0
+    drift_c = abs(mass[0][1] - mass[0][0])
+    drift_r = abs(mass[1][1] - mass[1][0])
+    say("  mass drift central", drift_c, "rusanov", drift_r)
+    assert drift_r <= 10 * drift_c + 1e-12 * scale
+    nm = t["nm"]
+    M = t["ammo"][:nm, :nm]
+    det = 1.0 / c["mesh"]["vect"][4]
+    d = out[1][2][:nm] - out[0][2][:nm]
+    ref = out[0][2][:nm]
+    eh = float(np.sqrt(np.sum(det * np.einsum("ie,ij,je->e", d, M, d)) /
+                       np.sum(det * np.einsum("ie,ij,je->e", ref, M, ref))))
+    say("  eta change", eh)
+    assert 0 < eh < 2e-2, eh
+
+
 def test_central_only_paths_refuse_the_option():
     t, c, ctx = lin_case()
     ctx.set_flux(1.0)
