@@ -236,15 +236,12 @@ def adapt_timing(ctx, state, ne, nc, ng, ngf, cpu, reps=20, fac=(0.1, 0.01, 1.0)
     return out
 
 
-def cpu_baseline(variant, nc, ng, ngf, budget_s=15.0, wl="c2"):
-    """The reference Fortran (oracle/_ref) on a bounded sample of the same
-    workload: same p, rule and physics on a reduced mesh (at most 1600
-    triangles, what the reference's static arrays hold), one core.  Rules of
-    more than 33 points exceed the reference's arrays; then the C
-    restatement runs instead (kind "port")."""
+def _cpu_case(variant, nc, ng, ngf, wl):
+    """The bounded CPU sample of workload wl: the reference Fortran
+    (oracle/_ref) when it can hold the rule, else the C restatement.
+    Returns (run(nsteps), ne, nm, kind)."""
     import oracle
     from spoc_amd import configs
-    t = {"nc": nc, "ng": ng, "ngf": ngf}
     O = oracle.Oracle()
     O.setup(nc, *quad_rule(ng, ngf))
     tabs = O.tables()
@@ -259,43 +256,91 @@ def cpu_baseline(variant, nc, ng, ngf, budget_s=15.0, wl="c2"):
     c = small[wl]()
     m, p = c["mesh"], c["phys"]
     ne, nm = m["ne"], tabs["nm"]
-    kind, runner = "port", None
     # the reference's static arrays hold at most 33 quadrature points
     if oracle.ref_available() and ng <= 33:
         try:
             R = oracle.RefLib(variant)
             R.setup(nc, *quad_rule(ng, ngf))
-            kind = "reference"
-            runner = lambda n: R.rk4(m, p, *c["state"], c["dt"], n)
+            return (lambda n: R.rk4(m, p, *c["state"], c["dt"], n)), ne, nm, "reference"
         except OSError:
-            runner = None
-    if runner is None:
-        kind = "port"
-        runner = lambda n: O.rk4(variant, m, p, *c["state"], c["dt"], n)
+            pass
+    return (lambda n: O.rk4(variant, m, p, *c["state"], c["dt"], n)), ne, nm, "port"
+
+
+def _timed(runner, n=None, budget_s=15.0):
+    """Run the CPU sample (on a thread with a large stack: the reference's
+    nonlinear variants keep big local arrays) for n steps, or for about
+    budget_s seconds; returns (steps, seconds)."""
     res = {}
 
     def work():
+        k = n
+        if k is None:
+            t0 = time.perf_counter()
+            runner(1)
+            one = time.perf_counter() - t0
+            k = max(1, int(budget_s / max(one, 1e-3)))
         t0 = time.perf_counter()
-        runner(1)
-        one = time.perf_counter() - t0
-        n = max(1, int(budget_s / max(one, 1e-3)))
-        t0 = time.perf_counter()
-        runner(n)
-        res["n"], res["s"] = n, time.perf_counter() - t0
+        runner(k)
+        res["n"], res["s"] = k, time.perf_counter() - t0
 
     old = threading.stack_size(512 * 1024 * 1024)
     th = threading.Thread(target=work)
     th.start()
     th.join()
     threading.stack_size(old)
-    val = 4 * ne * nm * 3 * res["n"] / res["s"]
-    return {"value": val, "unit": "element-DOF stage updates/s", "cores": 1, "kind": kind,
-            "sample": f"{variant} p={nc} ng={ng} on {ne} triangles (the workload's generator "
-                      f"at a reduced node count), {res['n']} RK4 steps in {res['s']:.1f} s, "
-                      f"1 thread, {platform.processor() or platform.machine()}"}
+    return res["n"], res["s"]
+
+
+def cpu_baseline(variant, nc, ng, ngf, budget_s=15.0, wl="c2", workers=16):
+    """The reference Fortran (oracle/_ref) on a bounded sample of the same
+    workload: same p, rule and physics on a reduced mesh (at most 1600
+    triangles, what the reference's static arrays hold), one core; then the
+    same sample in `workers` concurrent processes (at most the host's
+    cores), the reference being serial, as an all-core aggregate.  Rules of
+    more than 33 points exceed the reference's arrays; then the C
+    restatement runs instead (kind "port")."""
+    runner, ne, nm, kind = _cpu_case(variant, nc, ng, ngf, wl)
+    n, sec = _timed(runner, budget_s=budget_s)
+    val = 4 * ne * nm * 3 * n / sec
+    out = {"value": val, "unit": "element-DOF stage updates/s", "cores": 1, "kind": kind,
+           "sample": f"{variant} p={nc} ng={ng} on {ne} triangles (the workload's generator "
+                     f"at a reduced node count), {n} RK4 steps in {sec:.1f} s, "
+                     f"1 thread, {platform.processor() or platform.machine()}"}
+    k = max(1, min(workers, os.cpu_count() or 1))
+    if k > 1:
+        import subprocess
+        cmd = [sys.executable, os.path.abspath(__file__), "--cpu-worker",
+               ",".join(map(str, (wl, variant, nc, ng, ngf, n)))]
+        procs = [subprocess.Popen(cmd, stdout=subprocess.PIPE, text=True) for _ in range(k)]
+        runs = []
+        for pr in procs:
+            o, _ = pr.communicate()
+            if pr.returncode == 0:
+                runs.append(json.loads(o.strip().splitlines()[-1]))
+        if len(runs) == k:
+            slowest = max(r["s"] for r in runs)
+            out["all_cores"] = {
+                "value": 4 * ne * nm * 3 * sum(r["n"] for r in runs) / slowest,
+                "processes": k,
+                "sample": f"{k} concurrent processes of the same sample, {n} steps each, "
+                          f"slowest {slowest:.1f} s"}
+    return out
+
+
+def cpu_worker(spec):
+    """--cpu-worker wl,variant,nc,ng,ngf,nsteps: one process of the all-core
+    CPU measurement (no GPU use); prints {"n": steps, "s": seconds}."""
+    wl, variant, nc, ng, ngf, n = spec.split(",")
+    runner, _, _, _ = _cpu_case(variant, int(nc), int(ng), int(ngf), wl)
+    k, sec = _timed(runner, n=int(n))
+    print(json.dumps({"n": k, "s": sec}), flush=True)
 
 
 def main():
+    if len(sys.argv) == 3 and sys.argv[1] == "--cpu-worker":
+        cpu_worker(sys.argv[2])
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS),
