@@ -195,9 +195,11 @@ __device__ __forceinline__ void stage_lds(floatx4 *dst, const floatx4 *__restric
   constexpr int T = 64 * W, B = 8;
   for (int i0 = threadIdx.x; i0 < n; i0 += B * T) {
     floatx4 v[B];
+    // unconditional loads (the index clamped to the last word), so that all
+    // B are in flight together (conditional ones were issued one after the
+    // other); the stores are masked
 #pragma unroll
-    for (int j = 0; j < B; j++)
-      if (i0 + j * T < n) v[j] = src[i0 + j * T];
+    for (int j = 0; j < B; j++) v[j] = src[min(i0 + j * T, n - 1)];
 #pragma unroll
     for (int j = 0; j < B; j++)
       if (i0 + j * T < n) dst[i0 + j * T] = v[j];
