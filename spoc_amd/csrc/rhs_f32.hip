@@ -480,24 +480,26 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
   const int l = lane(), g = l >> 4, col = l & 15, e0 = eb * 16, e = e0 + col;
   const bool valid = e < A.mesh.ne;
   const size_t mb = (size_t)e0 * NM;
-  const float mu0 = A.fld.mu0, mb0 = A.fld.mb0;
-  const float *gg = A.mesh.geo + (size_t)(valid ? e : 0) * GEO;
-  const float ax = gg[0], bx = gg[1], ay = gg[2], by = gg[3], di = valid ? gg[4] : 0.f;
-  const bool wall = A.mesh.bdl[valid ? e : 0] != 0 && valid;
-  const bool anyw = __ballot(wall) != 0;
-  const float nwx = wall ? gg[14] : 0.f, nwy = wall ? gg[15] : 0.f;
-
+  // loads in the order of their use by the gathers below: the neighbour
+  // lists (the gathers' addresses need them; the wait counter is in
+  // order), then the stage input, then the geometry
+  const Nbr3 N = nbr3(A.mesh, valid ? e : 0);
   float bu[KM], bv[KM], bh[KM];
   bload<NM>(stage_in(A, 0) + mb, g, col, valid, bu);
   bload<NM>(stage_in(A, 1) + mb, g, col, valid, bv);
   bload<NM>(stage_in(A, 2) + mb, g, col, valid, bh);
+  const float mu0 = A.fld.mu0, mb0 = A.fld.mb0;
+  const float *gg = A.mesh.geo + (size_t)(valid ? e : 0) * GEO;
+  const float ax = gg[0], bx = gg[1], ay = gg[2], by = gg[3], di = valid ? gg[4] : 0.f;
+  const bool wall = A.mesh.bdl[valid ? e : 0] != 0 && valid;
+  const float g14 = gg[14], g15 = gg[15];
+  const float nwx = wall ? g14 : 0.f, nwy = wall ? g15 : 0.f;
 
   // B: q = weak gradient of (u, v) (src/calvarnl_fs.f:129-158); the face
   // terms use the averaged velocity traces times n/2
   floatx4 Q[4][RBM];  // qxx, qxy, qyx, qyy
   {
     float fb[4][KE];
-    const Nbr3 N = nbr3(A.mesh, valid ? e : 0);
     const float *trA = (A.stage > 0 && (A.stage & 1)) ? A.st.trA[1] : A.st.trA[0];
 #pragma unroll
     for (int s = 0; s < KE; s++) {
@@ -531,6 +533,9 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
       }
     }
   }
+  // (the wave's vote on walls only here, at its first use: it waits for the
+  // wall flags, and with them for every load issued before them)
+  const bool anyw = __ballot(wall) != 0;
   if (VAR == 1 && anyw) {
     // free-slip wall projection of the tensor (src/calvarnl_fs.f:165-204):
     // only the t-n component goes through the reduced basis
