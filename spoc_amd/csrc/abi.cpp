@@ -206,22 +206,26 @@ struct spoc_ctx {
   std::vector<int> kcount;
   F32Plan *f32 = nullptr;
   std::vector<double> h_pi2;  // pi2(ng, nm) for projecting fields onto quadrature points
-  // one RK4 step captured as a graph, valid while its key matches
+  // `steps` RK4 steps captured as a graph, valid while its key matches
   hipGraphExec_t graph = nullptr;
   struct GraphKey {
     int variant, precision;
     double dt;
     long version;
+    int steps;
     bool operator==(const GraphKey &o) const {
       return variant == o.variant && precision == o.precision && dt == o.dt &&
-             version == o.version;
+             version == o.version && steps == o.steps;
     }
-  } graph_key{-1, -1, 0.0, -1};
+  } graph_key{-1, -1, 0.0, -1, 0};
   long version = 0;     // bumped by every mesh/field change
   int use_graph = 1;    // spoc_set_graph
 };
 
 namespace {
+
+// RK4 steps per captured graph (see spoc_rk4_steps)
+constexpr int kGraphSteps = 8;
 
 void drop_graph(spoc_ctx *c) {
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
@@ -832,13 +836,18 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
   const bool graph = c->use_graph && !c->prof.on && nsteps >= 2;
   if (graph) {
     // the step's launches are identical every step: replay them as a graph
-    spoc_ctx::GraphKey key{variant, precision, dt, c->version};
+    // of kGraphSteps steps (one when fewer are asked for), which costs less
+    // per launch than one graph per step on small meshes; the steps left
+    // over run eagerly
+    const int per = nsteps >= kGraphSteps ? kGraphSteps : 1;
+    spoc_ctx::GraphKey key{variant, precision, dt, c->version, per};
     if (!c->graph || !(c->graph_key == key)) {
       if (c->graph) (void)hipGraphExecDestroy(c->graph);
       c->graph = nullptr;
       hipGraph_t g = nullptr;
       HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      int r = step(nullptr);
+      int r = SPOC_OK;
+      for (int k = 0; k < per && !r; k++) r = step(nullptr);
       hipError_t e = hipStreamEndCapture(c->stream, &g);
       if (r) {
         if (g) (void)hipGraphDestroy(g);
@@ -853,7 +862,9 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
       }
       c->graph_key = key;
     }
-    for (int it = 0; it < nsteps; it++) HIPCHK(hipGraphLaunch(c->graph, c->stream));
+    for (int it = 0; it + per <= nsteps; it += per) HIPCHK(hipGraphLaunch(c->graph, c->stream));
+    for (int it = nsteps - nsteps % per; it < nsteps; it++)
+      if ((rc = step(nullptr))) return rc;
   } else {
     for (int it = 0; it < nsteps; it++)
       if ((rc = step(prof_hook))) return rc;

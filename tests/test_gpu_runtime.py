@@ -5,7 +5,8 @@ to exact equality with the plain runs they replace:
   bits as each member run alone (every kernel treats an element column
   independently), for both arithmetic paths; per-member energies likewise;
 - graph replay of the RK4 step gives the same bits as eager launches,
-  across replays separated by eager work and after a change of dt."""
+  across replays separated by eager work, after a change of dt, and with
+  several steps per graph."""
 import sys
 
 import numpy as np
@@ -116,6 +117,8 @@ def test_graph_replay_equals_eager(prec, variant):
         ctx.rk4(variant, 2.0, 1, precision=prec)   # eager step between replays
         ctx.rk4(variant, 2.0, 4, precision=prec)   # cached graph replayed again
         ctx.rk4(variant, 1.5, 2, precision=prec)   # new dt: captured again
+        ctx.rk4(variant, 1.5, 19, precision=prec)  # 8-step graph twice, 3 eager
+        ctx.rk4(variant, 1.5, 8, precision=prec)   # the 8-step graph again
         return ctx.download()
 
     eager, graph = run(False), run(True)
