@@ -76,15 +76,17 @@ def algorithmic_counts(nm, ng, ngf, variant):
     e3 = 3 * ngf
     f4, f8 = 4, 8
     mesh = 16 * f8 + 6 * f4 + f4                       # geo64, nbr/nbe, bdl
-    # RK4, averaged over the 4 stages: c0 read at stages 0-2, acc at 1-3;
-    # x64 written at stages 0-2, acc at 0-2, c0 at 3
-    rk = 3 * nm * f8 * (0.75 + 0.75) + 3 * nm * f8 * (0.75 + 1.0)
+    # RK4, averaged over the 4 stages: c0 read at every stage and the three
+    # stored stage inputs Y2, Y3, Y4 at the last; one of them written at
+    # stages 0-2, c0 at 3
+    rk = 3 * nm * f8 * (1.0 + 0.75) + 3 * nm * f8 * (0.75 + 0.25)
     vol_fl = 2 * 4 * nm * nm                              # core: -g grad eta, div(hh0 u)
     face_fl = 2 * 6 * nm * e3                             # core: lifts, traces
     trl_by = 2 * e3 * f8 + rk + 2 * e3 * f8               # trL in, RK, trL out
     if variant == "lin":
-        # (the stage input is c0 at stage 0, read for the RK4 update anyway)
-        return {"lin_f64": (0, vol_fl + face_fl, 0.75 * 3 * nm * f8 + trl_by + mesh)}
+        # (the stage input is c0 at stage 0 and Y4 at stage 3, read for the
+        # RK4 update anyway)
+        return {"lin_f64": (0, vol_fl + face_fl, 0.5 * 3 * nm * f8 + trl_by + mesh)}
     # B+C (FP32 only): q (Dx, Dy, LF), traces of F and of the mass flux (TR),
     # values at and projection from the quadrature points, the volume
     # divergence (6 products)
@@ -93,9 +95,10 @@ def algorithmic_counts(nm, ng, ngf, variant):
              + 3 * nm * f4 + 3 * e3 * f4)      # x, trA, geo, nbr/nbe, bdl, fields | gvol, trS
     # E+G: the FP64 core (volume and face terms; the flux traces' face
     # values are lifted by the same product) from the stage input, which is
-    # c0 at stage 0 (read for the RK4 update anyway: 3 of 4 stages count)
+    # c0 at stage 0 and Y4 at stage 3 (read for the RK4 update anyway: 2 of 4
+    # stages count)
     eg_fl = 0
-    eg_by = (0.75 * 3 * nm * f8 + 3 * nm * f4 + 3 * e3 * f4 + mesh + f8 + trl_by
+    eg_by = (0.5 * 3 * nm * f8 + 3 * nm * f4 + 3 * e3 * f4 + mesh + f8 + trl_by
              + 2 * e3 * f4)
     return {"bc_f32": (bc_fl, 0, bc_by), "eg_f32f64": (eg_fl, vol_fl + face_fl, eg_by)}
 

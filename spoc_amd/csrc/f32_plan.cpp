@@ -347,31 +347,30 @@ int f32_mesh_set(F32Plan *p, int ne, int nf, const int *itm, const int *its, con
   // state / work (element-blocked, see f32_types.h)
   const int ne3 = 3 * ngf;
   const size_t n64 = (size_t)nepad * nm, nt = (size_t)nepad * ne3;
-  // FP64: c0 acc x64 k (12), lam xbar aw avol (12), aF aUH aL (8), xb (3), hbbar (1),
+  // FP64: c0 (3), ys (9), k (3), lam xbar aw avol (12), aF aUH aL (8), xb (3), hbbar (1),
   // gvol (3); trL (4 nt), abar (4 nt), abar2 (2 nt), btr (2 nt).  FP32: trA x2, trS (7 nt)
-  const size_t b64 = (39 * n64 + 12 * nt) * 8, b32 = (7 * nt) * 4;
+  const size_t b64 = (42 * n64 + 12 * nt) * 8, b32 = (7 * nt) * 4;
   if (hipMalloc(&p->dstate, b64 + b32 + 256) != hipSuccess) return -2;
   if (hipMemset(p->dstate, 0, b64 + b32 + 256) != hipSuccess) return -2;
   double *s64 = (double *)p->dstate;
-  float *s32 = (float *)(s64 + 39 * n64 + 12 * nt);
+  float *s32 = (float *)(s64 + 42 * n64 + 12 * nt);
   F32State &st = p->st;
   for (int f = 0; f < 3; f++) {
     st.c0[f] = s64 + f * n64;
-    st.acc[f] = s64 + (3 + f) * n64;
-    st.x64[f] = s64 + (6 + f) * n64;
-    st.k[f] = s64 + (9 + f) * n64;
+    for (int s = 0; s < 3; s++) st.ys[s][f] = s64 + (3 + 3 * s + f) * n64;
+    st.k[f] = s64 + (12 + f) * n64;
   }
-  st.trL[0] = s64 + 12 * n64;
+  st.trL[0] = s64 + 15 * n64;
   st.trL[1] = st.trL[0] + 2 * nt;
   for (int f = 0; f < 3; f++) {
-    double *a = s64 + 12 * n64 + 4 * nt;
+    double *a = s64 + 15 * n64 + 4 * nt;
     st.lam[f] = a + f * n64;
     st.xbar[f] = a + (3 + f) * n64;
     st.aw[f] = a + (6 + f) * n64;
     st.avol[f] = a + (9 + f) * n64;
   }
   {
-    double *a = s64 + 24 * n64 + 4 * nt;
+    double *a = s64 + 27 * n64 + 4 * nt;
     for (int f = 0; f < 4; f++) st.aF[f] = a + f * n64;
     for (int f = 0; f < 2; f++) {
       st.aUH[f] = a + (4 + f) * n64;
@@ -383,7 +382,7 @@ int f32_mesh_set(F32Plan *p, int ne, int nf, const int *itm, const int *its, con
     st.abar2 = st.abar + 4 * nt;
     st.btr = st.abar2 + 2 * nt;
   }
-  for (int f = 0; f < 3; f++) st.gvol[f] = (float *)(s64 + 36 * n64 + 12 * nt) + f * n64;
+  for (int f = 0; f < 3; f++) st.gvol[f] = (float *)(s64 + 39 * n64 + 12 * nt) + f * n64;
   st.trA[0] = s32;
   st.trA[1] = s32 + 2 * nt;
   st.trS = s32 + 4 * nt;
@@ -705,7 +704,7 @@ int f32_adj_run(F32Plan *p, hipStream_t s, int variant, double dt, int nsteps, i
           if (!rc) rc = call(p, fa, 2, nullptr, nullptr);
           for (int st = 0; st < 4 && !rc; st++) {
             // (the first stage's input is the step's start state itself)
-            rc = copy(base0 + (size_t)st * 3 * n64, st == 0 ? p->st.c0[0] : p->st.x64[0]);
+            rc = copy(base0 + (size_t)st * 3 * n64, st == 0 ? p->st.c0[0] : p->st.ys[st - 1][0]);
             if (st < 3 && !rc) {
               fa.stage = st;
               rc = rhs_kernels(p, fa, nullptr, nullptr);

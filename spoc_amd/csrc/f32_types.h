@@ -140,8 +140,8 @@ struct F32Fields {
 };
 
 struct F32State {
-  double *c0[3], *acc[3];  // [NM rows] psi coordinates: step start, RK4 sum
-  double *x64[3];          // [NM rows] stage input
+  double *c0[3];           // [NM rows] psi coordinates: step start
+  double *ys[3][3];        // [NM rows] stage inputs of RK4 stages 1, 2, 3 (Y2, Y3, Y4)
   double *k[3];            // [NM rows] psi tendencies (stage < 0)
   float *gvol[3];          // [NM rows] FP32 volume and source part of the tendency
                            // (B+C -> E+G; the FP64 core is E+G's own)

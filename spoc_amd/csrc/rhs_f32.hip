@@ -54,10 +54,10 @@ constexpr int bc_waves(int nc) { return nc >= 7 ? 2 : BC_WAVES; }
 constexpr int eg_waves(int nc) { return nc >= 7 ? 2 : eg_split(nc) ? 4 : EG_WAVES; }
 
 // Stage input of field f: the step's start state c0 at the first stage (and
-// for a bare right-hand side), else what the previous stage stored in x64.
-// (A select on the launch-uniform stage, not a branch.)
+// for a bare right-hand side), else what the previous stage stored in
+// ys[stage - 1].  (A select on the launch-uniform stage, not a branch.)
 __device__ __forceinline__ const double *stage_in(const F32Args &A, int f) {
-  return A.stage <= 0 ? A.st.c0[f] : A.st.x64[f];
+  return A.stage <= 0 ? A.st.c0[f] : A.st.ys[A.stage - 1][f];
 }
 
 // Linear gravity-wave core (VAR 0: the whole linear right-hand side,
@@ -300,18 +300,23 @@ __device__ __forceinline__ void traces_uv(const F32Args &A, const double *O, int
 }
 
 // RK4 stage update (src/shwwnl.f:242-284) of field f, FP64: x holds the
-// stage's tendency (B fragments) and leaves as the next stage input.  The
-// running sum acc includes the step's start state, acc = c0 + dt/6 k1 +
-// dt/3 k2 + dt/3 k3, so the last stage reads acc alone and writes the step's
-// result to c0, which is also the next step's first stage input (stage_in):
-// per step 6 field reads and 7 field writes instead of 7 and 8 for a sum of
-// bare tendencies with a separate copy of the stage input.  The order of the
-// additions differs from the reference's (c0 + dt/6 (((k1 + 2 k2) + 2 k3) +
-// k4)) by FP64 rounding only.  stage < 0: the tendency goes to k and x
-// leaves as 0.  Elements past the mesh (the padding of the last block) take
-// a zero tendency, so their stored values stay 0; rows past NM belong to the
-// next block and are not stored.  The stage is uniform over the launch, so
-// its branches are scalar and every load of a field issues at once.
+// stage's tendency (B fragments) and leaves as the next stage input.  Stages
+// 0-2 store their stage results Y2 = c0 + dt/2 k1, Y3 = c0 + dt/2 k2 and
+// Y4 = c0 + dt k3 in ys[0..2] and keep no running sum: the last stage forms
+// c0 + dt/6 k1 + dt/3 k2 + dt/3 k3 as c0 + ((Y2 - c0) + 2 (Y3 - c0) +
+// (Y4 - c0)) / 3 and writes the step's result to c0, which is also the next
+// step's first stage input (stage_in).  Per step that is 7 field reads and 4
+// writes (c0 at every stage, ys[0..2] at the last; ys[s] at stage s, c0 at
+// the last), of which two reads are the stage inputs of stages 0 and 3,
+// against 6 and 7 plus 2 stage inputs for a running sum.  Each difference
+// Y - c0 is exact up to the rounding of Y itself, so the step's result
+// differs from the reference's order of additions (c0 + dt/6 (((k1 + 2 k2)
+// + 2 k3) + k4)) by a few FP64 roundings of c0.  stage < 0: the tendency
+// goes to k and x leaves as 0.  Elements past the mesh (the padding of the
+// last block) take a zero tendency, so their stored values stay 0; rows past
+// NM belong to the next block and are not stored.  The stage is uniform over
+// the launch, so its branches are scalar and every load of a field issues
+// at once.
 template <int NM, int KM>
 __device__ __forceinline__ void update(const F32Args &A, int f, int eb, double (&x)[KM]) {
   const int g = lane() >> 4, col = lane() & 15;
@@ -337,31 +342,29 @@ __device__ __forceinline__ void update(const F32Args &A, int f, int eb, double (
     for (int s = 0; s < KM; s++) x[s] = 0.0;
     return;
   }
-  // weight of this stage's tendency in the step's sum
-  const double w = (stage == 0 || stage == 3) ? dt / 6.0 : dt / 3.0;
-  double y[KM];
+  double c[KM], y[KM];
+#pragma unroll
+  for (int s = 0; s < KM; s++) c[s] = A.st.c0[f][mb + ix[s]];
   if (stage == 3) {
+    double p[KM], q[KM], r[KM];
 #pragma unroll
-    for (int s = 0; s < KM; s++) y[s] = A.st.acc[f][mb + ix[s]];
+    for (int s = 0; s < KM; s++) p[s] = A.st.ys[0][f][mb + ix[s]];
 #pragma unroll
-    for (int s = 0; s < KM; s++) y[s] = y[s] + w * x[s];
-    store(A.st.c0[f], y);
-  } else {
-    double c[KM], a[KM];
+    for (int s = 0; s < KM; s++) q[s] = A.st.ys[1][f][mb + ix[s]];
 #pragma unroll
-    for (int s = 0; s < KM; s++) c[s] = A.st.c0[f][mb + ix[s]];
-    if (stage > 0) {
-#pragma unroll
-      for (int s = 0; s < KM; s++) a[s] = A.st.acc[f][mb + ix[s]];
-    }
-    const double h = stage == 2 ? dt : 0.5 * dt;
+    for (int s = 0; s < KM; s++) r[s] = A.st.ys[2][f][mb + ix[s]];
+    const double third = 1.0 / 3.0, w = dt / 6.0;
 #pragma unroll
     for (int s = 0; s < KM; s++) {
-      a[s] = (stage == 0 ? c[s] : a[s]) + w * x[s];
-      y[s] = c[s] + h * x[s];
+      const double d = ((p[s] - c[s]) + 2.0 * (q[s] - c[s])) + (r[s] - c[s]);
+      y[s] = c[s] + (third * d + w * x[s]);
     }
-    store(A.st.x64[f], y);
-    store(A.st.acc[f], a);
+    store(A.st.c0[f], y);
+  } else {
+    const double h = stage == 2 ? dt : 0.5 * dt;
+#pragma unroll
+    for (int s = 0; s < KM; s++) y[s] = c[s] + h * x[s];
+    store(A.st.ys[stage][f], y);
   }
 #pragma unroll
   for (int s = 0; s < KM; s++) x[s] = y[s];
