@@ -307,8 +307,10 @@ __device__ __forceinline__ void traces_uv(const F32Args &A, const double *O, int
 // (Y4 - c0)) / 3 and writes the step's result to c0, which is also the next
 // step's first stage input (stage_in).  Per step that is 7 field reads and 4
 // writes (c0 at every stage, ys[0..2] at the last; ys[s] at stage s, c0 at
-// the last), of which two reads are the stage inputs of stages 0 and 3,
-// against 6 and 7 plus 2 stage inputs for a running sum.  Each difference
+// the last), of which two reads are the stage inputs of stages 0 and 3, so
+// only those of stages 1 and 2 come on top: 13 field passes, against 16 for
+// a running sum (6 reads and 7 writes, and the stage inputs of stages 1-3
+// on top).  Each difference
 // Y - c0 is exact up to the rounding of Y itself, so the step's result
 // differs from the reference's order of additions (c0 + dt/6 (((k1 + 2 k2)
 // + 2 k3) + k4)) by a few FP64 roundings of c0.  stage < 0: the tendency
