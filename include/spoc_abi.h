@@ -13,7 +13,7 @@
  * the makefile links (src/mlin:12, src/mfs:11, src/mns:11, src/minv:11) and
  * reading the operator tables from COMMON /const/ /gauss/ /gaussf/ /gauss2/
  * /gauss3/ /matrices/ (src/shwwnl.f:61-71).  The drop-in Fortran cal_var
- * (spoc_amd/fortran/spoc_calvar.f90) forwards to the functions below.
+ * (spoc_amd/fortran/spoc_calvar.F90) forwards to the functions below.
  *
  * Conventions
  *  - Plain pointers and sizes, no library types.  Host pointers unless a
@@ -64,7 +64,10 @@ enum {
   SPOC_EMESH = -5,      /* mesh rejected: a triangle with two wall faces or a
                           wall face that is not its edge 1 (src/cmal_s.f:222-277),
                           or inconsistent connectivity */
-  SPOC_EFILE = -6       /* file missing, unreadable or not in the expected format */
+  SPOC_EFILE = -6,      /* file missing, unreadable or not in the expected format */
+  SPOC_ENAN = -7        /* the state is no longer finite (the driver's testnan,
+                           src/shwwnl.f:426-445, prints and STOPs); the results
+                           are still written and can be downloaded */
 };
 
 typedef struct spoc_ctx spoc_ctx;
@@ -157,7 +160,11 @@ int spoc_state_download(spoc_ctx *ctx, int ld, double *u0, double *v0, double *h
  * nsteps classic RK4 steps of the driver loop (src/shwwnl.f:233-284, same in
  * src/shwwnl_adap.f:302-353) on the device-resident state.  SPOC_FP64
  * reproduces the reference's arithmetic order, including
- * u0 + (dt/6)*(((k1 + 2k2) + 2k3) + k4).
+ * u0 + (dt/6)*(((k1 + 2k2) + 2k3) + k4).  Returns SPOC_ENAN if the state
+ * holds a NaN or an infinity after the last step (the driver's testnan,
+ * src/shwwnl.f:297, checks the kinetic energy every per2 steps; this checks
+ * every coefficient once per call).  At most 512 kernel dispatches are
+ * queued ahead of the host: the call waits for earlier steps as it goes.
  */
 int spoc_rk4_steps(spoc_ctx *ctx, int variant, int precision, double dt, int nsteps);
 
@@ -287,30 +294,33 @@ int spoc_mesh_file_write(const char *path, int nn, int ne, const double *x, cons
  * the mass matrix's upper Cholesky factor.
  *
  * spoc_interpol (interpol + interlin, src/adapt.f:1662-1824): nfield modal
- * fields u[f] (ld, ne) of the old mesh (nodes xgr, ygr; in, itm (3, ne);
- * tseg, pseg (2, nf)) onto the new mesh (xgrnew, ygrnew; innew (3, nenew)):
+ * fields u[f] (ld, ne) of the old mesh (nodes xgr, ygr (nn); in, itm (3, ne);
+ * tseg, pseg (2, nf)) onto the new mesh (xgrnew, ygrnew (nnnew); innew
+ * (3, nenew)):
  * element i with tinter[i] = t > 0 copies old element t, the others are the
  * L2 projection of the old field sampled at their quadrature points, each
  * located by the reference's walk (from the previous point's element; a
  * point outside the mesh is moved onto the wall edge it left by).  The walk
  * does not depend on the field, so one call carries u, v, h together.
- * SPOC_EMESH if a walk does not end.
+ * SPOC_EMESH if a walk does not end; SPOC_EINVAL, before any walk, if an
+ * index array points outside its range.
  *
- * spoc_calh2 (src/adapt.f:303-373): L2 projection onto the mesh of the
- * function ff(x, y, user) (the driver's wind and bathymetry), or h0's
- * element tinter[i] where tinter[i] > 0.
+ * spoc_calh2 (src/adapt.f:303-373): L2 projection onto the mesh (nodes
+ * xgr, ygr (nn); in (3, ne)) of the function ff(x, y, user) (the driver's
+ * wind and bathymetry), or h0's element tinter[i] where tinter[i] > 0 (h0
+ * is (ld, ne0); may be NULL when every tinter[i] is 0).
  */
 int spoc_interpol(int nc, int ng, const double *wi, const double *xi, const double *yi,
                   const double *pi2, int ld_g, const double *amm, int ld_m, int nn, int ne,
-                  const double *xgr, const double *ygr, const int *in, const int *itm,
-                  const int *tseg, const int *pseg, int nenew, const double *xgrnew,
+                  int nf, const double *xgr, const double *ygr, const int *in, const int *itm,
+                  const int *tseg, const int *pseg, int nnnew, int nenew, const double *xgrnew,
                   const double *ygrnew, const int *innew, const int *tinter, int nfield, int ld,
                   const double *const *u, double *const *unew);
 int spoc_calh2(int nc, int ng, const double *wi, const double *xi, const double *yi,
                const double *pi2, int ld_g, const double *amm, int ld_m,
-               double (*ff)(double x, double y, void *user), void *user, int ne,
-               const double *xgr, const double *ygr, const int *in, const int *tinter, int ld,
-               const double *h0, double *h1);
+               double (*ff)(double x, double y, void *user), void *user, int nn, int ne,
+               const double *xgr, const double *ygr, const int *in, const int *tinter, int ne0,
+               int ld, const double *h0, double *h1);
 
 /*
  * Refinement indicator of the device-resident state (checkconv,
@@ -339,7 +349,9 @@ int spoc_cal_conv(spoc_ctx *ctx, double fac1, double fac2, double fac3, int *fco
 /*
  * Global integrals of the device-resident state (src/energy.f:3-96), FP64:
  * kinetic 1/2 int (h+H)|u|^2, potential 1/2 g int h^2, mass int h.  Needs the
- * node arrays passed to spoc_mesh_set.
+ * node arrays passed to spoc_mesh_set.  The three values are written, and
+ * SPOC_ENAN returned, when the kinetic energy is NaN (the test of testnan,
+ * src/shwwnl.f:426-445).
  */
 int spoc_energy(spoc_ctx *ctx, double *ke, double *pe, double *mass);
 

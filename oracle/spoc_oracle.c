@@ -3,9 +3,10 @@
  * (the oracle).  Only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg use it; the product never does.
  *
- * Pinned by: (1) bit-exact comparison against the reference Fortran compiled
- * from its own sources (oracle/_ref, tests/test_oracle_ref.py) and against
- * golden vectors written from it (tests/golden/); (2) closed forms and
+ * Pinned by: (1) bit-exact comparison against golden vectors written by the
+ * reference Fortran compiled from its own sources (oracle/_ref;
+ * tests/golden/make_goldens.py writes them, tests/test_oracle_golden.py
+ * compares); (2) closed forms and
  * invariants (Legendre values, quadrature exactness, Cholesky vs numpy,
  * reduced basis vanishing on edge 1, rot_vec involution, mass conservation),
  * tests/test_oracle_math.py.

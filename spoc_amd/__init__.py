@@ -166,9 +166,9 @@ def interpol(tables: dict, old: dict, new_x, new_y, new_tri, tinter, fields):
     x, y = np.ascontiguousarray(old["x"], np.float64), np.ascontiguousarray(old["y"], np.float64)
     _check(lib.spoc_interpol(t["nc"], t["ng"], _d(t["wi"]), _d(t["xi"]), _d(t["yi"]),
                              _d(t["pi2"]), t["ldg"], _d(t["amm"]), LDM, len(x), tri.shape[1],
-                             _d(x), _d(y), _i(tri), _i(itm), _i(tseg), _i(pseg), nenew,
-                             _d(new_x), _d(new_y), _i(new_tri), _i(tinter), len(fields), ld, u,
-                             un))
+                             tseg.shape[1], _d(x), _d(y), _i(tri), _i(itm), _i(tseg), _i(pseg),
+                             len(new_x), nenew, _d(new_x), _d(new_y), _i(new_tri), _i(tinter),
+                             len(fields), ld, u, un))
     return out
 
 
@@ -187,10 +187,12 @@ def calh2(tables: dict, ff, x, y, tri, tinter=None, h0=None, ld: int = LDM):
     tinter = np.zeros(ne, np.int32) if tinter is None else np.ascontiguousarray(tinter, np.int32)
     h1 = np.zeros((ld, ne), order="F")
     cb = _FF(lambda a, b, _u: float(ff(a, b)))
-    h0p = _d(np.asfortranarray(h0, np.float64)) if h0 is not None else None
+    h0a = np.asfortranarray(h0, np.float64) if h0 is not None else None
+    h0p = _d(h0a) if h0a is not None else None
+    ne0 = h0a.shape[1] if h0a is not None else 0
     _check(lib.spoc_calh2(t["nc"], t["ng"], _d(t["wi"]), _d(t["xi"]), _d(t["yi"]), _d(t["pi2"]),
-                          t["ldg"], _d(t["amm"]), LDM, cb, None, ne, _d(x), _d(y), _i(tri),
-                          _i(tinter), ld, h0p, _d(h1)))
+                          t["ldg"], _d(t["amm"]), LDM, cb, None, len(x), ne, _d(x), _d(y),
+                          _i(tri), _i(tinter), ne0, ld, h0p, _d(h1)))
     return h1
 
 

@@ -26,6 +26,7 @@ int adapt_launch(hipStream_t st, int nc, int ngf, int ne, int nmem, int nch,
                  double *work, double *T, double *amax, bool decide, double fac1, double fac2,
                  int *flags);
 int adapt_chunk();
+int nonfinite_launch(hipStream_t st, size_t n, const double *a, int *flag);
 int energy_launch(hipStream_t st, int ne, int nm, int ng, const double *u, const double *v,
                   const double *h, const double *hb, const double *pi2, const double *wi,
                   const double *ammo1, const double *deta, double g0, int nr, const int *off,
@@ -220,6 +221,8 @@ struct spoc_ctx {
   } graph_key{-1, -1, 0.0, -1, 0};
   long version = 0;     // bumped by every mesh/field change
   int use_graph = 1;    // spoc_set_graph
+  QueueBound queue;     // dispatches queued ahead of the host in spoc_rk4_steps
+  int *dflag = nullptr; // device word of the non-finite test
 };
 
 namespace {
@@ -227,10 +230,26 @@ namespace {
 // RK4 steps per captured graph (see spoc_rk4_steps)
 constexpr int kGraphSteps = 8;
 
-void drop_graph(spoc_ctx *c) {
-  if (c->graph) (void)hipGraphExecDestroy(c->graph);
+// Destroys the captured graph once every launch of it has finished.
+void destroy_graph(spoc_ctx *c) {
+  if (!c->graph) return;
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipGraphExecDestroy(c->graph);
   c->graph = nullptr;
+}
+
+void drop_graph(spoc_ctx *c) {
+  destroy_graph(c);
   c->version++;
+}
+
+// RK4 kernel dispatches of one step: per right-hand side, the FP64 path runs
+// a face and an element pass (two of each for the nonlinear variants), the
+// mixed path one element kernel (two for the nonlinear variants).
+int step_dispatches(int variant, int precision) {
+  const int per_rhs = precision == SPOC_FP64 ? (variant == SPOC_LIN ? 2 : 4)
+                                             : (variant == SPOC_LIN ? 1 : 2);
+  return 4 * per_rhs;
 }
 
 void free_mesh(spoc_ctx *c) {
@@ -455,6 +474,10 @@ int spoc_ctx_create(spoc_ctx **out, int device, int nc, int ng, int ngf, const d
                      dptr[7], dptr[8], dptr[9], dptr[10], dptr[11], dptr[12]};
   (void)hipEventCreate(&c->t0);
   (void)hipEventCreate(&c->t1);
+  if (hipMalloc((void **)&c->dflag, 256) != hipSuccess) {
+    spoc_ctx_destroy(c);
+    return fail(SPOC_EHIP, "device allocation failed");
+  }
   c->h_pi2.assign((size_t)ng * nm, 0.0);
   for (int m = 0; m < nm; m++)
     for (int q = 0; q < ng; q++) c->h_pi2[q + (size_t)ng * m] = pi2[q + (size_t)ld_g * m];
@@ -474,6 +497,7 @@ int spoc_ctx_destroy(spoc_ctx *c) {
   free_mesh(c);
   f32_plan_destroy(c->f32);
   if (c->dtab) (void)hipFree(c->dtab);
+  if (c->dflag) (void)hipFree(c->dflag);
   for (auto e : c->prof.ev) (void)hipEventDestroy(e);
   if (c->t0) (void)hipEventDestroy(c->t0);
   if (c->t1) (void)hipEventDestroy(c->t1);
@@ -830,9 +854,17 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
     }
     return SPOC_OK;
   };
+  // an error leaves nothing queued behind it
+  auto bail = [&](int r) {
+    (void)hipStreamSynchronize(c->stream);
+    c->queue.reset();
+    return r;
+  };
+  c->queue.reset();
+  const int per_step = step_dispatches(variant, precision);
   HIPCHK(hipEventRecord(c->t0, c->stream));
   if (precision == SPOC_FP32_MFMA && (rc = f32_rk4_begin(c->f32, c->stream, variant, uw)))
-    return f32_fail(rc);
+    return bail(f32_fail(rc));
   const bool graph = c->use_graph && !c->prof.on && nsteps >= 2;
   if (graph) {
     // the step's launches are identical every step: replay them as a graph
@@ -842,37 +874,60 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
     const int per = nsteps >= kGraphSteps ? kGraphSteps : 1;
     spoc_ctx::GraphKey key{variant, precision, dt, c->version, per};
     if (!c->graph || !(c->graph_key == key)) {
-      if (c->graph) (void)hipGraphExecDestroy(c->graph);
-      c->graph = nullptr;
+      destroy_graph(c);
       hipGraph_t g = nullptr;
-      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+      if (e != hipSuccess)
+        return bail(fail(SPOC_EHIP, "graph capture: %s", hipGetErrorString(e)));
       int r = SPOC_OK;
       for (int k = 0; k < per && !r; k++) r = step(nullptr);
-      hipError_t e = hipStreamEndCapture(c->stream, &g);
+      e = hipStreamEndCapture(c->stream, &g);
       if (r) {
         if (g) (void)hipGraphDestroy(g);
-        return r;
+        return bail(r);
       }
-      if (e != hipSuccess) return fail(SPOC_EHIP, "graph capture: %s", hipGetErrorString(e));
+      if (e != hipSuccess) return bail(fail(SPOC_EHIP, "graph capture: %s", hipGetErrorString(e)));
       e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       if (e != hipSuccess) {
         c->graph = nullptr;
-        return fail(SPOC_EHIP, "graph instantiate: %s", hipGetErrorString(e));
+        return bail(fail(SPOC_EHIP, "graph instantiate: %s", hipGetErrorString(e)));
       }
       c->graph_key = key;
     }
-    for (int it = 0; it + per <= nsteps; it += per) HIPCHK(hipGraphLaunch(c->graph, c->stream));
-    for (int it = nsteps - nsteps % per; it < nsteps; it++)
-      if ((rc = step(nullptr))) return rc;
+    for (int it = 0; it + per <= nsteps; it += per) {
+      hipError_t e = c->queue.admit(per * per_step);
+      if (e == hipSuccess) e = hipGraphLaunch(c->graph, c->stream);
+      if (e == hipSuccess) e = c->queue.mark(c->stream);
+      if (e != hipSuccess) return bail(fail(SPOC_EHIP, "graph launch: %s", hipGetErrorString(e)));
+    }
+    for (int it = nsteps - nsteps % per; it < nsteps; it++) {
+      hipError_t e = c->queue.admit(per_step);
+      if (e != hipSuccess) return bail(fail(SPOC_EHIP, "queue wait: %s", hipGetErrorString(e)));
+      if ((rc = step(nullptr))) return bail(rc);
+      if ((e = c->queue.mark(c->stream)) != hipSuccess)
+        return bail(fail(SPOC_EHIP, "queue mark: %s", hipGetErrorString(e)));
+    }
   } else {
-    for (int it = 0; it < nsteps; it++)
-      if ((rc = step(prof_hook))) return rc;
+    for (int it = 0; it < nsteps; it++) {
+      hipError_t e = c->queue.admit(per_step);
+      if (e != hipSuccess) return bail(fail(SPOC_EHIP, "queue wait: %s", hipGetErrorString(e)));
+      if ((rc = step(prof_hook))) return bail(rc);
+      if ((e = c->queue.mark(c->stream)) != hipSuccess)
+        return bail(fail(SPOC_EHIP, "queue mark: %s", hipGetErrorString(e)));
+    }
   }
   if (precision == SPOC_FP32_MFMA && (rc = f32_rk4_end(c->f32, c->stream, variant, uw)))
-    return f32_fail(rc);
+    return bail(f32_fail(rc));
   HIPCHK(hipEventRecord(c->t1, c->stream));
+  // the driver's testnan: is every coefficient of the new state finite?
+  if (nonfinite_launch(c->stream, (size_t)3 * c->nm * c->ne, state_ptr(c, 0), c->dflag))
+    return bail(fail(SPOC_EHIP, "non-finite test launch failed"));
+  int bad = 0;
+  HIPCHK(hipMemcpyAsync(&bad, c->dflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipEventSynchronize(c->t1));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->queue.reset();
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, c->t0, c->t1));
   c->last_ms = ms;
@@ -913,6 +968,9 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
       }
     }
   }
+  if (bad)
+    return fail(SPOC_ENAN, "the state holds a NaN or an infinity after %d RK4 steps (dt=%g)",
+                nsteps, dt);
   return SPOC_OK;
 }
 
@@ -1097,6 +1155,7 @@ int spoc_energy(spoc_ctx *c, double *ke, double *pe, double *mass) {
   *ke = r[0];
   *pe = r[1];
   *mass = r[2];
+  if (std::isnan(r[0])) return fail(SPOC_ENAN, "spoc_energy: the kinetic energy is NaN");
   return SPOC_OK;
 }
 
@@ -1186,11 +1245,16 @@ int spoc_ensemble_energy(spoc_ctx *c, int nmem, double *ke, double *pe, double *
   std::vector<double> r(3 * (size_t)nmem);
   int rc = energies(c, nmem, c->dmem + 2, r.data());
   if (rc) return rc;
+  int nan_member = -1;
   for (int m = 0; m < nmem; m++) {
     ke[m] = r[3 * m];
     pe[m] = r[3 * m + 1];
     mass[m] = r[3 * m + 2];
+    if (nan_member < 0 && std::isnan(ke[m])) nan_member = m;
   }
+  if (nan_member >= 0)
+    return fail(SPOC_ENAN, "spoc_ensemble_energy: the kinetic energy of member %d is NaN",
+                nan_member);
   return SPOC_OK;
 }
 

@@ -71,4 +71,22 @@ int energy_launch(hipStream_t st, int ne, int nm, int ng, const double *u, const
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
+// flag[0] |= 1 if any of the n doubles of a is not finite (NaN or Inf).
+__global__ void nonfinite_kernel(size_t n, const double *__restrict__ a, int *__restrict__ flag) {
+  bool bad = false;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    bad = bad || !isfinite(a[i]);
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
+// Sets *flag (device) to 1 if a[0..n) holds a non-finite value, else 0.
+int nonfinite_launch(hipStream_t st, size_t n, const double *a, int *flag) {
+  if (hipMemsetAsync(flag, 0, sizeof(int), st) != hipSuccess) return -2;
+  const size_t want = (n + 255) / 256;
+  const unsigned grid = (unsigned)(want < 2048 ? (want ? want : 1) : 2048);
+  nonfinite_kernel<<<grid, 256, 0, st>>>(n, a, flag);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 }  // namespace spoc

@@ -658,6 +658,13 @@ int f32_adj_run(F32Plan *p, hipStream_t s, int variant, double dt, int nsteps, i
     return hipMemcpyAsync(dst, src, sbytes, hipMemcpyDeviceToDevice, s) == hipSuccess ? 0 : -2;
   };
   double *ckpt0 = state(0), *seg0 = state(nseg), *base0 = state(nseg + ncheck);
+  // dispatches queued ahead of the host, bounded per step: a forward step
+  // queues at most 8 kernels and a copy, a backward step the 4 stage inputs'
+  // recomputation (a copy, the trace pass, 3 right-hand sides, 4 copies) and
+  // 4 adjoint stages of at most 4 passes
+  QueueBound qb;
+  auto fence = [&](int n) { return qb.admit(n) == hipSuccess ? 0 : -2; };
+  auto mark = [&]() { return qb.mark(s) == hipSuccess ? 0 : -2; };
   do {
     if (variant != 0) {
       // forward sweep: checkpoints at the start of every segment
@@ -667,11 +674,13 @@ int f32_adj_run(F32Plan *p, hipStream_t s, int variant, double dt, int nsteps, i
       F32Args fa = args(p, s, variant, -1, dt);
       if ((rc = call(p, fa, 2, nullptr, nullptr))) break;
       for (int n = 0; n < nsteps && !rc; n++) {
-        if (n % ncheck == 0) rc = copy(ckpt0 + (size_t)(n / ncheck) * 3 * n64, p->st.c0[0]);
+        rc = fence(9);
+        if (!rc && n % ncheck == 0) rc = copy(ckpt0 + (size_t)(n / ncheck) * 3 * n64, p->st.c0[0]);
         for (int st = 0; st < 4 && !rc; st++) {
           fa.stage = st;
           rc = rhs_kernels(p, fa, nullptr, nullptr);
         }
+        if (!rc) rc = mark();
       }
       if (rc) break;
       if (hipMemsetAsync(p->st.hbbar, 0, n64 * sizeof(double), s) != hipSuccess) {
@@ -690,14 +699,17 @@ int f32_adj_run(F32Plan *p, hipStream_t s, int variant, double dt, int nsteps, i
         rc = copy(p->st.c0[0], ckpt0 + (size_t)sg * 3 * n64);
         if (!rc) rc = call(p, fa, 2, nullptr, nullptr);
         for (int n = n0; n < n1 && !rc; n++) {
-          rc = copy(seg0 + (size_t)(n - n0) * 3 * n64, p->st.c0[0]);
+          rc = fence(9);
+          if (!rc) rc = copy(seg0 + (size_t)(n - n0) * 3 * n64, p->st.c0[0]);
           for (int st = 0; st < 4 && !rc && n + 1 < n1; st++) {
             fa.stage = st;
             rc = rhs_kernels(p, fa, nullptr, nullptr);
           }
+          if (!rc) rc = mark();
         }
       }
       for (int n = n1 - 1; n >= n0 && !rc; n--) {
+        if ((rc = fence(32))) break;
         if (variant != 0) {
           // the four stage inputs of step n
           rc = copy(p->st.c0[0], seg0 + (size_t)(n - n0) * 3 * n64);
@@ -717,11 +729,13 @@ int f32_adj_run(F32Plan *p, hipStream_t s, int variant, double dt, int nsteps, i
             for (int f = 0; f < 3; f++) a.st.xb[f] = base0 + (size_t)(3 - as) * 3 * n64 + f * n64;
           rc = adj_rhs_passes(p, a);
         }
+        if (!rc) rc = mark();
       }
     }
     for (int f = 0; f < 3 && !rc; f++) rc = call_adj(p, a, 1, p->st.lam[f], lam[f]);
     if (!rc && hb && variant != 0) rc = call_adj(p, a, 1, p->st.hbbar, hb);
   } while (false);
+  if (rc) (void)hipStreamSynchronize(s);  // nothing left queued behind an error
   if (mem) {
     (void)hipStreamSynchronize(s);
     (void)hipFree(mem);

@@ -77,21 +77,45 @@ double fsol(const Rule &R, double x, double y, const double *a) {
   return s;
 }
 
+// Every entry of the n 1-based indices a lies in [lo, hi]; else the
+// message names the array and the first entry outside.
+bool in_range(const int *a, size_t n, int lo, int hi, const char *fn, const char *name) {
+  for (size_t i = 0; i < n; i++)
+    if (a[i] < lo || a[i] > hi) {
+      spoc::report(SPOC_EINVAL, "%s: %s[%zu]=%d outside [%d, %d]", fn, name, i, a[i], lo, hi);
+      return false;
+    }
+  return true;
+}
+
 }  // namespace
 
 extern "C" {
 
 int spoc_interpol(int nc, int ng, const double *wi, const double *xi, const double *yi,
                   const double *pi2, int ld_g, const double *amm, int ld_m, int nn, int ne,
-                  const double *xgr, const double *ygr, const int *in, const int *itm,
-                  const int *tseg, const int *pseg, int nenew, const double *xgrnew,
+                  int nf, const double *xgr, const double *ygr, const int *in, const int *itm,
+                  const int *tseg, const int *pseg, int nnnew, int nenew, const double *xgrnew,
                   const double *ygrnew, const int *innew, const int *tinter, int nfield, int ld,
                   const double *const *u, double *const *unew) {
   const int nm = (nc + 1) * (nc + 2) / 2;
-  if (nc < 0 || nm > 64 || ng <= 0 || ng > 64 || ld_g < ng || ld_m < nm || ld < nm || ne <= 0 ||
-      nenew <= 0 || nfield <= 0 || !wi || !xi || !yi || !pi2 || !amm || !xgr || !ygr || !in ||
-      !itm || !tseg || !pseg || !xgrnew || !ygrnew || !innew || !tinter || !u || !unew)
+  if (nc < 0 || nm > 64 || ng <= 0 || ng > 64 || ld_g < ng || ld_m < nm || ld < nm || nn <= 0 ||
+      ne <= 0 || nf <= 0 || nnnew <= 0 || nenew <= 0 || nfield <= 0 || !wi || !xi || !yi ||
+      !pi2 || !amm || !xgr || !ygr || !in || !itm || !tseg || !pseg || !xgrnew || !ygrnew ||
+      !innew || !tinter || !u || !unew)
     return spoc::report(SPOC_EINVAL, "spoc_interpol: bad arguments");
+  // the walk indexes nodes, faces and elements with these: check them first
+  const char *fn = "spoc_interpol";
+  if (!in_range(in, 3 * (size_t)ne, 1, nn, fn, "in") ||
+      !in_range(itm, 3 * (size_t)ne, 1, nf, fn, "itm") ||
+      !in_range(pseg, 2 * (size_t)nf, 1, nn, fn, "pseg") ||
+      !in_range(innew, 3 * (size_t)nenew, 1, nnnew, fn, "innew") ||
+      !in_range(tinter, (size_t)nenew, 0, ne, fn, "tinter"))
+    return SPOC_EINVAL;
+  for (int f = 0; f < nf; f++)
+    if (tseg[2 * f] < 1 || tseg[2 * f] > ne || tseg[2 * f + 1] < 0 || tseg[2 * f + 1] > ne)
+      return spoc::report(SPOC_EINVAL, "spoc_interpol: tseg(:,%d)=(%d,%d) out of range", f + 1,
+                          tseg[2 * f], tseg[2 * f + 1]);
   const Rule R{nc, nm, ng, wi, xi, yi, pi2, amm, ld_g, ld_m};
   auto X = [&](int n) { return xgr[n - 1]; };  // 1-based node access
   auto Y = [&](int n) { return ygr[n - 1]; };
@@ -101,8 +125,6 @@ int spoc_interpol(int nc, int ng, const double *wi, const double *xi, const doub
   for (int i = 0; i < nenew; i++) {
     if (tinter[i] != 0) {  // an element of the old mesh, kept: copy it
       const int t = tinter[i];
-      if (t < 1 || t > ne)
-        return spoc::report(SPOC_EINVAL, "spoc_interpol: tinter(%d)=%d out of range", i + 1, t);
       for (int f = 0; f < nfield; f++)
         for (int m = 0; m < nm; m++)
           unew[f][(size_t)ld * i + m] = u[f][(size_t)ld * (t - 1) + m];
@@ -165,19 +187,21 @@ int spoc_interpol(int nc, int ng, const double *wi, const double *xi, const doub
 
 int spoc_calh2(int nc, int ng, const double *wi, const double *xi, const double *yi,
                const double *pi2, int ld_g, const double *amm, int ld_m, double (*ff)(double x,
-               double y, void *user), void *user, int ne, const double *xgr, const double *ygr,
-               const int *in, const int *tinter, int ld, const double *h0, double *h1) {
+               double y, void *user), void *user, int nn, int ne, const double *xgr,
+               const double *ygr, const int *in, const int *tinter, int ne0, int ld,
+               const double *h0, double *h1) {
   const int nm = (nc + 1) * (nc + 2) / 2;
-  if (nc < 0 || nm > 64 || ng <= 0 || ng > 64 || ld_g < ng || ld_m < nm || ld < nm || ne <= 0 ||
-      !ff || !wi || !xi || !yi || !pi2 || !amm || !xgr || !ygr || !in || !tinter || !h1)
+  if (nc < 0 || nm > 64 || ng <= 0 || ng > 64 || ld_g < ng || ld_m < nm || ld < nm || nn <= 0 ||
+      ne <= 0 || ne0 < 0 || !ff || !wi || !xi || !yi || !pi2 || !amm || !xgr || !ygr || !in ||
+      !tinter || !h1)
     return spoc::report(SPOC_EINVAL, "spoc_calh2: bad arguments");
+  if (!in_range(in, 3 * (size_t)ne, 1, nn, "spoc_calh2", "in") ||
+      !in_range(tinter, (size_t)ne, 0, h0 ? ne0 : 0, "spoc_calh2", "tinter"))
+    return SPOC_EINVAL;
   const Rule R{nc, nm, ng, wi, xi, yi, pi2, amm, ld_g, ld_m};
   std::vector<double> mff(ng), b(nm);
   for (int i = 0; i < ne; i++) {
     if (tinter[i] != 0) {
-      if (!h0 || tinter[i] < 1)
-        return spoc::report(SPOC_EINVAL, "spoc_calh2: element %d copies from h0, which is missing "
-                            "or the index %d is bad", i + 1, tinter[i]);
       for (int m = 0; m < nm; m++) h1[(size_t)ld * i + m] = h0[(size_t)ld * (tinter[i] - 1) + m];
       continue;
     }

@@ -11,8 +11,11 @@
 ! SPOC_DEVICE=n selects the HIP device.
 !
 ! The mesh and the per-run fields are uploaded on the first call and again
-! whenever ne, nf or a checksum of the connectivity changes (the adaptive
-! driver remeshes between calls, src/shwwnl_adap.f:420-432).
+! whenever any of them changes: the shim keeps copies of every mesh operand
+! (itm, its, tseg, ssit, bdl, norm, vect, normt) and of rows 1..nm of the
+! fields (fc, w0, z0, hb) and compares them exactly on each call, since the
+! adaptive driver remeshes between calls (src/shwwnl_adap.f:420-432), possibly
+! to a mesh with the same ne and nf.
 
 #ifndef SPOC_VARIANT
 #define SPOC_VARIANT 1
@@ -136,10 +139,13 @@ subroutine cal_var(nf, ne, u0, v0, h0, u1, v1, h1, tseg, pseg, ssit, itm, its, n
        h1(nnmod, *), norm(2, *), vect(5, *), normt(2, *), fc(nnmod, *), w0(nnmod, *), &
        z0(nnmod, *), hb(nnmod, *), mu0, mb0, beta, g0, time
   integer, save :: last_ne = -1, last_nf = -1, prec = 0
-  integer(c_int64_t), save :: last_sum = -1
-  double precision, save :: last_par(4) = -1.d300
-  integer(c_int), allocatable, save :: ibdl(:)
-  integer(c_int64_t) :: s
+  double precision, save :: last_par(3) = -1.d300
+  ! copies of every operand the device holds, compared exactly on each call:
+  ! mesh (itm, its, tseg, ssit, bdl, norm, vect, normt) and fields (rows
+  ! 1..nm of fc, w0, z0, hb)
+  integer(c_int), allocatable, save :: s_itm(:), s_its(:), s_tseg(:), s_ssit(:), ibdl(:)
+  double precision, allocatable, save :: s_geo(:), s_fld(:, :, :)
+  logical :: remesh, refield
   integer :: i, dev, stat, l
   character(len=16) :: env
   if (.not. c_associated(spoc_ctx_h)) then
@@ -151,36 +157,108 @@ subroutine cal_var(nf, ne, u0, v0, h0, u1, v1, h1, tseg, pseg, ssit, itm, its, n
     call spoc_check(spoc_ctx_create(spoc_ctx_h, dev, nc, ng, ngf, wi, wif, pi2, ng_max, pico, ngf_max, &
          nnmod, amm, amml, agx, agy, ammo), 'spoc_ctx_create')
   end if
-  ! connectivity checksum: detects remeshing between calls
-  s = 0
-  do i = 1, ne
-    s = s + int(itm(1, i), c_int64_t) * 3 + int(itm(2, i), c_int64_t) * 5 + &
-         int(itm(3, i), c_int64_t) * 7 + int(i, c_int64_t) * int(its(1, i) + 2, c_int64_t)
-  end do
-  do i = 1, nf
-    s = s + int(tseg(1, i), c_int64_t) * 11 + int(tseg(2, i), c_int64_t) * 13
-  end do
-  if (ne /= last_ne .or. nf /= last_nf .or. s /= last_sum) then
-    if (allocated(ibdl)) deallocate(ibdl)
-    allocate(ibdl(ne))
+  ! remeshing between calls (src/shwwnl_adap.f:420-432) may keep ne and nf
+  remesh = ne /= last_ne .or. nf /= last_nf .or. .not. allocated(s_itm)
+  if (.not. remesh) remesh = int_differs(3 * ne, itm, s_itm) .or. &
+       int_differs(3 * ne, its, s_its) .or. int_differs(2 * nf, tseg, s_tseg) .or. &
+       int_differs(2 * nf, ssit, s_ssit) .or. bdl_differs() .or. geo_differs()
+  if (remesh) then
+    if (allocated(s_itm)) deallocate(s_itm, s_its, s_tseg, s_ssit, ibdl, s_geo)
+    allocate(s_itm(3 * ne), s_its(3 * ne), s_tseg(2 * nf), s_ssit(2 * nf), ibdl(ne), &
+         s_geo(2 * nf + 7 * ne))
+    call int_copy(3 * ne, itm, s_itm)
+    call int_copy(3 * ne, its, s_its)
+    call int_copy(2 * nf, tseg, s_tseg)
+    call int_copy(2 * nf, ssit, s_ssit)
     do i = 1, ne
       ibdl(i) = 0
       if (bdl(i)) ibdl(i) = 1
     end do
+    call real_copy(2 * nf, norm, s_geo(1))
+    call real_copy(5 * ne, vect, s_geo(2 * nf + 1))
+    call real_copy(2 * ne, normt, s_geo(2 * nf + 5 * ne + 1))
     call spoc_check(spoc_mesh_set(spoc_ctx_h, ne, nf, itm, its, tseg, ssit, norm, vect, normt, &
          ibdl, 0, c_null_ptr, c_null_ptr, c_null_ptr), 'spoc_mesh_set')
     spoc_ctx_ne = ne
     last_ne = ne
     last_nf = nf
-    last_sum = s
-    last_par = -1.d300
   end if
-  if (last_par(1) /= mu0 .or. last_par(2) /= mb0 .or. last_par(3) /= g0 .or. &
-       last_par(4) /= hb(1, 1)) then
+  ! a new mesh drops the device fields; otherwise compare them
+  refield = remesh .or. .not. allocated(s_fld)
+  if (.not. refield) refield = last_par(1) /= mu0 .or. last_par(2) /= mb0 .or. &
+       last_par(3) /= g0
+  if (.not. refield) refield = fld_differs(fc, 1) .or. fld_differs(w0, 2) .or. &
+       fld_differs(z0, 3) .or. fld_differs(hb, 4)
+  if (refield) then
+    if (allocated(s_fld)) deallocate(s_fld)
+    allocate(s_fld(nm, ne, 4))
+    do i = 1, ne
+      s_fld(:, i, 1) = fc(1:nm, i)
+      s_fld(:, i, 2) = w0(1:nm, i)
+      s_fld(:, i, 3) = z0(1:nm, i)
+      s_fld(:, i, 4) = hb(1:nm, i)
+    end do
     call spoc_check(spoc_fields_set(spoc_ctx_h, nnmod, fc, w0, z0, hb, mu0, mb0, g0), &
          'spoc_fields_set')
-    last_par = [mu0, mb0, g0, hb(1, 1)]
+    last_par = [mu0, mb0, g0]
   end if
   call spoc_check(spoc_rhs(spoc_ctx_h, SPOC_VARIANT, prec, nnmod, u0, v0, h0, u1, v1, h1), &
        'spoc_rhs')
+contains
+  logical function int_differs(n, a, b)
+    integer, intent(in) :: n
+    integer, intent(in) :: a(n)
+    integer(c_int), intent(in) :: b(n)
+    integer :: k
+    int_differs = .true.
+    do k = 1, n
+      if (a(k) /= b(k)) return
+    end do
+    int_differs = .false.
+  end function
+  subroutine int_copy(n, a, b)
+    integer, intent(in) :: n
+    integer, intent(in) :: a(n)
+    integer(c_int), intent(out) :: b(n)
+    b = a
+  end subroutine
+  subroutine real_copy(n, a, b)
+    integer, intent(in) :: n
+    double precision, intent(in) :: a(n)
+    double precision, intent(out) :: b(n)
+    b = a
+  end subroutine
+  logical function real_differs(n, a, b)
+    integer, intent(in) :: n
+    double precision, intent(in) :: a(n), b(n)
+    integer :: k
+    real_differs = .true.
+    do k = 1, n
+      if (a(k) /= b(k)) return
+    end do
+    real_differs = .false.
+  end function
+  logical function bdl_differs()
+    integer :: k
+    bdl_differs = .true.
+    do k = 1, ne
+      if (bdl(k) .neqv. (ibdl(k) /= 0)) return
+    end do
+    bdl_differs = .false.
+  end function
+  logical function geo_differs()
+    geo_differs = real_differs(2 * nf, norm, s_geo(1)) .or. &
+         real_differs(5 * ne, vect, s_geo(2 * nf + 1)) .or. &
+         real_differs(2 * ne, normt, s_geo(2 * nf + 5 * ne + 1))
+  end function
+  logical function fld_differs(a, k)
+    double precision, intent(in) :: a(nnmod, *)
+    integer, intent(in) :: k
+    integer :: e
+    fld_differs = .true.
+    do e = 1, ne
+      if (real_differs(nm, a(1, e), s_fld(1, e, k))) return
+    end do
+    fld_differs = .false.
+  end function
 end subroutine

@@ -102,3 +102,58 @@ def test_projection_onto_own_mesh_is_identity(nc, ng, ngf):
     P = t["pi2"][:ng, :nm]
     vo, vu = P @ out[:nm], P @ u[:nm]
     assert np.abs(vo - vu).max() < 1e-10 * np.abs(vu).max()
+
+
+def _bad(arr, idx, val):
+    a = np.array(arr, copy=True, order="F")
+    a.flat[idx] = val
+    return a
+
+
+@pytest.mark.parametrize("key,idx,val", [("in", 5, 0), ("in", 7, 10 ** 6), ("itm", 3, 0),
+                                         ("itm", 4, 10 ** 6), ("tseg", 0, 0), ("tseg", 1, -1),
+                                         ("tseg", 3, 10 ** 6), ("pseg", 2, 0),
+                                         ("pseg", 5, 10 ** 6)])
+def test_interpol_rejects_malformed_old_mesh(key, idx, val):
+    """Index arrays pointing outside their ranges are an error before any
+    walk reads through them (the reference would read out of bounds)."""
+    t = host_tables(3, 12, 4)
+    c = configs.random_case(t, 5, 4, seed=3, scale=0.01)
+    m = dict(c["mesh"])
+    m[key] = _bad(m[key], idx, val)
+    u = c["state"][0]
+    with pytest.raises(spoc_amd.SpocError, match=key):
+        spoc_amd.interpol(t, m, m["x"], m["y"], c["mesh"]["in"], np.zeros(m["ne"], np.int32), [u])
+
+
+@pytest.mark.parametrize("what", ["innew", "tinter"])
+def test_interpol_rejects_malformed_new_mesh(what):
+    t = host_tables(3, 12, 4)
+    c = configs.random_case(t, 5, 4, seed=3, scale=0.01)
+    m = c["mesh"]
+    tri = np.array(m["in"], order="F")
+    tinter = np.zeros(m["ne"], np.int32)
+    if what == "innew":
+        tri[1, 2] = len(m["x"]) + 1
+    else:
+        tinter[4] = m["ne"] + 1
+    with pytest.raises(spoc_amd.SpocError, match=what):
+        spoc_amd.interpol(t, m, m["x"], m["y"], tri, tinter, [c["state"][0]])
+
+
+def test_calh2_rejects_malformed_input():
+    t = host_tables(3, 12, 4)
+    c = configs.random_case(t, 5, 4, seed=3, scale=0.01)
+    m = c["mesh"]
+    tri = np.array(m["in"], order="F")
+    tri[0, 1] = 0
+    with pytest.raises(spoc_amd.SpocError, match="in"):
+        spoc_amd.calh2(t, _ff, m["x"], m["y"], tri)
+    h0 = c["phys"]["hb"]
+    tinter = np.zeros(m["ne"], np.int32)
+    tinter[2] = h0.shape[1] + 1  # past h0's elements
+    with pytest.raises(spoc_amd.SpocError, match="tinter"):
+        spoc_amd.calh2(t, _ff, m["x"], m["y"], m["in"], tinter, h0)
+    tinter[2] = 1  # copies from h0, which is missing
+    with pytest.raises(spoc_amd.SpocError, match="tinter"):
+        spoc_amd.calh2(t, _ff, m["x"], m["y"], m["in"], tinter, None)
