@@ -50,19 +50,19 @@ def quad_rule(ng, ngf):
 METRIC = "element-DOF RK4 stage updates/s, FP32-MFMA vs FP64 GPU at \u03b7 rel-L2 \u22641e-6"
 
 WORKLOADS = {
-    "c2": dict(shape=(6, 33, 7), steps=100, name=lambda nc, ng, ngf, ne, dt, a: (
+    "c2": dict(shape=(6, 33, 7), steps=100, gate=1000, name=lambda nc, ng, ngf, ne, dt, a: (
         f"calvarnl_fs p={nc} ng={ng} ngf={ngf} on {ne} triangles (jittered {a.nx}x{a.ny} nodes, "
         f"4000x2000 km, beta plane, double-gyre wind, mu0=100), dt={dt} s")),
-    "c1": dict(shape=(3, 12, 4), steps=1000, name=lambda nc, ng, ngf, ne, dt, a: (
+    "c1": dict(shape=(3, 12, 4), steps=1000, gate=1000, name=lambda nc, ng, ngf, ne, dt, a: (
         f"calvar_lin p={nc} ng={ng} ngf={ngf} on {ne} triangles (1000 km square, 16x16 "
         f"squares), H=4000, 1 m Gaussian eta, dt={dt} s")),
-    "c3": dict(shape=(7, 42, 8), steps=50, name=lambda nc, ng, ngf, ne, dt, a: (
+    "c3": dict(shape=(7, 42, 8), steps=50, gate=2000, name=lambda nc, ng, ngf, ne, dt, a: (
         f"calvarnl_ns p={nc} ng={ng} ngf={ngf} on {ne} triangles (4000x2000 km), 20 seamounts, "
         f"k^-3 eddies RMS 0.3 m/s, nu=100, dt={dt} s")),
-    "c4": dict(shape=(5, 25, 6), steps=100, name=lambda nc, ng, ngf, ne, dt, a: (
+    "c4": dict(shape=(5, 25, 6), steps=100, gate=500, name=lambda nc, ng, ngf, ne, dt, a: (
         f"calvarnl_inv p={nc} ng={ng} ngf={ngf} on {ne} triangles (4000x2000 km), mu0=0, "
         f"1 m Gaussian eta, dt={dt} s")),
-    "c5": dict(shape=(4, 25, 5), steps=50, name=lambda nc, ng, ngf, ne, dt, a: (
+    "c5": dict(shape=(4, 25, 5), steps=50, gate=500, name=lambda nc, ng, ngf, ne, dt, a: (
         f"ensemble of {a.members} calvarnl_fs members, p={nc} ng={ng} ngf={ngf}, 5k-60k "
         f"triangles each, {ne} in total, one union mesh per launch, dt={dt} s")),
 }
@@ -239,6 +239,122 @@ def adapt_timing(ctx, state, ne, nc, ng, ngf, cpu, reps=20, fac=(0.1, 0.01, 1.0)
     return out
 
 
+def graph_prime_steps(K):
+    """Steps of an untimed call that captures the HIP graph a K-step call
+    replays (8 steps per graph from K >= 8, one step per graph from K >= 2;
+    a single step runs eagerly)."""
+    return 8 if K >= 8 else (2 if K >= 2 else 0)
+
+
+def warm(ctx, variant, dt, W, K, prec):
+    """W untimed warm-up steps, then (if those captured a graph of another
+    shape) one untimed call that captures the graph the timed K steps
+    replay, so that no capture falls inside the timed region."""
+    if W:
+        ctx.rk4(variant, dt, W, precision=prec)
+    want = graph_prime_steps(K)
+    if want and (W < 2 or (W >= 8) != (K >= 8)):
+        ctx.rk4(variant, dt, want, precision=prec)
+        return want
+    return 0
+
+
+def integrals(ctx):
+    """(ke, pe, mass) of the resident state, summed over ensemble members."""
+    if ctx.nmem > 1:
+        ke, pe, mass = ctx.ensemble_energy()
+        return float(ke.sum()), float(pe.sum()), float(mass.sum())
+    return ctx.energy()
+
+
+def mass_rel_l2(t, vect, a, b):
+    """Relative L2 error of a against b in the mass-matrix norm:
+    sqrt(sum_e det_e d^T M d / sum_e det_e b^T M b)."""
+    nm = t["nm"]
+    ammo = t["ammo"][:nm, :nm]
+    det = 1.0 / vect[4]
+    d = a[:nm] - b[:nm]
+    n1 = np.sum(det * np.einsum("ie,ij,je->e", d, ammo, d))
+    n2 = np.sum(det * np.einsum("ie,ij,je->e", b[:nm], ammo, b[:nm]))
+    return float(np.sqrt(n1 / n2))
+
+
+def gate_leg(ctx, t, variant, dt, state0, vect, nsteps):
+    """The metric's accuracy gate: nsteps untimed RK4 steps of both paths
+    from the same initial state; relative L2 (mass norm) of the mixed path
+    against the FP64 path per field, and the drift of kinetic energy,
+    potential energy and mass (src/energy.f:3-96) over the run on each path."""
+    out = {"steps": nsteps}
+    fin = {}
+    for prec in ("fp32", "fp64"):
+        ctx.upload(*state0)
+        e0 = integrals(ctx)
+        ctx.rk4(variant, dt, nsteps, precision=prec)
+        e1 = integrals(ctx)
+        fin[prec] = ctx.download()
+        out[f"drift_{prec}"] = {k: (b - a) / abs(a) if a else b - a
+                                for k, a, b in zip(("ke", "pe", "mass"), e0, e1)}
+        out[f"integrals_{prec}"] = {"start": dict(zip(("ke", "pe", "mass"), e0)),
+                                    "end": dict(zip(("ke", "pe", "mass"), e1))}
+    out["rel_l2"] = {f: mass_rel_l2(t, vect, fin["fp32"][i], fin["fp64"][i])
+                     for i, f in enumerate(("u", "v", "eta"))}
+    out["eta_le_1e-6"] = out["rel_l2"]["eta"] <= 1e-6
+    return out
+
+
+def dropin_timing(ctx, t, variant, dt, state0, nsteps=2):
+    """The unchanged driver's loop (src/shwwnl.f:238-284): per step four
+    spoc_rhs calls on host (45, ne) arrays, each copying the state in and
+    the tendencies out over PCIe, and the stage updates on the host (numpy,
+    in the driver's expression order).  Returns ms per step and per call
+    for both paths, and the PCIe bytes each call moves."""
+    nm = t["nm"]
+    u0 = [np.array(a, order="F") for a in state0]
+    ne = u0[0].shape[1]
+    outs = [[np.zeros_like(u0[0], order="F") for _ in range(3)] for _ in range(4)]
+    res = {}
+    for prec in ("fp64", "fp32"):
+        ctx.cal_var(variant, *u0, precision=prec, out=outs[0])  # warm
+        t_rhs = 0.0
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            x = u0
+            for s in range(4):
+                a = time.perf_counter()
+                ctx.cal_var(variant, *x, precision=prec, out=outs[s])
+                t_rhs += time.perf_counter() - a
+                if s < 3:
+                    f = 0.5 if s < 2 else 1.0
+                    x = [u0[i] + dt * outs[s][i] * f if f != 1.0 else u0[i] + dt * outs[s][i]
+                         for i in range(3)]
+            for i in range(3):
+                u0[i] = u0[i] + (dt / 6.0) * (((outs[0][i] + 2.0 * outs[1][i]) +
+                                               2.0 * outs[2][i]) + outs[3][i])
+        wall = time.perf_counter() - t0
+        per_call = t_rhs / (4 * nsteps)
+        res[prec] = {"ms_per_step": 1e3 * wall / nsteps, "ms_per_call": 1e3 * per_call}
+    nbytes = 2 * 3 * nm * ne * 8
+    res["pcie_bytes_per_call"] = nbytes
+    res["pcie_GBps_fp32"] = nbytes / (res["fp32"]["ms_per_call"] * 1e-3) / 1e9
+    res["steps"] = nsteps
+    return res
+
+
+def host_info():
+    """CPU model, host core count and the reference's compile flags."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "compiler": "flang -O2 -fPIC (oracle/Makefile; no -march, so no FMA contraction)"}
+
+
 def _cpu_case(variant, nc, ng, ngf, wl):
     """The bounded CPU sample of workload wl: the reference Fortran
     (oracle/_ref) when it can hold the rule, else the C restatement.
@@ -299,9 +415,10 @@ def cpu_baseline(variant, nc, ng, ngf, budget_s=15.0, wl="c2", workers=16):
     """The reference Fortran (oracle/_ref) on a bounded sample of the same
     workload: same p, rule and physics on a reduced mesh (at most 1600
     triangles, what the reference's static arrays hold), one core; then the
-    same sample in `workers` concurrent processes (at most the host's
-    cores), the reference being serial, as an all-core aggregate.  Rules of
-    more than 33 points exceed the reference's arrays; then the C
+    same sample in concurrent processes, the reference being serial, as an
+    all-core aggregate: one per host core up to `workers` (16, the host's
+    CPU share for one GPU on the measurement machine, whatever nproc says).
+    Rules of more than 33 points exceed the reference's arrays; then the C
     restatement runs instead (kind "port")."""
     runner, ne, nm, kind = _cpu_case(variant, nc, ng, ngf, wl)
     n, sec = _timed(runner, budget_s=budget_s)
@@ -309,7 +426,8 @@ def cpu_baseline(variant, nc, ng, ngf, budget_s=15.0, wl="c2", workers=16):
     out = {"value": val, "unit": "element-DOF stage updates/s", "cores": 1, "kind": kind,
            "sample": f"{variant} p={nc} ng={ng} on {ne} triangles (the workload's generator "
                      f"at a reduced node count), {n} RK4 steps in {sec:.1f} s, "
-                     f"1 thread, {platform.processor() or platform.machine()}"}
+                     f"1 thread"}
+    out.update(host_info())
     k = max(1, min(workers, os.cpu_count() or 1))
     if k > 1:
         import subprocess
@@ -325,9 +443,9 @@ def cpu_baseline(variant, nc, ng, ngf, budget_s=15.0, wl="c2", workers=16):
             slowest = max(r["s"] for r in runs)
             out["all_cores"] = {
                 "value": 4 * ne * nm * 3 * sum(r["n"] for r in runs) / slowest,
-                "processes": k,
-                "sample": f"{k} concurrent processes of the same sample, {n} steps each, "
-                          f"slowest {slowest:.1f} s"}
+                "processes": k, "nproc": os.cpu_count(),
+                "sample": f"{k} concurrent processes of the same sample (min(nproc, {workers})), "
+                          f"{n} steps each, slowest {slowest:.1f} s"}
     return out
 
 
@@ -362,6 +480,12 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None,
                     help="steps of the plain FP64 path to time and to compare eta against "
                          "(default: same as --steps)")
+    ap.add_argument("--gate-steps", type=int, default=None,
+                    help="steps of the untimed accuracy leg (both paths from the same state; "
+                         "default: the configuration's run length, 1000 for c2); 0 skips it")
+    ap.add_argument("--dropin-steps", type=int, default=2,
+                    help="steps of the per-call drop-in loop (4 spoc_rhs calls per step on "
+                         "host arrays); 0 skips it")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -417,8 +541,7 @@ def main():
         calib_copy(args.calib_copy << 20)
     # mixed-precision path: warm up, then time K steps from the same initial state
     ctx.upload(*state0)
-    if W:
-        ctx.rk4(variant, dt, W, precision="fp32")
+    prime = warm(ctx, variant, dt, W, K, "fp32")
     ctx.upload(*state0)
     ctx.sync()
     t0 = time.perf_counter()
@@ -437,7 +560,7 @@ def main():
     out = {
         "metric": METRIC,
         "value": value, "unit": "element-DOF stage updates/s", "n_gpus": 1, "steps": K,
-        "warmup": W, "ms_per_step": 1e3 * wall / K, "higher_is_better": True, "scaling": "weak",
+        "warmup": W, "graph_prime_steps": prime, "ms_per_step": 1e3 * wall / K, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32" if variant != "lin" else "f64",
         "data": "synthetic",
         "config": {"workload": WORKLOADS[wl]["name"](nc, ng, ngf, ne, dt, args),
@@ -471,9 +594,11 @@ def main():
                   algorithmic_bytes_per_elem=by,
                   other={k: {"ms": v[0], "launches": v[1]} for k, v in kern.items()})
         out["roofline"] = rf
-    # plain FP64 path: same initial state, same steps; eta error of FP32
+    # plain FP64 path: same initial state, same warm-up and steps
     if not args.no_fp64:
         K64 = args.fp64_steps or K
+        ctx.upload(*state0)
+        warm(ctx, variant, dt, W, K64, "fp64")
         ctx.upload(*state0)
         ctx.sync()
         t0 = time.perf_counter()
@@ -482,19 +607,17 @@ def main():
         w64 = time.perf_counter() - t0
         s64 = ctx.download()
         out["fp64_value"] = dof * K64 / w64
+        out["fp64_ms_per_step"] = 1e3 * w64 / K64
         out["fp32_over_fp64"] = value / out["fp64_value"]
         if K64 == K:
-            ammo = t["ammo"][:nm, :nm]
-            det = 1.0 / vect[4]
-
-            def mrel(a, b):
-                d = a[:nm] - b[:nm]
-                n1 = np.sum(det * np.einsum("ie,ij,je->e", d, ammo, d))
-                n2 = np.sum(det * np.einsum("ie,ij,je->e", b[:nm], ammo, b[:nm]))
-                return float(np.sqrt(n1 / n2))
-
-            out["rel_l2_vs_fp64"] = {"u": mrel(s32[0], s64[0]), "v": mrel(s32[1], s64[1]),
-                                      "eta": mrel(s32[2], s64[2]), "steps": K}
+            out["rel_l2_vs_fp64"] = {f: mass_rel_l2(t, vect, s32[i], s64[i])
+                                     for i, f in enumerate(("u", "v", "eta"))}
+            out["rel_l2_vs_fp64"]["steps"] = K
+    G = WORKLOADS[wl]["gate"] if args.gate_steps is None else args.gate_steps
+    if G and not args.no_fp64:
+        out["accuracy_gate"] = gate_leg(ctx, t, variant, dt, state0, vect, G)
+    if args.dropin_steps and wl != "c5":
+        out["dropin"] = dropin_timing(ctx, t, variant, dt, state0, args.dropin_steps)
     if args.adjoint and wl == "c4":
         out["adjoint"] = adjoint_check(ctx, t, c, K, args.adjoint)
     out["cal_conv"] = adapt_timing(ctx, s32, ne, nc, ng, ngf, cpu=not args.no_cpu)

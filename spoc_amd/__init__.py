@@ -31,7 +31,14 @@ _lib = None
 
 
 class SpocError(RuntimeError):
-    pass
+    """A nonzero SPOC_E* status; .code holds it."""
+
+    def __init__(self, msg: str, code: int = 0):
+        super().__init__(msg)
+        self.code = code
+
+
+ENAN = -7  # SPOC_ENAN: the state is no longer finite
 
 
 def load_library(path: str = LIB_PATH) -> C.CDLL:
@@ -62,7 +69,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
 
 def _check(rc: int) -> None:
     if rc != 0:
-        raise SpocError(f"spoc error {rc}: {load_library().spoc_last_error().decode()}")
+        raise SpocError(f"spoc error {rc}: {load_library().spoc_last_error().decode()}", rc)
 
 
 def _d(a: np.ndarray):
@@ -289,13 +296,20 @@ class Context:
                                         C.c_double(mu0), C.c_double(mb0), C.c_double(g0)))
 
     # -- the reference call -------------------------------------------------
-    def cal_var(self, variant, u0, v0, h0, precision="fp64"):
-        """Tendencies (u1, v1, h1) of the given state, shaped like u0."""
+    def cal_var(self, variant, u0, v0, h0, precision="fp64", out=None):
+        """Tendencies (u1, v1, h1) of the given state, shaped like u0 (into
+        `out`, three Fortran-ordered float64 arrays of that shape, if given)."""
         var = VARIANTS[variant] if isinstance(variant, str) else variant
         prec = PRECISIONS[precision] if isinstance(precision, str) else precision
         u0, v0, h0 = (np.asfortranarray(a, np.float64) for a in (u0, v0, h0))
         ld = u0.shape[0]
-        out = [np.zeros_like(u0, order="F") for _ in range(3)]
+        if out is None:
+            out = [np.zeros_like(u0, order="F") for _ in range(3)]
+        else:
+            for a in out:
+                if a.shape != u0.shape or a.dtype != np.float64 or not a.flags.f_contiguous:
+                    raise ValueError("cal_var: out arrays must be Fortran-ordered float64 "
+                                     "of the state's shape")
         _check(self.lib.spoc_rhs(self._h, var, prec, ld, _d(u0), _d(v0), _d(h0),
                                  *(_d(a) for a in out)))
         return tuple(out)
