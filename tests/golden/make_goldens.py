@@ -25,8 +25,15 @@ Fixtures:
   interp_p<p>.npz      interpol onto a 4-split refinement (some elements kept)
                        and onto a larger jittered mesh; calh2 of a test function
                        (src/adapt.f:1662-1824, 303-373)
+  calvar_p4.npz,       one cal_var per variant at the shapes of configurations
+  calvar_p5.npz        5 (p=4, 25-point rule) and 4 (p=5, 25-point rule)
+  rk4_cfg.npz          10 RK4 steps with energy/mass before and after, from the
+                       initial states of the configuration generators on reduced
+                       meshes: c2 (fs, p=6, 1152 triangles), c4 (inv, p=5, 576),
+                       c5 (fs, p=4, one member of about 1100)
 
-    python tests/golden/make_goldens.py [/root/reference] [--only-calconv|--only-interp]
+    python tests/golden/make_goldens.py [/root/reference] [--only-calconv|--only-interp|
+                                                           --only-configs]
 """
 from __future__ import annotations
 
@@ -191,6 +198,10 @@ def main(refroot="/root/reference", *flags):
         interp_goldens(q)
         print("interpolation goldens written to", OUT)
         return
+    if "--only-configs" in flags:
+        config_goldens(q)
+        print("configuration goldens written to", OUT)
+        return
     for nc, ng, ngf in ((3, 12, 4), (6, 33, 7), (4, 25, 5), (5, 25, 6)):
         R = oracle.RefLib("fs")
         i1, i2 = R.setup(nc, *rule(q, nc, ng, ngf))
@@ -215,24 +226,8 @@ def main(refroot="/root/reference", *flags):
     np.savez_compressed(os.path.join(OUT, "mesh_jit.npz"), x=xj, y=yj, tri_in=rng_tri, tri=trij,
                         **{k: v for k, v in mj.items() if k != "nf"}, nf=mj["nf"])
     # single cal_var on seeded random states
-    for nc, ng, ngf, nx, ny in ((3, 12, 4, 17, 17), (6, 33, 7, 9, 9)):
-        out = {}
-        for var in ("lin", "fs", "ns", "inv"):
-            Rv = oracle.RefLib(var)
-            Rv.setup(nc, *rule(q, nc, ng, ngf))
-            tabs = dict(nc=nc, ng=ng, ngf=ngf, nm=(nc + 1) * (nc + 2) // 2)
-            c = configs.random_case(_tables_for(q, nc, ng, ngf), nx, ny, seed=11 + nc)
-            m = Rv.mesh(c["mesh"]["x"], c["mesh"]["y"], c["mesh"]["in"], nc, 1e-4, 2e-11)
-            p = c["phys"]
-            p["fc"] = m["fc"]
-            u1 = run_big_stack(Rv.cal_var, m, p, *c["state"])
-            nm = tabs["nm"]
-            for k, a in zip("uvh", u1):
-                out[f"{var}_{k}1"] = a[:nm]
-        st = c["state"]
-        np.savez_compressed(os.path.join(OUT, f"calvar_p{nc}.npz"), nx=nx, ny=ny, seed=11 + nc,
-                            u0=st[0][:nm], v0=st[1][:nm], h0=st[2][:nm], hb=p["hb"][:nm],
-                            w0=p["w0"][:nm], z0=p["z0"][:nm], fc=p["fc"][:nm], **out)
+    calvar_goldens(q, 3, 12, 4, 17, 17)
+    calvar_goldens(q, 6, 33, 7, 9, 9)
     # 10 RK4 steps from the config-1 initial state
     tb = _tables_for(q, 3, 12, 4)
     c = configs.config1(tb)
@@ -255,7 +250,65 @@ def main(refroot="/root/reference", *flags):
                         u0=c["state"][0][:10], v0=c["state"][1][:10], h0=c["state"][2][:10], **out)
     calconv_goldens(q)
     interp_goldens(q)
+    config_goldens(q)
     print("goldens written to", OUT)
+
+
+def calvar_goldens(q, nc, ng, ngf, nx, ny):
+    """One cal_var per variant on a seeded random_case state (the reference's
+    own mesh pointers, its fc)."""
+    out = {}
+    nm = (nc + 1) * (nc + 2) // 2
+    for var in ("lin", "fs", "ns", "inv"):
+        Rv = oracle.RefLib(var)
+        Rv.setup(nc, *rule(q, nc, ng, ngf))
+        c = configs.random_case(_tables_for(q, nc, ng, ngf), nx, ny, seed=11 + nc)
+        m = Rv.mesh(c["mesh"]["x"], c["mesh"]["y"], c["mesh"]["in"], nc, 1e-4, 2e-11)
+        p = c["phys"]
+        p["fc"] = m["fc"]
+        u1 = run_big_stack(Rv.cal_var, m, p, *c["state"])
+        for k, a in zip("uvh", u1):
+            out[f"{var}_{k}1"] = a[:nm]
+    st = c["state"]
+    np.savez_compressed(os.path.join(OUT, f"calvar_p{nc}.npz"), nx=nx, ny=ny, seed=11 + nc,
+                        u0=st[0][:nm], v0=st[1][:nm], h0=st[2][:nm], hb=p["hb"][:nm],
+                        w0=p["w0"][:nm], z0=p["z0"][:nm], fc=p["fc"][:nm], **out)
+
+
+# reduced meshes of the configuration generators (the reference holds at
+# most 1600 triangles): name -> (p, ng, ngf, generator, variant)
+CFG_CASES = {
+    "c2": (6, 33, 7, lambda tb: configs.config2(tb, nx=25, ny=25), "fs"),
+    "c4": (5, 25, 6, lambda tb: configs.config4(tb, nx=25, ny=13), "inv"),
+    "c5": (4, 25, 5, lambda tb: configs.config5(tb, nmem=1, ntri=(1100, 1100))[0], "fs"),
+}
+
+
+def config_goldens(q):
+    calvar_goldens(q, 4, 25, 5, 12, 10)
+    calvar_goldens(q, 5, 25, 6, 12, 10)
+    out = {}
+    for name, (nc, ng, ngf, gen, var) in CFG_CASES.items():
+        tb = _tables_for(q, nc, ng, ngf)
+        c = gen(tb)
+        nm = tb["nm"]
+        Rv = oracle.RefLib(var)
+        Rv.setup(nc, *rule(q, nc, ng, ngf))
+        m = Rv.mesh(c["mesh"]["x"], c["mesh"]["y"], c["mesh"]["in"], nc, 1e-4, 2e-11)
+        p = dict(c["phys"])
+        p["fc"] = m["fc"]
+        m["in"], m["x"], m["y"] = c["mesh"]["in"], c["mesh"]["x"], c["mesh"]["y"]
+        e0 = run_big_stack(Rv.energy, m, p, *c["state"])
+        r = run_big_stack(Rv.rk4, m, p, *c["state"], c["dt"], 10)
+        e1 = run_big_stack(Rv.energy, m, p, *r)
+        out[f"{name}_dt"] = c["dt"]
+        out[f"{name}_ne"] = c["mesh"]["ne"]
+        for k, a in zip("uvh", c["state"]):
+            out[f"{name}_{k}0"] = a[:nm]
+        for k, a in zip("uvh", r):
+            out[f"{name}_{k}"] = a[:nm]
+        out[f"{name}_energy"] = np.array([e0, e1])
+    np.savez_compressed(os.path.join(OUT, "rk4_cfg.npz"), **out)
 
 
 def _tables_for(q, nc, ng, ngf):

@@ -44,7 +44,7 @@ def _case(nc, ng, ngf):
     return O, c, g
 
 
-@pytest.mark.parametrize("nc,ng,ngf", [(3, 12, 4), (6, 33, 7)])
+@pytest.mark.parametrize("nc,ng,ngf", [(3, 12, 4), (4, 25, 5), (5, 25, 6), (6, 33, 7)])
 @pytest.mark.parametrize("variant", ["lin", "fs", "ns", "inv"])
 def test_cal_var_bit_exact(nc, ng, ngf, variant):
     O, c, g = _case(nc, ng, ngf)
@@ -99,3 +99,43 @@ def test_jittered_mesh_matches_reference_cm_and_pointers():
     m = mesh.build(x, y, tri2, 3, 1e-4, 2e-11)
     for k in ("itm", "its", "tseg", "pseg", "ssit", "norm", "vect", "normt", "bdl", "fc"):
         assert np.array_equal(m[k], g[k]), k
+
+
+# reduced meshes of the configuration generators, as written by
+# tests/golden/make_goldens.py (config_goldens)
+CFG = {
+    "c2": ((6, 33, 7), lambda tb: configs.config2(tb, nx=25, ny=25), "fs"),
+    "c4": ((5, 25, 6), lambda tb: configs.config4(tb, nx=25, ny=13), "inv"),
+    "c5": ((4, 25, 5), lambda tb: configs.config5(tb, nmem=1, ntri=(1100, 1100))[0], "fs"),
+}
+
+
+def cfg_case(name):
+    """(oracle, case, golden) of configuration `name` on its reduced mesh;
+    the generator reproduces the golden initial state and time step."""
+    (nc, ng, ngf), gen, variant = CFG[name]
+    g = gold("rk4_cfg.npz")
+    O = oracle_for(nc, ng, ngf)
+    tabs = O.tables()
+    w, x, y, wf, tf = quad_rule(ng, ngf)
+    tabs.update(nc=nc, ng=ng, ngf=ngf, nm=O.nm, wi=w, xi=x, yi=y, wif=wf, tif=tf)
+    c = gen(tabs)
+    assert c["variant"] == variant and c["dt"] == float(g[f"{name}_dt"])
+    assert c["mesh"]["ne"] == int(g[f"{name}_ne"])
+    for k, a in zip("uvh", c["state"]):
+        assert np.array_equal(a[:O.nm], g[f"{name}_{k}0"]), k
+    return O, c, g
+
+
+@pytest.mark.parametrize("name", sorted(CFG))
+def test_rk4_configurations_bit_exact(name):
+    """10 RK4 steps of configurations 2 (fs, p=6), 4 (inv, p=5) and 5 (fs,
+    p=4, 25-point rule) on reduced meshes, with energy and mass before and
+    after: the oracle equals the reference bit for bit."""
+    O, c, g = cfg_case(name)
+    e0 = O.energy(c["mesh"], c["phys"], *c["state"])
+    out = O.rk4(c["variant"], c["mesh"], c["phys"], *c["state"], c["dt"], 10)
+    e1 = O.energy(c["mesh"], c["phys"], *out)
+    for k, a in zip("uvh", out):
+        assert np.array_equal(a[:O.nm], g[f"{name}_{k}"]), k
+    assert np.array_equal(np.array([e0, e1]), g[f"{name}_energy"])
