@@ -282,20 +282,26 @@ def mass_rel_l2(t, vect, a, b):
 def gate_leg(ctx, t, variant, dt, state0, vect, nsteps):
     """The metric's accuracy gate: nsteps untimed RK4 steps of both paths
     from the same initial state; relative L2 (mass norm) of the mixed path
-    against the FP64 path per field, and the drift of kinetic energy,
-    potential energy and mass (src/energy.f:3-96) over the run on each path."""
+    against the FP64 path per field; kinetic and potential energy and mass
+    (src/energy.f:3-96) at the start and the end on each path, the drift of
+    the total energy and of the mass over the run on each path, and how far
+    the mixed path's end values are from the FP64 path's."""
     out = {"steps": nsteps}
-    fin = {}
+    fin, ends = {}, {}
     for prec in ("fp32", "fp64"):
         ctx.upload(*state0)
         e0 = integrals(ctx)
         ctx.rk4(variant, dt, nsteps, precision=prec)
         e1 = integrals(ctx)
-        fin[prec] = ctx.download()
-        out[f"drift_{prec}"] = {k: (b - a) / abs(a) if a else b - a
-                                for k, a, b in zip(("ke", "pe", "mass"), e0, e1)}
+        fin[prec], ends[prec] = ctx.download(), e1
+        E0, E1 = e0[0] + e0[1], e1[0] + e1[1]
         out[f"integrals_{prec}"] = {"start": dict(zip(("ke", "pe", "mass"), e0)),
                                     "end": dict(zip(("ke", "pe", "mass"), e1))}
+        out[f"drift_{prec}"] = {"energy": (E1 - E0) / abs(E0) if E0 else E1 - E0,
+                                "mass": (e1[2] - e0[2]) / abs(e0[2]) if e0[2] else e1[2] - e0[2]}
+    out["end_fp32_vs_fp64"] = {k: abs(a - b) / abs(b) if b else abs(a - b)
+                               for k, a, b in zip(("ke", "pe", "mass"), ends["fp32"],
+                                                  ends["fp64"])}
     out["rel_l2"] = {f: mass_rel_l2(t, vect, fin["fp32"][i], fin["fp64"][i])
                      for i, f in enumerate(("u", "v", "eta"))}
     out["eta_le_1e-6"] = out["rel_l2"]["eta"] <= 1e-6
@@ -458,6 +464,12 @@ def cpu_worker(spec):
     print(json.dumps({"n": k, "s": sec}), flush=True)
 
 
+def phase(msg):
+    """Progress marker on stderr (the JSON line alone goes to stdout), so
+    that a run that stops names the step it stopped in."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     if len(sys.argv) == 3 and sys.argv[1] == "--cpu-worker":
         cpu_worker(sys.argv[2])
@@ -534,14 +546,17 @@ def main():
         vect = m["vect"]
         ne = m["ne"]
     setup_s = time.time() - tb
+    phase(f"set up {wl}: {ne} elements in {setup_s:.1f} s")
     variant, dt, nm = c["variant"], c["dt"], t["nm"]
     K = args.steps if args.steps is not None else WORKLOADS[wl]["steps"]
     W = args.warmup
     if args.calib_copy:
         calib_copy(args.calib_copy << 20)
     # mixed-precision path: warm up, then time K steps from the same initial state
+    phase("mixed path: warm-up")
     ctx.upload(*state0)
     prime = warm(ctx, variant, dt, W, K, "fp32")
+    phase(f"mixed path: {K} timed steps")
     ctx.upload(*state0)
     ctx.sync()
     t0 = time.perf_counter()
@@ -550,6 +565,7 @@ def main():
     wall = time.perf_counter() - t0
     s32 = ctx.download()
     # per-kernel times (HIP events around each launch; eager launches)
+    phase("mixed path: per-kernel timing")
     ctx.upload(*state0)
     ctx.set_profiling(True)
     ctx.rk4(variant, dt, min(K, 20), precision="fp32")
@@ -597,6 +613,7 @@ def main():
     # plain FP64 path: same initial state, same warm-up and steps
     if not args.no_fp64:
         K64 = args.fp64_steps or K
+        phase(f"FP64 path: warm-up and {K64} timed steps")
         ctx.upload(*state0)
         warm(ctx, variant, dt, W, K64, "fp64")
         ctx.upload(*state0)
@@ -615,13 +632,18 @@ def main():
             out["rel_l2_vs_fp64"]["steps"] = K
     G = WORKLOADS[wl]["gate"] if args.gate_steps is None else args.gate_steps
     if G and not args.no_fp64:
+        phase(f"accuracy leg: {G} steps on both paths")
         out["accuracy_gate"] = gate_leg(ctx, t, variant, dt, state0, vect, G)
     if args.dropin_steps and wl != "c5":
+        phase("per-call drop-in loop")
         out["dropin"] = dropin_timing(ctx, t, variant, dt, state0, args.dropin_steps)
     if args.adjoint and wl == "c4":
+        phase("adjoint and central differences")
         out["adjoint"] = adjoint_check(ctx, t, c, K, args.adjoint)
+    phase("refinement indicator")
     out["cal_conv"] = adapt_timing(ctx, s32, ne, nc, ng, ngf, cpu=not args.no_cpu)
     if not args.no_cpu:
+        phase("CPU baseline")
         try:
             out["cpu_baseline"] = cpu_baseline(variant, nc, ng, ngf, wl=wl)
         except Exception as ex:  # reported, not fatal

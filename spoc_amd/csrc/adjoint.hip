@@ -130,7 +130,7 @@ __global__ __launch_bounds__(NT) void adj_lift_kernel(F32Args A, int nblk) {
   const bool valid = e < A.mesh.ne;
   const double *gg = A.mesh.geo64 + (size_t)(valid ? e : 0) * GEO;
   const double ax = gg[0], bx = gg[1], ay = gg[2], by = gg[3], di = valid ? gg[4] : 0.0;
-  const double g0 = A.fld.g0, hh0 = valid ? A.fld.hh0e[e] : 0.0;
+  const double g0 = A.fld.g0, hh0 = valid ? core_depth<VAR>(A.fld, e) : 0.0;
   const size_t mb = (size_t)e0 * NM;
   const bool wall = valid && A.mesh.bdl[e] != 0;
   const bool anyw = __ballot(wall) != 0;
@@ -207,7 +207,7 @@ __global__ __launch_bounds__(NT) void adj_lift_kernel(F32Args A, int nblk) {
       if (valid && m < NM) {
         A.st.avol[0][mb + m * 16 + col] = hh0 * bu[s];
         A.st.avol[1][mb + m * 16 + col] = hh0 * bv[s];
-        if (VAR != 0) {  // the same vector is the adjoint of the raw mass flux u (h + hb - hh0)
+        if (VAR != 0) {  // the same vector is the adjoint of the raw mass flux u (h + hb - H0)
           A.st.aUH[0][mb + m * 16 + col] = bu[s];
           A.st.aUH[1][mb + m * 16 + col] = bv[s];
         }
@@ -265,7 +265,7 @@ __global__ __launch_bounds__(NT) void adj_gather_kernel(F32Args A, int nblk) {
   const int l = lane(), g = l >> 4, col = l & 15, e0 = eb * 16, e = e0 + col;
   const bool valid = e < A.mesh.ne;
   const double *gg = A.mesh.geo64 + (size_t)(valid ? e : 0) * GEO;
-  const double hh0 = valid ? A.fld.hh0e[e] : 0.0;
+  const double hh0 = valid ? core_depth<VAR>(A.fld, e) : 0.0;
   const size_t mb = (size_t)e0 * NM;
   const bool wall = valid && A.mesh.bdl[e] != 0;
   const bool anyw = __ballot(wall) != 0;
@@ -373,7 +373,7 @@ __global__ __launch_bounds__(NT) void adj_mid_kernel(F32Args A, int nblk) {
   const bool valid = e < A.mesh.ne;
   const double *gg = A.mesh.geo64 + (size_t)(valid ? e : 0) * GEO;
   const double ax = gg[0], bx = gg[1], ay = gg[2], by = gg[3], di = valid ? gg[4] : 0.0;
-  const double hh0 = valid ? A.fld.hh0e[e] : 0.0;
+  const double hh0 = valid ? core_depth<VAR>(A.fld, e) : 0.0;
   const double mu0 = A.fld.mu0, mb0 = A.fld.mb0;
   const size_t mb = (size_t)e0 * NM;
   const bool wall = valid && A.mesh.bdl[e] != 0;

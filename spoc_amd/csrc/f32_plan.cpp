@@ -390,7 +390,7 @@ int f32_mesh_set(F32Plan *p, int ne, int nf, const int *itm, const int *its, con
   // 64 lanes x 4 floats), then fc3 (ne*3)
   const size_t nq = qf_floats(nepad, ng);
   if (hipMalloc(&p->dfld, (nq + (size_t)ne * 3) * 4 + 256) != hipSuccess) return -2;
-  if (hipMalloc(&p->dhh0e, (size_t)ne * sizeof(double)) != hipSuccess) return -2;
+  if (hipMalloc(&p->dhh0e, 2 * (size_t)ne * sizeof(double)) != hipSuccess) return -2;
   if (hipMemset(p->dfld, 0, (nq + (size_t)ne * 3) * 4 + 256) != hipSuccess) return -2;
   return 0;
 }
@@ -410,6 +410,8 @@ int f32_fields_set(F32Plan *p, int ld, const double *fc, const double *w0, const
   for (int e = 0; e < nepad; e++)
     for (int q = 0; q < 4 * kg; q++) at(e, q, 0) = 1.0f;  // finite hb where nothing is read
   const double *src[3] = {hb, w0, z0};
+  std::vector<double> hcore(ne);  // the nonlinear core's depth (core_depth)
+  for (int e = 0; e < ne; e++) hcore[e] = hb[(size_t)ld * e];
   for (int e = 0; e < ne; e++)
     for (int q = 0; q < ng; q++) {
       for (int f = 0; f < 3; f++) {
@@ -421,9 +423,9 @@ int f32_fields_set(F32Plan *p, int ld, const double *fc, const double *w0, const
         for (int m = 0; m < nm; m++) s += pi2[q + (size_t)ld_g * m] * src[f][m + (size_t)ld * e];
         at(e, q, f) = (float)s;
       }
-      double s = 0.0;  // hb - hh0 (FP32 mass flux)
+      double s = 0.0;  // hb - hcore (FP32 mass flux)
       for (int m = 0; m < nm; m++)
-        s += pi2[q + (size_t)ld_g * m] * (hb[m + (size_t)ld * e] - (m == 0 ? hh0e[e] : 0.0));
+        s += pi2[q + (size_t)ld_g * m] * (hb[m + (size_t)ld * e] - (m == 0 ? hcore[e] : 0.0));
       at(e, q, 3) = (float)s;
     }
   if (fc)
@@ -433,9 +435,11 @@ int f32_fields_set(F32Plan *p, int ld, const double *fc, const double *w0, const
       h[nq + 3 * (size_t)e + 2] = (float)fc[nc + 1 + (size_t)ld * e];
     }
   if (hipMemcpy(p->dfld, h.data(), h.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(p->dhh0e, hh0e, (size_t)ne * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      hipMemcpy(p->dhh0e, hh0e, (size_t)ne * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(p->dhh0e + ne, hcore.data(), (size_t)ne * sizeof(double),
+                hipMemcpyHostToDevice) != hipSuccess)
     return -2;
-  p->fld = F32Fields{p->dfld, p->dfld + nq, p->dhh0e, (float)mu0, (float)mb0, g0};
+  p->fld = F32Fields{p->dfld, p->dfld + nq, p->dhh0e, p->dhh0e + ne, (float)mu0, (float)mb0, g0};
   p->have_fields = true;
   return 0;
 }

