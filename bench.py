@@ -239,6 +239,29 @@ def adapt_timing(ctx, state, ne, nc, ng, ngf, cpu, reps=20, fac=(0.1, 0.01, 1.0)
     return out
 
 
+def renumber(c, nc, how, seed=17):
+    """The case with the caller's element numbering changed: "shuffled"
+    (a seeded random permutation) or "rows" (centroids sorted by row, then
+    by x, as a structured mesh generator numbers them); the mesh operands
+    are rebuilt from the permuted triangles (new face numbering too) and the
+    per-element fields and state permuted with them."""
+    from spoc_amd import mesh as M
+    m = c["mesh"]
+    if how == "shuffled":
+        perm = np.random.default_rng(seed).permutation(m["ne"])
+    else:
+        tri = np.asarray(m["in"]) - 1
+        cx, cy = m["x"][tri].mean(0), m["y"][tri].mean(0)
+        h = np.sqrt((np.ptp(cx) * np.ptp(cy)) / m["ne"] * 2.0)  # lattice spacing
+        perm = np.lexsort((cx, np.floor((cy - cy.min()) / h)))
+    m2 = M.build(m["x"], m["y"], np.asfortranarray(m["in"][:, perm]), nc, 0.0, 0.0)
+    p = {k: (np.asfortranarray(v[:, perm]) if isinstance(v, np.ndarray) and v.ndim == 2 else v)
+         for k, v in c["phys"].items()}
+    out = dict(c)
+    out.update(mesh=m2, phys=p, state=tuple(np.asfortranarray(a[:, perm]) for a in c["state"]))
+    return out
+
+
 def graph_prime_steps(K):
     """Steps of an untimed call that captures the HIP graph a K-step call
     replays (8 steps per graph from K >= 8, one step per graph from K >= 2;
@@ -498,6 +521,11 @@ def main():
     ap.add_argument("--dropin-steps", type=int, default=2,
                     help="steps of the per-call drop-in loop (4 spoc_rhs calls per step on "
                          "host arrays); 0 skips it")
+    ap.add_argument("--numbering", default="generator", choices=("generator", "shuffled", "rows"),
+                    help="the caller's element numbering: the generator's (Z order of the "
+                         "centroids), a random permutation, or row by row")
+    ap.add_argument("--ordering", default="hilbert", choices=("hilbert", "zcurve", "input"),
+                    help="the library's internal element numbering (spoc_set_ordering)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -517,8 +545,11 @@ def main():
     t = spoc_amd.build_tables(nc, *quad_rule(ng, ngf))
     tb = time.time()
     ctx = spoc_amd.Context(t)
+    ctx.set_ordering(args.ordering)
     if wl == "c5":
         members = configs.config5(t, nmem=args.members)
+        if args.numbering != "generator":
+            members = [renumber(mm, nc, args.numbering, 17 + i) for i, mm in enumerate(members)]
         c = members[0]
         ctx.set_ensemble_mesh([mm["mesh"] for mm in members])
         P = [mm["phys"] for mm in members]
@@ -539,6 +570,8 @@ def main():
             c = configs.config3(t)
         else:
             c = configs.config4(t)
+        if args.numbering != "generator":
+            c = renumber(c, nc, args.numbering)
         m, p = c["mesh"], c["phys"]
         ctx.set_mesh(m)
         ctx.set_fields(p["hb"], p["g0"], p["fc"], p["w0"], p["z0"], p["mu0"], p["mb0"])
@@ -580,7 +613,7 @@ def main():
         "vs_baseline": None, "dtype": "fp32" if variant != "lin" else "f64",
         "data": "synthetic",
         "config": {"workload": WORKLOADS[wl]["name"](nc, ng, ngf, ne, dt, args),
-                   "ne": ne, "nm": nm,
+                   "ne": ne, "nm": nm, "numbering": args.numbering, "ordering": args.ordering,
                    "precision": ("fp32-mfma quadrature, fp64-mfma linear gravity-wave core, "
                                  "fp64 state/RK4") if variant != "lin" else
                                 "fp64-mfma (the linear variant is all linear core)"},

@@ -136,6 +136,34 @@ int spoc_ensemble_mesh_set(spoc_ctx *ctx, int nmem, const int *ne_m, const int *
 int spoc_ensemble_energy(spoc_ctx *ctx, int nmem, double *ke, double *pe, double *mass);
 
 /*
+ * Element numbering inside the library (no reference counterpart).  The
+ * kernels work on blocks of 16 consecutive elements and gather neighbours'
+ * traces, so they are fastest when consecutive elements are close in space;
+ * the caller's numbering need not be (src/cmal_s.f:164-208 numbers as the
+ * mesh file lists triangles, and remeshing reorders).  SPOC_ORDER_HILBERT
+ * (the default) and SPOC_ORDER_ZCURVE renumber elements along that
+ * space-filling curve of their centroids inside each ensemble member
+ * (centroids reconstructed from vect and the face connectivity; no node
+ * arrays needed) and faces in order of first appearance; SPOC_ORDER_INPUT
+ * keeps the caller's numbering.  Every array crossing the C-ABI stays in
+ * the caller's numbering, every result is the same bit for bit, and
+ * energies are summed in the caller's element order.  Applies from the next
+ * spoc_mesh_set / spoc_ensemble_mesh_set.
+ */
+enum { SPOC_ORDER_INPUT = 0, SPOC_ORDER_HILBERT = 1, SPOC_ORDER_ZCURVE = 2 };
+int spoc_set_ordering(spoc_ctx *ctx, int mode);
+
+/*
+ * The internal numbering that mode (SPOC_ORDER_HILBERT or SPOC_ORDER_ZCURVE)
+ * gives a mesh (host side, no device work): arrays as for
+ * spoc_ensemble_mesh_set (nmem = 1 for one mesh); perm (sum ne_m) receives,
+ * for each internal position, the caller's element (1-based, members'
+ * numbers shifted as in the union mesh).
+ */
+int spoc_mesh_order(int mode, int nmem, const int *ne_m, const int *nf_m, const int *itm,
+                    const int *tseg, const int *ssit, const double *vect, int *perm);
+
+/*
  * Per-run physics operands of cal_var: fc (Coriolis, modal), w0/z0 (wind
  * stress), hb (resting depth), all (ld, ne); mu0, mb0, g0 as in oc.inp.
  * fc/w0/z0 may be NULL for SPOC_LIN (unused by src/calvar_lin.f).

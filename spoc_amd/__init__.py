@@ -60,7 +60,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                  "spoc_set_checkpoint", "spoc_checkconv", "spoc_cal_conv", "spoc_mesh_build",
                  "spoc_mesh_changemaille", "spoc_state_file_read", "spoc_state_file_write",
                  "spoc_mesh_file_size", "spoc_mesh_file_read", "spoc_mesh_file_write",
-                 "spoc_interpol", "spoc_calh2", "spoc_set_flux"):
+                 "spoc_interpol", "spoc_calh2", "spoc_set_flux", "spoc_set_ordering",
+                 "spoc_mesh_order"):
         getattr(lib, name).restype = C.c_int
     lib.spoc_ctx_destroy.argtypes = [C.c_void_p]
     _lib = lib
@@ -201,6 +202,26 @@ def calh2(tables: dict, ff, x, y, tri, tinter=None, h0=None, ld: int = LDM):
                           t["ldg"], _d(t["amm"]), LDM, cb, None, len(x), ne, _d(x), _d(y),
                           _i(tri), _i(tinter), ne0, ld, h0p, _d(h1)))
     return h1
+
+
+ORDERINGS = {"input": 0, "hilbert": 1, "zcurve": 2}
+
+
+def mesh_order(meshes, mode: str = "hilbert") -> np.ndarray:
+    """The library's internal element order of one mesh dict or a list of
+    member meshes (spoc_mesh_order): perm[i] = the caller's element (0-based,
+    union numbering) at internal position i."""
+    lib = load_library()
+    meshes = meshes if isinstance(meshes, (list, tuple)) else [meshes]
+    ne = np.array([int(m["itm"].shape[1]) for m in meshes], np.int32)
+    nf = np.array([int(m["nf"]) for m in meshes], np.int32)
+    cat = lambda k, dt: np.ascontiguousarray(
+        np.concatenate([np.asfortranarray(m[k], dt).ravel(order="F") for m in meshes]))
+    perm = np.zeros(int(ne.sum()), np.int32)
+    _check(lib.spoc_mesh_order(ORDERINGS[mode], len(meshes), _i(ne), _i(nf), _i(cat("itm", np.int32)),
+                               _i(cat("tseg", np.int32)), _i(cat("ssit", np.int32)),
+                               _d(cat("vect", np.float64)), _i(perm)))
+    return perm - 1
 
 
 class Context:
@@ -424,6 +445,13 @@ class Context:
     def set_tiles(self, variant, tile0=0, tile1=0, autotune=True) -> None:
         var = VARIANTS[variant] if isinstance(variant, str) else variant
         _check(self.lib.spoc_set_tiles(self._h, var, tile0, tile1, int(autotune)))
+
+    def set_ordering(self, mode: str = "hilbert") -> None:
+        """Internal element numbering from the next set_mesh: along the
+        Hilbert (default) or Z-order curve of the centroids, or the caller's
+        own ("input").  Results and every array crossing the C-ABI are the
+        same either way."""
+        _check(self.lib.spoc_set_ordering(self._h, ORDERINGS[mode]))
 
     def set_graph(self, on: bool) -> None:
         """Replay the RK4 step from a HIP graph (default on)."""

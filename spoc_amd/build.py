@@ -42,6 +42,7 @@ def _units():
     units.append(("mesh.cpp", "mesh.o", EXACT))
     units.append(("files.cpp", "files.o", []))
     units.append(("interp.cpp", "interp.o", EXACT))
+    units.append(("order.cpp", "order.o", []))
     units.append(("abi.cpp", "abi.o", EXACT + ["-x", "hip"]))
     return units
 

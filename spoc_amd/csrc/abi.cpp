@@ -27,10 +27,15 @@ int adapt_launch(hipStream_t st, int nc, int ngf, int ne, int nmem, int nch,
                  int *flags);
 int adapt_chunk();
 int nonfinite_launch(hipStream_t st, size_t n, const double *a, int *flag);
+void locality_order(int ne, int nf, const int *itm, const int *tseg, const int *ssit,
+                    const double *vect, const std::vector<int> &mem_e0, bool use_hilbert,
+                    std::vector<int> &perm);
+void face_order(int ne, int nf, const int *itm, const std::vector<int> &perm,
+                std::vector<int> &fperm);
 int energy_launch(hipStream_t st, int ne, int nm, int ng, const double *u, const double *v,
                   const double *h, const double *hb, const double *pi2, const double *wi,
                   const double *ammo1, const double *deta, double g0, int nr, const int *off,
-                  double *scratch, double *out);
+                  const int *order, double *scratch, double *out);
 struct F32Plan;
 int f32_plan_create(F32Plan **out, int nc, int ng, int ngf, const double *amm, const double *amml,
                     const double *agx, const double *agy, const double *pi2, const double *pico,
@@ -223,6 +228,12 @@ struct spoc_ctx {
   int use_graph = 1;    // spoc_set_graph
   QueueBound queue;     // dispatches queued ahead of the host in spoc_rk4_steps
   int *dflag = nullptr; // device word of the non-finite test
+  // internal element numbering (spoc_set_ordering): perm[internal] = the
+  // caller's element, iperm its inverse; both empty when they coincide.
+  // dorder: device copy of iperm (energy sums run in the caller's order)
+  int ordering = SPOC_ORDER_HILBERT;
+  std::vector<int> perm, iperm;
+  int *dorder = nullptr;
 };
 
 namespace {
@@ -269,6 +280,10 @@ void free_mesh(spoc_ctx *c) {
   c->dchunk = nullptr;
   if (c->dside) (void)hipFree(c->dside);
   c->dside = nullptr;
+  if (c->dorder) (void)hipFree(c->dorder);
+  c->dorder = nullptr;
+  c->perm.clear();
+  c->iperm.clear();
   if (c->dfields) (void)hipFree(c->dfields);
   if (c->dFptr) (void)hipFree(c->dFptr);
   c->dFptr = nullptr;
@@ -532,6 +547,66 @@ static int mesh_set(spoc_ctx *c, int ne, int nf, const int *itm, const int *its,
   (void)hipStreamSynchronize(c->stream);
   free_mesh(c);
   int nm = c->nm, ngf = c->ngf;
+  // the internal numbering: elements along a space-filling curve of their
+  // centroids inside each member, faces in order of first appearance; every
+  // operand is renumbered here, and upload, download, energies and the
+  // refinement indicator map back to the caller's numbering
+  std::vector<int> pitm, pits, ptseg, pssit, pbdl, pin;
+  std::vector<double> pnorm, pvect, pnormt;
+  if (c->ordering != SPOC_ORDER_INPUT) {
+    std::vector<int> perm, fperm;
+    locality_order(ne, nf, itm, tseg, ssit, vect, mem_e0, c->ordering == SPOC_ORDER_HILBERT,
+                   perm);
+    face_order(ne, nf, itm, perm, fperm);
+    bool same = true;
+    for (int e = 0; e < ne && same; e++) same = perm[e] == e;
+    for (int f = 0; f < nf && same; f++) same = fperm[f] == f;
+    if (!same) {
+      std::vector<int> iperm(ne), ifperm(nf);
+      for (int e = 0; e < ne; e++) iperm[perm[e]] = e;
+      for (int f = 0; f < nf; f++) ifperm[fperm[f]] = f;
+      pitm.resize(3 * (size_t)ne);
+      pits.resize(3 * (size_t)ne);
+      pvect.resize(5 * (size_t)ne);
+      pnormt.resize(2 * (size_t)ne);
+      pbdl.resize(ne);
+      if (in) pin.resize(3 * (size_t)ne);
+      for (int en = 0; en < ne; en++) {
+        const int e = perm[en];
+        for (int k = 0; k < 3; k++) {
+          pitm[3 * en + k] = ifperm[itm[3 * e + k] - 1] + 1;
+          pits[3 * en + k] = its[3 * e + k];
+          if (in) pin[3 * en + k] = in[3 * e + k];
+        }
+        for (int i = 0; i < 5; i++) pvect[5 * en + i] = vect[5 * e + i];
+        for (int i = 0; i < 2; i++) pnormt[2 * en + i] = normt[2 * e + i];
+        pbdl[en] = bdl[e];
+      }
+      ptseg.resize(2 * (size_t)nf);
+      pssit.resize(2 * (size_t)nf);
+      pnorm.resize(2 * (size_t)nf);
+      for (int fn = 0; fn < nf; fn++) {
+        const int f = fperm[fn];
+        for (int i = 0; i < 2; i++) {  // sides keep their order (the face sums do)
+          const int t = tseg[2 * f + i];
+          ptseg[2 * fn + i] = t == 0 ? 0 : iperm[t - 1] + 1;
+          pssit[2 * fn + i] = ssit[2 * f + i];
+          pnorm[2 * fn + i] = norm[2 * f + i];
+        }
+      }
+      itm = pitm.data();
+      its = pits.data();
+      tseg = ptseg.data();
+      ssit = pssit.data();
+      norm = pnorm.data();
+      vect = pvect.data();
+      normt = pnormt.data();
+      bdl = pbdl.data();
+      if (in) in = pin.data();
+      c->perm = std::move(perm);
+      c->iperm = std::move(iperm);
+    }
+  }
   std::vector<double> deta(ne, 0.0);
   c->have_nodes = (in && xgr && ygr && nn > 0);
   if (c->have_nodes) {
@@ -609,6 +684,10 @@ static int mesh_set(spoc_ctx *c, int ne, int nf, const int *itm, const int *its,
   HIPCHK(hipMalloc((void **)&c->dmem, sizeof(int) * off.size()));
   HIPCHK(hipMemcpy(c->dmem, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
   c->mem_e0 = mem_e0;
+  if (!c->iperm.empty()) {
+    HIPCHK(hipMalloc((void **)&c->dorder, sizeof(int) * ne));
+    HIPCHK(hipMemcpy(c->dorder, c->iperm.data(), sizeof(int) * ne, hipMemcpyHostToDevice));
+  }
   HIPCHK(hipMalloc((void **)&c->dFptr, sizeof(double *) * 4));
   double *fp[4];
   for (int i = 0; i < 4; i++) fp[i] = state_ptr(c, 12 + i);
@@ -667,6 +746,11 @@ int spoc_ensemble_mesh_set(spoc_ctx *c, int nmem, const int *ne_m, const int *nf
                   nodes ? ygr : nullptr, e0);
 }
 
+// The caller's element of internal element e.
+static inline size_t caller_elem(const spoc_ctx *c, int e) {
+  return c->perm.empty() ? (size_t)e : (size_t)c->perm[e];
+}
+
 static int upload_fields(spoc_ctx *c, int ld, const double *const *src, double *const *dst,
                          int n) {
   int nm = c->nm, ne = c->ne;
@@ -674,7 +758,8 @@ static int upload_fields(spoc_ctx *c, int ld, const double *const *src, double *
   for (int f = 0; f < n; f++) {
     if (!src[f]) continue;
     for (int e = 0; e < ne; e++)
-      std::memcpy(pack.data() + (size_t)e * nm, src[f] + (size_t)ld * e, sizeof(double) * nm);
+      std::memcpy(pack.data() + (size_t)e * nm, src[f] + (size_t)ld * caller_elem(c, e),
+                  sizeof(double) * nm);
     HIPCHK(hipMemcpyAsync(dst[f], pack.data(), sizeof(double) * nm * ne, hipMemcpyHostToDevice,
                           c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -690,7 +775,8 @@ static int download_fields(spoc_ctx *c, int ld, double *const *dst, double *cons
                           c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     for (int e = 0; e < ne; e++)
-      std::memcpy(dst[f] + (size_t)ld * e, pack.data() + (size_t)e * nm, sizeof(double) * nm);
+      std::memcpy(dst[f] + (size_t)ld * caller_elem(c, e), pack.data() + (size_t)e * nm,
+                  sizeof(double) * nm);
   }
   return SPOC_OK;
 }
@@ -714,8 +800,25 @@ int spoc_fields_set(spoc_ctx *c, int ld, const double *fc, const double *w0, con
       hh0e[e] = hb[(size_t)ld * c->mem_e0[m]];
   HIPCHK(hipMemcpy(c->dfields + 4 * n, hh0e.data(), sizeof(double) * c->ne, hipMemcpyHostToDevice));
   c->phys = Physics{mu0, mb0, g0, hb[0], c->dfields + 4 * n};
-  if (f32_fields_set(c->f32, ld, fc, w0, z0, hb, hh0e.data(), c->h_pi2.data(), c->ng, mu0, mb0,
-                     g0))
+  // the mixed path's set-up reads host arrays element by element: hand it
+  // the fields in the internal numbering
+  std::vector<double> internal;
+  const double *f4[4] = {fc, w0, z0, hb};
+  int ld4 = ld;
+  if (!c->perm.empty()) {
+    internal.assign(4 * n, 0.0);
+    for (int f = 0; f < 4; f++) {
+      if (!f4[f]) continue;
+      double *d = internal.data() + f * n;
+      for (int e = 0; e < c->ne; e++)
+        std::memcpy(d + (size_t)c->nm * e, f4[f] + (size_t)ld * caller_elem(c, e),
+                    sizeof(double) * c->nm);
+      f4[f] = d;
+    }
+    ld4 = c->nm;
+  }
+  if (f32_fields_set(c->f32, ld4, f4[0], f4[1], f4[2], f4[3], hh0e.data(), c->h_pi2.data(),
+                     c->ng, mu0, mb0, g0))
     return fail(SPOC_EHIP, "FP32 field upload failed");
   c->have_fields = true;
   return SPOC_OK;
@@ -1117,6 +1220,15 @@ int spoc_set_flux(spoc_ctx *c, double alpha) {
   return SPOC_OK;
 }
 
+int spoc_set_ordering(spoc_ctx *c, int mode) {
+  if (!c) return fail(SPOC_EINVAL, "null context");
+  if (mode != SPOC_ORDER_INPUT && mode != SPOC_ORDER_HILBERT && mode != SPOC_ORDER_ZCURVE)
+    return fail(SPOC_EINVAL, "ordering mode %d (SPOC_ORDER_INPUT 0, SPOC_ORDER_HILBERT 1, "
+                             "SPOC_ORDER_ZCURVE 2)", mode);
+  c->ordering = mode;
+  return SPOC_OK;
+}
+
 int spoc_set_graph(spoc_ctx *c, int on) {
   if (!c) return fail(SPOC_EINVAL, "null context");
   c->use_graph = on != 0;
@@ -1137,7 +1249,8 @@ static int energies(spoc_ctx *c, int nr, const int *off, double *r) {
   double *out = c->den + (size_t)c->ne * (2 + c->nm);
   int rc = energy_launch(c->stream, c->ne, c->nm, c->ng, state_ptr(c, 0), state_ptr(c, 1),
                          state_ptr(c, 2), c->dfields + 3 * n, c->tab.pi2, c->tab.wi,
-                         c->tab.ammo1, c->mesh.deta, c->phys.g0, nr, off, c->den, out);
+                         c->tab.ammo1, c->mesh.deta, c->phys.g0, nr, off, c->dorder, c->den,
+                         out);
   if (rc) return fail(SPOC_EHIP, "energy kernel launch failed");
   HIPCHK(hipMemcpyAsync(r, out, sizeof(double) * 3 * nr, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1199,8 +1312,11 @@ int spoc_checkconv(spoc_ctx *c, int field, double *amax) {
   int rc = adapt_run(c, false, 0.0, 0.0);
   if (rc) return rc;
   const double *d = adapt_bufs(c).amax + (size_t)field * c->ne;
-  HIPCHK(hipMemcpyAsync(amax, d, sizeof(double) * c->ne, hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> tmp(c->perm.empty() ? 0 : c->ne);
+  double *h = c->perm.empty() ? amax : tmp.data();
+  HIPCHK(hipMemcpyAsync(h, d, sizeof(double) * c->ne, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  for (size_t e = 0; e < tmp.size(); e++) amax[c->perm[e]] = tmp[e];
   return SPOC_OK;
 }
 
@@ -1216,15 +1332,26 @@ int spoc_cal_conv(spoc_ctx *c, double fac1, double fac2, double fac3, int *fconv
   const size_t ne = c->ne;
   AdaptBufs b = adapt_bufs(c);
   std::vector<double> r(2 * (size_t)nmem);
+  const bool perm = !c->perm.empty();
+  std::vector<int> fl(perm ? 2 * ne : 0);
+  std::vector<double> am(perm && amax3 ? 3 * ne : 0);
+  int *hf0 = perm ? fl.data() : fconv, *hf1 = perm ? fl.data() + ne : fderaff;
+  double *ha = perm ? am.data() : amax3;
   HIPCHK(hipMemcpyAsync(r.data(), b.res, sizeof(double) * r.size(), hipMemcpyDeviceToHost,
                         c->stream));
-  HIPCHK(hipMemcpyAsync(fconv, b.flags, sizeof(int) * ne, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipMemcpyAsync(fderaff, b.flags + ne, sizeof(int) * ne, hipMemcpyDeviceToHost,
-                        c->stream));
+  HIPCHK(hipMemcpyAsync(hf0, b.flags, sizeof(int) * ne, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(hf1, b.flags + ne, sizeof(int) * ne, hipMemcpyDeviceToHost, c->stream));
   if (amax3)
-    HIPCHK(hipMemcpyAsync(amax3, b.amax, sizeof(double) * 3 * ne, hipMemcpyDeviceToHost,
-                          c->stream));
+    HIPCHK(hipMemcpyAsync(ha, b.amax, sizeof(double) * 3 * ne, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  if (perm)  // back to the caller's numbering
+    for (size_t e = 0; e < ne; e++) {
+      const size_t o = c->perm[e];
+      fconv[o] = fl[e];
+      fderaff[o] = fl[ne + e];
+      if (amax3)
+        for (int f = 0; f < 3; f++) amax3[f * ne + o] = am[f * ne + e];
+    }
   for (int m = 0; m < nmem; m++) {  // src/adapt.f:1180-1189
     const double sup = r[m];
     deraff[m] = r[nmem + m] > 0.0;

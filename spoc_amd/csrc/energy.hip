@@ -41,21 +41,27 @@ __global__ void energy_terms(int ne, int nm, int ng, const double *__restrict__ 
   pe[e] = ap * d * 0.5 * g0;
 }
 
-// One thread per (range, quantity): elements [off[r], off[r+1]) in order.
+// One thread per (range, quantity): elements [off[r], off[r+1]) of the
+// caller's numbering, in that order; order[e] (may be null: the identity)
+// is where the caller's element e sits in the arrays.
 __global__ void energy_sum(int nr, const int *__restrict__ off, int nm,
-                           const double *__restrict__ ke, const double *__restrict__ pe,
-                           const double *__restrict__ mass, double *__restrict__ out) {
+                           const int *__restrict__ order, const double *__restrict__ ke,
+                           const double *__restrict__ pe, const double *__restrict__ mass,
+                           double *__restrict__ out) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 3 * nr) return;
   int r = t / 3, w = t % 3;
   size_t a = off[r], b = off[r + 1];
   double s = 0.0;
-  if (w == 0)
-    for (size_t e = a; e < b; e++) s = s + ke[e];
-  else if (w == 1)
-    for (size_t e = a; e < b; e++) s = s + pe[e];
-  else
-    for (size_t i = a * nm; i < b * nm; i++) s = s + mass[i];
+  for (size_t e = a; e < b; e++) {
+    const size_t i = order ? (size_t)order[e] : e;
+    if (w == 0)
+      s = s + ke[i];
+    else if (w == 1)
+      s = s + pe[i];
+    else
+      for (int m = 0; m < nm; m++) s = s + mass[i * nm + m];
+  }
   out[t] = s;
 }
 
@@ -63,11 +69,11 @@ __global__ void energy_sum(int nr, const int *__restrict__ off, int nm,
 int energy_launch(hipStream_t st, int ne, int nm, int ng, const double *u, const double *v,
                   const double *h, const double *hb, const double *pi2, const double *wi,
                   const double *ammo1, const double *deta, double g0, int nr, const int *off,
-                  double *scratch, double *out) {
+                  const int *order, double *scratch, double *out) {
   double *ke = scratch, *pe = scratch + ne, *mass = scratch + 2 * (size_t)ne;
   energy_terms<<<(ne + 127) / 128, 128, 0, st>>>(ne, nm, ng, u, v, h, hb, pi2, wi, ammo1, deta,
                                                  g0, ke, pe, mass);
-  energy_sum<<<(3 * nr + 63) / 64, 64, 0, st>>>(nr, off, nm, ke, pe, mass, out);
+  energy_sum<<<(3 * nr + 63) / 64, 64, 0, st>>>(nr, off, nm, order, ke, pe, mass, out);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
