@@ -524,7 +524,7 @@ def main():
     ap.add_argument("--numbering", default="generator", choices=("generator", "shuffled", "rows"),
                     help="the caller's element numbering: the generator's (Z order of the "
                          "centroids), a random permutation, or row by row")
-    ap.add_argument("--ordering", default="hilbert", choices=("hilbert", "zcurve", "input"),
+    ap.add_argument("--ordering", default="auto", choices=("auto", "hilbert", "zcurve", "input"),
                     help="the library's internal element numbering (spoc_set_ordering)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")

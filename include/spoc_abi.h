@@ -141,16 +141,18 @@ int spoc_ensemble_energy(spoc_ctx *ctx, int nmem, double *ke, double *pe, double
  * traces, so they are fastest when consecutive elements are close in space;
  * the caller's numbering need not be (src/cmal_s.f:164-208 numbers as the
  * mesh file lists triangles, and remeshing reorders).  SPOC_ORDER_HILBERT
- * (the default) and SPOC_ORDER_ZCURVE renumber elements along that
- * space-filling curve of their centroids inside each ensemble member
- * (centroids reconstructed from vect and the face connectivity; no node
- * arrays needed) and faces in order of first appearance; SPOC_ORDER_INPUT
- * keeps the caller's numbering.  Every array crossing the C-ABI stays in
- * the caller's numbering, every result is the same bit for bit, and
- * energies are summed in the caller's element order.  Applies from the next
- * spoc_mesh_set / spoc_ensemble_mesh_set.
+ * and SPOC_ORDER_ZCURVE renumber elements along that space-filling curve of
+ * their centroids inside each ensemble member (centroids reconstructed from
+ * vect and the face connectivity; no node arrays needed) and faces in order
+ * of first appearance; SPOC_ORDER_INPUT keeps the caller's numbering;
+ * SPOC_ORDER_AUTO (the default) takes the Hilbert numbering unless the
+ * caller's already puts at least as many face neighbours in the same block
+ * of 16 elements.  Every array crossing the C-ABI stays in the caller's
+ * numbering, every result is the same bit for bit, and energies are summed
+ * in the caller's element order.  Applies from the next spoc_mesh_set /
+ * spoc_ensemble_mesh_set.
  */
-enum { SPOC_ORDER_INPUT = 0, SPOC_ORDER_HILBERT = 1, SPOC_ORDER_ZCURVE = 2 };
+enum { SPOC_ORDER_INPUT = 0, SPOC_ORDER_HILBERT = 1, SPOC_ORDER_ZCURVE = 2, SPOC_ORDER_AUTO = 3 };
 int spoc_set_ordering(spoc_ctx *ctx, int mode);
 
 /*

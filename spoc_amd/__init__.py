@@ -204,7 +204,7 @@ def calh2(tables: dict, ff, x, y, tri, tinter=None, h0=None, ld: int = LDM):
     return h1
 
 
-ORDERINGS = {"input": 0, "hilbert": 1, "zcurve": 2}
+ORDERINGS = {"input": 0, "hilbert": 1, "zcurve": 2, "auto": 3}
 
 
 def mesh_order(meshes, mode: str = "hilbert") -> np.ndarray:
@@ -446,11 +446,12 @@ class Context:
         var = VARIANTS[variant] if isinstance(variant, str) else variant
         _check(self.lib.spoc_set_tiles(self._h, var, tile0, tile1, int(autotune)))
 
-    def set_ordering(self, mode: str = "hilbert") -> None:
+    def set_ordering(self, mode: str = "auto") -> None:
         """Internal element numbering from the next set_mesh: along the
-        Hilbert (default) or Z-order curve of the centroids, or the caller's
-        own ("input").  Results and every array crossing the C-ABI are the
-        same either way."""
+        Hilbert or Z-order curve of the centroids, the caller's own
+        ("input"), or ("auto", the default) Hilbert unless the caller's keeps
+        at least as many neighbours in a block of 16.  Results and every
+        array crossing the C-ABI are the same either way."""
         _check(self.lib.spoc_set_ordering(self._h, ORDERINGS[mode]))
 
     def set_graph(self, on: bool) -> None:

@@ -6,11 +6,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "common.h"
@@ -32,6 +34,7 @@ void locality_order(int ne, int nf, const int *itm, const int *tseg, const int *
                     std::vector<int> &perm);
 void face_order(int ne, int nf, const int *itm, const std::vector<int> &perm,
                 std::vector<int> &fperm);
+double block_share(int nf, const int *tseg, const int *inv);
 int energy_launch(hipStream_t st, int ne, int nm, int ng, const double *u, const double *v,
                   const double *h, const double *hb, const double *pi2, const double *wi,
                   const double *ammo1, const double *deta, double g0, int nr, const int *off,
@@ -231,7 +234,12 @@ struct spoc_ctx {
   // internal element numbering (spoc_set_ordering): perm[internal] = the
   // caller's element, iperm its inverse; both empty when they coincide.
   // dorder: device copy of iperm (energy sums run in the caller's order)
-  int ordering = SPOC_ORDER_HILBERT;
+  int ordering = SPOC_ORDER_AUTO;
+  // pinned host staging of the host <-> device copies: room for 4 fields of
+  // (nm, ne); one completion event per field
+  double *stage = nullptr;
+  size_t stage_cap = 0;
+  hipEvent_t fev[4] = {};
   std::vector<int> perm, iperm;
   int *dorder = nullptr;
 };
@@ -282,6 +290,9 @@ void free_mesh(spoc_ctx *c) {
   c->dside = nullptr;
   if (c->dorder) (void)hipFree(c->dorder);
   c->dorder = nullptr;
+  if (c->stage) (void)hipHostFree(c->stage);
+  c->stage = nullptr;
+  c->stage_cap = 0;
   c->perm.clear();
   c->iperm.clear();
   if (c->dfields) (void)hipFree(c->dfields);
@@ -318,6 +329,27 @@ int f32_fail(int rc) {
   if (rc == -3) return fail(SPOC_ESTATE, "FP32 path: mesh and fields must be set first");
   hipError_t e = hipGetLastError();
   return fail(SPOC_EHIP, "FP32 kernel launch failed: %s", hipGetErrorString(e));
+}
+
+// f(e0, e1) over [0, ne) split among up to 8 host threads (the repacking
+// between the caller's (ld, ne) arrays and the packed (nm, ne) layout is
+// bound by host memory bandwidth, which one thread does not fill).
+template <class F>
+void host_parallel(int ne, F f) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int nt = (int)std::min<unsigned>(8u, hw ? hw : 1u);
+  const int per = (ne + nt - 1) / nt;
+  if (nt <= 1 || ne < 4096) {
+    f(0, ne);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; t++) {
+    const int a = t * per, b = std::min(ne, a + per);
+    if (a < b) th.emplace_back([=, &f] { f(a, b); });
+  }
+  f(0, std::min(ne, per));
+  for (auto &t : th) t.join();
 }
 
 }  // namespace
@@ -513,6 +545,9 @@ int spoc_ctx_destroy(spoc_ctx *c) {
   f32_plan_destroy(c->f32);
   if (c->dtab) (void)hipFree(c->dtab);
   if (c->dflag) (void)hipFree(c->dflag);
+  if (c->stage) (void)hipHostFree(c->stage);
+  for (auto e : c->fev)
+    if (e) (void)hipEventDestroy(e);
   for (auto e : c->prof.ev) (void)hipEventDestroy(e);
   if (c->t0) (void)hipEventDestroy(c->t0);
   if (c->t1) (void)hipEventDestroy(c->t1);
@@ -555,15 +590,21 @@ static int mesh_set(spoc_ctx *c, int ne, int nf, const int *itm, const int *its,
   std::vector<double> pnorm, pvect, pnormt;
   if (c->ordering != SPOC_ORDER_INPUT) {
     std::vector<int> perm, fperm;
-    locality_order(ne, nf, itm, tseg, ssit, vect, mem_e0, c->ordering == SPOC_ORDER_HILBERT,
+    locality_order(ne, nf, itm, tseg, ssit, vect, mem_e0, c->ordering != SPOC_ORDER_ZCURVE,
                    perm);
     face_order(ne, nf, itm, perm, fperm);
     bool same = true;
     for (int e = 0; e < ne && same; e++) same = perm[e] == e;
     for (int f = 0; f < nf && same; f++) same = fperm[f] == f;
+    std::vector<int> iperm(ne);
+    for (int e = 0; e < ne; e++) iperm[perm[e]] = e;
+    // automatic: keep the caller's numbering when it already puts at least
+    // as many face neighbours in one block of 16 as the curve does
+    if (!same && c->ordering == SPOC_ORDER_AUTO &&
+        block_share(nf, tseg, nullptr) >= block_share(nf, tseg, iperm.data()))
+      same = true;
     if (!same) {
-      std::vector<int> iperm(ne), ifperm(nf);
-      for (int e = 0; e < ne; e++) iperm[perm[e]] = e;
+      std::vector<int> ifperm(nf);
       for (int f = 0; f < nf; f++) ifperm[fperm[f]] = f;
       pitm.resize(3 * (size_t)ne);
       pits.resize(3 * (size_t)ne);
@@ -751,32 +792,67 @@ static inline size_t caller_elem(const spoc_ctx *c, int e) {
   return c->perm.empty() ? (size_t)e : (size_t)c->perm[e];
 }
 
-static int upload_fields(spoc_ctx *c, int ld, const double *const *src, double *const *dst,
-                         int n) {
-  int nm = c->nm, ne = c->ne;
-  std::vector<double> pack((size_t)nm * ne);
-  for (int f = 0; f < n; f++) {
-    if (!src[f]) continue;
-    for (int e = 0; e < ne; e++)
-      std::memcpy(pack.data() + (size_t)e * nm, src[f] + (size_t)ld * caller_elem(c, e),
-                  sizeof(double) * nm);
-    HIPCHK(hipMemcpyAsync(dst[f], pack.data(), sizeof(double) * nm * ne, hipMemcpyHostToDevice,
-                          c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+// Pinned staging for n fields of (nm, ne) doubles, and the per-field events.
+static int ensure_stage(spoc_ctx *c, int n) {
+  const size_t want = (size_t)n * c->nm * c->ne;
+  if (c->stage_cap < want) {
+    if (c->stage) (void)hipHostFree(c->stage);
+    c->stage = nullptr;
+    c->stage_cap = 0;
+    const size_t cap = std::max(want, (size_t)4 * c->nm * c->ne);
+    HIPCHK(hipHostMalloc((void **)&c->stage, cap * sizeof(double), hipHostMallocDefault));
+    c->stage_cap = cap;
   }
+  for (auto &e : c->fev)
+    if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   return SPOC_OK;
 }
 
-static int download_fields(spoc_ctx *c, int ld, double *const *dst, double *const *src, int n) {
-  int nm = c->nm, ne = c->ne;
-  std::vector<double> pack((size_t)nm * ne);
+// Host (ld, ne) arrays in the caller's numbering -> device (nm, ne) arrays in
+// the internal numbering: each field is packed into pinned staging by the
+// host threads and copied while the next one is packed.
+static int upload_fields(spoc_ctx *c, int ld, const double *const *src, double *const *dst,
+                         int n) {
+  const int nm = c->nm, ne = c->ne;
+  const size_t fsz = (size_t)nm * ne;
+  int rc = ensure_stage(c, n);
+  if (rc) return rc;
+  // the staging may still feed an earlier call's copies
+  HIPCHK(hipStreamSynchronize(c->stream));
   for (int f = 0; f < n; f++) {
-    HIPCHK(hipMemcpyAsync(pack.data(), src[f], sizeof(double) * nm * ne, hipMemcpyDeviceToHost,
-                          c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    for (int e = 0; e < ne; e++)
-      std::memcpy(dst[f] + (size_t)ld * caller_elem(c, e), pack.data() + (size_t)e * nm,
-                  sizeof(double) * nm);
+    if (!src[f]) continue;
+    double *st = c->stage + f * fsz;
+    const double *s = src[f];
+    host_parallel(ne, [&](int e0, int e1) {
+      for (int e = e0; e < e1; e++)
+        std::memcpy(st + (size_t)e * nm, s + (size_t)ld * caller_elem(c, e), sizeof(double) * nm);
+    });
+    HIPCHK(hipMemcpyAsync(dst[f], st, sizeof(double) * fsz, hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SPOC_OK;
+}
+
+// Device (nm, ne) arrays -> host (ld, ne) arrays: every field's copy is
+// queued at once; each is unpacked as soon as it has arrived.
+static int download_fields(spoc_ctx *c, int ld, double *const *dst, double *const *src, int n) {
+  const int nm = c->nm, ne = c->ne;
+  const size_t fsz = (size_t)nm * ne;
+  int rc = ensure_stage(c, n);
+  if (rc) return rc;
+  for (int f = 0; f < n; f++) {
+    HIPCHK(hipMemcpyAsync(c->stage + f * fsz, src[f], sizeof(double) * fsz,
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipEventRecord(c->fev[f], c->stream));
+  }
+  for (int f = 0; f < n; f++) {
+    HIPCHK(hipEventSynchronize(c->fev[f]));
+    const double *st = c->stage + f * fsz;
+    double *d = dst[f];
+    host_parallel(ne, [&](int e0, int e1) {
+      for (int e = e0; e < e1; e++)
+        std::memcpy(d + (size_t)ld * caller_elem(c, e), st + (size_t)e * nm, sizeof(double) * nm);
+    });
   }
   return SPOC_OK;
 }
@@ -1222,9 +1298,10 @@ int spoc_set_flux(spoc_ctx *c, double alpha) {
 
 int spoc_set_ordering(spoc_ctx *c, int mode) {
   if (!c) return fail(SPOC_EINVAL, "null context");
-  if (mode != SPOC_ORDER_INPUT && mode != SPOC_ORDER_HILBERT && mode != SPOC_ORDER_ZCURVE)
+  if (mode != SPOC_ORDER_INPUT && mode != SPOC_ORDER_HILBERT && mode != SPOC_ORDER_ZCURVE &&
+      mode != SPOC_ORDER_AUTO)
     return fail(SPOC_EINVAL, "ordering mode %d (SPOC_ORDER_INPUT 0, SPOC_ORDER_HILBERT 1, "
-                             "SPOC_ORDER_ZCURVE 2)", mode);
+                             "SPOC_ORDER_ZCURVE 2, SPOC_ORDER_AUTO 3)", mode);
   c->ordering = mode;
   return SPOC_OK;
 }

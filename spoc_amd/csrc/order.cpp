@@ -146,6 +146,21 @@ void locality_order(int ne, int nf, const int *itm, const int *tseg, const int *
   }
 }
 
+// Share of the interior faces whose two elements lie in the same block of
+// 16 consecutive internal elements (the kernels' unit of work); inv[e] is
+// caller element e's internal position, or null for the caller's numbering.
+double block_share(int nf, const int *tseg, const int *inv) {
+  long inner = 0, same = 0;
+  for (int f = 0; f < nf; f++) {
+    const int a = tseg[2 * f] - 1, b = tseg[2 * f + 1] - 1;
+    if (b < 0) continue;
+    const int ia = inv ? inv[a] : a, ib = inv ? inv[b] : b;
+    inner++;
+    same += (ia >> 4) == (ib >> 4);
+  }
+  return inner ? (double)same / (double)inner : 1.0;
+}
+
 // Face order following the element order: faces numbered in order of first
 // appearance over the renumbered elements' local edges.  fperm[new] = old.
 void face_order(int ne, int nf, const int *itm, const std::vector<int> &perm,

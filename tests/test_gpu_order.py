@@ -16,7 +16,7 @@ from spoc_amd import configs
 from spoc_amd import mesh as M
 
 pytestmark = pytest.mark.gpu
-MODES = ("input", "hilbert", "zcurve")
+MODES = ("input", "hilbert", "zcurve", "auto")
 
 
 def say(*a):
