@@ -37,6 +37,7 @@ def _units():
         units.append(("rhs_f32.hip", f"rhs_f32_{tag}.o", d))
         units.append(("adjoint.hip", f"adj_f64_{tag}.o", d))
     units.append(("f32_plan.cpp", "f32_plan.o", ["-x", "hip"]))
+    units.append(("f64_plan.cpp", "f64_plan.o", EXACT + ["-x", "hip"]))
     units.append(("energy.hip", "energy.o", EXACT))
     units.append(("adapt.hip", "adapt.o", EXACT))
     units.append(("mesh.cpp", "mesh.o", EXACT))

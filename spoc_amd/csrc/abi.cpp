@@ -29,6 +29,22 @@ int adapt_launch(hipStream_t st, int nc, int ngf, int ne, int nmem, int nch,
                  int *flags);
 int adapt_chunk();
 int nonfinite_launch(hipStream_t st, size_t n, const double *a, int *flag);
+struct F64Plan;
+int f64_plan_create(F64Plan **out, int nc, int ng, int ngf);
+void f64_plan_destroy(F64Plan *p);
+int f64_mesh_set(F64Plan *p, int ne, int nf, const int *itm, const int *its, const int *bdl,
+                 const double *vect, const double *normt, const int *dtseg, const int *dssit,
+                 const double *dnorm, double *dface);
+int f64_fields_set(F64Plan *p, hipStream_t s, const DevTables &tab, const double *fc,
+                   const double *w0, const double *z0, const double *hb, const double *hh0e,
+                   double mu0, double mb0, double g0);
+int f64_rhs(F64Plan *p, hipStream_t s, const DevTables &tab, int variant,
+            const double *const *x, double *const *k, TimerHook hook, void *user);
+int f64_rk4_begin(F64Plan *p, hipStream_t s, const DevTables &tab, const double *const *u0);
+int f64_rk4_step(F64Plan *p, hipStream_t s, const DevTables &tab, int variant, double dt,
+                 TimerHook hook, void *user);
+int f64_rk4_end(F64Plan *p, hipStream_t s, const DevTables &tab, double *const *u0);
+int f64_kernels_per_rhs(const F64Plan *p, int variant);
 void locality_order(int ne, int nf, const int *itm, const int *tseg, const int *ssit,
                     const double *vect, const std::vector<int> &mem_e0, bool use_hilbert,
                     std::vector<int> &perm);
@@ -192,7 +208,6 @@ struct spoc_ctx {
   // state and work
   double *dstate = nullptr;  // u0 x3, x x3, acc x3, k x3, work x8
   double *dface = nullptr;
-  double **dFptr = nullptr;
   double *den = nullptr;     // energy scratch
   double *dadapt = nullptr;  // refinement indicator scratch (adapt_launch)
   int *dchunk = nullptr;     // its reduction chunks: starts (nch + 1), members' first (nmem + 1)
@@ -214,6 +229,7 @@ struct spoc_ctx {
   std::vector<double> kms;
   std::vector<int> kcount;
   F32Plan *f32 = nullptr;
+  F64Plan *f64 = nullptr;
   std::vector<double> h_pi2;  // pi2(ng, nm) for projecting fields onto quadrature points
   // `steps` RK4 steps captured as a graph, valid while its key matches
   hipGraphExec_t graph = nullptr;
@@ -296,8 +312,6 @@ void free_mesh(spoc_ctx *c) {
   c->perm.clear();
   c->iperm.clear();
   if (c->dfields) (void)hipFree(c->dfields);
-  if (c->dFptr) (void)hipFree(c->dFptr);
-  c->dFptr = nullptr;
   c->dmesh = nullptr;
   c->dstate = nullptr;
   c->dface = nullptr;
@@ -533,6 +547,7 @@ int spoc_ctx_create(spoc_ctx **out, int device, int nc, int ng, int ngf, const d
     spoc_ctx_destroy(c);
     return fail(SPOC_EHIP, "FP32 operator upload failed");
   }
+  f64_plan_create(&c->f64, nc, ng, ngf);
   *out = c;
   return SPOC_OK;
 }
@@ -543,6 +558,7 @@ int spoc_ctx_destroy(spoc_ctx *c) {
   (void)hipStreamSynchronize(c->stream);
   free_mesh(c);
   f32_plan_destroy(c->f32);
+  f64_plan_destroy(c->f64);
   if (c->dtab) (void)hipFree(c->dtab);
   if (c->dflag) (void)hipFree(c->dflag);
   if (c->stage) (void)hipHostFree(c->stage);
@@ -729,15 +745,13 @@ static int mesh_set(spoc_ctx *c, int ne, int nf, const int *itm, const int *its,
     HIPCHK(hipMalloc((void **)&c->dorder, sizeof(int) * ne));
     HIPCHK(hipMemcpy(c->dorder, c->iperm.data(), sizeof(int) * ne, hipMemcpyHostToDevice));
   }
-  HIPCHK(hipMalloc((void **)&c->dFptr, sizeof(double *) * 4));
-  double *fp[4];
-  for (int i = 0; i < 4; i++) fp[i] = state_ptr(c, 12 + i);
-  HIPCHK(hipMemcpy(c->dFptr, fp, sizeof fp, hipMemcpyHostToDevice));
   HIPCHK(hipMalloc((void **)&c->dadj, sizeof(double) * (size_t)4 * nm * ne));
   HIPCHK(hipMalloc((void **)&c->dfields, sizeof(double) * ((size_t)4 * nm * ne + ne)));
   HIPCHK(hipMemset(c->dfields, 0, sizeof(double) * ((size_t)4 * nm * ne + ne)));
   if (f32_mesh_set(c->f32, ne, nf, itm, its, tseg, ssit, norm, vect, normt, bdl))
     return fail(SPOC_EHIP, "FP32 mesh upload failed");
+  if (f64_mesh_set(c->f64, ne, nf, itm, its, b.data(), vect, normt, dtseg, dssit, dnorm, c->dface))
+    return fail(SPOC_EHIP, "FP64 mesh upload failed");
   return SPOC_OK;
 }
 
@@ -896,6 +910,9 @@ int spoc_fields_set(spoc_ctx *c, int ld, const double *fc, const double *w0, con
   if (f32_fields_set(c->f32, ld4, f4[0], f4[1], f4[2], f4[3], hh0e.data(), c->h_pi2.data(),
                      c->ng, mu0, mb0, g0))
     return fail(SPOC_EHIP, "FP32 field upload failed");
+  if (f64_fields_set(c->f64, c->stream, c->tab, c->dfields, c->dfields + n, c->dfields + 2 * n,
+                     c->dfields + 3 * n, c->dfields + 4 * n, mu0, mb0, g0))
+    return fail(SPOC_EHIP, "FP64 field set-up failed");
   c->have_fields = true;
   return SPOC_OK;
 }
@@ -919,54 +936,6 @@ int spoc_state_download(spoc_ctx *c, int ld, double *u0, double *v0, double *h0)
   double *dst[3] = {u0, v0, h0};
   double *src[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
   return download_fields(c, ld, dst, src, 3);
-}
-
-static RhsArgs make_args(spoc_ctx *c, int variant, const double *const *x, int stage,
-                         double dt) {
-  RhsArgs a{};
-  a.variant = variant;
-  a.stream = c->stream;
-  a.tab = c->tab;
-  a.mesh = c->mesh;
-  a.phys = c->phys;
-  size_t n = (size_t)c->nm * c->ne;
-  for (int i = 0; i < 3; i++) {
-    a.x[i] = x[i];
-    a.u0[i] = state_ptr(c, i);
-    a.xout[i] = state_ptr(c, 3 + i);
-    a.acc[i] = state_ptr(c, 6 + i);
-    a.k[i] = state_ptr(c, 9 + i);
-  }
-  for (int i = 0; i < 8; i++) a.work[i] = state_ptr(c, 12 + i);
-  a.fc = c->dfields;
-  a.w0 = c->dfields + n;
-  a.z0 = c->dfields + 2 * n;
-  a.hb = c->dfields + 3 * n;
-  a.stage = stage;
-  a.dt = dt;
-  a.Fptr = c->dFptr;
-  a.face = c->dface;
-  a.hook = prof_hook;
-  a.hook_user = c;
-  return a;
-}
-
-static int run_rhs(spoc_ctx *c, int precision, const RhsArgs &a) {
-  int rc = -1;
-  if (precision == SPOC_FP64) {
-#define SPOC_CALL(NC_, NG_, NGF_) \
-  if (c->nc == NC_ && c->ng == NG_ && c->ngf == NGF_) rc = rhs_f64_##NC_##_##NG_##_##NGF_(a);
-    SPOC_FOR_EACH_SHAPE(SPOC_CALL)
-#undef SPOC_CALL
-  } else {
-    return fail(SPOC_EINVAL, "unknown precision %d", precision);
-  }
-  if (rc == -1) return fail(SPOC_EINVAL, "no kernel for this shape/precision");
-  if (rc) {
-    hipError_t e = hipGetLastError();
-    return fail(SPOC_EHIP, "kernel launch failed: %s", hipGetErrorString(e));
-  }
-  return SPOC_OK;
 }
 
 static int check_ready(spoc_ctx *c, int variant) {
@@ -995,10 +964,12 @@ int spoc_rhs(spoc_ctx *c, int variant, int precision, int ld, const double *u0,
   if (precision == SPOC_FP32_MFMA) {
     rc = f32_rhs(c->f32, c->stream, variant, x, ks, nullptr, nullptr);
     if (rc) return f32_fail(rc);
+  } else if (precision == SPOC_FP64) {
+    rc = f64_rhs(c->f64, c->stream, c->tab, variant, x, ks, nullptr, nullptr);
+    if (rc) return fail(SPOC_EHIP, "FP64 kernel launch failed: %s",
+                        hipGetErrorString(hipGetLastError()));
   } else {
-    RhsArgs a = make_args(c, variant, x, -1, 0.0);
-    rc = run_rhs(c, precision, a);
-    if (rc) return rc;
+    return fail(SPOC_EINVAL, "unknown precision %d", precision);
   }
   double *dst[3] = {u1, v1, h1};
   return download_fields(c, ld, dst, ks, 3);
@@ -1017,7 +988,6 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
   c->prof.used = 0;
   c->prof.marks.clear();
   const double *u0[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
-  const double *xs[3] = {state_ptr(c, 3), state_ptr(c, 4), state_ptr(c, 5)};
   double *uw[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
   // one RK4 step (4 right-hand sides with their fused stage updates)
   auto step = [&](TimerHook hook) -> int {
@@ -1025,13 +995,10 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
       int r = f32_rk4_step(c->f32, c->stream, variant, dt, hook, c);
       return r ? f32_fail(r) : SPOC_OK;
     }
-    for (int s = 0; s < 4; s++) {
-      RhsArgs a = make_args(c, variant, s == 0 ? u0 : xs, s, dt);
-      if (!hook) a.hook = nullptr;
-      int r = run_rhs(c, precision, a);
-      if (r) return r;
-    }
-    return SPOC_OK;
+    int r = f64_rk4_step(c->f64, c->stream, c->tab, variant, dt, hook, c);
+    return r ? fail(SPOC_EHIP, "FP64 kernel launch failed: %s",
+                    hipGetErrorString(hipGetLastError()))
+             : SPOC_OK;
   };
   // an error leaves nothing queued behind it
   auto bail = [&](int r) {
@@ -1040,10 +1007,13 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
     return r;
   };
   c->queue.reset();
-  const int per_step = step_dispatches(variant, precision);
+  const int per_step = precision == SPOC_FP64 ? 4 * f64_kernels_per_rhs(c->f64, variant)
+                                              : step_dispatches(variant, precision);
   HIPCHK(hipEventRecord(c->t0, c->stream));
   if (precision == SPOC_FP32_MFMA && (rc = f32_rk4_begin(c->f32, c->stream, variant, uw)))
     return bail(f32_fail(rc));
+  if (precision == SPOC_FP64 && f64_rk4_begin(c->f64, c->stream, c->tab, u0))
+    return bail(fail(SPOC_EHIP, "FP64 state conversion failed"));
   const bool graph = c->use_graph && !c->prof.on && nsteps >= 2;
   if (graph) {
     // the step's launches are identical every step: replay them as a graph
@@ -1098,6 +1068,8 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
   }
   if (precision == SPOC_FP32_MFMA && (rc = f32_rk4_end(c->f32, c->stream, variant, uw)))
     return bail(f32_fail(rc));
+  if (precision == SPOC_FP64 && f64_rk4_end(c->f64, c->stream, c->tab, uw))
+    return bail(fail(SPOC_EHIP, "FP64 state conversion failed"));
   HIPCHK(hipEventRecord(c->t1, c->stream));
   // the driver's testnan: is every coefficient of the new state finite?
   if (nonfinite_launch(c->stream, (size_t)3 * c->nm * c->ne, state_ptr(c, 0), c->dflag))
@@ -1115,8 +1087,8 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
   c->kms.clear();
   c->kcount.clear();
   if (c->prof.on) {
-    std::vector<double> tot(8, 0.0);
-    std::vector<int> cnt(8, 0);
+    std::vector<double> tot(16, 0.0);
+    std::vector<int> cnt(16, 0);
     for (size_t i = 0; i + 1 < c->prof.marks.size(); i++) {
       int s0 = c->prof.marks[i].first, s1 = c->prof.marks[i + 1].first;
       if (s1 != s0 + 1) continue;
@@ -1127,9 +1099,10 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
         cnt[s0]++;
       }
     }
-    const char *lin[] = {"face_lin", "elem_lin"};
-    const char *nl[] = {"face_uv", "elem_bc", "face_nl2", "elem_eg"};
-    int nk = variant == SPOC_LIN ? 2 : 4;
+    const char *lin[] = {"face_lin", "elem_lin", "wall_lin"};
+    const char *nl[] = {"face_uv", "elem_q", "wall_q", "elem_c", "wall_c", "face_nl2", "elem_eg",
+                        "wall_eg"};
+    int nk = variant == SPOC_LIN ? 3 : 8;
     if (precision == SPOC_FP32_MFMA) {
       const char *l32[] = {"lin_f64"};
       const char *n32[] = {"bc_f32", "eg_f32f64"};

@@ -3,66 +3,117 @@
 // Every value is produced by the same sequence of IEEE operations as the
 // reference Fortran (no FMA contraction: this file is compiled with
 // -ffp-contract=off), so the device result equals the reference's bit for
-// bit.  The parallel structure is the GPU's own: elements are processed in
-// tiles of E per 256-thread workgroup with every per-element vector staged in
-// LDS; one thread owns one (element, output index) dot product and runs it
-// in the reference's summation order; face traces are a separate pass over
-// faces because the reference accumulates both sides of a face into one sum
-// (src/calflu.f:41-62).
+// bit.  Bit-exactness fixes the order of each output's sums, not who
+// computes it or where the data lives, so the work is laid out for the GPU:
+//
+//  - one element per lane: arrays are element-blocked by 64 (f64_types.h),
+//    so a wave reads and writes each row of its 64 elements as one
+//    contiguous segment, and each lane runs its element's sequential sums
+//    from its own registers;
+//  - the operator tables are read at wave-uniform addresses, as scalar loads
+//    into SGPRs (the table pointers are separate noalias kernel arguments);
+//  - each mass-matrix solve is a lane's triangular solve on its element's
+//    vector in registers, so no lane waits for another;
+//  - an element kernel's waves split the element's outputs (the q
+//    components, the three tendencies), so every kernel has several waves
+//    per block of 64 elements;
+//  - the wall projections (the free-slip and no-slip sequences of the
+//    reference), which only wall elements need, run afterwards in a small
+//    kernel over the list of wall elements; the element kernels leave those
+//    elements' raw vectors in a work array;
+//  - face traces are a separate pass (one thread per face and field),
+//    because the reference accumulates both sides of a face into one sum
+//    (src/calflu.f:41-62).
 //
 // Kernels per right-hand side:
-//   linear     : face_lin  -> elem_lin (+ RK4 stage update)
-//   nonlinear  : face_uv   -> elem_bc (velocity gradient q, quadrature terms)
-//                face_nl2  -> elem_eg (u, v, eta tendencies + RK4 update)
+//   linear    : face (eta, u, v) -> elem_lin -> wall_lin
+//   nonlinear : face (u, v) -> elem_q -> wall_q (free slip) -> elem_c ->
+//               wall_c -> face (F, uh, vh) -> elem_eg -> wall_eg
 #include <hip/hip_runtime.h>
 
 #include "common.h"
-#include "launch.h"
+#include "f64_types.h"
 
 namespace spoc {
 namespace f64 {
 
-constexpr int NT = 256;  // threads per workgroup
+constexpr int WB = F64_WB;
+
+// A compiler barrier: no instruction moves across it, and memory counts as
+// changed, so table values read from LDS before it are read again after it
+// rather than kept in registers.  Placed after each output of a fully
+// unrolled loop, it keeps the reads of later outputs from being hoisted and
+// earlier reads from being reused (either would run out of registers).
+__device__ __forceinline__ void fence() {
+  __builtin_amdgcn_sched_barrier(0);
+  asm volatile("" ::: "memory");
+}
 
 template <int NC, int NG, int NGF>
-struct Shape {
+struct Sh {
   static constexpr int NM = nm_of(NC);
-  static constexpr int NL = NM - NC - 1;          // reduced (zero on edge 1) basis
-  static constexpr int NMP = NM | 1;              // odd LDS stride: fewer bank conflicts
-  static constexpr int NGP = NG | 1;
-  static constexpr int E = NM <= 15 ? 32 : 16;    // elements per workgroup
+  static constexpr int NL = NM - NC - 1;  // reduced (zero on edge 1) basis
+  static constexpr int NE3 = 3 * NGF;
 };
 
-// ---------------------------------------------------------------------------
-// Per-element building blocks, each the reference routine's arithmetic.
-// ---------------------------------------------------------------------------
+__device__ __forceinline__ int wave_id() {
+  return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+}
 
-// ans_i = sum_j mat(j, i) * x(j), j ascending (src/calb2.f:14-18).
+// ---------------------------------------------------------------------------
+// Per-lane building blocks, each the reference routine's arithmetic.
+// ---------------------------------------------------------------------------
 template <int NM>
-__device__ __forceinline__ double dot_col(const double *__restrict__ matT, int i,
-                                          const double *x) {
+__device__ __forceinline__ void vload(double (&v)[NM], const double *p, int e) {
+#pragma unroll
+  for (int m = 0; m < NM; m++) v[m] = p[f64_at(e, m, NM)];
+}
+
+template <int NM>
+__device__ __forceinline__ void vstore(double *p, int e, const double (&v)[NM]) {
+#pragma unroll
+  for (int m = 0; m < NM; m++) p[f64_at(e, m, NM)] = v[m];
+}
+
+// ans = sum_j mat(j, i) x(j), j ascending (src/calb2.f:14-18); row i of the
+// LDS image holds mat(0..NM-1, i) contiguously, read at a uniform address.
+template <int NM>
+__device__ __forceinline__ double dot_col(const double *mrow, const double (&x)[NM]) {
   double ans = 0.0;
 #pragma unroll
-  for (int j = 0; j < NM; j++) ans = ans + matT[i + NM * j] * x[j];
+  for (int j = 0; j < NM; j++) ans = ans + mrow[j] * x[j];
   return ans;
 }
 
-// Edge lift of one face field to mode i (src/mat1.f:184-207).  fb holds the
-// element's 3 faces, NGF values each, for this field.
-template <int NM, int NGF>
-__device__ __forceinline__ double tri_f(double b, const double *__restrict__ picoT,
-                                        const double *__restrict__ wif, int i,
-                                        const double *fb, const int *its, double coeff) {
+// The three faces' values of face output o for this lane's element:
+// fb[k NGF + j].
+template <int NGF>
+__device__ __forceinline__ void face_vals(double (&fb)[3 * NGF], const double *__restrict__ face,
+                                          int nout, int o, const int (&fk)[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int j = 0; j < NGF; j++) fb[k * NGF + j] = face[((size_t)fk[k] * nout + o) * NGF + j];
+}
+
+// Edge lift of one face field to mode i (src/mat1.f:184-207): for each local
+// edge k the face points are taken forward (its = 1) or reversed (its = -1).
+// prow = P + 3 NGF i, the LDS image of pico(., i, k) (P[(3 i + k) NGF + j]);
+// the lane picks the point by its own address.
+template <int NGF>
+__device__ __forceinline__ double tri_f(double b, const double *prow,
+                                        const double *__restrict__ wif,
+                                        const double (&fb)[3 * NGF], const int (&its)[3],
+                                        double coeff) {
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    int isg = its[k];
-    int isg2 = (NGF + 1) * (1 - isg) / 2;
-    double sig = (double)isg * coeff * 0.5;
+    const int isg = its[k];
+    const double sig = (double)isg * coeff * 0.5;
+    const double *pk = prow + k * NGF;
     double ans = 0.0;
 #pragma unroll
     for (int j = 1; j <= NGF; j++) {
-      int j1 = isg2 + isg * j;
-      double var = picoT[i + NM * ((j1 - 1) + NGF * k)] * fb[k * NGF + (j - 1)];
+      const double var = pk[isg > 0 ? j - 1 : NGF - j] * fb[k * NGF + (j - 1)];
       ans = ans + wif[j - 1] * var;
     }
     b = b + ans * sig;
@@ -72,8 +123,7 @@ __device__ __forceinline__ double tri_f(double b, const double *__restrict__ pic
 
 // Cholesky solve U^T U x = b with b in registers (src/dtrsm1.f:58-75).
 template <int M, int LD>
-__device__ __forceinline__ void dtrsm4_reg(double (&b)[M > 0 ? M : 1],
-                                           const double *__restrict__ U,
+__device__ __forceinline__ void dtrsm4_reg(double (&b)[M], const double *__restrict__ U,
                                            const double *__restrict__ dia) {
 #pragma unroll
   for (int i = 0; i < M; i++) {
@@ -81,16 +131,18 @@ __device__ __forceinline__ void dtrsm4_reg(double (&b)[M > 0 ? M : 1],
 #pragma unroll
     for (int k = 0; k < i; k++) temp = temp - U[k + LD * i] * b[k];
     b[i] = temp * dia[i];
+    fence();
   }
 #pragma unroll
   for (int k = M - 1; k >= 0; k--) {
     b[k] = b[k] * dia[k];
 #pragma unroll
     for (int i = 0; i < k; i++) b[i] = b[i] - b[k] * U[i + LD * k];
+    fence();
   }
 }
 
-// Same solve on an LDS/private vector (boundary-element path).
+// The same solve on a private vector of m entries (wall path).
 template <int LD>
 __device__ void dtrsm4_mem(int m, double *b, const double *__restrict__ U,
                            const double *__restrict__ dia) {
@@ -129,42 +181,40 @@ __device__ void reduc_inv_gn(double *gn, const double *gnr) {
   gn[nm_of(NC) - 1] = 0.0;
 }
 
-// Free-slip projection of a vector field (gx, gy) on a wall element:
-// rotate to (n, t), solve the normal part in the reduced basis and the
-// tangential part with the full mass matrix, rotate back
-// (src/calvarnl_fs.f:447-459, src/calvar_lin.f:184-197).
+// Free-slip projection of a vector field (gx, gy) on a wall element: rotate
+// to (n, t), solve the normal part in the reduced basis and the tangential
+// part with the full mass matrix, rotate back (src/calvarnl_fs.f:447-459,
+// src/calvar_lin.f:184-197).
 template <int NC, int NM, int NL>
-__device__ void wall_rot_solve(double *gx, double *gy, int sx, double dnx, double dny,
+__device__ void wall_rot_solve(double *gx, double *gy, double dnx, double dny,
                                const DevTables &T) {
   double gn[NM], gt[NM], bu[NL > 0 ? NL : 1];
   for (int j = 0; j < NM; j++) {
-    gn[j] = gx[j * sx] * dnx + gy[j * sx] * dny;
-    gt[j] = gx[j * sx] * dny - gy[j * sx] * dnx;
+    gn[j] = gx[j] * dnx + gy[j] * dny;
+    gt[j] = gx[j] * dny - gy[j] * dnx;
   }
   reduc_gn<NC>(gn, bu);
   dtrsm4_mem<NM>(NL, bu, T.UL, T.dia2);
   dtrsm4_mem<NM>(NM, gt, T.U, T.dia1);
   reduc_inv_gn<NC>(gn, bu);
   for (int j = 0; j < NM; j++) {
-    gx[j * sx] = gn[j] * dnx + gt[j] * dny;
-    gy[j * sx] = gn[j] * dny - gt[j] * dnx;
+    gx[j] = gn[j] * dnx + gt[j] * dny;
+    gy[j] = gn[j] * dny - gt[j] * dnx;
   }
 }
 
-// No-slip projection: both components in the reduced basis
+// No-slip projection: the field in the reduced basis
 // (src/calvarnl_ns.f:253-264, 409-421).
 template <int NC, int NM, int NL>
-__device__ void wall_red_solve(double *g, int sx, const DevTables &T) {
-  double v[NM], bu[NL > 0 ? NL : 1];
-  for (int j = 0; j < NM; j++) v[j] = g[j * sx];
-  reduc_gn<NC>(v, bu);
+__device__ void wall_red_solve(double *g, const DevTables &T) {
+  double bu[NL > 0 ? NL : 1];
+  reduc_gn<NC>(g, bu);
   dtrsm4_mem<NM>(NL, bu, T.UL, T.dia2);
-  reduc_inv_gn<NC>(v, bu);
-  for (int j = 0; j < NM; j++) g[j * sx] = v[j];
+  reduc_inv_gn<NC>(g, bu);
 }
 
 // Free-slip projection of the velocity-gradient tensor on a wall element
-// (src/calvarnl_fs.f:171-204).  q[c] are the 4 components xx, xy, yx, yy.
+// (src/calvarnl_fs.f:171-204): q = xx, xy, yx, yy.
 template <int NC, int NM, int NL>
 __device__ void wall_tensor_solve(double *qxx, double *qxy, double *qyx, double *qyy,
                                   double dnx, double dny, const DevTables &T) {
@@ -193,655 +243,733 @@ __device__ void wall_tensor_solve(double *qxx, double *qxy, double *qyx, double 
   }
 }
 
-// ---------------------------------------------------------------------------
-// Face traces (src/calflu.f:6-71), one thread per face.  Each input field's
-// trace is multiplied by a component of the face normal (and for the linear
-// eta equation by hh0), as the loops after every cal_flu call do.
-// ---------------------------------------------------------------------------
-struct FaceSpec {
-  const double *in[6];  // modal inputs (NM, ne)
-  double pre[6];        // value fed to the trace is pre*a when haspre
-  int haspre[6];
-  int nout;
-  int src[6];           // input field of each output
-  int comp[6];          // normal component of each output (0: n_x, 1: n_y)
-  double post[6];       // extra factor after the normal when haspost
-  int haspost[6];
-  const double *hpost;  // if set: per-element factor (of the face's first element) in place of post
+// Pointer c of three (a select, not an index: a runtime index into the
+// kernel argument would copy the whole argument block to scratch).
+template <class T>
+__device__ __forceinline__ T pick(int c, T a, T b, T d) {
+  return c == 0 ? a : (c == 1 ? b : d);
+}
+
+// Where field c's RK4 stage update goes.
+struct Out {
+  double *k, *u0, *xo, *acc;
 };
-
-template <int NC, int NGF, int NIN>
-__global__ __launch_bounds__(NT) void face_kernel(int nf, const int *__restrict__ tseg,
-                                                  const int *__restrict__ ssit,
-                                                  const double *__restrict__ norm,
-                                                  const double *__restrict__ pico, FaceSpec spec,
-                                                  double *__restrict__ out) {
-  constexpr int NM = nm_of(NC);
-  int f = blockIdx.x * NT + threadIdx.x;
-  if (f >= nf) return;
-  int k1 = tseg[2 * f] - 1, k2 = tseg[2 * f + 1] - 1;
-  double acc[NIN][NGF];
-#pragma unroll
-  for (int c = 0; c < NIN; c++)
-#pragma unroll
-    for (int j = 0; j < NGF; j++) acc[c][j] = 0.0;
-  int s1 = (k2 < 0) ? 0 : ssit[2 * f] - 1;
-  for (int m = 0; m < NM; m++) {
-    const double *pc = pico + NGF * (m + NM * s1);
-#pragma unroll
-    for (int c = 0; c < NIN; c++) {
-      double a = spec.in[c][m + NM * k1];
-      if (spec.haspre[c]) a = spec.pre[c] * a;
-#pragma unroll
-      for (int j = 0; j < NGF; j++) acc[c][j] = acc[c][j] + a * pc[j];
-    }
-  }
-  if (k2 >= 0) {
-    int s2 = ssit[2 * f + 1] - 1;
-    for (int m = 0; m < NM; m++) {
-      const double *pc = pico + NGF * (m + NM * s2);
-#pragma unroll
-      for (int c = 0; c < NIN; c++) {
-        double a = spec.in[c][m + NM * k2];
-        if (spec.haspre[c]) a = spec.pre[c] * a;
-#pragma unroll
-        for (int j = 0; j < NGF; j++) acc[c][j] = acc[c][j] + a * pc[NGF - 1 - j];
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < NIN; c++)
-#pragma unroll
-      for (int j = 0; j < NGF; j++) acc[c][j] = 0.5 * acc[c][j];
-  }
-  double n[2] = {norm[2 * f], norm[2 * f + 1]};
-  for (int o = 0; o < spec.nout; o++) {
-    double *dst = out + ((size_t)f * spec.nout + o) * NGF;
-    int c = spec.src[o];
-    double nn = n[spec.comp[o]], post = spec.hpost ? spec.hpost[k1] : spec.post[o];
-    int haspost = spec.haspost[o];
-#pragma unroll
-    for (int cc = 0; cc < NIN; cc++)
-      if (cc == c) {
-#pragma unroll
-        for (int j = 0; j < NGF; j++) {
-          double v = acc[cc][j] * nn;
-          if (haspost) v = v * post;
-          dst[j] = v;
-        }
-      }
-  }
+__device__ __forceinline__ Out out_of(const F64Args &A, int c) {
+  return Out{pick(c, A.k[0], A.k[1], A.k[2]), pick(c, A.u0[0], A.u0[1], A.u0[2]),
+             pick(c, A.xo[0], A.xo[1], A.xo[2]), pick(c, A.acc[0], A.acc[1], A.acc[2])};
 }
 
-// ---------------------------------------------------------------------------
-// Tile helpers.
-// ---------------------------------------------------------------------------
-
-// Copy NF modal fields of the tile's elements into LDS [field][e][NMP].
-template <int NM, int NMP, int E>
-__device__ __forceinline__ void load_fields(double *lds, const double *const *src, int nfld,
-                                            int e0, int ne) {
-  for (int idx = threadIdx.x; idx < nfld * E * NM; idx += NT) {
-    int c = idx / (E * NM), r = idx % (E * NM), e = r / NM, m = r % NM;
-    double v = 0.0;
-    if (e0 + e < ne) v = src[c][(size_t)(e0 + e) * NM + m];
-    lds[(c * E + e) * NMP + m] = v;
-  }
-}
-
-// Gather the tile's face values: fb[((e*3 + k)*NOUT + o)*NGF + j].
-template <int NGF, int E, int NOUT>
-__device__ __forceinline__ void load_faces(double *fb, const double *__restrict__ face,
-                                           const int *__restrict__ itm, int e0, int ne) {
-  for (int idx = threadIdx.x; idx < E * 3 * NOUT * NGF; idx += NT) {
-    int j = idx % NGF, r = idx / NGF, o = r % NOUT, ek = r / NOUT, k = ek % 3, e = ek / 3;
-    double v = 0.0;
-    if (e0 + e < ne) {
-      int f = itm[3 * (e0 + e) + k] - 1;
-      v = face[((size_t)f * NOUT + o) * NGF + j];
-    }
-    fb[idx] = v;
-  }
-}
-
-// RK4 stage update of one value (src/shwwnl.f:242-284), stage s = 0..3.
-// acc accumulates ((k1 + 2k2) + 2k3) + k4 in the reference's order.
-__device__ __forceinline__ void rk_update(int s, double k, double dt, double dt6, double *u0,
-                                          double *x, double *acc) {
-  if (s == 0) {
-    *acc = k;
-    *x = *u0 + dt * k * 0.5;
+// RK4 stage update of one value (src/shwwnl.f:242-284), stage s = 0..3;
+// acc accumulates ((k1 + 2k2) + 2k3) + k4 in the reference's order.  Stage
+// -1: the tendency itself.
+__device__ __forceinline__ void emit(const F64Args &A, const Out &o, size_t g, double k) {
+  const int s = A.stage;
+  if (s < 0) {
+    o.k[g] = k;
+  } else if (s == 0) {
+    o.acc[g] = k;
+    o.xo[g] = o.u0[g] + A.dt * k * 0.5;
   } else if (s == 1) {
-    *acc = *acc + 2.0 * k;
-    *x = *u0 + dt * k * 0.5;
+    o.acc[g] = o.acc[g] + 2.0 * k;
+    o.xo[g] = o.u0[g] + A.dt * k * 0.5;
   } else if (s == 2) {
-    *acc = *acc + 2.0 * k;
-    *x = *u0 + dt * k;
+    o.acc[g] = o.acc[g] + 2.0 * k;
+    o.xo[g] = o.u0[g] + A.dt * k;
   } else {
-    double a = *acc + k;
-    *u0 = *u0 + dt6 * a;
+    const double a = o.acc[g] + k;
+    o.u0[g] = o.u0[g] + A.dt6 * a;
   }
 }
-
-// Where an element kernel sends its tendencies.
-struct StageOut {
-  double *k[3];        // raw tendencies (spoc_rhs), or null
-  double *u0[3];       // RK4 base state (read; written at stage 3)
-  double *x[3];        // next stage input
-  double *acc[3];      // RK4 accumulator
-  int stage;           // 0..3, or -1 for "tendencies only"
-  double dt, dt6;
-};
 
 template <int NM>
-__device__ __forceinline__ void emit(const StageOut &so, int c, size_t g, double kval) {
-  if (so.stage < 0) {
-    so.k[c][g] = kval;
-  } else {
-    rk_update(so.stage, kval, so.dt, so.dt6, &so.u0[c][g], &so.x[c][g], &so.acc[c][g]);
+__device__ __forceinline__ void emit_vec(const F64Args &A, int c, int e, const double (&g)[NM]) {
+  const Out o = out_of(A, c);
+#pragma unroll
+  for (int m = 0; m < NM; m++) emit(A, o, f64_at(e, m, NM), g[m]);
+}
+
+// Per-lane element data.
+struct Elem {
+  int e;
+  bool valid, wall;
+  double ax, bx, ay, by, detai, nx, ny;
+  int fk[3], its[3];
+};
+
+__device__ __forceinline__ Elem elem_of(const F64Mesh &M, int e) {
+  Elem q;
+  q.e = e;
+  q.valid = e < M.ne;
+  const double *g = M.geo;
+  q.ax = g[f64_at(e, G_AX, G_ROWS)];
+  q.bx = g[f64_at(e, G_BX, G_ROWS)];
+  q.ay = g[f64_at(e, G_AY, G_ROWS)];
+  q.by = g[f64_at(e, G_BY, G_ROWS)];
+  q.detai = g[f64_at(e, G_DETAI, G_ROWS)];
+  q.nx = g[f64_at(e, G_NX, G_ROWS)];
+  q.ny = g[f64_at(e, G_NY, G_ROWS)];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    q.fk[k] = M.ib[f64_at(e, I_ITM + k, I_ROWS)];
+    q.its[k] = M.ib[f64_at(e, I_ITS + k, I_ROWS)];
+  }
+  q.wall = M.ib[f64_at(e, I_BDL, I_ROWS)] != 0;
+  return q;
+}
+
+// Operator images in LDS, staged by each workgroup: agx and agy with row i
+// holding column i of the Fortran matrix (the terms of one output in order),
+// pico as P[(3 i + k) NGF + j], the mass-matrix factor U (column-major, as the
+// solves walk it) and 1 / U_ii.
+template <int NM, int NGF>
+struct OpsLds {
+  static constexpr int AX = 0, AY = NM * NM, P = 2 * NM * NM, UU = P + 3 * NM * NGF,
+                       DIA = UU + NM * NM, SIZE = DIA + NM;
+};
+
+template <int NM, int NGF>
+__device__ __forceinline__ void stage_ops(double *lds, const DevTables &T, int nthreads) {
+  using L = OpsLds<NM, NGF>;
+  for (int t = threadIdx.x; t < NM * NM; t += nthreads) {
+    const int i = t / NM, j = t % NM;
+    lds[L::AX + t] = T.agxT[i + NM * j];
+    lds[L::AY + t] = T.agyT[i + NM * j];
+    lds[L::UU + t] = T.U[t];
+  }
+  for (int t = threadIdx.x; t < 3 * NM * NGF; t += nthreads) {
+    const int j = t % NGF, k = (t / NGF) % 3, i = t / (3 * NGF);
+    lds[L::P + t] = T.picoT[i + NM * (j + NGF * k)];
+  }
+  for (int t = threadIdx.x; t < NM; t += nthreads) lds[L::DIA + t] = T.dia1[t];
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Face traces (src/calflu.f:6-71): one thread per (face, field); the field is
+// blockIdx.y.  Each field's trace is multiplied by a component of the face
+// normal (and for the linear eta equation by hh0), as the loops after every
+// cal_flu call do.
+// ---------------------------------------------------------------------------
+struct FaceSpec {
+  const double *in[6];  // blocked modal inputs
+  double pre[6];        // the trace is of pre * a where haspre
+  int haspre[6];
+  int nout;             // outputs per face
+  int nof[6];           // outputs of each field (1 or 2)
+  int out[6][2];        // their output indices
+  int comp[6][2];       // normal component of each (0: n_x, 1: n_y)
+  int haspost[6][2];    // times hpost of the face's first element
+  const double *hpost;
+};
+
+template <int NC, int NGF>
+__global__ __launch_bounds__(256) void face_kernel(int nf, const int *__restrict__ tseg,
+                                                   const int *__restrict__ ssit,
+                                                   const double *__restrict__ norm,
+                                                   const double *__restrict__ pico,
+                                                   FaceSpec spec, double *__restrict__ out) {
+  constexpr int NM = nm_of(NC);
+  __shared__ double sp[3 * NM * NGF];
+  for (int i = threadIdx.x; i < 3 * NM * NGF; i += 256) sp[i] = pico[i];
+  __syncthreads();
+  const int f = blockIdx.x * 256 + threadIdx.x, c = blockIdx.y;
+  if (f >= nf) return;
+  const int k1 = tseg[2 * f] - 1, k2 = tseg[2 * f + 1] - 1;
+  const int s1 = (k2 < 0) ? 0 : ssit[2 * f] - 1;
+  const double *in = spec.in[0];
+  bool haspre = spec.haspre[0] != 0;
+  double pre = spec.pre[0];
+  int nof = spec.nof[0], o0 = spec.out[0][0], o1 = spec.out[0][1], c0 = spec.comp[0][0],
+      c1 = spec.comp[0][1], h0 = spec.haspost[0][0], h1 = spec.haspost[0][1];
+#pragma unroll
+  for (int t = 1; t < 6; t++)  // (selects, not an index into the argument)
+    if (c == t) {
+      in = spec.in[t];
+      haspre = spec.haspre[t] != 0;
+      pre = spec.pre[t];
+      nof = spec.nof[t];
+      o0 = spec.out[t][0];
+      o1 = spec.out[t][1];
+      c0 = spec.comp[t][0];
+      c1 = spec.comp[t][1];
+      h0 = spec.haspost[t][0];
+      h1 = spec.haspost[t][1];
+    }
+  double acc[NGF];
+#pragma unroll
+  for (int j = 0; j < NGF; j++) acc[j] = 0.0;
+#pragma unroll 4
+  for (int m = 0; m < NM; m++) {
+    double a = in[f64_at(k1, m, NM)];
+    if (haspre) a = pre * a;
+    const double *pc = sp + NGF * (m + NM * s1);
+#pragma unroll
+    for (int j = 0; j < NGF; j++) acc[j] = acc[j] + a * pc[j];
+  }
+  if (k2 >= 0) {
+    const int s2 = ssit[2 * f + 1] - 1;
+#pragma unroll 4
+    for (int m = 0; m < NM; m++) {
+      double a = in[f64_at(k2, m, NM)];
+      if (haspre) a = pre * a;
+      const double *pc = sp + NGF * (m + NM * s2);
+#pragma unroll
+      for (int j = 0; j < NGF; j++) acc[j] = acc[j] + a * pc[NGF - 1 - j];
+    }
+#pragma unroll
+    for (int j = 0; j < NGF; j++) acc[j] = 0.5 * acc[j];
+  }
+  const double nxf = norm[2 * f], nyf = norm[2 * f + 1];
+  for (int t = 0; t < nof; t++) {
+    double *dst = out + ((size_t)f * spec.nout + (t ? o1 : o0)) * NGF;
+    const double nn = (t ? c1 : c0) ? nyf : nxf;
+    const bool post = (t ? h1 : h0) != 0;
+    const double pv = post ? spec.hpost[k1] : 1.0;
+#pragma unroll
+    for (int j = 0; j < NGF; j++) {
+      double v = acc[j] * nn;
+      if (post) v = v * pv;
+      dst[j] = v;
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
-// Linear right-hand side element kernel (src/calvar_lin.f:115-282).
-// Face inputs per face: fbdxx, fbdyy (of fxx = -g0*h0), fbdx, fbdy (of u0*hh0,
-// v0*hh0).  fxy = fyx = 0, whose terms add only zeros in the reference.
+// Passes over one output vector w (the element's row of a blocked work
+// array), one output at a time: the loop over outputs stays rolled, so only
+// the pass's input vector and a few accumulators are live.
+// ---------------------------------------------------------------------------
+// w[i] = (w[i] + dot(agx_i, x) c1) + dot(agy_i, x) c2; w[i] starts from 0.0
+// when init (the reference's "b = 0" followed by two cal_b calls).
+template <int NM>
+__device__ __forceinline__ void vol_pass(double *w, int e, const double (&x)[NM],
+                                         const double *AX, const double *AY, double c1,
+                                         double c2, bool init) {
+#pragma unroll 2
+  for (int i = 0; i < NM; i++) {
+    const size_t o = f64_at(e, i, NM);
+    double r = init ? 0.0 : w[o];
+    r = r + dot_col<NM>(AX + NM * i, x) * c1;
+    r = r + dot_col<NM>(AY + NM * i, x) * c2;
+    w[o] = r;
+  }
+}
+
+// w[i] = tri_f(w[i], ...) for the face values fb.
+template <int NM, int NGF>
+__device__ __forceinline__ void lift_pass(double *w, int e, const double (&fb)[3 * NGF],
+                                          const int (&its)[3], const double *P,
+                                          const double *__restrict__ wif, double coeff) {
+#pragma unroll 2
+  for (int i = 0; i < NM; i++) {
+    const size_t o = f64_at(e, i, NM);
+    w[o] = tri_f<NGF>(w[o], P + 3 * NGF * i, wif, fb, its, coeff);
+  }
+}
+
+// Solve the mass matrix for the output vector w and send it on: the RK4
+// stage update (emit) or, with store, back into w.
+template <int NM>
+__device__ __forceinline__ void solve_out(const F64Args &A, int c, int e, double *w,
+                                          const double *U, const double *dia, bool store) {
+  double g[NM];
+  vload(g, w, e);
+  dtrsm4_reg<NM, NM>(g, U, dia);
+  if (store)
+    vstore(w, e, g);
+  else
+    emit_vec(A, c, e, g);
+}
+
+// ---------------------------------------------------------------------------
+// Linear right-hand side (src/calvar_lin.f:115-282).  Wave 0: u tendency,
+// wave 1: v, wave 2: eta, each built in G[c].  Face outputs per face: fbdxx,
+// fbdyy (of fxx = -g0 h), fbdx, fbdy (of hh0 u, hh0 v); fxy = fyx = 0, whose
+// terms add only zeros in the reference.
 // ---------------------------------------------------------------------------
 template <int NC, int NG, int NGF>
-__global__ __launch_bounds__(NT) void elem_lin(DevMesh M, DevTables T, Physics P,
-                                               const double *__restrict__ u,
-                                               const double *__restrict__ v,
-                                               const double *__restrict__ h,
-                                               const double *__restrict__ face, StageOut so) {
-  using S = Shape<NC, NG, NGF>;
-  constexpr int NM = S::NM, NMP = S::NMP, E = S::E, NL = S::NL;
-  __shared__ double sf[3 * E * NMP];        // fxx, u0, v0
-  __shared__ double sg[3 * E * NMP];        // gx, gy, b (tendencies before the solve)
-  __shared__ double sfb[E * 3 * 4 * NGF];
-  __shared__ double svect[E * 5];
-  __shared__ int sits[E * 3], sbdl[E];
-  const int e0 = blockIdx.x * E, ne = M.ne;
-  {
-    const double *src[3] = {h, u, v};
-    load_fields<NM, NMP, E>(sf, src, 3, e0, ne);
-  }
-  load_faces<NGF, E, 4>(sfb, face, M.itm, e0, ne);
-  for (int idx = threadIdx.x; idx < E * 5; idx += NT)
-    svect[idx] = (e0 + idx / 5 < ne) ? M.vect[(size_t)e0 * 5 + idx] : 1.0;
-  for (int idx = threadIdx.x; idx < E * 3; idx += NT)
-    sits[idx] = (e0 + idx / 3 < ne) ? M.its[(size_t)e0 * 3 + idx] : 1;
-  for (int idx = threadIdx.x; idx < E; idx += NT) sbdl[idx] = (e0 + idx < ne) ? M.bdl[e0 + idx] : 0;
-  __syncthreads();
-  // fxx(j) = -g0 * h0(j) (src/calvar_lin.f:74-81)
-  for (int idx = threadIdx.x; idx < E * NM; idx += NT) {
-    int e = idx / NM, m = idx % NM;
-    sf[e * NMP + m] = -P.g0 * sf[e * NMP + m];
-  }
-  __syncthreads();
-  for (int idx = threadIdx.x; idx < E * NM; idx += NT) {
-    int e = idx / NM, i = idx % NM;
-    const double *vc = svect + 5 * e;
-    double ax = vc[0], bx = vc[1], ay = vc[2], by = vc[3], detai = vc[4];
-    const double *fx = sf + e * NMP;
-    const double *fb = sfb + e * 3 * 4 * NGF;
-    double Fx[3 * NGF], Fy[3 * NGF], Fu[3 * NGF], Fv[3 * NGF];
+__global__ __launch_bounds__(192) void elem_lin(F64Args A, const double *__restrict__ wif) {
+  constexpr int NM = Sh<NC, NG, NGF>::NM, NE3 = Sh<NC, NG, NGF>::NE3;
+  using L = OpsLds<NM, NGF>;
+  __shared__ double lds[L::SIZE];
+  stage_ops<NM, NGF>(lds, A.tab, 192);
+  const double *AX = lds + L::AX, *AY = lds + L::AY, *P = lds + L::P;
+  const int c = wave_id(), e = blockIdx.x * WB + (threadIdx.x & (WB - 1));
+  const Elem q = elem_of(A.mesh, e);
+  const double ax = q.ax, bx = q.bx, ay = q.ay, by = q.by, detai = q.detai;
+  double *w = pick(c, A.G[0], A.G[1], A.G[2]);
+  if (c < 2) {
+    double fx[NM];
+    vload(fx, A.x[2], e);
 #pragma unroll
-    for (int k = 0; k < 3; k++)
-#pragma unroll
-      for (int j = 0; j < NGF; j++) {
-        Fx[k * NGF + j] = fb[(k * 4 + 0) * NGF + j];
-        Fy[k * NGF + j] = fb[(k * 4 + 1) * NGF + j];
-        Fu[k * NGF + j] = fb[(k * 4 + 2) * NGF + j];
-        Fv[k * NGF + j] = fb[(k * 4 + 3) * NGF + j];
-      }
-    double sx = dot_col<NM>(T.agxT, i, fx), sy = dot_col<NM>(T.agyT, i, fx);
-    // u: cal_b(gx,gxx,agx,-by*detai), cal_b(gx,gxx,agy,ay*detai), tri_f x2
-    double gx = 0.0;
-    gx = gx + sx * (-by * detai);
-    gx = gx + sy * (ay * detai);
-    gx = tri_f<NM, NGF>(gx, T.picoT, T.wif, i, Fx, sits + 3 * e, detai);
-    // v: cal_b(gy,gyy,agx,-(-bx*detai)), cal_b(gy,gyy,agy,-(ax*detai)), tri_f
-    double gy = 0.0;
-    gy = gy + sx * (-(-bx * detai));
-    gy = gy + sy * (-(ax * detai));
-    gy = tri_f<NM, NGF>(gy, T.picoT, T.wif, i, Fy, sits + 3 * e, detai);
-    // h (src/calvar_lin.f:244-274)
-    const double H = P.hh0e[e0 + e < ne ? e0 + e : 0];
+    for (int m = 0; m < NM; m++) fx[m] = -A.g0 * fx[m];  // fxx (src/calvar_lin.f:74-81)
+    // gx: cal_b(agx, -by detai), cal_b(agy, ay detai); gy: cal_b(agx,
+    // -(-bx detai)), cal_b(agy, -(ax detai))
+    const double c1 = c == 0 ? -by * detai : -(-bx * detai);
+    const double c2 = c == 0 ? ay * detai : -(ax * detai);
+    vol_pass(w, e, fx, AX, AY, c1, c2, true);
+    double fb[NE3];
+    face_vals<NGF>(fb, A.face, 4, c, q.fk);
+    lift_pass<NM, NGF>(w, e, fb, q.its, P, wif, detai);
+  } else {
+    const double H = A.hh0e[q.valid ? e : 0];
     double ak[NM];
+    vload(ak, A.x[0], e);
 #pragma unroll
-    for (int j = 0; j < NM; j++) ak[j] = sf[(E + e) * NMP + j] * H;
-    double b = 0.0;
-    b = b + dot_col<NM>(T.agxT, i, ak) * (by * detai);
-    b = b + dot_col<NM>(T.agyT, i, ak) * (-ay * detai);
+    for (int m = 0; m < NM; m++) ak[m] = ak[m] * H;
+    vol_pass(w, e, ak, AX, AY, by * detai, -ay * detai, true);
+    vload(ak, A.x[1], e);
 #pragma unroll
-    for (int j = 0; j < NM; j++) ak[j] = sf[(2 * E + e) * NMP + j] * H;
-    b = b + dot_col<NM>(T.agxT, i, ak) * (-bx * detai);
-    b = b + dot_col<NM>(T.agyT, i, ak) * (ax * detai);
-    b = tri_f<NM, NGF>(b, T.picoT, T.wif, i, Fu, sits + 3 * e, -detai);
-    b = tri_f<NM, NGF>(b, T.picoT, T.wif, i, Fv, sits + 3 * e, -detai);
-    sg[e * NMP + i] = gx;
-    sg[(E + e) * NMP + i] = gy;
-    sg[(2 * E + e) * NMP + i] = b;
+    for (int m = 0; m < NM; m++) ak[m] = ak[m] * H;
+    vol_pass(w, e, ak, AX, AY, -bx * detai, ax * detai, false);
+    double fb[NE3];
+    face_vals<NGF>(fb, A.face, 4, 2, q.fk);
+    lift_pass<NM, NGF>(w, e, fb, q.its, P, wif, -detai);
+    face_vals<NGF>(fb, A.face, 4, 3, q.fk);
+    lift_pass<NM, NGF>(w, e, fb, q.its, P, wif, -detai);
   }
+  // The stage update below writes the next stage input in place of this
+  // stage's, which the block's other waves read above: wait for them.
   __syncthreads();
-  // Mass solves: one thread per (element, field); wall elements' velocity
-  // pair goes through the free-slip projection in a serial pass.
-  for (int idx = threadIdx.x; idx < 3 * E; idx += NT) {
-    int c = idx / E, e = idx % E;
-    if (c < 2 && sbdl[e]) continue;
-    double r[NM];
-    double *g = sg + (c * E + e) * NMP;
-#pragma unroll
-    for (int j = 0; j < NM; j++) r[j] = g[j];
-    dtrsm4_reg<NM, NM>(r, T.U, T.dia1);
-#pragma unroll
-    for (int j = 0; j < NM; j++) g[j] = r[j];
-  }
-  for (int e = threadIdx.x; e < E; e += NT)
-    if (sbdl[e]) {
-      const double *nt = M.normt + 2 * (size_t)(e0 + e);
-      wall_rot_solve<NC, NM, NL>(sg + e * NMP, sg + (E + e) * NMP, 1, nt[0], nt[1], T);
-    }
-  __syncthreads();
-  for (int idx = threadIdx.x; idx < 3 * E * NM; idx += NT) {
-    int c = idx / (E * NM), r = idx % (E * NM), e = r / NM, m = r % NM;
-    if (e0 + e >= ne) continue;
-    emit<NM>(so, c, (size_t)(e0 + e) * NM + m, sg[(c * E + e) * NMP + m]);
-  }
+  // the velocity pair of a wall element stays in G for wall_vel
+  if (!q.valid || (c < 2 && q.wall)) return;
+  solve_out<NM>(A, c, e, w, lds + L::UU, lds + L::DIA, false);
 }
 
 // ---------------------------------------------------------------------------
-// Nonlinear stage B + C (src/calvarnl_fs.f:115-332 and the ns/inv variants):
-// velocity gradient q with its wall projection, then the quadrature-point
-// products and the mass-flux solve.  Outputs per element: F = mu0*q - g0*h
-// (4 modal fields), the source terms -adu + w1 and -adv + z1, and uh, vh.
-// Face inputs: fbdxx, fbdxy, fbdyx, fbdyy of the velocity traces.
+// Nonlinear stage B (src/calvarnl_fs.f:115-204 and the ns/inv variants): the
+// velocity gradient q.  Wave r builds q component r (0 xx, 1 xy of u; 2 yx,
+// 3 yy of v) in Q[r] and, except for free-slip wall elements (wall_q),
+// solves it.  Face outputs: fbdxx, fbdxy, fbdyx, fbdyy of the velocity
+// traces.
 // ---------------------------------------------------------------------------
-struct BCOut {
-  double *F[4];
-  double *src[2];
-  double *uh[2];
+template <int NC, int NG, int NGF, int VAR>
+__global__ __launch_bounds__(256) void elem_q(F64Args A, const double *__restrict__ wif) {
+  constexpr int NM = Sh<NC, NG, NGF>::NM, NE3 = Sh<NC, NG, NGF>::NE3;
+  using L = OpsLds<NM, NGF>;
+  __shared__ double lds[L::SIZE];
+  stage_ops<NM, NGF>(lds, A.tab, 256);
+  const double *AX = lds + L::AX, *AY = lds + L::AY, *P = lds + L::P;
+  const int r = wave_id(), e = blockIdx.x * WB + (threadIdx.x & (WB - 1));
+  const Elem q = elem_of(A.mesh, e);
+  const double ax = q.ax, bx = q.bx, ay = q.ay, by = q.by, detai = q.detai;
+  double *w = r < 2 ? (r == 0 ? A.Q[0] : A.Q[1]) : (r == 2 ? A.Q[2] : A.Q[3]);
+  {
+    double a[NM];
+    vload(a, r < 2 ? A.x[0] : A.x[1], e);
+#pragma unroll 2
+    for (int i = 0; i < NM; i++) {
+      double gx1 = 0.0, gy1 = 0.0;
+      gx1 = gx1 + dot_col<NM>(AX + NM * i, a) * (-1.0);
+      gy1 = gy1 + dot_col<NM>(AY + NM * i, a) * (-1.0);
+      w[f64_at(e, i, NM)] =
+          (r & 1) ? (-bx * gx1 + ax * gy1) * detai : (by * gx1 - ay * gy1) * detai;
+    }
+  }
+  double fb[NE3];
+  face_vals<NGF>(fb, A.face, 4, r, q.fk);
+  lift_pass<NM, NGF>(w, e, fb, q.its, P, wif, detai);
+  if (!q.valid || (VAR == 1 && q.wall)) return;
+  solve_out<NM>(A, 0, e, w, lds + L::UU, lds + L::DIA, true);
+}
+
+// ---------------------------------------------------------------------------
+// Nonlinear stage C (src/calvarnl_fs.f:245-332; src/nonlin.f:35-133), one
+// lane per element: F = mu0 q - g0 h (src/calvarnl_fs.f:341-348); the nine
+// fields' values at the quadrature points (spectoxy); their products and
+// quotients, times the weights; the projections onto the modes (xytospec);
+// the sources -adu + w1, -adv + z1 and the mass fluxes uh, vh, which are
+// solved here except on wall elements (wall_c).  LDS: pi2 with row q holding
+// pi2(q, 0..NM-1), pi2 with row m holding pi2(0..NG-1, m), U, 1 / U_ii.
+// ---------------------------------------------------------------------------
+template <int NM, int NG>
+struct QuadLds {
+  static constexpr int PQ = 0, PM = NG * NM, UU = 2 * NG * NM, DIA = UU + NM * NM,
+                       SIZE = DIA + NM;
 };
 
 template <int NC, int NG, int NGF, int VAR>
-__global__ __launch_bounds__(NT) void elem_bc(DevMesh M, DevTables T, Physics P,
-                                              const double *__restrict__ u,
-                                              const double *__restrict__ v,
-                                              const double *__restrict__ h,
-                                              const double *__restrict__ hb,
-                                              const double *__restrict__ w0,
-                                              const double *__restrict__ z0,
-                                              const double *__restrict__ fc,
-                                              const double *__restrict__ face, BCOut out) {
-  using S = Shape<NC, NG, NGF>;
-  constexpr int NM = S::NM, NMP = S::NMP, E = S::E, NL = S::NL, NGP = S::NGP;
-  // modal fields: 0 u, 1 v, 2 qxx, 3 qxy, 4 qyx, 5 qyy, 6 h, 7 hb, 8 w0, 9 z0, 10 fc
-  __shared__ double sm[11 * E * NMP];
-  __shared__ double sq[8 * E * NGP];        // products at quadrature points
-  __shared__ double sfb[E * 3 * 4 * NGF];
-  __shared__ double svect[E * 5];
-  __shared__ int sits[E * 3], sbdl[E];
-  const int e0 = blockIdx.x * E, ne = M.ne;
-  {
-    const double *src[2] = {u, v};
-    load_fields<NM, NMP, E>(sm, src, 2, e0, ne);
-    const double *src2[5] = {h, hb, w0, z0, fc};
-    load_fields<NM, NMP, E>(sm + 6 * E * NMP, src2, 5, e0, ne);
+__global__ __launch_bounds__(256) void elem_c(F64Args A, const double *__restrict__ wi) {
+  constexpr int NM = Sh<NC, NG, NGF>::NM;
+  constexpr int MC = 4;  // modes per pass of the projections
+  using L = QuadLds<NM, NG>;
+  __shared__ double lds[L::SIZE];
+  for (int t = threadIdx.x; t < NG * NM; t += 256) {
+    lds[L::PQ + t] = A.tab.pi2T[t];  // pi2T[m + NM q] = pi2(q, m)
+    lds[L::PM + t] = A.tab.pi2[t];   // pi2[q + NG m]
   }
-  load_faces<NGF, E, 4>(sfb, face, M.itm, e0, ne);
-  for (int idx = threadIdx.x; idx < E * 5; idx += NT)
-    svect[idx] = (e0 + idx / 5 < ne) ? M.vect[(size_t)e0 * 5 + idx] : 1.0;
-  for (int idx = threadIdx.x; idx < E * 3; idx += NT)
-    sits[idx] = (e0 + idx / 3 < ne) ? M.its[(size_t)e0 * 3 + idx] : 1;
-  for (int idx = threadIdx.x; idx < E; idx += NT) sbdl[idx] = (e0 + idx < ne) ? M.bdl[e0 + idx] : 0;
+  for (int t = threadIdx.x; t < NM * NM; t += 256) lds[L::UU + t] = A.tab.U[t];
+  for (int t = threadIdx.x; t < NM; t += 256) lds[L::DIA + t] = A.tab.dia1[t];
   __syncthreads();
-  auto fld = [&](int c, int e) { return sm + (c * E + e) * NMP; };
-  // B: q = weak gradient of u and v (src/calvarnl_fs.f:129-158)
-  for (int idx = threadIdx.x; idx < 2 * E * NM; idx += NT) {
-    int c = idx / (E * NM), r = idx % (E * NM), e = r / NM, i = r % NM;
-    const double *vc = svect + 5 * e;
-    double ax = vc[0], bx = vc[1], ay = vc[2], by = vc[3], detai = vc[4];
-    const double *a = fld(c, e);
-    double gx1 = 0.0, gy1 = 0.0;
-    gx1 = gx1 + dot_col<NM>(T.agxT, i, a) * (-1.0);
-    gy1 = gy1 + dot_col<NM>(T.agyT, i, a) * (-1.0);
-    double gA = (by * gx1 - ay * gy1) * detai;
-    double gB = (-bx * gx1 + ax * gy1) * detai;
-    const double *fb = sfb + e * 3 * 4 * NGF;
-    double FA[3 * NGF], FB[3 * NGF];
-#pragma unroll
-    for (int k = 0; k < 3; k++)
-#pragma unroll
-      for (int j = 0; j < NGF; j++) {
-        FA[k * NGF + j] = fb[(k * 4 + 2 * c) * NGF + j];
-        FB[k * NGF + j] = fb[(k * 4 + 2 * c + 1) * NGF + j];
-      }
-    gA = tri_f<NM, NGF>(gA, T.picoT, T.wif, i, FA, sits + 3 * e, detai);
-    gB = tri_f<NM, NGF>(gB, T.picoT, T.wif, i, FB, sits + 3 * e, detai);
-    fld(2 + 2 * c, e)[i] = gA;      // qxx or qyx
-    fld(3 + 2 * c, e)[i] = gB;      // qxy or qyy
+  const double *PQ = lds + L::PQ, *PM = lds + L::PM;
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= A.mesh.nblk * WB) return;
+  const Elem q = elem_of(A.mesh, e);
+  const double mu0 = A.mu0, g0 = A.g0, mb0 = A.mb0;
+  // F (elementwise)
+#pragma unroll 4
+  for (int m = 0; m < NM; m++) {
+    const size_t i = f64_at(e, m, NM);
+    const double h0v = A.x[2][i];
+    A.F[0][i] = mu0 * A.Q[0][i] - g0 * h0v;
+    A.F[1][i] = mu0 * A.Q[1][i];
+    A.F[2][i] = mu0 * A.Q[2][i];
+    A.F[3][i] = mu0 * A.Q[3][i] - g0 * h0v;
   }
-  __syncthreads();
-  for (int idx = threadIdx.x; idx < 4 * E; idx += NT) {
-    int c = idx / E, e = idx % E;
-    if (VAR == 1 && sbdl[e]) continue;
-    double r[NM];
-    double *g = fld(2 + c, e);
-#pragma unroll
-    for (int j = 0; j < NM; j++) r[j] = g[j];
-    dtrsm4_reg<NM, NM>(r, T.U, T.dia1);
-#pragma unroll
-    for (int j = 0; j < NM; j++) g[j] = r[j];
-  }
-  if (VAR == 1) {
-    for (int e = threadIdx.x; e < E; e += NT)
-      if (sbdl[e]) {
-        const double *nt = M.normt + 2 * (size_t)(e0 + e);
-        wall_tensor_solve<NC, NM, NL>(fld(2, e), fld(3, e), fld(4, e), fld(5, e), nt[0], nt[1],
-                                      T);
-      }
-  }
-  __syncthreads();
-  // F = mu0*q - g0*h (src/calvarnl_fs.f:341-348); modal combinations of
-  // stage C (src/calvarnl_fs.f:245-255) overwrite h, hb, w0, z0 slots.
-  for (int idx = threadIdx.x; idx < E * NM; idx += NT) {
-    int e = idx / NM, m = idx % NM;
-    if (e0 + e < ne) {
-      size_t g = (size_t)(e0 + e) * NM + m;
-      double h0v = fld(6, e)[m];
-      out.F[0][g] = P.mu0 * fld(2, e)[m] - P.g0 * h0v;
-      out.F[1][g] = P.mu0 * fld(3, e)[m];
-      out.F[2][g] = P.mu0 * fld(4, e)[m];
-      out.F[3][g] = P.mu0 * fld(5, e)[m] - P.g0 * h0v;
+  // the nine fields at the quadrature points: u, v, gxx, gxy = qxy - fc,
+  // gyx = qyx + fc, gyy, hk = h + hb, wk = w0 - mb0 u, zk = z0 - mb0 v
+  for (int c = 0; c < 9; c++) {
+    // field c = X op Y: op 0 none, 1 minus, 2 plus, 3 minus mb0 times
+    const double *X, *Y = nullptr;
+    int op = 0;
+    switch (c) {
+      case 0: X = A.x[0]; break;
+      case 1: X = A.x[1]; break;
+      case 2: X = A.Q[0]; break;
+      case 3: X = A.Q[1]; Y = A.fc; op = 1; break;
+      case 4: X = A.Q[2]; Y = A.fc; op = 2; break;
+      case 5: X = A.Q[3]; break;
+      case 6: X = A.x[2]; Y = A.hb; op = 2; break;
+      case 7: X = A.w0; Y = A.x[0]; op = 3; break;
+      default: X = A.z0; Y = A.x[1]; op = 3; break;
     }
-  }
-  __syncthreads();
-  for (int idx = threadIdx.x; idx < E * NM; idx += NT) {
-    int e = idx / NM, m = idx % NM;
-    double uk = fld(0, e)[m], vk = fld(1, e)[m], f = fld(10, e)[m];
-    double hk = fld(6, e)[m] + fld(7, e)[m];
-    double wk = fld(8, e)[m] - P.mb0 * uk;
-    double zk = fld(9, e)[m] - P.mb0 * vk;
-    fld(3, e)[m] = fld(3, e)[m] - f;   // gxy = qxy - fc
-    fld(4, e)[m] = fld(4, e)[m] + f;   // gyx = qyx + fc
-    fld(6, e)[m] = hk;
-    fld(8, e)[m] = wk;
-    fld(9, e)[m] = zk;
-  }
-  __syncthreads();
-  // C: values at quadrature points and their products (src/nonlin.f:62-86,
-  // 111-133; src/calvarnl_fs.f:257-296)
-  for (int idx = threadIdx.x; idx < E * NG; idx += NT) {
-    int e = idx / NG, q = idx % NG;
-    double mf[9];
-    const int cs[9] = {0, 1, 2, 3, 4, 5, 6, 8, 9};
+    double v[NM];
+    vload(v, X, e);
+    if (op) {
+      double y[NM];
+      vload(y, Y, e);
+      if (op == 1) {
 #pragma unroll
-    for (int c = 0; c < 9; c++) mf[c] = 0.0;
-    for (int m = 0; m < NM; m++) {
-      double p = T.pi2[q + NG * m];
+        for (int m = 0; m < NM; m++) v[m] = v[m] - y[m];
+      } else if (op == 2) {
 #pragma unroll
-      for (int c = 0; c < 9; c++) mf[c] = mf[c] + fld(cs[c], e)[m] * p;
-    }
-    // mf: 0 u, 1 v, 2 gxx, 3 gxy, 4 gyx, 5 gyy, 6 hk, 7 wk, 8 zk
-    double *o = sq + e * NGP + q;
-    o[0 * E * NGP] = mf[0] * mf[2];   // uux
-    o[1 * E * NGP] = mf[1] * mf[3];   // vuy
-    o[2 * E * NGP] = mf[0] * mf[4];   // uvx
-    o[3 * E * NGP] = mf[1] * mf[5];   // vvy
-    o[4 * E * NGP] = mf[0] * mf[6];   // uhk
-    o[5 * E * NGP] = mf[1] * mf[6];   // vhk
-    o[6 * E * NGP] = mf[7] / mf[6];   // wk
-    o[7 * E * NGP] = mf[8] / mf[6];   // zk
-  }
-  __syncthreads();
-  // xytospec of the 8 products (src/nonlin.f:35-57, 91-107).  Results reuse
-  // LDS slots: 2 -> src_x, 3 -> src_y, 4 -> uhk, 5 -> vhk.
-  for (int idx = threadIdx.x; idx < E * NM; idx += NT) {
-    int e = idx / NM, m = idx % NM;
-    double r[8];
-#pragma unroll
-    for (int c = 0; c < 8; c++) r[c] = 0.0;
-    for (int q = 0; q < NG; q++) {
-      double p = T.pi2T[m + NM * q], w = T.wi[q];
-#pragma unroll
-      for (int c = 0; c < 8; c++) r[c] = r[c] + w * sq[(c * E + e) * NGP + q] * p;
-    }
-    double adu = r[0] + r[1], adv = r[2] + r[3];
-    fld(2, e)[m] = -adu + r[6];
-    fld(3, e)[m] = -adv + r[7];
-    fld(4, e)[m] = r[4];
-    fld(5, e)[m] = r[5];
-  }
-  __syncthreads();
-  for (int idx = threadIdx.x; idx < 2 * E; idx += NT) {
-    int c = idx / E, e = idx % E;
-    if (sbdl[e]) continue;
-    double r[NM];
-    double *g = fld(4 + c, e);
-#pragma unroll
-    for (int j = 0; j < NM; j++) r[j] = g[j];
-    dtrsm4_reg<NM, NM>(r, T.U, T.dia1);
-#pragma unroll
-    for (int j = 0; j < NM; j++) g[j] = r[j];
-  }
-  for (int e = threadIdx.x; e < E; e += NT)
-    if (sbdl[e]) {
-      if (VAR == 2) {
-        wall_red_solve<NC, NM, NL>(fld(4, e), 1, T);
-        wall_red_solve<NC, NM, NL>(fld(5, e), 1, T);
+        for (int m = 0; m < NM; m++) v[m] = v[m] + y[m];
       } else {
-        const double *nt = M.normt + 2 * (size_t)(e0 + e);
-        wall_rot_solve<NC, NM, NL>(fld(4, e), fld(5, e), 1, nt[0], nt[1], T);
+#pragma unroll
+        for (int m = 0; m < NM; m++) v[m] = v[m] - mb0 * y[m];
       }
     }
-  __syncthreads();
-  for (int idx = threadIdx.x; idx < 4 * E * NM; idx += NT) {
-    int c = idx / (E * NM), r = idx % (E * NM), e = r / NM, m = r % NM;
-    if (e0 + e >= ne) continue;
-    size_t g = (size_t)(e0 + e) * NM + m;
-    double val = fld(2 + c, e)[m];
-    if (c < 2) out.src[c][g] = val;
-    else out.uh[c - 2][g] = val;
+#pragma unroll 2
+    for (int qq = 0; qq < NG; qq++) {
+      const double *row = PQ + NM * qq;
+      double mf = 0.0;
+#pragma unroll
+      for (int m = 0; m < NM; m++) mf = mf + v[m] * row[m];
+      A.MF[f64_at(e, c * NG + qq, 9 * NG)] = mf;
+    }
+  }
+  // products at the points, times the weights
+#pragma unroll 3
+  for (int qq = 0; qq < NG; qq++) {
+    double mf[9];
+#pragma unroll
+    for (int c = 0; c < 9; c++) mf[c] = A.MF[f64_at(e, c * NG + qq, 9 * NG)];
+    double o[8];
+    o[0] = mf[0] * mf[2];  // uux
+    o[1] = mf[1] * mf[3];  // vuy
+    o[2] = mf[0] * mf[4];  // uvx
+    o[3] = mf[1] * mf[5];  // vvy
+    o[4] = mf[0] * mf[6];  // uhk
+    o[5] = mf[1] * mf[6];  // vhk
+    o[6] = mf[7] / mf[6];  // wk
+    o[7] = mf[8] / mf[6];  // zk
+    const double w = wi[qq];
+#pragma unroll
+    for (int c = 0; c < 8; c++) A.WS[f64_at(e, c * NG + qq, 8 * NG)] = w * o[c];
+  }
+  // projections onto the modes, MC modes at a time
+  for (int m0 = 0; m0 < NM; m0 += MC) {
+    double r[8][MC];
+#pragma unroll
+    for (int c = 0; c < 8; c++)
+#pragma unroll
+      for (int mm = 0; mm < MC; mm++) r[c][mm] = 0.0;
+#pragma unroll 3
+    for (int qq = 0; qq < NG; qq++) {
+      double ws[8];
+#pragma unroll
+      for (int c = 0; c < 8; c++) ws[c] = A.WS[f64_at(e, c * NG + qq, 8 * NG)];
+#pragma unroll
+      for (int mm = 0; mm < MC; mm++) {
+        const double p = PQ[(m0 + mm < NM ? m0 + mm : NM - 1) + NM * qq];
+#pragma unroll
+        for (int c = 0; c < 8; c++) r[c][mm] = r[c][mm] + ws[c] * p;
+      }
+    }
+#pragma unroll
+    for (int mm = 0; mm < MC; mm++)
+      if (m0 + mm < NM) {
+        const size_t i = f64_at(e, m0 + mm, NM);
+        const double adu = r[0][mm] + r[1][mm], adv = r[2][mm] + r[3][mm];
+        A.S[0][i] = -adu + r[6][mm];
+        A.S[1][i] = -adv + r[7][mm];
+        A.UH[0][i] = r[4][mm];
+        A.UH[1][i] = r[5][mm];
+      }
+  }
+  (void)PM;
+  if (q.wall) return;  // wall_c
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    double g[NM];
+    vload(g, c == 0 ? A.UH[0] : A.UH[1], e);
+    dtrsm4_reg<NM, NM>(g, lds + L::UU, lds + L::DIA);
+    vstore(c == 0 ? A.UH[0] : A.UH[1], e, g);
   }
 }
 
 // ---------------------------------------------------------------------------
-// Nonlinear stages E + G (src/calvarnl_fs.f:382-544): u, v tendencies from
-// the stress tensor, sources and wall projection; eta tendency from the mass
-// fluxes.  Face inputs per face: fbdxx, fbdxy, fbdyx, fbdyy (of F) and fbdx,
-// fbdy (of uh, vh).
+// Nonlinear stages E + G (src/calvarnl_fs.f:382-544): waves 0 and 1 build the
+// u and v tendencies from the stress tensor F and the sources, wave 2 the eta
+// tendency from the mass fluxes, in G[c]; then the solve and the RK4 stage
+// update, except for the velocity pair of wall elements (wall_vel).  Face
+// outputs per face: fbdxx, fbdxy, fbdyx, fbdyy (of F) and fbdx, fbdy (of uh,
+// vh).
 // ---------------------------------------------------------------------------
 template <int NC, int NG, int NGF, int VAR>
-__global__ __launch_bounds__(NT) void elem_eg(DevMesh M, DevTables T, Physics P,
-                                              const double *const *__restrict__ Fp,
-                                              const double *__restrict__ srcx,
-                                              const double *__restrict__ srcy,
-                                              const double *__restrict__ uh,
-                                              const double *__restrict__ vh,
-                                              const double *__restrict__ face, StageOut so) {
-  using S = Shape<NC, NG, NGF>;
-  constexpr int NM = S::NM, NMP = S::NMP, E = S::E, NL = S::NL;
-  // 0..3 F, 4 uh, 5 vh ; outputs 0 gx, 1 gy, 2 b
-  __shared__ double sm[6 * E * NMP];
-  __shared__ double sg[3 * E * NMP];
-  __shared__ double sfb[E * 3 * 6 * NGF];
-  __shared__ double svect[E * 5];
-  __shared__ int sits[E * 3], sbdl[E];
-  const int e0 = blockIdx.x * E, ne = M.ne;
+__global__ __launch_bounds__(192) void elem_eg(F64Args A, const double *__restrict__ wif) {
+  constexpr int NM = Sh<NC, NG, NGF>::NM, NE3 = Sh<NC, NG, NGF>::NE3;
+  using L = OpsLds<NM, NGF>;
+  __shared__ double lds[L::SIZE];
+  stage_ops<NM, NGF>(lds, A.tab, 192);
+  const double *AX = lds + L::AX, *AY = lds + L::AY, *P = lds + L::P;
+  const int c = wave_id(), e = blockIdx.x * WB + (threadIdx.x & (WB - 1));
+  const Elem q = elem_of(A.mesh, e);
+  const double ax = q.ax, bx = q.bx, ay = q.ay, by = q.by, detai = q.detai;
+  double *w = pick(c, A.G[0], A.G[1], A.G[2]);
+  // the volume passes: u, v (gx = -adu + w1, then cal_b x4,
+  // src/calvarnl_fs.f:393-429) or eta (b = 0, cal_b x4,
+  // src/calvarnl_fs.f:509-536)
+  const double *A1 = pick<const double *>(c, A.F[0], A.F[2], A.UH[0]);
+  const double *A2 = pick<const double *>(c, A.F[1], A.F[3], A.UH[1]);
+  const double c1 = c < 2 ? -by * detai : by * detai, c2 = c < 2 ? ay * detai : -ay * detai;
+  const double c3 = c < 2 ? bx * detai : -bx * detai, c4 = c < 2 ? -ax * detai : ax * detai;
+  if (c < 2) {
+    const double *src = c == 0 ? A.S[0] : A.S[1];
+#pragma unroll 4
+    for (int i = 0; i < NM; i++) w[f64_at(e, i, NM)] = src[f64_at(e, i, NM)];
+  }
   {
-    const double *src[6] = {Fp[0], Fp[1], Fp[2], Fp[3], uh, vh};
-    load_fields<NM, NMP, E>(sm, src, 6, e0, ne);
-    const double *src2[2] = {srcx, srcy};
-    load_fields<NM, NMP, E>(sg, src2, 2, e0, ne);
+    double a[NM];
+    vload(a, A1, e);
+    vol_pass(w, e, a, AX, AY, c1, c2, c == 2);
+    vload(a, A2, e);
+    vol_pass(w, e, a, AX, AY, c3, c4, false);
   }
-  load_faces<NGF, E, 6>(sfb, face, M.itm, e0, ne);
-  for (int idx = threadIdx.x; idx < E * 5; idx += NT)
-    svect[idx] = (e0 + idx / 5 < ne) ? M.vect[(size_t)e0 * 5 + idx] : 1.0;
-  for (int idx = threadIdx.x; idx < E * 3; idx += NT)
-    sits[idx] = (e0 + idx / 3 < ne) ? M.its[(size_t)e0 * 3 + idx] : 1;
-  for (int idx = threadIdx.x; idx < E; idx += NT) sbdl[idx] = (e0 + idx < ne) ? M.bdl[e0 + idx] : 0;
-  __syncthreads();
-  auto fld = [&](int c, int e) { return sm + (c * E + e) * NMP; };
-  for (int idx = threadIdx.x; idx < 3 * E * NM; idx += NT) {
-    int c = idx / (E * NM), r = idx % (E * NM), e = r / NM, i = r % NM;
-    const double *vc = svect + 5 * e;
-    double ax = vc[0], bx = vc[1], ay = vc[2], by = vc[3], detai = vc[4];
-    const double *fb = sfb + e * 3 * 6 * NGF;
-    double FA[3 * NGF], FB[3 * NGF];
-    int oa = (c == 2) ? 4 : 2 * c, ob = oa + 1;
-#pragma unroll
-    for (int k = 0; k < 3; k++)
-#pragma unroll
-      for (int j = 0; j < NGF; j++) {
-        FA[k * NGF + j] = fb[(k * 6 + oa) * NGF + j];
-        FB[k * NGF + j] = fb[(k * 6 + ob) * NGF + j];
-      }
-    double g;
-    if (c < 2) {
-      // gx = -adu + w1 ; cal_b x4 ; tri_f x2  (src/calvarnl_fs.f:393-429)
-      const double *A = fld(2 * c, e), *B = fld(2 * c + 1, e);
-      g = sg[(c * E + e) * NMP + i];
-      g = g + dot_col<NM>(T.agxT, i, A) * (-by * detai);
-      g = g + dot_col<NM>(T.agyT, i, A) * (ay * detai);
-      g = g + dot_col<NM>(T.agxT, i, B) * (bx * detai);
-      g = g + dot_col<NM>(T.agyT, i, B) * (-ax * detai);
-      g = tri_f<NM, NGF>(g, T.picoT, T.wif, i, FA, sits + 3 * e, detai);
-      g = tri_f<NM, NGF>(g, T.picoT, T.wif, i, FB, sits + 3 * e, detai);
-    } else {
-      // eta (src/calvarnl_fs.f:509-536)
-      const double *A = fld(4, e), *B = fld(5, e);
-      g = 0.0;
-      g = g + dot_col<NM>(T.agxT, i, A) * (by * detai);
-      g = g + dot_col<NM>(T.agyT, i, A) * (-ay * detai);
-      g = g + dot_col<NM>(T.agxT, i, B) * (-bx * detai);
-      g = g + dot_col<NM>(T.agyT, i, B) * (ax * detai);
-      g = tri_f<NM, NGF>(g, T.picoT, T.wif, i, FA, sits + 3 * e, -detai);
-      g = tri_f<NM, NGF>(g, T.picoT, T.wif, i, FB, sits + 3 * e, -detai);
-    }
-    sg[(c * E + e) * NMP + i] = g;  // only this thread reads or writes this slot
-  }
-  __syncthreads();
-  for (int idx = threadIdx.x; idx < 3 * E; idx += NT) {
-    int c = idx / E, e = idx % E;
-    if (c < 2 && sbdl[e]) continue;
-    double r[NM];
-    double *g = sg + (c * E + e) * NMP;
-#pragma unroll
-    for (int j = 0; j < NM; j++) r[j] = g[j];
-    dtrsm4_reg<NM, NM>(r, T.U, T.dia1);
-#pragma unroll
-    for (int j = 0; j < NM; j++) g[j] = r[j];
-  }
-  for (int e = threadIdx.x; e < E; e += NT)
-    if (sbdl[e]) {
-      if (VAR == 2) {
-        wall_red_solve<NC, NM, NL>(sg + e * NMP, 1, T);
-        wall_red_solve<NC, NM, NL>(sg + (E + e) * NMP, 1, T);
-      } else {
-        const double *nt = M.normt + 2 * (size_t)(e0 + e);
-        wall_rot_solve<NC, NM, NL>(sg + e * NMP, sg + (E + e) * NMP, 1, nt[0], nt[1], T);
-      }
-    }
-  __syncthreads();
-  for (int idx = threadIdx.x; idx < 3 * E * NM; idx += NT) {
-    int c = idx / (E * NM), r = idx % (E * NM), e = r / NM, m = r % NM;
-    if (e0 + e >= ne) continue;
-    emit<NM>(so, c, (size_t)(e0 + e) * NM + m, sg[(c * E + e) * NMP + m]);
-  }
+  // the face lifts: F's faces with detai, or the mass fluxes' with -detai
+  const double cf = c < 2 ? detai : -detai;
+  double fb[NE3];
+  face_vals<NGF>(fb, A.face, 6, c < 2 ? 2 * c : 4, q.fk);
+  lift_pass<NM, NGF>(w, e, fb, q.its, P, wif, cf);
+  face_vals<NGF>(fb, A.face, 6, c < 2 ? 2 * c + 1 : 5, q.fk);
+  lift_pass<NM, NGF>(w, e, fb, q.its, P, wif, cf);
+  if (!q.valid || (c < 2 && q.wall)) return;  // (wall_vel)
+  solve_out<NM>(A, c, e, w, lds + L::UU, lds + L::DIA, false);
 }
 
 // ---------------------------------------------------------------------------
-// Host launchers.
+// Wall elements, one thread each: the reference's projection sequences on
+// the raw vectors the element kernels left.
+// ---------------------------------------------------------------------------
+template <int NM>
+__device__ void gload(double *v, const double *p, int e) {
+  for (int m = 0; m < NM; m++) v[m] = p[f64_at(e, m, NM)];
+}
+template <int NM>
+__device__ void gstore(double *p, int e, const double *v) {
+  for (int m = 0; m < NM; m++) p[f64_at(e, m, NM)] = v[m];
+}
+
+// the velocity pair of the linear variant and of E+G: free slip (lin, fs,
+// inv) or no slip (ns), then the RK4 stage update
+template <int NC, int NG, int NGF, int VAR>
+__global__ __launch_bounds__(64) void wall_vel(F64Args A, DevTables T) {
+  constexpr int NM = Sh<NC, NG, NGF>::NM, NL = Sh<NC, NG, NGF>::NL;
+  const int t = blockIdx.x * 64 + threadIdx.x;
+  if (t >= A.mesh.nwall) return;
+  const int e = A.mesh.wall[t];
+  double gx[NM], gy[NM];
+  gload<NM>(gx, A.G[0], e);
+  gload<NM>(gy, A.G[1], e);
+  if (VAR == 2) {
+    wall_red_solve<NC, NM, NL>(gx, T);
+    wall_red_solve<NC, NM, NL>(gy, T);
+  } else {
+    const double nx = A.mesh.geo[f64_at(e, G_NX, G_ROWS)], ny = A.mesh.geo[f64_at(e, G_NY, G_ROWS)];
+    wall_rot_solve<NC, NM, NL>(gx, gy, nx, ny, T);
+  }
+  const Out o0 = out_of(A, 0), o1 = out_of(A, 1);
+  for (int m = 0; m < NM; m++) {
+    emit(A, o0, f64_at(e, m, NM), gx[m]);
+    emit(A, o1, f64_at(e, m, NM), gy[m]);
+  }
+}
+
+// the velocity-gradient tensor of free-slip wall elements (stage B)
+template <int NC, int NG, int NGF>
+__global__ __launch_bounds__(64) void wall_q(F64Args A, DevTables T) {
+  constexpr int NM = Sh<NC, NG, NGF>::NM, NL = Sh<NC, NG, NGF>::NL;
+  const int t = blockIdx.x * 64 + threadIdx.x;
+  if (t >= A.mesh.nwall) return;
+  const int e = A.mesh.wall[t];
+  double q[4][NM];
+  for (int c = 0; c < 4; c++) gload<NM>(q[c], A.Q[c], e);
+  const double nx = A.mesh.geo[f64_at(e, G_NX, G_ROWS)], ny = A.mesh.geo[f64_at(e, G_NY, G_ROWS)];
+  wall_tensor_solve<NC, NM, NL>(q[0], q[1], q[2], q[3], nx, ny, T);
+  for (int c = 0; c < 4; c++) gstore<NM>(A.Q[c], e, q[c]);
+}
+
+// the mass fluxes of wall elements (stage C)
+template <int NC, int NG, int NGF, int VAR>
+__global__ __launch_bounds__(64) void wall_c(F64Args A, DevTables T) {
+  constexpr int NM = Sh<NC, NG, NGF>::NM, NL = Sh<NC, NG, NGF>::NL;
+  const int t = blockIdx.x * 64 + threadIdx.x;
+  if (t >= A.mesh.nwall) return;
+  const int e = A.mesh.wall[t];
+  double u[NM], v[NM];
+  gload<NM>(u, A.UH[0], e);
+  gload<NM>(v, A.UH[1], e);
+  if (VAR == 2) {
+    wall_red_solve<NC, NM, NL>(u, T);
+    wall_red_solve<NC, NM, NL>(v, T);
+  } else {
+    const double nx = A.mesh.geo[f64_at(e, G_NX, G_ROWS)], ny = A.mesh.geo[f64_at(e, G_NY, G_ROWS)];
+    wall_rot_solve<NC, NM, NL>(u, v, nx, ny, T);
+  }
+  gstore<NM>(A.UH[0], e, u);
+  gstore<NM>(A.UH[1], e, v);
+}
+
+// ---------------------------------------------------------------------------
+// Layout changes between the packed (NM, ne) arrays of the C-ABI and the
+// blocked arrays of this path (padding elements read as zero).
+// ---------------------------------------------------------------------------
+template <int NM>
+__global__ __launch_bounds__(256) void to_blocked(int ne, int nblk, const double *__restrict__ in,
+                                                  double *__restrict__ out) {
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (size_t)nblk * NM * WB) return;
+  const int c = (int)(idx % WB), r = (int)((idx / WB) % NM), b = (int)(idx / ((size_t)WB * NM));
+  const int e = b * WB + c;
+  out[idx] = e < ne ? in[(size_t)e * NM + r] : 0.0;
+}
+
+template <int NM>
+__global__ __launch_bounds__(256) void from_blocked(int ne, const double *__restrict__ in,
+                                                    double *__restrict__ out) {
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (size_t)ne * NM) return;
+  const int e = (int)(idx / NM), r = (int)(idx % NM);
+  out[idx] = in[f64_at(e, r, NM)];
+}
+
+// ---------------------------------------------------------------------------
+// Host launcher.
 // ---------------------------------------------------------------------------
 template <int NC, int NG, int NGF>
-int launch_rhs(const RhsArgs &a) {
-  using S = Shape<NC, NG, NGF>;
-  constexpr int NM = S::NM, E = S::E;
-  const DevMesh &M = a.mesh;
-  int nfb = (M.nf + NT - 1) / NT, neb = (M.ne + E - 1) / E;
+int launch(const F64Args &a, int phase, const double *in, double *out) {
+  constexpr int NM = nm_of(NC);
+  const F64Mesh &M = a.mesh;
   hipStream_t st = a.stream;
-  Physics P = a.phys;
-  StageOut so;
-  for (int c = 0; c < 3; c++) {
-    so.k[c] = a.k[c];
-    so.u0[c] = a.u0[c];
-    so.x[c] = a.xout[c];
-    so.acc[c] = a.acc[c];
+  if (phase == 0) {
+    const size_t n = (size_t)M.nblk * NM * WB;
+    to_blocked<NM><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(M.ne, M.nblk, in, out);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
   }
-  so.stage = a.stage;
-  so.dt = a.dt;
-  so.dt6 = a.dt / 6.0;
+  if (phase == 1) {
+    const size_t n = (size_t)M.ne * NM;
+    from_blocked<NM><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(M.ne, in, out);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+  }
+  const DevTables &T = a.tab;
+  const dim3 gf2((M.nf + 255) / 256, 2), gf3((M.nf + 255) / 256, 3), gf6((M.nf + 255) / 256, 6);
+  const unsigned nb = (unsigned)M.nblk, nw = (unsigned)((M.nwall + 63) / 64);
+  const unsigned nc4 = (unsigned)((M.nblk * WB + 255) / 256);
   if (a.variant == 0) {
     FaceSpec fs = {};
     fs.in[0] = a.x[2];
-    fs.pre[0] = -P.g0;
+    fs.pre[0] = -a.g0;
     fs.haspre[0] = 1;
     fs.in[1] = a.x[0];
     fs.in[2] = a.x[1];
     fs.nout = 4;
-    int src[4] = {0, 0, 1, 2}, comp[4] = {0, 1, 0, 1};
-    for (int o = 0; o < 4; o++) {
-      fs.src[o] = src[o];
-      fs.comp[o] = comp[o];
-      fs.post[o] = P.hh0;
-      fs.haspost[o] = o >= 2;
-    }
-    fs.hpost = P.hh0e;
-    a.timer(0, st);
-    face_kernel<NC, NGF, 3><<<nfb, NT, 0, st>>>(M.nf, M.tseg, M.ssit, M.norm, a.tab.pico, fs,
-                                                  a.face);
-    a.timer(1, st);
-    elem_lin<NC, NG, NGF><<<neb, NT, 0, st>>>(M, a.tab, P, a.x[0], a.x[1], a.x[2], a.face, so);
-    a.timer(2, st);
+    fs.nof[0] = 2;
+    fs.out[0][0] = 0;
+    fs.comp[0][0] = 0;
+    fs.out[0][1] = 1;
+    fs.comp[0][1] = 1;
+    fs.nof[1] = 1;
+    fs.out[1][0] = 2;
+    fs.comp[1][0] = 0;
+    fs.haspost[1][0] = 1;
+    fs.nof[2] = 1;
+    fs.out[2][0] = 3;
+    fs.comp[2][0] = 1;
+    fs.haspost[2][0] = 1;
+    fs.hpost = a.hh0e;
+    a.timer(0);
+    face_kernel<NC, NGF><<<gf3, 256, 0, st>>>(M.nf, M.tseg, M.ssit, M.norm, T.pico, fs, a.face);
+    a.timer(1);
+    elem_lin<NC, NG, NGF><<<nb, 192, 0, st>>>(a, T.wif);
+    a.timer(2);
+    if (nw) wall_vel<NC, NG, NGF, 0><<<nw, 64, 0, st>>>(a, T);
+    a.timer(3);
     return hipGetLastError() == hipSuccess ? 0 : -2;
   }
-  // nonlinear
   FaceSpec fa = {};
   fa.in[0] = a.x[0];
   fa.in[1] = a.x[1];
   fa.nout = 4;
-  {
-    int src[4] = {0, 0, 1, 1}, comp[4] = {0, 1, 0, 1};
-    for (int o = 0; o < 4; o++) {
-      fa.src[o] = src[o];
-      fa.comp[o] = comp[o];
-    }
+  for (int c = 0; c < 2; c++) {
+    fa.nof[c] = 2;
+    fa.out[c][0] = 2 * c;
+    fa.comp[c][0] = 0;
+    fa.out[c][1] = 2 * c + 1;
+    fa.comp[c][1] = 1;
   }
-  a.timer(0, st);
-  face_kernel<NC, NGF, 2><<<nfb, NT, 0, st>>>(M.nf, M.tseg, M.ssit, M.norm, a.tab.pico, fa,
-                                                a.face);
-  BCOut bo;
-  for (int c = 0; c < 4; c++) bo.F[c] = a.work[c];
-  bo.src[0] = a.work[4];
-  bo.src[1] = a.work[5];
-  bo.uh[0] = a.work[6];
-  bo.uh[1] = a.work[7];
-  a.timer(1, st);
-#define SPOC_BC(V)                                                                        \
-  elem_bc<NC, NG, NGF, V><<<neb, NT, 0, st>>>(M, a.tab, P, a.x[0], a.x[1], a.x[2], a.hb, \
-                                              a.w0, a.z0, a.fc, a.face, bo)
-  if (a.variant == 1) SPOC_BC(1);
-  else if (a.variant == 2) SPOC_BC(2);
-  else SPOC_BC(3);
-#undef SPOC_BC
+  a.timer(0);
+  face_kernel<NC, NGF><<<gf2, 256, 0, st>>>(M.nf, M.tseg, M.ssit, M.norm, T.pico, fa, a.face);
+  a.timer(1);
+#define SPOC_V(V, STMT) \
+  if (a.variant == V) { STMT; }
+#define SPOC_Q(V) elem_q<NC, NG, NGF, V><<<nb, 256, 0, st>>>(a, T.wif)
+  SPOC_V(1, SPOC_Q(1)) SPOC_V(2, SPOC_Q(2)) SPOC_V(3, SPOC_Q(3))
+  a.timer(2);
+  if (a.variant == 1 && nw) wall_q<NC, NG, NGF><<<nw, 64, 0, st>>>(a, T);
+  a.timer(3);
+#define SPOC_C(V) elem_c<NC, NG, NGF, V><<<nc4, 256, 0, st>>>(a, T.wi)
+  SPOC_V(1, SPOC_C(1)) SPOC_V(2, SPOC_C(2)) SPOC_V(3, SPOC_C(3))
+  a.timer(4);
+  if (nw) {
+    SPOC_V(1, (wall_c<NC, NG, NGF, 1><<<nw, 64, 0, st>>>(a, T)))
+    SPOC_V(2, (wall_c<NC, NG, NGF, 2><<<nw, 64, 0, st>>>(a, T)))
+    SPOC_V(3, (wall_c<NC, NG, NGF, 3><<<nw, 64, 0, st>>>(a, T)))
+  }
   FaceSpec f2 = {};
+  const double *fin[6] = {a.F[0], a.F[1], a.F[2], a.F[3], a.UH[0], a.UH[1]};
   for (int c = 0; c < 6; c++) {
-    f2.in[c] = a.work[c < 4 ? c : c + 2];
-    f2.src[c] = c;
-    f2.comp[c] = (c == 0 || c == 2 || c == 4) ? 0 : 1;
+    f2.in[c] = fin[c];
+    f2.nof[c] = 1;
+    f2.out[c][0] = c;
+    f2.comp[c][0] = c & 1;
   }
   f2.nout = 6;
-  a.timer(2, st);
-  face_kernel<NC, NGF, 6><<<nfb, NT, 0, st>>>(M.nf, M.tseg, M.ssit, M.norm, a.tab.pico, f2,
-                                                a.face);
-  a.timer(3, st);
-#define SPOC_EG(V)                                                                         \
-  elem_eg<NC, NG, NGF, V><<<neb, NT, 0, st>>>(M, a.tab, P, a.Fptr, a.work[4], a.work[5], \
-                                              a.work[6], a.work[7], a.face, so)
-  if (a.variant == 1) SPOC_EG(1);
-  else if (a.variant == 2) SPOC_EG(2);
-  else SPOC_EG(3);
+  a.timer(5);
+  face_kernel<NC, NGF><<<gf6, 256, 0, st>>>(M.nf, M.tseg, M.ssit, M.norm, T.pico, f2, a.face);
+  a.timer(6);
+#define SPOC_EG(V) elem_eg<NC, NG, NGF, V><<<nb, 192, 0, st>>>(a, T.wif)
+  SPOC_V(1, SPOC_EG(1)) SPOC_V(2, SPOC_EG(2)) SPOC_V(3, SPOC_EG(3))
+  a.timer(7);
+  if (nw) {
+    SPOC_V(1, (wall_vel<NC, NG, NGF, 1><<<nw, 64, 0, st>>>(a, T)))
+    SPOC_V(2, (wall_vel<NC, NG, NGF, 2><<<nw, 64, 0, st>>>(a, T)))
+    SPOC_V(3, (wall_vel<NC, NG, NGF, 3><<<nw, 64, 0, st>>>(a, T)))
+  }
+  a.timer(8);
+#undef SPOC_V
+#undef SPOC_Q
+#undef SPOC_C
 #undef SPOC_EG
-  a.timer(4, st);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -850,11 +978,11 @@ int launch_rhs(const RhsArgs &a) {
 // One translation unit per shape (the build compiles this file once per
 // shape with SPOC_NC/SPOC_NG/SPOC_NGF defined) keeps compile times parallel.
 #if defined(SPOC_NC)
-int rhs_f64_shape(const RhsArgs &a, int nc, int ng, int ngf);
 #define SPOC_CAT3(a, b, c, d) a##_##b##_##c##_##d
 #define SPOC_NAME(a, b, c, d) SPOC_CAT3(a, b, c, d)
-int SPOC_NAME(rhs_f64, SPOC_NC, SPOC_NG, SPOC_NGF)(const RhsArgs &a) {
-  return f64::launch_rhs<SPOC_NC, SPOC_NG, SPOC_NGF>(a);
+int SPOC_NAME(rhs_f64, SPOC_NC, SPOC_NG, SPOC_NGF)(const F64Args &a, int phase, const double *in,
+                                                   double *out) {
+  return f64::launch<SPOC_NC, SPOC_NG, SPOC_NGF>(a, phase, in, out);
 }
 #endif
 
