@@ -142,40 +142,32 @@ __device__ __forceinline__ void dtrsm4_reg(double (&b)[M], const double *__restr
   }
 }
 
-// The same solve on a private vector of m entries (wall path).
-template <int LD>
-__device__ void dtrsm4_mem(int m, double *b, const double *__restrict__ U,
-                           const double *__restrict__ dia) {
-  for (int i = 0; i < m; i++) {
-    double temp = b[i];
-    for (int k = 0; k < i; k++) temp = temp - U[k + LD * i] * b[k];
-    b[i] = temp * dia[i];
-  }
-  for (int k = m - 1; k >= 0; k--) {
-    b[k] = b[k] * dia[k];
-    for (int i = 0; i < k; i++) b[i] = b[i] - b[k] * U[i + LD * k];
-  }
-}
-
 // Projection onto the zero-on-edge-1 basis (src/mat1.f:418-439).
-template <int NC>
-__device__ void reduc_gn(const double *gn, double *gnr) {
+template <int NC, int NL>
+__device__ __forceinline__ void reduc_gn(const double *gn, double (&gnr)[NL]) {
+#pragma unroll
   for (int j = 0; j <= NC - 1; j++) {
-    int c1 = ip_of(NC, j), c2 = ip_of(NC - 1, j);
+    const int c1 = ip_of(NC, j), c2 = ip_of(NC - 1, j);
+#pragma unroll
     for (int l = 0; l <= NC - 1 - j; l += 2) gnr[c2 + l] = gn[c1 + l + 1] + gn[c1];
+#pragma unroll
     for (int l = 1; l <= NC - 1 - j; l += 2) gnr[c2 + l] = gn[c1 + l + 1] - gn[c1];
   }
 }
 
 // Expansion back to the full basis (src/mat1.f:445-473).
-template <int NC>
-__device__ void reduc_inv_gn(double *gn, const double *gnr) {
+template <int NC, int NL>
+__device__ __forceinline__ void reduc_inv_gn(double *gn, const double (&gnr)[NL]) {
+#pragma unroll
   for (int j = 0; j <= NC - 1; j++) {
-    int c1 = ip_of(NC, j), c2 = ip_of(NC - 1, j);
+    const int c1 = ip_of(NC, j), c2 = ip_of(NC - 1, j);
     double s = 0.0;
+#pragma unroll
     for (int l = 0; l <= NC - j - 1; l += 2) s = s + gnr[c2 + l];
+#pragma unroll
     for (int l = 1; l <= NC - j - 1; l += 2) s = s - gnr[c2 + l];
     gn[c1] = s;
+#pragma unroll
     for (int l = 0; l <= NC - j - 1; l++) gn[c1 + l + 1] = gnr[c2 + l];
   }
   gn[nm_of(NC) - 1] = 0.0;
@@ -184,58 +176,72 @@ __device__ void reduc_inv_gn(double *gn, const double *gnr) {
 // Free-slip projection of a vector field (gx, gy) on a wall element: rotate
 // to (n, t), solve the normal part in the reduced basis and the tangential
 // part with the full mass matrix, rotate back (src/calvarnl_fs.f:447-459,
-// src/calvar_lin.f:184-197).
+// src/calvar_lin.f:184-197).  In place; U, UL: the factors, column-major
+// with leading dimension NM.
 template <int NC, int NM, int NL>
-__device__ void wall_rot_solve(double *gx, double *gy, double dnx, double dny,
-                               const DevTables &T) {
-  double gn[NM], gt[NM], bu[NL > 0 ? NL : 1];
+__device__ __forceinline__ void wall_rot_solve(double (&gx)[NM], double (&gy)[NM], double dnx,
+                                               double dny, const double *U, const double *dia1,
+                                               const double *UL, const double *dia2) {
+  double bu[NL];
+#pragma unroll
   for (int j = 0; j < NM; j++) {
-    gn[j] = gx[j] * dnx + gy[j] * dny;
-    gt[j] = gx[j] * dny - gy[j] * dnx;
+    const double gn = gx[j] * dnx + gy[j] * dny, gt = gx[j] * dny - gy[j] * dnx;
+    gx[j] = gn;
+    gy[j] = gt;
   }
-  reduc_gn<NC>(gn, bu);
-  dtrsm4_mem<NM>(NL, bu, T.UL, T.dia2);
-  dtrsm4_mem<NM>(NM, gt, T.U, T.dia1);
-  reduc_inv_gn<NC>(gn, bu);
+  reduc_gn<NC, NL>(gx, bu);
+  dtrsm4_reg<NL, NM>(bu, UL, dia2);
+  dtrsm4_reg<NM, NM>(gy, U, dia1);
+  reduc_inv_gn<NC, NL>(gx, bu);
+#pragma unroll
   for (int j = 0; j < NM; j++) {
-    gx[j] = gn[j] * dnx + gt[j] * dny;
-    gy[j] = gn[j] * dny - gt[j] * dnx;
+    const double gn = gx[j], gt = gy[j];
+    gx[j] = gn * dnx + gt * dny;
+    gy[j] = gn * dny - gt * dnx;
   }
 }
 
 // No-slip projection: the field in the reduced basis
 // (src/calvarnl_ns.f:253-264, 409-421).
 template <int NC, int NM, int NL>
-__device__ void wall_red_solve(double *g, const DevTables &T) {
-  double bu[NL > 0 ? NL : 1];
-  reduc_gn<NC>(g, bu);
-  dtrsm4_mem<NM>(NL, bu, T.UL, T.dia2);
-  reduc_inv_gn<NC>(g, bu);
+__device__ __forceinline__ void wall_red_solve(double (&g)[NM], const double *UL,
+                                               const double *dia2) {
+  double bu[NL];
+  reduc_gn<NC, NL>(g, bu);
+  dtrsm4_reg<NL, NM>(bu, UL, dia2);
+  reduc_inv_gn<NC, NL>(g, bu);
 }
 
 // Free-slip projection of the velocity-gradient tensor on a wall element
-// (src/calvarnl_fs.f:171-204): q = xx, xy, yx, yy.
+// (src/calvarnl_fs.f:171-204), in place: q = xx, xy, yx, yy hold gnn, gnt,
+// gtn, gtt between the two rotations.
 template <int NC, int NM, int NL>
-__device__ void wall_tensor_solve(double *qxx, double *qxy, double *qyx, double *qyy,
-                                  double dnx, double dny, const DevTables &T) {
-  double gnn[NM], gnt[NM], gtn[NM], gtt[NM], bu[NL > 0 ? NL : 1];
+__device__ __forceinline__ void wall_tensor_solve(double (&qxx)[NM], double (&qxy)[NM],
+                                                  double (&qyx)[NM], double (&qyy)[NM],
+                                                  double dnx, double dny, const double *U,
+                                                  const double *dia1, const double *UL,
+                                                  const double *dia2) {
+  double bu[NL];
+#pragma unroll
   for (int j = 0; j < NM; j++) {
-    double gnx = qxx[j] * dnx + qyx[j] * dny, gtx = qxx[j] * dny - qyx[j] * dnx;
-    double gny = qxy[j] * dnx + qyy[j] * dny, gty = qxy[j] * dny - qyy[j] * dnx;
-    gnn[j] = gnx * dnx + gny * dny;
-    gnt[j] = gnx * dny - gny * dnx;
-    gtn[j] = gtx * dnx + gty * dny;
-    gtt[j] = gtx * dny - gty * dnx;
+    const double gnx = qxx[j] * dnx + qyx[j] * dny, gtx = qxx[j] * dny - qyx[j] * dnx;
+    const double gny = qxy[j] * dnx + qyy[j] * dny, gty = qxy[j] * dny - qyy[j] * dnx;
+    qxx[j] = gnx * dnx + gny * dny;  // gnn
+    qxy[j] = gnx * dny - gny * dnx;  // gnt
+    qyx[j] = gtx * dnx + gty * dny;  // gtn
+    qyy[j] = gtx * dny - gty * dnx;  // gtt
   }
-  reduc_gn<NC>(gtn, bu);
-  dtrsm4_mem<NM>(NM, gnn, T.U, T.dia1);
-  dtrsm4_mem<NM>(NM, gnt, T.U, T.dia1);
-  dtrsm4_mem<NM>(NL, bu, T.UL, T.dia2);
-  dtrsm4_mem<NM>(NM, gtt, T.U, T.dia1);
-  reduc_inv_gn<NC>(gtn, bu);
+  reduc_gn<NC, NL>(qyx, bu);
+  dtrsm4_reg<NM, NM>(qxx, U, dia1);
+  dtrsm4_reg<NM, NM>(qxy, U, dia1);
+  dtrsm4_reg<NL, NM>(bu, UL, dia2);
+  dtrsm4_reg<NM, NM>(qyy, U, dia1);
+  reduc_inv_gn<NC, NL>(qyx, bu);
+#pragma unroll
   for (int j = 0; j < NM; j++) {
-    double gnx = gnn[j] * dnx + gnt[j] * dny, gny = gnn[j] * dny - gnt[j] * dnx;
-    double gtx = gtn[j] * dnx + gtt[j] * dny, gty = gtn[j] * dny - gtt[j] * dnx;
+    const double gnn = qxx[j], gnt = qxy[j], gtn = qyx[j], gtt = qyy[j];
+    const double gnx = gnn * dnx + gnt * dny, gny = gnn * dny - gnt * dnx;
+    const double gtx = gtn * dnx + gtt * dny, gty = gtn * dny - gtt * dnx;
     qxx[j] = gnx * dnx + gtx * dny;
     qyx[j] = gnx * dny - gtx * dnx;
     qxy[j] = gny * dnx + gty * dny;
@@ -574,51 +580,48 @@ __global__ __launch_bounds__(256) void elem_q(F64Args A, const double *__restric
 }
 
 // ---------------------------------------------------------------------------
-// Nonlinear stage C (src/calvarnl_fs.f:245-332; src/nonlin.f:35-133), one
-// lane per element: F = mu0 q - g0 h (src/calvarnl_fs.f:341-348); the nine
+// Nonlinear stage C (src/calvarnl_fs.f:245-332; src/nonlin.f:35-133): F =
+// mu0 q - g0 h (src/calvarnl_fs.f:341-348); the nine
 // fields' values at the quadrature points (spectoxy); their products and
 // quotients, times the weights; the projections onto the modes (xytospec);
 // the sources -adu + w1, -adv + z1 and the mass fluxes uh, vh, which are
-// solved here except on wall elements (wall_c).  LDS: pi2 with row q holding
-// pi2(q, 0..NM-1), pi2 with row m holding pi2(0..NG-1, m), U, 1 / U_ii.
+// solved here except on wall elements (wall_c).  The block's four waves
+// split each phase; LDS: pi2 with row q holding pi2(q, 0..NM-1), U, 1/U_ii.
 // ---------------------------------------------------------------------------
 template <int NM, int NG>
 struct QuadLds {
-  static constexpr int PQ = 0, PM = NG * NM, UU = 2 * NG * NM, DIA = UU + NM * NM,
-                       SIZE = DIA + NM;
+  static constexpr int PQ = 0, UU = NG * NM, DIA = UU + NM * NM, SIZE = DIA + NM;
 };
 
 template <int NC, int NG, int NGF, int VAR>
 __global__ __launch_bounds__(256) void elem_c(F64Args A, const double *__restrict__ wi) {
   constexpr int NM = Sh<NC, NG, NGF>::NM;
-  constexpr int MC = 4;  // modes per pass of the projections
+  constexpr int MC = 7;  // modes per pass of the source projections
   using L = QuadLds<NM, NG>;
   __shared__ double lds[L::SIZE];
-  for (int t = threadIdx.x; t < NG * NM; t += 256) {
-    lds[L::PQ + t] = A.tab.pi2T[t];  // pi2T[m + NM q] = pi2(q, m)
-    lds[L::PM + t] = A.tab.pi2[t];   // pi2[q + NG m]
-  }
+  for (int t = threadIdx.x; t < NG * NM; t += 256) lds[L::PQ + t] = A.tab.pi2T[t];
   for (int t = threadIdx.x; t < NM * NM; t += 256) lds[L::UU + t] = A.tab.U[t];
   for (int t = threadIdx.x; t < NM; t += 256) lds[L::DIA + t] = A.tab.dia1[t];
   __syncthreads();
-  const double *PQ = lds + L::PQ, *PM = lds + L::PM;
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= A.mesh.nblk * WB) return;
+  const double *PQ = lds + L::PQ;  // row q: pi2(q, 0..NM-1)
+  const int w = wave_id(), e = blockIdx.x * WB + (threadIdx.x & (WB - 1));
   const Elem q = elem_of(A.mesh, e);
   const double mu0 = A.mu0, g0 = A.g0, mb0 = A.mb0;
-  // F (elementwise)
+  // F = mu0 q - g0 h (elementwise; wave w the component w)
+  {
+    const double *Qw = w < 2 ? (w == 0 ? A.Q[0] : A.Q[1]) : (w == 2 ? A.Q[2] : A.Q[3]);
+    double *Fw = w < 2 ? (w == 0 ? A.F[0] : A.F[1]) : (w == 2 ? A.F[2] : A.F[3]);
+    const bool gh = w == 0 || w == 3;
 #pragma unroll 4
-  for (int m = 0; m < NM; m++) {
-    const size_t i = f64_at(e, m, NM);
-    const double h0v = A.x[2][i];
-    A.F[0][i] = mu0 * A.Q[0][i] - g0 * h0v;
-    A.F[1][i] = mu0 * A.Q[1][i];
-    A.F[2][i] = mu0 * A.Q[2][i];
-    A.F[3][i] = mu0 * A.Q[3][i] - g0 * h0v;
+    for (int m = 0; m < NM; m++) {
+      const size_t i = f64_at(e, m, NM);
+      Fw[i] = gh ? mu0 * Qw[i] - g0 * A.x[2][i] : mu0 * Qw[i];
+    }
   }
-  // the nine fields at the quadrature points: u, v, gxx, gxy = qxy - fc,
-  // gyx = qyx + fc, gyy, hk = h + hb, wk = w0 - mb0 u, zk = z0 - mb0 v
-  for (int c = 0; c < 9; c++) {
+  // the nine fields at the quadrature points (wave w: fields w, w+4, w+8):
+  // u, v, gxx, gxy = qxy - fc, gyx = qyx + fc, gyy, hk = h + hb,
+  // wk = w0 - mb0 u, zk = z0 - mb0 v
+  for (int c = w; c < 9; c += 4) {
     // field c = X op Y: op 0 none, 1 minus, 2 plus, 3 minus mb0 times
     const double *X, *Y = nullptr;
     int op = 0;
@@ -658,9 +661,9 @@ __global__ __launch_bounds__(256) void elem_c(F64Args A, const double *__restric
       A.MF[f64_at(e, c * NG + qq, 9 * NG)] = mf;
     }
   }
-  // products at the points, times the weights
-#pragma unroll 3
-  for (int qq = 0; qq < NG; qq++) {
+  __syncthreads();  // (global memory written by the block's waves: visible after)
+  // products at the points, times the weights (wave w: points w, w+4, ...)
+  for (int qq = w; qq < NG; qq += 4) {
     double mf[9];
 #pragma unroll
     for (int c = 0; c < 9; c++) mf[c] = A.MF[f64_at(e, c * NG + qq, 9 * NG)];
@@ -673,48 +676,58 @@ __global__ __launch_bounds__(256) void elem_c(F64Args A, const double *__restric
     o[5] = mf[1] * mf[6];  // vhk
     o[6] = mf[7] / mf[6];  // wk
     o[7] = mf[8] / mf[6];  // zk
-    const double w = wi[qq];
+    const double wq = wi[qq];
 #pragma unroll
-    for (int c = 0; c < 8; c++) A.WS[f64_at(e, c * NG + qq, 8 * NG)] = w * o[c];
+    for (int c = 0; c < 8; c++) A.WS[f64_at(e, c * NG + qq, 8 * NG)] = wq * o[c];
   }
-  // projections onto the modes, MC modes at a time
-  for (int m0 = 0; m0 < NM; m0 += MC) {
-    double r[8][MC];
+  __syncthreads();
+  // projections onto the modes: wave 0 the u source -adu + w1 (products 0,
+  // 1, 6), wave 1 the v source -adv + z1 (2, 3, 7), waves 2 and 3 the mass
+  // fluxes uh (4) and vh (5), solved here except on wall elements (wall_c)
+  if (w < 2) {
+    const int p0 = w == 0 ? 0 : 2, p1 = p0 + 1, p2 = w == 0 ? 6 : 7;
+    double *Sw = w == 0 ? A.S[0] : A.S[1];
+    for (int m0 = 0; m0 < NM; m0 += MC) {
+      double r[3][MC];
 #pragma unroll
-    for (int c = 0; c < 8; c++)
+      for (int t = 0; t < 3; t++)
 #pragma unroll
-      for (int mm = 0; mm < MC; mm++) r[c][mm] = 0.0;
+        for (int mm = 0; mm < MC; mm++) r[t][mm] = 0.0;
 #pragma unroll 3
-    for (int qq = 0; qq < NG; qq++) {
-      double ws[8];
+      for (int qq = 0; qq < NG; qq++) {
+        const double ws[3] = {A.WS[f64_at(e, p0 * NG + qq, 8 * NG)],
+                              A.WS[f64_at(e, p1 * NG + qq, 8 * NG)],
+                              A.WS[f64_at(e, p2 * NG + qq, 8 * NG)]};
+        const double *row = PQ + NM * qq;
 #pragma unroll
-      for (int c = 0; c < 8; c++) ws[c] = A.WS[f64_at(e, c * NG + qq, 8 * NG)];
+        for (int mm = 0; mm < MC; mm++) {
+          const double p = row[m0 + mm < NM ? m0 + mm : NM - 1];
 #pragma unroll
-      for (int mm = 0; mm < MC; mm++) {
-        const double p = PQ[(m0 + mm < NM ? m0 + mm : NM - 1) + NM * qq];
-#pragma unroll
-        for (int c = 0; c < 8; c++) r[c][mm] = r[c][mm] + ws[c] * p;
+          for (int t = 0; t < 3; t++) r[t][mm] = r[t][mm] + ws[t] * p;
+        }
       }
+#pragma unroll
+      for (int mm = 0; mm < MC; mm++)
+        if (m0 + mm < NM) {
+          const double ad = r[0][mm] + r[1][mm];
+          Sw[f64_at(e, m0 + mm, NM)] = -ad + r[2][mm];
+        }
     }
-#pragma unroll
-    for (int mm = 0; mm < MC; mm++)
-      if (m0 + mm < NM) {
-        const size_t i = f64_at(e, m0 + mm, NM);
-        const double adu = r[0][mm] + r[1][mm], adv = r[2][mm] + r[3][mm];
-        A.S[0][i] = -adu + r[6][mm];
-        A.S[1][i] = -adv + r[7][mm];
-        A.UH[0][i] = r[4][mm];
-        A.UH[1][i] = r[5][mm];
-      }
-  }
-  (void)PM;
-  if (q.wall) return;  // wall_c
-#pragma unroll
-  for (int c = 0; c < 2; c++) {
+  } else {
+    const int pc = w == 2 ? 4 : 5;
+    double *Uw = w == 2 ? A.UH[0] : A.UH[1];
     double g[NM];
-    vload(g, c == 0 ? A.UH[0] : A.UH[1], e);
-    dtrsm4_reg<NM, NM>(g, lds + L::UU, lds + L::DIA);
-    vstore(c == 0 ? A.UH[0] : A.UH[1], e, g);
+#pragma unroll
+    for (int m = 0; m < NM; m++) g[m] = 0.0;
+#pragma unroll 2
+    for (int qq = 0; qq < NG; qq++) {
+      const double ws = A.WS[f64_at(e, pc * NG + qq, 8 * NG)];
+      const double *row = PQ + NM * qq;
+#pragma unroll
+      for (int m = 0; m < NM; m++) g[m] = g[m] + ws * row[m];
+    }
+    if (!q.wall) dtrsm4_reg<NM, NM>(g, lds + L::UU, lds + L::DIA);  // (else wall_c)
+    vstore(Uw, e, g);
   }
 }
 
@@ -768,76 +781,99 @@ __global__ __launch_bounds__(192) void elem_eg(F64Args A, const double *__restri
 }
 
 // ---------------------------------------------------------------------------
-// Wall elements, one thread each: the reference's projection sequences on
-// the raw vectors the element kernels left.
+// Wall elements, one lane each, vectors in registers: the reference's
+// projection sequences on the raw vectors the element kernels left.  The
+// factors are staged in LDS (U, 1/U_ii, UL, 1/UL_ii).
 // ---------------------------------------------------------------------------
 template <int NM>
-__device__ void gload(double *v, const double *p, int e) {
-  for (int m = 0; m < NM; m++) v[m] = p[f64_at(e, m, NM)];
-}
-template <int NM>
-__device__ void gstore(double *p, int e, const double *v) {
-  for (int m = 0; m < NM; m++) p[f64_at(e, m, NM)] = v[m];
+__device__ __forceinline__ const double *stage_factors(double *lds, const DevTables &T) {
+  for (int t = threadIdx.x; t < NM * NM; t += 64) {
+    lds[t] = T.U[t];
+    lds[NM * NM + t] = T.UL[t];
+  }
+  for (int t = threadIdx.x; t < NM; t += 64) {
+    lds[2 * NM * NM + t] = T.dia1[t];
+    lds[2 * NM * NM + NM + t] = T.dia2[t];
+  }
+  __syncthreads();
+  return lds;
 }
 
 // the velocity pair of the linear variant and of E+G: free slip (lin, fs,
 // inv) or no slip (ns), then the RK4 stage update
 template <int NC, int NG, int NGF, int VAR>
-__global__ __launch_bounds__(64) void wall_vel(F64Args A, DevTables T) {
+__global__ __launch_bounds__(64) void wall_vel(F64Args A) {
   constexpr int NM = Sh<NC, NG, NGF>::NM, NL = Sh<NC, NG, NGF>::NL;
+  __shared__ double lds[2 * NM * NM + 2 * NM];
+  const double *U = stage_factors<NM>(lds, A.tab), *UL = U + NM * NM;
+  const double *dia1 = U + 2 * NM * NM, *dia2 = dia1 + NM;
   const int t = blockIdx.x * 64 + threadIdx.x;
   if (t >= A.mesh.nwall) return;
   const int e = A.mesh.wall[t];
-  double gx[NM], gy[NM];
-  gload<NM>(gx, A.G[0], e);
-  gload<NM>(gy, A.G[1], e);
-  if (VAR == 2) {
-    wall_red_solve<NC, NM, NL>(gx, T);
-    wall_red_solve<NC, NM, NL>(gy, T);
+  if (VAR == 2) {  // each component alone
+#pragma unroll 1
+    for (int c = 0; c < 2; c++) {
+      double g[NM];
+      vload(g, c == 0 ? A.G[0] : A.G[1], e);
+      wall_red_solve<NC, NM, NL>(g, UL, dia2);
+      emit_vec(A, c, e, g);
+    }
   } else {
+    double gx[NM], gy[NM];
+    vload(gx, A.G[0], e);
+    vload(gy, A.G[1], e);
     const double nx = A.mesh.geo[f64_at(e, G_NX, G_ROWS)], ny = A.mesh.geo[f64_at(e, G_NY, G_ROWS)];
-    wall_rot_solve<NC, NM, NL>(gx, gy, nx, ny, T);
-  }
-  const Out o0 = out_of(A, 0), o1 = out_of(A, 1);
-  for (int m = 0; m < NM; m++) {
-    emit(A, o0, f64_at(e, m, NM), gx[m]);
-    emit(A, o1, f64_at(e, m, NM), gy[m]);
+    wall_rot_solve<NC, NM, NL>(gx, gy, nx, ny, U, dia1, UL, dia2);
+    emit_vec(A, 0, e, gx);
+    emit_vec(A, 1, e, gy);
   }
 }
 
 // the velocity-gradient tensor of free-slip wall elements (stage B)
 template <int NC, int NG, int NGF>
-__global__ __launch_bounds__(64) void wall_q(F64Args A, DevTables T) {
+__global__ __launch_bounds__(64) void wall_q(F64Args A) {
   constexpr int NM = Sh<NC, NG, NGF>::NM, NL = Sh<NC, NG, NGF>::NL;
+  __shared__ double lds[2 * NM * NM + 2 * NM];
+  const double *U = stage_factors<NM>(lds, A.tab), *UL = U + NM * NM;
+  const double *dia1 = U + 2 * NM * NM, *dia2 = dia1 + NM;
   const int t = blockIdx.x * 64 + threadIdx.x;
   if (t >= A.mesh.nwall) return;
   const int e = A.mesh.wall[t];
-  double q[4][NM];
-  for (int c = 0; c < 4; c++) gload<NM>(q[c], A.Q[c], e);
+  double q0[NM], q1[NM], q2[NM], q3[NM];
+  vload(q0, A.Q[0], e);
+  vload(q1, A.Q[1], e);
+  vload(q2, A.Q[2], e);
+  vload(q3, A.Q[3], e);
   const double nx = A.mesh.geo[f64_at(e, G_NX, G_ROWS)], ny = A.mesh.geo[f64_at(e, G_NY, G_ROWS)];
-  wall_tensor_solve<NC, NM, NL>(q[0], q[1], q[2], q[3], nx, ny, T);
-  for (int c = 0; c < 4; c++) gstore<NM>(A.Q[c], e, q[c]);
+  wall_tensor_solve<NC, NM, NL>(q0, q1, q2, q3, nx, ny, U, dia1, UL, dia2);
+  vstore(A.Q[0], e, q0);
+  vstore(A.Q[1], e, q1);
+  vstore(A.Q[2], e, q2);
+  vstore(A.Q[3], e, q3);
 }
 
 // the mass fluxes of wall elements (stage C)
 template <int NC, int NG, int NGF, int VAR>
-__global__ __launch_bounds__(64) void wall_c(F64Args A, DevTables T) {
+__global__ __launch_bounds__(64) void wall_c(F64Args A) {
   constexpr int NM = Sh<NC, NG, NGF>::NM, NL = Sh<NC, NG, NGF>::NL;
+  __shared__ double lds[2 * NM * NM + 2 * NM];
+  const double *U = stage_factors<NM>(lds, A.tab), *UL = U + NM * NM;
+  const double *dia1 = U + 2 * NM * NM, *dia2 = dia1 + NM;
   const int t = blockIdx.x * 64 + threadIdx.x;
   if (t >= A.mesh.nwall) return;
   const int e = A.mesh.wall[t];
   double u[NM], v[NM];
-  gload<NM>(u, A.UH[0], e);
-  gload<NM>(v, A.UH[1], e);
+  vload(u, A.UH[0], e);
+  vload(v, A.UH[1], e);
   if (VAR == 2) {
-    wall_red_solve<NC, NM, NL>(u, T);
-    wall_red_solve<NC, NM, NL>(v, T);
+    wall_red_solve<NC, NM, NL>(u, UL, dia2);
+    wall_red_solve<NC, NM, NL>(v, UL, dia2);
   } else {
     const double nx = A.mesh.geo[f64_at(e, G_NX, G_ROWS)], ny = A.mesh.geo[f64_at(e, G_NY, G_ROWS)];
-    wall_rot_solve<NC, NM, NL>(u, v, nx, ny, T);
+    wall_rot_solve<NC, NM, NL>(u, v, nx, ny, U, dia1, UL, dia2);
   }
-  gstore<NM>(A.UH[0], e, u);
-  gstore<NM>(A.UH[1], e, v);
+  vstore(A.UH[0], e, u);
+  vstore(A.UH[1], e, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -884,7 +920,6 @@ int launch(const F64Args &a, int phase, const double *in, double *out) {
   const DevTables &T = a.tab;
   const dim3 gf2((M.nf + 255) / 256, 2), gf3((M.nf + 255) / 256, 3), gf6((M.nf + 255) / 256, 6);
   const unsigned nb = (unsigned)M.nblk, nw = (unsigned)((M.nwall + 63) / 64);
-  const unsigned nc4 = (unsigned)((M.nblk * WB + 255) / 256);
   if (a.variant == 0) {
     FaceSpec fs = {};
     fs.in[0] = a.x[2];
@@ -912,7 +947,7 @@ int launch(const F64Args &a, int phase, const double *in, double *out) {
     a.timer(1);
     elem_lin<NC, NG, NGF><<<nb, 192, 0, st>>>(a, T.wif);
     a.timer(2);
-    if (nw) wall_vel<NC, NG, NGF, 0><<<nw, 64, 0, st>>>(a, T);
+    if (nw) wall_vel<NC, NG, NGF, 0><<<nw, 64, 0, st>>>(a);
     a.timer(3);
     return hipGetLastError() == hipSuccess ? 0 : -2;
   }
@@ -935,15 +970,15 @@ int launch(const F64Args &a, int phase, const double *in, double *out) {
 #define SPOC_Q(V) elem_q<NC, NG, NGF, V><<<nb, 256, 0, st>>>(a, T.wif)
   SPOC_V(1, SPOC_Q(1)) SPOC_V(2, SPOC_Q(2)) SPOC_V(3, SPOC_Q(3))
   a.timer(2);
-  if (a.variant == 1 && nw) wall_q<NC, NG, NGF><<<nw, 64, 0, st>>>(a, T);
+  if (a.variant == 1 && nw) wall_q<NC, NG, NGF><<<nw, 64, 0, st>>>(a);
   a.timer(3);
-#define SPOC_C(V) elem_c<NC, NG, NGF, V><<<nc4, 256, 0, st>>>(a, T.wi)
+#define SPOC_C(V) elem_c<NC, NG, NGF, V><<<nb, 256, 0, st>>>(a, T.wi)
   SPOC_V(1, SPOC_C(1)) SPOC_V(2, SPOC_C(2)) SPOC_V(3, SPOC_C(3))
   a.timer(4);
   if (nw) {
-    SPOC_V(1, (wall_c<NC, NG, NGF, 1><<<nw, 64, 0, st>>>(a, T)))
-    SPOC_V(2, (wall_c<NC, NG, NGF, 2><<<nw, 64, 0, st>>>(a, T)))
-    SPOC_V(3, (wall_c<NC, NG, NGF, 3><<<nw, 64, 0, st>>>(a, T)))
+    SPOC_V(1, (wall_c<NC, NG, NGF, 1><<<nw, 64, 0, st>>>(a)))
+    SPOC_V(2, (wall_c<NC, NG, NGF, 2><<<nw, 64, 0, st>>>(a)))
+    SPOC_V(3, (wall_c<NC, NG, NGF, 3><<<nw, 64, 0, st>>>(a)))
   }
   FaceSpec f2 = {};
   const double *fin[6] = {a.F[0], a.F[1], a.F[2], a.F[3], a.UH[0], a.UH[1]};
@@ -961,9 +996,9 @@ int launch(const F64Args &a, int phase, const double *in, double *out) {
   SPOC_V(1, SPOC_EG(1)) SPOC_V(2, SPOC_EG(2)) SPOC_V(3, SPOC_EG(3))
   a.timer(7);
   if (nw) {
-    SPOC_V(1, (wall_vel<NC, NG, NGF, 1><<<nw, 64, 0, st>>>(a, T)))
-    SPOC_V(2, (wall_vel<NC, NG, NGF, 2><<<nw, 64, 0, st>>>(a, T)))
-    SPOC_V(3, (wall_vel<NC, NG, NGF, 3><<<nw, 64, 0, st>>>(a, T)))
+    SPOC_V(1, (wall_vel<NC, NG, NGF, 1><<<nw, 64, 0, st>>>(a)))
+    SPOC_V(2, (wall_vel<NC, NG, NGF, 2><<<nw, 64, 0, st>>>(a)))
+    SPOC_V(3, (wall_vel<NC, NG, NGF, 3><<<nw, 64, 0, st>>>(a)))
   }
   a.timer(8);
 #undef SPOC_V
