@@ -386,6 +386,23 @@ int spoc_cal_conv(spoc_ctx *ctx, double fac1, double fac2, double fac3, int *fco
 int spoc_energy(spoc_ctx *ctx, double *ke, double *pe, double *mass);
 
 /*
+ * spoc_energy without allocation (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops).  Queues the integrals of the device-resident state
+ * on the context's stream and returns without waiting: out_dev (device
+ * memory, 3 x nmem doubles, nmem = 1 for one mesh) receives ke, pe, mass per
+ * ensemble member, bit-identical to spoc_energy / spoc_ensemble_energy,
+ * once the stream has run (spoc_sync).  The per-element partial integrals
+ * live in a fixed device array of 2^23 doubles (64 MiB) that each device the
+ * library runs on holds until the process ends; meshes with ne (nm + 2)
+ * above that are rejected (SPOC_EINVAL).  No NaN test (the values are not
+ * read on the host).  All calls on one device share that array: calls that
+ * overlap on one device (several contexts on several host threads or
+ * streams) race; calls queued on one context's stream do not.
+ */
+int spoc_energy_noalloc(spoc_ctx *ctx, double *out_dev);
+
+/*
  * Duration (milliseconds) of the last spoc_rk4_steps call measured with HIP
  * events on the context stream, and the per-kernel totals of that call:
  * names[i] (NUL-separated, caller buffer of len bytes), ms[i], launches[i]

@@ -61,7 +61,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                  "spoc_mesh_changemaille", "spoc_state_file_read", "spoc_state_file_write",
                  "spoc_mesh_file_size", "spoc_mesh_file_read", "spoc_mesh_file_write",
                  "spoc_interpol", "spoc_calh2", "spoc_set_flux", "spoc_set_ordering",
-                 "spoc_mesh_order"):
+                 "spoc_mesh_order", "spoc_energy_noalloc"):
         getattr(lib, name).restype = C.c_int
     lib.spoc_ctx_destroy.argtypes = [C.c_void_p]
     _lib = lib
@@ -397,6 +397,12 @@ class Context:
         ke, pe, mass = C.c_double(), C.c_double(), C.c_double()
         _check(self.lib.spoc_energy(self._h, C.byref(ke), C.byref(pe), C.byref(mass)))
         return ke.value, pe.value, mass.value
+
+    def energy_noalloc(self, out_dev_ptr: int) -> None:
+        """Queue (ke, pe, mass) per member into the device buffer at
+        out_dev_ptr (3 nmem doubles) without any allocation; asynchronous on
+        the context's stream (sync() before reading)."""
+        _check(self.lib.spoc_energy_noalloc(self._h, C.c_void_p(out_dev_ptr)))
 
     def set_profiling(self, on: bool) -> None:
         _check(self.lib.spoc_set_profiling(self._h, int(on)))

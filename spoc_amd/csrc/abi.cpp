@@ -29,6 +29,8 @@ int adapt_launch(hipStream_t st, int nc, int ngf, int ne, int nmem, int nch,
                  int *flags);
 int adapt_chunk();
 int nonfinite_launch(hipStream_t st, size_t n, const double *a, int *flag);
+size_t energy_scratch_doubles();
+int energy_scratch(int device, double **out);
 struct F64Plan;
 int f64_plan_create(F64Plan **out, int nc, int ng, int ngf);
 void f64_plan_destroy(F64Plan *p);
@@ -1409,6 +1411,32 @@ int spoc_cal_conv(spoc_ctx *c, double fac1, double fac2, double fac3, int *fconv
     conv[m] = (sup > fac1 || deraff[m]) ? 1 : 0;
     if (supmax) supmax[m] = sup;
   }
+  return SPOC_OK;
+}
+
+int spoc_energy_noalloc(spoc_ctx *c, double *out_dev) {
+  if (!c || !c->have_state || !c->have_fields)
+    return fail(SPOC_ESTATE, "spoc_energy_noalloc: need mesh, fields and state");
+  if (!c->have_nodes) return fail(SPOC_ESTATE, "spoc_energy_noalloc: mesh set without nodes");
+  if (!out_dev) return fail(SPOC_EINVAL, "spoc_energy_noalloc: null output");
+  const size_t need = (size_t)c->ne * (2 + (size_t)c->nm), cap = energy_scratch_doubles();
+  if (need > cap)
+    return fail(SPOC_EINVAL,
+                "spoc_energy_noalloc: %d elements of %d modes need %zu doubles of scratch, more "
+                "than the %zu it holds (ne (nm + 2) <= %zu); use spoc_energy",
+                c->ne, c->nm, need, cap, cap);
+  HIPCHK(hipSetDevice(c->device));
+  double *scr = nullptr;
+  const int r = energy_scratch(c->device, &scr);
+  if (r == -1) return fail(SPOC_EINVAL, "spoc_energy_noalloc: device %d out of range", c->device);
+  if (r) return fail(SPOC_EHIP, "spoc_energy_noalloc: hipGetSymbolAddress failed");
+  const size_t n = (size_t)c->nm * c->ne;
+  const int nmem = (int)c->mem_e0.size() - 1;
+  const int rc = energy_launch(c->stream, c->ne, c->nm, c->ng, state_ptr(c, 0), state_ptr(c, 1),
+                               state_ptr(c, 2), c->dfields + 3 * n, c->tab.pi2, c->tab.wi,
+                               c->tab.ammo1, c->mesh.deta, c->phys.g0, nmem, c->dmem + 2,
+                               c->dorder, scr, out_dev);
+  if (rc) return fail(SPOC_EHIP, "spoc_energy_noalloc: kernel launch failed");
   return SPOC_OK;
 }
 

@@ -6,9 +6,43 @@
 // the reference's sequential accumulation bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "common.h"
 
 namespace spoc {
+
+namespace {
+
+// The partial integrals of spoc_energy_noalloc: ne (nm + 2) doubles, at most
+// kScratchDoubles (64 MiB), held by every device the library runs on until
+// the process ends.  In host code `scratch` names a host-side stand-in, not a
+// device address: only the per-device address from hipGetSymbolAddress
+// (energy_scratch) is passed to kernels.
+constexpr size_t kScratchDoubles = (size_t)1 << 23;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+
+}  // namespace
+
+size_t energy_scratch_doubles() { return kScratchDoubles; }
+
+// The device address of `scratch` on `device`, which must be current; looked
+// up once per device.  0 on success, -1 device out of range, -2 HIP error.
+int energy_scratch(int device, double **out) {
+  if (device < 0 || device >= kMaxDevices) return -1;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -2;
+    g_scratch[device] = (double *)p;
+  }
+  *out = g_scratch[device];
+  return 0;
+}
 
 __global__ void energy_terms(int ne, int nm, int ng, const double *__restrict__ u,
                              const double *__restrict__ v, const double *__restrict__ h,
