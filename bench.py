@@ -526,6 +526,9 @@ def main():
                          "centroids), a random permutation, or row by row")
     ap.add_argument("--ordering", default="auto", choices=("auto", "hilbert", "zcurve", "input"),
                     help="the library's internal element numbering (spoc_set_ordering)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every kernel of the RK4 loop eagerly instead of replaying "
+                         "captured HIP graphs (spoc_set_graph 0)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -546,6 +549,8 @@ def main():
     tb = time.time()
     ctx = spoc_amd.Context(t)
     ctx.set_ordering(args.ordering)
+    if args.no_graph:
+        ctx.set_graph(False)
     if wl == "c5":
         members = configs.config5(t, nmem=args.members)
         if args.numbering != "generator":
@@ -614,6 +619,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": WORKLOADS[wl]["name"](nc, ng, ngf, ne, dt, args),
                    "ne": ne, "nm": nm, "numbering": args.numbering, "ordering": args.ordering,
+                   "graphs": not args.no_graph,
                    "precision": ("fp32-mfma quadrature, fp64-mfma linear gravity-wave core, "
                                  "fp64 state/RK4") if variant != "lin" else
                                 "fp64-mfma (the linear variant is all linear core)"},

@@ -17,7 +17,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libspoc_amd.so")
+# SPOC_AMD_LIB names another build of the library (the host sanitizer build)
+LIB_PATH = os.environ.get("SPOC_AMD_LIB") or os.path.join(HERE, "libspoc_amd.so")
 
 LIN, FS, NS, INV = 0, 1, 2, 3
 VARIANTS = {"lin": LIN, "fs": FS, "ns": NS, "inv": INV}

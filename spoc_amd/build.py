@@ -50,7 +50,8 @@ def _units():
 
 def _compile(u):
     src, obj, flags = u
-    cmd = [HIPCC] + BASE + flags + ["-c", os.path.join(CSRC, src), "-o", os.path.join(OBJ, obj)]
+    base = [f for f in BASE if f != "-O3"] if "-O1" in flags else BASE
+    cmd = [HIPCC] + base + flags + ["-c", os.path.join(CSRC, src), "-o", os.path.join(OBJ, obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stderr[-4000:]}")
@@ -72,6 +73,49 @@ def build(jobs: int | None = None, verbose: bool = True) -> str:
         print(f"built {LIB}")
     build_fortran(verbose)
     return LIB
+
+
+# Host sanitizer build (CPU only): the host translation units again with
+# AddressSanitizer and UndefinedBehaviorSanitizer on their host side, linked
+# with the device objects of build().  The result is loaded by setting
+# SPOC_AMD_LIB to it and preloading the sanitizer runtime (SAN_RUNTIME).
+SAN_DIR = os.path.join(OBJ, "san")
+SAN_LIB = os.path.join(SAN_DIR, "libspoc_amd_san.so")
+SAN_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+             "-Xarch_host", "-fno-sanitize-recover=all", "-Xarch_host", "-fno-omit-frame-pointer"]
+HOST_UNITS = ("f32_plan.cpp", "f64_plan.cpp", "energy.hip", "adapt.hip", "mesh.cpp", "files.cpp",
+              "interp.cpp", "order.cpp", "abi.cpp")
+
+
+def san_runtime() -> str:
+    """Path of the compiler's shared AddressSanitizer runtime."""
+    clang = os.path.join(os.path.dirname(os.path.realpath(HIPCC)), "..", "llvm", "bin", "clang")
+    if not os.path.exists(clang):
+        clang = "/opt/rocm/llvm/bin/clang"
+    r = subprocess.run([clang, "--print-file-name=libclang_rt.asan-x86_64.so"],
+                       capture_output=True, text=True, check=True)
+    return r.stdout.strip()
+
+
+def build_sanitized(jobs: int | None = None, verbose: bool = True) -> str:
+    os.makedirs(SAN_DIR, exist_ok=True)
+    jobs = jobs or min(16, os.cpu_count() or 4)
+    host, dev = [], []
+    for src, obj, flags in _units():
+        if src in HOST_UNITS:
+            host.append((src, os.path.join("san", obj), flags + ["-O1", "-g"] + SAN_FLAGS))
+        else:
+            dev.append((src, obj, flags))
+    missing = [u for u in dev if not os.path.exists(os.path.join(OBJ, u[1]))]
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(_compile, host + missing))
+    objs += [u[1] for u in dev if u not in missing]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", SAN_LIB] + SAN_FLAGS + \
+          [os.path.join(OBJ, o) for o in objs]
+    subprocess.run(cmd, check=True)
+    if verbose:
+        print(f"built {SAN_LIB}")
+    return SAN_LIB
 
 
 def build_fortran(verbose: bool = True) -> None:
@@ -99,4 +143,7 @@ def build_fortran(verbose: bool = True) -> None:
 
 
 if __name__ == "__main__":
-    build(int(sys.argv[1]) if len(sys.argv) > 1 else None)
+    if len(sys.argv) > 1 and sys.argv[1] == "sanitized":
+        build_sanitized()
+    else:
+        build(int(sys.argv[1]) if len(sys.argv) > 1 else None)
