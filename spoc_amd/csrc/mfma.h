@@ -143,16 +143,19 @@ __device__ __forceinline__ T face_avg(const T *__restrict__ tr, const F32Mesh &M
 // from a trace array.  A wall point averages the element's own trace on its
 // local edge 1 with itself: 0.5 (t + t) == t exactly.
 struct Nbr3 {
-  int nb[3], kn[3];  // facing element (negative: wall) and its local edge
+  int nb0, nb1, nb2;  // facing element across local edge k (negative: wall)
+  int kn0, kn1, kn2;  // its local edge
 };
+// (each value passes through an empty asm, which keeps it a register value:
+// otherwise a select between them can be folded into a load from a stack
+// copy at a runtime offset, i.e. scratch memory)
+__device__ __forceinline__ int in_reg(int v) {
+  asm("" : "+v"(v));
+  return v;
+}
 __device__ __forceinline__ Nbr3 nbr3(const F32Mesh &M, int e) {
-  Nbr3 n;
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    n.nb[k] = M.nbr[e * 3 + k];
-    n.kn[k] = M.nbe[e * 3 + k];
-  }
-  return n;
+  const int *n = M.nbr + e * 3, *k = M.nbe + e * 3;
+  return Nbr3{n[0], n[1], n[2], k[0], k[1], k[2]};
 }
 struct FacePt {
   int ea, ka, eb, kb;
@@ -160,8 +163,10 @@ struct FacePt {
 template <int NGF>
 __device__ __forceinline__ FacePt face_pt(const Nbr3 &N, int e, int kj) {
   const int k = kj / NGF, j = kj - k * NGF;
-  const int nb = k == 0 ? N.nb[0] : (k == 1 ? N.nb[1] : N.nb[2]);
-  const int kn = k == 0 ? N.kn[0] : (k == 1 ? N.kn[1] : N.kn[2]);
+  const int n0 = in_reg(N.nb0), n1 = in_reg(N.nb1), n2 = in_reg(N.nb2);
+  const int m0 = in_reg(N.kn0), m1 = in_reg(N.kn1), m2 = in_reg(N.kn2);
+  const int nb = k == 0 ? n0 : (k == 1 ? n1 : n2);
+  const int kn = k == 0 ? m0 : (k == 1 ? m1 : m2);
   const bool wall = nb < 0;
   return {e, wall ? j : kj, wall ? e : nb, wall ? j : kn * NGF + (NGF - 1 - j)};
 }

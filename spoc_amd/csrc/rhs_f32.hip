@@ -691,7 +691,9 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
       const int q = drow(qb, r, g), qc = q < NG ? q : NG - 1;
       const bool ok = valid && q < NG;
       const float f = fc0 + fc1 * O[P.yq + qc] + fc2 * O[P.xq + qc];
-      const float hk = Vh[r] + F[r][0], rhk = 1.0f / hk;
+      // (the hardware reciprocal, within 1 ulp, in place of the correctly
+      // rounded division's eight-instruction sequence)
+      const float hk = Vh[r] + F[r][0], rhk = __builtin_amdgcn_rcpf(hk);
       const float u = Vu[r], v = Vv[r];
       const float p0 = -(u * Vxx[r] + v * (Vxy[r] - f)) + (F[r][1] - mb0 * u) * rhk;
       const float p1 = -(u * (Vyx[r] + f) + v * Vyy[r]) + (F[r][2] - mb0 * v) * rhk;
