@@ -103,19 +103,18 @@ def algorithmic_counts(nm, ng, ngf, variant):
     return {"bc_f32": (bc_fl, 0, bc_by), "eg_f32f64": (eg_fl, vol_fl + face_fl, eg_by)}
 
 
-def measured_traffic(kernel, ne):
-    """HBM bytes per launch of `kernel` from the committed counter profile
-    (FETCH_SIZE and WRITE_SIZE scaled by a calibration copy), when that
-    profile is of this workload; else None."""
-    path = os.path.join(ROOT, "profiles", "fs_p6_199k_traffic.json")
+def measured_traffic(kernel, nc, ne):
+    """HBM bytes per launch of `kernel` (averaged over the RK4 stages) from
+    the committed counter profiles: FETCH_SIZE and WRITE_SIZE, each scaled
+    by what it reads for a device copy of known size.  None when no profile
+    of this degree and mesh size is committed."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             d = json.load(f)
     except OSError:
         return None
-    if f"on {ne} triangles" not in d.get("workload", ""):
-        return None
-    return d["bytes_per_launch"].get(kernel)
+    return d.get("workloads", {}).get(f"p{nc}_ne{ne}", {}).get(kernel)
 
 
 def adjoint_check(ctx, t, c, nsteps, ndir, seed=4001):
@@ -644,7 +643,7 @@ def main():
         else:
             rf = {"bound": "mfma", "achieved": flops_s / 1e12, "peak": fpeak / 1e12,
                   "unit": "TFLOP/s", "frac": flops_s / fpeak}
-        rf.update(kernel=name, avg_launch_us=avg * 1e6, traffic=measured_traffic(name, ne),
+        rf.update(kernel=name, avg_launch_us=avg * 1e6, traffic=measured_traffic(name, nc, ne),
                   algorithmic_flops_per_elem=fl, algorithmic_fp64_flops_per_elem=fl64,
                   algorithmic_bytes_per_elem=by,
                   other={k: {"ms": v[0], "launches": v[1]} for k, v in kern.items()})
@@ -665,6 +664,24 @@ def main():
         out["fp64_value"] = dof * K64 / w64
         out["fp64_ms_per_step"] = 1e3 * w64 / K64
         out["fp32_over_fp64"] = value / out["fp64_value"]
+        # the FP64 path's own kernels (HIP events, eager launches) and its
+        # dominant kernel against HBM, with the counter-measured traffic
+        phase("FP64 path: per-kernel timing")
+        ctx.upload(*state0)
+        ctx.set_profiling(True)
+        ctx.rk4(variant, dt, min(K64, 4), precision="fp64")
+        _, k64 = ctx.timing()
+        ctx.set_profiling(False)
+        if k64:
+            name64, (ms64, n64) = max(k64.items(), key=lambda kv: kv[1][0])
+            avg64 = ms64 / max(n64, 1) / 1e3
+            tr64 = measured_traffic(name64, nc, ne)
+            out["fp64_roofline"] = {
+                "bound": "hbm", "kernel": name64, "avg_launch_us": avg64 * 1e6,
+                "traffic": tr64,
+                "achieved": tr64 / avg64 / 1e9 if tr64 else None, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": tr64 / avg64 / HBM_PEAK if tr64 else None,
+                "per_kernel_us": {k: 1e3 * v[0] / max(v[1], 1) for k, v in k64.items()}}
         if K64 == K:
             out["rel_l2_vs_fp64"] = {f: mass_rel_l2(t, vect, s32[i], s64[i])
                                      for i, f in enumerate(("u", "v", "eta"))}

@@ -49,9 +49,12 @@ constexpr bool eg_split(int nc) { return nc == 5 || nc == 6; }
 #ifndef EG_WAVES
 #define EG_WAVES 3
 #endif
+#ifndef EG_SPLIT_WAVES
+#define EG_SPLIT_WAVES 4
+#endif
 // (at p >= 7, 36 modes, both kernels get two waves, or they spill)
 constexpr int bc_waves(int nc) { return nc >= 7 ? 2 : BC_WAVES; }
-constexpr int eg_waves(int nc) { return nc >= 7 ? 2 : eg_split(nc) ? 4 : EG_WAVES; }
+constexpr int eg_waves(int nc) { return nc >= 7 ? 2 : eg_split(nc) ? EG_SPLIT_WAVES : EG_WAVES; }
 
 // Stage input of field f: the step's start state c0 at the first stage (and
 // for a bare right-hand side), else what the previous stage stored in

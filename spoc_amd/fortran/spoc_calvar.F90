@@ -28,6 +28,10 @@ module spoc_abi
   ! element count of the mesh it holds
   type(c_ptr), save :: spoc_ctx_h = c_null_ptr
   integer, save :: spoc_ctx_ne = -1
+  ! status codes of the C-ABI (include/spoc_abi.h)
+  integer(c_int), parameter :: SPOC_OK = 0, SPOC_EINVAL = -1, SPOC_EHIP = -2, &
+       SPOC_ESTATE = -3, SPOC_ESINGULAR = -4, SPOC_EMESH = -5, SPOC_EFILE = -6, &
+       SPOC_ENAN = -7
   interface
     integer(c_int) function spoc_ctx_create(ctx, device, nc, ng, ngf, wi, wif, pi2, ld_g, pico, &
          ld_f, ld_m, amm, amml, agx, agy, ammo) bind(C, name="spoc_ctx_create")
