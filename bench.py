@@ -589,7 +589,11 @@ def main():
     W = args.warmup
     if args.calib_copy:
         calib_copy(args.calib_copy << 20)
-    # mixed-precision path: warm up, then time K steps from the same initial state
+    # mixed-precision path: the autotuner's choice of kernel form and
+    # workgroup size on this mesh (it would run by itself on the first call)
+    phase("mixed path: autotune")
+    tiles = ctx.autotune(variant)
+    # warm up, then time K steps from the same initial state
     phase("mixed path: warm-up")
     ctx.upload(*state0)
     prime = warm(ctx, variant, dt, W, K, "fp32")
@@ -619,6 +623,8 @@ def main():
         "config": {"workload": WORKLOADS[wl]["name"](nc, ng, ngf, ne, dt, args),
                    "ne": ne, "nm": nm, "numbering": args.numbering, "ordering": args.ordering,
                    "graphs": not args.no_graph,
+                   "tiles": {"waves_per_workgroup": [t & 15 for t in tiles],
+                             "kernel_form": [t >> 4 for t in tiles]},
                    "precision": ("fp32-mfma quadrature, fp64-mfma linear gravity-wave core, "
                                  "fp64 state/RK4") if variant != "lin" else
                                 "fp64-mfma (the linear variant is all linear core)"},

@@ -240,12 +240,16 @@ int spoc_rk4_adjoint_steps(spoc_ctx *ctx, int variant, double dt, int nsteps);
 /*
  * Autotuner of the mixed-precision element kernels (no reference
  * counterpart).  Each element kernel (lin_kernel, or bc_kernel and
- * eg_kernel) can run with 2, 4 or 8 waves of 16 elements per workgroup;
- * spoc_autotune times every choice on the current mesh and keeps the
- * fastest for `variant` (tiles[0] = lin/bc, tiles[1] = eg, 0 for lin).  It
- * also runs by itself on the first mixed-precision call after a mesh
- * change.  spoc_set_tiles forces a choice (0 = automatic) and turns the
- * automatic tuning on or off.  Results do not depend on the tiles.
+ * eg_kernel) is compiled for every polynomial degree in 3 forms (register
+ * budget as resident waves per SIMD, and for eg_kernel whether a block of
+ * 16 elements is split over two waves) and can run with 2, 4 or 8 waves of
+ * 16 elements per workgroup.  A tile is coded as waves + 16 x form (4 is
+ * form 0 at 4 waves).  spoc_autotune times all 9 pairs of every kernel on
+ * the current mesh and keeps the fastest for `variant` (tiles[0] = lin/bc,
+ * tiles[1] = eg, 0 for lin).  It also runs by itself on the first
+ * mixed-precision call after a mesh change.  spoc_set_tiles forces a code
+ * (0 = automatic) and turns the automatic tuning on or off.  Results do not
+ * depend on the tiles.
  */
 int spoc_autotune(spoc_ctx *ctx, int variant, int *tiles);
 int spoc_set_tiles(spoc_ctx *ctx, int variant, int tile0, int tile1, int autotune);

@@ -437,8 +437,9 @@ class Context:
         return r
 
     def autotune(self, variant):
-        """Time the element kernels at 2, 4 and 8 waves per workgroup on this
-        mesh; returns the (lin|bc, eg) tiles kept."""
+        """Time the element kernels in each compiled form at 2, 4 and 8 waves
+        per workgroup on this mesh; returns the (lin|bc, eg) tile codes kept
+        (waves per workgroup + 16 x form)."""
         var = VARIANTS[variant] if isinstance(variant, str) else variant
         out = (C.c_int * 2)()
         _check(self.lib.spoc_autotune(self._h, var, out))

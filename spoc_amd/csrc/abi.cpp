@@ -17,6 +17,7 @@
 
 #include "common.h"
 #include "error.h"
+#include "f32_types.h"
 #include "launch.h"
 
 using namespace spoc;
@@ -1255,9 +1256,13 @@ int spoc_autotune(spoc_ctx *c, int variant, int *tiles) {
 int spoc_set_tiles(spoc_ctx *c, int variant, int tile0, int tile1, int autotune) {
   if (!c) return fail(SPOC_EINVAL, "null context");
   if (variant < SPOC_LIN || variant > SPOC_INV) return fail(SPOC_EINVAL, "variant=%d", variant);
-  for (int t : {tile0, tile1})
-    if (t != 0 && t != 2 && t != 4 && t != 8)
-      return fail(SPOC_EINVAL, "tile %d: choose 2, 4 or 8 waves per workgroup (0: automatic)", t);
+  for (int t : {tile0, tile1}) {
+    const int wg = t & 15, form = t >> 4;
+    if (t != 0 && (t < 0 || (wg != 2 && wg != 4 && wg != 8) || form >= TILE_FORMS))
+      return fail(SPOC_EINVAL,
+                  "tile %d: 2, 4 or 8 waves per workgroup plus 16 x a kernel form 0..%d "
+                  "(0: automatic)", t, TILE_FORMS - 1);
+  }
   drop_graph(c);
   f32_set_tiles(c->f32, variant, tile0, tile1, autotune);
   return SPOC_OK;

@@ -477,10 +477,12 @@ static int rhs_kernels(F32Plan *p, const F32Args &a, TimerHook hook, void *user)
   return rc;
 }
 
-// Time each element kernel of `variant` at every workgroup size in
-// TILE_CHOICES on the current mesh (tendency evaluations, which leave the
-// RK4 state untouched) and keep the fastest.  Results do not depend on the
-// choice: it only changes how element blocks map to workgroups.
+// Time each element kernel of `variant` in every compiled form (register
+// budget, E+G split) at every workgroup size in TILE_CHOICES on the current
+// mesh (tendency evaluations, which leave the RK4 state untouched) and keep
+// the fastest, coded as waves + 16 x form.  Results do not depend on the
+// choice: it only changes how element blocks map to waves and workgroups
+// and how the compiler allocates registers.
 int f32_ready(const F32Plan *p);
 int f32_autotune(F32Plan *p, hipStream_t s, int variant, int *out) {
   int rc = f32_ready(p);
@@ -500,8 +502,10 @@ int f32_autotune(F32Plan *p, hipStream_t s, int variant, int *out) {
     const int phase = variant == 0 ? 3 : 4 + k;
     float best = 1e30f;
     int pick = 4;
-    for (int wg : TILE_CHOICES) {
-      a.tile[k] = wg;
+    for (int code = 0; code < 16 * TILE_FORMS && !rc; code++) {
+      const int wg = code & 15;
+      if (wg != TILE_CHOICES[0] && wg != TILE_CHOICES[1] && wg != TILE_CHOICES[2]) continue;
+      a.tile[k] = code;
       if ((rc = call(p, a, phase, nullptr, nullptr))) break;  // warm-up
       (void)hipEventRecord(e0, s);
       for (int r = 0; r < 3 && !rc; r++) rc = call(p, a, phase, nullptr, nullptr);
@@ -514,7 +518,7 @@ int f32_autotune(F32Plan *p, hipStream_t s, int variant, int *out) {
       }
       if (ms < best) {
         best = ms;
-        pick = wg;
+        pick = code;
       }
     }
     a.tile[k] = pick;

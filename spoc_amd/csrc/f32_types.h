@@ -182,9 +182,11 @@ struct F32State {
   double *hbbar;
 };
 
-// Workgroup sizes (waves of 16 elements) the element kernels can run at;
-// the autotuner picks one per kernel.
+// Workgroup sizes (waves of 16 elements) the element kernels can run at,
+// and the number of compiled forms of each (register budget, E+G split); the
+// autotuner picks one pair per kernel, coded as waves + 16 x form.
 constexpr int TILE_CHOICES[3] = {2, 4, 8};
+constexpr int TILE_FORMS = 3;
 
 struct F32Args {
   int variant;

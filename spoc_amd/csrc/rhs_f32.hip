@@ -33,28 +33,35 @@
 namespace spoc {
 namespace f32 {
 
-// E+G gives each block two waves (one for the u and v equations, one for
-// eta) at p = 5 and 6: either half fits in 128 registers where the whole
-// needs about 165, so four waves share a SIMD instead of three (measured:
-// E+G 240 -> 234 us at p = 6 on 199,362 triangles, 52 -> 46 us at p = 5 on
-// 49,952).  At p = 4 the whole already fits four waves and at p = 7 the
-// halves still need two; there one wave per block measured faster.
-constexpr bool eg_split(int nc) { return nc == 5 || nc == 6; }
-// Register budgets as waves per SIMD (512 registers per lane shared by the
-// resident waves): the element kernels are memory-latency bound, so the
-// compiler is held to at least three resident waves (four for a split E+G).
-#ifndef BC_WAVES
-#define BC_WAVES 3
-#endif
-#ifndef EG_WAVES
-#define EG_WAVES 3
-#endif
-#ifndef EG_SPLIT_WAVES
-#define EG_SPLIT_WAVES 4
-#endif
-// (at p >= 7, 36 modes, both kernels get two waves, or they spill)
-constexpr int bc_waves(int nc) { return nc >= 7 ? 2 : BC_WAVES; }
-constexpr int eg_waves(int nc) { return nc >= 7 ? 2 : eg_split(nc) ? EG_SPLIT_WAVES : EG_WAVES; }
+// Kernel forms.  Each element kernel is compiled in three forms, which the
+// autotuner times on the caller's mesh with each workgroup size (a tile code
+// is waves per workgroup + 16 x form).  The forms differ in the register
+// budget, given as resident waves per SIMD (512 registers per lane shared by
+// the resident waves), and for E+G in the split of a block's work:
+//  - split: each block of 16 elements gets two waves, one for the u and v
+//    equations and one for eta; at p = 5 and 6 either half fits in 128
+//    registers where the whole needs about 165, so four waves share a SIMD
+//    instead of three (measured: E+G 240 -> 234 us at p = 6 on 199,362
+//    triangles, 52 -> 46 us at p = 5 on 49,952);
+//  - form 0 is the measured default: B+C at 3 waves (2 at p = 7, whose 36
+//    modes spill at 3), E+G split at 4 waves at p = 5 and 6 and whole at 3
+//    (p <= 4) or 2 (p = 7), the linear kernel at 3;
+//  - form 1 trades waves for registers (B+C and the linear kernel one wave
+//    fewer; E+G the other split: whole at 3 or 2 waves where form 0 splits,
+//    split where form 0 does not);
+//  - form 2 asks for one wave more than form 0 (B+C and the linear kernel)
+//    or gives the default E+G split more registers (one wave fewer).
+constexpr int NFORMS = TILE_FORMS;
+constexpr bool eg_split0(int nc) { return nc == 5 || nc == 6; }
+constexpr int bc_waves(int nc, int form) {
+  return form == 1 ? (nc >= 7 ? 1 : 2) : form == 2 ? (nc >= 7 ? 3 : 4) : (nc >= 7 ? 2 : 3);
+}
+constexpr bool eg_split(int nc, int form) { return form == 1 ? !eg_split0(nc) : eg_split0(nc); }
+constexpr int eg_waves(int nc, int form) {
+  return form == 2 ? (nc >= 7 ? 1 : eg_split0(nc) ? 3 : 2)
+                   : (nc >= 7 ? 2 : eg_split(nc, form) ? 4 : 3);
+}
+constexpr int lin_waves(int form) { return form == 1 ? 2 : form == 2 ? 4 : 3; }
 
 // Stage input of field f: the step's start state c0 at the first stage (and
 // for a bare right-hand side), else what the previous stage stored in
@@ -441,8 +448,8 @@ __global__ __launch_bounds__(NT) void prep_kernel(F32Args A, int nblk) {
 // ---------------------------------------------------------------------------
 // Linear right-hand side (src/calvar_lin.f:7-287) + RK4 stage, all FP64.
 // ---------------------------------------------------------------------------
-template <int NC, int NG, int NGF, int WG, bool LF>
-__global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(3))) void lin_kernel(F32Args A, int nblk) {
+template <int NC, int NG, int NGF, int WG, bool LF, int FORM>
+__global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(lin_waves(FORM)))) void lin_kernel(F32Args A, int nblk) {
   using Sh = S<NC, NG, NGF>;
   constexpr Pack64 P = Sh::P64;
   __shared__ floatx4 lds[P.total / 2];
@@ -470,8 +477,8 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(3))) vo
 // Stores that volume part (gvol, FP32, as computed) and the flux traces
 // (trS, FP32); the linear core and every face term are E+G's.
 // ---------------------------------------------------------------------------
-template <int NC, int NG, int NGF, int VAR, int WG>
-__global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_waves(NC)))) void bc_kernel(F32Args A, int nblk) {
+template <int NC, int NG, int NGF, int VAR, int WG, int FORM>
+__global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_waves(NC, FORM)))) void bc_kernel(F32Args A, int nblk) {
   using Sh = S<NC, NG, NGF>;
   constexpr int NM = Sh::NM, NE3 = Sh::NE3, KM = Sh::KM, KE = Sh::KE, KG = Sh::KG,
                 RBM = Sh::RBM, RBE = Sh::RBE, RBG = Sh::RBG;
@@ -841,12 +848,12 @@ __device__ __forceinline__ void eg_part(const F32Args &A, const double *O64, int
 // RK4 stage, velocity traces) and one for the eta equation (core, lift, RK4
 // stage, eta traces); WG waves per workgroup then cover WG / 2 blocks.
 // Otherwise one wave does the whole block.
-template <int NC, int NG, int NGF, int VAR, int WG, bool RUS>
-__global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(eg_waves(NC)))) void eg_kernel(F32Args A, int nblk) {
+template <int NC, int NG, int NGF, int VAR, int WG, bool RUS, int FORM>
+__global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(eg_waves(NC, FORM)))) void eg_kernel(F32Args A, int nblk) {
   using Sh = S<NC, NG, NGF>;
   constexpr Pack64 P64 = Sh::P64;
   constexpr int N64 = P64.total / 2;  // 16-byte words
-  constexpr bool SPLIT = eg_split(NC);
+  constexpr bool SPLIT = eg_split(NC, FORM);
   __shared__ floatx4 lds[N64];
   const int b0 = first_block<SPLIT ? WG / 2 : WG>();
   if (b0 >= nblk) return;
@@ -926,31 +933,40 @@ __global__ __launch_bounds__(PSI_WG) void from_psi_kernel(int ne, const double *
 }
 
 // Element kernels of one right-hand side at a chosen workgroup size (waves
-// of 16 elements per workgroup: the tile the autotuner picks).
-template <int NC, int NG, int NGF, int WG>
+// of 16 elements per workgroup) and kernel form: the tile the autotuner
+// picks.  The Lax-Friedrichs option runs form 0 only.
+template <int NC, int NG, int NGF, int WG, int FORM>
 int launch_elem(const F32Args &A, int phase, int nblk) {
   // a multiple of the 8 XCDs; a split E+G gives each block two waves
-  const int grid = 8 * cdiv(cdiv(nblk, phase == 5 && eg_split(NC) ? WG / 2 : WG), 8);
+  const int grid = 8 * cdiv(cdiv(nblk, phase == 5 && eg_split(NC, FORM) ? WG / 2 : WG), 8);
   hipStream_t st = A.stream;
   const int v = A.variant;
   const bool lf = A.lf_alpha != 0.0;
   if (phase == 3) {
-    if (lf) lin_kernel<NC, NG, NGF, WG, true><<<grid, 64 * WG, 0, st>>>(A, nblk);
-    else lin_kernel<NC, NG, NGF, WG, false><<<grid, 64 * WG, 0, st>>>(A, nblk);
+    if (lf) lin_kernel<NC, NG, NGF, WG, true, 0><<<grid, 64 * WG, 0, st>>>(A, nblk);
+    else lin_kernel<NC, NG, NGF, WG, false, FORM><<<grid, 64 * WG, 0, st>>>(A, nblk);
   } else if (phase == 4) {
-    if (v == 1) bc_kernel<NC, NG, NGF, 1, WG><<<grid, 64 * WG, 0, st>>>(A, nblk);
-    else if (v == 2) bc_kernel<NC, NG, NGF, 2, WG><<<grid, 64 * WG, 0, st>>>(A, nblk);
-    else bc_kernel<NC, NG, NGF, 3, WG><<<grid, 64 * WG, 0, st>>>(A, nblk);
+    if (v == 1) bc_kernel<NC, NG, NGF, 1, WG, FORM><<<grid, 64 * WG, 0, st>>>(A, nblk);
+    else if (v == 2) bc_kernel<NC, NG, NGF, 2, WG, FORM><<<grid, 64 * WG, 0, st>>>(A, nblk);
+    else bc_kernel<NC, NG, NGF, 3, WG, FORM><<<grid, 64 * WG, 0, st>>>(A, nblk);
   } else if (lf) {
-    if (v == 1) eg_kernel<NC, NG, NGF, 1, WG, true><<<grid, 64 * WG, 0, st>>>(A, nblk);
-    else if (v == 2) eg_kernel<NC, NG, NGF, 2, WG, true><<<grid, 64 * WG, 0, st>>>(A, nblk);
-    else eg_kernel<NC, NG, NGF, 3, WG, true><<<grid, 64 * WG, 0, st>>>(A, nblk);
+    const int g0 = 8 * cdiv(cdiv(nblk, eg_split(NC, 0) ? WG / 2 : WG), 8);
+    if (v == 1) eg_kernel<NC, NG, NGF, 1, WG, true, 0><<<g0, 64 * WG, 0, st>>>(A, nblk);
+    else if (v == 2) eg_kernel<NC, NG, NGF, 2, WG, true, 0><<<g0, 64 * WG, 0, st>>>(A, nblk);
+    else eg_kernel<NC, NG, NGF, 3, WG, true, 0><<<g0, 64 * WG, 0, st>>>(A, nblk);
   } else {
-    if (v == 1) eg_kernel<NC, NG, NGF, 1, WG, false><<<grid, 64 * WG, 0, st>>>(A, nblk);
-    else if (v == 2) eg_kernel<NC, NG, NGF, 2, WG, false><<<grid, 64 * WG, 0, st>>>(A, nblk);
-    else eg_kernel<NC, NG, NGF, 3, WG, false><<<grid, 64 * WG, 0, st>>>(A, nblk);
+    if (v == 1) eg_kernel<NC, NG, NGF, 1, WG, false, FORM><<<grid, 64 * WG, 0, st>>>(A, nblk);
+    else if (v == 2) eg_kernel<NC, NG, NGF, 2, WG, false, FORM><<<grid, 64 * WG, 0, st>>>(A, nblk);
+    else eg_kernel<NC, NG, NGF, 3, WG, false, FORM><<<grid, 64 * WG, 0, st>>>(A, nblk);
   }
   return 0;
+}
+
+template <int NC, int NG, int NGF, int WG>
+int launch_form(const F32Args &A, int phase, int nblk, int form) {
+  if (form == 1) return launch_elem<NC, NG, NGF, WG, 1>(A, phase, nblk);
+  if (form == 2) return launch_elem<NC, NG, NGF, WG, 2>(A, phase, nblk);
+  return launch_elem<NC, NG, NGF, WG, 0>(A, phase, nblk);
 }
 
 template <int NC, int NG, int NGF>
@@ -977,10 +993,12 @@ int launch(const F32Args &A, int phase, const double *in, double *out) {
     case 4:  // nonlinear B+C
     case 5: {  // nonlinear E+G
       if (phase > 3 && v == 0) return -1;
-      const int wg = A.tile[phase == 5 ? 1 : 0];
-      if (wg == 2) launch_elem<NC, NG, NGF, 2>(A, phase, nblk);
-      else if (wg == 8) launch_elem<NC, NG, NGF, 8>(A, phase, nblk);
-      else launch_elem<NC, NG, NGF, 4>(A, phase, nblk);
+      // tile code: waves per workgroup + 16 x kernel form
+      const int code = A.tile[phase == 5 ? 1 : 0], wg = code & 15, form = code >> 4;
+      if (form < 0 || form >= NFORMS) return -1;
+      if (wg == 2) launch_form<NC, NG, NGF, 2>(A, phase, nblk, form);
+      else if (wg == 8) launch_form<NC, NG, NGF, 8>(A, phase, nblk, form);
+      else launch_form<NC, NG, NGF, 4>(A, phase, nblk, form);
       break;
     }
     default:

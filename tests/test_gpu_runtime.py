@@ -130,16 +130,20 @@ def test_graph_replay_equals_eager(prec, variant):
 @pytest.mark.parametrize("variant", ["lin", "fs", "ns"])
 def test_tiles_do_not_change_results(variant):
     """The autotuner's choices (2, 4 or 8 waves of 16 elements per
-    workgroup) only change how element blocks map to workgroups."""
+    workgroup, each kernel in 3 compiled forms: register budget and the E+G
+    split) only change how element blocks map to waves and workgroups."""
     t, ctx = ctx_for(6, 33, 7)
     c = configs.random_case(t, 14, 12, seed=77, scale=0.01)
     ctx.set_mesh(c["mesh"])
     set_fields(ctx, c["phys"])
     tiles = ctx.autotune(variant)
     say("autotuned tiles", variant, tiles)
-    assert tiles[0] in (2, 4, 8) and tiles[1] in ((0,) if variant == "lin" else (2, 4, 8))
+    assert tiles[0] & 15 in (2, 4, 8) and tiles[0] >> 4 in (0, 1, 2)
+    assert tiles[1] == 0 if variant == "lin" else (tiles[1] & 15 in (2, 4, 8) and
+                                                   tiles[1] >> 4 in (0, 1, 2))
     res = []
-    for t0, t1 in [(2, 2), (4, 4), (8, 8), (2, 8), (8, 4)]:
+    for t0, t1 in [(2, 2), (4, 4), (8, 8), (2, 8), (8, 4), (4 + 16, 4 + 16), (8 + 32, 2 + 32),
+                   (2 + 16, 8 + 32), (4 + 32, 4 + 16)]:
         ctx.set_tiles(variant, t0, t1)
         ctx.upload(*c["state"])
         ctx.rk4(variant, 2.0, 3, precision="fp32")
