@@ -122,7 +122,9 @@ def adjoint_check(ctx, t, c, nsteps, ndir, seed=4001):
     within 200 km of the centre, after nsteps, with respect to the initial
     eta and to hb: adjoint time (checkpoints every 50 steps) and relative
     agreement with central differences of the FP64 path (step 1e-6 relative
-    to the direction's scale) along ndir seeded random directions."""
+    to the direction's scale) along ndir seeded random directions.  With
+    ndir = 0 only the adjoint is timed, and J and its seed come from the
+    mixed path's forward run (no FP64 forward runs)."""
     m, p = c["mesh"], c["phys"]
     nm, ne, variant, dt = t["nm"], m["ne"], c["variant"], c["dt"]
     M = t["ammo"][:nm, :nm]
@@ -132,15 +134,15 @@ def adjoint_check(ctx, t, c, nsteps, ndir, seed=4001):
     inside = (cx ** 2 + cy ** 2) < 200e3 ** 2
     x0 = [np.asfortranarray(a) for a in c["state"]]
 
-    def fwd(x, hb):
+    def fwd(x, hb, prec="fp64"):
         ctx.set_fields(hb, p["g0"], p["fc"], p["w0"], p["z0"], p["mu0"], p["mb0"])
         ctx.upload(*x)
-        ctx.rk4(variant, dt, nsteps, precision="fp64")
+        ctx.rk4(variant, dt, nsteps, precision=prec)
         h = ctx.download()[2][:nm]
         return h, float(np.sum(det[inside] * np.einsum("ie,ij,je->e", h[:, inside], M,
                                                         h[:, inside])))
 
-    h, J0 = fwd(x0, p["hb"])
+    h, J0 = fwd(x0, p["hb"], "fp64" if ndir else "fp32")
     lam = np.zeros((x0[0].shape[0], ne), order="F")
     lam[:nm, inside] = 2.0 * det[inside] * (M @ h[:, inside])
     zero = np.zeros_like(lam)
@@ -508,6 +510,9 @@ def main():
                          "eta^2 integral over the central 200 km disc after --steps steps, "
                          "with respect to the initial eta and to hb) and check it against "
                          "central differences of the FP64 path along NDIR seeded directions")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="with --adjoint: time the adjoint alone, without the central-difference "
+                         "check (no FP64 forward runs)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--nx", type=int, default=448)
     ap.add_argument("--ny", type=int, default=224)
@@ -701,7 +706,7 @@ def main():
         out["dropin"] = dropin_timing(ctx, t, variant, dt, state0, args.dropin_steps)
     if args.adjoint and wl == "c4":
         phase("adjoint and central differences")
-        out["adjoint"] = adjoint_check(ctx, t, c, K, args.adjoint)
+        out["adjoint"] = adjoint_check(ctx, t, c, K, 0 if args.adjoint_only else args.adjoint)
     phase("refinement indicator")
     out["cal_conv"] = adapt_timing(ctx, s32, ne, nc, ng, ngf, cpu=not args.no_cpu)
     if not args.no_cpu:
