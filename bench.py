@@ -120,7 +120,7 @@ def measured_traffic(kernel, nc, ne):
 def adjoint_check(ctx, t, c, nsteps, ndir, seed=4001):
     """Gradient of J = integral of eta^2 over the elements whose centroid is
     within 200 km of the centre, after nsteps, with respect to the initial
-    eta and to hb: adjoint time (checkpoints every 50 steps) and relative
+    eta and to hb: adjoint time (automatic checkpointing) and relative
     agreement with central differences of the FP64 path (step 1e-6 relative
     to the direction's scale) along ndir seeded random directions.  With
     ndir = 0 only the adjoint is timed, and J and its seed come from the
@@ -148,7 +148,7 @@ def adjoint_check(ctx, t, c, nsteps, ndir, seed=4001):
     zero = np.zeros_like(lam)
     ctx.upload(*x0)
     ctx.adjoint_upload(zero, zero, lam)
-    ctx.set_checkpoint(50)
+    ctx.set_checkpoint(0)  # every stage input kept when it fits, else checkpoints every 50
     ctx.sync()
     t0 = time.perf_counter()
     ctx.rk4_adjoint(variant, dt, nsteps)
@@ -178,7 +178,8 @@ def adjoint_check(ctx, t, c, nsteps, ndir, seed=4001):
     return {"J": J0, "steps": nsteps, "adjoint_s": adj_s,
             "adjoint_ms_per_step": 1e3 * adj_s / max(nsteps, 1),
             "rel_err_vs_central_differences": {k: max(v) if v else None for k, v in errs.items()},
-            "directions": ndir, "checkpoint_every": 50}
+            "directions": ndir, "checkpoints": "automatic (all stage inputs kept when they fit "
+                                                "in half the free device memory, else every 50 steps)"}
 
 
 def calib_copy(nbytes):

@@ -223,9 +223,13 @@ int spoc_set_graph(spoc_ctx *ctx, int on);
  * returns that of the initial state, spoc_adjoint_hb_download the gradient
  * with respect to hb.  For the nonlinear variants the trajectory starts at
  * the device-resident state (spoc_state_upload; left unchanged) and is
- * recomputed with the mixed-precision forward path from checkpoints taken
- * every spoc_set_checkpoint steps (default 50).  Everything runs in FP64 on
- * MFMA.
+ * computed by the mixed-precision forward path.  spoc_set_checkpoint(ctx,
+ * every): every >= 1 keeps checkpoints every `every` steps and recomputes
+ * the steps between them; every = 0 (the default) keeps every step's four
+ * stage inputs from the forward sweep when they fit in half the device's
+ * free memory, so that nothing is recomputed, and takes checkpoints every 50
+ * steps otherwise.  Results do not depend on the choice.  Everything runs
+ * in FP64 on MFMA.
  */
 int spoc_rhs_adjoint(spoc_ctx *ctx, int variant, int ld, const double *u0, const double *v0,
                      const double *h0, const double *au, const double *av, const double *ah,

@@ -357,6 +357,9 @@ class Context:
         return out
 
     def set_checkpoint(self, every: int) -> None:
+        """Nonlinear adjoint: checkpoints every `every` steps (recomputing
+        the steps between), or 0 (the default) to keep every step's stage
+        inputs when they fit in device memory."""
         _check(self.lib.spoc_set_checkpoint(self._h, every))
 
     def adjoint_upload(self, au, av, ah) -> None:

@@ -223,7 +223,7 @@ struct spoc_ctx {
   int *dmem = nullptr;
   double *dadj = nullptr;    // adjoint state, reference layout (nm, ne) x 3, then hb's adjoint
   bool have_adj = false, have_adj_hb = false;
-  int ncheck = 50;           // checkpoint interval of the nonlinear adjoint
+  int ncheck = 0;            // checkpoint interval of the nonlinear adjoint (0: automatic)
   bool have_state = false;
   hipEvent_t t0 = nullptr, t1 = nullptr;
   double last_ms = 0.0;
@@ -1212,7 +1212,7 @@ int spoc_adjoint_hb_download(spoc_ctx *c, int ld, double *ahb) {
 
 int spoc_set_checkpoint(spoc_ctx *c, int every) {
   if (!c) return fail(SPOC_EINVAL, "null context");
-  if (every < 1) return fail(SPOC_EINVAL, "checkpoint interval %d", every);
+  if (every < 0) return fail(SPOC_EINVAL, "checkpoint interval %d", every);
   c->ncheck = every;
   return SPOC_OK;
 }

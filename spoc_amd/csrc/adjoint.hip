@@ -346,21 +346,27 @@ __global__ __launch_bounds__(NT) void adj_btr_kernel(F32Args A, int nblk) {
 }
 
 // Pass 2 of R^T for the nonlinear variants (FP64, operators read from
-// global memory).  In order: the face adjoints of pass 1 (eta trace,
-// mass-flux trace, viscous-flux traces); the adjoint of the viscous flux
-// q-bar = mu F-bar + T^T (traces); the base state's values at the
-// quadrature points (u, v, eta and the base gradient q, recomputed from xb
-// and btr as the forward B stage computes them); the transposed pointwise
-// products of src/calvarnl_fs.f:245-296 and src/nonlin.f:35-133
+// global memory), in two kernels.  adj_mid1: the face adjoints of pass 1
+// (eta trace, mass-flux trace, viscous-flux traces) gathered and lifted,
+// added in place to the element-local work arrays of pass 1: avol (the
+// partial result), aUH (the mass-flux adjoints) and aF, which then holds
+// the adjoint of the gradient q, q-bar = mu F-bar + T^T (traces).
+// adj_mid2: the base state's values at the quadrature points (u, v, eta and
+// the base gradient q, recomputed from xb and btr as the forward B stage
+// computes them); the transposed pointwise products of
+// src/calvarnl_fs.f:245-296 and src/nonlin.f:35-133
 //   gx = -(u qxx + v (qxy - f)) + (w0 - mb0 u) / (eta + hb)
 //   gy = -(u (qyx + f) + v qyy) + (z0 - mb0 v) / (eta + hb)
 //   uh = u (eta + hb),  vh = v (eta + hb);
 // projection back to modes (PSI^T), which also gives the adjoint of hb at
 // the quadrature points (hb enters only through eta + hb); the wall
 // projection of q-bar and the transposed gradient of src/calvarnl_fs.f:129-158.
-// Writes the partial result to avol and the gradient's face adjoints to abar2.
+// Writes the partial result to avol and the gradient's face adjoints to
+// abar2.  (One kernel holding all of it needed about 660 registers per lane
+// at p = 5 and spilled; the hand-off through the work arrays it already
+// reads costs less.)
 template <int NC, int NG, int NGF, int VAR>
-__global__ __launch_bounds__(NT) void adj_mid_kernel(F32Args A, int nblk) {
+__global__ __launch_bounds__(NT) void adj_mid1_kernel(F32Args A, int nblk) {
   using Sh = S<NC, NG, NGF>;
   constexpr int NM = Sh::NM, NE3 = Sh::NE3, KM = Sh::KM, KE = Sh::KE, KG = Sh::KG, RBM = Sh::RBM,
                 RBE = Sh::RBE, RBG = Sh::RBG;
@@ -380,7 +386,7 @@ __global__ __launch_bounds__(NT) void adj_mid_kernel(F32Args A, int nblk) {
   const bool anyw = __ballot(wall) != 0;
   const double nwx = wall ? gg[14] : 0.0, nwy = wall ? gg[15] : 0.0;
 
-  // (1) face adjoints of pass 1
+  // face adjoints of pass 1
   double yu[KM], yv[KM], yh[KM], uhr[KM], vhr[KM];
   double q0[KM], q1[KM], q2[KM], q3[KM];  // adjoint of the gradient q (B fragments)
   {
@@ -457,6 +463,48 @@ __global__ __launch_bounds__(NT) void adj_mid_kernel(F32Args A, int nblk) {
       q3[s] += mu0 * f3;
     }
   }
+#pragma unroll
+  for (int s = 0; s < KM; s++) {
+    const int m = 4 * s + g;
+    if (valid && m < NM) {
+      const size_t idx = mb + m * 16 + col;
+      A.st.avol[0][idx] = yu[s];
+      A.st.avol[1][idx] = yv[s];
+      A.st.avol[2][idx] = yh[s];
+      A.st.aUH[0][idx] = uhr[s];
+      A.st.aUH[1][idx] = vhr[s];
+      A.st.aF[0][idx] = q0[s];
+      A.st.aF[1][idx] = q1[s];
+      A.st.aF[2][idx] = q2[s];
+      A.st.aF[3][idx] = q3[s];
+    }
+  }
+}
+
+template <int NC, int NG, int NGF, int VAR>
+__global__ __launch_bounds__(NT) void adj_mid2_kernel(F32Args A, int nblk) {
+  using Sh = S<NC, NG, NGF>;
+  constexpr int NM = Sh::NM, NE3 = Sh::NE3, KM = Sh::KM, KE = Sh::KE, KG = Sh::KG, RBM = Sh::RBM,
+                RBE = Sh::RBE, RBG = Sh::RBG;
+  constexpr Pack64 P = Sh::P64;
+  constexpr Pack64T T = pack64t(NM, NG, NE3);
+  const int eb = first_block() + (threadIdx.x >> 6);
+  if (eb >= nblk) return;
+  const double *O = A.ops.pack64, *OT = A.ops.pack64t;
+  const int l = lane(), g = l >> 4, col = l & 15, e0 = eb * 16, e = e0 + col;
+  const bool valid = e < A.mesh.ne;
+  const double *gg = A.mesh.geo64 + (size_t)(valid ? e : 0) * GEO;
+  const double ax = gg[0], bx = gg[1], ay = gg[2], by = gg[3], di = valid ? gg[4] : 0.0;
+  const double hh0 = valid ? core_depth<VAR>(A.fld, e) : 0.0;
+  const double mu0 = A.fld.mu0, mb0 = A.fld.mb0;
+  const size_t mb = (size_t)e0 * NM;
+  const bool wall = valid && A.mesh.bdl[e] != 0;
+  const bool anyw = __ballot(wall) != 0;
+  const double nwx = wall ? gg[14] : 0.0, nwy = wall ? gg[15] : 0.0;
+
+  double uhr[KM], vhr[KM];
+  bload<NM>(A.st.aUH[0] + mb, g, col, valid, uhr);
+  bload<NM>(A.st.aUH[1] + mb, g, col, valid, vhr);
   double lsu[KM], lsv[KM];
   bload<NM>(A.st.aL[0] + mb, g, col, valid, lsu);
   bload<NM>(A.st.aL[1] + mb, g, col, valid, lsv);
@@ -566,6 +614,15 @@ __global__ __launch_bounds__(NT) void adj_mid_kernel(F32Args A, int nblk) {
     }
   }
 
+  // the partial result and q-bar of adj_mid1
+  double yu[KM], yv[KM], yh[KM], q0[KM], q1[KM], q2[KM], q3[KM];
+  bload<NM>(A.st.avol[0] + mb, g, col, valid, yu);
+  bload<NM>(A.st.avol[1] + mb, g, col, valid, yv);
+  bload<NM>(A.st.avol[2] + mb, g, col, valid, yh);
+  bload<NM>(A.st.aF[0] + mb, g, col, valid, q0);
+  bload<NM>(A.st.aF[1] + mb, g, col, valid, q1);
+  bload<NM>(A.st.aF[2] + mb, g, col, valid, q2);
+  bload<NM>(A.st.aF[3] + mb, g, col, valid, q3);
   // (4) combine; the quadrature part of the eta adjoint is also hb's
 #pragma unroll
   for (int s = 0; s < KM; s++) {
@@ -790,9 +847,16 @@ int adj_launch(const F32Args &A, int phase, const double *in, double *out) {
       break;
     case 4:  // pass 2 (the last pass of the linear variant)
       if (v == 0) adj_gather_kernel<NC, NG, NGF, 0><<<grid, NT, 0, st>>>(A, nblk);
-      else if (v == 1) adj_mid_kernel<NC, NG, NGF, 1><<<grid, NT, 0, st>>>(A, nblk);
-      else if (v == 2) adj_mid_kernel<NC, NG, NGF, 2><<<grid, NT, 0, st>>>(A, nblk);
-      else adj_mid_kernel<NC, NG, NGF, 3><<<grid, NT, 0, st>>>(A, nblk);
+      else if (v == 1) {
+        adj_mid1_kernel<NC, NG, NGF, 1><<<grid, NT, 0, st>>>(A, nblk);
+        adj_mid2_kernel<NC, NG, NGF, 1><<<grid, NT, 0, st>>>(A, nblk);
+      } else if (v == 2) {
+        adj_mid1_kernel<NC, NG, NGF, 2><<<grid, NT, 0, st>>>(A, nblk);
+        adj_mid2_kernel<NC, NG, NGF, 2><<<grid, NT, 0, st>>>(A, nblk);
+      } else {
+        adj_mid1_kernel<NC, NG, NGF, 3><<<grid, NT, 0, st>>>(A, nblk);
+        adj_mid2_kernel<NC, NG, NGF, 3><<<grid, NT, 0, st>>>(A, nblk);
+      }
       break;
     case 5:  // pass 3 (nonlinear)
       if (v == 0) return -1;

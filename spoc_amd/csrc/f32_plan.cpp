@@ -642,10 +642,14 @@ int f32_adj_rhs(F32Plan *p, hipStream_t s, int variant, const double *const *x,
 
 // nsteps adjoint RK4 steps backwards from lam (reference coordinates, in
 // place), linearised along the trajectory that starts at x0 (reference
-// coordinates; unused by the linear variant) and is recomputed by the
-// mixed-precision forward path: checkpoints every `ncheck` steps, then per
-// segment the step states, then per step its four stage inputs.  hb (may be
-// null) receives the accumulated adjoint of the bathymetry.
+// coordinates; unused by the linear variant), computed by the
+// mixed-precision forward path.  ncheck >= 1: checkpoints every `ncheck`
+// steps, then per segment the step states are recomputed, then per step its
+// four stage inputs.  ncheck <= 0 (automatic): when every step's four stage
+// inputs fit in half the device's free memory (at 49,952 triangles and
+// p = 5, 100 MB per step) the forward sweep keeps them all and nothing is
+// recomputed; otherwise checkpoints every 50 steps.  hb (may be null)
+// receives the accumulated adjoint of the bathymetry.
 int f32_adj_run(F32Plan *p, hipStream_t s, int variant, double dt, int nsteps, int ncheck,
                 const double *const *x0, double *const *lam, double *hb) {
   int rc = f32_ready(p);
@@ -654,18 +658,27 @@ int f32_adj_run(F32Plan *p, hipStream_t s, int variant, double dt, int nsteps, i
   const size_t n64 = (size_t)p->nepad * p->nm, sbytes = 3 * n64 * sizeof(double);
   double *mem = nullptr;
   int nseg = 0;
+  bool keep_all = false;  // every step's four stage inputs from the forward sweep
   if (variant != 0) {
     if (!x0) return -1;
-    if (ncheck < 1) ncheck = 1;
+    if (ncheck < 1) {
+      size_t freeb = 0, totalb = 0;
+      if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) return -2;
+      keep_all = nsteps > 0 && sbytes * 4 * (size_t)nsteps <= freeb / 2;
+      ncheck = keep_all ? std::max(nsteps, 1) : 50;
+    }
     nseg = (nsteps + ncheck - 1) / ncheck;
-    // checkpoints (nseg), one segment's step states (ncheck), the 4 stage inputs
-    if (hipMalloc(&mem, sbytes * (nseg + ncheck + 4)) != hipSuccess) return -2;
+    // keep_all: 4 stage inputs per step; else checkpoints (nseg), one
+    // segment's step states (ncheck) and the 4 stage inputs of one step
+    const size_t nstates = keep_all ? 4 * (size_t)nsteps : (size_t)(nseg + ncheck + 4);
+    if (hipMalloc(&mem, sbytes * nstates) != hipSuccess) return -2;
   }
-  auto state = [&](int i) { return mem + (size_t)i * 3 * n64; };
+  auto state = [&](size_t i) { return mem + i * 3 * n64; };
   auto copy = [&](double *dst, const double *src) {
     return hipMemcpyAsync(dst, src, sbytes, hipMemcpyDeviceToDevice, s) == hipSuccess ? 0 : -2;
   };
   double *ckpt0 = state(0), *seg0 = state(nseg), *base0 = state(nseg + ncheck);
+  // keep_all: stage input k of step n at state(4 n + k)
   // dispatches queued ahead of the host, bounded per step: a forward step
   // queues at most 8 kernels and a copy, a backward step the 4 stage inputs'
   // recomputation (a copy, the trace pass, 3 right-hand sides, 4 copies) and
@@ -682,11 +695,22 @@ int f32_adj_run(F32Plan *p, hipStream_t s, int variant, double dt, int nsteps, i
       F32Args fa = args(p, s, variant, -1, dt);
       if ((rc = call(p, fa, 2, nullptr, nullptr))) break;
       for (int n = 0; n < nsteps && !rc; n++) {
-        rc = fence(9);
-        if (!rc && n % ncheck == 0) rc = copy(ckpt0 + (size_t)(n / ncheck) * 3 * n64, p->st.c0[0]);
-        for (int st = 0; st < 4 && !rc; st++) {
-          fa.stage = st;
-          rc = rhs_kernels(p, fa, nullptr, nullptr);
+        rc = fence(13);
+        if (keep_all) {
+          // stage input 0 is the step's start state, 1..3 the stored Y2..Y4
+          if (!rc) rc = copy(state(4 * (size_t)n), p->st.c0[0]);
+          for (int st = 0; st < 4 && !rc; st++) {
+            fa.stage = st;
+            rc = rhs_kernels(p, fa, nullptr, nullptr);
+            if (!rc && st < 3) rc = copy(state(4 * (size_t)n + st + 1), p->st.ys[st][0]);
+          }
+        } else {
+          if (!rc && n % ncheck == 0)
+            rc = copy(ckpt0 + (size_t)(n / ncheck) * 3 * n64, p->st.c0[0]);
+          for (int st = 0; st < 4 && !rc; st++) {
+            fa.stage = st;
+            rc = rhs_kernels(p, fa, nullptr, nullptr);
+          }
         }
         if (!rc) rc = mark();
       }
@@ -702,7 +726,7 @@ int f32_adj_run(F32Plan *p, hipStream_t s, int variant, double dt, int nsteps, i
       const int n0 = variant != 0 ? sg * ncheck : 0;
       const int n1 = variant != 0 ? std::min(nsteps, n0 + ncheck) : nsteps;
       F32Args fa = args(p, s, variant, -1, dt);
-      if (variant != 0) {
+      if (variant != 0 && !keep_all) {
         // the segment's step states
         rc = copy(p->st.c0[0], ckpt0 + (size_t)sg * 3 * n64);
         if (!rc) rc = call(p, fa, 2, nullptr, nullptr);
@@ -718,7 +742,8 @@ int f32_adj_run(F32Plan *p, hipStream_t s, int variant, double dt, int nsteps, i
       }
       for (int n = n1 - 1; n >= n0 && !rc; n--) {
         if ((rc = fence(32))) break;
-        if (variant != 0) {
+        double *base = keep_all ? state(4 * (size_t)n) : base0;
+        if (variant != 0 && !keep_all) {
           // the four stage inputs of step n
           rc = copy(p->st.c0[0], seg0 + (size_t)(n - n0) * 3 * n64);
           if (!rc) rc = call(p, fa, 2, nullptr, nullptr);
@@ -734,7 +759,7 @@ int f32_adj_run(F32Plan *p, hipStream_t s, int variant, double dt, int nsteps, i
         for (int as = 0; as < 4 && !rc; as++) {
           a.stage = as;
           if (variant != 0)
-            for (int f = 0; f < 3; f++) a.st.xb[f] = base0 + (size_t)(3 - as) * 3 * n64 + f * n64;
+            for (int f = 0; f < 3; f++) a.st.xb[f] = base + (size_t)(3 - as) * 3 * n64 + f * n64;
           rc = adj_rhs_passes(p, a);
         }
         if (!rc) rc = mark();
