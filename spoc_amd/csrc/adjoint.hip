@@ -365,8 +365,10 @@ __global__ __launch_bounds__(NT) void adj_btr_kernel(F32Args A, int nblk) {
 // abar2.  (One kernel holding all of it needed about 660 registers per lane
 // at p = 5 and spilled; the hand-off through the work arrays it already
 // reads costs less.)
+// (adj_mid1 fits two waves per SIMD in 256 registers up to p = 6: 4% off
+// the adjoint step at p = 5 against one wave with the registers it asks for)
 template <int NC, int NG, int NGF, int VAR>
-__global__ __launch_bounds__(NT) void adj_mid1_kernel(F32Args A, int nblk) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NC <= 6 ? 2 : 1))) void adj_mid1_kernel(F32Args A, int nblk) {
   using Sh = S<NC, NG, NGF>;
   constexpr int NM = Sh::NM, NE3 = Sh::NE3, KM = Sh::KM, KE = Sh::KE, KG = Sh::KG, RBM = Sh::RBM,
                 RBE = Sh::RBE, RBG = Sh::RBG;

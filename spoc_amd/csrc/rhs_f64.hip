@@ -596,7 +596,10 @@ struct QuadLds {
 template <int NC, int NG, int NGF, int VAR>
 __global__ __launch_bounds__(256) void elem_c(F64Args A, const double *__restrict__ wi) {
   constexpr int NM = Sh<NC, NG, NGF>::NM;
-  constexpr int MC = 7;  // modes per pass of the source projections
+  // modes per pass of the source projections: each pass re-reads the weighted
+  // products, so fewer passes where the registers allow (14 at p >= 6: elem_c
+  // -3% at p = 6; at p = 4 and 5 it would halve the resident waves)
+  constexpr int MC = NM >= 28 ? 14 : 7;
   using L = QuadLds<NM, NG>;
   __shared__ double lds[L::SIZE];
   for (int t = threadIdx.x; t < NG * NM; t += 256) lds[L::PQ + t] = A.tab.pi2T[t];
