@@ -45,10 +45,12 @@ namespace f32 {
 //    triangles, 52 -> 46 us at p = 5 on 49,952);
 //  - form 0 is the measured default: B+C at 3 waves (2 at p = 7, whose 36
 //    modes spill at 3), E+G split at 4 waves at p = 5 and 6 and whole at 3
-//    (p <= 4) or 2 (p = 7), the linear kernel at 3;
-//  - form 1 trades waves for registers (B+C and the linear kernel one wave
-//    fewer; E+G the other split: whole at 3 or 2 waves where form 0 splits,
-//    split where form 0 does not);
+//    (p <= 4) or 2 (p = 7), the linear kernel whole at 3;
+//  - form 1 trades waves for registers (B+C one wave fewer; E+G the other
+//    split: whole at 3 or 2 waves where form 0 splits, split where form 0
+//    does not) or, for the linear kernel, splits its block like E+G, which
+//    halves the dependent chain of each wave on meshes too small to fill
+//    the chip;
 //  - form 2 asks for one wave more than form 0 (B+C and the linear kernel)
 //    or gives the default E+G split more registers (one wave fewer).
 constexpr int NFORMS = TILE_FORMS;
@@ -61,7 +63,8 @@ constexpr int eg_waves(int nc, int form) {
   return form == 2 ? (nc >= 7 ? 1 : eg_split0(nc) ? 3 : 2)
                    : (nc >= 7 ? 2 : eg_split(nc, form) ? 4 : 3);
 }
-constexpr int lin_waves(int form) { return form == 1 ? 2 : form == 2 ? 4 : 3; }
+constexpr bool lin_split(int form) { return form == 1; }
+constexpr int lin_waves(int form) { return form == 2 ? 4 : 3; }
 
 // Stage input of field f: the step's start state c0 at the first stage (and
 // for a bare right-hand side), else what the previous stage stored in
@@ -452,20 +455,32 @@ template <int NC, int NG, int NGF, int WG, bool LF, int FORM>
 __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(lin_waves(FORM)))) void lin_kernel(F32Args A, int nblk) {
   using Sh = S<NC, NG, NGF>;
   constexpr Pack64 P = Sh::P64;
+  constexpr bool SPLIT = lin_split(FORM);
   __shared__ floatx4 lds[P.total / 2];
-  const int b0 = first_block<WG>();
+  const int b0 = first_block<SPLIT ? WG / 2 : WG>();
   if (b0 >= nblk) return;
   stage_lds<WG>(lds, reinterpret_cast<const floatx4 *>(A.ops.pack64), P.total / 2);
   __syncthreads();
-  const int eb = wave_block(b0);
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int eb = b0 + (SPLIT ? wv >> 1 : wv);
   if (eb >= nblk) return;
   const double *O = reinterpret_cast<const double *>(lds);
   const int e = eb * 16 + (lane() & 15);
   const Nbr3 N = nbr3(A.mesh, e < A.mesh.ne ? e : 0);
   doublex4 G[3][Sh::RBM];
-  core64<NC, NG, NGF, 0, LF>(A, O + P.dx, O + P.dy, O + P.lf, N, eb, G);
-  phase_fence();
-  finish64<NC, NG, NGF, 0>(A, O, eb, G);
+  if (!SPLIT) {
+    core64<NC, NG, NGF, 0, LF>(A, O + P.dx, O + P.dy, O + P.lf, N, eb, G);
+    phase_fence();
+    finish64<NC, NG, NGF, 0>(A, O, eb, G);
+  } else if (wv & 1) {  // the eta equation
+    core64<NC, NG, NGF, 0, LF, false, 2>(A, O + P.dx, O + P.dy, O + P.lf, N, eb, G);
+    phase_fence();
+    finish64<NC, NG, NGF, 0, 2>(A, O, eb, G);
+  } else {  // the u and v equations
+    core64<NC, NG, NGF, 0, LF, false, 1>(A, O + P.dx, O + P.dy, O + P.lf, N, eb, G);
+    phase_fence();
+    finish64<NC, NG, NGF, 0, 1>(A, O, eb, G);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -938,12 +953,13 @@ __global__ __launch_bounds__(PSI_WG) void from_psi_kernel(int ne, const double *
 template <int NC, int NG, int NGF, int WG, int FORM>
 int launch_elem(const F32Args &A, int phase, int nblk) {
   // a multiple of the 8 XCDs; a split E+G gives each block two waves
-  const int grid = 8 * cdiv(cdiv(nblk, phase == 5 && eg_split(NC, FORM) ? WG / 2 : WG), 8);
+  const bool two = (phase == 5 && eg_split(NC, FORM)) || (phase == 3 && lin_split(FORM));
+  const int grid = 8 * cdiv(cdiv(nblk, two ? WG / 2 : WG), 8);
   hipStream_t st = A.stream;
   const int v = A.variant;
   const bool lf = A.lf_alpha != 0.0;
   if (phase == 3) {
-    if (lf) lin_kernel<NC, NG, NGF, WG, true, 0><<<grid, 64 * WG, 0, st>>>(A, nblk);
+    if (lf) lin_kernel<NC, NG, NGF, WG, true, 0><<<8 * cdiv(cdiv(nblk, WG), 8), 64 * WG, 0, st>>>(A, nblk);
     else lin_kernel<NC, NG, NGF, WG, false, FORM><<<grid, 64 * WG, 0, st>>>(A, nblk);
   } else if (phase == 4) {
     if (v == 1) bc_kernel<NC, NG, NGF, 1, WG, FORM><<<grid, 64 * WG, 0, st>>>(A, nblk);
