@@ -185,6 +185,14 @@ __device__ __forceinline__ int first_block() {
   return ((blockIdx.x & 7) * g8 + (blockIdx.x >> 3)) * W;
 }
 
+// The same ranges, each swept from its end: workgroup j of an XCD takes the
+// j-th group of blocks counted from the end of that XCD's range.
+template <int W = NW>
+__device__ __forceinline__ int last_block() {
+  int g8 = gridDim.x >> 3;
+  return ((blockIdx.x & 7) * g8 + (g8 - 1 - (blockIdx.x >> 3))) * W;
+}
+
 // Element block of wave threadIdx.x / 64 of a workgroup whose first block is
 // b0, as a scalar (the compiler cannot see that it is wave-uniform): the
 // addresses of the element's rows then split into a scalar base and a

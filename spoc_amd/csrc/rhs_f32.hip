@@ -870,7 +870,11 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(eg_wave
   constexpr int N64 = P64.total / 2;  // 16-byte words
   constexpr bool SPLIT = eg_split(NC, FORM);
   __shared__ floatx4 lds[N64];
-  const int b0 = first_block<SPLIT ? WG / 2 : WG>();
+  // The sweep runs backwards through each XCD's range of blocks: B+C went
+  // forwards, so the hand-off it wrote last is still in that XCD's L2 when
+  // this starts, and the next B+C, forwards again, starts where this ended
+  // (C2: 1.418 -> 1.387 ms per RK4 step).
+  const int b0 = last_block<SPLIT ? WG / 2 : WG>();
   if (b0 >= nblk) return;
   stage_lds<WG>(lds, reinterpret_cast<const floatx4 *>(A.ops.pack64), N64);
   __syncthreads();

@@ -142,6 +142,48 @@ __device__ __forceinline__ void dtrsm4_reg(double (&b)[M], const double *__restr
   }
 }
 
+// Two independent solves of dtrsm4_reg, row by row side by side: each keeps
+// its own order of operations, and the two dependent chains interleave (the
+// wall kernels run one lane per wall element, so their time is the chains'
+// latency).
+template <int M1, int M2, int LD>
+__device__ __forceinline__ void dtrsm4_pair(double (&b1)[M1], const double *__restrict__ U1,
+                                            const double *__restrict__ d1, double (&b2)[M2],
+                                            const double *__restrict__ U2,
+                                            const double *__restrict__ d2) {
+  constexpr int M = M1 > M2 ? M1 : M2;
+#pragma unroll
+  for (int i = 0; i < M; i++) {
+    if (i < M1) {
+      double temp = b1[i];
+#pragma unroll
+      for (int k = 0; k < i; k++) temp = temp - U1[k + LD * i] * b1[k];
+      b1[i] = temp * d1[i];
+    }
+    if (i < M2) {
+      double temp = b2[i];
+#pragma unroll
+      for (int k = 0; k < i; k++) temp = temp - U2[k + LD * i] * b2[k];
+      b2[i] = temp * d2[i];
+    }
+    fence();
+  }
+#pragma unroll
+  for (int k = M - 1; k >= 0; k--) {
+    if (k < M1) {
+      b1[k] = b1[k] * d1[k];
+#pragma unroll
+      for (int i = 0; i < k; i++) b1[i] = b1[i] - b1[k] * U1[i + LD * k];
+    }
+    if (k < M2) {
+      b2[k] = b2[k] * d2[k];
+#pragma unroll
+      for (int i = 0; i < k; i++) b2[i] = b2[i] - b2[k] * U2[i + LD * k];
+    }
+    fence();
+  }
+}
+
 // Projection onto the zero-on-edge-1 basis (src/mat1.f:418-439).
 template <int NC, int NL>
 __device__ __forceinline__ void reduc_gn(const double *gn, double (&gnr)[NL]) {
@@ -190,8 +232,7 @@ __device__ __forceinline__ void wall_rot_solve(double (&gx)[NM], double (&gy)[NM
     gy[j] = gt;
   }
   reduc_gn<NC, NL>(gx, bu);
-  dtrsm4_reg<NL, NM>(bu, UL, dia2);
-  dtrsm4_reg<NM, NM>(gy, U, dia1);
+  dtrsm4_pair<NL, NM, NM>(bu, UL, dia2, gy, U, dia1);
   reduc_inv_gn<NC, NL>(gx, bu);
 #pragma unroll
   for (int j = 0; j < NM; j++) {
@@ -232,10 +273,8 @@ __device__ __forceinline__ void wall_tensor_solve(double (&qxx)[NM], double (&qx
     qyy[j] = gtx * dny - gty * dnx;  // gtt
   }
   reduc_gn<NC, NL>(qyx, bu);
-  dtrsm4_reg<NM, NM>(qxx, U, dia1);
-  dtrsm4_reg<NM, NM>(qxy, U, dia1);
-  dtrsm4_reg<NL, NM>(bu, UL, dia2);
-  dtrsm4_reg<NM, NM>(qyy, U, dia1);
+  dtrsm4_pair<NM, NM, NM>(qxx, U, dia1, qxy, U, dia1);
+  dtrsm4_pair<NL, NM, NM>(bu, UL, dia2, qyy, U, dia1);
   reduc_inv_gn<NC, NL>(qyx, bu);
 #pragma unroll
   for (int j = 0; j < NM; j++) {
@@ -813,7 +852,7 @@ __global__ __launch_bounds__(64) void wall_vel(F64Args A) {
   const int t = blockIdx.x * 64 + threadIdx.x;
   if (t >= A.mesh.nwall) return;
   const int e = A.mesh.wall[t];
-  if (VAR == 2) {  // each component alone
+  if (VAR == 2) {  // each component alone (both at once: 3x slower at p = 7, registers)
 #pragma unroll 1
     for (int c = 0; c < 2; c++) {
       double g[NM];
