@@ -361,6 +361,46 @@ int spoc_calh2(int nc, int ng, const double *wi, const double *xi, const double 
                int ld, const double *h0, double *h1);
 
 /*
+ * Mesh refinement and coarsening of the adaptive driver, host side, equal
+ * to the reference's meshes bit for bit (node coordinates, node and
+ * triangle numbering).  A spoc_amr holds the refinement hierarchy, the
+ * reference's COMMON /adap_const/, /adap_int/, /adap_vec/, /adap_log/.
+ *
+ * spoc_amr_create (init_adap, src/adapt.f:380-449): the hierarchy of a
+ * mesh of nn nodes (xgr, ygr) and ne triangles (in (3, ne), 1-based), with
+ * its faces (itm (3, ne), tseg (2, nf)) as spoc_mesh_build numbers them.
+ *
+ * spoc_amr_adapt: one adaptation as global_refinment runs it between
+ * cal_conv and the interpolation (src/adapt.f:89-119): the flags of the
+ * current mesh (fconv: refine, fderaff: coarsen; ne values each, 0/1, as
+ * spoc_cal_conv returns them) onto the hierarchy, deraffiner (1393-1583),
+ * adapmaille (532-788), protridouble (789-998), elimpptri (999-1121).  The
+ * reference runs it when cal_conv's conv is true.  *nn_new, *ne_new: the
+ * new mesh's sizes.  The per-element fderaff is read by hierarchy number,
+ * as the reference's call does (src/adapt.f:117): numbers past ne count as
+ * false.
+ *
+ * spoc_amr_mesh: the current mesh, xgr, ygr (nn_new), in (3, ne_new),
+ * 1-based, CCW when the input mesh was; tabsol (ne_new, may be NULL) the
+ * hierarchy number of each triangle (past the hierarchy: green halves).
+ * spoc_amr_levels: the refinement level of each triangle (a green half at
+ * its parent's).
+ *
+ * spoc_amr_inter_only_new (inter_only_new, src/adapt.f:1233-1258):
+ * tinter (nenew) for spoc_interpol: the old triangle that reappears
+ * unchanged as each new triangle, or 0.
+ */
+typedef struct spoc_amr spoc_amr;
+int spoc_amr_create(spoc_amr **amr, int nn, int ne, int nf, const double *xgr, const double *ygr,
+                    const int *in, const int *tseg, const int *itm);
+int spoc_amr_adapt(spoc_amr *amr, int ne, const int *fconv, const int *fderaff, int *nn_new,
+                   int *ne_new);
+int spoc_amr_mesh(const spoc_amr *amr, double *xgr, double *ygr, int *in, int *tabsol);
+int spoc_amr_levels(const spoc_amr *amr, int *level);
+void spoc_amr_destroy(spoc_amr *amr);
+int spoc_amr_inter_only_new(int ne, const int *in, int nenew, const int *innew, int *tinter);
+
+/*
  * Refinement indicator of the device-resident state (checkconv,
  * src/adapt.f:1260-1332), FP64, bit-identical to the reference: for field
  * 0 (u), 1 (v) or 2 (h), amax[e] (ne values) = the largest |difference| of

@@ -279,9 +279,10 @@ class RefCM:
 
 
 class RefAdapt:
-    """The reference's AMR decision routines checkconv and cal_conv
-    (src/adapt.f), compiled from its sources.  Modal arrays are (45, ne)
-    Fortran-ordered; pico is the (9, 45, 3) table."""
+    """The reference's AMR routines (src/adapt.f), compiled from its
+    sources: the decisions checkconv and cal_conv, the interpolations, and
+    the refinement hierarchy.  Modal arrays are (45, ne) Fortran-ordered;
+    pico is the (9, 45, 3) table."""
 
     def __init__(self):
         self.lib = C.CDLL(os.path.join(REF_DIR, "libspoc_ref_adapt.so"), mode=os.RTLD_LOCAL)
@@ -349,3 +350,40 @@ class RefAdapt:
         return {"fconv": fconv, "fderaff": fderaff, "conv": conv.value, "deraff": deraff.value,
                 "timedir": timedir.value}
 
+    # the reference's hierarchy sizes (include/maille.inc:6-7)
+    NNDIM, NEDIM = 800, 1600
+
+    def adap_init(self, x, y, tri, tseg, itm):
+        """init_adap (src/adapt.f:380-449): the hierarchy of this mesh
+        (tri, itm (3, ne) and tseg (2, nf), 1-based)."""
+        tri = np.asfortranarray(tri, np.int32)
+        self.lib.ref_adap_init(len(x), tri.shape[1], _d(np.ascontiguousarray(x, np.float64)),
+                               _d(np.ascontiguousarray(y, np.float64)), _i(tri),
+                               _i(np.asfortranarray(tseg, np.int32)),
+                               _i(np.asfortranarray(itm, np.int32)))
+
+    def adap_cycle(self, nn, fconv, fderaff):
+        """One adaptation of the current hierarchy (global_refinment's
+        sequence, src/adapt.f:89-119) from per-element flags; returns the new
+        mesh x, y, tri (3, ne) and tabsol."""
+        ne = len(fconv)
+        fconv = np.ascontiguousarray(fconv, np.int32)
+        fder = np.zeros(2 * self.NEDIM, np.int32)
+        fder[:ne] = fderaff
+        xn, yn = np.zeros(2 * self.NNDIM), np.zeros(2 * self.NNDIM)
+        trin = np.zeros((3, self.NEDIM), np.int32, order="F")
+        tabs = np.zeros(self.NEDIM, np.int32)
+        nnn, nen = C.c_int(), C.c_int()
+        self.lib.ref_adap_cycle(nn, ne, _i(fconv), _i(fder), C.byref(nnn), C.byref(nen), _d(xn),
+                                _d(yn), _i(trin), _i(tabs))
+        n, e = nnn.value, nen.value
+        return xn[:n].copy(), yn[:n].copy(), np.asfortranarray(trin[:, :e]), tabs[:e].copy()
+
+    def inter_only_new(self, tri, trin):
+        """inter_only_new (src/adapt.f:1233-1258): tinter (nenew)."""
+        tri = np.asfortranarray(tri, np.int32)
+        ext = np.zeros((3, trin.shape[1] + 1), np.int32, order="F")  # a zero column past the end
+        ext[:, :trin.shape[1]] = trin
+        tinter = np.zeros(trin.shape[1], np.int32)
+        self.lib.ref_inter_only_new(tri.shape[1], trin.shape[1], _i(tri), _i(ext), _i(tinter))
+        return tinter

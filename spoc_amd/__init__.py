@@ -62,9 +62,12 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                  "spoc_mesh_changemaille", "spoc_state_file_read", "spoc_state_file_write",
                  "spoc_mesh_file_size", "spoc_mesh_file_read", "spoc_mesh_file_write",
                  "spoc_interpol", "spoc_calh2", "spoc_set_flux", "spoc_set_ordering",
-                 "spoc_mesh_order", "spoc_energy_noalloc"):
+                 "spoc_mesh_order", "spoc_energy_noalloc", "spoc_amr_create", "spoc_amr_adapt",
+                 "spoc_amr_mesh", "spoc_amr_levels", "spoc_amr_inter_only_new"):
         getattr(lib, name).restype = C.c_int
     lib.spoc_ctx_destroy.argtypes = [C.c_void_p]
+    lib.spoc_amr_destroy.argtypes = [C.c_void_p]
+    lib.spoc_amr_destroy.restype = None
     _lib = lib
     return lib
 
@@ -203,6 +206,71 @@ def calh2(tables: dict, ff, x, y, tri, tinter=None, h0=None, ld: int = LDM):
                           t["ldg"], _d(t["amm"]), LDM, cb, None, len(x), ne, _d(x), _d(y),
                           _i(tri), _i(tinter), ne0, ld, h0p, _d(h1)))
     return h1
+
+
+class Amr:
+    """The adaptive driver's refinement hierarchy (spoc_amr_*; src/adapt.f
+    init_adap, deraffiner, adapmaille, protridouble, elimpptri), host side.
+    Built from a mesh dict of mesh.build (x, y, in, itm, tseg, nf);
+    adapt(fconv, fderaff) runs one adaptation from per-element 0/1 flags
+    (as cal_conv returns them) and returns the new mesh (x, y, tri (3, ne)
+    1-based)."""
+
+    def __init__(self, m: dict):
+        self.lib = load_library()
+        self.h = C.c_void_p()
+        x, y = (np.ascontiguousarray(m[k], np.float64) for k in ("x", "y"))
+        tri, itm, tseg = (np.asfortranarray(m[k], np.int32) for k in ("in", "itm", "tseg"))
+        _check(self.lib.spoc_amr_create(C.byref(self.h), len(x), tri.shape[1], int(m["nf"]), _d(x),
+                                        _d(y), _i(tri), _i(tseg), _i(itm)))
+        self.nn, self.ne = len(x), tri.shape[1]
+
+    def adapt(self, fconv, fderaff=None):
+        fconv = np.ascontiguousarray(fconv, np.int32)
+        fder = np.zeros(self.ne, np.int32) if fderaff is None else \
+            np.ascontiguousarray(fderaff, np.int32)
+        if len(fconv) != self.ne or len(fder) != self.ne:
+            raise SpocError(f"flags must have the mesh's {self.ne} entries", -1)
+        nn, ne = C.c_int(), C.c_int()
+        _check(self.lib.spoc_amr_adapt(self.h, self.ne, _i(fconv), _i(fder), C.byref(nn),
+                                       C.byref(ne)))
+        self.nn, self.ne = nn.value, ne.value
+        return self.mesh()[:3]
+
+    def mesh(self):
+        """Current mesh: x, y (nn), tri (3, ne) 1-based, tabsol (ne)."""
+        x, y = np.zeros(self.nn), np.zeros(self.nn)
+        tri = np.zeros((3, self.ne), np.int32, order="F")
+        tabsol = np.zeros(self.ne, np.int32)
+        _check(self.lib.spoc_amr_mesh(self.h, _d(x), _d(y), _i(tri), _i(tabsol)))
+        return x, y, tri, tabsol
+
+    def levels(self) -> np.ndarray:
+        lev = np.zeros(self.ne, np.int32)
+        _check(self.lib.spoc_amr_levels(self.h, _i(lev)))
+        return lev
+
+    def close(self):
+        if self.h:
+            self.lib.spoc_amr_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def inter_only_new(tri, new_tri) -> np.ndarray:
+    """tinter for interpol: the old triangle that reappears unchanged as each
+    new triangle, or 0 (inter_only_new, src/adapt.f:1233-1258)."""
+    lib = load_library()
+    tri, new_tri = (np.asfortranarray(a, np.int32) for a in (tri, new_tri))
+    tinter = np.zeros(new_tri.shape[1], np.int32)
+    _check(lib.spoc_amr_inter_only_new(tri.shape[1], _i(tri), new_tri.shape[1], _i(new_tri),
+                                       _i(tinter)))
+    return tinter
 
 
 ORDERINGS = {"input": 0, "hilbert": 1, "zcurve": 2, "auto": 3}

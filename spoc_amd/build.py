@@ -43,6 +43,7 @@ def _units():
     units.append(("mesh.cpp", "mesh.o", EXACT))
     units.append(("files.cpp", "files.o", []))
     units.append(("interp.cpp", "interp.o", EXACT))
+    units.append(("amr.cpp", "amr.o", EXACT))
     units.append(("order.cpp", "order.o", []))
     units.append(("abi.cpp", "abi.o", EXACT + ["-x", "hip"]))
     return units
@@ -84,7 +85,7 @@ SAN_LIB = os.path.join(SAN_DIR, "libspoc_amd_san.so")
 SAN_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
              "-Xarch_host", "-fno-sanitize-recover=all", "-Xarch_host", "-fno-omit-frame-pointer"]
 HOST_UNITS = ("f32_plan.cpp", "f64_plan.cpp", "energy.hip", "adapt.hip", "mesh.cpp", "files.cpp",
-              "interp.cpp", "order.cpp", "abi.cpp")
+              "interp.cpp", "amr.cpp", "order.cpp", "abi.cpp")
 
 
 def san_runtime() -> str:

@@ -27,13 +27,16 @@ Fixtures:
                        (src/adapt.f:1662-1824, 303-373)
   calvar_p4.npz,       one cal_var per variant at the shapes of configurations
   calvar_p5.npz        5 (p=4, 25-point rule) and 4 (p=5, 25-point rule)
+  amr_cycles.npz       five adaptations each of the config-1 and jittered meshes
+                       through the refinement hierarchy (src/adapt.f:380-1258):
+                       flags, new meshes, tabsol, tinter
   rk4_cfg.npz          10 RK4 steps with energy/mass before and after, from the
                        initial states of the configuration generators on reduced
                        meshes: c2 (fs, p=6, 1152 triangles), c4 (inv, p=5, 576),
                        c5 (fs, p=4, one member of about 1100)
 
     python tests/golden/make_goldens.py [/root/reference] [--only-calconv|--only-interp|
-                                                           --only-configs]
+                                                           --only-configs|--only-amr]
 """
 from __future__ import annotations
 
@@ -187,6 +190,86 @@ def interp_goldens(q):
         np.savez_compressed(os.path.join(OUT, f"interp_p{nc}.npz"), **out)
 
 
+def _safe_families(amr, m, ne):
+    """Families (four sibling leaves) that can be coarsened without leaving
+    a triangle next to one two levels finer: hierarchy numbers of the four
+    children, all <= ne (the reference reads the per-element coarsening
+    flags by hierarchy number), with no current neighbour finer than them."""
+    _, _, tri, tabsol = amr.mesh()
+    lev = amr.levels()
+    pos = {int(h): e for e, h in enumerate(tabsol)}
+    nbr = [[] for _ in range(ne)]
+    for f in range(m["nf"]):
+        a, b = int(m["tseg"][0, f]) - 1, int(m["tseg"][1, f]) - 1
+        if b >= 0:
+            nbr[a].append(b)
+            nbr[b].append(a)
+    fams = []
+    for h in sorted(pos):
+        if h % 4 != 1 or h <= 4:
+            continue
+        kids = [h, h + 1, h + 2, h + 3]
+        if any(k not in pos or k > ne for k in kids):
+            continue
+        es = [pos[k] for k in kids]
+        if len({int(lev[e]) for e in es}) != 1:
+            continue
+        if all(lev[n] <= lev[es[0]] for e in es for n in nbr[e]):
+            fams.append(kids)
+    return fams
+
+
+def amr_goldens():
+    """The refinement hierarchy (init_adap, deraffiner, adapmaille,
+    protridouble, elimpptri, inter_only_new; src/adapt.f:380-1258) over
+    several adaptations of two meshes: seeded refinement flags (scattered
+    and clustered, so that the closure and the green halves run at several
+    levels), and coarsening flags on whole families of leaves.  Each cycle
+    stores its flags and the reference's new mesh, tabsol and tinter."""
+    import spoc_amd
+    A = oracle.RefAdapt()
+    out = {}
+    d1 = np.load(os.path.join(OUT, "mesh_c1.npz"))
+    dj = np.load(os.path.join(OUT, "mesh_jit.npz"))
+    cases = {"c1": (d1["x"], d1["y"], d1["tri"], (0.03, 0.03, 0.02, 0.02, 0.02), (150e3, 0.0)),
+             "jit": (dj["x"], dj["y"], dj["tri"], (0.05, 0.08, 0.06, 0.04, 0.05), (150e3, 0.0))}
+    for name, (x, y, tri, rates, (rad, _)) in cases.items():
+        rng = np.random.default_rng(61 if name == "c1" else 62)
+        m = mesh.build(x, y, tri, 3)
+        A.adap_init(x, y, tri, m["tseg"], m["itm"])
+        ours = spoc_amd.Amr(m)  # (only to choose coarsening flags the reference can take)
+        out[f"{name}_x"], out[f"{name}_y"], out[f"{name}_tri"] = x, y, tri
+        out[f"{name}_tseg"], out[f"{name}_itm"] = m["tseg"], m["itm"]
+        cx, cy, ct = x, y, tri
+        cxm, cym = x.mean(), y.mean()
+        for cyc, rate in enumerate(rates):
+            ne = ct.shape[1]
+            cen_x = cx[ct - 1].mean(axis=0)
+            cen_y = cy[ct - 1].mean(axis=0)
+            # (the cluster halves at each cycle: the reference's tables hold
+            # 1,600 nodes and 1,600 triangles, include/maille.inc:6-7)
+            near = (cen_x - cxm) ** 2 + (cen_y - cym) ** 2 < (rad / 2 ** cyc) ** 2
+            fconv = ((rng.random(ne) < rate) | (near & (cyc % 2 == 0))).astype(np.int32)
+            fder = np.zeros(ne, np.int32)
+            if cyc == 3:
+                fams = _safe_families(ours, mesh.build(cx, cy, ct, 3), ne)
+                for kids in fams[:: max(1, len(fams) // 4)][:4]:
+                    fder[np.array(kids) - 1] = 1
+                fconv[np.array([k - 1 for kids in fams for k in kids], dtype=int)] = 0
+            rx, ry, rt, rtab = A.adap_cycle(len(cx), fconv, fder)
+            ours.adapt(fconv, fder)
+            tin = A.inter_only_new(ct, rt)
+            out[f"{name}_{cyc}_fconv"], out[f"{name}_{cyc}_fderaff"] = fconv, fder
+            out[f"{name}_{cyc}_x"], out[f"{name}_{cyc}_y"] = rx, ry
+            out[f"{name}_{cyc}_tri"], out[f"{name}_{cyc}_tabsol"] = rt, rtab
+            out[f"{name}_{cyc}_tinter"] = tin
+            print(f"amr {name} cycle {cyc}: refine {int(fconv.sum())} coarsen {int(fder.sum())} "
+                  f"-> nn {len(rx)} ne {rt.shape[1]} (kept {int((tin > 0).sum())})")
+            cx, cy, ct = rx, ry, rt
+        out[f"{name}_cycles"] = len(rates)
+    np.savez_compressed(os.path.join(OUT, "amr_cycles.npz"), **out)
+
+
 def main(refroot="/root/reference", *flags):
     oracle.build(ref=True)
     q = quad_tables(refroot)
@@ -197,6 +280,10 @@ def main(refroot="/root/reference", *flags):
     if "--only-interp" in flags:
         interp_goldens(q)
         print("interpolation goldens written to", OUT)
+        return
+    if "--only-amr" in flags:
+        amr_goldens()
+        print("refinement goldens written to", OUT)
         return
     if "--only-configs" in flags:
         config_goldens(q)
@@ -251,6 +338,7 @@ def main(refroot="/root/reference", *flags):
     calconv_goldens(q)
     interp_goldens(q)
     config_goldens(q)
+    amr_goldens()
     print("goldens written to", OUT)
 
 

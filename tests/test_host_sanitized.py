@@ -1,10 +1,10 @@
 """Host code under AddressSanitizer and UndefinedBehaviorSanitizer (CPU).
 
 The host translation units of libspoc_amd.so (mesh operands, files,
-re-projection, element order, the C-ABI's validation and table builder) are
-rebuilt with both sanitizers on their host side (spoc_amd.build.build_sanitized)
-and the CPU tests that drive them run again in a child process that loads
-that build and preloads the sanitizer runtime.  Any invalid access, overflow
+re-projection, refinement hierarchy, element order, the C-ABI's validation
+and table builder) are rebuilt with both sanitizers on their host side
+(spoc_amd.build.build_sanitized) and the CPU tests that drive them run again
+in a child process that loads that build and preloads the sanitizer runtime.  Any invalid access, overflow
 or undefined operation aborts the child with a report.
 """
 import os
@@ -16,7 +16,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HOST_TESTS = ["tests/test_abi_cpu.py", "tests/test_files.py", "tests/test_interp.py",
-              "tests/test_mesh_native.py", "tests/test_order.py"]
+              "tests/test_mesh_native.py", "tests/test_order.py", "tests/test_amr.py"]
 
 
 @pytest.fixture(scope="module")
