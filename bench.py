@@ -63,8 +63,9 @@ WORKLOADS = {
         f"calvarnl_inv p={nc} ng={ng} ngf={ngf} on {ne} triangles (4000x2000 km), mu0=0, "
         f"1 m Gaussian eta, dt={dt} s")),
     "c5": dict(shape=(4, 25, 5), steps=50, gate=500, name=lambda nc, ng, ngf, ne, dt, a: (
-        f"ensemble of {a.members} calvarnl_fs members, p={nc} ng={ng} ngf={ngf}, 5k-60k "
-        f"triangles each, {ne} in total, one union mesh per launch, dt={dt} s")),
+        f"ensemble of {a.members} calvarnl_fs members, p={nc} ng={ng} ngf={ngf}, meshes refined "
+        f"by the adaptive hierarchy around each member's eta bump to 5k-60k triangles, {ne} in "
+        f"total, one union mesh per launch, dt={dt} s")),
 }
 
 

@@ -281,30 +281,107 @@ def config4(tables: dict, nx: int = 224, ny: int = 113, seed: int = 4001) -> dic
     return c
 
 
+def checkconv_indicator(tables: dict, m: dict, a: np.ndarray) -> np.ndarray:
+    """The refinement indicator of checkconv (src/adapt.f:1260-1332) of one
+    modal field, in numpy, for generating refined meshes: per element the
+    largest |difference| of the two traces at the edge points of its faces
+    that have a second element (the second element's points reversed along
+    the face), over max |a(1, .)| (0 when that is <= 1e-6).  (The library's
+    spoc_checkconv computes it on the device, in the reference's order.)"""
+    nm, ngf = tables["nm"], tables["ngf"]
+    pico = np.asarray(tables["pico"])[:ngf, :nm, :]  # (ngf, nm, 3)
+    tseg, ssit = m["tseg"], m["ssit"]
+    inner = tseg[1] > 0
+    e1, e2 = tseg[0, inner] - 1, tseg[1, inner] - 1
+    k1, k2 = ssit[0, inner] - 1, ssit[1, inner] - 1
+    # traces (ngf, 3, ne): one product with the (ngf * 3, nm) trace table
+    T = (pico.transpose(0, 2, 1).reshape(ngf * 3, nm) @ a[:nm]).reshape(ngf, 3, -1)
+    jump = np.abs(T[:, k1, e1] - T[::-1, k2, e2]).max(axis=0)
+    amax = np.zeros(m["ne"])
+    np.maximum.at(amax, e1, jump)
+    np.maximum.at(amax, e2, jump)
+    vmax = np.abs(a[0]).max()
+    return amax / vmax if vmax > 1e-6 else np.zeros(m["ne"])
+
+
+def refined_mesh(tables: dict, x, y, tri, f, target: int, fac1: float = 1e-10,
+                 max_cycles: int = 12):
+    """A mesh refined the way the adaptive driver refines one
+    (src/shwwnl_adap.f:385-432, src/adapt.f:5-186): the field f(x, y) is
+    projected onto the mesh (calh), the checkconv indicator flags the
+    elements above fac1, and the refinement hierarchy (spoc_amr_*:
+    adapmaille, protridouble, elimpptri) refines them, again on the new
+    mesh, until the mesh has about `target` triangles.  Each cycle refines
+    at most the (target - ne) / 4 elements with the largest indicators (a
+    refinement adds three triangles, its closure about one more).  The
+    indicator falls by about 2^(p+1) per level (a smooth field's jumps go as
+    h^(p+1)); fac1 = 1e-10 lets a 100 km bump at p = 4 refine four levels.
+    Returns x, y, tri (3, ne) 1-based."""
+    from . import Amr
+
+    nc, nm, ng = tables["nc"], tables["nm"], tables["ng"]
+    xi, yi = tables["xi"], tables["yi"]
+    # the projection for the indicator as one (nm, ng) matrix: (U^T U)^-1
+    # pi2^T diag(w)
+    U = np.triu(tables["amm"][:nm, :nm])
+    P = np.linalg.solve(U.T @ U, tables["pi2"][:ng, :nm].T * tables["wi"][None, :])
+
+    def proj(x, y, tri):
+        i1, i2, i3 = tri[0] - 1, tri[1] - 1, tri[2] - 1
+        X = 0.5 * (x[i2] - x[i1])[None] * (xi[:, None] + 1.0) + \
+            0.5 * (x[i3] - x[i1])[None] * (yi[:, None] + 1.0) + x[i1][None]
+        Y = 0.5 * (y[i2] - y[i1])[None] * (xi[:, None] + 1.0) + \
+            0.5 * (y[i3] - y[i1])[None] * (yi[:, None] + 1.0) + y[i1][None]
+        return P @ f(X, Y)
+
+    m = M.build(x, y, tri, nc)
+    amr = Amr(m)
+    for _ in range(max_cycles):
+        ne = m["ne"]
+        if ne >= target:
+            break
+        ind = checkconv_indicator(tables, m, proj(x, y, tri))
+        cand = np.nonzero(ind > fac1)[0]
+        if len(cand) == 0:
+            break
+        k = max(1, (target - ne) // 4)
+        pick = cand[np.argsort(-ind[cand], kind="stable")[:k]]
+        flags = np.zeros(ne, np.int32)
+        flags[pick] = 1
+        x, y, tri = amr.adapt(flags)
+        m = M.build(x, y, tri, nc)
+    amr.close()
+    return x, y, tri
+
+
 def config5(tables: dict, nmem: int = 64, seed: int = 5001, ntri=(5e3, 60e3)) -> list:
     """Ensemble of nmem free-slip members (p=4, 25-point rule, 5 edge points;
-    the 12-point rule is singular at p=4).  Member m (seed 5001 + m): a
-    jittered mesh of the config-2 basin whose triangle count is log-uniform
-    in 5k..60k (standing in for the adaptive driver's refined meshes), a
-    Gaussian eta bump of amplitude U(0.5, 2) m and radius 100 km centred at
-    U(-200, 200) km in x and y, the config-2 wind scaled by U(0.8, 1.2),
-    beta plane, H=1000 m, mu0=100, mb0=0.  All members share one dt (the
-    smallest member's CFL step) and run 500 steps."""
+    the 12-point rule is singular at p=4).  Every member starts from one
+    jittered mesh of the config-2 basin of about 0.8 ntri[0] triangles
+    (seed 5000) and is refined by the adaptive driver's hierarchy around its
+    own initial eta (refined_mesh: the checkconv indicator of the projected
+    field, adapmaille / protridouble / elimpptri) to a size log-uniform in
+    ntri.  Member m (seed 5001 + m): a Gaussian eta bump of amplitude
+    U(0.5, 2) m and radius 100 km centred at U(-200, 200) km in x and y,
+    the config-2 wind scaled by U(0.8, 1.2), beta plane, H=1000 m, mu0=100,
+    mb0=0.  All members share one dt (the smallest member's CFL step) and
+    run 500 steps."""
     lx, ly = 4000e3, 2000e3
     tau = 0.1 / 1000.0
+    ny = max(4, int(round(math.sqrt(0.8 * ntri[0] / 4.0))) + 1)
+    xb, yb, trib = jittered_mesh(2 * ny - 1, ny, lx, ly, 0.3, seed - 1)
     out = []
     for m in range(nmem):
         rng = np.random.default_rng(seed + m)
         nt = math.exp(rng.uniform(math.log(ntri[0]), math.log(ntri[1])))
-        ny = max(4, int(round(math.sqrt(nt / 4.0))) + 1)
-        nx = 2 * ny - 1
-        x, y, tri = jittered_mesh(nx, ny, lx, ly, 0.3, seed + m)
         a, x0, y0 = rng.uniform(0.5, 2.0), rng.uniform(-200e3, 200e3), rng.uniform(-200e3, 200e3)
         ws = rng.uniform(0.8, 1.2)
+        eta = lambda X, Y, a=a, x0=x0, y0=y0: \
+            a * np.exp(-((X - x0) ** 2 + (Y - y0) ** 2) / 100e3 ** 2)
+        x, y, tri = refined_mesh(tables, xb, yb, trib, eta, int(round(nt)))
         c = make_case(tables, x, y, tri, f0=1e-4, beta=2e-11, g0=9.81, H=1000.0, mu0=100.0,
                       mb0=0.0, wind=lambda X, Y, ws=ws: ws * tau * np.sin(math.pi * Y / (ly / 2)),
-                      eta=lambda X, Y, a=a, x0=x0, y0=y0:
-                      a * np.exp(-((X - x0) ** 2 + (Y - y0) ** 2) / 100e3 ** 2))
+                      eta=eta)
         c["dt"] = cfl_dt_altitude(x, y, tri, tables["nc"], 1000.0, 9.81)
         c["variant"], c["nsteps"] = "fs", 500
         out.append(c)
