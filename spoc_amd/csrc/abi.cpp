@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -255,10 +256,10 @@ struct spoc_ctx {
   // dorder: device copy of iperm (energy sums run in the caller's order)
   int ordering = SPOC_ORDER_AUTO;
   // pinned host staging of the host <-> device copies: room for 4 fields of
-  // (nm, ne); one completion event per field
+  // (nm, ne); completion events of the copies (fields x pieces)
   double *stage = nullptr;
   size_t stage_cap = 0;
-  hipEvent_t fev[4] = {};
+  hipEvent_t fev[4 * 4] = {};
   std::vector<int> perm, iperm;
   int *dorder = nullptr;
 };
@@ -348,25 +349,12 @@ int f32_fail(int rc) {
   return fail(SPOC_EHIP, "FP32 kernel launch failed: %s", hipGetErrorString(e));
 }
 
-// f(e0, e1) over [0, ne) split among up to 8 host threads (the repacking
-// between the caller's (ld, ne) arrays and the packed (nm, ne) layout is
-// bound by host memory bandwidth, which one thread does not fill).
-template <class F>
-void host_parallel(int ne, F f) {
+// Host threads for the repacking between the caller's (ld, ne) arrays and
+// the packed (nm, ne) layout, which is bound by host memory bandwidth (one
+// thread does not fill it): up to 16.
+int host_threads() {
   const unsigned hw = std::thread::hardware_concurrency();
-  const int nt = (int)std::min<unsigned>(8u, hw ? hw : 1u);
-  const int per = (ne + nt - 1) / nt;
-  if (nt <= 1 || ne < 4096) {
-    f(0, ne);
-    return;
-  }
-  std::vector<std::thread> th;
-  for (int t = 1; t < nt; t++) {
-    const int a = t * per, b = std::min(ne, a + per);
-    if (a < b) th.emplace_back([=, &f] { f(a, b); });
-  }
-  f(0, std::min(ne, per));
-  for (auto &t : th) t.join();
+  return (int)std::min<unsigned>(16u, hw ? hw : 1u);
 }
 
 }  // namespace
@@ -826,8 +814,11 @@ static int ensure_stage(spoc_ctx *c, int n) {
 }
 
 // Host (ld, ne) arrays in the caller's numbering -> device (nm, ne) arrays in
-// the internal numbering: each field is packed into pinned staging by the
-// host threads and copied while the next one is packed.
+// the internal numbering.  Each host thread owns a slice of the elements:
+// it packs its slice of every field into pinned staging and queues that
+// slice's copy at once, so the copies run while the other slices are being
+// packed (the stream takes copies from several threads; their order does
+// not matter, the slices do not overlap).
 static int upload_fields(spoc_ctx *c, int ld, const double *const *src, double *const *dst,
                          int n) {
   const int nm = c->nm, ne = c->ne;
@@ -836,41 +827,78 @@ static int upload_fields(spoc_ctx *c, int ld, const double *const *src, double *
   if (rc) return rc;
   // the staging may still feed an earlier call's copies
   HIPCHK(hipStreamSynchronize(c->stream));
-  for (int f = 0; f < n; f++) {
-    if (!src[f]) continue;
-    double *st = c->stage + f * fsz;
-    const double *s = src[f];
-    host_parallel(ne, [&](int e0, int e1) {
+  const int nt = ne < 4096 ? 1 : host_threads();
+  std::atomic<int> bad{0};
+  auto work = [&](int t) {
+    const int e0 = (int)((long long)ne * t / nt), e1 = (int)((long long)ne * (t + 1) / nt);
+    for (int f = 0; f < n && e1 > e0; f++) {
+      if (!src[f]) continue;
+      double *st = c->stage + f * fsz;
+      const double *s = src[f];
       for (int e = e0; e < e1; e++)
         std::memcpy(st + (size_t)e * nm, s + (size_t)ld * caller_elem(c, e), sizeof(double) * nm);
-    });
-    HIPCHK(hipMemcpyAsync(dst[f], st, sizeof(double) * fsz, hipMemcpyHostToDevice, c->stream));
-  }
+      if (hipMemcpyAsync(dst[f] + (size_t)e0 * nm, st + (size_t)e0 * nm,
+                         sizeof(double) * nm * (e1 - e0), hipMemcpyHostToDevice,
+                         c->stream) != hipSuccess)
+        bad = 1;
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; t++) th.emplace_back(work, t);
+  work(0);
+  for (auto &t : th) t.join();
   HIPCHK(hipStreamSynchronize(c->stream));
+  if (bad) return fail(SPOC_EHIP, "host-to-device copy failed");
   return SPOC_OK;
 }
 
-// Device (nm, ne) arrays -> host (ld, ne) arrays: every field's copy is
-// queued at once; each is unpacked as soon as it has arrived.
+// Device (nm, ne) arrays -> host (ld, ne) arrays.  Every field is copied in
+// kPieces pieces of elements, each with its completion event, all queued at
+// once; each host thread owns a slice of the elements and unpacks it from
+// every field as soon as the piece holding it has arrived, so the unpacking
+// (the bound: it writes 28 of every 45 doubles of the caller's arrays)
+// runs while the later pieces are still being copied.
 static int download_fields(spoc_ctx *c, int ld, double *const *dst, double *const *src, int n) {
+  constexpr int kPieces = 4;
   const int nm = c->nm, ne = c->ne;
   const size_t fsz = (size_t)nm * ne;
   int rc = ensure_stage(c, n);
   if (rc) return rc;
-  for (int f = 0; f < n; f++) {
-    HIPCHK(hipMemcpyAsync(c->stage + f * fsz, src[f], sizeof(double) * fsz,
-                          hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipEventRecord(c->fev[f], c->stream));
-  }
-  for (int f = 0; f < n; f++) {
-    HIPCHK(hipEventSynchronize(c->fev[f]));
-    const double *st = c->stage + f * fsz;
-    double *d = dst[f];
-    host_parallel(ne, [&](int e0, int e1) {
-      for (int e = e0; e < e1; e++)
-        std::memcpy(d + (size_t)ld * caller_elem(c, e), st + (size_t)e * nm, sizeof(double) * nm);
-    });
-  }
+  auto cut = [&](int k, int parts) { return (int)((long long)ne * k / parts); };
+  for (int f = 0; f < n; f++)
+    for (int p = 0; p < kPieces; p++) {
+      const size_t e0 = cut(p, kPieces), e1 = cut(p + 1, kPieces);
+      if (e1 > e0)
+        HIPCHK(hipMemcpyAsync(c->stage + f * fsz + e0 * nm, src[f] + e0 * nm,
+                              sizeof(double) * nm * (e1 - e0), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipEventRecord(c->fev[f * kPieces + p], c->stream));
+    }
+  const int nt = ne < 4096 ? 1 : host_threads();
+  std::atomic<int> bad{0};
+  auto work = [&](int t) {
+    const int e0 = cut(t, nt), e1 = cut(t + 1, nt);
+    for (int f = 0; f < n && e1 > e0; f++) {
+      const double *st = c->stage + f * fsz;
+      double *d = dst[f];
+      // the pieces that hold [e0, e1)
+      for (int p = 0; p < kPieces; p++) {
+        const int a = std::max(e0, cut(p, kPieces)), b = std::min(e1, cut(p + 1, kPieces));
+        if (a >= b) continue;
+        if (hipEventSynchronize(c->fev[f * kPieces + p]) != hipSuccess) {
+          bad = 1;
+          return;
+        }
+        for (int e = a; e < b; e++)
+          std::memcpy(d + (size_t)ld * caller_elem(c, e), st + (size_t)e * nm,
+                      sizeof(double) * nm);
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; t++) th.emplace_back(work, t);
+  work(0);
+  for (auto &t : th) t.join();
+  if (bad) return fail(SPOC_EHIP, "device-to-host copy failed");
   return SPOC_OK;
 }
 
