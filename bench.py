@@ -537,6 +537,9 @@ def main():
                          "captured HIP graphs (spoc_set_graph 0)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--tiles", default=None, metavar="T0,T1",
+                    help="force the element kernels' tile codes (waves per workgroup + 16 x "
+                         "form; lin or bc, then eg) instead of autotuning")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="before the timed region, copy MIB MiB once device-to-device (the HIP "
                          "runtime's blit kernel) so that a counter profile holds a kernel of "
@@ -598,8 +601,14 @@ def main():
         calib_copy(args.calib_copy << 20)
     # mixed-precision path: the autotuner's choice of kernel form and
     # workgroup size on this mesh (it would run by itself on the first call)
-    phase("mixed path: autotune")
-    tiles = ctx.autotune(variant)
+    if args.tiles:
+        # forced tile codes (profiling passes: the same instances as a traced run)
+        t0c, t1c = (int(v) for v in args.tiles.split(","))
+        ctx.set_tiles(variant, t0c, t1c, autotune=False)
+        tiles = (t0c, t1c)
+    else:
+        phase("mixed path: autotune")
+        tiles = ctx.autotune(variant)
     # warm up, then time K steps from the same initial state
     phase("mixed path: warm-up")
     ctx.upload(*state0)
