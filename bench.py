@@ -69,11 +69,12 @@ WORKLOADS = {
 }
 
 
-def algorithmic_counts(nm, ng, ngf, variant):
+def algorithmic_counts(nm, ng, ngf, variant, steps=1):
     """Useful (unpadded) FP32 and FP64 flops and compulsory HBM bytes per
-    element per right-hand side of each kernel, from its product shapes and
-    the arrays it must read and write once (neighbour traces counted once,
-    RK4 traffic averaged over the four stages)."""
+    element per launch of each kernel, from its product shapes and the
+    arrays it must read and write once (neighbour traces counted once, RK4
+    traffic averaged over the four stages); a per-stage kernel's launch is
+    one right-hand side, the persistent kernel's is `steps` RK4 steps."""
     e3 = 3 * ngf
     f4, f8 = 4, 8
     mesh = 16 * f8 + 6 * f4 + f4                       # geo64, nbr/nbe, bdl
@@ -87,7 +88,12 @@ def algorithmic_counts(nm, ng, ngf, variant):
     if variant == "lin":
         # (the stage input is c0 at stage 0 and Y4 at stage 3, read for the
         # RK4 update anyway)
-        return {"lin_f64": (0, vol_fl + face_fl, 0.5 * 3 * nm * f8 + trl_by + mesh)}
+        # persistent: c0 read and written once, the mesh read once, and per
+        # stage only the traces (neighbours' in, own out); the rest of the
+        # state stays in registers
+        return {"lin_f64": (0, vol_fl + face_fl, 0.5 * 3 * nm * f8 + trl_by + mesh),
+                "lin_persistent": (0, 4 * steps * (vol_fl + face_fl),
+                                   2 * 3 * nm * f8 + mesh + 4 * steps * 4 * e3 * f8)}
     # B+C (FP32 only): q (Dx, Dy, LF), traces of F and of the mass flux (TR),
     # values at and projection from the quadrature points, the volume
     # divergence (6 products)
@@ -621,11 +627,26 @@ def main():
     ctx.sync()
     wall = time.perf_counter() - t0
     s32 = ctx.download()
+    per_stage = None
+    if variant == "lin" and ne <= 4096:
+        # small linear meshes run every step in one persistent launch; the
+        # same steps launched per stage (HIP graphs), for comparison
+        ctx.set_persistent(False)
+        ctx.upload(*state0)
+        warm(ctx, variant, dt, W, K, "fp32")
+        ctx.upload(*state0)
+        ctx.sync()
+        t1 = time.perf_counter()
+        ctx.rk4(variant, dt, K, precision="fp32")
+        ctx.sync()
+        per_stage = 1e3 * (time.perf_counter() - t1) / K
+        ctx.set_persistent(True)
     # per-kernel times (HIP events around each launch; eager launches)
     phase("mixed path: per-kernel timing")
     ctx.upload(*state0)
     ctx.set_profiling(True)
-    ctx.rk4(variant, dt, min(K, 20), precision="fp32")
+    kprof = min(K, 20)
+    ctx.rk4(variant, dt, kprof, precision="fp32")
     dev_ms, kern = ctx.timing()
     ctx.set_profiling(False)
     dof = 4 * ne * nm * 3
@@ -639,14 +660,17 @@ def main():
         "config": {"workload": WORKLOADS[wl]["name"](nc, ng, ngf, ne, dt, args),
                    "ne": ne, "nm": nm, "numbering": args.numbering, "ordering": args.ordering,
                    "graphs": not args.no_graph,
+                   "persistent": per_stage is not None,
                    "tiles": {"waves_per_workgroup": [t & 15 for t in tiles],
                              "kernel_form": [t >> 4 for t in tiles]},
                    "precision": ("fp32-mfma quadrature, fp64-mfma linear gravity-wave core, "
                                  "fp64 state/RK4") if variant != "lin" else
                                 "fp64-mfma (the linear variant is all linear core)"},
     }
+    if per_stage is not None:
+        out["per_stage_ms_per_step"] = per_stage
     # roofline of the dominant kernel (HIP events on the context stream)
-    counts = algorithmic_counts(nm, ng, ngf, variant)
+    counts = algorithmic_counts(nm, ng, ngf, variant, steps=kprof)
     best = max(kern.items(), key=lambda kv: kv[1][0]) if kern else None
     if best:
         name, (ms_tot, nlaunch) = best

@@ -208,6 +208,19 @@ int spoc_rk4_steps(spoc_ctx *ctx, int variant, int precision, double dt, int nst
 int spoc_set_graph(spoc_ctx *ctx, int on);
 
 /*
+ * Run the linear variant's RK4 steps on the FP32-MFMA path in one
+ * persistent launch (on = 1, the default) where the mesh has at most 4096
+ * elements (256 blocks of 16) and the central flux is used: the launch keeps
+ * every element block on its wave for all the steps and separates the
+ * stages with a grid barrier, in place of four launches per step (with
+ * profiling on, spoc_last_timing names it "lin_persistent").  Results are
+ * identical bit for bit to the launch-per-stage path.  The barrier's wait is bounded: if it runs out, spoc_rk4_steps
+ * returns SPOC_EHIP.  No reference counterpart (the reference's RK4 loop,
+ * src/shwwnl.f:233-284, calls the right-hand side once per stage).
+ */
+int spoc_set_persistent(spoc_ctx *ctx, int on);
+
+/*
  * Discrete adjoint (no reference counterpart).  With R the right-hand side
  * in the reference's modal coordinates (what spoc_rhs computes), the
  * adjoint of one evaluation is (ru,rv,rh) = J_R(u0,v0,h0)^T (au,av,ah)

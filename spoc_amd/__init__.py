@@ -54,7 +54,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     for name in ("spoc_ctx_create", "spoc_ctx_destroy", "spoc_mesh_set", "spoc_fields_set",
                  "spoc_rhs", "spoc_state_upload", "spoc_state_download", "spoc_rk4_steps",
                  "spoc_energy", "spoc_sync", "spoc_tables_build", "spoc_last_timing",
-                 "spoc_set_profiling", "spoc_set_graph", "spoc_version",
+                 "spoc_set_profiling", "spoc_set_graph", "spoc_set_persistent", "spoc_version",
                  "spoc_ensemble_mesh_set", "spoc_ensemble_energy", "spoc_rhs_adjoint",
                  "spoc_adjoint_upload", "spoc_adjoint_download", "spoc_rk4_adjoint_steps",
                  "spoc_autotune", "spoc_set_tiles", "spoc_adjoint_hb_download",
@@ -536,6 +536,12 @@ class Context:
     def set_graph(self, on: bool) -> None:
         """Replay the RK4 step from a HIP graph (default on)."""
         _check(self.lib.spoc_set_graph(self._h, int(on)))
+
+    def set_persistent(self, on: bool) -> None:
+        """Run the linear variant's RK4 steps on the mixed path in one
+        persistent launch where the mesh has at most 4096 elements (default
+        on); the results are the same bit for bit."""
+        _check(self.lib.spoc_set_persistent(self._h, int(on)))
 
     def timing(self):
         tot = C.c_double()

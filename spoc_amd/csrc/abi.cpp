@@ -76,6 +76,8 @@ int f32_rhs(F32Plan *p, hipStream_t s, int variant, const double *const *x, doub
 int f32_rk4_begin(F32Plan *p, hipStream_t s, int variant, double *const *u0);
 int f32_rk4_step(F32Plan *p, hipStream_t s, int variant, double dt, TimerHook hook, void *user);
 int f32_rk4_end(F32Plan *p, hipStream_t s, int variant, double *const *u0);
+int f32_rk4_persist(F32Plan *p, hipStream_t s, int variant, double dt, int nsteps,
+                    const unsigned **abort, TimerHook hook, void *user);
 int f32_autotune(F32Plan *p, hipStream_t s, int variant, int *out);
 void f32_set_tiles(F32Plan *p, int variant, int t0, int t1, int autotune);
 void f32_set_flux(F32Plan *p, double alpha);
@@ -249,6 +251,7 @@ struct spoc_ctx {
   } graph_key{-1, -1, 0.0, -1, 0};
   long version = 0;     // bumped by every mesh/field change
   int use_graph = 1;    // spoc_set_graph
+  int use_persist = 1;  // spoc_set_persistent
   QueueBound queue;     // dispatches queued ahead of the host in spoc_rk4_steps
   int *dflag = nullptr; // device word of the non-finite test
   // internal element numbering (spoc_set_ordering): perm[internal] = the
@@ -1045,8 +1048,18 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
     return bail(f32_fail(rc));
   if (precision == SPOC_FP64 && f64_rk4_begin(c->f64, c->stream, c->tab, u0))
     return bail(fail(SPOC_EHIP, "FP64 state conversion failed"));
+  // small linear meshes on the mixed path: every step in one persistent
+  // launch (f32_rk4_persist says where it applies)
+  const unsigned *abort_word = nullptr;
+  int persist = 1;
+  if (precision == SPOC_FP32_MFMA && c->use_persist &&
+      (persist = f32_rk4_persist(c->f32, c->stream, variant, dt, nsteps, &abort_word,
+                                 c->prof.on ? prof_hook : nullptr, c)) < 0)
+    return bail(f32_fail(persist));
   const bool graph = c->use_graph && !c->prof.on && nsteps >= 2;
-  if (graph) {
+  if (persist == 0) {
+    // (the steps ran above)
+  } else if (graph) {
     // the step's launches are identical every step: replay them as a graph
     // of kGraphSteps steps (one when fewer are asked for), which costs less
     // per launch than one graph per step on small meshes; the steps left
@@ -1106,7 +1119,11 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
   if (nonfinite_launch(c->stream, (size_t)3 * c->nm * c->ne, state_ptr(c, 0), c->dflag))
     return bail(fail(SPOC_EHIP, "non-finite test launch failed"));
   int bad = 0;
+  unsigned aborted = 0;
   HIPCHK(hipMemcpyAsync(&bad, c->dflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  if (abort_word)
+    HIPCHK(hipMemcpyAsync(&aborted, abort_word, sizeof(unsigned), hipMemcpyDeviceToHost,
+                          c->stream));
   HIPCHK(hipEventSynchronize(c->t1));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->queue.reset();
@@ -1135,7 +1152,7 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
                         "wall_eg"};
     int nk = variant == SPOC_LIN ? 3 : 8;
     if (precision == SPOC_FP32_MFMA) {
-      const char *l32[] = {"lin_f64"};
+      const char *l32[] = {persist == 0 ? "lin_persistent" : "lin_f64"};
       const char *n32[] = {"bc_f32", "eg_f32f64"};
       nk = variant == SPOC_LIN ? 1 : 2;
       for (int k = 0; k < nk; k++) {
@@ -1151,6 +1168,10 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
       }
     }
   }
+  if (aborted)
+    return fail(SPOC_EHIP, "persistent RK4 kernel: a stage barrier waited too long (not every "
+                           "wave of the launch was resident); the state is not advanced "
+                           "consistently: call spoc_set_persistent(ctx, 0) and run again");
   if (bad)
     return fail(SPOC_ENAN, "the state holds a NaN or an infinity after %d RK4 steps (dt=%g)",
                 nsteps, dt);
@@ -1318,6 +1339,12 @@ int spoc_set_graph(spoc_ctx *c, int on) {
   if (!c) return fail(SPOC_EINVAL, "null context");
   c->use_graph = on != 0;
   if (!on) drop_graph(c);
+  return SPOC_OK;
+}
+
+int spoc_set_persistent(spoc_ctx *c, int on) {
+  if (!c) return fail(SPOC_EINVAL, "null context");
+  c->use_persist = on != 0;
   return SPOC_OK;
 }
 

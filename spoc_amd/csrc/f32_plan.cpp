@@ -92,6 +92,9 @@ struct F32Plan {
   int forced[4][2] = {};
   bool autotune = true;
   double lf_alpha = 0.0;  // Lax-Friedrichs option (0: central flux)
+  // persistent linear RK4: its barrier counter and abort word (allocated at
+  // its first use)
+  unsigned *dsync = nullptr;
 };
 
 void f32_plan_destroy(F32Plan *p);
@@ -277,7 +280,7 @@ void f32_plan_destroy(F32Plan *p) {
   if (!p) return;
   free_mesh(p);
   for (void *d : {(void *)p->dpack32, (void *)p->dgl32, (void *)p->dpack64, (void *)p->dgl64,
-                  (void *)p->dpack64t})
+                  (void *)p->dpack64t, (void *)p->dsync})
     if (d) (void)hipFree(d);
   delete p;
 }
@@ -590,6 +593,34 @@ int f32_rk4_step(F32Plan *p, hipStream_t s, int variant, double dt, TimerHook ho
     int rc = rhs_kernels(p, a, hook, user);
     if (rc) return rc;
   }
+  return 0;
+}
+
+// nsteps linear RK4 steps in the persistent kernel, one launch per
+// PERSIST_STEPS steps, after f32_rk4_begin (which made the stage-0 traces).
+// Returns 1, launching nothing, where it does not apply: a nonlinear
+// variant, the Lax-Friedrichs flux, a mesh of more than PERSIST_BLOCKS
+// blocks.  *abort: the device word that a wave raises when a stage barrier
+// waits too long (every wave then leaves the launch); the caller reads it
+// once the stream has drained.  hook (may be null) brackets the launches.
+int f32_rk4_persist(F32Plan *p, hipStream_t s, int variant, double dt, int nsteps,
+                    const unsigned **abort, TimerHook hook, void *user) {
+  if (variant != 0 || p->lf_alpha != 0.0 || (p->ne + 15) / 16 > PERSIST_BLOCKS || nsteps < 1)
+    return 1;
+  if (!p->dsync && hipMalloc((void **)&p->dsync, 2 * sizeof(unsigned)) != hipSuccess) {
+    p->dsync = nullptr;
+    return -2;
+  }
+  F32Args a = args(p, s, variant, 0, dt);
+  a.sync = p->dsync;
+  if (hook) hook(user, 0, s);
+  for (int n = 0; n < nsteps; n += PERSIST_STEPS) {
+    a.steps = std::min(PERSIST_STEPS, nsteps - n);
+    int rc = call(p, a, 6, nullptr, nullptr);
+    if (rc) return rc;
+  }
+  if (hook) hook(user, 1, s);
+  *abort = p->dsync + 1;
   return 0;
 }
 

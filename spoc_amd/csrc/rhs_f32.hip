@@ -67,10 +67,11 @@ constexpr bool lin_split(int form) { return form == 1; }
 constexpr int lin_waves(int form) { return form == 2 ? 4 : 3; }
 
 // Stage input of field f: the step's start state c0 at the first stage (and
-// for a bare right-hand side), else what the previous stage stored in
-// ys[stage - 1].  (A select on the launch-uniform stage, not a branch.)
-__device__ __forceinline__ const double *stage_in(const F32Args &A, int f) {
-  return A.stage <= 0 ? A.st.c0[f] : A.st.ys[A.stage - 1][f];
+// for a bare right-hand side, stage < 0), else what the previous stage
+// stored in ys[stage - 1].  (A select on the launch-uniform stage, not a
+// branch.)
+__device__ __forceinline__ const double *stage_in(const F32Args &A, int stage, int f) {
+  return stage <= 0 ? A.st.c0[f] : A.st.ys[stage - 1][f];
 }
 
 // Linear gravity-wave core (VAR 0: the whole linear right-hand side,
@@ -84,10 +85,14 @@ __device__ __forceinline__ const double *stage_in(const F32Args &A, int f) {
 // core's own before the lift, which is linear, so one FP64 product lifts
 // both, and their gathers issue with the core's.  ROLE: 0 all three
 // equations, 1 those of u and v only (G[0], G[1]), 2 that of eta only (G[2]).
-template <int NC, int NG, int NGF, int VAR, bool LF = false, bool FLUX = false, int ROLE = 0>
-__device__ __forceinline__ void core64(const F32Args &A, const double *dx, const double *dy,
-                                       const double *lf, const Nbr3 &N, int eb,
-                                       doublex4 (&G)[3][S<NC, NG, NGF>::RBM]) {
+// RES: the stage input comes from registers (xin, B fragments, zero past the
+// mesh and past NM) instead of memory.
+template <int NC, int NG, int NGF, int VAR, bool LF = false, bool FLUX = false, int ROLE = 0,
+          bool RES = false>
+__device__ __forceinline__ void core64(const F32Args &A, int stage, const double *dx,
+                                       const double *dy, const double *lf, const Nbr3 &N, int eb,
+                                       doublex4 (&G)[3][S<NC, NG, NGF>::RBM],
+                                       const double (*xin)[S<NC, NG, NGF>::KM] = nullptr) {
   using Sh = S<NC, NG, NGF>;
   constexpr int NM = Sh::NM, NE3 = Sh::NE3, KM = Sh::KM, KE = Sh::KE, RBM = Sh::RBM;
   const int l = lane(), g = l >> 4, col = l & 15, e0 = eb * 16, e = e0 + col;
@@ -98,7 +103,7 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
   const size_t mb = (size_t)e0 * NM;
   // (a select, not an index: a runtime index into the kernel argument would
   // copy the whole argument block to scratch)
-  const double *trL = (A.stage > 0 && (A.stage & 1)) ? A.st.trL[1] : A.st.trL[0];
+  const double *trL = (stage > 0 && (stage & 1)) ? A.st.trL[1] : A.st.trL[0];
   // local Lax-Friedrichs option: F.n_out gains (c/2) |n| (q_own - q_nbr) for
   // q = eta, u, v with c = alpha sqrt(g hh0); lifted as (c/4) |n| (q_nbr -
   // q_own) (a wall point is its own neighbour: no jump).  A compile-time
@@ -106,11 +111,16 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
   // in each face iteration would keep the loads of successive faces apart.
   constexpr bool lfo = LF;
   const double lfc = lfo ? A.lf_alpha * sqrt(g0 * A.fld.hh0e[valid ? e : 0]) : 0.0;
-  const float *trA = (A.stage > 0 && (A.stage & 1)) ? A.st.trA[1] : A.st.trA[0];
+  const float *trA = (stage > 0 && (stage & 1)) ? A.st.trA[1] : A.st.trA[0];
   if (ROLE != 2) {  // u, v: -g grad(eta), volume and face terms
     double bh[KM], fx[KE], fy[KE];
     {
-      bload<NM>(stage_in(A, 2) + mb, g, col, valid, bh);
+      if (RES) {
+#pragma unroll
+        for (int s = 0; s < KM; s++) bh[s] = xin[2][s];
+      } else {
+        bload<NM>(stage_in(A, stage, 2) + mb, g, col, valid, bh);
+      }
 #pragma unroll
       for (int s = 0; s < KM; s++) bh[s] *= -g0;
     }
@@ -167,8 +177,16 @@ __device__ __forceinline__ void core64(const F32Args &A, const double *dx, const
   if (ROLE != 1) {  // eta: -div(H0 u); the nonlinear variants project the flux on walls
     double bu[KM], bv[KM], fh[KE];
     {
-      bload<NM>(stage_in(A, 0) + mb, g, col, valid, bu);
-      bload<NM>(stage_in(A, 1) + mb, g, col, valid, bv);
+      if (RES) {
+#pragma unroll
+        for (int s = 0; s < KM; s++) {
+          bu[s] = xin[0][s];
+          bv[s] = xin[1][s];
+        }
+      } else {
+        bload<NM>(stage_in(A, stage, 0) + mb, g, col, valid, bu);
+        bload<NM>(stage_in(A, stage, 1) + mb, g, col, valid, bv);
+      }
 #pragma unroll
       for (int s = 0; s < KM; s++) {
         bu[s] *= hh0;
@@ -333,11 +351,11 @@ __device__ __forceinline__ void traces_uv(const F32Args &A, const double *O, int
 // the launch, so its branches are scalar and every load of a field issues
 // at once.
 template <int NM, int KM>
-__device__ __forceinline__ void update(const F32Args &A, int f, int eb, double (&x)[KM]) {
+__device__ __forceinline__ void update(const F32Args &A, int stage, int f, int eb,
+                                       double (&x)[KM]) {
   const int g = lane() >> 4, col = lane() & 15;
   const bool valid = eb * 16 + col < A.mesh.ne;
   const size_t mb = (size_t)eb * 16 * NM;
-  const int stage = A.stage;
   const double dt = A.dt;
   int ix[KM];
 #pragma unroll
@@ -390,19 +408,19 @@ __device__ __forceinline__ void update(const F32Args &A, int f, int eb, double (
 // src/calvarnl_*.f at the end of cal_var), then either the tendencies
 // (stage < 0) or the RK4 stage and the traces of the next stage input.
 template <int NC, int NG, int NGF, int VAR, int ROLE = 0>
-__device__ __forceinline__ void finish64(const F32Args &A, const double *O, int eb,
+__device__ __forceinline__ void finish64(const F32Args &A, int stage, const double *O, int eb,
                                          doublex4 (&G)[3][S<NC, NG, NGF>::RBM]) {
   using Sh = S<NC, NG, NGF>;
   constexpr int NM = Sh::NM, KM = Sh::KM;
   const int col = lane() & 15, e = eb * 16 + col;
   const bool valid = e < A.mesh.ne;
-  const int w = (A.stage + 1) & 1;
+  const int w = (stage + 1) & 1;
   if (ROLE != 1) {
     double xh[KM];
 #pragma unroll
     for (int s = 0; s < KM; s++) xh[s] = G[2][s >> 2][s & 3];
-    update<NM>(A, 2, eb, xh);
-    if (A.stage >= 0) traces_h<NC, NG, NGF>(A, O, eb, w, xh);
+    update<NM>(A, stage, 2, eb, xh);
+    if (stage >= 0) traces_h<NC, NG, NGF>(A, O, eb, w, xh);
   }
   if (ROLE == 2) return;
   phase_fence();
@@ -417,9 +435,9 @@ __device__ __forceinline__ void finish64(const F32Args &A, const double *O, int 
     const double *gw = A.mesh.geo64 + (size_t)(valid ? e : 0) * GEO;
     wall_pair64<NM, VAR == 2>(A.ops.gl64, wall, gw[14], gw[15], xu, xv);
   }
-  update<NM>(A, 0, eb, xu);
-  update<NM>(A, 1, eb, xv);
-  if (A.stage >= 0) traces_uv<NC, NG, NGF, VAR>(A, O, eb, w, xu, xv);
+  update<NM>(A, stage, 0, eb, xu);
+  update<NM>(A, stage, 1, eb, xv);
+  if (stage >= 0) traces_uv<NC, NG, NGF, VAR>(A, O, eb, w, xu, xv);
 }
 
 // ---------------------------------------------------------------------------
@@ -469,18 +487,235 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(lin_wav
   const Nbr3 N = nbr3(A.mesh, e < A.mesh.ne ? e : 0);
   doublex4 G[3][Sh::RBM];
   if (!SPLIT) {
-    core64<NC, NG, NGF, 0, LF>(A, O + P.dx, O + P.dy, O + P.lf, N, eb, G);
+    core64<NC, NG, NGF, 0, LF>(A, A.stage, O + P.dx, O + P.dy, O + P.lf, N, eb, G);
     phase_fence();
-    finish64<NC, NG, NGF, 0>(A, O, eb, G);
+    finish64<NC, NG, NGF, 0>(A, A.stage, O, eb, G);
   } else if (wv & 1) {  // the eta equation
-    core64<NC, NG, NGF, 0, LF, false, 2>(A, O + P.dx, O + P.dy, O + P.lf, N, eb, G);
+    core64<NC, NG, NGF, 0, LF, false, 2>(A, A.stage, O + P.dx, O + P.dy, O + P.lf, N, eb, G);
     phase_fence();
-    finish64<NC, NG, NGF, 0, 2>(A, O, eb, G);
+    finish64<NC, NG, NGF, 0, 2>(A, A.stage, O, eb, G);
   } else {  // the u and v equations
-    core64<NC, NG, NGF, 0, LF, false, 1>(A, O + P.dx, O + P.dy, O + P.lf, N, eb, G);
+    core64<NC, NG, NGF, 0, LF, false, 1>(A, A.stage, O + P.dx, O + P.dy, O + P.lf, N, eb, G);
     phase_fence();
-    finish64<NC, NG, NGF, 0, 1>(A, O, eb, G);
+    finish64<NC, NG, NGF, 0, 1>(A, A.stage, O, eb, G);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent linear RK4 for small meshes: A.steps RK4 steps of the linear
+// right-hand side in one launch.  Each wave keeps its element block for the
+// whole launch, and with it the block's state in registers: the step's
+// start c, the stage input x and the running sum d of the stage increments
+// (the RK4 stage update of `update`, with Y2 - c, 2 (Y3 - c) and Y4 - c
+// added as each stage forms them, in the order the last stage adds them
+// there, so the results are the same bit for bit).  Only the traces of each
+// stage input go through memory, since a stage reads its neighbours'; the
+// waves meet at a grid barrier between stages, and c0 is written once, at
+// the end.  The operator images are staged in LDS once.  Meant for meshes of
+// at most PERSIST_BLOCKS blocks, far fewer waves than the chip holds at one
+// wave per SIMD, so every wave is resident and the barrier can wait for all
+// of them; the register budget is that of one wave per SIMD.
+// ---------------------------------------------------------------------------
+constexpr unsigned kSpinLimit = 1u << 20;
+
+// RK4 stage update of field f with the block's state in registers (see
+// `update` for the arithmetic): x holds the stage's tendency and leaves as
+// the stage result; c (the step's start) and d (the running sum) advance;
+// the last stage of the launch stores the step's result to c0.
+template <int NM, int KM>
+__device__ __forceinline__ void update_res(const F32Args &A, int stage, int f, int eb,
+                                           double (&x)[KM], double (&c)[KM], double (&d)[KM],
+                                           bool last) {
+  const int g = lane() >> 4, col = lane() & 15;
+  const bool valid = eb * 16 + col < A.mesh.ne;
+  const double dt = A.dt;
+#pragma unroll
+  for (int s = 0; s < KM; s++) x[s] = valid ? x[s] : 0.0;
+  double y[KM];
+  if (stage == 3) {
+    const double third = 1.0 / 3.0, w = dt / 6.0;
+#pragma unroll
+    for (int s = 0; s < KM; s++) {
+      y[s] = c[s] + (third * d[s] + w * x[s]);
+      c[s] = y[s];
+    }
+    if (last) {
+      const size_t mb = (size_t)eb * 16 * NM;
+#pragma unroll
+      for (int s = 0; s < KM; s++)
+        if (4 * s + g < NM) A.st.c0[f][mb + (4 * s + g) * 16 + col] = y[s];
+    }
+  } else {
+    const double h = stage == 2 ? dt : 0.5 * dt;
+#pragma unroll
+    for (int s = 0; s < KM; s++) {
+      y[s] = c[s] + h * x[s];
+      const double inc = y[s] - c[s];
+      d[s] = stage == 0 ? inc : stage == 1 ? d[s] + 2.0 * inc : d[s] + inc;
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < KM; s++) x[s] = y[s];
+}
+
+// traces_uv for the linear variant with the central flux, its per-lane data
+// from registers: hh0 (the core depth) and the face normal of each trace row
+// this lane stores (tnx, tny: row drow(rb, r, g) at 4 rb + r).
+template <int NC, int NG, int NGF>
+__device__ __forceinline__ void traces_uv_res(const F32Args &A, const double *O, int eb, int w,
+                                              const double (&xu)[S<NC, NG, NGF>::KM],
+                                              const double (&xv)[S<NC, NG, NGF>::KM], double hh0,
+                                              const double (&tnx)[4 * S<NC, NG, NGF>::RBE],
+                                              const double (&tny)[4 * S<NC, NG, NGF>::RBE]) {
+  using Sh = S<NC, NG, NGF>;
+  constexpr int NM = Sh::NM, NE3 = Sh::NE3, KM = Sh::KM, RBE = Sh::RBE;
+  const int l = lane(), g = l >> 4, col = l & 15;
+  double *trL = w ? A.st.trL[1] : A.st.trL[0];
+#pragma unroll
+  for (int rb = 0; rb < RBE; rb++) {
+    double at[KM];
+    afrag64<NM>(O + Sh::P64.tr, rb, at);
+    doublex4 Tu = chain64(at, xu, zero64()), Tv = chain64(at, xv, zero64());
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      int kj = drow(rb, r, g);
+      if (kj < NE3)
+        tr_put<NE3, 2>(trL, eb, col, 1, kj,
+                       hh0 * (Tu[r] * tnx[4 * rb + r] + Tv[r] * tny[4 * rb + r]));
+    }
+  }
+}
+
+// Stage barrier, one arrival per workgroup: every wave makes its stores
+// visible to the device (release fence: they have left the wave and the
+// XCD's L2 has written them back), the workgroup's waves meet, wave 0
+// counts the arrival and waits for `target` arrivals, the waves meet again
+// and each takes the device's stores (acquire fence).  The wait is bounded:
+// a workgroup that gives up (or sees another's abort) raises the abort word
+// and returns false, so every wave leaves the launch and the host reports it.
+__device__ __forceinline__ bool stage_barrier(unsigned *sync, unsigned target, int wv) {
+  __shared__ int ok;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (wv == 0) {
+    int good = 1;
+    if (lane() == 0) {
+      __hip_atomic_fetch_add(sync, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (unsigned it = 0;
+           __hip_atomic_load(sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; it++) {
+        if (it >= kSpinLimit ||
+            __hip_atomic_load(sync + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+          __hip_atomic_store(sync + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          good = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      ok = good;
+    }
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return __builtin_amdgcn_readfirstlane(ok) != 0;
+}
+
+template <int NC, int NG, int NGF>  // (one per shape's object file)
+__global__ void sync_reset_kernel(unsigned *sync) {
+  if (threadIdx.x < 2) sync[threadIdx.x] = 0u;
+}
+
+template <int NC, int NG, int NGF, int WG>
+__global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(1, 1))) void lin_persistent_kernel(F32Args A, int nblk) {
+  using Sh = S<NC, NG, NGF>;
+  constexpr int NM = Sh::NM, NE3 = Sh::NE3, KM = Sh::KM, RBE = Sh::RBE;
+  constexpr Pack64 P = Sh::P64;
+  constexpr int NGL = img64_size(NM, NM) / 2;  // the wall projection's image
+  __shared__ floatx4 lds[P.total / 2];
+  __shared__ floatx4 lgl[NGL];
+  const int b0 = first_block<WG>();
+  if (b0 >= nblk) return;  // (a whole workgroup: not counted at the barriers)
+  stage_lds<WG>(lds, reinterpret_cast<const floatx4 *>(A.ops.pack64), P.total / 2);
+  stage_lds<WG>(lgl, reinterpret_cast<const floatx4 *>(A.ops.gl64), NGL);
+  __syncthreads();
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int eb = b0 + wv;
+  // a wave past the last block computes nothing but stays for the barriers
+  const bool active = eb < nblk;
+  const double *O = reinterpret_cast<const double *>(lds);
+  const int g = lane() >> 4, col = lane() & 15, e = eb * 16 + col;
+  const bool valid = active && e < A.mesh.ne;
+  const Nbr3 N = nbr3(A.mesh, valid ? e : 0);
+  // the block's state: c = the step's start (rows past NM repeat row NM - 1,
+  // as `update` reads them), x = the stage input (zero past the mesh and
+  // past NM, as bload gives it), d = the running sum of the increments
+  double c[3][KM], x[3][KM], d[3][KM];
+  // and what the stage's epilogue reads of the mesh, which does not change:
+  // the wall flag and normal, the core depth and the face normal of each
+  // trace row the lane stores
+  const double *gg = A.mesh.geo64 + (size_t)(valid ? e : 0) * GEO;
+  const bool wall = A.mesh.bdl[valid ? e : 0] != 0 && valid;
+  const double wnx = gg[14], wny = gg[15];
+  const double hh0 = (valid ? 1.0 : 0.0) * core_depth<0>(A.fld, valid ? e : 0);
+  double tnx[4 * RBE], tny[4 * RBE];
+#pragma unroll
+  for (int rb = 0; rb < RBE; rb++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int kj = drow(rb, r, g), k = (kj < NE3 ? kj : NE3 - 1) / NGF;
+      tnx[4 * rb + r] = gg[5 + k];
+      tny[4 * rb + r] = gg[8 + k];
+    }
+  if (active) {
+    const size_t mb = (size_t)eb * 16 * NM;
+#pragma unroll
+    for (int f = 0; f < 3; f++)
+#pragma unroll
+      for (int s = 0; s < KM; s++) {
+        const int m = 4 * s + g;
+        c[f][s] = A.st.c0[f][mb + (m < NM ? m : NM - 1) * 16 + col];
+        x[f][s] = valid && m < NM ? c[f][s] : 0.0;
+        d[f][s] = 0.0;
+      }
+  }
+  // workgroups with a block: first_block maps them onto 0, WG, 2 WG, ...
+  const unsigned nwg = (unsigned)cdiv(nblk, WG);
+  unsigned target = 0;
+  for (int n = 0; n < A.steps; n++)
+    for (int st = 0; st < 4; st++) {
+      const bool last = n + 1 == A.steps && st == 3;
+      if (active) {
+        doublex4 G[3][Sh::RBM];
+        core64<NC, NG, NGF, 0, false, false, 0, true>(A, st, O + P.dx, O + P.dy, O + P.lf, N,
+                                                      eb, G, x);
+        phase_fence();
+        // finish64's epilogue (VAR 0) with the register update
+        const int w = (st + 1) & 1;
+#pragma unroll
+        for (int s = 0; s < KM; s++) x[2][s] = G[2][s >> 2][s & 3];
+        update_res<NM>(A, st, 2, eb, x[2], c[2], d[2], last);
+        traces_h<NC, NG, NGF>(A, O, eb, w, x[2]);
+        phase_fence();
+#pragma unroll
+        for (int s = 0; s < KM; s++) {
+          x[0][s] = G[0][s >> 2][s & 3];
+          x[1][s] = G[1][s >> 2][s & 3];
+        }
+        if (__ballot(wall))
+          wall_pair64<NM, false>(reinterpret_cast<const double *>(lgl), wall, wnx, wny, x[0],
+                                 x[1]);
+        update_res<NM>(A, st, 0, eb, x[0], c[0], d[0], last);
+        update_res<NM>(A, st, 1, eb, x[1], c[1], d[1], last);
+        traces_uv_res<NC, NG, NGF>(A, O, eb, w, x[0], x[1], hh0, tnx, tny);
+        // the next stage input as bload would give it
+#pragma unroll
+        for (int f = 0; f < 3; f++)
+#pragma unroll
+          for (int s = 0; s < KM; s++) x[f][s] = valid && 4 * s + g < NM ? x[f][s] : 0.0;
+      }
+      if (last) break;
+      target += nwg;
+      if (!stage_barrier(A.sync, target, wv)) return;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -515,9 +750,9 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(bc_wave
   // order), then the stage input, then the geometry
   const Nbr3 N = nbr3(A.mesh, valid ? e : 0);
   float bu[KM], bv[KM], bh[KM];
-  bload<NM>(stage_in(A, 0) + mb, g, col, valid, bu);
-  bload<NM>(stage_in(A, 1) + mb, g, col, valid, bv);
-  bload<NM>(stage_in(A, 2) + mb, g, col, valid, bh);
+  bload<NM>(stage_in(A, A.stage, 0) + mb, g, col, valid, bu);
+  bload<NM>(stage_in(A, A.stage, 1) + mb, g, col, valid, bv);
+  bload<NM>(stage_in(A, A.stage, 2) + mb, g, col, valid, bh);
   const float mu0 = A.fld.mu0, mb0 = A.fld.mb0;
   const float *gg = A.mesh.geo + (size_t)(valid ? e : 0) * GEO;
   const float ax = gg[0], bx = gg[1], ay = gg[2], by = gg[3], di = valid ? gg[4] : 0.f;
@@ -842,8 +1077,8 @@ __device__ __forceinline__ void eg_part(const F32Args &A, const double *O64, int
   // then the FP32 volume part B+C stored (gvol) added to it (read only now,
   // so that its registers are free while the core keeps its loads in flight)
   doublex4 G[3][RBM];
-  core64<NC, NG, NGF, VAR, RUS, true, ROLE>(A, O64 + P64.dx, O64 + P64.dy, O64 + P64.lf, N, eb,
-                                            G);
+  core64<NC, NG, NGF, VAR, RUS, true, ROLE>(A, A.stage, O64 + P64.dx, O64 + P64.dy, O64 + P64.lf,
+                                            N, eb, G);
 #pragma unroll
   for (int f = (ROLE == 2 ? 2 : 0); f < (ROLE == 1 ? 2 : 3); f++)
 #pragma unroll
@@ -855,7 +1090,7 @@ __device__ __forceinline__ void eg_part(const F32Args &A, const double *O64, int
         G[f][rb][r] += (valid && m < NM) ? (double)v : 0.0;
       }
   phase_fence();
-  finish64<NC, NG, NGF, VAR, ROLE>(A, O64, eb, G);
+  finish64<NC, NG, NGF, VAR, ROLE>(A, A.stage, O64, eb, G);
 }
 
 // Nonlinear stages E + G.  Split (eg_split): each block of 16 elements gets
@@ -982,6 +1217,24 @@ int launch_elem(const F32Args &A, int phase, int nblk) {
   return 0;
 }
 
+// Launches the persistent kernel when the device can hold its whole grid at
+// once (its occupancy per CU times the CUs), else returns 1 and launches
+// nothing.
+template <int NC, int NG, int NGF, int WG>
+int launch_persist(const F32Args &A, int nblk) {
+  const int grid = 8 * cdiv(cdiv(nblk, WG), 8);
+  const void *kern = reinterpret_cast<const void *>(&lin_persistent_kernel<NC, NG, NGF, WG>);
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * WG, 0) != hipSuccess)
+    return -2;
+  if ((long)per_cu * cus < grid) return 1;
+  sync_reset_kernel<NC, NG, NGF><<<1, 64, 0, A.stream>>>(A.sync);
+  lin_persistent_kernel<NC, NG, NGF, WG><<<grid, 64 * WG, 0, A.stream>>>(A, nblk);
+  return 0;
+}
+
 template <int NC, int NG, int NGF, int WG>
 int launch_form(const F32Args &A, int phase, int nblk, int form) {
   if (form == 1) return launch_elem<NC, NG, NGF, WG, 1>(A, phase, nblk);
@@ -1019,6 +1272,17 @@ int launch(const F32Args &A, int phase, const double *in, double *out) {
       if (wg == 2) launch_form<NC, NG, NGF, 2>(A, phase, nblk, form);
       else if (wg == 8) launch_form<NC, NG, NGF, 8>(A, phase, nblk, form);
       else launch_form<NC, NG, NGF, 4>(A, phase, nblk, form);
+      break;
+    }
+    case 6: {  // persistent linear RK4 (A.steps steps), small meshes
+      if (v != 0 || A.lf_alpha != 0.0 || nblk > PERSIST_BLOCKS || !A.sync || A.steps < 1 ||
+          A.steps > PERSIST_STEPS) return -1;
+      // two waves per workgroup (timed on the 512-element basin: 20.3 us
+      // per step, against 22.2 at one and 24.4 at four) and one form (all
+      // three equations per wave, which keeps the whole state of a block
+      // together); the tile codes of the per-stage kernels do not apply
+      const int r = launch_persist<NC, NG, NGF, 2>(A, nblk);
+      if (r) return r;  // 1: the grid does not fit at once (nothing launched)
       break;
     }
     default:

@@ -1,0 +1,94 @@
+"""Persistent linear RK4 on the mixed path (spoc_set_persistent; no reference
+counterpart): every step of a spoc_rk4_steps call in one launch whose waves
+keep their element blocks and meet at a grid barrier between stages.  Held
+to bit-for-bit equality with the launch-per-stage path it replaces (the
+reference's loop, src/shwwnl.f:233-284, one right-hand side per stage), in
+every compiled form and workgroup size, across calls of different lengths
+and time steps, up to the largest mesh it takes (4096 elements) and just
+past it (where it hands over to the per-stage path)."""
+import sys
+
+import numpy as np
+import pytest
+
+import spoc_amd
+from conftest import host_tables
+from spoc_amd import configs
+
+pytestmark = pytest.mark.gpu
+_ctx = {}
+
+
+def ctx_for(nc, ng, ngf):
+    if (nc, ng, ngf) not in _ctx:
+        t = host_tables(nc, ng, ngf)
+        _ctx[(nc, ng, ngf)] = (t, spoc_amd.Context(t))
+    return _ctx[(nc, ng, ngf)]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def set_fields(ctx, p):
+    ctx.set_fields(p["hb"], p["g0"], p["fc"], p["w0"], p["z0"], p["mu0"], p["mb0"])
+
+
+def calls(ctx, c, persist):
+    ctx.set_persistent(persist)
+    ctx.upload(*c["state"])
+    ctx.rk4("lin", 2.0, 1, precision="fp32")
+    ctx.rk4("lin", 2.0, 5, precision="fp32")
+    ctx.rk4("lin", 1.5, 3, precision="fp32")
+    out = ctx.download()
+    ctx.set_persistent(True)
+    return out
+
+
+def same(a, b):
+    for f in range(3):
+        assert np.array_equal(a[f], b[f]), (f, np.abs(a[f] - b[f]).max())
+
+
+@pytest.mark.parametrize("nc,ng,ngf", [(3, 12, 4), (6, 33, 7), (2, 6, 3)])
+def test_persistent_equals_per_stage_launches(nc, ng, ngf):
+    t, ctx = ctx_for(nc, ng, ngf)
+    c = configs.random_case(t, 12, 10, seed=31 + nc, scale=0.01)
+    ctx.set_mesh(c["mesh"])
+    set_fields(ctx, c["phys"])
+    for form in range(3):
+        for wg in (2, 4, 8):
+            say("persistent", nc, ng, ngf, "form", form, "waves", wg)
+            ctx.set_tiles("lin", wg + 16 * form, 0, autotune=False)
+            same(calls(ctx, c, False), calls(ctx, c, True))
+    ctx.set_tiles("lin", 0, 0, autotune=True)
+
+
+@pytest.mark.parametrize("n", [46, 47])
+def test_persistent_at_the_mesh_limit(n):
+    """4050 elements (254 blocks): persistent; 4232: the per-stage path."""
+    t, ctx = ctx_for(3, 12, 4)
+    c = configs.random_case(t, n, n, seed=5, scale=0.01)
+    say("persistent limit", c["mesh"]["ne"])
+    ctx.set_mesh(c["mesh"])
+    set_fields(ctx, c["phys"])
+    same(calls(ctx, c, False), calls(ctx, c, True))
+
+
+def test_persistent_config1_long_run():
+    """The 512-element workload: 300 steps in one call, against the
+    per-stage path (graphs of 8 steps)."""
+    t, ctx = ctx_for(3, 12, 4)
+    cfg = configs.config1(t)
+    ctx.set_mesh(cfg["mesh"])
+    set_fields(ctx, cfg["phys"])
+    say("persistent config1", cfg["mesh"]["ne"])
+    out = []
+    for persist in (False, True):
+        ctx.set_persistent(persist)
+        ctx.upload(*cfg["state"])
+        ctx.rk4("lin", cfg["dt"], 300, precision="fp32")
+        out.append(ctx.download())
+    ctx.set_persistent(True)
+    same(out[0], out[1])

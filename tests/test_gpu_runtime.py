@@ -112,6 +112,7 @@ def test_graph_replay_equals_eager(prec, variant):
 
     def run(graph):
         ctx.set_graph(graph)
+        ctx.set_persistent(False)  # (the linear steps would not use the graph)
         ctx.upload(*c["state"])
         ctx.rk4(variant, 2.0, 3, precision=prec)   # graph captured, replayed 3x
         ctx.rk4(variant, 2.0, 1, precision=prec)   # eager step between replays
@@ -123,6 +124,7 @@ def test_graph_replay_equals_eager(prec, variant):
 
     eager, graph = run(False), run(True)
     ctx.set_graph(True)
+    ctx.set_persistent(True)
     for f in range(3):
         assert np.array_equal(eager[f], graph[f]), (f, np.abs(eager[f] - graph[f]).max())
 
