@@ -7,6 +7,7 @@
 #include <cmath>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -603,6 +604,16 @@ int f32_rk4_step(F32Plan *p, hipStream_t s, int variant, double dt, TimerHook ho
 // blocks.  *abort: the device word that a wave raises when a stage barrier
 // waits too long (every wave then leaves the launch); the caller reads it
 // once the stream has drained.  hook (may be null) brackets the launches.
+namespace {
+// Persistent launches on one device run one after another, whatever plan or
+// stream they come from: two at once could each hold part of the device and
+// wait for each other's workgroups at their barriers (until the bounded wait
+// ended both with an error).  persist_done[device]: the last one's end.
+constexpr int kMaxDevices = 64;
+std::mutex persist_mu;
+hipEvent_t persist_done[kMaxDevices] = {};
+}  // namespace
+
 int f32_rk4_persist(F32Plan *p, hipStream_t s, int variant, double dt, int nsteps,
                     const unsigned **abort, TimerHook hook, void *user) {
   if (variant != 0 || p->lf_alpha != 0.0 || (p->ne + 15) / 16 > PERSIST_BLOCKS || nsteps < 1)
@@ -611,15 +622,25 @@ int f32_rk4_persist(F32Plan *p, hipStream_t s, int variant, double dt, int nstep
     p->dsync = nullptr;
     return -2;
   }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return -2;
+  std::lock_guard<std::mutex> lock(persist_mu);
+  hipEvent_t &done = persist_done[dev];
+  if (!done && hipEventCreateWithFlags(&done, hipEventDisableTiming) != hipSuccess) {
+    done = nullptr;
+    return -2;
+  }
+  if (hipStreamWaitEvent(s, done, 0) != hipSuccess) return -2;
   F32Args a = args(p, s, variant, 0, dt);
   a.sync = p->dsync;
   if (hook) hook(user, 0, s);
   for (int n = 0; n < nsteps; n += PERSIST_STEPS) {
     a.steps = std::min(PERSIST_STEPS, nsteps - n);
     int rc = call(p, a, 6, nullptr, nullptr);
-    if (rc) return rc;
+    if (rc) return rc;  // (1 at the first launch: the grid does not fit)
   }
   if (hook) hook(user, 1, s);
+  if (hipEventRecord(done, s) != hipSuccess) return -2;
   *abort = p->dsync + 1;
   return 0;
 }

@@ -92,3 +92,40 @@ def test_persistent_config1_long_run():
         out.append(ctx.download())
     ctx.set_persistent(True)
     same(out[0], out[1])
+
+
+def test_persistent_two_contexts_at_once():
+    """Two contexts (two streams) running persistent launches from two host
+    threads at the same time: the launches on one device run one after the
+    other, and each context gets the bits it gets alone."""
+    import threading
+    t, _ = ctx_for(3, 12, 4)
+    cases = [configs.random_case(t, 20 + 4 * i, 18, seed=60 + i, scale=0.01) for i in range(2)]
+    ctxs = [spoc_amd.Context(t) for _ in range(2)]
+    alone = []
+    for ctx, c in zip(ctxs, cases):
+        ctx.set_mesh(c["mesh"])
+        set_fields(ctx, c["phys"])
+        ctx.upload(*c["state"])
+        ctx.rk4("lin", 2.0, 200, precision="fp32")
+        alone.append(ctx.download())
+        ctx.upload(*c["state"])
+    say("persistent two contexts", [c["mesh"]["ne"] for c in cases])
+    errors = []
+
+    def run(ctx):
+        try:
+            ctx.rk4("lin", 2.0, 200, precision="fp32")
+        except Exception as ex:  # reported below, in the main thread
+            errors.append(ex)
+
+    th = [threading.Thread(target=run, args=(ctx,)) for ctx in ctxs]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    assert not errors, errors
+    for ctx, ref in zip(ctxs, alone):
+        same(ctx.download(), ref)
+    for ctx in ctxs:
+        ctx.close()
