@@ -211,10 +211,11 @@ int spoc_set_graph(spoc_ctx *ctx, int on);
  * Run the linear variant's RK4 steps on the FP32-MFMA path in one
  * persistent launch (on = 1, the default) where the mesh has at most 4096
  * elements (256 blocks of 16) and the central flux is used: the launch keeps
- * every element block on its wave for all the steps and separates the
- * stages with a grid barrier, in place of four launches per step (with
+ * every element block on its wave for all the steps, and before each stage
+ * a block waits only for its neighbouring blocks, in place of four launches
+ * per step (with
  * profiling on, spoc_last_timing names it "lin_persistent").  Results are
- * identical bit for bit to the launch-per-stage path.  The barrier's wait is bounded: if it runs out, spoc_rk4_steps
+ * identical bit for bit to the launch-per-stage path.  The wait is bounded: if it runs out, spoc_rk4_steps
  * returns SPOC_EHIP.  No reference counterpart (the reference's RK4 loop,
  * src/shwwnl.f:233-284, calls the right-hand side once per stage).
  */

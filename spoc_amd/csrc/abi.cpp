@@ -1169,8 +1169,8 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
     }
   }
   if (aborted)
-    return fail(SPOC_EHIP, "persistent RK4 kernel: a stage barrier waited too long (not every "
-                           "wave of the launch was resident); the state is not advanced "
+    return fail(SPOC_EHIP, "persistent RK4 kernel: a wait for a neighbouring block ran out (not "
+                           "every wave of the launch was resident); the state is not advanced "
                            "consistently: call spoc_set_persistent(ctx, 0) and run again");
   if (bad)
     return fail(SPOC_ENAN, "the state holds a NaN or an infinity after %d RK4 steps (dt=%g)",

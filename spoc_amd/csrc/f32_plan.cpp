@@ -93,8 +93,8 @@ struct F32Plan {
   int forced[4][2] = {};
   bool autotune = true;
   double lf_alpha = 0.0;  // Lax-Friedrichs option (0: central flux)
-  // persistent linear RK4: its barrier counter and abort word (allocated at
-  // its first use)
+  // persistent linear RK4: its abort word and per-block stage counts
+  // (PERSIST_SYNC_WORDS, allocated at its first use)
   unsigned *dsync = nullptr;
 };
 
@@ -601,14 +601,14 @@ int f32_rk4_step(F32Plan *p, hipStream_t s, int variant, double dt, TimerHook ho
 // PERSIST_STEPS steps, after f32_rk4_begin (which made the stage-0 traces).
 // Returns 1, launching nothing, where it does not apply: a nonlinear
 // variant, the Lax-Friedrichs flux, a mesh of more than PERSIST_BLOCKS
-// blocks.  *abort: the device word that a wave raises when a stage barrier
-// waits too long (every wave then leaves the launch); the caller reads it
+// blocks.  *abort: the device word that a wave raises when its wait for a
+// neighbouring block runs out (every wave then leaves the launch); the caller reads it
 // once the stream has drained.  hook (may be null) brackets the launches.
 namespace {
 // Persistent launches on one device run one after another, whatever plan or
 // stream they come from: two at once could each hold part of the device and
-// wait for each other's workgroups at their barriers (until the bounded wait
-// ended both with an error).  persist_done[device]: the last one's end.
+// wait for each other's blocks (until the bounded waits ended both with an
+// error).  persist_done[device]: the last one's end.
 constexpr int kMaxDevices = 64;
 std::mutex persist_mu;
 hipEvent_t persist_done[kMaxDevices] = {};
@@ -618,7 +618,8 @@ int f32_rk4_persist(F32Plan *p, hipStream_t s, int variant, double dt, int nstep
                     const unsigned **abort, TimerHook hook, void *user) {
   if (variant != 0 || p->lf_alpha != 0.0 || (p->ne + 15) / 16 > PERSIST_BLOCKS || nsteps < 1)
     return 1;
-  if (!p->dsync && hipMalloc((void **)&p->dsync, 2 * sizeof(unsigned)) != hipSuccess) {
+  if (!p->dsync &&
+      hipMalloc((void **)&p->dsync, PERSIST_SYNC_WORDS * sizeof(unsigned)) != hipSuccess) {
     p->dsync = nullptr;
     return -2;
   }

@@ -190,17 +190,22 @@ constexpr int TILE_FORMS = 3;
 
 // Persistent linear RK4 kernel: meshes of at most PERSIST_BLOCKS blocks of 16
 // elements (every wave of the launch resident at once), and at most
-// PERSIST_STEPS steps per launch (its barrier counter, 4 x steps x waves
-// arrivals, stays below 2^32).
+// PERSIST_STEPS steps per launch (each block's count of finished stages, up
+// to 4 x steps, stays far below 2^32).  Its sync words: the abort word at
+// [1], block b's count at [PERSIST_LINE * (1 + b)], each count on a 128-byte
+// line of its own.
 constexpr int PERSIST_BLOCKS = 256;
 constexpr int PERSIST_STEPS = 1 << 20;
+constexpr int PERSIST_LINE = 32;
+constexpr int PERSIST_SYNC_WORDS = PERSIST_LINE * (1 + PERSIST_BLOCKS);
 
 struct F32Args {
   int variant;
   int stage;  // -1: tendencies only, 0..3: RK4 stage
   int steps;  // persistent RK4 kernel: steps to run in the launch
-  unsigned *sync;  // persistent RK4 kernel: [0] arrivals at its stage barriers,
-                   // [1] abort (a wait that ran out), both zeroed before the launch
+  unsigned *sync;  // persistent RK4 kernel: [1] abort (a wait that ran out),
+                   // [PERSIST_LINE (1 + b)] stages block b has finished; zeroed
+                   // before the launch
   int tile[2];  // waves per workgroup: [0] lin_kernel or bc_kernel, [1] eg_kernel
   double dt;
   // local Lax-Friedrichs (Rusanov) penalty of the gravity-wave core's face

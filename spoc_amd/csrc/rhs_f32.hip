@@ -509,12 +509,13 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(lin_wav
 // (the RK4 stage update of `update`, with Y2 - c, 2 (Y3 - c) and Y4 - c
 // added as each stage forms them, in the order the last stage adds them
 // there, so the results are the same bit for bit).  Only the traces of each
-// stage input go through memory, since a stage reads its neighbours'; the
-// waves meet at a grid barrier between stages, and c0 is written once, at
-// the end.  The operator images are staged in LDS once.  Meant for meshes of
-// at most PERSIST_BLOCKS blocks, far fewer waves than the chip holds at one
-// wave per SIMD, so every wave is resident and the barrier can wait for all
-// of them; the register budget is that of one wave per SIMD.
+// stage input go through memory, since a stage reads its neighbours'; before
+// a stage a wave waits for the blocks around its own to finish the previous
+// one (stage_done / wait_nbrs), and c0 is written once, at the end.  The
+// operator images are staged in LDS once.  Meant for meshes of at most
+// PERSIST_BLOCKS blocks, far fewer waves than the chip holds at one wave per
+// SIMD, so every wave is resident and a wait for another block can end; the
+// register budget is that of one wave per SIMD.
 // ---------------------------------------------------------------------------
 constexpr unsigned kSpinLimit = 1u << 20;
 
@@ -586,42 +587,50 @@ __device__ __forceinline__ void traces_uv_res(const F32Args &A, const double *O,
   }
 }
 
-// Stage barrier, one arrival per workgroup: every wave makes its stores
-// visible to the device (release fence: they have left the wave and the
-// XCD's L2 has written them back), the workgroup's waves meet, wave 0
-// counts the arrival and waits for `target` arrivals, the waves meet again
-// and each takes the device's stores (acquire fence).  The wait is bounded:
-// a workgroup that gives up (or sees another's abort) raises the abort word
-// and returns false, so every wave leaves the launch and the host reports it.
-__device__ __forceinline__ bool stage_barrier(unsigned *sync, unsigned target, int wv) {
-  __shared__ int ok;
+// Stage hand-off between neighbouring blocks (sync: [1] the abort word,
+// [PERSIST_LINE (1 + b)] the number of stages block b has finished).  After a stage a wave
+// makes its stores visible to the device (release fence) and sets its
+// block's count; before the next it waits until every block that holds a
+// face neighbour of its elements has finished as many stages: their traces
+// of the new stage input are then written, and they are done reading the
+// traces this stage overwrites (the traces are double-buffered, alternating
+// per stage, and the neighbour relation is symmetric), so no grid-wide
+// barrier is needed.  Then it takes the device's stores (acquire fence).
+// Every block finishes stage k - 1 waiting only for counts of k - 1, so the
+// waits end; they are also bounded: a wave that gives up (or sees another's
+// abort) raises the abort word and returns false, so every wave leaves the
+// launch and the host reports it.
+__device__ __forceinline__ void stage_done(unsigned *sync, int eb, unsigned k) {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  __syncthreads();
-  if (wv == 0) {
-    int good = 1;
-    if (lane() == 0) {
-      __hip_atomic_fetch_add(sync, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      for (unsigned it = 0;
-           __hip_atomic_load(sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; it++) {
-        if (it >= kSpinLimit ||
-            __hip_atomic_load(sync + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
-          __hip_atomic_store(sync + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          good = 0;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-      }
-      ok = good;
+  if (lane() == 0)
+    __hip_atomic_store(sync + PERSIST_LINE * (1 + eb), k, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool wait_nbrs(unsigned *sync, const Nbr3 &N, int eb, unsigned k) {
+  const unsigned *fl = sync + PERSIST_LINE;
+  const int b0 = PERSIST_LINE * (N.nb0 >= 0 ? N.nb0 >> 4 : eb),
+            b1 = PERSIST_LINE * (N.nb1 >= 0 ? N.nb1 >> 4 : eb),
+            b2 = PERSIST_LINE * (N.nb2 >= 0 ? N.nb2 >> 4 : eb);
+  for (unsigned it = 0;; it++) {
+    const unsigned f0 = __hip_atomic_load(fl + b0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                   f1 = __hip_atomic_load(fl + b1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                   f2 = __hip_atomic_load(fl + b2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (__ballot(f0 < k || f1 < k || f2 < k) == 0) break;
+    if (it >= kSpinLimit ||
+        __hip_atomic_load(sync + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+      if (lane() == 0) __hip_atomic_store(sync + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
     }
+    __builtin_amdgcn_s_sleep(1);
   }
-  __syncthreads();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  return __builtin_amdgcn_readfirstlane(ok) != 0;
+  return true;
 }
 
 template <int NC, int NG, int NGF>  // (one per shape's object file)
-__global__ void sync_reset_kernel(unsigned *sync) {
-  if (threadIdx.x < 2) sync[threadIdx.x] = 0u;
+__global__ void sync_reset_kernel(unsigned *sync, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) sync[i] = 0u;
 }
 
 template <int NC, int NG, int NGF, int WG>
@@ -633,17 +642,16 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(1, 1)))
   __shared__ floatx4 lds[P.total / 2];
   __shared__ floatx4 lgl[NGL];
   const int b0 = first_block<WG>();
-  if (b0 >= nblk) return;  // (a whole workgroup: not counted at the barriers)
+  if (b0 >= nblk) return;
   stage_lds<WG>(lds, reinterpret_cast<const floatx4 *>(A.ops.pack64), P.total / 2);
   stage_lds<WG>(lgl, reinterpret_cast<const floatx4 *>(A.ops.gl64), NGL);
   __syncthreads();
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int eb = b0 + wv;
-  // a wave past the last block computes nothing but stays for the barriers
-  const bool active = eb < nblk;
+  if (eb >= nblk) return;  // (no block: nobody waits for it)
   const double *O = reinterpret_cast<const double *>(lds);
   const int g = lane() >> 4, col = lane() & 15, e = eb * 16 + col;
-  const bool valid = active && e < A.mesh.ne;
+  const bool valid = e < A.mesh.ne;
   const Nbr3 N = nbr3(A.mesh, valid ? e : 0);
   // the block's state: c = the step's start (rows past NM repeat row NM - 1,
   // as `update` reads them), x = the stage input (zero past the mesh and
@@ -665,7 +673,7 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(1, 1)))
       tnx[4 * rb + r] = gg[5 + k];
       tny[4 * rb + r] = gg[8 + k];
     }
-  if (active) {
+  {
     const size_t mb = (size_t)eb * 16 * NM;
 #pragma unroll
     for (int f = 0; f < 3; f++)
@@ -677,13 +685,12 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(1, 1)))
         d[f][s] = 0.0;
       }
   }
-  // workgroups with a block: first_block maps them onto 0, WG, 2 WG, ...
-  const unsigned nwg = (unsigned)cdiv(nblk, WG);
-  unsigned target = 0;
+  unsigned done = 0;  // stages finished
   for (int n = 0; n < A.steps; n++)
     for (int st = 0; st < 4; st++) {
       const bool last = n + 1 == A.steps && st == 3;
-      if (active) {
+      if (done > 0 && !wait_nbrs(A.sync, N, eb, done)) return;
+      {
         doublex4 G[3][Sh::RBM];
         core64<NC, NG, NGF, 0, false, false, 0, true>(A, st, O + P.dx, O + P.dy, O + P.lf, N,
                                                       eb, G, x);
@@ -713,8 +720,7 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(1, 1)))
           for (int s = 0; s < KM; s++) x[f][s] = valid && 4 * s + g < NM ? x[f][s] : 0.0;
       }
       if (last) break;
-      target += nwg;
-      if (!stage_barrier(A.sync, target, wv)) return;
+      stage_done(A.sync, eb, ++done);
     }
 }
 
@@ -1230,7 +1236,7 @@ int launch_persist(const F32Args &A, int nblk) {
       hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * WG, 0) != hipSuccess)
     return -2;
   if ((long)per_cu * cus < grid) return 1;
-  sync_reset_kernel<NC, NG, NGF><<<1, 64, 0, A.stream>>>(A.sync);
+  sync_reset_kernel<NC, NG, NGF><<<1, 256, 0, A.stream>>>(A.sync, PERSIST_LINE * (1 + nblk));
   lin_persistent_kernel<NC, NG, NGF, WG><<<grid, 64 * WG, 0, A.stream>>>(A, nblk);
   return 0;
 }
@@ -1277,11 +1283,12 @@ int launch(const F32Args &A, int phase, const double *in, double *out) {
     case 6: {  // persistent linear RK4 (A.steps steps), small meshes
       if (v != 0 || A.lf_alpha != 0.0 || nblk > PERSIST_BLOCKS || !A.sync || A.steps < 1 ||
           A.steps > PERSIST_STEPS) return -1;
-      // two waves per workgroup (timed on the 512-element basin: 20.3 us
-      // per step, against 22.2 at one and 24.4 at four) and one form (all
-      // three equations per wave, which keeps the whole state of a block
-      // together); the tile codes of the per-stage kernels do not apply
-      const int r = launch_persist<NC, NG, NGF, 2>(A, nblk);
+      // one wave per workgroup, so each block's wave has a CU to itself
+      // (timed on the 512-element basin: 18.2 us per step, against 19.3 at
+      // two and 25.1 at four), and one form (all three equations per wave,
+      // which keeps the whole state of a block together); the tile codes of
+      // the per-stage kernels do not apply
+      const int r = launch_persist<NC, NG, NGF, 1>(A, nblk);
       if (r) return r;  // 1: the grid does not fit at once (nothing launched)
       break;
     }

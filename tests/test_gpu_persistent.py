@@ -1,11 +1,12 @@
 """Persistent linear RK4 on the mixed path (spoc_set_persistent; no reference
 counterpart): every step of a spoc_rk4_steps call in one launch whose waves
-keep their element blocks and meet at a grid barrier between stages.  Held
-to bit-for-bit equality with the launch-per-stage path it replaces (the
-reference's loop, src/shwwnl.f:233-284, one right-hand side per stage), in
-every compiled form and workgroup size, across calls of different lengths
-and time steps, up to the largest mesh it takes (4096 elements) and just
-past it (where it hands over to the per-stage path)."""
+keep their element blocks and, between stages, wait for their neighbouring
+blocks.  Held to bit-for-bit equality with the launch-per-stage path it
+replaces (the reference's loop, src/shwwnl.f:233-284, one right-hand side
+per stage), against that path in every compiled form and workgroup size,
+across calls of different lengths and time steps, up to the largest mesh it
+takes (4096 elements) and just past it (where it hands over to the
+per-stage path)."""
 import sys
 
 import numpy as np
