@@ -588,18 +588,19 @@ __device__ __forceinline__ void traces_uv_res(const F32Args &A, const double *O,
 }
 
 // Stage hand-off between neighbouring blocks (sync: [1] the abort word,
-// [PERSIST_LINE (1 + b)] the number of stages block b has finished).  After a stage a wave
-// makes its stores visible to the device (release fence) and sets its
-// block's count; before the next it waits until every block that holds a
-// face neighbour of its elements has finished as many stages: their traces
-// of the new stage input are then written, and they are done reading the
-// traces this stage overwrites (the traces are double-buffered, alternating
-// per stage, and the neighbour relation is symmetric), so no grid-wide
-// barrier is needed.  Then it takes the device's stores (acquire fence).
-// Every block finishes stage k - 1 waiting only for counts of k - 1, so the
-// waits end; they are also bounded: a wave that gives up (or sees another's
-// abort) raises the abort word and returns false, so every wave leaves the
-// launch and the host reports it.
+// [PERSIST_LINE (1 + b)] the number of stages block b has finished, each
+// count on a cache line of its own).  After a stage a wave makes its stores
+// visible to the device (release fence) and sets its block's count; before
+// the next it waits until every block that holds a face neighbour of its
+// elements has finished as many stages: their traces of the new stage input
+// are then written, and they are done reading the traces this stage
+// overwrites (the traces are double-buffered, alternating per stage, and the
+// neighbour relation is symmetric), so no grid-wide barrier is needed.  Then
+// it takes the device's stores (acquire fence).  Every block finishes stage
+// k - 1 waiting only for counts of k - 1, so the waits end; they are also
+// bounded: a wave that gives up (or sees another's abort) raises the abort
+// word and returns false, so every wave leaves the launch and the host
+// reports it.
 __device__ __forceinline__ void stage_done(unsigned *sync, int eb, unsigned k) {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   if (lane() == 0)
