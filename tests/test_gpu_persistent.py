@@ -130,3 +130,43 @@ def test_persistent_two_contexts_at_once():
         same(ctx.download(), ref)
     for ctx in ctxs:
         ctx.close()
+
+
+def test_persistent_ensemble_union():
+    """Members of uneven size (40 to 1,200 triangles) as one union mesh: the
+    persistent launch advances them all, each member equal bit for bit to
+    the per-stage launches of the union and to its own run alone."""
+    t, ctx = ctx_for(3, 12, 4)
+    sizes = [(6, 5), (30, 22), (14, 9), (25, 25)]
+    mem = [configs.random_case(t, nx, ny, seed=80 + i, scale=0.01)
+           for i, (nx, ny) in enumerate(sizes)]
+    say("persistent ensemble", [c["mesh"]["ne"] for c in mem])
+    alone = []
+    for c in mem:
+        ctx.set_mesh(c["mesh"])
+        set_fields(ctx, c["phys"])
+        ctx.upload(*c["state"])
+        ctx.rk4("lin", 2.0, 40, precision="fp32")
+        alone.append(ctx.download())
+    ctx.set_ensemble_mesh([c["mesh"] for c in mem])
+    P = [c["phys"] for c in mem]
+    cat = spoc_amd.Context.concat
+    ctx.set_fields(cat([p["hb"] for p in P]), P[0]["g0"], cat([p["fc"] for p in P]),
+                   cat([p["w0"] for p in P]), cat([p["z0"] for p in P]), P[0]["mu0"],
+                   P[0]["mb0"])
+    state = tuple(cat([c["state"][f] for c in mem]) for f in range(3))
+    out = []
+    for persist in (False, True):
+        ctx.set_persistent(persist)
+        ctx.upload(*state)
+        ctx.rk4("lin", 2.0, 40, precision="fp32")
+        out.append(ctx.download())
+    ctx.set_persistent(True)
+    same(out[0], out[1])
+    e0 = 0
+    for m, c in enumerate(mem):
+        ne = c["mesh"]["ne"]
+        for f in range(3):
+            a, b = out[1][f][:t["nm"], e0:e0 + ne], alone[m][f][:t["nm"]]
+            assert np.array_equal(a, b), (m, f)
+        e0 += ne
