@@ -65,9 +65,10 @@ enum {
                           wall face that is not its edge 1 (src/cmal_s.f:222-277),
                           or inconsistent connectivity */
   SPOC_EFILE = -6,      /* file missing, unreadable or not in the expected format */
-  SPOC_ENAN = -7        /* the state is no longer finite (the driver's testnan,
+  SPOC_ENAN = -7,       /* the state is no longer finite (the driver's testnan,
                            src/shwwnl.f:426-445, prints and STOPs); the results
                            are still written and can be downloaded */
+  SPOC_ENOMEM = -8      /* host memory, or a host thread, could not be had */
 };
 
 typedef struct spoc_ctx spoc_ctx;

@@ -40,6 +40,7 @@ class SpocError(RuntimeError):
 
 
 ENAN = -7  # SPOC_ENAN: the state is no longer finite
+ENOMEM = -8  # SPOC_ENOMEM: host memory or a host thread could not be had
 
 
 def load_library(path: str = LIB_PATH) -> C.CDLL:

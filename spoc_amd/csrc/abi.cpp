@@ -371,199 +371,205 @@ int spoc_tables_build(int nc, int ng, int ngf, const double *wi, const double *x
                       const double *yi, const double *wif, const double *tif, int ld_g,
                       int ld_f, int ld_m, double *pi2, double *pico, double *amm, double *amml,
                       double *agx, double *agy, double *ammo) {
-  (void)wif;
-  int nm = nm_of(nc);
-  if (nc < 1 || nc > 8 || ng < 1 || ngf < 1 || ld_g < ng || ld_f < ngf || ld_m < nm)
-    return fail(SPOC_EINVAL, "spoc_tables_build: bad sizes nc=%d ng=%d ngf=%d", nc, ng, ngf);
-  // basis phi_m = P_a(xi) P_b(eta), m = ip(nc, a) + b (src/nonlin.f:19-28)
-  int m = 0;
-  for (int a = 0; a <= nc; a++)
-    for (int b = 0; b <= nc - a; b++, m++) {
-      for (int q = 0; q < ng; q++) pi2[q + (size_t)ld_g * m] = legendre(a, xi[q]) * legendre(b, yi[q]);
-      for (int j = 0; j < ngf; j++) {
-        double t = tif[j];
-        pico[j + (size_t)ld_f * (m + (size_t)ld_m * 0)] = legendre(a, t) * legendre(b, -1.0);
-        pico[j + (size_t)ld_f * (m + (size_t)ld_m * 1)] = legendre(a, -t) * legendre(b, t);
-        pico[j + (size_t)ld_f * (m + (size_t)ld_m * 2)] = legendre(a, -1.0) * legendre(b, -t);
-      }
-    }
-  // mass and derivative matrices by quadrature (src/mat1.f:90-142)
-  std::vector<double> M0((size_t)ld_m * ld_m, 0.0);
-  for (double *mat : {M0.data(), agx, agy})
-    for (int i = 0; i < ld_m * ld_m; i++) mat[i] = 0.0;
-  int i1 = 0;
-  for (int k1 = 0; k1 <= nc; k1++)
-    for (int k2 = 0; k2 <= nc - k1; k2++, i1++) {
-      int i2 = 0;
-      for (int k3 = 0; k3 <= nc; k3++)
-        for (int k4 = 0; k4 <= nc - k3; k4++, i2++) {
-          double s0 = 0.0, sx = 0.0, sy = 0.0;
-          for (int q = 0; q < ng; q++) {
-            double x = xi[q], y = yi[q];
-            s0 = s0 + wi[q] * (legendre(k1, x) * legendre(k2, y) * legendre(k3, x) * legendre(k4, y));
-            sx = sx + wi[q] * (legendre(k1, x) * legendre(k2, y) * dlegendre(k3, x) * legendre(k4, y));
-            sy = sy + wi[q] * (legendre(k1, x) * legendre(k2, y) * legendre(k3, x) * dlegendre(k4, y));
-          }
-          M0[i1 + (size_t)ld_m * i2] = s0;
-          agx[i1 + (size_t)ld_m * i2] = sx;
-          agy[i1 + (size_t)ld_m * i2] = sy;
+  return spoc::guarded([&]() -> int {
+    (void)wif;
+    int nm = nm_of(nc);
+    if (nc < 1 || nc > 8 || ng < 1 || ngf < 1 || ld_g < ng || ld_f < ngf || ld_m < nm)
+      return fail(SPOC_EINVAL, "spoc_tables_build: bad sizes nc=%d ng=%d ngf=%d", nc, ng, ngf);
+    // basis phi_m = P_a(xi) P_b(eta), m = ip(nc, a) + b (src/nonlin.f:19-28)
+    int m = 0;
+    for (int a = 0; a <= nc; a++)
+      for (int b = 0; b <= nc - a; b++, m++) {
+        for (int q = 0; q < ng; q++) pi2[q + (size_t)ld_g * m] = legendre(a, xi[q]) * legendre(b, yi[q]);
+        for (int j = 0; j < ngf; j++) {
+          double t = tif[j];
+          pico[j + (size_t)ld_f * (m + (size_t)ld_m * 0)] = legendre(a, t) * legendre(b, -1.0);
+          pico[j + (size_t)ld_f * (m + (size_t)ld_m * 1)] = legendre(a, -t) * legendre(b, t);
+          pico[j + (size_t)ld_f * (m + (size_t)ld_m * 2)] = legendre(a, -1.0) * legendre(b, -t);
         }
-    }
-  // reduced-basis mass matrix (src/mat1.f:285-368)
-  for (int i = 0; i < ld_m * ld_m; i++) amml[i] = 0.0;
-  const int js[4] = {0, 1, 0, 1}, ls[4] = {0, 0, 1, 1};
-  for (int pass = 0; pass < 4; pass++)
-    for (int i = 0; i <= nc - 1; i++)
-      for (int j = js[pass]; j <= nc - 1 - i; j += 2)
-        for (int k = 0; k <= nc - 1; k++)
-          for (int l = ls[pass]; l <= nc - 1 - k; l += 2) {
-            int r = ip_of(nc - 1, i) + j, c = ip_of(nc - 1, k) + l;
-            int ri = ip_of(nc, i) + j + 1, ci = ip_of(nc, k) + l + 1;
-            int r0 = ip_of(nc, i), c0 = ip_of(nc, k);
-            double a22 = M0[ri + (size_t)ld_m * ci], a02 = M0[r0 + (size_t)ld_m * ci];
-            double a20 = M0[ri + (size_t)ld_m * c0], a00 = M0[r0 + (size_t)ld_m * c0];
-            double v = pass == 0   ? a22 + a02 + a20 + a00
-                       : pass == 1 ? a22 - a02 + a20 - a00
-                       : pass == 2 ? a22 + a02 - a20 - a00
-                                   : a22 - a02 - a20 + a00;
-            amml[r + (size_t)ld_m * c] = v;
+      }
+    // mass and derivative matrices by quadrature (src/mat1.f:90-142)
+    std::vector<double> M0((size_t)ld_m * ld_m, 0.0);
+    for (double *mat : {M0.data(), agx, agy})
+      for (int i = 0; i < ld_m * ld_m; i++) mat[i] = 0.0;
+    int i1 = 0;
+    for (int k1 = 0; k1 <= nc; k1++)
+      for (int k2 = 0; k2 <= nc - k1; k2++, i1++) {
+        int i2 = 0;
+        for (int k3 = 0; k3 <= nc; k3++)
+          for (int k4 = 0; k4 <= nc - k3; k4++, i2++) {
+            double s0 = 0.0, sx = 0.0, sy = 0.0;
+            for (int q = 0; q < ng; q++) {
+              double x = xi[q], y = yi[q];
+              s0 = s0 + wi[q] * (legendre(k1, x) * legendre(k2, y) * legendre(k3, x) * legendre(k4, y));
+              sx = sx + wi[q] * (legendre(k1, x) * legendre(k2, y) * dlegendre(k3, x) * legendre(k4, y));
+              sy = sy + wi[q] * (legendre(k1, x) * legendre(k2, y) * legendre(k3, x) * dlegendre(k4, y));
+            }
+            M0[i1 + (size_t)ld_m * i2] = s0;
+            agx[i1 + (size_t)ld_m * i2] = sx;
+            agy[i1 + (size_t)ld_m * i2] = sy;
           }
-  for (int i = 0; i < ld_m * ld_m; i++) {
-    ammo[i] = M0[i];
-    amm[i] = M0[i];
-  }
-  int info1 = cholesky_upper(nm, amm, ld_m);
-  int info2 = cholesky_upper(nm - nc - 1, amml, ld_m);
-  if (info1 || info2)
-    return fail(SPOC_ESINGULAR,
-                "mass matrix not positive definite (p=%d with a %d-point rule: need >= %d points)",
-                nc, ng, nm);
-  return SPOC_OK;
+      }
+    // reduced-basis mass matrix (src/mat1.f:285-368)
+    for (int i = 0; i < ld_m * ld_m; i++) amml[i] = 0.0;
+    const int js[4] = {0, 1, 0, 1}, ls[4] = {0, 0, 1, 1};
+    for (int pass = 0; pass < 4; pass++)
+      for (int i = 0; i <= nc - 1; i++)
+        for (int j = js[pass]; j <= nc - 1 - i; j += 2)
+          for (int k = 0; k <= nc - 1; k++)
+            for (int l = ls[pass]; l <= nc - 1 - k; l += 2) {
+              int r = ip_of(nc - 1, i) + j, c = ip_of(nc - 1, k) + l;
+              int ri = ip_of(nc, i) + j + 1, ci = ip_of(nc, k) + l + 1;
+              int r0 = ip_of(nc, i), c0 = ip_of(nc, k);
+              double a22 = M0[ri + (size_t)ld_m * ci], a02 = M0[r0 + (size_t)ld_m * ci];
+              double a20 = M0[ri + (size_t)ld_m * c0], a00 = M0[r0 + (size_t)ld_m * c0];
+              double v = pass == 0   ? a22 + a02 + a20 + a00
+                         : pass == 1 ? a22 - a02 + a20 - a00
+                         : pass == 2 ? a22 + a02 - a20 - a00
+                                     : a22 - a02 - a20 + a00;
+              amml[r + (size_t)ld_m * c] = v;
+            }
+    for (int i = 0; i < ld_m * ld_m; i++) {
+      ammo[i] = M0[i];
+      amm[i] = M0[i];
+    }
+    int info1 = cholesky_upper(nm, amm, ld_m);
+    int info2 = cholesky_upper(nm - nc - 1, amml, ld_m);
+    if (info1 || info2)
+      return fail(SPOC_ESINGULAR,
+                  "mass matrix not positive definite (p=%d with a %d-point rule: need >= %d points)",
+                  nc, ng, nm);
+    return SPOC_OK;
+  });
 }
 
 int spoc_ctx_create(spoc_ctx **out, int device, int nc, int ng, int ngf, const double *wi,
                     const double *wif, const double *pi2, int ld_g, const double *pico,
                     int ld_f, int ld_m, const double *amm, const double *amml,
                     const double *agx, const double *agy, const double *ammo) {
-  if (!out) return fail(SPOC_EINVAL, "spoc_ctx_create: null ctx pointer");
-  *out = nullptr;
-  int nm = nm_of(nc);
-  bool ok = false;
+  return spoc::guarded([&]() -> int {
+    if (!out) return fail(SPOC_EINVAL, "spoc_ctx_create: null ctx pointer");
+    *out = nullptr;
+    int nm = nm_of(nc);
+    bool ok = false;
 #define SPOC_OKSHAPE(NC_, NG_, NGF_) ok = ok || (nc == NC_ && ng == NG_ && ngf == NGF_);
-  SPOC_FOR_EACH_SHAPE(SPOC_OKSHAPE)
+    SPOC_FOR_EACH_SHAPE(SPOC_OKSHAPE)
 #undef SPOC_OKSHAPE
-  if (!ok)
-    return fail(SPOC_EINVAL,
-                "unsupported (p, triangle points, edge points) = (%d, %d, %d); supported: "
-                "(2,6,3) (3,12,4) (4,25,5) (5,25,6) (6,33,7) (7,42,8)",
-                nc, ng, ngf);
-  if (ld_g < ng || ld_f < ngf || ld_m < nm) return fail(SPOC_EINVAL, "leading dimension too small");
-  for (int i = 0; i < nm; i++)
-    if (!(amm[i + (size_t)ld_m * i] > 0.0))
-      return fail(SPOC_ESINGULAR, "amm factor has a non-positive diagonal (DPOTRF failed?)");
-  HIPCHK(hipSetDevice(device));
-  spoc_ctx *c = new spoc_ctx();
-  c->device = device;
-  c->nc = nc;
-  c->ng = ng;
-  c->ngf = ngf;
-  c->nm = nm;
-  int nl = nm - nc - 1;
-  // host-side layout conversion
-  std::vector<double> agxT(nm * nm), agyT(nm * nm), p2(ng * nm), p2T(nm * ng),
-      pc(ngf * nm * 3), pcT(nm * ngf * 3), U(nm * nm, 0.0), UL(nm * nm, 0.0), d1(nm), d2(nm, 0.0),
-      w(ng), wf(ngf), a1(nm);
-  for (int j = 0; j < nm; j++)
-    for (int i = 0; i < nm; i++) {
-      agxT[i + nm * j] = agx[j + (size_t)ld_m * i];
-      agyT[i + nm * j] = agy[j + (size_t)ld_m * i];
-    }
-  for (int m = 0; m < nm; m++)
-    for (int q = 0; q < ng; q++) {
-      p2[q + ng * m] = pi2[q + (size_t)ld_g * m];
-      p2T[m + nm * q] = pi2[q + (size_t)ld_g * m];
-    }
-  for (int k = 0; k < 3; k++)
-    for (int m = 0; m < nm; m++)
-      for (int j = 0; j < ngf; j++) {
-        double v = pico[j + (size_t)ld_f * (m + (size_t)ld_m * k)];
-        pc[j + ngf * (m + nm * k)] = v;
-        pcT[m + nm * (j + ngf * k)] = v;
+    if (!ok)
+      return fail(SPOC_EINVAL,
+                  "unsupported (p, triangle points, edge points) = (%d, %d, %d); supported: "
+                  "(2,6,3) (3,12,4) (4,25,5) (5,25,6) (6,33,7) (7,42,8)",
+                  nc, ng, ngf);
+    if (ld_g < ng || ld_f < ngf || ld_m < nm) return fail(SPOC_EINVAL, "leading dimension too small");
+    for (int i = 0; i < nm; i++)
+      if (!(amm[i + (size_t)ld_m * i] > 0.0))
+        return fail(SPOC_ESINGULAR, "amm factor has a non-positive diagonal (DPOTRF failed?)");
+    HIPCHK(hipSetDevice(device));
+    spoc_ctx *c = new spoc_ctx();
+    c->device = device;
+    c->nc = nc;
+    c->ng = ng;
+    c->ngf = ngf;
+    c->nm = nm;
+    int nl = nm - nc - 1;
+    // host-side layout conversion
+    std::vector<double> agxT(nm * nm), agyT(nm * nm), p2(ng * nm), p2T(nm * ng),
+        pc(ngf * nm * 3), pcT(nm * ngf * 3), U(nm * nm, 0.0), UL(nm * nm, 0.0), d1(nm), d2(nm, 0.0),
+        w(ng), wf(ngf), a1(nm);
+    for (int j = 0; j < nm; j++)
+      for (int i = 0; i < nm; i++) {
+        agxT[i + nm * j] = agx[j + (size_t)ld_m * i];
+        agyT[i + nm * j] = agy[j + (size_t)ld_m * i];
       }
-  for (int i = 0; i < nm; i++)
-    for (int k = 0; k <= i; k++) U[k + nm * i] = amm[k + (size_t)ld_m * i];
-  for (int i = 0; i < nl; i++)
-    for (int k = 0; k <= i; k++) UL[k + nm * i] = amml[k + (size_t)ld_m * i];
-  for (int i = 0; i < nm; i++) d1[i] = 1.0 / amm[i + (size_t)ld_m * i];
-  for (int i = 0; i < nl; i++) d2[i] = 1.0 / amml[i + (size_t)ld_m * i];
-  for (int q = 0; q < ng; q++) w[q] = wi[q];
-  for (int j = 0; j < ngf; j++) wf[j] = wif[j];
-  for (int i = 0; i < nm; i++) a1[i] = ammo[i];
-  c->h_wi = w;
-  c->h_ammo1 = a1;
-  std::vector<std::vector<double> *> parts = {&agxT, &agyT, &p2, &p2T, &pc, &pcT, &U,
-                                              &UL,   &d1,   &d2, &w,   &wf,  &a1};
-  size_t tot = 0;
-  for (auto *p : parts) tot += (p->size() + 31) / 32 * 32;
-  hipError_t he = hipStreamCreate(&c->stream);
-  if (he == hipSuccess) he = hipMalloc(&c->dtab, tot * sizeof(double));
-  if (he != hipSuccess) {
-    delete c;
-    return fail(SPOC_EHIP, "device allocation failed: %s", hipGetErrorString(he));
-  }
-  std::vector<double> host(tot, 0.0);
-  std::vector<const double *> dptr;
-  size_t off = 0;
-  for (auto *p : parts) {
-    std::memcpy(host.data() + off, p->data(), p->size() * sizeof(double));
-    dptr.push_back(c->dtab + off);
-    off += (p->size() + 31) / 32 * 32;
-  }
-  he = hipMemcpy(c->dtab, host.data(), tot * sizeof(double), hipMemcpyHostToDevice);
-  if (he != hipSuccess) {
-    (void)hipFree(c->dtab);
-    delete c;
-    return fail(SPOC_EHIP, "table upload failed: %s", hipGetErrorString(he));
-  }
-  c->tab = DevTables{dptr[0], dptr[1], dptr[2], dptr[3], dptr[4],  dptr[5], dptr[6],
-                     dptr[7], dptr[8], dptr[9], dptr[10], dptr[11], dptr[12]};
-  (void)hipEventCreate(&c->t0);
-  (void)hipEventCreate(&c->t1);
-  if (hipMalloc((void **)&c->dflag, 256) != hipSuccess) {
-    spoc_ctx_destroy(c);
-    return fail(SPOC_EHIP, "device allocation failed");
-  }
-  c->h_pi2.assign((size_t)ng * nm, 0.0);
-  for (int m = 0; m < nm; m++)
-    for (int q = 0; q < ng; q++) c->h_pi2[q + (size_t)ng * m] = pi2[q + (size_t)ld_g * m];
-  if (f32_plan_create(&c->f32, nc, ng, ngf, amm, amml, agx, agy, pi2, pico, wi, wif, ld_g, ld_f,
-                      ld_m, c->tab.U) != 0) {
-    spoc_ctx_destroy(c);
-    return fail(SPOC_EHIP, "FP32 operator upload failed");
-  }
-  f64_plan_create(&c->f64, nc, ng, ngf);
-  *out = c;
-  return SPOC_OK;
+    for (int m = 0; m < nm; m++)
+      for (int q = 0; q < ng; q++) {
+        p2[q + ng * m] = pi2[q + (size_t)ld_g * m];
+        p2T[m + nm * q] = pi2[q + (size_t)ld_g * m];
+      }
+    for (int k = 0; k < 3; k++)
+      for (int m = 0; m < nm; m++)
+        for (int j = 0; j < ngf; j++) {
+          double v = pico[j + (size_t)ld_f * (m + (size_t)ld_m * k)];
+          pc[j + ngf * (m + nm * k)] = v;
+          pcT[m + nm * (j + ngf * k)] = v;
+        }
+    for (int i = 0; i < nm; i++)
+      for (int k = 0; k <= i; k++) U[k + nm * i] = amm[k + (size_t)ld_m * i];
+    for (int i = 0; i < nl; i++)
+      for (int k = 0; k <= i; k++) UL[k + nm * i] = amml[k + (size_t)ld_m * i];
+    for (int i = 0; i < nm; i++) d1[i] = 1.0 / amm[i + (size_t)ld_m * i];
+    for (int i = 0; i < nl; i++) d2[i] = 1.0 / amml[i + (size_t)ld_m * i];
+    for (int q = 0; q < ng; q++) w[q] = wi[q];
+    for (int j = 0; j < ngf; j++) wf[j] = wif[j];
+    for (int i = 0; i < nm; i++) a1[i] = ammo[i];
+    c->h_wi = w;
+    c->h_ammo1 = a1;
+    std::vector<std::vector<double> *> parts = {&agxT, &agyT, &p2, &p2T, &pc, &pcT, &U,
+                                                &UL,   &d1,   &d2, &w,   &wf,  &a1};
+    size_t tot = 0;
+    for (auto *p : parts) tot += (p->size() + 31) / 32 * 32;
+    hipError_t he = hipStreamCreate(&c->stream);
+    if (he == hipSuccess) he = hipMalloc(&c->dtab, tot * sizeof(double));
+    if (he != hipSuccess) {
+      delete c;
+      return fail(SPOC_EHIP, "device allocation failed: %s", hipGetErrorString(he));
+    }
+    std::vector<double> host(tot, 0.0);
+    std::vector<const double *> dptr;
+    size_t off = 0;
+    for (auto *p : parts) {
+      std::memcpy(host.data() + off, p->data(), p->size() * sizeof(double));
+      dptr.push_back(c->dtab + off);
+      off += (p->size() + 31) / 32 * 32;
+    }
+    he = hipMemcpy(c->dtab, host.data(), tot * sizeof(double), hipMemcpyHostToDevice);
+    if (he != hipSuccess) {
+      (void)hipFree(c->dtab);
+      delete c;
+      return fail(SPOC_EHIP, "table upload failed: %s", hipGetErrorString(he));
+    }
+    c->tab = DevTables{dptr[0], dptr[1], dptr[2], dptr[3], dptr[4],  dptr[5], dptr[6],
+                       dptr[7], dptr[8], dptr[9], dptr[10], dptr[11], dptr[12]};
+    (void)hipEventCreate(&c->t0);
+    (void)hipEventCreate(&c->t1);
+    if (hipMalloc((void **)&c->dflag, 256) != hipSuccess) {
+      spoc_ctx_destroy(c);
+      return fail(SPOC_EHIP, "device allocation failed");
+    }
+    c->h_pi2.assign((size_t)ng * nm, 0.0);
+    for (int m = 0; m < nm; m++)
+      for (int q = 0; q < ng; q++) c->h_pi2[q + (size_t)ng * m] = pi2[q + (size_t)ld_g * m];
+    if (f32_plan_create(&c->f32, nc, ng, ngf, amm, amml, agx, agy, pi2, pico, wi, wif, ld_g, ld_f,
+                        ld_m, c->tab.U) != 0) {
+      spoc_ctx_destroy(c);
+      return fail(SPOC_EHIP, "FP32 operator upload failed");
+    }
+    f64_plan_create(&c->f64, nc, ng, ngf);
+    *out = c;
+    return SPOC_OK;
+  });
 }
 
 int spoc_ctx_destroy(spoc_ctx *c) {
-  if (!c) return SPOC_OK;
-  (void)hipSetDevice(c->device);
-  (void)hipStreamSynchronize(c->stream);
-  free_mesh(c);
-  f32_plan_destroy(c->f32);
-  f64_plan_destroy(c->f64);
-  if (c->dtab) (void)hipFree(c->dtab);
-  if (c->dflag) (void)hipFree(c->dflag);
-  if (c->stage) (void)hipHostFree(c->stage);
-  for (auto e : c->fev)
-    if (e) (void)hipEventDestroy(e);
-  for (auto e : c->prof.ev) (void)hipEventDestroy(e);
-  if (c->t0) (void)hipEventDestroy(c->t0);
-  if (c->t1) (void)hipEventDestroy(c->t1);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
-  delete c;
-  return SPOC_OK;
+  return spoc::guarded([&]() -> int {
+    if (!c) return SPOC_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_mesh(c);
+    f32_plan_destroy(c->f32);
+    f64_plan_destroy(c->f64);
+    if (c->dtab) (void)hipFree(c->dtab);
+    if (c->dflag) (void)hipFree(c->dflag);
+    if (c->stage) (void)hipHostFree(c->stage);
+    for (auto e : c->fev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto e : c->prof.ev) (void)hipEventDestroy(e);
+    if (c->t0) (void)hipEventDestroy(c->t0);
+    if (c->t1) (void)hipEventDestroy(c->t1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SPOC_OK;
+  });
 }
 
 static int mesh_set(spoc_ctx *c, int ne, int nf, const int *itm, const int *its,
@@ -752,8 +758,10 @@ static int mesh_set(spoc_ctx *c, int ne, int nf, const int *itm, const int *its,
 int spoc_mesh_set(spoc_ctx *c, int ne, int nf, const int *itm, const int *its, const int *tseg,
                   const int *ssit, const double *norm, const double *vect, const double *normt,
                   const int *bdl, int nn, const int *in, const double *xgr, const double *ygr) {
-  return mesh_set(c, ne, nf, itm, its, tseg, ssit, norm, vect, normt, bdl, nn, in, xgr, ygr,
-                  {0, ne});
+  return spoc::guarded([&]() -> int {
+    return mesh_set(c, ne, nf, itm, its, tseg, ssit, norm, vect, normt, bdl, nn, in, xgr, ygr,
+                    {0, ne});
+  });
 }
 
 int spoc_ensemble_mesh_set(spoc_ctx *c, int nmem, const int *ne_m, const int *nf_m,
@@ -761,38 +769,40 @@ int spoc_ensemble_mesh_set(spoc_ctx *c, int nmem, const int *ne_m, const int *nf
                            const int *ssit, const double *norm, const double *vect,
                            const double *normt, const int *bdl, const int *in,
                            const double *xgr, const double *ygr) {
-  if (!c) return fail(SPOC_EINVAL, "null context");
-  if (nmem < 1 || !ne_m || !nf_m) return fail(SPOC_EINVAL, "spoc_ensemble_mesh_set: nmem=%d", nmem);
-  const bool nodes = nn_m && in && xgr && ygr;
-  long ne = 0, nf = 0, nn = 0;
-  std::vector<int> e0(nmem + 1, 0);
-  for (int m = 0; m < nmem; m++) {
-    if (ne_m[m] <= 0 || nf_m[m] <= 0 || (nodes && nn_m[m] <= 0))
-      return fail(SPOC_EINVAL, "member %d: ne=%d nf=%d", m, ne_m[m], nf_m[m]);
-    ne += ne_m[m];
-    nf += nf_m[m];
-    if (nodes) nn += nn_m[m];
-    e0[m + 1] = (int)ne;
-    if (ne > (1L << 30) || nf > (1L << 30)) return fail(SPOC_EINVAL, "ensemble too large");
-  }
-  // union mesh: member m's 1-based element, face and node numbers shifted by
-  // the totals of the members before it (0, a wall, stays 0)
-  std::vector<int> uitm(3 * ne), utseg(2 * nf), uin(nodes ? 3 * ne : 0);
-  long eo = 0, fo = 0, no = 0;
-  for (int m = 0; m < nmem; m++) {
-    for (long i = 3 * eo; i < 3 * (eo + ne_m[m]); i++) {
-      uitm[i] = itm[i] + (int)fo;
-      if (nodes) uin[i] = in[i] + (int)no;
+  return spoc::guarded([&]() -> int {
+    if (!c) return fail(SPOC_EINVAL, "null context");
+    if (nmem < 1 || !ne_m || !nf_m) return fail(SPOC_EINVAL, "spoc_ensemble_mesh_set: nmem=%d", nmem);
+    const bool nodes = nn_m && in && xgr && ygr;
+    long ne = 0, nf = 0, nn = 0;
+    std::vector<int> e0(nmem + 1, 0);
+    for (int m = 0; m < nmem; m++) {
+      if (ne_m[m] <= 0 || nf_m[m] <= 0 || (nodes && nn_m[m] <= 0))
+        return fail(SPOC_EINVAL, "member %d: ne=%d nf=%d", m, ne_m[m], nf_m[m]);
+      ne += ne_m[m];
+      nf += nf_m[m];
+      if (nodes) nn += nn_m[m];
+      e0[m + 1] = (int)ne;
+      if (ne > (1L << 30) || nf > (1L << 30)) return fail(SPOC_EINVAL, "ensemble too large");
     }
-    for (long i = 2 * fo; i < 2 * (fo + nf_m[m]); i++)
-      utseg[i] = tseg[i] == 0 ? 0 : tseg[i] + (int)eo;
-    eo += ne_m[m];
-    fo += nf_m[m];
-    if (nodes) no += nn_m[m];
-  }
-  return mesh_set(c, (int)ne, (int)nf, uitm.data(), its, utseg.data(), ssit, norm, vect, normt,
-                  bdl, nodes ? (int)nn : 0, nodes ? uin.data() : nullptr, nodes ? xgr : nullptr,
-                  nodes ? ygr : nullptr, e0);
+    // union mesh: member m's 1-based element, face and node numbers shifted by
+    // the totals of the members before it (0, a wall, stays 0)
+    std::vector<int> uitm(3 * ne), utseg(2 * nf), uin(nodes ? 3 * ne : 0);
+    long eo = 0, fo = 0, no = 0;
+    for (int m = 0; m < nmem; m++) {
+      for (long i = 3 * eo; i < 3 * (eo + ne_m[m]); i++) {
+        uitm[i] = itm[i] + (int)fo;
+        if (nodes) uin[i] = in[i] + (int)no;
+      }
+      for (long i = 2 * fo; i < 2 * (fo + nf_m[m]); i++)
+        utseg[i] = tseg[i] == 0 ? 0 : tseg[i] + (int)eo;
+      eo += ne_m[m];
+      fo += nf_m[m];
+      if (nodes) no += nn_m[m];
+    }
+    return mesh_set(c, (int)ne, (int)nf, uitm.data(), its, utseg.data(), ssit, norm, vect, normt,
+                    bdl, nodes ? (int)nn : 0, nodes ? uin.data() : nullptr, nodes ? xgr : nullptr,
+                    nodes ? ygr : nullptr, e0);
+  });
 }
 
 // The caller's element of internal element e.
@@ -907,69 +917,75 @@ static int download_fields(spoc_ctx *c, int ld, double *const *dst, double *cons
 
 int spoc_fields_set(spoc_ctx *c, int ld, const double *fc, const double *w0, const double *z0,
                     const double *hb, double mu0, double mb0, double g0) {
-  if (!c || !c->dmesh) return fail(SPOC_ESTATE, "spoc_fields_set: set the mesh first");
-  if (ld < c->nm || !hb) return fail(SPOC_EINVAL, "spoc_fields_set: ld=%d, hb=%p", ld, hb);
-  (void)hipSetDevice(c->device);
-  drop_graph(c);
-  size_t n = (size_t)c->nm * c->ne;
-  double *dst[4] = {c->dfields, c->dfields + n, c->dfields + 2 * n, c->dfields + 3 * n};
-  const double *src[4] = {fc, w0, z0, hb};
-  HIPCHK(hipMemset(c->dfields, 0, sizeof(double) * 3 * n));
-  int rc = upload_fields(c, ld, src, dst, 4);
-  if (rc) return rc;
-  // hh0 = hb(1,1) of each element's own mesh (src/calvar_lin.f:65)
-  std::vector<double> hh0e(c->ne);
-  for (size_t m = 0; m + 1 < c->mem_e0.size(); m++)
-    for (int e = c->mem_e0[m]; e < c->mem_e0[m + 1]; e++)
-      hh0e[e] = hb[(size_t)ld * c->mem_e0[m]];
-  HIPCHK(hipMemcpy(c->dfields + 4 * n, hh0e.data(), sizeof(double) * c->ne, hipMemcpyHostToDevice));
-  c->phys = Physics{mu0, mb0, g0, hb[0], c->dfields + 4 * n};
-  // the mixed path's set-up reads host arrays element by element: hand it
-  // the fields in the internal numbering
-  std::vector<double> internal;
-  const double *f4[4] = {fc, w0, z0, hb};
-  int ld4 = ld;
-  if (!c->perm.empty()) {
-    internal.assign(4 * n, 0.0);
-    for (int f = 0; f < 4; f++) {
-      if (!f4[f]) continue;
-      double *d = internal.data() + f * n;
-      for (int e = 0; e < c->ne; e++)
-        std::memcpy(d + (size_t)c->nm * e, f4[f] + (size_t)ld * caller_elem(c, e),
-                    sizeof(double) * c->nm);
-      f4[f] = d;
+  return spoc::guarded([&]() -> int {
+    if (!c || !c->dmesh) return fail(SPOC_ESTATE, "spoc_fields_set: set the mesh first");
+    if (ld < c->nm || !hb) return fail(SPOC_EINVAL, "spoc_fields_set: ld=%d, hb=%p", ld, hb);
+    (void)hipSetDevice(c->device);
+    drop_graph(c);
+    size_t n = (size_t)c->nm * c->ne;
+    double *dst[4] = {c->dfields, c->dfields + n, c->dfields + 2 * n, c->dfields + 3 * n};
+    const double *src[4] = {fc, w0, z0, hb};
+    HIPCHK(hipMemset(c->dfields, 0, sizeof(double) * 3 * n));
+    int rc = upload_fields(c, ld, src, dst, 4);
+    if (rc) return rc;
+    // hh0 = hb(1,1) of each element's own mesh (src/calvar_lin.f:65)
+    std::vector<double> hh0e(c->ne);
+    for (size_t m = 0; m + 1 < c->mem_e0.size(); m++)
+      for (int e = c->mem_e0[m]; e < c->mem_e0[m + 1]; e++)
+        hh0e[e] = hb[(size_t)ld * c->mem_e0[m]];
+    HIPCHK(hipMemcpy(c->dfields + 4 * n, hh0e.data(), sizeof(double) * c->ne, hipMemcpyHostToDevice));
+    c->phys = Physics{mu0, mb0, g0, hb[0], c->dfields + 4 * n};
+    // the mixed path's set-up reads host arrays element by element: hand it
+    // the fields in the internal numbering
+    std::vector<double> internal;
+    const double *f4[4] = {fc, w0, z0, hb};
+    int ld4 = ld;
+    if (!c->perm.empty()) {
+      internal.assign(4 * n, 0.0);
+      for (int f = 0; f < 4; f++) {
+        if (!f4[f]) continue;
+        double *d = internal.data() + f * n;
+        for (int e = 0; e < c->ne; e++)
+          std::memcpy(d + (size_t)c->nm * e, f4[f] + (size_t)ld * caller_elem(c, e),
+                      sizeof(double) * c->nm);
+        f4[f] = d;
+      }
+      ld4 = c->nm;
     }
-    ld4 = c->nm;
-  }
-  if (f32_fields_set(c->f32, ld4, f4[0], f4[1], f4[2], f4[3], hh0e.data(), c->h_pi2.data(),
-                     c->ng, mu0, mb0, g0))
-    return fail(SPOC_EHIP, "FP32 field upload failed");
-  if (f64_fields_set(c->f64, c->stream, c->tab, c->dfields, c->dfields + n, c->dfields + 2 * n,
-                     c->dfields + 3 * n, c->dfields + 4 * n, mu0, mb0, g0))
-    return fail(SPOC_EHIP, "FP64 field set-up failed");
-  c->have_fields = true;
-  return SPOC_OK;
+    if (f32_fields_set(c->f32, ld4, f4[0], f4[1], f4[2], f4[3], hh0e.data(), c->h_pi2.data(),
+                       c->ng, mu0, mb0, g0))
+      return fail(SPOC_EHIP, "FP32 field upload failed");
+    if (f64_fields_set(c->f64, c->stream, c->tab, c->dfields, c->dfields + n, c->dfields + 2 * n,
+                       c->dfields + 3 * n, c->dfields + 4 * n, mu0, mb0, g0))
+      return fail(SPOC_EHIP, "FP64 field set-up failed");
+    c->have_fields = true;
+    return SPOC_OK;
+  });
 }
 
 int spoc_state_upload(spoc_ctx *c, int ld, const double *u0, const double *v0,
                       const double *h0) {
-  if (!c || !c->dmesh) return fail(SPOC_ESTATE, "spoc_state_upload: set the mesh first");
-  if (ld < c->nm || !u0 || !v0 || !h0) return fail(SPOC_EINVAL, "spoc_state_upload: bad args");
-  (void)hipSetDevice(c->device);
-  const double *src[3] = {u0, v0, h0};
-  double *dst[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
-  int rc = upload_fields(c, ld, src, dst, 3);
-  if (!rc) c->have_state = true;
-  return rc;
+  return spoc::guarded([&]() -> int {
+    if (!c || !c->dmesh) return fail(SPOC_ESTATE, "spoc_state_upload: set the mesh first");
+    if (ld < c->nm || !u0 || !v0 || !h0) return fail(SPOC_EINVAL, "spoc_state_upload: bad args");
+    (void)hipSetDevice(c->device);
+    const double *src[3] = {u0, v0, h0};
+    double *dst[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
+    int rc = upload_fields(c, ld, src, dst, 3);
+    if (!rc) c->have_state = true;
+    return rc;
+  });
 }
 
 int spoc_state_download(spoc_ctx *c, int ld, double *u0, double *v0, double *h0) {
-  if (!c || !c->have_state) return fail(SPOC_ESTATE, "spoc_state_download: no state");
-  if (ld < c->nm || !u0 || !v0 || !h0) return fail(SPOC_EINVAL, "spoc_state_download: bad args");
-  (void)hipSetDevice(c->device);
-  double *dst[3] = {u0, v0, h0};
-  double *src[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
-  return download_fields(c, ld, dst, src, 3);
+  return spoc::guarded([&]() -> int {
+    if (!c || !c->have_state) return fail(SPOC_ESTATE, "spoc_state_download: no state");
+    if (ld < c->nm || !u0 || !v0 || !h0) return fail(SPOC_EINVAL, "spoc_state_download: bad args");
+    (void)hipSetDevice(c->device);
+    double *dst[3] = {u0, v0, h0};
+    double *src[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
+    return download_fields(c, ld, dst, src, 3);
+  });
 }
 
 static int check_ready(spoc_ctx *c, int variant) {
@@ -982,200 +998,204 @@ static int check_ready(spoc_ctx *c, int variant) {
 
 int spoc_rhs(spoc_ctx *c, int variant, int precision, int ld, const double *u0,
              const double *v0, const double *h0, double *u1, double *v1, double *h1) {
-  int rc = check_ready(c, variant);
-  if (rc) return rc;
-  if (ld < c->nm) return fail(SPOC_EINVAL, "ld=%d < nm=%d", ld, c->nm);
-  (void)hipSetDevice(c->device);
-  // stage input goes to the x slots (3..5); tendencies come back from k (9..11)
-  const double *src[3] = {u0, v0, h0};
-  double *xs[3] = {state_ptr(c, 3), state_ptr(c, 4), state_ptr(c, 5)};
-  rc = upload_fields(c, ld, src, xs, 3);
-  if (rc) return rc;
-  const double *x[3] = {xs[0], xs[1], xs[2]};
-  double *ks[3] = {state_ptr(c, 9), state_ptr(c, 10), state_ptr(c, 11)};
-  if (precision != SPOC_FP32_MFMA && f32_flux(c->f32) != 0.0)
-    return fail(SPOC_EINVAL, "the Lax-Friedrichs flux option runs on the FP32-MFMA path only");
-  if (precision == SPOC_FP32_MFMA) {
-    rc = f32_rhs(c->f32, c->stream, variant, x, ks, nullptr, nullptr);
-    if (rc) return f32_fail(rc);
-  } else if (precision == SPOC_FP64) {
-    rc = f64_rhs(c->f64, c->stream, c->tab, variant, x, ks, nullptr, nullptr);
-    if (rc) return fail(SPOC_EHIP, "FP64 kernel launch failed: %s",
-                        hipGetErrorString(hipGetLastError()));
-  } else {
-    return fail(SPOC_EINVAL, "unknown precision %d", precision);
-  }
-  double *dst[3] = {u1, v1, h1};
-  return download_fields(c, ld, dst, ks, 3);
+  return spoc::guarded([&]() -> int {
+    int rc = check_ready(c, variant);
+    if (rc) return rc;
+    if (ld < c->nm) return fail(SPOC_EINVAL, "ld=%d < nm=%d", ld, c->nm);
+    (void)hipSetDevice(c->device);
+    // stage input goes to the x slots (3..5); tendencies come back from k (9..11)
+    const double *src[3] = {u0, v0, h0};
+    double *xs[3] = {state_ptr(c, 3), state_ptr(c, 4), state_ptr(c, 5)};
+    rc = upload_fields(c, ld, src, xs, 3);
+    if (rc) return rc;
+    const double *x[3] = {xs[0], xs[1], xs[2]};
+    double *ks[3] = {state_ptr(c, 9), state_ptr(c, 10), state_ptr(c, 11)};
+    if (precision != SPOC_FP32_MFMA && f32_flux(c->f32) != 0.0)
+      return fail(SPOC_EINVAL, "the Lax-Friedrichs flux option runs on the FP32-MFMA path only");
+    if (precision == SPOC_FP32_MFMA) {
+      rc = f32_rhs(c->f32, c->stream, variant, x, ks, nullptr, nullptr);
+      if (rc) return f32_fail(rc);
+    } else if (precision == SPOC_FP64) {
+      rc = f64_rhs(c->f64, c->stream, c->tab, variant, x, ks, nullptr, nullptr);
+      if (rc) return fail(SPOC_EHIP, "FP64 kernel launch failed: %s",
+                          hipGetErrorString(hipGetLastError()));
+    } else {
+      return fail(SPOC_EINVAL, "unknown precision %d", precision);
+    }
+    double *dst[3] = {u1, v1, h1};
+    return download_fields(c, ld, dst, ks, 3);
+  });
 }
 
 int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nsteps) {
-  int rc = check_ready(c, variant);
-  if (rc) return rc;
-  if (!c->have_state) return fail(SPOC_ESTATE, "no state: call spoc_state_upload first");
-  if (nsteps < 0) return fail(SPOC_EINVAL, "nsteps=%d", nsteps);
-  if (precision != SPOC_FP64 && precision != SPOC_FP32_MFMA)
-    return fail(SPOC_EINVAL, "unknown precision %d", precision);
-  if (precision != SPOC_FP32_MFMA && f32_flux(c->f32) != 0.0)
-    return fail(SPOC_EINVAL, "the Lax-Friedrichs flux option runs on the FP32-MFMA path only");
-  (void)hipSetDevice(c->device);
-  c->prof.used = 0;
-  c->prof.marks.clear();
-  const double *u0[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
-  double *uw[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
-  // one RK4 step (4 right-hand sides with their fused stage updates)
-  auto step = [&](TimerHook hook) -> int {
-    if (precision == SPOC_FP32_MFMA) {
-      int r = f32_rk4_step(c->f32, c->stream, variant, dt, hook, c);
-      return r ? f32_fail(r) : SPOC_OK;
-    }
-    int r = f64_rk4_step(c->f64, c->stream, c->tab, variant, dt, hook, c);
-    return r ? fail(SPOC_EHIP, "FP64 kernel launch failed: %s",
-                    hipGetErrorString(hipGetLastError()))
-             : SPOC_OK;
-  };
-  // an error leaves nothing queued behind it
-  auto bail = [&](int r) {
-    (void)hipStreamSynchronize(c->stream);
+  return spoc::guarded([&]() -> int {
+    int rc = check_ready(c, variant);
+    if (rc) return rc;
+    if (!c->have_state) return fail(SPOC_ESTATE, "no state: call spoc_state_upload first");
+    if (nsteps < 0) return fail(SPOC_EINVAL, "nsteps=%d", nsteps);
+    if (precision != SPOC_FP64 && precision != SPOC_FP32_MFMA)
+      return fail(SPOC_EINVAL, "unknown precision %d", precision);
+    if (precision != SPOC_FP32_MFMA && f32_flux(c->f32) != 0.0)
+      return fail(SPOC_EINVAL, "the Lax-Friedrichs flux option runs on the FP32-MFMA path only");
+    (void)hipSetDevice(c->device);
+    c->prof.used = 0;
+    c->prof.marks.clear();
+    const double *u0[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
+    double *uw[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
+    // one RK4 step (4 right-hand sides with their fused stage updates)
+    auto step = [&](TimerHook hook) -> int {
+      if (precision == SPOC_FP32_MFMA) {
+        int r = f32_rk4_step(c->f32, c->stream, variant, dt, hook, c);
+        return r ? f32_fail(r) : SPOC_OK;
+      }
+      int r = f64_rk4_step(c->f64, c->stream, c->tab, variant, dt, hook, c);
+      return r ? fail(SPOC_EHIP, "FP64 kernel launch failed: %s",
+                      hipGetErrorString(hipGetLastError()))
+               : SPOC_OK;
+    };
+    // an error leaves nothing queued behind it
+    auto bail = [&](int r) {
+      (void)hipStreamSynchronize(c->stream);
+      c->queue.reset();
+      return r;
+    };
     c->queue.reset();
-    return r;
-  };
-  c->queue.reset();
-  const int per_step = precision == SPOC_FP64 ? 4 * f64_kernels_per_rhs(c->f64, variant)
-                                              : step_dispatches(variant, precision);
-  HIPCHK(hipEventRecord(c->t0, c->stream));
-  if (precision == SPOC_FP32_MFMA && (rc = f32_rk4_begin(c->f32, c->stream, variant, uw)))
-    return bail(f32_fail(rc));
-  if (precision == SPOC_FP64 && f64_rk4_begin(c->f64, c->stream, c->tab, u0))
-    return bail(fail(SPOC_EHIP, "FP64 state conversion failed"));
-  // small linear meshes on the mixed path: every step in one persistent
-  // launch (f32_rk4_persist says where it applies)
-  const unsigned *abort_word = nullptr;
-  int persist = 1;
-  if (precision == SPOC_FP32_MFMA && c->use_persist &&
-      (persist = f32_rk4_persist(c->f32, c->stream, variant, dt, nsteps, &abort_word,
-                                 c->prof.on ? prof_hook : nullptr, c)) < 0)
-    return bail(f32_fail(persist));
-  const bool graph = c->use_graph && !c->prof.on && nsteps >= 2;
-  if (persist == 0) {
-    // (the steps ran above)
-  } else if (graph) {
-    // the step's launches are identical every step: replay them as a graph
-    // of kGraphSteps steps (one when fewer are asked for), which costs less
-    // per launch than one graph per step on small meshes; the steps left
-    // over run eagerly
-    const int per = nsteps >= kGraphSteps ? kGraphSteps : 1;
-    spoc_ctx::GraphKey key{variant, precision, dt, c->version, per};
-    if (!c->graph || !(c->graph_key == key)) {
-      destroy_graph(c);
-      hipGraph_t g = nullptr;
-      hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
-      if (e != hipSuccess)
-        return bail(fail(SPOC_EHIP, "graph capture: %s", hipGetErrorString(e)));
-      int r = SPOC_OK;
-      for (int k = 0; k < per && !r; k++) r = step(nullptr);
-      e = hipStreamEndCapture(c->stream, &g);
-      if (r) {
-        if (g) (void)hipGraphDestroy(g);
-        return bail(r);
+    const int per_step = precision == SPOC_FP64 ? 4 * f64_kernels_per_rhs(c->f64, variant)
+                                                : step_dispatches(variant, precision);
+    HIPCHK(hipEventRecord(c->t0, c->stream));
+    if (precision == SPOC_FP32_MFMA && (rc = f32_rk4_begin(c->f32, c->stream, variant, uw)))
+      return bail(f32_fail(rc));
+    if (precision == SPOC_FP64 && f64_rk4_begin(c->f64, c->stream, c->tab, u0))
+      return bail(fail(SPOC_EHIP, "FP64 state conversion failed"));
+    // small linear meshes on the mixed path: every step in one persistent
+    // launch (f32_rk4_persist says where it applies)
+    const unsigned *abort_word = nullptr;
+    int persist = 1;
+    if (precision == SPOC_FP32_MFMA && c->use_persist &&
+        (persist = f32_rk4_persist(c->f32, c->stream, variant, dt, nsteps, &abort_word,
+                                   c->prof.on ? prof_hook : nullptr, c)) < 0)
+      return bail(f32_fail(persist));
+    const bool graph = c->use_graph && !c->prof.on && nsteps >= 2;
+    if (persist == 0) {
+      // (the steps ran above)
+    } else if (graph) {
+      // the step's launches are identical every step: replay them as a graph
+      // of kGraphSteps steps (one when fewer are asked for), which costs less
+      // per launch than one graph per step on small meshes; the steps left
+      // over run eagerly
+      const int per = nsteps >= kGraphSteps ? kGraphSteps : 1;
+      spoc_ctx::GraphKey key{variant, precision, dt, c->version, per};
+      if (!c->graph || !(c->graph_key == key)) {
+        destroy_graph(c);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess)
+          return bail(fail(SPOC_EHIP, "graph capture: %s", hipGetErrorString(e)));
+        int r = SPOC_OK;
+        for (int k = 0; k < per && !r; k++) r = step(nullptr);
+        e = hipStreamEndCapture(c->stream, &g);
+        if (r) {
+          if (g) (void)hipGraphDestroy(g);
+          return bail(r);
+        }
+        if (e != hipSuccess) return bail(fail(SPOC_EHIP, "graph capture: %s", hipGetErrorString(e)));
+        e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+          c->graph = nullptr;
+          return bail(fail(SPOC_EHIP, "graph instantiate: %s", hipGetErrorString(e)));
+        }
+        c->graph_key = key;
       }
-      if (e != hipSuccess) return bail(fail(SPOC_EHIP, "graph capture: %s", hipGetErrorString(e)));
-      e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      if (e != hipSuccess) {
-        c->graph = nullptr;
-        return bail(fail(SPOC_EHIP, "graph instantiate: %s", hipGetErrorString(e)));
+      for (int it = 0; it + per <= nsteps; it += per) {
+        hipError_t e = c->queue.admit(per * per_step);
+        if (e == hipSuccess) e = hipGraphLaunch(c->graph, c->stream);
+        if (e == hipSuccess) e = c->queue.mark(c->stream);
+        if (e != hipSuccess) return bail(fail(SPOC_EHIP, "graph launch: %s", hipGetErrorString(e)));
       }
-      c->graph_key = key;
-    }
-    for (int it = 0; it + per <= nsteps; it += per) {
-      hipError_t e = c->queue.admit(per * per_step);
-      if (e == hipSuccess) e = hipGraphLaunch(c->graph, c->stream);
-      if (e == hipSuccess) e = c->queue.mark(c->stream);
-      if (e != hipSuccess) return bail(fail(SPOC_EHIP, "graph launch: %s", hipGetErrorString(e)));
-    }
-    for (int it = nsteps - nsteps % per; it < nsteps; it++) {
-      hipError_t e = c->queue.admit(per_step);
-      if (e != hipSuccess) return bail(fail(SPOC_EHIP, "queue wait: %s", hipGetErrorString(e)));
-      if ((rc = step(nullptr))) return bail(rc);
-      if ((e = c->queue.mark(c->stream)) != hipSuccess)
-        return bail(fail(SPOC_EHIP, "queue mark: %s", hipGetErrorString(e)));
-    }
-  } else {
-    for (int it = 0; it < nsteps; it++) {
-      hipError_t e = c->queue.admit(per_step);
-      if (e != hipSuccess) return bail(fail(SPOC_EHIP, "queue wait: %s", hipGetErrorString(e)));
-      if ((rc = step(prof_hook))) return bail(rc);
-      if ((e = c->queue.mark(c->stream)) != hipSuccess)
-        return bail(fail(SPOC_EHIP, "queue mark: %s", hipGetErrorString(e)));
-    }
-  }
-  if (precision == SPOC_FP32_MFMA && (rc = f32_rk4_end(c->f32, c->stream, variant, uw)))
-    return bail(f32_fail(rc));
-  if (precision == SPOC_FP64 && f64_rk4_end(c->f64, c->stream, c->tab, uw))
-    return bail(fail(SPOC_EHIP, "FP64 state conversion failed"));
-  HIPCHK(hipEventRecord(c->t1, c->stream));
-  // the driver's testnan: is every coefficient of the new state finite?
-  if (nonfinite_launch(c->stream, (size_t)3 * c->nm * c->ne, state_ptr(c, 0), c->dflag))
-    return bail(fail(SPOC_EHIP, "non-finite test launch failed"));
-  int bad = 0;
-  unsigned aborted = 0;
-  HIPCHK(hipMemcpyAsync(&bad, c->dflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  if (abort_word)
-    HIPCHK(hipMemcpyAsync(&aborted, abort_word, sizeof(unsigned), hipMemcpyDeviceToHost,
-                          c->stream));
-  HIPCHK(hipEventSynchronize(c->t1));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  c->queue.reset();
-  float ms = 0.f;
-  HIPCHK(hipEventElapsedTime(&ms, c->t0, c->t1));
-  c->last_ms = ms;
-  // per-kernel totals from the hook events: slot k -> k+1 brackets kernel k
-  c->knames.clear();
-  c->kms.clear();
-  c->kcount.clear();
-  if (c->prof.on) {
-    std::vector<double> tot(16, 0.0);
-    std::vector<int> cnt(16, 0);
-    for (size_t i = 0; i + 1 < c->prof.marks.size(); i++) {
-      int s0 = c->prof.marks[i].first, s1 = c->prof.marks[i + 1].first;
-      if (s1 != s0 + 1) continue;
-      float m = 0.f;
-      if (hipEventElapsedTime(&m, c->prof.ev[c->prof.marks[i].second],
-                              c->prof.ev[c->prof.marks[i + 1].second]) == hipSuccess) {
-        tot[s0] += m;
-        cnt[s0]++;
-      }
-    }
-    const char *lin[] = {"face_lin", "elem_lin", "wall_lin"};
-    const char *nl[] = {"face_uv", "elem_q", "wall_q", "elem_c", "wall_c", "face_nl2", "elem_eg",
-                        "wall_eg"};
-    int nk = variant == SPOC_LIN ? 3 : 8;
-    if (precision == SPOC_FP32_MFMA) {
-      const char *l32[] = {persist == 0 ? "lin_persistent" : "lin_f64"};
-      const char *n32[] = {"bc_f32", "eg_f32f64"};
-      nk = variant == SPOC_LIN ? 1 : 2;
-      for (int k = 0; k < nk; k++) {
-        c->knames.push_back(variant == SPOC_LIN ? l32[k] : n32[k]);
-        c->kms.push_back(tot[k]);
-        c->kcount.push_back(cnt[k]);
+      for (int it = nsteps - nsteps % per; it < nsteps; it++) {
+        hipError_t e = c->queue.admit(per_step);
+        if (e != hipSuccess) return bail(fail(SPOC_EHIP, "queue wait: %s", hipGetErrorString(e)));
+        if ((rc = step(nullptr))) return bail(rc);
+        if ((e = c->queue.mark(c->stream)) != hipSuccess)
+          return bail(fail(SPOC_EHIP, "queue mark: %s", hipGetErrorString(e)));
       }
     } else {
-      for (int k = 0; k < nk; k++) {
-        c->knames.push_back(variant == SPOC_LIN ? lin[k] : nl[k]);
-        c->kms.push_back(tot[k]);
-        c->kcount.push_back(cnt[k]);
+      for (int it = 0; it < nsteps; it++) {
+        hipError_t e = c->queue.admit(per_step);
+        if (e != hipSuccess) return bail(fail(SPOC_EHIP, "queue wait: %s", hipGetErrorString(e)));
+        if ((rc = step(prof_hook))) return bail(rc);
+        if ((e = c->queue.mark(c->stream)) != hipSuccess)
+          return bail(fail(SPOC_EHIP, "queue mark: %s", hipGetErrorString(e)));
       }
     }
-  }
-  if (aborted)
-    return fail(SPOC_EHIP, "persistent RK4 kernel: a wait for a neighbouring block ran out (not "
-                           "every wave of the launch was resident); the state is not advanced "
-                           "consistently: call spoc_set_persistent(ctx, 0) and run again");
-  if (bad)
-    return fail(SPOC_ENAN, "the state holds a NaN or an infinity after %d RK4 steps (dt=%g)",
-                nsteps, dt);
-  return SPOC_OK;
+    if (precision == SPOC_FP32_MFMA && (rc = f32_rk4_end(c->f32, c->stream, variant, uw)))
+      return bail(f32_fail(rc));
+    if (precision == SPOC_FP64 && f64_rk4_end(c->f64, c->stream, c->tab, uw))
+      return bail(fail(SPOC_EHIP, "FP64 state conversion failed"));
+    HIPCHK(hipEventRecord(c->t1, c->stream));
+    // the driver's testnan: is every coefficient of the new state finite?
+    if (nonfinite_launch(c->stream, (size_t)3 * c->nm * c->ne, state_ptr(c, 0), c->dflag))
+      return bail(fail(SPOC_EHIP, "non-finite test launch failed"));
+    int bad = 0;
+    unsigned aborted = 0;
+    HIPCHK(hipMemcpyAsync(&bad, c->dflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (abort_word)
+      HIPCHK(hipMemcpyAsync(&aborted, abort_word, sizeof(unsigned), hipMemcpyDeviceToHost,
+                            c->stream));
+    HIPCHK(hipEventSynchronize(c->t1));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->queue.reset();
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->t0, c->t1));
+    c->last_ms = ms;
+    // per-kernel totals from the hook events: slot k -> k+1 brackets kernel k
+    c->knames.clear();
+    c->kms.clear();
+    c->kcount.clear();
+    if (c->prof.on) {
+      std::vector<double> tot(16, 0.0);
+      std::vector<int> cnt(16, 0);
+      for (size_t i = 0; i + 1 < c->prof.marks.size(); i++) {
+        int s0 = c->prof.marks[i].first, s1 = c->prof.marks[i + 1].first;
+        if (s1 != s0 + 1) continue;
+        float m = 0.f;
+        if (hipEventElapsedTime(&m, c->prof.ev[c->prof.marks[i].second],
+                                c->prof.ev[c->prof.marks[i + 1].second]) == hipSuccess) {
+          tot[s0] += m;
+          cnt[s0]++;
+        }
+      }
+      const char *lin[] = {"face_lin", "elem_lin", "wall_lin"};
+      const char *nl[] = {"face_uv", "elem_q", "wall_q", "elem_c", "wall_c", "face_nl2", "elem_eg",
+                          "wall_eg"};
+      int nk = variant == SPOC_LIN ? 3 : 8;
+      if (precision == SPOC_FP32_MFMA) {
+        const char *l32[] = {persist == 0 ? "lin_persistent" : "lin_f64"};
+        const char *n32[] = {"bc_f32", "eg_f32f64"};
+        nk = variant == SPOC_LIN ? 1 : 2;
+        for (int k = 0; k < nk; k++) {
+          c->knames.push_back(variant == SPOC_LIN ? l32[k] : n32[k]);
+          c->kms.push_back(tot[k]);
+          c->kcount.push_back(cnt[k]);
+        }
+      } else {
+        for (int k = 0; k < nk; k++) {
+          c->knames.push_back(variant == SPOC_LIN ? lin[k] : nl[k]);
+          c->kms.push_back(tot[k]);
+          c->kcount.push_back(cnt[k]);
+        }
+      }
+    }
+    if (aborted)
+      return fail(SPOC_EHIP, "persistent RK4 kernel: a wait for a neighbouring block ran out (not "
+                             "every wave of the launch was resident); the state is not advanced "
+                             "consistently: call spoc_set_persistent(ctx, 0) and run again");
+    if (bad)
+      return fail(SPOC_ENAN, "the state holds a NaN or an infinity after %d RK4 steps (dt=%g)",
+                  nsteps, dt);
+    return SPOC_OK;
+  });
 }
 
 static int adj_fail(int rc) {
@@ -1186,172 +1206,198 @@ static int adj_fail(int rc) {
 int spoc_rhs_adjoint(spoc_ctx *c, int variant, int ld, const double *u0, const double *v0,
                      const double *h0, const double *au, const double *av, const double *ah,
                      double *ru, double *rv, double *rh, double *rhb) {
-  int rc = check_ready(c, variant);
-  if (rc) return rc;
-  if (ld < c->nm || !au || !av || !ah || !ru || !rv || !rh)
-    return fail(SPOC_EINVAL, "spoc_rhs_adjoint: bad arguments");
-  if (variant != SPOC_LIN && (!u0 || !v0 || !h0))
-    return fail(SPOC_EINVAL, "spoc_rhs_adjoint: the nonlinear variants need the state (u0,v0,h0)");
-  if (f32_flux(c->f32) != 0.0)
-    return fail(SPOC_EINVAL, "the adjoint is of the central flux: spoc_set_flux(ctx, 0) first");
-  (void)hipSetDevice(c->device);
-  size_t n = (size_t)c->nm * c->ne;
-  const double *src[3] = {au, av, ah};
-  double *lam[3] = {c->dadj, c->dadj + n, c->dadj + 2 * n};
-  double *ys[3] = {state_ptr(c, 9), state_ptr(c, 10), state_ptr(c, 11)};
-  double *xs[3] = {state_ptr(c, 12), state_ptr(c, 13), state_ptr(c, 14)};
-  if ((rc = upload_fields(c, ld, src, lam, 3))) return rc;
-  if (variant != SPOC_LIN) {
-    const double *xsrc[3] = {u0, v0, h0};
-    if ((rc = upload_fields(c, ld, xsrc, xs, 3))) return rc;
-  }
-  double *hb = c->dadj + 3 * n;
-  rc = f32_adj_rhs(c->f32, c->stream, variant, variant != SPOC_LIN ? xs : nullptr, lam, ys,
-                   variant != SPOC_LIN ? hb : nullptr);
-  if (rc) return adj_fail(rc);
-  c->have_adj = c->have_adj_hb = false;  // the adjoint slots now hold this call's data
-  double *dst[3] = {ru, rv, rh};
-  if ((rc = download_fields(c, ld, dst, ys, 3))) return rc;
-  if (rhb) {
-    if (variant == SPOC_LIN) {
-      for (int e = 0; e < c->ne; e++)
-        for (int m = 0; m < c->nm; m++) rhb[m + (size_t)ld * e] = 0.0;
-    } else {
-      double *d1[1] = {rhb}, *s1[1] = {hb};
-      if ((rc = download_fields(c, ld, d1, s1, 1))) return rc;
+  return spoc::guarded([&]() -> int {
+    int rc = check_ready(c, variant);
+    if (rc) return rc;
+    if (ld < c->nm || !au || !av || !ah || !ru || !rv || !rh)
+      return fail(SPOC_EINVAL, "spoc_rhs_adjoint: bad arguments");
+    if (variant != SPOC_LIN && (!u0 || !v0 || !h0))
+      return fail(SPOC_EINVAL, "spoc_rhs_adjoint: the nonlinear variants need the state (u0,v0,h0)");
+    if (f32_flux(c->f32) != 0.0)
+      return fail(SPOC_EINVAL, "the adjoint is of the central flux: spoc_set_flux(ctx, 0) first");
+    (void)hipSetDevice(c->device);
+    size_t n = (size_t)c->nm * c->ne;
+    const double *src[3] = {au, av, ah};
+    double *lam[3] = {c->dadj, c->dadj + n, c->dadj + 2 * n};
+    double *ys[3] = {state_ptr(c, 9), state_ptr(c, 10), state_ptr(c, 11)};
+    double *xs[3] = {state_ptr(c, 12), state_ptr(c, 13), state_ptr(c, 14)};
+    if ((rc = upload_fields(c, ld, src, lam, 3))) return rc;
+    if (variant != SPOC_LIN) {
+      const double *xsrc[3] = {u0, v0, h0};
+      if ((rc = upload_fields(c, ld, xsrc, xs, 3))) return rc;
     }
-  }
-  return SPOC_OK;
+    double *hb = c->dadj + 3 * n;
+    rc = f32_adj_rhs(c->f32, c->stream, variant, variant != SPOC_LIN ? xs : nullptr, lam, ys,
+                     variant != SPOC_LIN ? hb : nullptr);
+    if (rc) return adj_fail(rc);
+    c->have_adj = c->have_adj_hb = false;  // the adjoint slots now hold this call's data
+    double *dst[3] = {ru, rv, rh};
+    if ((rc = download_fields(c, ld, dst, ys, 3))) return rc;
+    if (rhb) {
+      if (variant == SPOC_LIN) {
+        for (int e = 0; e < c->ne; e++)
+          for (int m = 0; m < c->nm; m++) rhb[m + (size_t)ld * e] = 0.0;
+      } else {
+        double *d1[1] = {rhb}, *s1[1] = {hb};
+        if ((rc = download_fields(c, ld, d1, s1, 1))) return rc;
+      }
+    }
+    return SPOC_OK;
+  });
 }
 
 int spoc_adjoint_upload(spoc_ctx *c, int ld, const double *au, const double *av,
                         const double *ah) {
-  if (!c || !c->dmesh) return fail(SPOC_ESTATE, "spoc_adjoint_upload: set the mesh first");
-  if (ld < c->nm || !au || !av || !ah) return fail(SPOC_EINVAL, "spoc_adjoint_upload: bad args");
-  (void)hipSetDevice(c->device);
-  size_t n = (size_t)c->nm * c->ne;
-  const double *src[3] = {au, av, ah};
-  double *dst[3] = {c->dadj, c->dadj + n, c->dadj + 2 * n};
-  int rc = upload_fields(c, ld, src, dst, 3);
-  if (!rc) c->have_adj = true;
-  c->have_adj_hb = false;
-  return rc;
+  return spoc::guarded([&]() -> int {
+    if (!c || !c->dmesh) return fail(SPOC_ESTATE, "spoc_adjoint_upload: set the mesh first");
+    if (ld < c->nm || !au || !av || !ah) return fail(SPOC_EINVAL, "spoc_adjoint_upload: bad args");
+    (void)hipSetDevice(c->device);
+    size_t n = (size_t)c->nm * c->ne;
+    const double *src[3] = {au, av, ah};
+    double *dst[3] = {c->dadj, c->dadj + n, c->dadj + 2 * n};
+    int rc = upload_fields(c, ld, src, dst, 3);
+    if (!rc) c->have_adj = true;
+    c->have_adj_hb = false;
+    return rc;
+  });
 }
 
 int spoc_adjoint_download(spoc_ctx *c, int ld, double *au, double *av, double *ah) {
-  if (!c || !c->have_adj) return fail(SPOC_ESTATE, "spoc_adjoint_download: no adjoint state");
-  if (ld < c->nm || !au || !av || !ah) return fail(SPOC_EINVAL, "spoc_adjoint_download: bad args");
-  (void)hipSetDevice(c->device);
-  size_t n = (size_t)c->nm * c->ne;
-  double *dst[3] = {au, av, ah};
-  double *src[3] = {c->dadj, c->dadj + n, c->dadj + 2 * n};
-  return download_fields(c, ld, dst, src, 3);
+  return spoc::guarded([&]() -> int {
+    if (!c || !c->have_adj) return fail(SPOC_ESTATE, "spoc_adjoint_download: no adjoint state");
+    if (ld < c->nm || !au || !av || !ah) return fail(SPOC_EINVAL, "spoc_adjoint_download: bad args");
+    (void)hipSetDevice(c->device);
+    size_t n = (size_t)c->nm * c->ne;
+    double *dst[3] = {au, av, ah};
+    double *src[3] = {c->dadj, c->dadj + n, c->dadj + 2 * n};
+    return download_fields(c, ld, dst, src, 3);
+  });
 }
 
 int spoc_adjoint_hb_download(spoc_ctx *c, int ld, double *ahb) {
-  if (!c || !c->have_adj_hb)
-    return fail(SPOC_ESTATE, "spoc_adjoint_hb_download: no bathymetry adjoint (run "
-                             "spoc_rk4_adjoint_steps with a nonlinear variant first)");
-  if (ld < c->nm || !ahb) return fail(SPOC_EINVAL, "spoc_adjoint_hb_download: bad args");
-  (void)hipSetDevice(c->device);
-  size_t n = (size_t)c->nm * c->ne;
-  double *dst[1] = {ahb}, *src[1] = {c->dadj + 3 * n};
-  return download_fields(c, ld, dst, src, 1);
+  return spoc::guarded([&]() -> int {
+    if (!c || !c->have_adj_hb)
+      return fail(SPOC_ESTATE, "spoc_adjoint_hb_download: no bathymetry adjoint (run "
+                               "spoc_rk4_adjoint_steps with a nonlinear variant first)");
+    if (ld < c->nm || !ahb) return fail(SPOC_EINVAL, "spoc_adjoint_hb_download: bad args");
+    (void)hipSetDevice(c->device);
+    size_t n = (size_t)c->nm * c->ne;
+    double *dst[1] = {ahb}, *src[1] = {c->dadj + 3 * n};
+    return download_fields(c, ld, dst, src, 1);
+  });
 }
 
 int spoc_set_checkpoint(spoc_ctx *c, int every) {
-  if (!c) return fail(SPOC_EINVAL, "null context");
-  if (every < 0) return fail(SPOC_EINVAL, "checkpoint interval %d", every);
-  c->ncheck = every;
-  return SPOC_OK;
+  return spoc::guarded([&]() -> int {
+    if (!c) return fail(SPOC_EINVAL, "null context");
+    if (every < 0) return fail(SPOC_EINVAL, "checkpoint interval %d", every);
+    c->ncheck = every;
+    return SPOC_OK;
+  });
 }
 
 int spoc_rk4_adjoint_steps(spoc_ctx *c, int variant, double dt, int nsteps) {
-  int rc = check_ready(c, variant);
-  if (rc) return rc;
-  if (!c->have_adj) return fail(SPOC_ESTATE, "no adjoint state: call spoc_adjoint_upload first");
-  if (variant != SPOC_LIN && !c->have_state)
-    return fail(SPOC_ESTATE, "the nonlinear adjoint needs the initial state: spoc_state_upload");
-  if (nsteps < 0) return fail(SPOC_EINVAL, "nsteps=%d", nsteps);
-  if (f32_flux(c->f32) != 0.0)
-    return fail(SPOC_EINVAL, "the adjoint is of the central flux: spoc_set_flux(ctx, 0) first");
-  (void)hipSetDevice(c->device);
-  drop_graph(c);
-  size_t n = (size_t)c->nm * c->ne;
-  double *lam[3] = {c->dadj, c->dadj + n, c->dadj + 2 * n};
-  const double *x0[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
-  HIPCHK(hipEventRecord(c->t0, c->stream));
-  rc = f32_adj_run(c->f32, c->stream, variant, dt, nsteps, c->ncheck, x0, lam,
-                   variant != SPOC_LIN ? c->dadj + 3 * n : nullptr);
-  if (rc) return adj_fail(rc);
-  HIPCHK(hipEventRecord(c->t1, c->stream));
-  HIPCHK(hipEventSynchronize(c->t1));
-  float ms = 0.f;
-  HIPCHK(hipEventElapsedTime(&ms, c->t0, c->t1));
-  c->last_ms = ms;
-  c->have_adj_hb = variant != SPOC_LIN;
-  return SPOC_OK;
+  return spoc::guarded([&]() -> int {
+    int rc = check_ready(c, variant);
+    if (rc) return rc;
+    if (!c->have_adj) return fail(SPOC_ESTATE, "no adjoint state: call spoc_adjoint_upload first");
+    if (variant != SPOC_LIN && !c->have_state)
+      return fail(SPOC_ESTATE, "the nonlinear adjoint needs the initial state: spoc_state_upload");
+    if (nsteps < 0) return fail(SPOC_EINVAL, "nsteps=%d", nsteps);
+    if (f32_flux(c->f32) != 0.0)
+      return fail(SPOC_EINVAL, "the adjoint is of the central flux: spoc_set_flux(ctx, 0) first");
+    (void)hipSetDevice(c->device);
+    drop_graph(c);
+    size_t n = (size_t)c->nm * c->ne;
+    double *lam[3] = {c->dadj, c->dadj + n, c->dadj + 2 * n};
+    const double *x0[3] = {state_ptr(c, 0), state_ptr(c, 1), state_ptr(c, 2)};
+    HIPCHK(hipEventRecord(c->t0, c->stream));
+    rc = f32_adj_run(c->f32, c->stream, variant, dt, nsteps, c->ncheck, x0, lam,
+                     variant != SPOC_LIN ? c->dadj + 3 * n : nullptr);
+    if (rc) return adj_fail(rc);
+    HIPCHK(hipEventRecord(c->t1, c->stream));
+    HIPCHK(hipEventSynchronize(c->t1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->t0, c->t1));
+    c->last_ms = ms;
+    c->have_adj_hb = variant != SPOC_LIN;
+    return SPOC_OK;
+  });
 }
 
 int spoc_autotune(spoc_ctx *c, int variant, int *tiles) {
-  int rc = check_ready(c, variant);
-  if (rc) return rc;
-  (void)hipSetDevice(c->device);
-  drop_graph(c);
-  rc = f32_autotune(c->f32, c->stream, variant, tiles);
-  return rc ? f32_fail(rc) : SPOC_OK;
+  return spoc::guarded([&]() -> int {
+    int rc = check_ready(c, variant);
+    if (rc) return rc;
+    (void)hipSetDevice(c->device);
+    drop_graph(c);
+    rc = f32_autotune(c->f32, c->stream, variant, tiles);
+    return rc ? f32_fail(rc) : SPOC_OK;
+  });
 }
 
 int spoc_set_tiles(spoc_ctx *c, int variant, int tile0, int tile1, int autotune) {
-  if (!c) return fail(SPOC_EINVAL, "null context");
-  if (variant < SPOC_LIN || variant > SPOC_INV) return fail(SPOC_EINVAL, "variant=%d", variant);
-  for (int t : {tile0, tile1}) {
-    const int wg = t & 15, form = t >> 4;
-    if (t != 0 && (t < 0 || (wg != 2 && wg != 4 && wg != 8) || form >= TILE_FORMS))
-      return fail(SPOC_EINVAL,
-                  "tile %d: 2, 4 or 8 waves per workgroup plus 16 x a kernel form 0..%d "
-                  "(0: automatic)", t, TILE_FORMS - 1);
-  }
-  drop_graph(c);
-  f32_set_tiles(c->f32, variant, tile0, tile1, autotune);
-  return SPOC_OK;
+  return spoc::guarded([&]() -> int {
+    if (!c) return fail(SPOC_EINVAL, "null context");
+    if (variant < SPOC_LIN || variant > SPOC_INV) return fail(SPOC_EINVAL, "variant=%d", variant);
+    for (int t : {tile0, tile1}) {
+      const int wg = t & 15, form = t >> 4;
+      if (t != 0 && (t < 0 || (wg != 2 && wg != 4 && wg != 8) || form >= TILE_FORMS))
+        return fail(SPOC_EINVAL,
+                    "tile %d: 2, 4 or 8 waves per workgroup plus 16 x a kernel form 0..%d "
+                    "(0: automatic)", t, TILE_FORMS - 1);
+    }
+    drop_graph(c);
+    f32_set_tiles(c->f32, variant, tile0, tile1, autotune);
+    return SPOC_OK;
+  });
 }
 
 int spoc_set_flux(spoc_ctx *c, double alpha) {
-  if (!c) return fail(SPOC_EINVAL, "null context");
-  if (!(alpha >= 0.0) || alpha > 100.0) return fail(SPOC_EINVAL, "flux alpha=%g", alpha);
-  drop_graph(c);
-  f32_set_flux(c->f32, alpha);
-  return SPOC_OK;
+  return spoc::guarded([&]() -> int {
+    if (!c) return fail(SPOC_EINVAL, "null context");
+    if (!(alpha >= 0.0) || alpha > 100.0) return fail(SPOC_EINVAL, "flux alpha=%g", alpha);
+    drop_graph(c);
+    f32_set_flux(c->f32, alpha);
+    return SPOC_OK;
+  });
 }
 
 int spoc_set_ordering(spoc_ctx *c, int mode) {
-  if (!c) return fail(SPOC_EINVAL, "null context");
-  if (mode != SPOC_ORDER_INPUT && mode != SPOC_ORDER_HILBERT && mode != SPOC_ORDER_ZCURVE &&
-      mode != SPOC_ORDER_AUTO)
-    return fail(SPOC_EINVAL, "ordering mode %d (SPOC_ORDER_INPUT 0, SPOC_ORDER_HILBERT 1, "
-                             "SPOC_ORDER_ZCURVE 2, SPOC_ORDER_AUTO 3)", mode);
-  c->ordering = mode;
-  return SPOC_OK;
+  return spoc::guarded([&]() -> int {
+    if (!c) return fail(SPOC_EINVAL, "null context");
+    if (mode != SPOC_ORDER_INPUT && mode != SPOC_ORDER_HILBERT && mode != SPOC_ORDER_ZCURVE &&
+        mode != SPOC_ORDER_AUTO)
+      return fail(SPOC_EINVAL, "ordering mode %d (SPOC_ORDER_INPUT 0, SPOC_ORDER_HILBERT 1, "
+                               "SPOC_ORDER_ZCURVE 2, SPOC_ORDER_AUTO 3)", mode);
+    c->ordering = mode;
+    return SPOC_OK;
+  });
 }
 
 int spoc_set_graph(spoc_ctx *c, int on) {
-  if (!c) return fail(SPOC_EINVAL, "null context");
-  c->use_graph = on != 0;
-  if (!on) drop_graph(c);
-  return SPOC_OK;
+  return spoc::guarded([&]() -> int {
+    if (!c) return fail(SPOC_EINVAL, "null context");
+    c->use_graph = on != 0;
+    if (!on) drop_graph(c);
+    return SPOC_OK;
+  });
 }
 
 int spoc_set_persistent(spoc_ctx *c, int on) {
-  if (!c) return fail(SPOC_EINVAL, "null context");
-  c->use_persist = on != 0;
-  return SPOC_OK;
+  return spoc::guarded([&]() -> int {
+    if (!c) return fail(SPOC_EINVAL, "null context");
+    c->use_persist = on != 0;
+    return SPOC_OK;
+  });
 }
 
 int spoc_sync(spoc_ctx *c) {
-  if (!c) return fail(SPOC_EINVAL, "null context");
-  HIPCHK(hipStreamSynchronize(c->stream));
-  return SPOC_OK;
+  return spoc::guarded([&]() -> int {
+    if (!c) return fail(SPOC_EINVAL, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SPOC_OK;
+  });
 }
 
 // (ke, pe, mass) of nr element ranges given by the device offsets off.
@@ -1370,18 +1416,20 @@ static int energies(spoc_ctx *c, int nr, const int *off, double *r) {
 }
 
 int spoc_energy(spoc_ctx *c, double *ke, double *pe, double *mass) {
-  if (!c || !c->have_state || !c->have_fields)
-    return fail(SPOC_ESTATE, "spoc_energy: need mesh, fields and state");
-  if (!c->have_nodes) return fail(SPOC_ESTATE, "spoc_energy: mesh was set without node arrays");
-  (void)hipSetDevice(c->device);
-  double r[3];
-  int rc = energies(c, 1, c->dmem, r);
-  if (rc) return rc;
-  *ke = r[0];
-  *pe = r[1];
-  *mass = r[2];
-  if (std::isnan(r[0])) return fail(SPOC_ENAN, "spoc_energy: the kinetic energy is NaN");
-  return SPOC_OK;
+  return spoc::guarded([&]() -> int {
+    if (!c || !c->have_state || !c->have_fields)
+      return fail(SPOC_ESTATE, "spoc_energy: need mesh, fields and state");
+    if (!c->have_nodes) return fail(SPOC_ESTATE, "spoc_energy: mesh was set without node arrays");
+    (void)hipSetDevice(c->device);
+    double r[3];
+    int rc = energies(c, 1, c->dmem, r);
+    if (rc) return rc;
+    *ke = r[0];
+    *pe = r[1];
+    *mass = r[2];
+    if (std::isnan(r[0])) return fail(SPOC_ENAN, "spoc_energy: the kinetic energy is NaN");
+    return SPOC_OK;
+  });
 }
 
 // The refinement indicator of the resident state (adapt_launch).  Layout of
@@ -1417,137 +1465,149 @@ static int adapt_run(spoc_ctx *c, bool decide, double fac1, double fac2) {
 }
 
 int spoc_checkconv(spoc_ctx *c, int field, double *amax) {
-  if (!c || !c->have_state) return fail(SPOC_ESTATE, "spoc_checkconv: no state");
-  if (field < 0 || field > 2 || !amax)
-    return fail(SPOC_EINVAL, "spoc_checkconv: field=%d (0 u, 1 v, 2 h)", field);
-  (void)hipSetDevice(c->device);
-  int rc = adapt_run(c, false, 0.0, 0.0);
-  if (rc) return rc;
-  const double *d = adapt_bufs(c).amax + (size_t)field * c->ne;
-  std::vector<double> tmp(c->perm.empty() ? 0 : c->ne);
-  double *h = c->perm.empty() ? amax : tmp.data();
-  HIPCHK(hipMemcpyAsync(h, d, sizeof(double) * c->ne, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  for (size_t e = 0; e < tmp.size(); e++) amax[c->perm[e]] = tmp[e];
-  return SPOC_OK;
+  return spoc::guarded([&]() -> int {
+    if (!c || !c->have_state) return fail(SPOC_ESTATE, "spoc_checkconv: no state");
+    if (field < 0 || field > 2 || !amax)
+      return fail(SPOC_EINVAL, "spoc_checkconv: field=%d (0 u, 1 v, 2 h)", field);
+    (void)hipSetDevice(c->device);
+    int rc = adapt_run(c, false, 0.0, 0.0);
+    if (rc) return rc;
+    const double *d = adapt_bufs(c).amax + (size_t)field * c->ne;
+    std::vector<double> tmp(c->perm.empty() ? 0 : c->ne);
+    double *h = c->perm.empty() ? amax : tmp.data();
+    HIPCHK(hipMemcpyAsync(h, d, sizeof(double) * c->ne, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t e = 0; e < tmp.size(); e++) amax[c->perm[e]] = tmp[e];
+    return SPOC_OK;
+  });
 }
 
 int spoc_cal_conv(spoc_ctx *c, double fac1, double fac2, double fac3, int *fconv, int *fderaff,
                   int *conv, int *deraff, int *timedir, double *supmax, double *amax3) {
-  if (!c || !c->have_state) return fail(SPOC_ESTATE, "spoc_cal_conv: no state");
-  if (!fconv || !fderaff || !conv || !deraff || !timedir)
-    return fail(SPOC_EINVAL, "spoc_cal_conv: null output");
-  (void)hipSetDevice(c->device);
-  int rc = adapt_run(c, true, fac1, fac2);
-  if (rc) return rc;
-  const int nmem = (int)c->mem_e0.size() - 1;
-  const size_t ne = c->ne;
-  AdaptBufs b = adapt_bufs(c);
-  std::vector<double> r(2 * (size_t)nmem);
-  const bool perm = !c->perm.empty();
-  std::vector<int> fl(perm ? 2 * ne : 0);
-  std::vector<double> am(perm && amax3 ? 3 * ne : 0);
-  int *hf0 = perm ? fl.data() : fconv, *hf1 = perm ? fl.data() + ne : fderaff;
-  double *ha = perm ? am.data() : amax3;
-  HIPCHK(hipMemcpyAsync(r.data(), b.res, sizeof(double) * r.size(), hipMemcpyDeviceToHost,
-                        c->stream));
-  HIPCHK(hipMemcpyAsync(hf0, b.flags, sizeof(int) * ne, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipMemcpyAsync(hf1, b.flags + ne, sizeof(int) * ne, hipMemcpyDeviceToHost, c->stream));
-  if (amax3)
-    HIPCHK(hipMemcpyAsync(ha, b.amax, sizeof(double) * 3 * ne, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  if (perm)  // back to the caller's numbering
-    for (size_t e = 0; e < ne; e++) {
-      const size_t o = c->perm[e];
-      fconv[o] = fl[e];
-      fderaff[o] = fl[ne + e];
-      if (amax3)
-        for (int f = 0; f < 3; f++) amax3[f * ne + o] = am[f * ne + e];
+  return spoc::guarded([&]() -> int {
+    if (!c || !c->have_state) return fail(SPOC_ESTATE, "spoc_cal_conv: no state");
+    if (!fconv || !fderaff || !conv || !deraff || !timedir)
+      return fail(SPOC_EINVAL, "spoc_cal_conv: null output");
+    (void)hipSetDevice(c->device);
+    int rc = adapt_run(c, true, fac1, fac2);
+    if (rc) return rc;
+    const int nmem = (int)c->mem_e0.size() - 1;
+    const size_t ne = c->ne;
+    AdaptBufs b = adapt_bufs(c);
+    std::vector<double> r(2 * (size_t)nmem);
+    const bool perm = !c->perm.empty();
+    std::vector<int> fl(perm ? 2 * ne : 0);
+    std::vector<double> am(perm && amax3 ? 3 * ne : 0);
+    int *hf0 = perm ? fl.data() : fconv, *hf1 = perm ? fl.data() + ne : fderaff;
+    double *ha = perm ? am.data() : amax3;
+    HIPCHK(hipMemcpyAsync(r.data(), b.res, sizeof(double) * r.size(), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipMemcpyAsync(hf0, b.flags, sizeof(int) * ne, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(hf1, b.flags + ne, sizeof(int) * ne, hipMemcpyDeviceToHost, c->stream));
+    if (amax3)
+      HIPCHK(hipMemcpyAsync(ha, b.amax, sizeof(double) * 3 * ne, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (perm)  // back to the caller's numbering
+      for (size_t e = 0; e < ne; e++) {
+        const size_t o = c->perm[e];
+        fconv[o] = fl[e];
+        fderaff[o] = fl[ne + e];
+        if (amax3)
+          for (int f = 0; f < 3; f++) amax3[f * ne + o] = am[f * ne + e];
+      }
+    for (int m = 0; m < nmem; m++) {  // src/adapt.f:1180-1189
+      const double sup = r[m];
+      deraff[m] = r[nmem + m] > 0.0;
+      timedir[m] = sup > fac3 ? 0 : 1;
+      conv[m] = (sup > fac1 || deraff[m]) ? 1 : 0;
+      if (supmax) supmax[m] = sup;
     }
-  for (int m = 0; m < nmem; m++) {  // src/adapt.f:1180-1189
-    const double sup = r[m];
-    deraff[m] = r[nmem + m] > 0.0;
-    timedir[m] = sup > fac3 ? 0 : 1;
-    conv[m] = (sup > fac1 || deraff[m]) ? 1 : 0;
-    if (supmax) supmax[m] = sup;
-  }
-  return SPOC_OK;
+    return SPOC_OK;
+  });
 }
 
 int spoc_energy_noalloc(spoc_ctx *c, double *out_dev) {
-  if (!c || !c->have_state || !c->have_fields)
-    return fail(SPOC_ESTATE, "spoc_energy_noalloc: need mesh, fields and state");
-  if (!c->have_nodes) return fail(SPOC_ESTATE, "spoc_energy_noalloc: mesh set without nodes");
-  if (!out_dev) return fail(SPOC_EINVAL, "spoc_energy_noalloc: null output");
-  const size_t need = (size_t)c->ne * (2 + (size_t)c->nm), cap = energy_scratch_doubles();
-  if (need > cap)
-    return fail(SPOC_EINVAL,
-                "spoc_energy_noalloc: %d elements of %d modes need %zu doubles of scratch, more "
-                "than the %zu it holds (ne (nm + 2) <= %zu); use spoc_energy",
-                c->ne, c->nm, need, cap, cap);
-  HIPCHK(hipSetDevice(c->device));
-  double *scr = nullptr;
-  const int r = energy_scratch(c->device, &scr);
-  if (r == -1) return fail(SPOC_EINVAL, "spoc_energy_noalloc: device %d out of range", c->device);
-  if (r) return fail(SPOC_EHIP, "spoc_energy_noalloc: hipGetSymbolAddress failed");
-  const size_t n = (size_t)c->nm * c->ne;
-  const int nmem = (int)c->mem_e0.size() - 1;
-  const int rc = energy_launch(c->stream, c->ne, c->nm, c->ng, state_ptr(c, 0), state_ptr(c, 1),
-                               state_ptr(c, 2), c->dfields + 3 * n, c->tab.pi2, c->tab.wi,
-                               c->tab.ammo1, c->mesh.deta, c->phys.g0, nmem, c->dmem + 2,
-                               c->dorder, scr, out_dev);
-  if (rc) return fail(SPOC_EHIP, "spoc_energy_noalloc: kernel launch failed");
-  return SPOC_OK;
+  return spoc::guarded([&]() -> int {
+    if (!c || !c->have_state || !c->have_fields)
+      return fail(SPOC_ESTATE, "spoc_energy_noalloc: need mesh, fields and state");
+    if (!c->have_nodes) return fail(SPOC_ESTATE, "spoc_energy_noalloc: mesh set without nodes");
+    if (!out_dev) return fail(SPOC_EINVAL, "spoc_energy_noalloc: null output");
+    const size_t need = (size_t)c->ne * (2 + (size_t)c->nm), cap = energy_scratch_doubles();
+    if (need > cap)
+      return fail(SPOC_EINVAL,
+                  "spoc_energy_noalloc: %d elements of %d modes need %zu doubles of scratch, more "
+                  "than the %zu it holds (ne (nm + 2) <= %zu); use spoc_energy",
+                  c->ne, c->nm, need, cap, cap);
+    HIPCHK(hipSetDevice(c->device));
+    double *scr = nullptr;
+    const int r = energy_scratch(c->device, &scr);
+    if (r == -1) return fail(SPOC_EINVAL, "spoc_energy_noalloc: device %d out of range", c->device);
+    if (r) return fail(SPOC_EHIP, "spoc_energy_noalloc: hipGetSymbolAddress failed");
+    const size_t n = (size_t)c->nm * c->ne;
+    const int nmem = (int)c->mem_e0.size() - 1;
+    const int rc = energy_launch(c->stream, c->ne, c->nm, c->ng, state_ptr(c, 0), state_ptr(c, 1),
+                                 state_ptr(c, 2), c->dfields + 3 * n, c->tab.pi2, c->tab.wi,
+                                 c->tab.ammo1, c->mesh.deta, c->phys.g0, nmem, c->dmem + 2,
+                                 c->dorder, scr, out_dev);
+    if (rc) return fail(SPOC_EHIP, "spoc_energy_noalloc: kernel launch failed");
+    return SPOC_OK;
+  });
 }
 
 int spoc_ensemble_energy(spoc_ctx *c, int nmem, double *ke, double *pe, double *mass) {
-  if (!c || !c->have_state || !c->have_fields)
-    return fail(SPOC_ESTATE, "spoc_ensemble_energy: need mesh, fields and state");
-  if (!c->have_nodes) return fail(SPOC_ESTATE, "spoc_ensemble_energy: mesh set without nodes");
-  if (nmem != (int)c->mem_e0.size() - 1)
-    return fail(SPOC_EINVAL, "spoc_ensemble_energy: nmem=%d, the mesh has %d members", nmem,
-                (int)c->mem_e0.size() - 1);
-  std::vector<double> r(3 * (size_t)nmem);
-  int rc = energies(c, nmem, c->dmem + 2, r.data());
-  if (rc) return rc;
-  int nan_member = -1;
-  for (int m = 0; m < nmem; m++) {
-    ke[m] = r[3 * m];
-    pe[m] = r[3 * m + 1];
-    mass[m] = r[3 * m + 2];
-    if (nan_member < 0 && std::isnan(ke[m])) nan_member = m;
-  }
-  if (nan_member >= 0)
-    return fail(SPOC_ENAN, "spoc_ensemble_energy: the kinetic energy of member %d is NaN",
-                nan_member);
-  return SPOC_OK;
+  return spoc::guarded([&]() -> int {
+    if (!c || !c->have_state || !c->have_fields)
+      return fail(SPOC_ESTATE, "spoc_ensemble_energy: need mesh, fields and state");
+    if (!c->have_nodes) return fail(SPOC_ESTATE, "spoc_ensemble_energy: mesh set without nodes");
+    if (nmem != (int)c->mem_e0.size() - 1)
+      return fail(SPOC_EINVAL, "spoc_ensemble_energy: nmem=%d, the mesh has %d members", nmem,
+                  (int)c->mem_e0.size() - 1);
+    std::vector<double> r(3 * (size_t)nmem);
+    int rc = energies(c, nmem, c->dmem + 2, r.data());
+    if (rc) return rc;
+    int nan_member = -1;
+    for (int m = 0; m < nmem; m++) {
+      ke[m] = r[3 * m];
+      pe[m] = r[3 * m + 1];
+      mass[m] = r[3 * m + 2];
+      if (nan_member < 0 && std::isnan(ke[m])) nan_member = m;
+    }
+    if (nan_member >= 0)
+      return fail(SPOC_ENAN, "spoc_ensemble_energy: the kinetic energy of member %d is NaN",
+                  nan_member);
+    return SPOC_OK;
+  });
 }
 
 int spoc_set_profiling(spoc_ctx *c, int on) {
-  if (!c) return fail(SPOC_EINVAL, "null context");
-  c->prof.on = on != 0;
-  return SPOC_OK;
+  return spoc::guarded([&]() -> int {
+    if (!c) return fail(SPOC_EINVAL, "null context");
+    c->prof.on = on != 0;
+    return SPOC_OK;
+  });
 }
 
 int spoc_last_timing(spoc_ctx *c, double *total_ms, int *nkernels, char *names, int len,
                      double *ms, int *launches) {
-  if (!c) return fail(SPOC_EINVAL, "null context");
-  if (total_ms) *total_ms = c->last_ms;
-  if (!nkernels) return SPOC_OK;
-  int cap = *nkernels, n = (int)c->knames.size();
-  *nkernels = n;
-  int pos = 0;
-  for (int i = 0; i < n && i < cap; i++) {
-    if (ms) ms[i] = c->kms[i];
-    if (launches) launches[i] = c->kcount[i];
-    if (names) {
-      int l = (int)c->knames[i].size();
-      if (pos + l + 1 > len) break;
-      std::memcpy(names + pos, c->knames[i].c_str(), l + 1);
-      pos += l + 1;
+  return spoc::guarded([&]() -> int {
+    if (!c) return fail(SPOC_EINVAL, "null context");
+    if (total_ms) *total_ms = c->last_ms;
+    if (!nkernels) return SPOC_OK;
+    int cap = *nkernels, n = (int)c->knames.size();
+    *nkernels = n;
+    int pos = 0;
+    for (int i = 0; i < n && i < cap; i++) {
+      if (ms) ms[i] = c->kms[i];
+      if (launches) launches[i] = c->kcount[i];
+      if (names) {
+        int l = (int)c->knames[i].size();
+        if (pos + l + 1 > len) break;
+        std::memcpy(names + pos, c->knames[i].c_str(), l + 1);
+        pos += l + 1;
+      }
     }
-  }
-  return SPOC_OK;
+    return SPOC_OK;
+  });
 }
 
 }  // extern "C"

@@ -414,148 +414,158 @@ extern "C" {
 
 int spoc_amr_create(spoc_amr **out, int nn, int ne, int nf, const double *xgr, const double *ygr,
                     const int *in, const int *tseg, const int *itm) {
-  if (!out || nn < 3 || ne < 1 || nf < 1 || !xgr || !ygr || !in || !tseg || !itm)
-    return spoc::report(SPOC_EINVAL, "spoc_amr_create: bad arguments");
-  *out = nullptr;
-  for (int i = 0; i < 3 * ne; i++)
-    if (in[i] < 1 || in[i] > nn || itm[i] < 1 || itm[i] > nf)
-      return spoc::report(SPOC_EINVAL, "spoc_amr_create: in or itm out of range at %d", i);
-  for (int i = 0; i < 2 * nf; i++)
-    if (tseg[i] < 0 || tseg[i] > ne || (i % 2 == 0 && tseg[i] == 0))
-      return spoc::report(SPOC_EINVAL, "spoc_amr_create: tseg out of range at %d", i);
-  spoc_amr *A = new spoc_amr();
-  A->reserve_tri(ne);
-  A->reserve_node(nn);
-  // init_adap (src/adapt.f:380-449)
-  for (int i = 1; i <= ne; i++) {
-    A->tlevel[i] = 0;
-    A->torigi[i] = 0;
-    A->tutili[i] = 1;
-    A->tdoubl[i] = 0;
-    for (int j = 1; j <= 3; j++) A->IN(j, i) = in[3 * (i - 1) + j - 1];
-  }
-  A->tabsol.assign((size_t)ne + 1, 0);
-  for (int i = 1; i <= ne; i++) A->tabsol[i] = i;
-  A->ne_ab = A->ne = ne;
-  A->nn_ab = nn;
-  for (int i = 1; i <= nn; i++) {
-    A->x[i] = xgr[i - 1];
-    A->y[i] = ygr[i - 1];
-  }
-  A->ndoub = 0;
-  for (int i = 1; i <= ne; i++)
-    for (int j = 1; j <= 3; j++) {
-      const int f = itm[3 * (i - 1) + j - 1], s1 = tseg[2 * (f - 1)], s2 = tseg[2 * (f - 1) + 1];
-      int t = -1;
-      if (s1 == i) t = s2;
-      if (s2 == i) t = s1;
-      if (t < 0) {
-        delete A;
-        return spoc::report(SPOC_EMESH, "spoc_amr_create: face %d of element %d does not list it",
-                            f, i);
-      }
-      A->TMT(j, i) = t;
+  return spoc::guarded([&]() -> int {
+    if (!out || nn < 3 || ne < 1 || nf < 1 || !xgr || !ygr || !in || !tseg || !itm)
+      return spoc::report(SPOC_EINVAL, "spoc_amr_create: bad arguments");
+    *out = nullptr;
+    for (int i = 0; i < 3 * ne; i++)
+      if (in[i] < 1 || in[i] > nn || itm[i] < 1 || itm[i] > nf)
+        return spoc::report(SPOC_EINVAL, "spoc_amr_create: in or itm out of range at %d", i);
+    for (int i = 0; i < 2 * nf; i++)
+      if (tseg[i] < 0 || tseg[i] > ne || (i % 2 == 0 && tseg[i] == 0))
+        return spoc::report(SPOC_EINVAL, "spoc_amr_create: tseg out of range at %d", i);
+    spoc_amr *A = new spoc_amr();
+    A->reserve_tri(ne);
+    A->reserve_node(nn);
+    // init_adap (src/adapt.f:380-449)
+    for (int i = 1; i <= ne; i++) {
+      A->tlevel[i] = 0;
+      A->torigi[i] = 0;
+      A->tutili[i] = 1;
+      A->tdoubl[i] = 0;
+      for (int j = 1; j <= 3; j++) A->IN(j, i) = in[3 * (i - 1) + j - 1];
     }
-  *out = A;
-  return 0;
+    A->tabsol.assign((size_t)ne + 1, 0);
+    for (int i = 1; i <= ne; i++) A->tabsol[i] = i;
+    A->ne_ab = A->ne = ne;
+    A->nn_ab = nn;
+    for (int i = 1; i <= nn; i++) {
+      A->x[i] = xgr[i - 1];
+      A->y[i] = ygr[i - 1];
+    }
+    A->ndoub = 0;
+    for (int i = 1; i <= ne; i++)
+      for (int j = 1; j <= 3; j++) {
+        const int f = itm[3 * (i - 1) + j - 1], s1 = tseg[2 * (f - 1)], s2 = tseg[2 * (f - 1) + 1];
+        int t = -1;
+        if (s1 == i) t = s2;
+        if (s2 == i) t = s1;
+        if (t < 0) {
+          delete A;
+          return spoc::report(SPOC_EMESH, "spoc_amr_create: face %d of element %d does not list it",
+                              f, i);
+        }
+        A->TMT(j, i) = t;
+      }
+    *out = A;
+    return 0;
+  });
 }
 
 int spoc_amr_adapt(spoc_amr *A, int ne, const int *fconv, const int *fderaff, int *nn_new,
                    int *ne_new) {
-  if (!A || ne != A->ne || !fconv || !fderaff || !nn_new || !ne_new)
-    return spoc::report(SPOC_EINVAL, "spoc_amr_adapt: bad arguments (ne must be the mesh's %d)",
-                        A ? A->ne : 0);
-  // the flags onto the hierarchy (src/adapt.f:89-101); a green half's
-  // parent is a leaf again
-  std::vector<char> fconv_ab((size_t)A->ne_ab + 1, 0);
-  for (int i = 1; i <= ne; i++) {
-    const int tk = A->tabsol[i];
-    int tk1 = tk;
-    if (tk > A->ne_ab) {
-      tk1 = A->torigi[tk];
-      A->tutili[tk1] = 1;
+  return spoc::guarded([&]() -> int {
+    if (!A || ne != A->ne || !fconv || !fderaff || !nn_new || !ne_new)
+      return spoc::report(SPOC_EINVAL, "spoc_amr_adapt: bad arguments (ne must be the mesh's %d)",
+                          A ? A->ne : 0);
+    // the flags onto the hierarchy (src/adapt.f:89-101); a green half's
+    // parent is a leaf again
+    std::vector<char> fconv_ab((size_t)A->ne_ab + 1, 0);
+    for (int i = 1; i <= ne; i++) {
+      const int tk = A->tabsol[i];
+      int tk1 = tk;
+      if (tk > A->ne_ab) {
+        tk1 = A->torigi[tk];
+        A->tutili[tk1] = 1;
+      }
+      if (fconv[i - 1]) fconv_ab[tk1] = 1;
     }
-    if (fconv[i - 1]) fconv_ab[tk1] = 1;
-  }
-  // deraffiner reads the per-element flags by hierarchy number
-  std::vector<char> fder((size_t)std::max(A->ne_ab, ne) + 1, 0);
-  for (int i = 1; i <= ne; i++) fder[i] = fderaff[i - 1] != 0;
-  // A family goes when its four children are flagged (and not flagged for
-  // refinement).  If one of them has children of its own, the reference
-  // removes it all the same and its children are left without a parent,
-  // overlapping the parent that comes back: refused here, with the
-  // hierarchy as it was.
-  for (int i = 1; i <= A->ne_ab; i++) {
-    if (A->TEN(1, i) <= 0) continue;
-    int k = 0, inner = 0;
-    for (int j = 1; j <= 4; j++) {
-      const int c = A->TEN(j, i);
-      if (fder[c] && !fconv_ab[c]) k++;
-      if (A->TEN(1, c) > 0) inner++;
+    // deraffiner reads the per-element flags by hierarchy number
+    std::vector<char> fder((size_t)std::max(A->ne_ab, ne) + 1, 0);
+    for (int i = 1; i <= ne; i++) fder[i] = fderaff[i - 1] != 0;
+    // A family goes when its four children are flagged (and not flagged for
+    // refinement).  If one of them has children of its own, the reference
+    // removes it all the same and its children are left without a parent,
+    // overlapping the parent that comes back: refused here, with the
+    // hierarchy as it was.
+    for (int i = 1; i <= A->ne_ab; i++) {
+      if (A->TEN(1, i) <= 0) continue;
+      int k = 0, inner = 0;
+      for (int j = 1; j <= 4; j++) {
+        const int c = A->TEN(j, i);
+        if (fder[c] && !fconv_ab[c]) k++;
+        if (A->TEN(1, c) > 0) inner++;
+      }
+      if (k == 4 && inner) {
+        // (a green half's parent was made a leaf again above: undo that)
+        for (int e = 1; e <= ne; e++)
+          if (A->tabsol[e] > A->ne_ab) A->tutili[A->torigi[A->tabsol[e]]] = 0;
+        return spoc::report(SPOC_EINVAL, "spoc_amr_adapt: the coarsening flags remove the children "
+                            "of %d, of which one is refined", i);
+      }
     }
-    if (k == 4 && inner) {
-      // (a green half's parent was made a leaf again above: undo that)
-      for (int e = 1; e <= ne; e++)
-        if (A->tabsol[e] > A->ne_ab) A->tutili[A->torigi[A->tabsol[e]]] = 0;
-      return spoc::report(SPOC_EINVAL, "spoc_amr_adapt: the coarsening flags remove the children "
-                          "of %d, of which one is refined", i);
-    }
-  }
-  deraffiner(*A, fconv_ab, fder);
-  int nnnew = 0, nemaxnew = 0, rc;
-  // (after an error here the hierarchy is half-changed: destroy it)
-  if ((rc = adapmaille(*A, fconv_ab, nnnew, nemaxnew))) return rc;
-  if ((rc = protridouble(*A, nnnew, nemaxnew))) return rc;
-  elimpptri(*A, nnnew, nemaxnew);
-  *nn_new = A->nn_ab;
-  *ne_new = A->ne;
-  return 0;
+    deraffiner(*A, fconv_ab, fder);
+    int nnnew = 0, nemaxnew = 0, rc;
+    // (after an error here the hierarchy is half-changed: destroy it)
+    if ((rc = adapmaille(*A, fconv_ab, nnnew, nemaxnew))) return rc;
+    if ((rc = protridouble(*A, nnnew, nemaxnew))) return rc;
+    elimpptri(*A, nnnew, nemaxnew);
+    *nn_new = A->nn_ab;
+    *ne_new = A->ne;
+    return 0;
+  });
 }
 
 int spoc_amr_mesh(const spoc_amr *A, double *xgr, double *ygr, int *in, int *tabsol) {
-  if (!A || !xgr || !ygr || !in) return spoc::report(SPOC_EINVAL, "spoc_amr_mesh: bad arguments");
-  spoc_amr &B = const_cast<spoc_amr &>(*A);
-  for (int i = 1; i <= A->nn_ab; i++) {
-    xgr[i - 1] = A->x[i];
-    ygr[i - 1] = A->y[i];
-  }
-  for (int i = 1; i <= A->ne; i++) {
-    const int t = A->tabsol[i];
-    for (int j = 1; j <= 3; j++) in[3 * (i - 1) + j - 1] = B.IN(j, t);
-    if (tabsol) tabsol[i - 1] = t;
-  }
-  return 0;
+  return spoc::guarded([&]() -> int {
+    if (!A || !xgr || !ygr || !in) return spoc::report(SPOC_EINVAL, "spoc_amr_mesh: bad arguments");
+    spoc_amr &B = const_cast<spoc_amr &>(*A);
+    for (int i = 1; i <= A->nn_ab; i++) {
+      xgr[i - 1] = A->x[i];
+      ygr[i - 1] = A->y[i];
+    }
+    for (int i = 1; i <= A->ne; i++) {
+      const int t = A->tabsol[i];
+      for (int j = 1; j <= 3; j++) in[3 * (i - 1) + j - 1] = B.IN(j, t);
+      if (tabsol) tabsol[i - 1] = t;
+    }
+    return 0;
+  });
 }
 
 int spoc_amr_levels(const spoc_amr *A, int *level) {
-  if (!A || !level) return spoc::report(SPOC_EINVAL, "spoc_amr_levels: bad arguments");
-  for (int i = 1; i <= A->ne; i++) {
-    const int t = A->tabsol[i];
-    // a green half is at its parent's level
-    level[i - 1] = A->tlevel[t > A->ne_ab ? A->torigi[t] : t];
-  }
-  return 0;
+  return spoc::guarded([&]() -> int {
+    if (!A || !level) return spoc::report(SPOC_EINVAL, "spoc_amr_levels: bad arguments");
+    for (int i = 1; i <= A->ne; i++) {
+      const int t = A->tabsol[i];
+      // a green half is at its parent's level
+      level[i - 1] = A->tlevel[t > A->ne_ab ? A->torigi[t] : t];
+    }
+    return 0;
+  });
 }
 
 void spoc_amr_destroy(spoc_amr *A) { delete A; }
 
 int spoc_amr_inter_only_new(int ne, const int *in, int nenew, const int *innew, int *tinter) {
-  if (ne < 0 || nenew < 0 || (ne && !in) || (nenew && (!innew || !tinter)))
-    return spoc::report(SPOC_EINVAL, "spoc_amr_inter_only_new: bad arguments");
-  // src/adapt.f:1233-1258: the old triangles, in order, that reappear with
-  // the same nodes in the same order as the next unmatched new triangle
-  for (int i = 0; i < nenew; i++) tinter[i] = 0;
-  int tk = 1;
-  for (int i = 1; i <= ne && tk <= nenew; i++) {
-    int k = 0;
-    for (int j = 0; j < 3; j++)
-      if (in[3 * (i - 1) + j] == innew[3 * (tk - 1) + j]) k++;
-    if (k == 3) {
-      tinter[tk - 1] = i;
-      tk++;
+  return spoc::guarded([&]() -> int {
+    if (ne < 0 || nenew < 0 || (ne && !in) || (nenew && (!innew || !tinter)))
+      return spoc::report(SPOC_EINVAL, "spoc_amr_inter_only_new: bad arguments");
+    // src/adapt.f:1233-1258: the old triangles, in order, that reappear with
+    // the same nodes in the same order as the next unmatched new triangle
+    for (int i = 0; i < nenew; i++) tinter[i] = 0;
+    int tk = 1;
+    for (int i = 1; i <= ne && tk <= nenew; i++) {
+      int k = 0;
+      for (int j = 0; j < 3; j++)
+        if (in[3 * (i - 1) + j] == innew[3 * (tk - 1) + j]) k++;
+      if (k == 3) {
+        tinter[tk - 1] = i;
+        tk++;
+      }
     }
-  }
-  return 0;
+    return 0;
+  });
 }
 
 }  // extern "C"

@@ -83,92 +83,102 @@ extern "C" {
 
 int spoc_state_file_read(const char *path, int nm, int ne, int ld, double *time, double *u0,
                          double *v0, double *h0, double *w0, double *z0, double *hb) {
-  if (!path || nm <= 0 || ne <= 0 || ld < nm || !time)
-    return spoc::report(SPOC_EINVAL, "spoc_state_file_read: bad arguments");
-  File F(path, "rb");
-  if (!F.f) return spoc::report(SPOC_EFILE, "%s: %s", path, std::strerror(errno));
-  if (read_record(F.f, time, 8))
-    return spoc::report(SPOC_EFILE, "%s: record 1 is not one 8-byte time", path);
-  std::vector<double> buf((size_t)nm * ne);
-  double *out[6] = {u0, v0, h0, w0, z0, hb};
-  int rec = 2;
-  for (double *o : out) {
-    if (read_record(F.f, buf.data(), 8 * (uint64_t)buf.size()))
-      return spoc::report(SPOC_EFILE, "%s: record %d is not %d x %d doubles", path, rec, nm, ne);
-    rec++;
-    if (!o) continue;
-    for (int e = 0; e < ne; e++)
-      for (int m = 0; m < nm; m++) o[(size_t)ld * e + m] = buf[(size_t)nm * e + m];
-  }
-  return SPOC_OK;
+  return spoc::guarded([&]() -> int {
+    if (!path || nm <= 0 || ne <= 0 || ld < nm || !time)
+      return spoc::report(SPOC_EINVAL, "spoc_state_file_read: bad arguments");
+    File F(path, "rb");
+    if (!F.f) return spoc::report(SPOC_EFILE, "%s: %s", path, std::strerror(errno));
+    if (read_record(F.f, time, 8))
+      return spoc::report(SPOC_EFILE, "%s: record 1 is not one 8-byte time", path);
+    std::vector<double> buf((size_t)nm * ne);
+    double *out[6] = {u0, v0, h0, w0, z0, hb};
+    int rec = 2;
+    for (double *o : out) {
+      if (read_record(F.f, buf.data(), 8 * (uint64_t)buf.size()))
+        return spoc::report(SPOC_EFILE, "%s: record %d is not %d x %d doubles", path, rec, nm, ne);
+      rec++;
+      if (!o) continue;
+      for (int e = 0; e < ne; e++)
+        for (int m = 0; m < nm; m++) o[(size_t)ld * e + m] = buf[(size_t)nm * e + m];
+    }
+    return SPOC_OK;
+  });
 }
 
 int spoc_state_file_write(const char *path, int nm, int ne, int ld, double time,
                           const double *u0, const double *v0, const double *h0, const double *w0,
                           const double *z0, const double *hb) {
-  const double *in[6] = {u0, v0, h0, w0, z0, hb};
-  if (!path || nm <= 0 || ne <= 0 || ld < nm)
-    return spoc::report(SPOC_EINVAL, "spoc_state_file_write: bad arguments");
-  for (const double *a : in)
-    if (!a) return spoc::report(SPOC_EINVAL, "spoc_state_file_write: null field");
-  File F(path, "wb");
-  if (!F.f) return spoc::report(SPOC_EFILE, "%s: %s", path, std::strerror(errno));
-  if (write_record(F.f, &time, 8)) return spoc::report(SPOC_EFILE, "%s: write failed", path);
-  std::vector<double> buf((size_t)nm * ne);
-  for (const double *a : in) {
-    for (int e = 0; e < ne; e++)
-      for (int m = 0; m < nm; m++) buf[(size_t)nm * e + m] = a[(size_t)ld * e + m];
-    if (write_record(F.f, buf.data(), 8 * (uint64_t)buf.size()))
-      return spoc::report(SPOC_EFILE, "%s: write failed (records of 2 GiB or more are not "
-                          "supported)", path);
-  }
-  return std::fflush(F.f) == 0 ? SPOC_OK : spoc::report(SPOC_EFILE, "%s: write failed", path);
+  return spoc::guarded([&]() -> int {
+    const double *in[6] = {u0, v0, h0, w0, z0, hb};
+    if (!path || nm <= 0 || ne <= 0 || ld < nm)
+      return spoc::report(SPOC_EINVAL, "spoc_state_file_write: bad arguments");
+    for (const double *a : in)
+      if (!a) return spoc::report(SPOC_EINVAL, "spoc_state_file_write: null field");
+    File F(path, "wb");
+    if (!F.f) return spoc::report(SPOC_EFILE, "%s: %s", path, std::strerror(errno));
+    if (write_record(F.f, &time, 8)) return spoc::report(SPOC_EFILE, "%s: write failed", path);
+    std::vector<double> buf((size_t)nm * ne);
+    for (const double *a : in) {
+      for (int e = 0; e < ne; e++)
+        for (int m = 0; m < nm; m++) buf[(size_t)nm * e + m] = a[(size_t)ld * e + m];
+      if (write_record(F.f, buf.data(), 8 * (uint64_t)buf.size()))
+        return spoc::report(SPOC_EFILE, "%s: write failed (records of 2 GiB or more are not "
+                            "supported)", path);
+    }
+    return std::fflush(F.f) == 0 ? SPOC_OK : spoc::report(SPOC_EFILE, "%s: write failed", path);
+  });
 }
 
 int spoc_mesh_file_size(const char *path, int *nn, int *ne) {
-  if (!path || !nn || !ne) return spoc::report(SPOC_EINVAL, "spoc_mesh_file_size: bad arguments");
-  File F(path, "r");
-  if (!F.f) return spoc::report(SPOC_EFILE, "%s: %s", path, std::strerror(errno));
-  if (!read_int(F.f, nn) || !read_int(F.f, ne) || *nn <= 0 || *ne <= 0)
-    return spoc::report(SPOC_EFILE, "%s: no \"nn ne\" header", path);
-  return SPOC_OK;
+  return spoc::guarded([&]() -> int {
+    if (!path || !nn || !ne) return spoc::report(SPOC_EINVAL, "spoc_mesh_file_size: bad arguments");
+    File F(path, "r");
+    if (!F.f) return spoc::report(SPOC_EFILE, "%s: %s", path, std::strerror(errno));
+    if (!read_int(F.f, nn) || !read_int(F.f, ne) || *nn <= 0 || *ne <= 0)
+      return spoc::report(SPOC_EFILE, "%s: no \"nn ne\" header", path);
+    return SPOC_OK;
+  });
 }
 
 int spoc_mesh_file_read(const char *path, int nn, int ne, double scx, double scy, double *x,
                         double *y, int *in) {
-  if (!path || nn <= 0 || ne <= 0 || !x || !y || !in)
-    return spoc::report(SPOC_EINVAL, "spoc_mesh_file_read: bad arguments");
-  File F(path, "r");
-  if (!F.f) return spoc::report(SPOC_EFILE, "%s: %s", path, std::strerror(errno));
-  int a, b, idx;
-  if (!read_int(F.f, &a) || !read_int(F.f, &b) || a != nn || b != ne)
-    return spoc::report(SPOC_EFILE, "%s: header is not \"%d %d\"", path, nn, ne);
-  for (int i = 0; i < nn; i++)  // the index column is read and ignored, as the reference does
-    if (!read_int(F.f, &idx) || !read_real(F.f, &x[i]) || !read_real(F.f, &y[i]))
-      return spoc::report(SPOC_EFILE, "%s: node line %d unreadable", path, i + 1);
-  for (int e = 0; e < ne; e++)
-    if (!read_int(F.f, &idx) || !read_int(F.f, &in[3 * e]) || !read_int(F.f, &in[3 * e + 1]) ||
-        !read_int(F.f, &in[3 * e + 2]))
-      return spoc::report(SPOC_EFILE, "%s: triangle line %d unreadable", path, e + 1);
-  for (int i = 0; i < nn; i++) {  // src/cmal_s.f:31-34
-    x[i] = x[i] * scx;
-    y[i] = y[i] * scy;
-  }
-  return SPOC_OK;
+  return spoc::guarded([&]() -> int {
+    if (!path || nn <= 0 || ne <= 0 || !x || !y || !in)
+      return spoc::report(SPOC_EINVAL, "spoc_mesh_file_read: bad arguments");
+    File F(path, "r");
+    if (!F.f) return spoc::report(SPOC_EFILE, "%s: %s", path, std::strerror(errno));
+    int a, b, idx;
+    if (!read_int(F.f, &a) || !read_int(F.f, &b) || a != nn || b != ne)
+      return spoc::report(SPOC_EFILE, "%s: header is not \"%d %d\"", path, nn, ne);
+    for (int i = 0; i < nn; i++)  // the index column is read and ignored, as the reference does
+      if (!read_int(F.f, &idx) || !read_real(F.f, &x[i]) || !read_real(F.f, &y[i]))
+        return spoc::report(SPOC_EFILE, "%s: node line %d unreadable", path, i + 1);
+    for (int e = 0; e < ne; e++)
+      if (!read_int(F.f, &idx) || !read_int(F.f, &in[3 * e]) || !read_int(F.f, &in[3 * e + 1]) ||
+          !read_int(F.f, &in[3 * e + 2]))
+        return spoc::report(SPOC_EFILE, "%s: triangle line %d unreadable", path, e + 1);
+    for (int i = 0; i < nn; i++) {  // src/cmal_s.f:31-34
+      x[i] = x[i] * scx;
+      y[i] = y[i] * scy;
+    }
+    return SPOC_OK;
+  });
 }
 
 int spoc_mesh_file_write(const char *path, int nn, int ne, const double *x, const double *y,
                          const int *in, double scx, double scy) {
-  if (!path || nn <= 0 || ne <= 0 || !x || !y || !in)
-    return spoc::report(SPOC_EINVAL, "spoc_mesh_file_write: bad arguments");
-  File F(path, "w");
-  if (!F.f) return spoc::report(SPOC_EFILE, "%s: %s", path, std::strerror(errno));
-  std::fprintf(F.f, "%d %d\n", nn, ne);
-  for (int i = 0; i < nn; i++)  // 17 significant digits: read back to the same double
-    std::fprintf(F.f, "%d %.17g %.17g\n", i + 1, x[i] / scx, y[i] / scy);
-  for (int e = 0; e < ne; e++)
-    std::fprintf(F.f, "%d %d %d %d\n", e + 1, in[3 * e], in[3 * e + 1], in[3 * e + 2]);
-  return std::fflush(F.f) == 0 ? SPOC_OK : spoc::report(SPOC_EFILE, "%s: write failed", path);
+  return spoc::guarded([&]() -> int {
+    if (!path || nn <= 0 || ne <= 0 || !x || !y || !in)
+      return spoc::report(SPOC_EINVAL, "spoc_mesh_file_write: bad arguments");
+    File F(path, "w");
+    if (!F.f) return spoc::report(SPOC_EFILE, "%s: %s", path, std::strerror(errno));
+    std::fprintf(F.f, "%d %d\n", nn, ne);
+    for (int i = 0; i < nn; i++)  // 17 significant digits: read back to the same double
+      std::fprintf(F.f, "%d %.17g %.17g\n", i + 1, x[i] / scx, y[i] / scy);
+    for (int e = 0; e < ne; e++)
+      std::fprintf(F.f, "%d %d %d %d\n", e + 1, in[3 * e], in[3 * e + 1], in[3 * e + 2]);
+    return std::fflush(F.f) == 0 ? SPOC_OK : spoc::report(SPOC_EFILE, "%s: write failed", path);
+  });
 }
 
 }  // extern "C"

@@ -187,41 +187,43 @@ void face_order(int ne, int nf, const int *itm, const std::vector<int> &perm,
 extern "C" int spoc_mesh_order(int mode, int nmem, const int *ne_m, const int *nf_m,
                                const int *itm, const int *tseg, const int *ssit,
                                const double *vect, int *perm) {
-  if (nmem < 1 || !ne_m || !nf_m || !itm || !tseg || !ssit || !vect || !perm ||
-      (mode != SPOC_ORDER_HILBERT && mode != SPOC_ORDER_ZCURVE))
-    return spoc::report(SPOC_EINVAL, "spoc_mesh_order: bad arguments");
-  long ne = 0, nf = 0;
-  std::vector<int> e0(nmem + 1, 0);
-  for (int m = 0; m < nmem; m++) {
-    if (ne_m[m] <= 0 || nf_m[m] <= 0)
-      return spoc::report(SPOC_EINVAL, "spoc_mesh_order: member %d: ne=%d nf=%d", m, ne_m[m],
-                          nf_m[m]);
-    ne += ne_m[m];
-    nf += nf_m[m];
-    e0[m + 1] = (int)ne;
-    if (ne > (1L << 30) || nf > (1L << 30))
-      return spoc::report(SPOC_EINVAL, "spoc_mesh_order: mesh too large");
-  }
-  // the members' own 1-based numbering, shifted to the union's
-  std::vector<int> uitm(3 * ne), utseg(2 * nf);
-  long eo = 0, fo = 0;
-  for (int m = 0; m < nmem; m++) {
-    for (long i = 3 * eo; i < 3 * (eo + ne_m[m]); i++) {
-      if (itm[i] < 1 || itm[i] > nf_m[m])
-        return spoc::report(SPOC_EINVAL, "spoc_mesh_order: itm out of range in member %d", m);
-      uitm[i] = itm[i] + (int)fo;
+  return spoc::guarded([&]() -> int {
+    if (nmem < 1 || !ne_m || !nf_m || !itm || !tseg || !ssit || !vect || !perm ||
+        (mode != SPOC_ORDER_HILBERT && mode != SPOC_ORDER_ZCURVE))
+      return spoc::report(SPOC_EINVAL, "spoc_mesh_order: bad arguments");
+    long ne = 0, nf = 0;
+    std::vector<int> e0(nmem + 1, 0);
+    for (int m = 0; m < nmem; m++) {
+      if (ne_m[m] <= 0 || nf_m[m] <= 0)
+        return spoc::report(SPOC_EINVAL, "spoc_mesh_order: member %d: ne=%d nf=%d", m, ne_m[m],
+                            nf_m[m]);
+      ne += ne_m[m];
+      nf += nf_m[m];
+      e0[m + 1] = (int)ne;
+      if (ne > (1L << 30) || nf > (1L << 30))
+        return spoc::report(SPOC_EINVAL, "spoc_mesh_order: mesh too large");
     }
-    for (long i = 2 * fo; i < 2 * (fo + nf_m[m]); i++) {
-      if (tseg[i] < 0 || tseg[i] > ne_m[m] || (i % 2 == 0 && tseg[i] == 0))
-        return spoc::report(SPOC_EINVAL, "spoc_mesh_order: tseg out of range in member %d", m);
-      utseg[i] = tseg[i] == 0 ? 0 : tseg[i] + (int)eo;
+    // the members' own 1-based numbering, shifted to the union's
+    std::vector<int> uitm(3 * ne), utseg(2 * nf);
+    long eo = 0, fo = 0;
+    for (int m = 0; m < nmem; m++) {
+      for (long i = 3 * eo; i < 3 * (eo + ne_m[m]); i++) {
+        if (itm[i] < 1 || itm[i] > nf_m[m])
+          return spoc::report(SPOC_EINVAL, "spoc_mesh_order: itm out of range in member %d", m);
+        uitm[i] = itm[i] + (int)fo;
+      }
+      for (long i = 2 * fo; i < 2 * (fo + nf_m[m]); i++) {
+        if (tseg[i] < 0 || tseg[i] > ne_m[m] || (i % 2 == 0 && tseg[i] == 0))
+          return spoc::report(SPOC_EINVAL, "spoc_mesh_order: tseg out of range in member %d", m);
+        utseg[i] = tseg[i] == 0 ? 0 : tseg[i] + (int)eo;
+      }
+      eo += ne_m[m];
+      fo += nf_m[m];
     }
-    eo += ne_m[m];
-    fo += nf_m[m];
-  }
-  std::vector<int> p;
-  spoc::locality_order((int)ne, (int)nf, uitm.data(), utseg.data(), ssit, vect, e0,
-                       mode == SPOC_ORDER_HILBERT, p);
-  for (long i = 0; i < ne; i++) perm[i] = p[i] + 1;
-  return SPOC_OK;
+    std::vector<int> p;
+    spoc::locality_order((int)ne, (int)nf, uitm.data(), utseg.data(), ssit, vect, e0,
+                         mode == SPOC_ORDER_HILBERT, p);
+    for (long i = 0; i < ne; i++) perm[i] = p[i] + 1;
+    return SPOC_OK;
+  });
 }

@@ -67,3 +67,45 @@ def test_adaptive_driver_links_the_dropin_cal_conv():
     undefined = {ln.split()[-1] for ln in syms.splitlines() if ln.strip().startswith("U ")}
     assert {"cal_conv_", "cal_conv_reference_", "cal_var_", "checkconv_"} <= defined
     assert "spoc_cal_conv" in undefined and "spoc_rhs" in undefined
+
+
+_ENOMEM_SCRIPT = r'''
+import ctypes as C, resource, sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+import spoc_amd
+from spoc_amd import configs, mesh
+lib = spoc_amd.load_library()
+x, y, tri = configs.structured_mesh(401, 401, 0.0, 1e6, 0.0, 1e6)   # 320,000 triangles
+tri = np.asfortranarray(tri, dtype=np.int32)
+nn, ne = len(x), tri.shape[1]
+i32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+f64 = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+z = lambda *s, t=np.float64: np.zeros(s, t, order="F")
+out = [z(3, ne, t=np.int32), z(3, ne, t=np.int32), z(2, 3 * ne, t=np.int32),
+       z(2, 3 * ne, t=np.int32), z(2, 3 * ne, t=np.int32), z(2, 3 * ne), z(5, ne), z(2, ne),
+       z(ne, t=np.int32), z(mesh.LDM, ne), z(ne)]
+nf = C.c_int()
+args = [nn, ne, i32(tri), f64(np.ascontiguousarray(x)), f64(np.ascontiguousarray(y)), 3,
+        C.c_double(1e-4), C.c_double(0.0), mesh.LDM, 1, C.byref(nf)]
+args += [i32(a) if a.dtype == np.int32 else f64(a) for a in out]
+vm = int([l for l in open("/proc/self/status") if l.startswith("VmSize")][0].split()[1]) * 1024
+resource.setrlimit(resource.RLIMIT_AS, (vm + (8 << 20), resource.RLIM_INFINITY))
+rc = lib.spoc_mesh_build(*args)
+resource.setrlimit(resource.RLIMIT_AS, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+print(rc, lib.spoc_last_error().decode())
+'''
+
+
+def test_host_allocation_failure_is_an_error_code():
+    """An entry point whose host allocations fail returns SPOC_ENOMEM with a
+    message and the caller's process goes on (no C++ exception crosses the
+    C-ABI).  The mesh builder's maps for 320,000 triangles are asked for
+    under an address-space limit 8 MiB above what the process holds."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, "-c", _ENOMEM_SCRIPT, ROOT], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    rc, msg = r.stdout.strip().split(" ", 1)
+    assert int(rc) == spoc_amd.ENOMEM and "memory" in msg, r.stdout
