@@ -393,7 +393,8 @@ int spoc_calh2(int nc, int ng, const double *wi, const double *xi, const double 
  * reference runs it when cal_conv's conv is true.  *nn_new, *ne_new: the
  * new mesh's sizes.  The per-element fderaff is read by hierarchy number,
  * as the reference's call does (src/adapt.f:117): numbers past ne count as
- * false.
+ * false.  After an error other than SPOC_EINVAL for the arguments (e.g.
+ * SPOC_ENOMEM midway) the hierarchy is not usable: destroy it.
  *
  * spoc_amr_mesh: the current mesh, xgr, ygr (nn_new), in (3, ne_new),
  * 1-based, CCW when the input mesh was; tabsol (ne_new, may be NULL) the

@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cmath>
 #include <set>
+#include <memory>
 #include <unordered_map>
 #include <vector>
 
@@ -424,7 +425,7 @@ int spoc_amr_create(spoc_amr **out, int nn, int ne, int nf, const double *xgr, c
     for (int i = 0; i < 2 * nf; i++)
       if (tseg[i] < 0 || tseg[i] > ne || (i % 2 == 0 && tseg[i] == 0))
         return spoc::report(SPOC_EINVAL, "spoc_amr_create: tseg out of range at %d", i);
-    spoc_amr *A = new spoc_amr();
+    std::unique_ptr<spoc_amr> A(new spoc_amr());  // (freed on every early return)
     A->reserve_tri(ne);
     A->reserve_node(nn);
     // init_adap (src/adapt.f:380-449)
@@ -450,14 +451,12 @@ int spoc_amr_create(spoc_amr **out, int nn, int ne, int nf, const double *xgr, c
         int t = -1;
         if (s1 == i) t = s2;
         if (s2 == i) t = s1;
-        if (t < 0) {
-          delete A;
+        if (t < 0)
           return spoc::report(SPOC_EMESH, "spoc_amr_create: face %d of element %d does not list it",
                               f, i);
-        }
         A->TMT(j, i) = t;
       }
-    *out = A;
+    *out = A.release();
     return 0;
   });
 }
