@@ -104,6 +104,8 @@ def test_host_allocation_failure_is_an_error_code():
     under an address-space limit 8 MiB above what the process holds."""
     import subprocess
     import sys
+    if os.environ.get("SPOC_AMD_LIB"):
+        pytest.skip("the sanitizer build's allocator reports running out and stops instead")
     r = subprocess.run([sys.executable, "-c", _ENOMEM_SCRIPT, ROOT], capture_output=True,
                        text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
