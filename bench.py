@@ -627,8 +627,16 @@ def main():
     ctx.sync()
     wall = time.perf_counter() - t0
     s32 = ctx.download()
+    # per-kernel times (HIP events around each launch; eager launches)
+    phase("mixed path: per-kernel timing")
+    ctx.upload(*state0)
+    ctx.set_profiling(True)
+    kprof = min(K, 20)
+    ctx.rk4(variant, dt, kprof, precision="fp32")
+    dev_ms, kern = ctx.timing()
+    ctx.set_profiling(False)
     per_stage = None
-    if variant == "lin" and ne <= 4096:
+    if "lin_persistent" in kern:
         # small linear meshes run every step in one persistent launch; the
         # same steps launched per stage (HIP graphs), for comparison
         ctx.set_persistent(False)
@@ -641,14 +649,6 @@ def main():
         ctx.sync()
         per_stage = 1e3 * (time.perf_counter() - t1) / K
         ctx.set_persistent(True)
-    # per-kernel times (HIP events around each launch; eager launches)
-    phase("mixed path: per-kernel timing")
-    ctx.upload(*state0)
-    ctx.set_profiling(True)
-    kprof = min(K, 20)
-    ctx.rk4(variant, dt, kprof, precision="fp32")
-    dev_ms, kern = ctx.timing()
-    ctx.set_profiling(False)
     dof = 4 * ne * nm * 3
     value = dof * K / wall
     out = {

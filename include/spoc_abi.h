@@ -210,15 +210,17 @@ int spoc_set_graph(spoc_ctx *ctx, int on);
 
 /*
  * Run the linear variant's RK4 steps on the FP32-MFMA path in one
- * persistent launch (on = 1, the default) where the mesh has at most 4096
- * elements (256 blocks of 16) and the central flux is used: the launch keeps
- * every element block on its wave for all the steps, and before each stage
- * a block waits only for its neighbouring blocks, in place of four launches
- * per step (with
- * profiling on, spoc_last_timing names it "lin_persistent").  Results are
- * identical bit for bit to the launch-per-stage path.  The wait is bounded: if it runs out, spoc_rk4_steps
- * returns SPOC_EHIP.  No reference counterpart (the reference's RK4 loop,
- * src/shwwnl.f:233-284, calls the right-hand side once per stage).
+ * persistent launch (on = 1, the default) on small meshes, where that is
+ * faster than a launch per stage: at most 1,536 elements at p <= 3, 1,024
+ * at p = 4, 768 at p = 5 (none above), with the central flux.  The launch
+ * keeps every element block on its wave for all the steps, and before each
+ * stage a block waits only for its neighbouring blocks, in place of four
+ * launches per step (with profiling on, spoc_last_timing names it
+ * "lin_persistent").  Results are identical bit for bit to the
+ * launch-per-stage path.  The wait is bounded: if it runs out,
+ * spoc_rk4_steps returns SPOC_EHIP.  No reference counterpart (the
+ * reference's RK4 loop, src/shwwnl.f:233-284, calls the right-hand side
+ * once per stage).
  */
 int spoc_set_persistent(spoc_ctx *ctx, int on);
 

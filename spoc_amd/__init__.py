@@ -540,8 +540,9 @@ class Context:
 
     def set_persistent(self, on: bool) -> None:
         """Run the linear variant's RK4 steps on the mixed path in one
-        persistent launch where the mesh has at most 4096 elements (default
-        on); the results are the same bit for bit."""
+        persistent launch on small meshes (at most 1,536 elements at p <= 3,
+        1,024 at p = 4, 768 at p = 5; default on); the results are the same
+        bit for bit."""
         _check(self.lib.spoc_set_persistent(self._h, int(on)))
 
     def timing(self):

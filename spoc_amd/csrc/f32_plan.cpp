@@ -600,7 +600,7 @@ int f32_rk4_step(F32Plan *p, hipStream_t s, int variant, double dt, TimerHook ho
 // nsteps linear RK4 steps in the persistent kernel, one launch per
 // PERSIST_STEPS steps, after f32_rk4_begin (which made the stage-0 traces).
 // Returns 1, launching nothing, where it does not apply: a nonlinear
-// variant, the Lax-Friedrichs flux, a mesh of more than PERSIST_BLOCKS
+// variant, the Lax-Friedrichs flux, a mesh of more than persist_blocks(nc)
 // blocks.  *abort: the device word that a wave raises when its wait for a
 // neighbouring block runs out (every wave then leaves the launch); the caller reads it
 // once the stream has drained.  hook (may be null) brackets the launches.
@@ -616,7 +616,8 @@ hipEvent_t persist_done[kMaxDevices] = {};
 
 int f32_rk4_persist(F32Plan *p, hipStream_t s, int variant, double dt, int nsteps,
                     const unsigned **abort, TimerHook hook, void *user) {
-  if (variant != 0 || p->lf_alpha != 0.0 || (p->ne + 15) / 16 > PERSIST_BLOCKS || nsteps < 1)
+  if (variant != 0 || p->lf_alpha != 0.0 || (p->ne + 15) / 16 > persist_blocks(p->nc) ||
+      nsteps < 1)
     return 1;
   if (!p->dsync &&
       hipMalloc((void **)&p->dsync, PERSIST_SYNC_WORDS * sizeof(unsigned)) != hipSuccess) {

@@ -188,13 +188,19 @@ struct F32State {
 constexpr int TILE_CHOICES[3] = {2, 4, 8};
 constexpr int TILE_FORMS = 3;
 
-// Persistent linear RK4 kernel: meshes of at most PERSIST_BLOCKS blocks of 16
-// elements (every wave of the launch resident at once), and at most
-// PERSIST_STEPS steps per launch (each block's count of finished stages, up
-// to 4 x steps, stays far below 2^32).  Its sync words: the abort word at
-// [1], block b's count at [PERSIST_LINE * (1 + b)], each count on a 128-byte
-// line of its own.
-constexpr int PERSIST_BLOCKS = 256;
+// Persistent linear RK4 kernel: meshes of at most persist_blocks(nc) blocks of
+// 16 elements, where it measured faster than the per-stage launches (5-20%;
+// every wave's release writes back its XCD's L2, a cost that grows with the
+// waves per XCD: at p = 3 on 2,048 elements it is already slower, and at
+// p >= 6 it never won), PERSIST_BLOCKS at most (far fewer waves than the
+// chip holds, so all are resident at once), and at most PERSIST_STEPS steps per launch
+// (each block's count of finished stages, up to 4 x steps, stays far below
+// 2^32).  Its sync words: the abort word at [1], block b's count at
+// [PERSIST_LINE * (1 + b)], each count on a 128-byte line of its own.
+constexpr int PERSIST_BLOCKS = 96;
+__host__ __device__ constexpr int persist_blocks(int nc) {
+  return nc <= 3 ? PERSIST_BLOCKS : nc == 4 ? 64 : nc == 5 ? 48 : 0;
+}
 constexpr int PERSIST_STEPS = 1 << 20;
 constexpr int PERSIST_LINE = 32;
 constexpr int PERSIST_SYNC_WORDS = PERSIST_LINE * (1 + PERSIST_BLOCKS);

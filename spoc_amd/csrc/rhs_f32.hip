@@ -513,9 +513,10 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(lin_wav
 // a stage a wave waits for the blocks around its own to finish the previous
 // one (stage_done / wait_nbrs), and c0 is written once, at the end.  The
 // operator images are staged in LDS once.  Meant for meshes of at most
-// PERSIST_BLOCKS blocks, far fewer waves than the chip holds at one wave per
-// SIMD, so every wave is resident and a wait for another block can end; the
-// register budget is that of one wave per SIMD.
+// persist_blocks(NC) blocks (where it is faster than the per-stage
+// launches), far fewer waves than the chip holds at one wave per SIMD, so
+// every wave is resident and a wait for another block can end; the register
+// budget is that of one wave per SIMD.
 // ---------------------------------------------------------------------------
 constexpr unsigned kSpinLimit = 1u << 20;
 
@@ -1282,15 +1283,18 @@ int launch(const F32Args &A, int phase, const double *in, double *out) {
       break;
     }
     case 6: {  // persistent linear RK4 (A.steps steps), small meshes
-      if (v != 0 || A.lf_alpha != 0.0 || nblk > PERSIST_BLOCKS || !A.sync || A.steps < 1 ||
-          A.steps > PERSIST_STEPS) return -1;
+      if (v != 0 || A.lf_alpha != 0.0 || !A.sync || A.steps < 1 || A.steps > PERSIST_STEPS)
+        return -1;
+      if (nblk > persist_blocks(NC)) return 1;  // (the per-stage launches are faster)
       // one wave per workgroup, so each block's wave has a CU to itself
       // (timed on the 512-element basin: 18.2 us per step, against 19.3 at
       // two and 25.1 at four), and one form (all three equations per wave,
       // which keeps the whole state of a block together); the tile codes of
       // the per-stage kernels do not apply
-      const int r = launch_persist<NC, NG, NGF, 1>(A, nblk);
-      if (r) return r;  // 1: the grid does not fit at once (nothing launched)
+      if constexpr (persist_blocks(NC) > 0) {
+        const int r = launch_persist<NC, NG, NGF, 1>(A, nblk);
+        if (r) return r;  // 1: the grid does not fit at once (nothing launched)
+      }
       break;
     }
     default:

@@ -5,8 +5,9 @@ blocks.  Held to bit-for-bit equality with the launch-per-stage path it
 replaces (the reference's loop, src/shwwnl.f:233-284, one right-hand side
 per stage), against that path in every compiled form and workgroup size,
 across calls of different lengths and time steps, up to the largest mesh it
-takes (4096 elements) and just past it (where it hands over to the
-per-stage path)."""
+takes at p = 3 (1,536 elements; 1,024 at p = 4, 768 at p = 5, none above,
+where the per-stage launches are faster) and just past it (where it hands
+over to the per-stage path)."""
 import sys
 
 import numpy as np
@@ -52,7 +53,7 @@ def same(a, b):
         assert np.array_equal(a[f], b[f]), (f, np.abs(a[f] - b[f]).max())
 
 
-@pytest.mark.parametrize("nc,ng,ngf", [(3, 12, 4), (6, 33, 7), (2, 6, 3)])
+@pytest.mark.parametrize("nc,ng,ngf", [(3, 12, 4), (5, 25, 6), (2, 6, 3)])
 def test_persistent_equals_per_stage_launches(nc, ng, ngf):
     t, ctx = ctx_for(nc, ng, ngf)
     c = configs.random_case(t, 12, 10, seed=31 + nc, scale=0.01)
@@ -66,9 +67,10 @@ def test_persistent_equals_per_stage_launches(nc, ng, ngf):
     ctx.set_tiles("lin", 0, 0, autotune=True)
 
 
-@pytest.mark.parametrize("n", [46, 47])
+@pytest.mark.parametrize("n", [28, 29])
 def test_persistent_at_the_mesh_limit(n):
-    """4050 elements (254 blocks): persistent; 4232: the per-stage path."""
+    """p = 3: 1,458 elements (92 blocks): persistent; 1,568: the per-stage
+    path."""
     t, ctx = ctx_for(3, 12, 4)
     c = configs.random_case(t, n, n, seed=5, scale=0.01)
     say("persistent limit", c["mesh"]["ne"])
@@ -133,11 +135,12 @@ def test_persistent_two_contexts_at_once():
 
 
 def test_persistent_ensemble_union():
-    """Members of uneven size (40 to 1,200 triangles) as one union mesh: the
-    persistent launch advances them all, each member equal bit for bit to
-    the per-stage launches of the union and to its own run alone."""
+    """Members of uneven size (40 to 646 triangles, 1,192 in all) as one
+    union mesh: the persistent launch advances them all, each member equal
+    bit for bit to the per-stage launches of the union and to its own run
+    alone."""
     t, ctx = ctx_for(3, 12, 4)
-    sizes = [(6, 5), (30, 22), (14, 9), (25, 25)]
+    sizes = [(6, 5), (16, 12), (12, 9), (20, 18)]
     mem = [configs.random_case(t, nx, ny, seed=80 + i, scale=0.01)
            for i, (nx, ny) in enumerate(sizes)]
     say("persistent ensemble", [c["mesh"]["ne"] for c in mem])
