@@ -501,30 +501,52 @@ int f32_autotune(F32Plan *p, hipStream_t s, int variant, int *out) {
     (void)hipEventDestroy(e0);
     return -2;
   }
+  // ms of `reps` launches of tile code `code` after one untimed launch
+  auto time_tile = [&](int k, int phase, int code, int reps, float *ms) -> int {
+    a.tile[k] = code;
+    int r = call(p, a, phase, nullptr, nullptr);  // warm-up
+    if (r) return r;
+    (void)hipEventRecord(e0, s);
+    for (int i = 0; i < reps && !r; i++) r = call(p, a, phase, nullptr, nullptr);
+    (void)hipEventRecord(e1, s);
+    if (r) return r;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(ms, e0, e1) != hipSuccess)
+      return -2;
+    return 0;
+  };
   const int nk = variant == 0 ? 1 : 2;
   for (int k = 0; k < nk && !rc; k++) {
     const int phase = variant == 0 ? 3 : 4 + k;
-    float best = 1e30f;
-    int pick = 4;
+    // every candidate once, then the three fastest again, in turns (four
+    // rounds, the best of each kept), so that noise or a clock that settles
+    // during the first pass does not decide between close candidates
+    constexpr int kFinal = 3, kRounds = 4;
+    std::vector<std::pair<float, int>> cand;
     for (int code = 0; code < 16 * TILE_FORMS && !rc; code++) {
       const int wg = code & 15;
       if (wg != TILE_CHOICES[0] && wg != TILE_CHOICES[1] && wg != TILE_CHOICES[2]) continue;
-      a.tile[k] = code;
-      if ((rc = call(p, a, phase, nullptr, nullptr))) break;  // warm-up
-      (void)hipEventRecord(e0, s);
-      for (int r = 0; r < 3 && !rc; r++) rc = call(p, a, phase, nullptr, nullptr);
-      (void)hipEventRecord(e1, s);
       float ms = 0.f;
-      if (rc || hipEventSynchronize(e1) != hipSuccess ||
-          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-        rc = rc ? rc : -2;
-        break;
-      }
-      if (ms < best) {
-        best = ms;
-        pick = code;
-      }
+      if ((rc = time_tile(k, phase, code, 3, &ms))) break;
+      cand.push_back({ms, code});
     }
+    if (rc) break;
+    std::sort(cand.begin(), cand.end());
+    const int nf = std::min<int>(kFinal, (int)cand.size());
+    std::vector<float> best(nf, 1e30f);
+    for (int round = 0; round < kRounds && !rc; round++)
+      for (int i = 0; i < nf && !rc; i++) {
+        float ms = 0.f;
+        if ((rc = time_tile(k, phase, cand[i].second, 4, &ms))) break;
+        best[i] = std::min(best[i], ms);
+      }
+    if (rc) break;
+    int pick = cand.empty() ? 4 : cand[0].second;
+    float pbest = 1e30f;
+    for (int i = 0; i < nf; i++)
+      if (best[i] < pbest) {
+        pbest = best[i];
+        pick = cand[i].second;
+      }
     a.tile[k] = pick;
     p->tuned[variant & 3][k] = pick;
   }
