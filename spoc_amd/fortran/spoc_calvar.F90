@@ -31,7 +31,7 @@ module spoc_abi
   ! status codes of the C-ABI (include/spoc_abi.h)
   integer(c_int), parameter :: SPOC_OK = 0, SPOC_EINVAL = -1, SPOC_EHIP = -2, &
        SPOC_ESTATE = -3, SPOC_ESINGULAR = -4, SPOC_EMESH = -5, SPOC_EFILE = -6, &
-       SPOC_ENAN = -7
+       SPOC_ENAN = -7, SPOC_ENOMEM = -8
   interface
     integer(c_int) function spoc_ctx_create(ctx, device, nc, ng, ngf, wi, wif, pi2, ld_g, pico, &
          ld_f, ld_m, amm, amml, agx, agy, ammo) bind(C, name="spoc_ctx_create")
