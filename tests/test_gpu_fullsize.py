@@ -1,6 +1,7 @@
 """The benchmark configurations at their full sizes (199,362, 99,856 and
-49,952 triangles), where the oracle would take hours, held to properties
-that do not depend on the mesh size:
+49,952 triangles, and the 64-member ensemble of 1,358,532), where the
+oracle would take hours, held to properties that do not depend on the mesh
+size:
 
 - mass (the integral of eta, src/energy.f) is conserved by both paths to
   roundoff (1e-11 relative) over 20 RK4 steps;
@@ -8,7 +9,8 @@ that do not depend on the mesh size:
   norm) of the plain FP64 path, field by field;
 - the FP64 path's results do not depend on the element numbering: the
   caller's numbering and the library's internal Hilbert order give the same
-  bits (every sum keeps the reference's order whatever the numbering)."""
+  bits (every sum keeps the reference's order whatever the numbering);
+- an ensemble member gives the same bits in the union as alone."""
 import sys
 
 import numpy as np
@@ -75,3 +77,55 @@ def test_full_size_properties(name):
         err = mrel(t, m["vect"], out[("hilbert", "fp32")][f], out[("hilbert", "fp64")][f])
         say(name, "field", f, "mixed vs FP64", err)
         assert err <= 1e-6, (f, err)
+
+
+def test_full_size_ensemble():
+    """Configuration 5 whole: 64 refined members, 1,358,532 triangles in one
+    union mesh.  Every member's mass is conserved on both paths, the mixed
+    path is within 1e-6 of the FP64 path member by member, and the smallest
+    and the largest member give the same bits in the union as alone."""
+    t = host_tables(4, 25, 5)
+    mem = configs.config5(t)
+    sizes = [mm["mesh"]["ne"] for mm in mem]
+    dt = mem[0]["dt"]
+    say("full size ensemble", len(mem), "members", sum(sizes), "triangles")
+    P = [mm["phys"] for mm in mem]
+    cat = spoc_amd.Context.concat
+    ctx = spoc_amd.Context(t)
+    ctx.set_ensemble_mesh([mm["mesh"] for mm in mem])
+    ctx.set_fields(cat([q["hb"] for q in P]), P[0]["g0"], cat([q["fc"] for q in P]),
+                   cat([q["w0"] for q in P]), cat([q["z0"] for q in P]), P[0]["mu0"], P[0]["mb0"])
+    state = tuple(cat([mm["state"][f] for mm in mem]) for f in range(3))
+    out = {}
+    for prec in ("fp64", "fp32"):
+        ctx.upload(*state)
+        m0 = ctx.ensemble_energy()[2]
+        ctx.rk4("fs", dt, 20, precision=prec)
+        m1 = ctx.ensemble_energy()[2]
+        out[prec] = ctx.download()
+        drift = np.abs(m1 - m0) / np.abs(m0)
+        say(prec, "largest member mass drift", drift.max())
+        assert drift.max() <= 1e-11, (prec, drift.max())
+    ctx.close()
+    starts = np.concatenate([[0], np.cumsum(sizes)])
+    worst = 0.0
+    for m, mm in enumerate(mem):
+        e0, e1 = starts[m], starts[m + 1]
+        for f in range(3):
+            err = mrel(t, mm["mesh"]["vect"], out["fp32"][f][:, e0:e1], out["fp64"][f][:, e0:e1])
+            worst = max(worst, err)
+    say("mixed vs FP64, worst member and field", worst)
+    assert worst <= 1e-6
+    solo = spoc_amd.Context(t)
+    for m in (int(np.argmin(sizes)), int(np.argmax(sizes))):
+        mm = mem[m]
+        solo.set_mesh(mm["mesh"])
+        p = mm["phys"]
+        solo.set_fields(p["hb"], p["g0"], p["fc"], p["w0"], p["z0"], p["mu0"], p["mb0"])
+        solo.upload(*mm["state"])
+        solo.rk4("fs", dt, 20, precision="fp32")
+        alone = solo.download()
+        e0, e1 = starts[m], starts[m + 1]
+        for f in range(3):
+            assert np.array_equal(out["fp32"][f][:t["nm"], e0:e1], alone[f][:t["nm"]]), (m, f)
+    solo.close()
