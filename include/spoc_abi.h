@@ -218,9 +218,11 @@ int spoc_set_graph(spoc_ctx *ctx, int on);
  * launches per step (with profiling on, spoc_last_timing names it
  * "lin_persistent").  Results are identical bit for bit to the
  * launch-per-stage path.  The wait is bounded: if it runs out,
- * spoc_rk4_steps returns SPOC_EHIP.  No reference counterpart (the
- * reference's RK4 loop, src/shwwnl.f:233-284, calls the right-hand side
- * once per stage).
+ * spoc_rk4_steps returns SPOC_EHIP.  on = 2 runs the launch with its abort
+ * word raised from the start, so that every wave leaves at its first wait
+ * and the call returns SPOC_EHIP: the failure path, forced from the host,
+ * for tests.  No reference counterpart (the reference's RK4 loop,
+ * src/shwwnl.f:233-284, calls the right-hand side once per stage).
  */
 int spoc_set_persistent(spoc_ctx *ctx, int on);
 

@@ -538,12 +538,12 @@ class Context:
         """Replay the RK4 step from a HIP graph (default on)."""
         _check(self.lib.spoc_set_graph(self._h, int(on)))
 
-    def set_persistent(self, on: bool) -> None:
+    def set_persistent(self, on) -> None:
         """Run the linear variant's RK4 steps on the mixed path in one
         persistent launch on small meshes (at most 1,536 elements at p <= 3,
         1,024 at p = 4, 768 at p = 5; default on); the results are the same
         bit for bit."""
-        _check(self.lib.spoc_set_persistent(self._h, int(on)))
+        _check(self.lib.spoc_set_persistent(self._h, int(on)))  # (2: test of the failure path)
 
     def timing(self):
         tot = C.c_double()

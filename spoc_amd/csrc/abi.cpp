@@ -77,7 +77,7 @@ int f32_rk4_begin(F32Plan *p, hipStream_t s, int variant, double *const *u0);
 int f32_rk4_step(F32Plan *p, hipStream_t s, int variant, double dt, TimerHook hook, void *user);
 int f32_rk4_end(F32Plan *p, hipStream_t s, int variant, double *const *u0);
 int f32_rk4_persist(F32Plan *p, hipStream_t s, int variant, double dt, int nsteps,
-                    const unsigned **abort, TimerHook hook, void *user);
+                    const unsigned **abort, TimerHook hook, void *user, bool test_abort);
 int f32_autotune(F32Plan *p, hipStream_t s, int variant, int *out);
 void f32_set_tiles(F32Plan *p, int variant, int t0, int t1, int autotune);
 void f32_set_flux(F32Plan *p, double alpha);
@@ -251,7 +251,7 @@ struct spoc_ctx {
   } graph_key{-1, -1, 0.0, -1, 0};
   long version = 0;     // bumped by every mesh/field change
   int use_graph = 1;    // spoc_set_graph
-  int use_persist = 1;  // spoc_set_persistent
+  int use_persist = 1;  // spoc_set_persistent (2: with the abort word raised)
   QueueBound queue;     // dispatches queued ahead of the host in spoc_rk4_steps
   int *dflag = nullptr; // device word of the non-finite test
   // internal element numbering (spoc_set_ordering): perm[internal] = the
@@ -1073,7 +1073,7 @@ int spoc_rk4_steps(spoc_ctx *c, int variant, int precision, double dt, int nstep
     int persist = 1;
     if (precision == SPOC_FP32_MFMA && c->use_persist &&
         (persist = f32_rk4_persist(c->f32, c->stream, variant, dt, nsteps, &abort_word,
-                                   c->prof.on ? prof_hook : nullptr, c)) < 0)
+                                   c->prof.on ? prof_hook : nullptr, c, c->use_persist == 2)) < 0)
       return bail(f32_fail(persist));
     const bool graph = c->use_graph && !c->prof.on && nsteps >= 2;
     if (persist == 0) {
@@ -1387,7 +1387,8 @@ int spoc_set_graph(spoc_ctx *c, int on) {
 int spoc_set_persistent(spoc_ctx *c, int on) {
   return spoc::guarded([&]() -> int {
     if (!c) return fail(SPOC_EINVAL, "null context");
-    c->use_persist = on != 0;
+    if (on < 0 || on > 2) return fail(SPOC_EINVAL, "spoc_set_persistent: on=%d (0, 1 or 2)", on);
+    c->use_persist = on;
     return SPOC_OK;
   });
 }

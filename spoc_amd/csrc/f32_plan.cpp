@@ -626,6 +626,8 @@ int f32_rk4_step(F32Plan *p, hipStream_t s, int variant, double dt, TimerHook ho
 // blocks.  *abort: the device word that a wave raises when its wait for a
 // neighbouring block runs out (every wave then leaves the launch); the caller reads it
 // once the stream has drained.  hook (may be null) brackets the launches.
+// test_abort: the abort word starts raised, so that every wave leaves at
+// its first wait (the failure path, forced from the host).
 namespace {
 // Persistent launches on one device run one after another, whatever plan or
 // stream they come from: two at once could each hold part of the device and
@@ -637,7 +639,7 @@ hipEvent_t persist_done[kMaxDevices] = {};
 }  // namespace
 
 int f32_rk4_persist(F32Plan *p, hipStream_t s, int variant, double dt, int nsteps,
-                    const unsigned **abort, TimerHook hook, void *user) {
+                    const unsigned **abort, TimerHook hook, void *user, bool test_abort) {
   if (variant != 0 || p->lf_alpha != 0.0 || (p->ne + 15) / 16 > persist_blocks(p->nc) ||
       nsteps < 1)
     return 1;
@@ -657,6 +659,7 @@ int f32_rk4_persist(F32Plan *p, hipStream_t s, int variant, double dt, int nstep
   if (hipStreamWaitEvent(s, done, 0) != hipSuccess) return -2;
   F32Args a = args(p, s, variant, 0, dt);
   a.sync = p->dsync;
+  a.abort0 = test_abort ? 1u : 0u;
   if (hook) hook(user, 0, s);
   for (int n = 0; n < nsteps; n += PERSIST_STEPS) {
     a.steps = std::min(PERSIST_STEPS, nsteps - n);

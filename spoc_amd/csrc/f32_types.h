@@ -211,7 +211,9 @@ struct F32Args {
   int steps;  // persistent RK4 kernel: steps to run in the launch
   unsigned *sync;  // persistent RK4 kernel: [1] abort (a wait that ran out),
                    // [PERSIST_LINE (1 + b)] stages block b has finished; zeroed
-                   // before the launch
+                   // before the launch (abort0: the abort word's start value, 1
+                   // only to test the failure path)
+  unsigned abort0;
   int tile[2];  // waves per workgroup: [0] lin_kernel or bc_kernel, [1] eg_kernel
   double dt;
   // local Lax-Friedrichs (Rusanov) penalty of the gravity-wave core's face

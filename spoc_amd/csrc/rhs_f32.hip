@@ -631,8 +631,8 @@ __device__ __forceinline__ bool wait_nbrs(unsigned *sync, const Nbr3 &N, int eb,
 }
 
 template <int NC, int NG, int NGF>  // (one per shape's object file)
-__global__ void sync_reset_kernel(unsigned *sync, int n) {
-  for (int i = threadIdx.x; i < n; i += blockDim.x) sync[i] = 0u;
+__global__ void sync_reset_kernel(unsigned *sync, int n, unsigned abort0) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) sync[i] = i == 1 ? abort0 : 0u;
 }
 
 template <int NC, int NG, int NGF, int WG>
@@ -1238,7 +1238,8 @@ int launch_persist(const F32Args &A, int nblk) {
       hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * WG, 0) != hipSuccess)
     return -2;
   if ((long)per_cu * cus < grid) return 1;
-  sync_reset_kernel<NC, NG, NGF><<<1, 256, 0, A.stream>>>(A.sync, PERSIST_LINE * (1 + nblk));
+  sync_reset_kernel<NC, NG, NGF><<<1, 256, 0, A.stream>>>(A.sync, PERSIST_LINE * (1 + nblk),
+                                                            A.abort0);
   lin_persistent_kernel<NC, NG, NGF, WG><<<grid, 64 * WG, 0, A.stream>>>(A, nblk);
   return 0;
 }

@@ -173,3 +173,30 @@ def test_persistent_ensemble_union():
             a, b = out[1][f][:t["nm"], e0:e0 + ne], alone[m][f][:t["nm"]]
             assert np.array_equal(a, b), (m, f)
         e0 += ne
+
+
+def test_persistent_failure_path_forced_from_the_host():
+    """With its abort word raised before the launch (spoc_set_persistent 2),
+    every wave leaves at its first wait that is not yet satisfied and the
+    call returns SPOC_EHIP with its message, not a hang; the next call, with
+    the word left alone, runs normally."""
+    t, ctx = ctx_for(3, 12, 4)
+    c = configs.random_case(t, 12, 10, seed=9, scale=0.01)
+    ctx.set_mesh(c["mesh"])
+    set_fields(ctx, c["phys"])
+    ctx.set_persistent(False)
+    ctx.upload(*c["state"])
+    ctx.rk4("lin", 2.0, 5, precision="fp32")
+    ref = ctx.download()
+    say("persistent forced abort")
+    ctx.set_persistent(2)
+    ctx.upload(*c["state"])
+    with pytest.raises(spoc_amd.SpocError) as ei:
+        ctx.rk4("lin", 2.0, 5, precision="fp32")
+    assert ei.value.code == -2 and "neighbouring block" in str(ei.value), str(ei.value)
+    ctx.set_persistent(True)
+    ctx.upload(*c["state"])
+    ctx.rk4("lin", 2.0, 5, precision="fp32")
+    same(ctx.download(), ref)
+    with pytest.raises(spoc_amd.SpocError):
+        ctx.set_persistent(3)
