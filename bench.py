@@ -19,7 +19,7 @@ separate run of min(K, 20) steps with HIP events around each launch.
 no-slip eddies at p=7, c4 inviscid at p=5, c5 ensemble of 64 members at
 p=4); the default, c2, is the headline.
 
-Prints ONE JSON line.  Multi-GPU runs are not part of this build.
+Prints ONE JSON line.  It runs on one GPU; --gpus above 1 is refused.
 """
 from __future__ import annotations
 
@@ -552,7 +552,7 @@ def main():
                          "known traffic")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported",
+        print(f"bench.py: runs on one GPU only; --gpus {args.gpus} is not supported",
               file=sys.stderr)
         sys.exit(2)
     import spoc_amd
