@@ -5,8 +5,9 @@ coefficient of the new state is NaN or infinite, spoc_energy when the
 kinetic energy is NaN; the state stays downloadable.  The non-finite value
 is planted in the input on the host; every kernel runs to its normal end.
 
-Also: a long run queues thousands of dispatches, which the driver bounds to
-512 ahead of the host; the results equal an unbounded eager run's."""
+Also: a long run queues thousands of dispatches, which spoc_rk4_steps
+bounds to 512 ahead of the host; the results equal an unbounded eager
+run's."""
 import sys
 
 import numpy as np
