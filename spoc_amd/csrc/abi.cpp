@@ -295,6 +295,7 @@ int step_dispatches(int variant, int precision) {
 }
 
 void free_mesh(spoc_ctx *c) {
+  (void)hipStreamSynchronize(c->stream);  // (copies may still target what is freed)
   drop_graph(c);
   if (c->dmesh) (void)hipFree(c->dmesh);
   if (c->dmem) (void)hipFree(c->dmem);
@@ -814,6 +815,8 @@ static inline size_t caller_elem(const spoc_ctx *c, int e) {
 static int ensure_stage(spoc_ctx *c, int n) {
   const size_t want = (size_t)n * c->nm * c->ne;
   if (c->stage_cap < want) {
+    // (copies of an earlier call may still read or write the old staging)
+    HIPCHK(hipStreamSynchronize(c->stream));
     if (c->stage) (void)hipHostFree(c->stage);
     c->stage = nullptr;
     c->stage_cap = 0;
@@ -860,20 +863,27 @@ static int upload_fields(spoc_ctx *c, int ld, const double *const *src, double *
   for (int t = 1; t < nt; t++) th.emplace_back(work, t);
   work(0);
   for (auto &t : th) t.join();
-  HIPCHK(hipStreamSynchronize(c->stream));
-  if (bad) return fail(SPOC_EHIP, "host-to-device copy failed");
+  // (no wait for the copies: the stream orders everything after them, the
+  // staging is reused only by later copies on the same stream or after the
+  // wait at the top of the next upload, and it is freed only after a wait)
+  if (bad) {
+    (void)hipStreamSynchronize(c->stream);
+    return fail(SPOC_EHIP, "host-to-device copy failed");
+  }
   return SPOC_OK;
 }
 
 // Device (nm, ne) arrays -> host (ld, ne) arrays.  Every field is copied in
-// kPieces pieces of elements, each with its completion event, all queued at
-// once; each host thread owns a slice of the elements and unpacks it from
+// kPieces pieces of elements (one on a small mesh), each with its
+// completion event, all queued at once; each host thread owns a slice of the elements and unpacks it from
 // every field as soon as the piece holding it has arrived, so the unpacking
 // (the bound: it writes 28 of every 45 doubles of the caller's arrays)
 // runs while the later pieces are still being copied.
 static int download_fields(spoc_ctx *c, int ld, double *const *dst, double *const *src, int n) {
-  constexpr int kPieces = 4;
   const int nm = c->nm, ne = c->ne;
+  // (a small mesh, unpacked by one thread, gains nothing from pieces: one
+  // copy per field)
+  const int kPieces = ne < 4096 ? 1 : 4;
   const size_t fsz = (size_t)nm * ne;
   int rc = ensure_stage(c, n);
   if (rc) return rc;
