@@ -844,6 +844,10 @@ static int upload_fields(spoc_ctx *c, int ld, const double *const *src, double *
   // the staging may still feed an earlier call's copies
   HIPCHK(hipStreamSynchronize(c->stream));
   const int nt = ne < 4096 ? 1 : host_threads();
+  // a small mesh whose fields land side by side on the device goes in one
+  // copy (its fixed cost is most of the time there)
+  bool one = nt == 1;
+  for (int f = 0; f < n && one; f++) one = src[f] && dst[f] == dst[0] + f * fsz;
   std::atomic<int> bad{0};
   auto work = [&](int t) {
     const int e0 = (int)((long long)ne * t / nt), e1 = (int)((long long)ne * (t + 1) / nt);
@@ -853,11 +857,15 @@ static int upload_fields(spoc_ctx *c, int ld, const double *const *src, double *
       const double *s = src[f];
       for (int e = e0; e < e1; e++)
         std::memcpy(st + (size_t)e * nm, s + (size_t)ld * caller_elem(c, e), sizeof(double) * nm);
+      if (one) continue;
       if (hipMemcpyAsync(dst[f] + (size_t)e0 * nm, st + (size_t)e0 * nm,
                          sizeof(double) * nm * (e1 - e0), hipMemcpyHostToDevice,
                          c->stream) != hipSuccess)
         bad = 1;
     }
+    if (one && hipMemcpyAsync(dst[0], c->stage, sizeof(double) * fsz * n, hipMemcpyHostToDevice,
+                              c->stream) != hipSuccess)
+      bad = 1;
   };
   std::vector<std::thread> th;
   for (int t = 1; t < nt; t++) th.emplace_back(work, t);
@@ -888,7 +896,16 @@ static int download_fields(spoc_ctx *c, int ld, double *const *dst, double *cons
   int rc = ensure_stage(c, n);
   if (rc) return rc;
   auto cut = [&](int k, int parts) { return (int)((long long)ne * k / parts); };
-  for (int f = 0; f < n; f++)
+  // (a small mesh whose fields lie side by side on the device: one copy,
+  // and every field's event marks its end)
+  bool one = kPieces == 1;
+  for (int f = 0; f < n && one; f++) one = src[f] == src[0] + f * fsz;
+  if (one) {
+    HIPCHK(hipMemcpyAsync(c->stage, src[0], sizeof(double) * fsz * n, hipMemcpyDeviceToHost,
+                          c->stream));
+    for (int f = 0; f < n; f++) HIPCHK(hipEventRecord(c->fev[f], c->stream));
+  }
+  for (int f = 0; f < n && !one; f++)
     for (int p = 0; p < kPieces; p++) {
       const size_t e0 = cut(p, kPieces), e1 = cut(p + 1, kPieces);
       if (e1 > e0)
