@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -583,18 +584,38 @@ int f32_ready(const F32Plan *p) {
 
 // Tendencies of the reference-layout FP64 state x[3] (device, [e][nm]) into
 // k[3] (device, same layout).
+// Phase 0 or 1 (basis change) of the three fields in[f] -> out[f]: one
+// launch when both sides hold them at equal strides, else one per field.
+static int basis3(const F32Plan *p, F32Args a, int phase, const double *const *in,
+                  double *const *out) {
+  auto stride = [](const void *p0, const void *p1, const void *p2) -> size_t {
+    const uintptr_t a0 = (uintptr_t)p0, a1 = (uintptr_t)p1, a2 = (uintptr_t)p2;
+    if (a1 <= a0 || a2 - a1 != a1 - a0 || (a1 - a0) % sizeof(double)) return 0;
+    return (a1 - a0) / sizeof(double);
+  };
+  const size_t si = stride(in[0], in[1], in[2]), so = stride(out[0], out[1], out[2]);
+  if (si && so) {
+    a.nf = 3;
+    a.sin = si;
+    a.sout = so;
+    return call(p, a, phase, in[0], out[0]);
+  }
+  for (int f = 0; f < 3; f++) {
+    const int rc = call(p, a, phase, in[f], out[f]);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
 int f32_rhs(F32Plan *p, hipStream_t s, int variant, const double *const *x, double *const *k,
             TimerHook hook, void *user) {
   int rc = f32_ready(p);
   if (rc || (rc = maybe_tune(p, s, variant))) return rc;
   F32Args a = args(p, s, variant, -1, 0.0);
-  for (int f = 0; f < 3; f++)
-    if ((rc = call(p, a, 0, x[f], p->st.c0[f]))) return rc;
+  if ((rc = basis3(p, a, 0, x, p->st.c0))) return rc;
   if ((rc = call(p, a, 2, nullptr, nullptr))) return rc;
   if ((rc = rhs_kernels(p, a, hook, user))) return rc;
-  for (int f = 0; f < 3; f++)
-    if ((rc = call(p, a, 1, p->st.k[f], k[f]))) return rc;
-  return 0;
+  return basis3(p, a, 1, p->st.k, k);
 }
 
 // The RK4 loop in three parts so that the caller can capture the step in a
@@ -604,8 +625,7 @@ int f32_rk4_begin(F32Plan *p, hipStream_t s, int variant, double *const *u0) {
   int rc = f32_ready(p);
   if (rc || (rc = maybe_tune(p, s, variant))) return rc;
   F32Args a = args(p, s, variant, -1, 0.0);
-  for (int f = 0; f < 3; f++)
-    if ((rc = call(p, a, 0, u0[f], p->st.c0[f]))) return rc;
+  if ((rc = basis3(p, a, 0, u0, p->st.c0))) return rc;
   return call(p, a, 2, nullptr, nullptr);
 }
 
@@ -674,11 +694,7 @@ int f32_rk4_persist(F32Plan *p, hipStream_t s, int variant, double dt, int nstep
 
 int f32_rk4_end(F32Plan *p, hipStream_t s, int variant, double *const *u0) {
   F32Args a = args(p, s, variant, -1, 0.0);
-  for (int f = 0; f < 3; f++) {
-    int rc = call(p, a, 1, p->st.c0[f], u0[f]);
-    if (rc) return rc;
-  }
-  return 0;
+  return basis3(p, a, 1, p->st.c0, u0);
 }
 
 // Adjoint (transpose of the linearised right-hand side and of the RK4

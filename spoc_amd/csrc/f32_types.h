@@ -214,6 +214,10 @@ struct F32Args {
                    // before the launch (abort0: the abort word's start value, 1
                    // only to test the failure path)
   unsigned abort0;
+  // phases 0 and 1 (basis change): nf fields at once, field f at
+  // in + f * sin, out + f * sout (nf 0 counts as 1)
+  int nf;
+  size_t sin, sout;
   int tile[2];  // waves per workgroup: [0] lin_kernel or bc_kernel, [1] eg_kernel
   double dt;
   // local Lax-Friedrichs (Rusanov) penalty of the gravity-wave core's face

@@ -5,6 +5,7 @@
 // and around each spoc_rhs.
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <cstring>
 #include <vector>
 
@@ -193,21 +194,41 @@ int f64_fields_set(F64Plan *p, hipStream_t s, const DevTables &tab, const double
   return 0;
 }
 
+// Phase 0 or 1 (layout change) of the three fields in[f] -> out[f]: one
+// launch when both sides hold them at equal strides, else one per field.
+static int layout3(F64Plan *p, F64Args a, int phase, const double *const *in,
+                   double *const *out) {
+  auto stride = [](const void *p0, const void *p1, const void *p2) -> size_t {
+    const uintptr_t a0 = (uintptr_t)p0, a1 = (uintptr_t)p1, a2 = (uintptr_t)p2;
+    if (a1 <= a0 || a2 - a1 != a1 - a0 || (a1 - a0) % sizeof(double)) return 0;
+    return (a1 - a0) / sizeof(double);
+  };
+  const size_t si = stride(in[0], in[1], in[2]), so = stride(out[0], out[1], out[2]);
+  if (si && so) {
+    a.nf = 3;
+    a.sin = si;
+    a.sout = so;
+    return call(p, a, phase, in[0], out[0]);
+  }
+  for (int c = 0; c < 3; c++) {
+    const int rc = call(p, a, phase, in[c], out[c]);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
 // Tendencies of the packed device state x[3] into k[3] (packed).
 int f64_rhs(F64Plan *p, hipStream_t s, const DevTables &tab, int variant,
             const double *const *x, double *const *k, TimerHook hook, void *user) {
   if (!p->dwork || !p->have_fields) return -3;
   F64Args a = args(p, s, tab, variant, -1, 0.0);
   int rc;
-  for (int c = 0; c < 3; c++)
-    if ((rc = call(p, a, 0, x[c], p->x[c]))) return rc;
+  if ((rc = layout3(p, a, 0, x, p->x))) return rc;
   for (int c = 0; c < 3; c++) a.x[c] = p->x[c];
   a.hook = hook;
   a.hook_user = user;
   if ((rc = call(p, a, 2, nullptr, nullptr))) return rc;
-  for (int c = 0; c < 3; c++)
-    if ((rc = call(p, a, 1, p->k[c], k[c]))) return rc;
-  return 0;
+  return layout3(p, a, 1, p->k, k);
 }
 
 // The RK4 loop in three parts, so that the caller can capture the step in a
@@ -216,11 +237,7 @@ int f64_rk4_begin(F64Plan *p, hipStream_t s, const DevTables &tab,
                   const double *const *u0) {
   if (!p->dwork || !p->have_fields) return -3;
   const F64Args a = args(p, s, tab, 0, -1, 0.0);
-  for (int c = 0; c < 3; c++) {
-    int rc = call(p, a, 0, u0[c], p->u0[c]);
-    if (rc) return rc;
-  }
-  return 0;
+  return layout3(p, a, 0, u0, p->u0);
 }
 
 int f64_rk4_step(F64Plan *p, hipStream_t s, const DevTables &tab, int variant, double dt,
@@ -237,11 +254,7 @@ int f64_rk4_step(F64Plan *p, hipStream_t s, const DevTables &tab, int variant, d
 
 int f64_rk4_end(F64Plan *p, hipStream_t s, const DevTables &tab, double *const *u0) {
   const F64Args a = args(p, s, tab, 0, -1, 0.0);
-  for (int c = 0; c < 3; c++) {
-    int rc = call(p, a, 1, p->u0[c], u0[c]);
-    if (rc) return rc;
-  }
-  return 0;
+  return layout3(p, a, 1, p->u0, u0);
 }
 
 // Kernels per right-hand side (for the dispatch bound of the RK4 driver).

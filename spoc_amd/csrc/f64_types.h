@@ -52,6 +52,10 @@ struct F64Args {
   // rows), weighted products (8 NG rows)
   double *Q[4], *F[4], *S[2], *UH[2], *G[3], *MF, *WS;
   double *face;         // face values: [f][out][j], 6 outputs
+  // phases 0 and 1: nf fields at once, field f at in + f * sin, out + f * sout
+  // (nf 0 counts as 1)
+  int nf;
+  size_t sin, sout;
   TimerHook hook;
   void *hook_user;
   void timer(int slot) const {

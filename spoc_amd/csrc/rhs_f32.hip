@@ -1139,10 +1139,14 @@ __global__ __launch_bounds__(64 * WG) __attribute__((amdgpu_waves_per_eu(eg_wave
 // reads it at the same address.  The element's coefficients are read
 // straight from global memory: an LDS-staged variant (contiguous reads)
 // measured slower, its uniform LDS reads of U raising register use to 244.
+// (blockIdx.y: the field, at a + y * sa, c + y * sc)
 template <int NM>
 __global__ __launch_bounds__(256) void to_psi_kernel(int ne, const double *__restrict__ U,
                                                      const double *__restrict__ a,
-                                                     double *__restrict__ c) {
+                                                     double *__restrict__ c, size_t sa,
+                                                     size_t sc) {
+  a += blockIdx.y * sa;
+  c += blockIdx.y * sc;
   __shared__ double Us[NM * NM];
   for (int i = threadIdx.x; i < NM * NM; i += 256) Us[i] = U[i];
   __syncthreads();
@@ -1167,7 +1171,10 @@ __global__ __launch_bounds__(256) void to_psi_kernel(int ne, const double *__res
 template <int NM>
 __global__ __launch_bounds__(PSI_WG) void from_psi_kernel(int ne, const double *__restrict__ U,
                                                           const double *__restrict__ c,
-                                                          double *__restrict__ a) {
+                                                          double *__restrict__ a, size_t sc,
+                                                          size_t sa) {
+  c += blockIdx.y * sc;
+  a += blockIdx.y * sa;
   __shared__ double Us[NM * NM];
   __shared__ double T[PSI_WG * (NM + 1)];
   const int e0 = blockIdx.x * PSI_WG, n = min(PSI_WG, ne - e0);
@@ -1259,11 +1266,13 @@ int launch(const F32Args &A, int phase, const double *in, double *out) {
   hipStream_t st = A.stream;
   const int v = A.variant;
   switch (phase) {
-    case 0:  // to psi
-      to_psi_kernel<NM><<<(ne + 255) / 256, 256, 0, st>>>(ne, A.ops.U, in, out);
+    case 0:  // to psi (A.nf fields)
+      to_psi_kernel<NM><<<dim3((ne + 255) / 256, A.nf > 1 ? A.nf : 1), 256, 0, st>>>(
+          ne, A.ops.U, in, out, A.sin, A.sout);
       break;
-    case 1:  // from psi
-      from_psi_kernel<NM><<<(ne + PSI_WG - 1) / PSI_WG, PSI_WG, 0, st>>>(ne, A.ops.U, in, out);
+    case 1:  // from psi (A.nf fields)
+      from_psi_kernel<NM><<<dim3((ne + PSI_WG - 1) / PSI_WG, A.nf > 1 ? A.nf : 1), PSI_WG, 0, st>>>(
+          ne, A.ops.U, in, out, A.sin, A.sout);
       break;
     case 2:  // traces of the first stage input
       if (v == 0) prep_kernel<NC, NG, NGF, 0><<<grid, NT, 0, st>>>(A, nblk);

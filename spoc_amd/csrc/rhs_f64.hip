@@ -922,9 +922,13 @@ __global__ __launch_bounds__(64) void wall_c(F64Args A) {
 // Layout changes between the packed (NM, ne) arrays of the C-ABI and the
 // blocked arrays of this path (padding elements read as zero).
 // ---------------------------------------------------------------------------
+// (blockIdx.y: the field, at in + y * sin, out + y * sout)
 template <int NM>
 __global__ __launch_bounds__(256) void to_blocked(int ne, int nblk, const double *__restrict__ in,
-                                                  double *__restrict__ out) {
+                                                  double *__restrict__ out, size_t sin,
+                                                  size_t sout) {
+  in += blockIdx.y * sin;
+  out += blockIdx.y * sout;
   const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (idx >= (size_t)nblk * NM * WB) return;
   const int c = (int)(idx % WB), r = (int)((idx / WB) % NM), b = (int)(idx / ((size_t)WB * NM));
@@ -934,7 +938,10 @@ __global__ __launch_bounds__(256) void to_blocked(int ne, int nblk, const double
 
 template <int NM>
 __global__ __launch_bounds__(256) void from_blocked(int ne, const double *__restrict__ in,
-                                                    double *__restrict__ out) {
+                                                    double *__restrict__ out, size_t sin,
+                                                    size_t sout) {
+  in += blockIdx.y * sin;
+  out += blockIdx.y * sout;
   const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (idx >= (size_t)ne * NM) return;
   const int e = (int)(idx / NM), r = (int)(idx % NM);
@@ -949,14 +956,17 @@ int launch(const F64Args &a, int phase, const double *in, double *out) {
   constexpr int NM = nm_of(NC);
   const F64Mesh &M = a.mesh;
   hipStream_t st = a.stream;
+  const unsigned nf = a.nf > 1 ? (unsigned)a.nf : 1u;
   if (phase == 0) {
     const size_t n = (size_t)M.nblk * NM * WB;
-    to_blocked<NM><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(M.ne, M.nblk, in, out);
+    to_blocked<NM><<<dim3((unsigned)((n + 255) / 256), nf), 256, 0, st>>>(M.ne, M.nblk, in, out,
+                                                                          a.sin, a.sout);
     return hipGetLastError() == hipSuccess ? 0 : -2;
   }
   if (phase == 1) {
     const size_t n = (size_t)M.ne * NM;
-    from_blocked<NM><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(M.ne, in, out);
+    from_blocked<NM><<<dim3((unsigned)((n + 255) / 256), nf), 256, 0, st>>>(M.ne, in, out, a.sin,
+                                                                            a.sout);
     return hipGetLastError() == hipSuccess ? 0 : -2;
   }
   const DevTables &T = a.tab;
