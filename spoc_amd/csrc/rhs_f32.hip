@@ -364,10 +364,12 @@ __device__ __forceinline__ void update(const F32Args &A, int stage, int f, int e
     ix[s] = (m < NM ? m : NM - 1) * 16 + col;
     x[s] = valid ? x[s] : 0.0;
   }
+  // the stored results are read by a later launch only (non-temporal: they
+  // do not displace this launch's neighbour gathers from L2)
   auto store = [&](double *p, const double(&v)[KM]) {
 #pragma unroll
     for (int s = 0; s < KM; s++)
-      if (4 * s + g < NM) p[mb + ix[s]] = v[s];
+      if (4 * s + g < NM) __builtin_nontemporal_store(v[s], p + mb + ix[s]);
   };
   if (stage < 0) {
     store(A.st.k[f], x);
@@ -381,11 +383,11 @@ __device__ __forceinline__ void update(const F32Args &A, int stage, int f, int e
   if (stage == 3) {
     double p[KM], q[KM], r[KM];
 #pragma unroll
-    for (int s = 0; s < KM; s++) p[s] = A.st.ys[0][f][mb + ix[s]];
+    for (int s = 0; s < KM; s++) p[s] = __builtin_nontemporal_load(A.st.ys[0][f] + mb + ix[s]);
 #pragma unroll
-    for (int s = 0; s < KM; s++) q[s] = A.st.ys[1][f][mb + ix[s]];
+    for (int s = 0; s < KM; s++) q[s] = __builtin_nontemporal_load(A.st.ys[1][f] + mb + ix[s]);
 #pragma unroll
-    for (int s = 0; s < KM; s++) r[s] = A.st.ys[2][f][mb + ix[s]];
+    for (int s = 0; s < KM; s++) r[s] = __builtin_nontemporal_load(A.st.ys[2][f] + mb + ix[s]);
     const double third = 1.0 / 3.0, w = dt / 6.0;
 #pragma unroll
     for (int s = 0; s < KM; s++) {
@@ -1083,7 +1085,9 @@ __device__ __forceinline__ void eg_part(const F32Args &A, const double *O64, int
   // the FP64 gravity-wave core (volume and face terms, from the FP64 stage
   // input and traces) with the face values of the flux traces lifted along,
   // then the FP32 volume part B+C stored (gvol) added to it (read only now,
-  // so that its registers are free while the core keeps its loads in flight)
+  // so that its registers are free while the core keeps its loads in flight;
+  // read with the non-temporal hint, as it is read once, while bc_kernel
+  // stores it without, so that where it fits in L2 it is found there)
   doublex4 G[3][RBM];
   core64<NC, NG, NGF, VAR, RUS, true, ROLE>(A, A.stage, O64 + P64.dx, O64 + P64.dy, O64 + P64.lf,
                                             N, eb, G);
@@ -1094,7 +1098,7 @@ __device__ __forceinline__ void eg_part(const F32Args &A, const double *O64, int
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int m = drow(rb, r, g), mc = m < NM ? m : NM - 1;
-        const float v = A.st.gvol[f][mb + mc * 16 + col];
+        const float v = __builtin_nontemporal_load(A.st.gvol[f] + mb + mc * 16 + col);
         G[f][rb][r] += (valid && m < NM) ? (double)v : 0.0;
       }
   phase_fence();
